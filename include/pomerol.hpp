@@ -1,0 +1,11 @@
+// Umbrella header of the pomerol-compatible C++ API (reference: include/pomerol.hpp).
+#pragma once
+
+#include "pomerol/Misc.hpp"
+#include "pomerol/Operators.hpp"
+#include "pomerol/LatticePresets.hpp"
+#include "pomerol/HilbertSpace.hpp"
+#include "pomerol/Hamiltonian.hpp"
+#include "pomerol/MonomialOperator.hpp"
+#include "pomerol/GreensFunction.hpp"
+#include "pomerol/TwoParticleGF.hpp"

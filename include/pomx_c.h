@@ -1,0 +1,149 @@
+/* pomx_c.h -- C ABI of the MI355X (gfx950) full-ED kernels.
+ *
+ * Plain C: opaque context, integer status codes (POMX_OK == 0), plain device pointers and
+ * sizes.  No exceptions cross this boundary; pomx_last_error() returns the message of the last
+ * failing call on a context.  One context per host thread; every call is stream-ordered on the
+ * context's HIP stream and returns after the work it reports (counts, scalars) is complete.
+ *
+ * The pomerol C++ API (the headers under include/pomerol, implemented in pomerol_amd/src) is
+ * the only intended caller; each entry point names the reference routine whose loop it replaces.
+ * Every matrix is FP64, dense, row-major unless a pomx_view says otherwise.
+ */
+#ifndef POMX_C_H
+#define POMX_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pomx_ctx pomx_ctx;
+
+enum {
+    POMX_OK = 0,
+    POMX_ERR_ARG = 1,   /* invalid argument (sizes, null pointers, unsupported option) */
+    POMX_ERR_HIP = 2,   /* HIP runtime error (launch, copy, allocation) */
+    POMX_ERR_RANGE = 3, /* value outside the range a kernel supports (documented per call) */
+    POMX_ERR_NODEV = 4  /* no usable gfx950 device */
+};
+
+/* ---- context and memory ------------------------------------------------------------- */
+int pomx_ctx_create(int device, pomx_ctx** ctx);
+void pomx_ctx_destroy(pomx_ctx* ctx);
+const char* pomx_last_error(const pomx_ctx* ctx);
+int pomx_malloc(pomx_ctx* ctx, size_t bytes, void** d_ptr);
+int pomx_free(pomx_ctx* ctx, void* d_ptr);
+int pomx_h2d(pomx_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
+int pomx_d2h(pomx_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
+int pomx_fill_zero(pomx_ctx* ctx, void* d_dst, size_t bytes);
+int pomx_synchronize(pomx_ctx* ctx);
+
+/* ---- K1 block assembly ---------------------------------------------------------------
+ * Replaces HamiltonianPart::prepareImpl (src/pomerol/HamiltonianPart.cpp:56-76).
+ * d_mat[d_pos[e]] += d_val[e] for e < nnz; d_mat (mat_len doubles) is zeroed first. */
+int pomx_assemble(pomx_ctx* ctx, int64_t nnz, const int64_t* d_pos, const double* d_val, double* d_mat,
+                  int64_t mat_len);
+
+/* ---- K2 batched symmetric eigensolve -------------------------------------------------
+ * Replaces HamiltonianPart::computeImpl (src/pomerol/HamiltonianPart.cpp:90-102), i.e. Eigen's
+ * SelfAdjointEigenSolver per block.  Block b (size h_n[b]) is the row-major matrix at
+ * d_mat + h_mat_off[b]; on return it holds the eigenvectors as columns and d_eval + h_val_off[b]
+ * the eigenvalues in ascending order.  Householder tridiagonalisation, implicit QL on the
+ * tridiagonal matrix, Householder back-transformation; one workgroup per block. */
+int pomx_eigh_batched(pomx_ctx* ctx, int nblocks, const int* h_n, const int64_t* h_mat_off,
+                      const int64_t* h_val_off, double* d_mat, double* d_eval);
+
+/* ---- K5 Boltzmann weights ------------------------------------------------------------
+ * Replaces DensityMatrixPart::computeUnnormalized/normalize (DensityMatrixPart.cpp:23-31) and
+ * Hamiltonian::computeGroundEnergy (Hamiltonian.cpp:125-130): E0 = min E, w = exp(-beta(E-E0))/Z. */
+int pomx_boltzmann(pomx_ctx* ctx, int64_t nstates, const double* d_E, double beta, double* d_w, double* h_E0,
+                   double* h_Z);
+
+/* ---- K3 operator rotation ------------------------------------------------------------
+ * Replaces MonomialOperatorPart::computeImpl (MonomialOperatorPart.cpp:48-92):
+ *   out[i, j] = sum_t U_to[t, i] * X[t, j],  X[t, j] = sgn(t) * U_from[src(t), j]
+ * where d_src[src_off + t] = +-(1 + source inner index) or 0 (no preimage).  Entries with
+ * |out| <= cutoff are stored as exact zeros (Eigen's sparseView(Tol) keeps |x| > Tol*1e-12,
+ * so the caller passes cutoff = Tol * 1e-12).  FP64 MFMA (v_mfma_f64_16x16x4f64). */
+typedef struct {
+    int n_to, n_from;
+    int64_t u_to_off, u_from_off, src_off, out_off;
+} pomx_rot_job;
+int pomx_rotate_batched(pomx_ctx* ctx, int njobs, const pomx_rot_job* h_jobs, const double* d_U, const int* d_src,
+                        double* d_out, double cutoff);
+
+/* Strided device matrix view: element (r, c) is base[r * rs + c * cs]. */
+typedef struct {
+    const double* base;
+    int64_t rs, cs;
+} pomx_view;
+
+/* ---- K6/K10 + K8 two-operator Lehmann terms (G and chi) ------------------------------
+ * Replaces GreensFunctionPart::computeImpl (GreensFunctionPart.cpp:65-110) for kind
+ * POMX_KIND_GF and SusceptibilityPart::computeImpl (SusceptibilityPart.cpp:76-127) for kind
+ * POMX_KIND_CHI, plus the TermList merge (TermList.hpp:65-76): terms are binned by
+ * round(pole/pole_res), grouped by (part, bin) with a stable sort, residues summed in
+ * emission order, the first pole of a group kept.  Outputs (device, freed with pomx_free):
+ * residues and poles of all parts back to back; h_count[p] terms for part p; for CHI also
+ * h_zero_pole[p] (the |pole| < pole_res weight). */
+enum { POMX_KIND_GF = 0, POMX_KIND_CHI = 1 };
+typedef struct {
+    int n_out, n_in;
+    pomx_view A; /* <out|A|in>  n_out x n_in */
+    pomx_view B; /* <in|B|out>  n_in x n_out */
+    const double *E_out, *E_in, *w_out, *w_in;
+} pomx_pair_part;
+int pomx_pair_terms(pomx_ctx* ctx, int kind, int nparts, const pomx_pair_part* h_parts, double pole_res,
+                    double coeff_tol, double** d_residue, double** d_pole, int64_t* h_count, double* h_zero_pole);
+
+/* ---- K9 Lehmann sum of pole terms on a frequency list --------------------------------
+ * GreensFunction::operator() (GreensFunction.hpp:127-136): acc[w] += sum_t R_t / (z_w - P_t);
+ * CHI (SusceptibilityPart.hpp:216-220): acc[w] += -sum_t R_t / (z_w - P_t).  z and acc are
+ * interleaved complex (re, im) arrays of nz entries. */
+int pomx_pair_eval(pomx_ctx* ctx, int kind, int64_t nterms, const double* d_residue, const double* d_pole, int nz,
+                   const double* d_z, double* d_acc);
+
+/* ---- K7 + K8 + K9 two-particle GF part -----------------------------------------------
+ * Replaces TwoParticleGFPart::computeImpl/addMultiterm (TwoParticleGFPart.cpp:189-298), the
+ * NonResonant/Resonant TermList merges (TwoParticleGFPart.hpp:83-142,208-268) and the
+ * frequency fill (TwoParticleGF.cpp:132-148, TwoParticleGFPart.cpp:309-322).
+ * Terms are generated for every (i1,i2,i3,i4) with all four matrix elements non-zero and
+ * w1+w2+w3+w4 > coeff_tol, merged on (kind, round(P1/res), round(P2/res), round(P3/res))
+ * with count-weighted pole means, and (if nfreq > 0) evaluated at the triplets d_freq
+ * (nfreq x 3 complex, the caller's (z1, z2, z3)) into d_acc (nfreq complex, accumulated).
+ * h_counts receives the merged term counts (see pomx_g2_terms), h_raw_terms the number of
+ * terms emitted before merging.  keep == NULL discards the merged terms after evaluation. */
+typedef struct {
+    int n1, n2, n3, n4;
+    pomx_view O1, O2, O3, CX4; /* <S1|O1|S2> <S2|O2|S3> <S3|O3|S4> <S4|CX4|S1> */
+    const double *E1, *E2, *E3, *E4;
+    const double *w1, *w2, *w3, *w4;
+    int perm;    /* 0..5: permutation of (c_i, c_j, c^dag_k), Misc.cpp:41-42 order */
+    double sign; /* its signature */
+} pomx_g2_part;
+
+/* Merged terms of one part, packed on the device (freed with pomx_free(ctx, data)).
+ * count[0] non-resonant z2 terms, count[1] non-resonant z4 terms (isz4), count[2] resonant
+ * z2z3 terms, count[3] resonant z1z2 terms (isz1z2).  Layout: NR terms as SoA
+ * (coeff, P1, P2, P3) of n_nr = count[0]+count[1] doubles each, then R terms as SoA
+ * (res_coeff, nonres_coeff, P1, P2, P3) of n_r = count[2]+count[3] doubles each. */
+typedef struct {
+    int64_t count[4];
+    double* data;
+} pomx_g2_terms;
+
+int pomx_g2_part_compute(pomx_ctx* ctx, const pomx_g2_part* part, double beta, double pole_res, double coeff_tol,
+                         int nfreq, const double* d_freq, double* d_acc, pomx_g2_terms* keep, int64_t* h_counts,
+                         int64_t* h_raw_terms);
+int pomx_g2_terms_eval(pomx_ctx* ctx, const pomx_g2_terms* terms, int perm, double pole_res, int nfreq,
+                       const double* d_freq, double* d_acc);
+
+/* ---- timing of the dominant kernel (HIP events on the context stream) ---------------- */
+int pomx_eval_timing(pomx_ctx* ctx, int enable, double* h_total_ms, int64_t* h_launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* POMX_C_H */

@@ -1,0 +1,701 @@
+// K7 (two-particle GF term generation), K8 (pole merge) and K9 (Lehmann sum on a frequency
+// triplet list) for one TwoParticleGFPart.
+//
+// Reference: TwoParticleGFPart::computeImpl and addMultiterm (TwoParticleGFPart.cpp:189-298),
+// the NonResonantTerm/ResonantTerm TermLists (TwoParticleGFPart.hpp:83-142,208-268, .cpp:35-43,
+// 86-95) and their evaluation (TwoParticleGFPart.hpp:475-495, .cpp:309-322).
+//
+// Pipeline (all on the device):
+//  1. rank tables: every pole is a difference of two block energies, P1 = E2[i2]-E1[i1],
+//     P2 = E3[i3]-E2[i2], P3 = E4[i4]-E3[i3]; the bins round(P/res) of each block pair are
+//     sorted once and replaced by dense ranks r12, r23, r34, so a term's TermList identity
+//     (kind, bin1, bin2, bin3) becomes one 64-bit key (2 + 20 + 21 + 21 bits);
+//  2. chunking: (i1,i2) pairs are visited in r12 order and cut into chunks only at r12 group
+//     boundaries, so every group of mergeable terms lives in one chunk;
+//  3. per chunk: count terms per (i1,i2,i3) row, prefix-sum, emit (key, packed indices),
+//     stable radix sort, and reduce every key group (coefficients summed, poles averaged,
+//     dropped if the sum is negligible);
+//  4. evaluate the merged terms on the frequency list (LDS-staged terms, frequencies in
+//     registers, partial sums per term slice reduced in a fixed order).
+#include "pomx_internal.hpp"
+
+#include <algorithm>
+
+namespace {
+
+struct G2Dev {
+    int n1, n2, n3, n4;
+    pomx_view O1, O2, O3, CX4;
+    const double *E1, *E2, *E3, *E4;
+    const double *w1, *w2, *w3, *w4;
+    double sign, beta, tol;
+};
+
+__device__ inline double mat(pomx_view const& v, int r, int c) { return v.base[r * v.rs + c * v.cs]; }
+
+// ---- 1. rank tables -------------------------------------------------------------------------
+constexpr int64_t kBinOff = int64_t(1) << 61;
+
+__global__ void bins_kernel(int nx, int ny, const double* __restrict__ Ex, const double* __restrict__ Ey, double res,
+                            uint64_t* __restrict__ keys, uint64_t* __restrict__ idx, int* __restrict__ err) {
+    int64_t g = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(g >= int64_t(nx) * ny)
+        return;
+    int x = int(g / ny), y = int(g % ny);
+    double q = round((Ey[y] - Ex[x]) / res);
+    if(!(fabs(q) < 1e18)) {
+        err[0] = 1;
+        q = 0;
+    }
+    keys[g] = uint64_t(int64_t(q) + kBinOff);
+    idx[g] = uint64_t(g);
+}
+
+__global__ void rank_heads_kernel(int64_t n, const uint64_t* __restrict__ keys, int64_t* __restrict__ head) {
+    int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(k < n)
+        head[k] = (k == 0 || keys[k] != keys[k - 1]) ? 1 : 0;
+}
+
+__global__ void rank_scatter_kernel(int64_t n, const int64_t* __restrict__ head, const int64_t* __restrict__ excl,
+                                    const uint64_t* __restrict__ idx, uint32_t* __restrict__ rank) {
+    int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(k < n)
+        rank[idx[k]] = uint32_t(excl[k] + head[k] - 1);
+}
+
+// ---- 3. term counting / emission ------------------------------------------------------------
+// The reference's per-quadruple logic (TwoParticleGFPart.cpp:241-297): weights summed in the
+// order w1+w2+w3+w4, matrix element ((O1*O2)*O3)*CX4*sign, four candidate terms with their
+// own coefficient thresholds.
+struct Quad {
+    double C, P1, P2, P3, wi, wj, wk, wl;
+};
+
+template <class F>
+__device__ inline void for_each_term(G2Dev const& D, int i1, int i2, int i3, double ab, double s3, F&& f) {
+    for(int i4 = 0; i4 < D.n4; ++i4) {
+        double c = mat(D.O3, i3, i4);
+        double d = mat(D.CX4, i4, i1);
+        if(c == 0.0 || d == 0.0)
+            continue;
+        double wl = D.w4[i4];
+        if(!(s3 + wl > D.tol))
+            continue;
+        double C = ab * c * d * D.sign;
+        double wi = D.w1[i1], wj = D.w2[i2], wk = D.w3[i3];
+        double tol = D.tol;
+        // kind 0: non-resonant z2, 1: non-resonant z4, 2: resonant z2z3, 3: resonant z1z2
+        if(fabs(C * (wj + wk)) > tol)
+            f(0, i4);
+        if(fabs(C * (wi + wl)) > tol)
+            f(1, i4);
+        if(fabs(C * D.beta * wj) > tol || fabs(C * (wj - wl)) > tol)
+            f(2, i4);
+        if(fabs(C * D.beta * wi) > tol || fabs(C * (wk - wi)) > tol)
+            f(3, i4);
+    }
+}
+
+struct RowCtx {
+    int i1, i2, i3;
+    double ab, s3;
+    bool active;
+};
+
+__device__ inline RowCtx row_ctx(G2Dev const& D, const uint64_t* __restrict__ pairs, int64_t pair_pos, int i3) {
+    RowCtx r;
+    uint64_t pr = pairs[pair_pos];
+    r.i1 = int(pr / uint64_t(D.n2));
+    r.i2 = int(pr % uint64_t(D.n2));
+    r.i3 = i3;
+    double a = mat(D.O1, r.i1, r.i2), b = mat(D.O2, r.i2, i3);
+    r.ab = a * b;
+    r.s3 = D.w1[r.i1] + D.w2[r.i2] + D.w3[i3];
+    // w4 is non-increasing (ascending energies), so w4[0] bounds every i4
+    r.active = (a != 0.0 && b != 0.0) && (r.s3 + D.w4[0] > D.tol);
+    return r;
+}
+
+__global__ void g2_count_kernel(G2Dev D, const uint64_t* __restrict__ pairs, int64_t nrows, int* __restrict__ cnt) {
+    int64_t row = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(row >= nrows)
+        return;
+    RowCtx r = row_ctx(D, pairs, row / D.n3, int(row % D.n3));
+    int c = 0;
+    if(r.active)
+        for_each_term(D, r.i1, r.i2, r.i3, r.ab, r.s3, [&](int, int) { ++c; });
+    cnt[row] = c;
+}
+
+__global__ void pair_sum_kernel(int64_t npairs, int n3, const int* __restrict__ cnt, int64_t* __restrict__ out) {
+    int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(p >= npairs)
+        return;
+    int64_t s = 0;
+    for(int i3 = 0; i3 < n3; ++i3)
+        s += cnt[p * n3 + i3];
+    out[p] = s;
+}
+
+__global__ void widen_kernel(int64_t n, const int* __restrict__ in, int64_t* __restrict__ out) {
+    int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(k < n)
+        out[k] = in[k];
+}
+
+__global__ void g2_emit_kernel(G2Dev D, const uint64_t* __restrict__ pairs, int64_t row0, int64_t nrows, int64_t ntot,
+                               const int64_t* __restrict__ off, const uint32_t* __restrict__ r12,
+                               const uint32_t* __restrict__ r23, const uint32_t* __restrict__ r34,
+                               uint64_t* __restrict__ keys, uint64_t* __restrict__ vals) {
+    int64_t lr = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(lr >= nrows)
+        return;
+    int64_t row = row0 + lr;
+    RowCtx r = row_ctx(D, pairs, row / D.n3, int(row % D.n3));
+    if(!r.active)
+        return;
+    int64_t at = off[lr];
+    int64_t const end = (lr + 1 < nrows) ? off[lr + 1] : ntot; // guard: never write past this row's slots
+    uint64_t k12 = uint64_t(r12[int64_t(r.i1) * D.n2 + r.i2]) << 42;
+    uint64_t k23 = uint64_t(r23[int64_t(r.i2) * D.n3 + r.i3]) << 21;
+    uint64_t base = uint64_t(r.i1) | (uint64_t(r.i2) << 16) | (uint64_t(r.i3) << 32);
+    for_each_term(D, r.i1, r.i2, r.i3, r.ab, r.s3, [&](int kind, int i4) {
+        if(at >= end)
+            return;
+        keys[at] = (uint64_t(kind) << 62) | k12 | k23 | uint64_t(r34[int64_t(r.i3) * D.n4 + i4]);
+        vals[at] = base | (uint64_t(i4) << 48);
+        ++at;
+    });
+}
+
+__global__ void heads_kernel(int64_t n, const uint64_t* __restrict__ keys, int64_t* __restrict__ head) {
+    int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(k < n)
+        head[k] = (k == 0 || keys[k] != keys[k - 1]) ? 1 : 0;
+}
+__global__ void group_start_kernel(int64_t n, const int64_t* __restrict__ head, const int64_t* __restrict__ gid,
+                                   int64_t* __restrict__ start) {
+    int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(k < n && head[k])
+        start[gid[k]] = k;
+}
+
+// One thread per key group: sum the members' coefficients (pole = count-weighted mean).
+__global__ void g2_reduce_kernel(G2Dev D, int64_t n, int64_t ngroups, const int64_t* __restrict__ start,
+                                 const uint64_t* __restrict__ keys, const uint64_t* __restrict__ vals, double thr,
+                                 double* __restrict__ g, int64_t* __restrict__ live) {
+    int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(gi >= ngroups)
+        return;
+    int64_t s = start[gi], e = (gi + 1 < ngroups) ? start[gi + 1] : n;
+    int kind = int(keys[s] >> 62);
+    bool alive = false;
+    double c0 = 0, c1 = 0, p1 = 0, p2 = 0, p3 = 0;
+    long cnt = 0;
+    for(int64_t k = s; k < e; ++k) {
+        uint64_t v = vals[k];
+        int i1 = int(v & 0xffff), i2 = int((v >> 16) & 0xffff), i3 = int((v >> 32) & 0xffff), i4 = int(v >> 48);
+        double C = mat(D.O1, i1, i2) * mat(D.O2, i2, i3) * mat(D.O3, i3, i4) * mat(D.CX4, i4, i1) * D.sign;
+        double wi = D.w1[i1], wj = D.w2[i2], wk = D.w3[i3], wl = D.w4[i4];
+        double a, b = 0.0;
+        if(kind == 0)
+            a = -C * (wj + wk);
+        else if(kind == 1)
+            a = C * (wi + wl);
+        else if(kind == 2) {
+            a = -C * D.beta * wj;
+            b = C * (wj - wl);
+        } else {
+            a = C * D.beta * wi;
+            b = C * (wk - wi);
+        }
+        double P1 = D.E2[i2] - D.E1[i1], P2 = D.E3[i3] - D.E2[i2], P3 = D.E4[i4] - D.E3[i3];
+        if(!alive) {
+            alive = true;
+            c0 = a;
+            c1 = b;
+            p1 = P1;
+            p2 = P2;
+            p3 = P3;
+            cnt = 1;
+        } else {
+            c0 += a;
+            c1 += b;
+            p1 += P1;
+            p2 += P2;
+            p3 += P3;
+            ++cnt;
+            if(fabs(c0) <= thr && fabs(c1) <= thr)
+                alive = false;
+        }
+    }
+    double inv = 1.0 / double(cnt);
+    g[gi * 5 + 0] = c0;
+    g[gi * 5 + 1] = c1;
+    g[gi * 5 + 2] = p1 * inv;
+    g[gi * 5 + 3] = p2 * inv;
+    g[gi * 5 + 4] = p3 * inv;
+    live[gi] = alive ? 1 : 0;
+}
+
+// scatter live groups into kind-segmented SoA output; seg_base[kind] = first index of the kind
+__global__ void g2_compact_kernel(int64_t ngroups, const int64_t* __restrict__ live, const int64_t* __restrict__ lpos,
+                                  const double* __restrict__ g, const int64_t* __restrict__ start,
+                                  const uint64_t* __restrict__ keys, int64_t nr_cap, int64_t r_cap,
+                                  const int64_t* __restrict__ kind_first, double* __restrict__ nr,
+                                  double* __restrict__ rs) {
+    int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(gi >= ngroups || !live[gi])
+        return;
+    int kind = int(keys[start[gi]] >> 62);
+    int64_t at = lpos[gi];
+    if(kind < 2) {
+        nr[0 * nr_cap + at] = g[gi * 5 + 0];
+        nr[1 * nr_cap + at] = g[gi * 5 + 2];
+        nr[2 * nr_cap + at] = g[gi * 5 + 3];
+        nr[3 * nr_cap + at] = g[gi * 5 + 4];
+    } else {
+        at -= kind_first[2];
+        rs[0 * r_cap + at] = g[gi * 5 + 0];
+        rs[1 * r_cap + at] = g[gi * 5 + 1];
+        rs[2 * r_cap + at] = g[gi * 5 + 2];
+        rs[3 * r_cap + at] = g[gi * 5 + 3];
+        rs[4 * r_cap + at] = g[gi * 5 + 4];
+    }
+}
+
+__global__ void kind_count_kernel(int64_t ngroups, const int64_t* __restrict__ live, const int64_t* __restrict__ start,
+                                  const uint64_t* __restrict__ keys, unsigned long long* __restrict__ cnt) {
+    int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(gi < ngroups && live[gi])
+        atomicAdd(&cnt[keys[start[gi]] >> 62], 1ull);
+}
+
+// ---- 4. evaluation --------------------------------------------------------------------------
+constexpr int kChunk = 256;
+constexpr int kFPT = 2; // frequencies per thread
+
+__device__ inline double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+__constant__ int c_perm3[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+
+struct Freq {
+    double z1r, z1i, z2r, z2i, z3r, z3i;
+};
+
+__device__ inline Freq load_freq(const double* __restrict__ fz, int f, int perm) {
+    // TwoParticleGFPart.cpp:309-315: F = {z1, z2, -z3}, permuted by the part's permutation
+    double F[3][2] = {{fz[6 * f + 0], fz[6 * f + 1]}, {fz[6 * f + 2], fz[6 * f + 3]}, {-fz[6 * f + 4], -fz[6 * f + 5]}};
+    Freq q;
+    int a = c_perm3[perm][0], b = c_perm3[perm][1], c = c_perm3[perm][2];
+    q.z1r = F[a][0];
+    q.z1i = F[a][1];
+    q.z2r = F[b][0];
+    q.z2i = F[b][1];
+    q.z3r = F[c][0];
+    q.z3i = F[c][1];
+    return q;
+}
+
+// KIND 0/1: NonResonantTerm::operator() (TwoParticleGFPart.hpp:475-479)
+// KIND 2/3: ResonantTerm::operator() (TwoParticleGFPart.hpp:481-495), resonance |D| <= res
+template <int KIND>
+__global__ void __launch_bounds__(256) g2_eval_kernel(int64_t nterms, int64_t slice_len, const double* __restrict__ T,
+                                                      int64_t cap, int nfreq, const double* __restrict__ fz, int perm,
+                                                      double res, double* __restrict__ partial) {
+    constexpr int NF = (KIND < 2) ? 4 : 5;
+    __shared__ double sT[NF][kChunk];
+    int const fbase = (blockIdx.x * blockDim.x + threadIdx.x) * kFPT;
+    int64_t const t0 = int64_t(blockIdx.y) * slice_len;
+    int64_t const t1 = min(nterms, t0 + slice_len);
+    Freq q[kFPT];
+    double s12r[kFPT], s12i[kFPT], s23r[kFPT], s23i[kFPT], s123r[kFPT], s123i[kFPT];
+    double ar[kFPT], ai[kFPT];
+#pragma unroll
+    for(int u = 0; u < kFPT; ++u) {
+        int f = min(fbase + u, nfreq - 1);
+        q[u] = load_freq(fz, f, perm);
+        s12r[u] = q[u].z1r + q[u].z2r;
+        s12i[u] = q[u].z1i + q[u].z2i;
+        s23r[u] = q[u].z2r + q[u].z3r;
+        s23i[u] = q[u].z2i + q[u].z3i;
+        s123r[u] = s12r[u] + q[u].z3r;
+        s123i[u] = s12i[u] + q[u].z3i;
+        ar[u] = 0.0;
+        ai[u] = 0.0;
+    }
+    double const res2 = res * res;
+    for(int64_t c0 = t0; c0 < t1; c0 += kChunk) {
+        int cnt = int(min<int64_t>(kChunk, t1 - c0));
+        __syncthreads();
+        if(threadIdx.x < cnt) {
+#pragma unroll
+            for(int k = 0; k < NF; ++k)
+                sT[k][threadIdx.x] = T[k * cap + c0 + threadIdx.x];
+        }
+        __syncthreads();
+        for(int k = 0; k < cnt; ++k) {
+            double const P1 = sT[NF - 3][k], P2 = sT[NF - 2][k], P3 = sT[NF - 1][k];
+            if(KIND < 2) {
+                double const C = sT[0][k];
+#pragma unroll
+                for(int u = 0; u < kFPT; ++u) {
+                    double d1r = q[u].z1r - P1, d1i = q[u].z1i;
+                    double d3r = q[u].z3r - P3, d3i = q[u].z3i;
+                    double d2r, d2i;
+                    if(KIND == 0) {
+                        d2r = q[u].z2r - P2;
+                        d2i = q[u].z2i;
+                    } else {
+                        d2r = s123r[u] - P1 - P2 - P3;
+                        d2i = s123i[u];
+                    }
+                    double xr = d1r * d2r - d1i * d2i, xi = d1r * d2i + d1i * d2r;
+                    double Dr = xr * d3r - xi * d3i, Di = xr * d3i + xi * d3r;
+                    double s = C * rcp_nr(fma(Dr, Dr, Di * Di));
+                    ar[u] = fma(s, Dr, ar[u]);
+                    ai[u] = fma(-s, Di, ai[u]);
+                }
+            } else {
+                double const Rc = sT[0][k], Nc = sT[1][k];
+#pragma unroll
+                for(int u = 0; u < kFPT; ++u) {
+                    double d1r = q[u].z1r - P1, d1i = q[u].z1i;
+                    double d3r = q[u].z3r - P3, d3i = q[u].z3i;
+                    double Qr = d1r * d3r - d1i * d3i, Qi = d1r * d3i + d1i * d3r;
+                    double Er, Ei;
+                    if(KIND == 3) {
+                        Er = s12r[u] - P1 - P2;
+                        Ei = s12i[u];
+                    } else {
+                        Er = s23r[u] - P2 - P3;
+                        Ei = s23i[u];
+                    }
+                    bool const resonant = fma(Er, Er, Ei * Ei) <= res2;
+                    // value = (resonant ? Rc : Nc / E) / Q = K / G
+                    double Gr = resonant ? Qr : (Er * Qr - Ei * Qi);
+                    double Gi = resonant ? Qi : (Er * Qi + Ei * Qr);
+                    double K = resonant ? Rc : Nc;
+                    double s = K * rcp_nr(fma(Gr, Gr, Gi * Gi));
+                    ar[u] = fma(s, Gr, ar[u]);
+                    ai[u] = fma(-s, Gi, ai[u]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for(int u = 0; u < kFPT; ++u) {
+        int f = fbase + u;
+        if(f < nfreq) {
+            int64_t o = (int64_t(blockIdx.y) * nfreq + f) * 2;
+            partial[o] += ar[u];
+            partial[o + 1] += ai[u];
+        }
+    }
+}
+
+__global__ void g2_eval_reduce_kernel(int nfreq, int nslices, const double* __restrict__ partial, double* __restrict__ acc) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if(f >= nfreq)
+        return;
+    double sr = 0, si = 0;
+    for(int s = 0; s < nslices; ++s) {
+        sr += partial[(int64_t(s) * nfreq + f) * 2];
+        si += partial[(int64_t(s) * nfreq + f) * 2 + 1];
+    }
+    acc[2 * f] += sr;
+    acc[2 * f + 1] += si;
+}
+
+struct TermSpan {
+    const double* base; // SoA base (NF arrays of stride cap)
+    int64_t cap, begin, count;
+};
+
+int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res, int nfreq, const double* d_freq,
+                   double* d_acc) {
+    int64_t total = 0;
+    for(int k = 0; k < 4; ++k)
+        total += spans[k].count;
+    if(total == 0 || nfreq == 0)
+        return POMX_OK;
+    int ftiles = (nfreq + 256 * kFPT - 1) / (256 * kFPT);
+    int64_t want = std::max<int64_t>(1, 4096 / ftiles);
+    int64_t slice = std::max<int64_t>(kChunk, (total + want - 1) / want);
+    slice = (slice + kChunk - 1) / kChunk * kChunk;
+    int nslices = 1;
+    for(int k = 0; k < 4; ++k)
+        nslices = std::max<int64_t>(nslices, (spans[k].count + slice - 1) / slice);
+    WsBuf<double> partial(ctx, size_t(nslices) * nfreq * 2);
+    int rc = pomx_fill_zero(ctx, partial.p, sizeof(double) * size_t(nslices) * nfreq * 2);
+    if(rc)
+        return rc;
+    eval_timer_begin(ctx);
+    for(int k = 0; k < 4; ++k) {
+        TermSpan const& s = spans[k];
+        if(!s.count)
+            continue;
+        int ns = int((s.count + slice - 1) / slice);
+        dim3 grid(ftiles, ns);
+        const double* T = s.base + s.begin;
+        switch(k) {
+        case 0: g2_eval_kernel<0><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, nfreq, d_freq, perm, res, partial); break;
+        case 1: g2_eval_kernel<1><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, nfreq, d_freq, perm, res, partial); break;
+        case 2: g2_eval_kernel<2><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, nfreq, d_freq, perm, res, partial); break;
+        default: g2_eval_kernel<3><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, nfreq, d_freq, perm, res, partial); break;
+        }
+    }
+    g2_eval_reduce_kernel<<<grid_for(nfreq), kBlock, 0, ctx->stream>>>(nfreq, nslices, partial, d_acc);
+    POMX_CHECK_LAUNCH(ctx);
+    eval_timer_end(ctx);
+    return POMX_OK;
+}
+
+int read_i64(pomx_ctx* ctx, const int64_t* d, int64_t* h) {
+    POMX_HIP(ctx, hipMemcpyAsync(h, d, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return POMX_OK;
+}
+
+// dense ranks of round((Ey - Ex)/res) over all (x, y); 'sorted' receives the (x*ny+y) order
+int rank_table(pomx_ctx* ctx, int nx, int ny, const double* Ex, const double* Ey, double res, WsBuf<uint32_t>& rank,
+               WsBuf<uint64_t>* sorted, int64_t* nranks, int* d_err) {
+    int64_t n = int64_t(nx) * ny;
+    WsBuf<uint64_t> k0(ctx, n), k1(ctx, n), v0(ctx, n), v1(ctx, n);
+    WsBuf<int64_t> head(ctx, n), excl(ctx, n);
+    bins_kernel<<<grid_for(n), kBlock, 0, ctx->stream>>>(nx, ny, Ex, Ey, res, k0, v0, d_err);
+    POMX_CHECK_LAUNCH(ctx);
+    int rc = pomx_sort_pairs(ctx, n, k0, k1, v0, v1, 0, 64);
+    if(rc)
+        return rc;
+    rank_heads_kernel<<<grid_for(n), kBlock, 0, ctx->stream>>>(n, k1, head);
+    if((rc = pomx_exclusive_scan(ctx, n, head, excl)))
+        return rc;
+    rank = WsBuf<uint32_t>(ctx, n);
+    rank_scatter_kernel<<<grid_for(n), kBlock, 0, ctx->stream>>>(n, head, excl, v1, rank);
+    POMX_CHECK_LAUNCH(ctx);
+    int64_t lh = 0, le = 0;
+    if((rc = read_i64(ctx, head.p + n - 1, &lh)) || (rc = read_i64(ctx, excl.p + n - 1, &le)))
+        return rc;
+    *nranks = lh + le;
+    if(sorted) {
+        *sorted = WsBuf<uint64_t>(ctx, n);
+        POMX_HIP(ctx, hipMemcpyAsync(sorted->p, v1.p, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    return POMX_OK;
+}
+
+// Chunk budget in terms (keys + values + sort buffers: 32 bytes per term).
+constexpr int64_t kChunkTerms = int64_t(1) << 27;
+
+} // namespace
+
+extern "C" int pomx_g2_terms_eval(pomx_ctx* ctx, const pomx_g2_terms* t, int perm, double res, int nfreq,
+                                  const double* d_freq, double* d_acc) {
+    POMX_REQUIRE(ctx, t && perm >= 0 && perm < 6 && nfreq >= 0, "pomx_g2_terms_eval: bad arguments");
+    int64_t nnr = t->count[0] + t->count[1], nr = t->count[2] + t->count[3];
+    const double* NR = t->data;
+    const double* R = t->data + 4 * nnr;
+    TermSpan spans[4] = {{NR, nnr, 0, t->count[0]}, {NR, nnr, t->count[0], t->count[1]},
+                         {R, nr, 0, t->count[2]},   {R, nr, t->count[2], t->count[3]}};
+    int rc = launch_g2_eval(ctx, spans, perm, res, nfreq, d_freq, d_acc);
+    if(rc)
+        return rc;
+    POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return POMX_OK;
+}
+
+extern "C" int pomx_g2_part_compute(pomx_ctx* ctx, const pomx_g2_part* part, double beta, double res, double tol,
+                                    int nfreq, const double* d_freq, double* d_acc, pomx_g2_terms* keep,
+                                    int64_t* h_counts, int64_t* h_raw_terms) {
+    POMX_REQUIRE(ctx, part && res > 0.0 && nfreq >= 0 && (nfreq == 0 || (d_freq && d_acc)),
+                 "pomx_g2_part_compute: bad arguments");
+    POMX_REQUIRE(ctx, part->perm >= 0 && part->perm < 6, "pomx_g2_part_compute: bad permutation");
+    pomx_g2_part const& P = *part;
+    POMX_REQUIRE(ctx, P.n1 >= 1 && P.n2 >= 1 && P.n3 >= 1 && P.n4 >= 1, "pomx_g2_part_compute: empty block");
+    if(P.n1 > 65535 || P.n2 > 65535 || P.n3 > 65535 || P.n4 > 65535)
+        return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_g2_part_compute: block larger than 65535");
+    G2Dev D{P.n1, P.n2, P.n3, P.n4, P.O1, P.O2, P.O3, P.CX4, P.E1, P.E2, P.E3, P.E4,
+            P.w1, P.w2, P.w3, P.w4, P.sign, beta, tol};
+    int64_t counts[4] = {0, 0, 0, 0};
+    if(keep) {
+        keep->data = nullptr;
+        for(int k = 0; k < 4; ++k)
+            keep->count[k] = 0;
+    }
+    WsBuf<int> err(ctx, 1);
+    int rc = pomx_fill_zero(ctx, err.p, sizeof(int));
+    if(rc)
+        return rc;
+    // 1. rank tables
+    WsBuf<uint32_t> r12, r23, r34;
+    WsBuf<uint64_t> pairs;
+    int64_t nr12 = 0, nr23 = 0, nr34 = 0;
+    if((rc = rank_table(ctx, P.n1, P.n2, P.E1, P.E2, res, r12, &pairs, &nr12, err)) ||
+       (rc = rank_table(ctx, P.n2, P.n3, P.E2, P.E3, res, r23, nullptr, &nr23, err)) ||
+       (rc = rank_table(ctx, P.n3, P.n4, P.E3, P.E4, res, r34, nullptr, &nr34, err)))
+        return rc;
+    if(nr12 >= (int64_t(1) << 20) || nr23 >= (int64_t(1) << 21) || nr34 >= (int64_t(1) << 21))
+        return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_g2_part_compute: too many distinct pole bins for the 64-bit key");
+    // 2. count terms per row, per pair, and cut chunks at r12 group boundaries
+    int64_t const npairs = int64_t(P.n1) * P.n2;
+    int64_t const nrows = npairs * P.n3;
+    WsBuf<int> rowcnt(ctx, nrows);
+    g2_count_kernel<<<grid_for(nrows), kBlock, 0, ctx->stream>>>(D, pairs, nrows, rowcnt);
+    WsBuf<int64_t> paircnt(ctx, npairs);
+    pair_sum_kernel<<<grid_for(npairs), kBlock, 0, ctx->stream>>>(npairs, P.n3, rowcnt, paircnt);
+    POMX_CHECK_LAUNCH(ctx);
+    std::vector<int64_t> hpair(npairs);
+    std::vector<uint32_t> hr12(npairs);
+    std::vector<uint64_t> hpairs(npairs);
+    POMX_HIP(ctx, hipMemcpyAsync(hpair.data(), paircnt.p, sizeof(int64_t) * npairs, hipMemcpyDeviceToHost, ctx->stream));
+    POMX_HIP(ctx, hipMemcpyAsync(hr12.data(), r12.p, sizeof(uint32_t) * npairs, hipMemcpyDeviceToHost, ctx->stream));
+    POMX_HIP(ctx, hipMemcpyAsync(hpairs.data(), pairs.p, sizeof(uint64_t) * npairs, hipMemcpyDeviceToHost, ctx->stream));
+    int herr = 0;
+    POMX_HIP(ctx, hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if(herr)
+        return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_g2_part_compute: |pole/pole_res| out of range");
+    int64_t quads_total = 0;
+    std::vector<std::pair<int64_t, int64_t>> chunks; // [pair_begin, pair_end)
+    {
+        int64_t b = 0, acc = 0;
+        for(int64_t p = 0; p < npairs; ++p) {
+            bool boundary = (p == 0) || hr12[hpairs[p]] != hr12[hpairs[p - 1]];
+            if(boundary && acc > 0 && acc + hpair[p] > kChunkTerms) {
+                chunks.emplace_back(b, p);
+                b = p;
+                acc = 0;
+            }
+            acc += hpair[p];
+            quads_total += hpair[p];
+        }
+        if(npairs > b)
+            chunks.emplace_back(b, npairs);
+    }
+    struct KeptChunk {
+        double* nr;
+        double* r;
+        int64_t nr_cap, r_cap, c[4];
+    };
+    std::vector<KeptChunk> kept;
+    for(auto const& ch : chunks) {
+        int64_t row0 = ch.first * P.n3, nrow = (ch.second - ch.first) * P.n3;
+        int64_t n = 0;
+        for(int64_t p = ch.first; p < ch.second; ++p)
+            n += hpair[p];
+        if(n == 0)
+            continue;
+        WsBuf<int64_t> off(ctx, nrow);
+        WsBuf<int64_t> wide(ctx, nrow);
+        widen_kernel<<<grid_for(nrow), kBlock, 0, ctx->stream>>>(nrow, rowcnt.p + row0, wide);
+        if((rc = pomx_exclusive_scan(ctx, nrow, wide, off)))
+            return rc;
+        WsBuf<uint64_t> k0(ctx, n), k1(ctx, n), v0(ctx, n), v1(ctx, n);
+        g2_emit_kernel<<<grid_for(nrow), kBlock, 0, ctx->stream>>>(D, pairs, row0, nrow, n, off, r12, r23, r34, k0, v0);
+        POMX_CHECK_LAUNCH(ctx);
+        if((rc = pomx_sort_pairs(ctx, n, k0, k1, v0, v1, 0, 64)))
+            return rc;
+        WsBuf<int64_t> head(ctx, n), gid(ctx, n);
+        heads_kernel<<<grid_for(n), kBlock, 0, ctx->stream>>>(n, k1, head);
+        if((rc = pomx_exclusive_scan(ctx, n, head, gid)))
+            return rc;
+        int64_t lh = 0, lg = 0;
+        if((rc = read_i64(ctx, head.p + n - 1, &lh)) || (rc = read_i64(ctx, gid.p + n - 1, &lg)))
+            return rc;
+        int64_t ng = lh + lg;
+        WsBuf<int64_t> start(ctx, ng), live(ctx, ng), lpos(ctx, ng);
+        WsBuf<double> g(ctx, size_t(ng) * 5);
+        group_start_kernel<<<grid_for(n), kBlock, 0, ctx->stream>>>(n, head, gid, start);
+        // TermList erases a merged term when |coeff| <= tol/(size+1); size is taken as the
+        // number of groups in this chunk
+        double thr = tol / double(ng + 1);
+        g2_reduce_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, n, ng, start, k1, v1, thr, g, live);
+        WsBuf<unsigned long long> kc(ctx, 4);
+        if((rc = pomx_fill_zero(ctx, kc.p, 4 * sizeof(unsigned long long))))
+            return rc;
+        kind_count_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(ng, live, start, k1, kc);
+        if((rc = pomx_exclusive_scan(ctx, ng, live, lpos)))
+            return rc;
+        unsigned long long hk[4];
+        POMX_HIP(ctx, hipMemcpyAsync(hk, kc.p, sizeof(hk), hipMemcpyDeviceToHost, ctx->stream));
+        POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        int64_t c[4] = {int64_t(hk[0]), int64_t(hk[1]), int64_t(hk[2]), int64_t(hk[3])};
+        int64_t nnr = c[0] + c[1], nr = c[2] + c[3];
+        int64_t kf[4] = {0, c[0], c[0] + c[1], c[0] + c[1] + c[2]};
+        WsBuf<int64_t> d_kf(ctx, 4);
+        POMX_HIP(ctx, hipMemcpyAsync(d_kf.p, kf, sizeof(kf), hipMemcpyHostToDevice, ctx->stream));
+        double* nrbuf = static_cast<double*>(ws_get(ctx, sizeof(double) * 4 * std::max<int64_t>(nnr, 1)));
+        double* rbuf = static_cast<double*>(ws_get(ctx, sizeof(double) * 5 * std::max<int64_t>(nr, 1)));
+        if(!nrbuf || !rbuf)
+            return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+        g2_compact_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(ng, live, lpos, g, start, k1, std::max<int64_t>(nnr, 1),
+                                                                     std::max<int64_t>(nr, 1), d_kf, nrbuf, rbuf);
+        POMX_CHECK_LAUNCH(ctx);
+        for(int k = 0; k < 4; ++k)
+            counts[k] += c[k];
+        if(nfreq > 0) {
+            TermSpan spans[4] = {{nrbuf, std::max<int64_t>(nnr, 1), 0, c[0]},
+                                 {nrbuf, std::max<int64_t>(nnr, 1), c[0], c[1]},
+                                 {rbuf, std::max<int64_t>(nr, 1), 0, c[2]},
+                                 {rbuf, std::max<int64_t>(nr, 1), c[2], c[3]}};
+            if((rc = launch_g2_eval(ctx, spans, P.perm, res, nfreq, d_freq, d_acc)))
+                return rc;
+        }
+        if(keep)
+            kept.push_back(KeptChunk{nrbuf, rbuf, std::max<int64_t>(nnr, 1), std::max<int64_t>(nr, 1),
+                                     {c[0], c[1], c[2], c[3]}});
+        else {
+            ws_put(ctx, nrbuf);
+            ws_put(ctx, rbuf);
+        }
+    }
+    if(keep) {
+        int64_t nnr = counts[0] + counts[1], nr = counts[2] + counts[3];
+        size_t bytes = sizeof(double) * size_t(4 * nnr + 5 * nr + 1);
+        POMX_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&keep->data), bytes));
+        double* NR = keep->data;
+        double* R = keep->data + 4 * nnr;
+        // kind-major concatenation of the chunks
+        int64_t at[4] = {0, counts[0], 0, counts[2]};
+        for(auto const& kc : kept) {
+            for(int k = 0; k < 2; ++k) {
+                int64_t src0 = (k == 0) ? 0 : kc.c[0];
+                for(int f = 0; f < 4; ++f)
+                    if(kc.c[k])
+                        POMX_HIP(ctx, hipMemcpyAsync(NR + f * nnr + at[k], kc.nr + f * kc.nr_cap + src0,
+                                                     sizeof(double) * kc.c[k], hipMemcpyDeviceToDevice, ctx->stream));
+                at[k] += kc.c[k];
+            }
+            for(int k = 2; k < 4; ++k) {
+                int64_t src0 = (k == 2) ? 0 : kc.c[2];
+                for(int f = 0; f < 5; ++f)
+                    if(kc.c[k])
+                        POMX_HIP(ctx, hipMemcpyAsync(R + f * nr + at[k], kc.r + f * kc.r_cap + src0,
+                                                     sizeof(double) * kc.c[k], hipMemcpyDeviceToDevice, ctx->stream));
+                at[k] += kc.c[k];
+            }
+        }
+        POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for(auto const& kc : kept) {
+            ws_put(ctx, kc.nr);
+            ws_put(ctx, kc.r);
+        }
+        for(int k = 0; k < 4; ++k)
+            keep->count[k] = counts[k];
+    }
+    POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if(h_counts)
+        for(int k = 0; k < 4; ++k)
+            h_counts[k] = counts[k];
+    if(h_raw_terms)
+        *h_raw_terms = quads_total;
+    return POMX_OK;
+}

@@ -1,0 +1,98 @@
+// Internal definitions shared by the gfx950 kernels behind include/pomx_c.h.
+#pragma once
+
+#include "../../include/pomx_c.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+struct pomx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // caching pool for temporaries: size -> free blocks
+    std::multimap<size_t, void*> free_blocks;
+    std::map<void*, size_t> live_blocks;
+    // timing of the Lehmann evaluation kernels (the dominant kernel)
+    bool timing = false;
+    double eval_ms = 0.0;
+    int64_t eval_launches = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+int pomx_fail(pomx_ctx* ctx, int code, std::string const& msg);
+
+#define POMX_HIP(ctx, expr)                                                                          \
+    do {                                                                                             \
+        hipError_t pomx_e_ = (expr);                                                                 \
+        if(pomx_e_ != hipSuccess)                                                                    \
+            return pomx_fail(ctx, POMX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(pomx_e_)); \
+    } while(0)
+
+#define POMX_CHECK_LAUNCH(ctx) POMX_HIP(ctx, hipGetLastError())
+
+#define POMX_REQUIRE(ctx, cond, msg)                                                                 \
+    do {                                                                                             \
+        if(!(cond))                                                                                  \
+            return pomx_fail(ctx, POMX_ERR_ARG, msg);                                                \
+    } while(0)
+
+// Temporary device memory from the context's caching pool (returned with ws_put).
+void* ws_get(pomx_ctx* ctx, size_t bytes);
+void ws_put(pomx_ctx* ctx, void* p);
+
+// RAII holder for a pooled temporary.
+template <class T> struct WsBuf {
+    pomx_ctx* ctx = nullptr;
+    T* p = nullptr;
+    WsBuf() = default;
+    WsBuf(pomx_ctx* c, size_t n) : ctx(c), p(static_cast<T*>(ws_get(c, (n ? n : 1) * sizeof(T)))) {}
+    WsBuf(WsBuf const&) = delete;
+    WsBuf& operator=(WsBuf const&) = delete;
+    WsBuf(WsBuf&& o) noexcept : ctx(o.ctx), p(o.p) { o.p = nullptr; }
+    WsBuf& operator=(WsBuf&& o) noexcept {
+        if(this != &o) {
+            if(p)
+                ws_put(ctx, p);
+            ctx = o.ctx;
+            p = o.p;
+            o.p = nullptr;
+        }
+        return *this;
+    }
+    ~WsBuf() {
+        if(p)
+            ws_put(ctx, p);
+    }
+    operator T*() const { return p; }
+};
+
+// Stable LSD radix sort of (uint64 key, uint64 value) pairs on the context stream.
+// keys_in/vals_in are consumed; sorted output lands in keys_out/vals_out.
+int pomx_sort_pairs(pomx_ctx* ctx, int64_t n, uint64_t* keys_in, uint64_t* keys_out, uint64_t* vals_in,
+                    uint64_t* vals_out, int begin_bit, int end_bit);
+// Exclusive prefix sum of int64 (out may alias in).
+int pomx_exclusive_scan(pomx_ctx* ctx, int64_t n, const int64_t* in, int64_t* out);
+
+// Evaluation-kernel timing hooks (HIP events on the context stream).
+void eval_timer_begin(pomx_ctx* ctx);
+void eval_timer_end(pomx_ctx* ctx);
+
+constexpr int kBlock = 256;
+inline unsigned grid_for(int64_t n, int per_block = kBlock) {
+    int64_t g = (n + per_block - 1) / per_block;
+    if(g < 1)
+        g = 1;
+    if(g > (int64_t(1) << 30))
+        g = int64_t(1) << 30;
+    return unsigned(g);
+}
+
+// Matsubara-style complex helpers (device)
+struct dcplx {
+    double re, im;
+};

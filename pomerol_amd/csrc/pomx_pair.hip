@@ -1,0 +1,395 @@
+// K6/K10 (two-operator Lehmann terms), K8 (pole merge) and K9 (evaluation) for the
+// single-particle Green's function and the dynamical susceptibility.
+//
+// Term generation follows GreensFunctionPart::computeImpl (GreensFunctionPart.cpp:65-110) and
+// SusceptibilityPart::computeImpl (SusceptibilityPart.cpp:76-127): candidates are enumerated
+// in the reference's emission order (part, outer index, inner index), flagged, compacted by a
+// prefix sum, keyed by (part, round(pole/res)) and stably radix-sorted, so each group of a
+// TermList (same hash bin, TermList.hpp:65-76) is contiguous and in emission order.  A
+// group's residues are summed in that order and its first pole is kept (GF/chi terms do not
+// average poles, GreensFunctionPart.cpp:34-37).  A running sum that becomes negligible
+// (|R| <= tol/(terms+1)) is dropped and the group restarts, as TermList::add_term does.
+#include "pomx_internal.hpp"
+
+#include <algorithm>
+
+namespace {
+
+struct PairPartDev {
+    int n_out, n_in;
+    pomx_view A, B;
+    const double *E_out, *E_in, *w_out, *w_in;
+    int64_t cand_off; // prefix of n_out * n_in
+};
+
+constexpr int kPartShift = 40;
+constexpr uint64_t kBinMask = (uint64_t(1) << kPartShift) - 1;
+constexpr int64_t kBinOffset = int64_t(1) << 39;
+constexpr uint64_t kZeroPoleBin = kBinMask; // chi |P| < res entries
+
+__device__ inline int find_part(const PairPartDev* parts, int nparts, int64_t g) {
+    int lo = 0, hi = nparts - 1;
+    while(lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if(parts[mid].cand_off <= g)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+struct Cand {
+    int part, i1, i2;
+    double a, b, P;
+};
+
+__device__ inline Cand load_cand(const PairPartDev* parts, int nparts, int64_t g) {
+    Cand c;
+    c.part = find_part(parts, nparts, g);
+    PairPartDev const& p = parts[c.part];
+    int64_t loc = g - p.cand_off;
+    c.i1 = int(loc / p.n_in);
+    c.i2 = int(loc % p.n_in);
+    c.a = p.A.base[c.i1 * p.A.rs + c.i2 * p.A.cs];
+    c.b = p.B.base[c.i2 * p.B.rs + c.i1 * p.B.cs];
+    c.P = p.E_in[c.i2] - p.E_out[c.i1];
+    return c;
+}
+
+// 0: not emitted, 1: term, 2: chi zero-pole contribution
+__device__ inline int classify(int kind, const PairPartDev* parts, Cand const& c, double res, double tol, double& R) {
+    if(c.a == 0.0 || c.b == 0.0)
+        return 0;
+    PairPartDev const& p = parts[c.part];
+    if(kind == POMX_KIND_GF) {
+        R = c.a * c.b * (p.w_out[c.i1] + p.w_in[c.i2]);
+        return fabs(R) > tol ? 1 : 0;
+    }
+    if(fabs(c.P) < res) {
+        R = c.a * c.b * p.w_out[c.i1];
+        return 2;
+    }
+    R = c.a * c.b * (p.w_out[c.i1] - p.w_in[c.i2]);
+    return fabs(R) > tol ? 1 : 0;
+}
+
+__global__ void pair_mark_kernel(int kind, const PairPartDev* __restrict__ parts, int nparts, int64_t ncand,
+                                 double res, double tol, int64_t* __restrict__ flag) {
+    int64_t g = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(g >= ncand)
+        return;
+    Cand c = load_cand(parts, nparts, g);
+    double R;
+    flag[g] = classify(kind, parts, c, res, tol, R) ? 1 : 0;
+}
+
+__global__ void pair_emit_kernel(int kind, const PairPartDev* __restrict__ parts, int nparts, int64_t ncand,
+                                 double res, double tol, const int64_t* __restrict__ pos, int64_t nout,
+                                 uint64_t* __restrict__ keys, uint64_t* __restrict__ vals, int* __restrict__ err) {
+    int64_t g = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(g >= ncand)
+        return;
+    Cand c = load_cand(parts, nparts, g);
+    double R;
+    int cl = classify(kind, parts, c, res, tol, R);
+    if(!cl)
+        return;
+    uint64_t bin;
+    if(cl == 2)
+        bin = kZeroPoleBin;
+    else {
+        double q = round(c.P / res);
+        if(!(fabs(q) < double(kBinOffset - 1))) {
+            err[0] = 1;
+            q = 0;
+        }
+        bin = uint64_t(int64_t(q) + kBinOffset);
+    }
+    int64_t at = pos[g];
+    if(at >= nout)
+        return;
+    keys[at] = (uint64_t(c.part) << kPartShift) | bin;
+    vals[at] = uint64_t(g);
+}
+
+__global__ void heads_kernel(int64_t n, const uint64_t* __restrict__ keys, int64_t* __restrict__ head) {
+    int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(k < n)
+        head[k] = (k == 0 || keys[k] != keys[k - 1]) ? 1 : 0;
+}
+
+__global__ void group_start_kernel(int64_t n, const int64_t* __restrict__ head, const int64_t* __restrict__ gid,
+                                   int64_t* __restrict__ start) {
+    int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(k < n && head[k])
+        start[gid[k]] = k;
+}
+
+// One thread per group: walk members in emission order.
+__global__ void pair_reduce_kernel(int kind, const PairPartDev* __restrict__ parts, int nparts, int64_t n,
+                                   int64_t ngroups, const int64_t* __restrict__ start, const uint64_t* __restrict__ keys,
+                                   const uint64_t* __restrict__ vals, const int64_t* __restrict__ part_groups,
+                                   double res, double tol, double* __restrict__ gR, double* __restrict__ gP,
+                                   int64_t* __restrict__ live, double* __restrict__ zero_pole) {
+    int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(gi >= ngroups)
+        return;
+    int64_t s = start[gi], e = (gi + 1 < ngroups) ? start[gi + 1] : n;
+    uint64_t key = keys[s];
+    int part = int(key >> kPartShift);
+    if((key & kBinMask) == kZeroPoleBin) {
+        double z = 0.0;
+        for(int64_t k = s; k < e; ++k) {
+            Cand c = load_cand(parts, nparts, int64_t(vals[k]));
+            double R;
+            classify(kind, parts, c, res, tol, R);
+            z += R;
+        }
+        zero_pole[part] = z;
+        live[gi] = 0;
+        return;
+    }
+    double thr = tol / double(part_groups[part] + 1);
+    bool alive = false;
+    double sum = 0.0, pole = 0.0;
+    for(int64_t k = s; k < e; ++k) {
+        Cand c = load_cand(parts, nparts, int64_t(vals[k]));
+        double R;
+        classify(kind, parts, c, res, tol, R);
+        if(!alive) {
+            alive = true;
+            sum = R;
+            pole = c.P;
+        } else {
+            sum += R;
+            if(fabs(sum) <= thr)
+                alive = false;
+        }
+    }
+    gR[gi] = sum;
+    gP[gi] = pole;
+    live[gi] = alive ? 1 : 0;
+}
+
+__global__ void part_group_count_kernel(int64_t ngroups, const int64_t* __restrict__ start,
+                                        const uint64_t* __restrict__ keys, unsigned long long* __restrict__ cnt) {
+    int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(gi >= ngroups)
+        return;
+    uint64_t key = keys[start[gi]];
+    if((key & kBinMask) != kZeroPoleBin)
+        atomicAdd(&cnt[key >> kPartShift], 1ull);
+}
+
+__global__ void compact_kernel(int64_t ngroups, const int64_t* __restrict__ live, const int64_t* __restrict__ pos,
+                               const double* __restrict__ gR, const double* __restrict__ gP,
+                               const int64_t* __restrict__ start, const uint64_t* __restrict__ keys,
+                               double* __restrict__ outR, double* __restrict__ outP,
+                               unsigned long long* __restrict__ part_count) {
+    int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(gi >= ngroups || !live[gi])
+        return;
+    int64_t at = pos[gi];
+    outR[at] = gR[gi];
+    outP[at] = gP[gi];
+    atomicAdd(&part_count[keys[start[gi]] >> kPartShift], 1ull);
+}
+
+// ---- K9: acc[w] += sgn * sum_t R_t / (z_w - P_t) --------------------------------------------
+constexpr int kEvalChunk = 256;
+
+__device__ inline double rcp_nr(double x) {
+    // reciprocal: hardware estimate refined by two Newton-Raphson steps
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+__global__ void __launch_bounds__(256) pair_eval_kernel(int64_t nterms, int64_t slice_len,
+                                                        const double* __restrict__ R, const double* __restrict__ P,
+                                                        int nz, const double* __restrict__ z,
+                                                        double* __restrict__ partial) {
+    __shared__ double sR[kEvalChunk], sP[kEvalChunk];
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t0 = int64_t(blockIdx.y) * slice_len;
+    int64_t t1 = min(nterms, t0 + slice_len);
+    double zr = 0, zi = 0;
+    if(f < nz) {
+        zr = z[2 * f];
+        zi = z[2 * f + 1];
+    }
+    double ar = 0, ai = 0;
+    for(int64_t c0 = t0; c0 < t1; c0 += kEvalChunk) {
+        int64_t cnt = min<int64_t>(kEvalChunk, t1 - c0);
+        __syncthreads();
+        if(threadIdx.x < cnt) {
+            sR[threadIdx.x] = R[c0 + threadIdx.x];
+            sP[threadIdx.x] = P[c0 + threadIdx.x];
+        }
+        __syncthreads();
+        for(int k = 0; k < cnt; ++k) {
+            double dr = zr - sP[k];
+            double inv = rcp_nr(fma(dr, dr, zi * zi));
+            double s = sR[k] * inv;
+            ar = fma(s, dr, ar);
+            ai = fma(-s, zi, ai);
+        }
+    }
+    if(f < nz) {
+        int64_t o = (int64_t(blockIdx.y) * nz + f) * 2;
+        partial[o] = ar;
+        partial[o + 1] = ai;
+    }
+}
+
+__global__ void pair_eval_reduce_kernel(int nz, int nslices, double sgn, const double* __restrict__ partial,
+                                        double* __restrict__ acc) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if(f >= nz)
+        return;
+    double sr = 0, si = 0;
+    for(int s = 0; s < nslices; ++s) {
+        sr += partial[(int64_t(s) * nz + f) * 2];
+        si += partial[(int64_t(s) * nz + f) * 2 + 1];
+    }
+    acc[2 * f] += sgn * sr;
+    acc[2 * f + 1] += sgn * si;
+}
+
+} // namespace
+
+extern "C" int pomx_pair_terms(pomx_ctx* ctx, int kind, int nparts, const pomx_pair_part* h_parts, double res,
+                               double tol, double** d_residue, double** d_pole, int64_t* h_count, double* h_zero_pole) {
+    POMX_REQUIRE(ctx, (kind == POMX_KIND_GF || kind == POMX_KIND_CHI) && nparts >= 0 && d_residue && d_pole && h_count,
+                 "pomx_pair_terms: bad arguments");
+    POMX_REQUIRE(ctx, res > 0.0, "pomx_pair_terms: pole resolution must be positive");
+    POMX_REQUIRE(ctx, nparts < (1 << 23), "pomx_pair_terms: too many parts");
+    *d_residue = nullptr;
+    *d_pole = nullptr;
+    std::vector<PairPartDev> parts(nparts);
+    int64_t ncand = 0;
+    for(int p = 0; p < nparts; ++p) {
+        pomx_pair_part const& q = h_parts[p];
+        POMX_REQUIRE(ctx, q.n_out >= 1 && q.n_in >= 1, "pomx_pair_terms: empty part");
+        parts[p] = PairPartDev{q.n_out, q.n_in, q.A, q.B, q.E_out, q.E_in, q.w_out, q.w_in, ncand};
+        ncand += int64_t(q.n_out) * q.n_in;
+        h_count[p] = 0;
+        if(h_zero_pole)
+            h_zero_pole[p] = 0.0;
+    }
+    if(ncand == 0) {
+        void* a = nullptr;
+        void* b = nullptr;
+        POMX_HIP(ctx, hipMalloc(&a, 8));
+        POMX_HIP(ctx, hipMalloc(&b, 8));
+        *d_residue = static_cast<double*>(a);
+        *d_pole = static_cast<double*>(b);
+        return POMX_OK;
+    }
+    WsBuf<PairPartDev> d_parts(ctx, nparts);
+    WsBuf<int64_t> flag(ctx, ncand), pos(ctx, ncand);
+    WsBuf<int> err(ctx, 1);
+    POMX_HIP(ctx, hipMemcpyAsync(d_parts.p, parts.data(), sizeof(PairPartDev) * nparts, hipMemcpyHostToDevice, ctx->stream));
+    int rc = pomx_fill_zero(ctx, err.p, sizeof(int));
+    if(rc)
+        return rc;
+    pair_mark_kernel<<<grid_for(ncand), kBlock, 0, ctx->stream>>>(kind, d_parts, nparts, ncand, res, tol, flag);
+    POMX_CHECK_LAUNCH(ctx);
+    if((rc = pomx_exclusive_scan(ctx, ncand, flag, pos)))
+        return rc;
+    int64_t last_flag = 0, last_pos = 0;
+    POMX_HIP(ctx, hipMemcpyAsync(&last_flag, flag.p + ncand - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
+    POMX_HIP(ctx, hipMemcpyAsync(&last_pos, pos.p + ncand - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
+    POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int64_t n = last_pos + last_flag;
+    std::vector<unsigned long long> cnt(nparts, 0);
+    std::vector<double> zp(nparts, 0.0);
+    double *outR = nullptr, *outP = nullptr;
+    if(n > 0) {
+        WsBuf<uint64_t> k0(ctx, n), k1(ctx, n), v0(ctx, n), v1(ctx, n);
+        pair_emit_kernel<<<grid_for(ncand), kBlock, 0, ctx->stream>>>(kind, d_parts, nparts, ncand, res, tol, pos, n, k0, v0, err);
+        POMX_CHECK_LAUNCH(ctx);
+        int part_bits = 1;
+        while((1 << part_bits) < nparts)
+            ++part_bits;
+        if((rc = pomx_sort_pairs(ctx, n, k0, k1, v0, v1, 0, kPartShift + part_bits)))
+            return rc;
+        WsBuf<int64_t> head(ctx, n), gid(ctx, n);
+        heads_kernel<<<grid_for(n), kBlock, 0, ctx->stream>>>(n, k1, head);
+        if((rc = pomx_exclusive_scan(ctx, n, head, gid)))
+            return rc;
+        int64_t lh = 0, lg = 0;
+        POMX_HIP(ctx, hipMemcpyAsync(&lh, head.p + n - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
+        POMX_HIP(ctx, hipMemcpyAsync(&lg, gid.p + n - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
+        POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        int64_t ngroups = lh + lg;
+        WsBuf<int64_t> start(ctx, ngroups), live(ctx, ngroups), lpos(ctx, ngroups), pgroups(ctx, nparts);
+        WsBuf<double> gR(ctx, ngroups), gP(ctx, ngroups), d_zp(ctx, nparts);
+        WsBuf<unsigned long long> d_cnt(ctx, nparts);
+        group_start_kernel<<<grid_for(n), kBlock, 0, ctx->stream>>>(n, head, gid, start);
+        if((rc = pomx_fill_zero(ctx, pgroups.p, sizeof(int64_t) * nparts)) || (rc = pomx_fill_zero(ctx, d_zp.p, sizeof(double) * nparts)) ||
+           (rc = pomx_fill_zero(ctx, d_cnt.p, sizeof(unsigned long long) * nparts)))
+            return rc;
+        part_group_count_kernel<<<grid_for(ngroups), kBlock, 0, ctx->stream>>>(
+            ngroups, start, k1, reinterpret_cast<unsigned long long*>(pgroups.p));
+        pair_reduce_kernel<<<grid_for(ngroups), kBlock, 0, ctx->stream>>>(kind, d_parts, nparts, n, ngroups, start, k1, v1,
+                                                                           pgroups, res, tol, gR, gP, live, d_zp);
+        if((rc = pomx_exclusive_scan(ctx, ngroups, live, lpos)))
+            return rc;
+        int64_t ll = 0, lp = 0;
+        POMX_HIP(ctx, hipMemcpyAsync(&ll, live.p + ngroups - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
+        POMX_HIP(ctx, hipMemcpyAsync(&lp, lpos.p + ngroups - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
+        POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        int64_t nlive = ll + lp;
+        POMX_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&outR), sizeof(double) * std::max<int64_t>(nlive, 1)));
+        POMX_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&outP), sizeof(double) * std::max<int64_t>(nlive, 1)));
+        compact_kernel<<<grid_for(ngroups), kBlock, 0, ctx->stream>>>(ngroups, live, lpos, gR, gP, start, k1, outR, outP, d_cnt);
+        POMX_CHECK_LAUNCH(ctx);
+        POMX_HIP(ctx, hipMemcpyAsync(cnt.data(), d_cnt.p, sizeof(unsigned long long) * nparts, hipMemcpyDeviceToHost, ctx->stream));
+        POMX_HIP(ctx, hipMemcpyAsync(zp.data(), d_zp.p, sizeof(double) * nparts, hipMemcpyDeviceToHost, ctx->stream));
+        int herr = 0;
+        POMX_HIP(ctx, hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if(herr) {
+            hipFree(outR);
+            hipFree(outP);
+            return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_pair_terms: |pole/pole_res| exceeds 2^39");
+        }
+    } else {
+        POMX_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&outR), 8));
+        POMX_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&outP), 8));
+    }
+    for(int p = 0; p < nparts; ++p) {
+        h_count[p] = int64_t(cnt[p]);
+        if(h_zero_pole)
+            h_zero_pole[p] = zp[p];
+    }
+    *d_residue = outR;
+    *d_pole = outP;
+    return POMX_OK;
+}
+
+extern "C" int pomx_pair_eval(pomx_ctx* ctx, int kind, int64_t nterms, const double* d_residue, const double* d_pole,
+                              int nz, const double* d_z, double* d_acc) {
+    POMX_REQUIRE(ctx, (kind == POMX_KIND_GF || kind == POMX_KIND_CHI) && nterms >= 0 && nz >= 0,
+                 "pomx_pair_eval: bad arguments");
+    if(nterms == 0 || nz == 0)
+        return POMX_OK;
+    int ftiles = (nz + 255) / 256;
+    int64_t want = std::max<int64_t>(1, 2048 / ftiles);
+    int64_t slice = std::max<int64_t>(kEvalChunk, (nterms + want - 1) / want);
+    slice = (slice + kEvalChunk - 1) / kEvalChunk * kEvalChunk;
+    int nslices = int((nterms + slice - 1) / slice);
+    WsBuf<double> partial(ctx, size_t(nslices) * nz * 2);
+    eval_timer_begin(ctx);
+    pair_eval_kernel<<<dim3(ftiles, nslices), 256, 0, ctx->stream>>>(nterms, slice, d_residue, d_pole, nz, d_z, partial);
+    pair_eval_reduce_kernel<<<grid_for(nz), kBlock, 0, ctx->stream>>>(nz, nslices, kind == POMX_KIND_GF ? 1.0 : -1.0,
+                                                                       partial, d_acc);
+    POMX_CHECK_LAUNCH(ctx);
+    eval_timer_end(ctx);
+    POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return POMX_OK;
+}

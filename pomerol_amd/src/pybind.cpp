@@ -1,0 +1,344 @@
+// Thin Python binding of the pomerol-compatible C++ API (tests, benchmark, autograd wrapper).
+//
+// An ED session builds the model expression with the C++ operator DSL from a term list
+// [(coeff, [(dagger, (label, orbital, spin)), ...]), ...] and then runs the API exactly as a
+// pomerol program would: IndexClassification -> HilbertSpace -> StatesClassification ->
+// Hamiltonian -> DensityMatrix -> FieldOperatorContainer -> GreensFunction / TwoParticleGF /
+// Susceptibility.  Single-particle indices are ParticleIndex values (sorted index tuples).
+#include "pomerol.hpp"
+
+#include <pybind11/complex.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <chrono>
+#include <memory>
+
+namespace py = pybind11;
+using namespace Pomerol;
+using Expr = LatticePresets::RealExpr;
+using IndexInfoT = IndexClassification<std::string, unsigned short, LatticePresets::spin>;
+using HST = HilbertSpace<std::string, unsigned short, LatticePresets::spin>;
+
+namespace {
+
+Expr build_expr(py::list terms) {
+    Expr H;
+    for(auto t : terms) {
+        auto tt = t.cast<py::tuple>();
+        double coeff = tt[0].cast<double>();
+        Expr::monomial_type m;
+        for(auto o : tt[1].cast<py::list>()) {
+            auto ot = o.cast<py::tuple>();
+            auto idx = ot[1].cast<py::tuple>();
+            m.push_back(Expr::op_type{ot[0].cast<bool>(),
+                                      Expr::index_type(idx[0].cast<std::string>(), idx[1].cast<unsigned short>(),
+                                                       LatticePresets::spin(idx[2].cast<int>()))});
+        }
+        H += Expr(coeff, m);
+    }
+    return H;
+}
+
+py::array_t<double> to_np(std::vector<double> const& v) { return py::array_t<double>(v.size(), v.data()); }
+py::array_t<std::complex<double>> to_np(std::vector<ComplexType> const& v) {
+    return py::array_t<std::complex<double>>(v.size(), v.data());
+}
+py::array_t<double> to_np(MatrixType<false> const& m) {
+    py::array_t<double> a({m.rows(), m.cols()});
+    std::copy(m.data(), m.data() + m.size(), a.mutable_data());
+    return a;
+}
+
+struct ED {
+    Expr HExpr;
+    double beta;
+    std::unique_ptr<IndexInfoT> info;
+    std::unique_ptr<HST> HS;
+    std::unique_ptr<StatesClassification> S;
+    std::unique_ptr<Hamiltonian> H;
+    std::unique_ptr<DensityMatrix> rho;
+    std::unique_ptr<FieldOperatorContainer> ops;
+    std::vector<MatrixType<false>> hmats;
+    double t_eigh = 0, t_rot = 0;
+
+    ED(py::list terms, double beta, bool keep_hmat) : HExpr(build_expr(terms)), beta(beta) {
+        info.reset(new IndexInfoT(MakeIndexClassification(HExpr)));
+        HS.reset(new HST(*info, HExpr));
+        HS->compute();
+        S.reset(new StatesClassification());
+        S->compute(*HS);
+        H.reset(new Hamiltonian(*S));
+        H->prepare(HExpr, *HS);
+        if(keep_hmat)
+            for(BlockNumber b = 0; b < S->getNumberOfBlocks(); ++b)
+                hmats.push_back(H->getPart(b).getMatrix<false>());
+        auto t0 = std::chrono::steady_clock::now();
+        H->compute();
+        t_eigh = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        rho.reset(new DensityMatrix(*S, *H, beta));
+        rho->prepare();
+        rho->compute();
+    }
+
+    FieldOperatorContainer& field_ops() {
+        if(!ops) {
+            auto t0 = std::chrono::steady_clock::now();
+            ops.reset(new FieldOperatorContainer(*info, *HS, *S, *H));
+            ops->prepareAll(*HS);
+            ops->computeAll();
+            t_rot = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        return *ops;
+    }
+
+    Expr mono_expr(py::list ops_) {
+        Expr::monomial_type m;
+        for(auto o : ops_) {
+            auto ot = o.cast<py::tuple>();
+            auto const& t = info->getInfo(ot[1].cast<ParticleIndex>());
+            m.push_back(Expr::op_type{ot[0].cast<bool>(), t});
+        }
+        return Expr(1.0, m);
+    }
+};
+
+FreqVec3 to_freqs(py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> zs) {
+    if(zs.ndim() != 2 || zs.shape(1) != 3)
+        throw std::runtime_error("frequency triplets must have shape (N, 3)");
+    FreqVec3 f;
+    auto r = zs.unchecked<2>();
+    for(py::ssize_t w = 0; w < zs.shape(0); ++w)
+        f.emplace_back(r(w, 0), r(w, 1), r(w, 2));
+    return f;
+}
+
+std::vector<ComplexType> to_points(py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> z) {
+    return std::vector<ComplexType>(z.data(), z.data() + z.size());
+}
+
+} // namespace
+
+PYBIND11_MODULE(_pomerol, m) {
+    m.doc() = "MI355X full-ED path behind pomerol's C++ API";
+    m.def("eval_timing", [](int enable) {
+        double ms = 0;
+        int64_t n = 0;
+        detail::check(pomx_eval_timing(detail::gpu(), enable, &ms, &n), "timing");
+        return py::make_tuple(ms, n);
+    }, py::arg("enable") = -1);
+    m.def("device_ok", []() {
+        try {
+            detail::gpu();
+            return true;
+        } catch(std::exception const&) {
+            return false;
+        }
+    });
+
+    py::class_<ED>(m, "ED")
+        .def(py::init<py::list, double, bool>(), py::arg("terms"), py::arg("beta"), py::arg("keep_hmat") = false)
+        .def_readonly("t_eigh", &ED::t_eigh)
+        .def_readonly("t_rot", &ED::t_rot)
+        .def_property_readonly("E0", [](ED& e) { return e.H->getGroundEnergy(); })
+        .def_property_readonly("Z", [](ED& e) { return e.rho->getPartitionFunction(); })
+        .def("indices", [](ED& e) {
+            py::list l;
+            for(ParticleIndex i = 0; i < e.info->getIndexSize(); ++i) {
+                auto const& t = e.info->getInfo(i);
+                l.append(py::make_tuple(std::get<0>(t), std::get<1>(t), int(std::get<2>(t))));
+            }
+            return l;
+        })
+        .def("n_blocks", [](ED& e) { return e.S->getNumberOfBlocks(); })
+        .def("blocks", [](ED& e) {
+            std::vector<std::vector<QuantumState>> b;
+            for(BlockNumber k = 0; k < e.S->getNumberOfBlocks(); ++k)
+                b.push_back(e.S->getFockStates(k));
+            return b;
+        })
+        .def("hmat", [](ED& e, int b) { return to_np(e.hmats.at(std::size_t(b))); })
+        .def("evals", [](ED& e, int b) {
+            auto const& v = e.H->getEigenValues(b);
+            return to_np(std::vector<double>(v.begin(), v.end()));
+        })
+        .def("evecs", [](ED& e, int b) { return to_np(e.H->getPart(b).getMatrix<false>()); })
+        .def("weights", [](ED& e, int b) {
+            auto const& w = e.rho->getPart(b).getWeights();
+            return to_np(std::vector<double>(w.begin(), w.end()));
+        })
+        .def("truncate_blocks", [](ED& e, double tol) { e.rho->truncateBlocks(tol, false); })
+        .def("retained", [](ED& e) {
+            std::vector<bool> r;
+            for(BlockNumber b = 0; b < e.S->getNumberOfBlocks(); ++b)
+                r.push_back(e.rho->isRetained(b));
+            return r;
+        })
+        .def("rotate", [](ED& e, py::list op, double tol) {
+            MonomialOperator M(e.mono_expr(op), *e.HS, *e.S, *e.H);
+            M.prepare(*e.HS);
+            M.compute(tol);
+            py::list out;
+            for(auto const& p : M.getParts()) {
+                py::dict d;
+                d["from"] = p.getRightIndex();
+                d["to"] = p.getLeftIndex();
+                d["m"] = to_np(p.getRowMajorValue<false>());
+                out.append(d);
+            }
+            return out;
+        }, py::arg("op"), py::arg("tol") = 1e-8)
+        .def("field_operator", [](ED& e, int i, bool dag) {
+            auto& ops = e.field_ops();
+            MonomialOperator const& M = dag ? static_cast<MonomialOperator const&>(ops.getCreationOperator(ParticleIndex(i)))
+                                            : static_cast<MonomialOperator const&>(ops.getAnnihilationOperator(ParticleIndex(i)));
+            py::list out;
+            for(auto const& p : M.getParts()) {
+                py::dict d;
+                d["from"] = p.getRightIndex();
+                d["to"] = p.getLeftIndex();
+                d["m"] = to_np(p.getRowMajorValue<false>());
+                out.append(d);
+            }
+            return out;
+        })
+        .def("gf", [](ED& e, int i, int j, py::array_t<std::complex<double>> z, double pr, double ct) {
+            auto& ops = e.field_ops();
+            GreensFunction G(*e.S, *e.H, ops.getAnnihilationOperator(ParticleIndex(i)), ops.getCreationOperator(ParticleIndex(j)), *e.rho);
+            G.PoleResolution = pr;
+            G.CoefficientTolerance = ct;
+            G.prepare();
+            G.compute();
+            py::dict d;
+            d["values"] = to_np(G.evaluate(to_points(z)));
+            d["n_parts"] = G.getNumParts();
+            d["n_terms"] = G.getNumTerms();
+            std::vector<double> poles;
+            std::vector<ComplexType> res;
+            for(auto const& p : G.getParts())
+                for(auto const& t : p.getTerms()) {
+                    poles.push_back(t.Pole);
+                    res.push_back(t.Residue);
+                }
+            d["poles"] = to_np(poles);
+            d["residues"] = to_np(res);
+            return d;
+        }, py::arg("i"), py::arg("j"), py::arg("z"), py::arg("pole_res") = 1e-8, py::arg("coeff_tol") = 1e-8)
+        .def("gf_pointwise", [](ED& e, int i, int j, std::vector<long> n) {
+            auto& ops = e.field_ops();
+            GreensFunction G(*e.S, *e.H, ops.getAnnihilationOperator(ParticleIndex(i)), ops.getCreationOperator(ParticleIndex(j)), *e.rho);
+            G.prepare();
+            G.compute();
+            std::vector<ComplexType> v;
+            for(long k : n)
+                v.push_back(G(k));
+            return to_np(v);
+        })
+        .def("chi", [](ED& e, py::list A, py::list B, py::array_t<std::complex<double>> z, double pr, double ct, bool sub) {
+            MonomialOperator MA(e.mono_expr(A), *e.HS, *e.S, *e.H), MB(e.mono_expr(B), *e.HS, *e.S, *e.H);
+            MA.prepare(*e.HS);
+            MB.prepare(*e.HS);
+            MonomialOperator::computeBatch({&MA, &MB}, 1e-8);
+            Susceptibility X(*e.S, *e.H, MA, MB, *e.rho);
+            X.PoleResolution = pr;
+            X.CoefficientTolerance = ct;
+            X.prepare();
+            X.compute();
+            if(sub)
+                X.subtractDisconnected();
+            py::dict d;
+            d["values"] = to_np(X.evaluate(to_points(z)));
+            d["n_parts"] = X.getNumParts();
+            d["n_terms"] = X.getNumTerms();
+            return d;
+        }, py::arg("A"), py::arg("B"), py::arg("z"), py::arg("pole_res") = 1e-8, py::arg("coeff_tol") = 1e-8,
+           py::arg("subtract_disconnected") = false)
+        .def("ensemble_average", [](ED& e, py::list A) {
+            MonomialOperator MA(e.mono_expr(A), *e.HS, *e.S, *e.H);
+            MA.prepare(*e.HS);
+            MA.compute();
+            EnsembleAverage EA(MA, *e.rho);
+            EA.compute();
+            return EA();
+        })
+        .def("g2", [](ED& e, int i, int j, int k, int l, py::array_t<std::complex<double>> zs, double pr, double ct,
+                      py::object parts, bool keep) {
+            auto& ops = e.field_ops();
+            TwoParticleGF G2(*e.S, *e.H, ops.getAnnihilationOperator(ParticleIndex(i)), ops.getAnnihilationOperator(ParticleIndex(j)),
+                             ops.getCreationOperator(ParticleIndex(k)), ops.getCreationOperator(ParticleIndex(l)), *e.rho);
+            G2.PoleResolution = pr;
+            G2.CoefficientTolerance = ct;
+            G2.prepare();
+            FreqVec3 f = to_freqs(zs);
+            std::vector<std::size_t> which;
+            if(parts.is_none())
+                for(std::size_t q = 0; q < G2.getNumParts(); ++q)
+                    which.push_back(q);
+            else
+                which = parts.cast<std::vector<std::size_t>>();
+            auto t0 = std::chrono::steady_clock::now();
+            std::vector<ComplexType> vals = G2.computeParts(which, !keep, f);
+            double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            py::dict d;
+            d["values"] = to_np(vals);
+            d["seconds"] = secs;
+            d["n_parts_total"] = G2.getNumParts();
+            int64_t nnr = 0, nr = 0, raw = 0;
+            std::vector<int64_t> pnr, pr_, pb;
+            for(std::size_t q : which) {
+                auto const& p = G2.getParts()[q];
+                nnr += int64_t(p.getNumNonResonantTerms());
+                nr += int64_t(p.getNumResonantTerms());
+                raw += p.getNumRawTerms();
+                pnr.push_back(int64_t(p.getNumNonResonantTerms()));
+                pr_.push_back(int64_t(p.getNumResonantTerms()));
+                auto b = p.getBlocks();
+                int perm = 0;
+                for(int pp = 0; pp < 6; ++pp)
+                    if(permutations3[pp] == p.getPermutation())
+                        perm = pp;
+                for(int x : {b[0], b[1], b[2], b[3], perm})
+                    pb.push_back(x);
+            }
+            d["n_parts"] = which.size();
+            d["n_nonres"] = nnr;
+            d["n_res"] = nr;
+            d["n_raw"] = raw;
+            d["part_nonres"] = pnr;
+            d["part_res"] = pr_;
+            d["part_blocks"] = pb;
+            if(keep) {
+                // lazy path: TwoParticleGF::operator() on the kept terms, point by point
+                std::vector<ComplexType> lazy;
+                for(auto const& t : f)
+                    lazy.push_back(G2(std::get<0>(t), std::get<1>(t), std::get<2>(t)));
+                d["values_lazy"] = to_np(lazy);
+            }
+            return d;
+        }, py::arg("i"), py::arg("j"), py::arg("k"), py::arg("l"), py::arg("zs"), py::arg("pole_res") = 1e-8,
+           py::arg("coeff_tol") = 1e-16, py::arg("parts") = py::none(), py::arg("keep") = false)
+        .def("g2_container", [](ED& e, std::vector<std::tuple<int, int, int, int>> idx, std::vector<std::tuple<long, long, long>> n,
+                                double pr, double ct, std::vector<std::tuple<int, int, int, int>> query) {
+            auto& ops = e.field_ops();
+            TwoParticleGFContainer C(*e.info, *e.S, *e.H, *e.rho, ops);
+            C.PoleResolution = pr;
+            C.CoefficientTolerance = ct;
+            std::set<IndexCombination4> s;
+            for(auto const& t : idx)
+                s.emplace(ParticleIndex(std::get<0>(t)), ParticleIndex(std::get<1>(t)), ParticleIndex(std::get<2>(t)),
+                          ParticleIndex(std::get<3>(t)));
+            C.prepareAll(s);
+            C.computeAll(false);
+            py::dict out;
+            for(auto const& q : query) {
+                auto& el = C(IndexCombination4(ParticleIndex(std::get<0>(q)), ParticleIndex(std::get<1>(q)),
+                                               ParticleIndex(std::get<2>(q)), ParticleIndex(std::get<3>(q))));
+                std::vector<ComplexType> v;
+                for(auto const& t : n)
+                    v.push_back(el(std::get<0>(t), std::get<1>(t), std::get<2>(t)));
+                out[py::cast(q)] = to_np(v);
+            }
+            return out;
+        });
+}
