@@ -1,0 +1,167 @@
+"""Benchmark: two-particle GF Lehmann terms x frequencies per second on one MI355X.
+
+Workload (config 2 of the benchmark set): single-orbital Anderson impurity with 5 seeded bath
+sites (12 spin-orbitals, 4096 states, largest N/Sz block 400), U = 3, beta = 20, Boltzmann
+cutoff 1e-12 (DensityMatrix::truncateBlocks(1e-12) and CoefficientTolerance = 1e-12,
+PoleResolution 1e-8), 2PGF <c_up c_dn c^+_up c^+_dn> on the 32 x 32 x 8 Matsubara grid
+(8192 frequency triplets), all FP64.
+
+One step = one pass of the TwoParticleGF hot path (term generation, pole merge, Lehmann sum on
+all 8192 triplets, TwoParticleGFPart::compute + frequency fill) over a fixed batch of parts of
+that component.  The full component holds ~1e11 merged terms (hours of GPU time), so a step is a
+deterministic sample of its parts (see --batch); the metric counts exactly the merged terms the
+timed region processes, times 8192.
+
+Setup (eigensolve, rotation, part enumeration) is outside the timed region; inputs are resident
+in HBM.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOP_NONRES = 28  # complex division 11, products 12, pole subtractions 3, accumulate 2
+FLOP_RES = 36
+PEAK_FP64_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def select_parts(parts, budget_quads: float):
+    """Deterministic batch: parts in prepare() order, every part whose quadruple volume
+    n1*n2*n3*n4 is at most budget_quads, until the sum reaches 2*budget_quads."""
+    chosen, acc = [], 0.0
+    for k, p in enumerate(parts):
+        vol = float(p[5]) * p[6] * p[7] * p[8]
+        if vol <= budget_quads:
+            chosen.append(k)
+            acc += vol
+            if acc >= 2 * budget_quads:
+                break
+    return chosen
+
+
+def cpu_baseline(e, H, beta, zs, ijkl, parts, sample_parts):
+    """The oracle (pomerol's algorithm restated on the CPU, OpenMP over frequencies) on a
+    bounded sample of parts, fed the same spectrum as the GPU."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pomerol_oracle
+
+    o = pomerol_oracle.ED(len(H.indices()), H.mode_terms(), beta)
+    for b in range(e.n_blocks()):
+        o.set_spectrum(b, e.evals(b), e.evecs(b).ravel(), e.weights(b))
+    o.truncate_blocks(1e-12)
+    t0 = time.perf_counter()
+    r = o.g2(*ijkl, zs.ravel(), 1e-8, 1e-12, -1, sample_parts)
+    secs = time.perf_counter() - t0
+    terms = r["n_nonres"] + r["n_res"]
+    return terms * len(zs) / secs, secs, terms
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=float, default=2.0e8, help="quadruple-volume budget of the part sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--survey", action="store_true", help="print per-part term counts (no frequencies)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path's multi-GPU "
+              "sharding is described in DESIGN.md and not built", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import pomerol_amd
+    from pomerol_amd import workloads as W
+
+    c = W.cfg2()
+    H = c.expr()
+    idx = H.indices()
+    u, d = idx.index(("C", 0, 1)), idx.index(("C", 0, 0))
+    ijkl = (u, d, u, d)
+    t0 = time.perf_counter()
+    e = pomerol_amd.ED(W.terms(H), c.beta)
+    e.truncate_blocks(1e-12)
+    zs = W.g2_grid(32, 8, c.beta)
+    parts = e.g2_parts(*ijkl)
+    log(f"[bench] setup {time.perf_counter() - t0:.2f}s: {e.n_blocks()} blocks, eigh {e.t_eigh:.3f}s, "
+        f"rot {e.t_rot:.3f}s, {len(parts)} parts, {len(zs)} frequency triplets")
+
+    if args.survey:
+        for k, p in enumerate(parts):
+            t = time.perf_counter()
+            r = e.g2(*ijkl, zs[:0], 1e-8, 1e-12, [k], False)
+            log(f"part {k} blocks {p[:4]} perm {p[4]} dims {p[5:]} raw {r['n_raw']} merged "
+                f"{r['n_nonres'] + r['n_res']} ({time.perf_counter() - t:.2f}s)")
+        return
+
+    batch = select_parts(parts, args.batch)
+    log(f"[bench] batch: {len(batch)} parts {batch}")
+
+    def step():
+        return e.g2(*ijkl, zs, 1e-8, 1e-12, batch, False)
+
+    for _ in range(args.warmup):
+        step()
+    pomerol_amd.eval_timing(1)
+    t0 = time.perf_counter()
+    terms_nr = terms_r = 0
+    for _ in range(args.steps):
+        r = step()
+        terms_nr += r["n_nonres"]
+        terms_r += r["n_res"]
+    wall = time.perf_counter() - t0
+    eval_ms, launches = pomerol_amd.eval_timing(0)
+    nfreq = len(zs)
+    term_freqs = (terms_nr + terms_r) * nfreq
+    value = term_freqs / wall
+    flops = (FLOP_NONRES * terms_nr + FLOP_RES * terms_r) * nfreq
+    achieved = flops / (eval_ms * 1e-3) / 1e12 if eval_ms > 0 else 0.0
+
+    cpu = None
+    if not args.no_cpu:
+        small = [k for k in batch if float(parts[k][5]) * parts[k][6] * parts[k][7] * parts[k][8] <= 2e6][:6]
+        v, secs, terms = cpu_baseline(e, H, c.beta, zs, ijkl, parts, small)
+        cpu = {"value": v, "unit": "terms*freqs/s", "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
+               "kind": "port",
+               "sample": f"oracle TwoParticleGFPart compute + fill of {len(small)} config-2 parts ({terms} merged "
+                         f"terms x {nfreq} triplets, {secs:.1f}s); generation serial, fill OpenMP"}
+
+    out = {
+        "metric": "2PGF terms*freqs/s",
+        "value": value,
+        "unit": "terms*freqs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Anderson bath parameters)",
+        "config": {"workload": "config 2: Anderson 1+5 bath, 4096 states, U=3, beta=20, 2PGF up-dn-up-dn on "
+                               "32x32x8 Matsubara triplets, cutoff 1e-12",
+                   "parts_per_step": len(batch), "merged_terms_per_step": (terms_nr + terms_r) // args.steps,
+                   "freq_triplets": nfreq},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "g2_eval_kernel (FP64 Lehmann sum)", "eval_ms": eval_ms, "eval_launches": launches},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

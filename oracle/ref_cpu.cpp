@@ -248,6 +248,16 @@ ED::ED(int n_modes_, std::vector<HTerm> H_, double beta_) : n_modes(n_modes_), b
     cache_ok_.assign(n_modes, 0);
 }
 
+void ED::set_spectrum(int b, std::vector<double> const& ev, std::vector<double> const& vec, std::vector<double> const& w) {
+    std::size_t n = S.blocks.at(b).size();
+    if(ev.size() != n || vec.size() != n * n || w.size() != n)
+        throw std::runtime_error("set_spectrum: size mismatch");
+    evals[b] = ev;
+    evecs[b] = vec;
+    weights[b] = w;
+    std::fill(cache_ok_.begin(), cache_ok_.end(), 0);
+}
+
 void ED::truncate_blocks(double tol) {
     for(std::size_t b = 0; b < weights.size(); ++b) {
         retained[b] = 0;

@@ -96,6 +96,9 @@ public:
     double E0 = 0, Z = 0;
 
     void truncate_blocks(double tol); // DensityMatrix.cpp:77-92
+    // Replace block b's spectrum (for parity checks of the term pipeline on identical input).
+    void set_spectrum(int b, std::vector<double> const& evals, std::vector<double> const& evecs,
+                      std::vector<double> const& weights);
 
     RotOp rotate(Mono const& op, double tol = 1e-8) const;  // MonomialOperatorPart.cpp:48-92
     static RotOp adjoint(RotOp const& cdag);                  // MonomialOperatorPart.cpp:94-111

@@ -59,6 +59,14 @@ PYBIND11_MODULE(pomerol_oracle, m) {
         .def("weights", [](ED const& e, int b) { return arr(e.weights[b]); })
         .def("retained", [](ED const& e) { return std::vector<bool>(e.retained.begin(), e.retained.end()); })
         .def("truncate_blocks", &ED::truncate_blocks)
+        .def("set_spectrum",
+             [](ED& e, int b, py::array_t<double, py::array::c_style | py::array::forcecast> ev,
+                py::array_t<double, py::array::c_style | py::array::forcecast> vec,
+                py::array_t<double, py::array::c_style | py::array::forcecast> w) {
+                 e.set_spectrum(b, std::vector<double>(ev.data(), ev.data() + ev.size()),
+                                std::vector<double>(vec.data(), vec.data() + vec.size()),
+                                std::vector<double>(w.data(), w.data() + w.size()));
+             })
         .def("rotate",
              [](ED const& e, py::list op, double tol) {
                  RotOp R = e.rotate(to_mono(op), tol);

@@ -318,6 +318,24 @@ PYBIND11_MODULE(_pomerol, m) {
             return d;
         }, py::arg("i"), py::arg("j"), py::arg("k"), py::arg("l"), py::arg("zs"), py::arg("pole_res") = 1e-8,
            py::arg("coeff_tol") = 1e-16, py::arg("parts") = py::none(), py::arg("keep") = false)
+        .def("g2_parts", [](ED& e, int i, int j, int k, int l) {
+            // part list of TwoParticleGF::prepare: (S1, S2, S3, S4, permutation, n1, n2, n3, n4)
+            auto& ops = e.field_ops();
+            TwoParticleGF G2(*e.S, *e.H, ops.getAnnihilationOperator(ParticleIndex(i)), ops.getAnnihilationOperator(ParticleIndex(j)),
+                             ops.getCreationOperator(ParticleIndex(k)), ops.getCreationOperator(ParticleIndex(l)), *e.rho);
+            G2.prepare();
+            std::vector<std::vector<int64_t>> out;
+            for(auto const& p : G2.getParts()) {
+                auto b = p.getBlocks();
+                int perm = 0;
+                for(int pp = 0; pp < 6; ++pp)
+                    if(permutations3[pp] == p.getPermutation())
+                        perm = pp;
+                out.push_back({b[0], b[1], b[2], b[3], perm, int64_t(e.S->getBlockSize(b[0])), int64_t(e.S->getBlockSize(b[1])),
+                               int64_t(e.S->getBlockSize(b[2])), int64_t(e.S->getBlockSize(b[3]))});
+            }
+            return out;
+        })
         .def("g2_container", [](ED& e, std::vector<std::tuple<int, int, int, int>> idx, std::vector<std::tuple<long, long, long>> n,
                                 double pr, double ct, std::vector<std::tuple<int, int, int, int>> query) {
             auto& ops = e.field_ops();

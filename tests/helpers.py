@@ -1,0 +1,86 @@
+"""Model builders shared by the tests (the reference tests' models, pomerol_amd.presets)."""
+import json
+import os
+
+import numpy as np
+
+from pomerol_amd.presets import CoulombS, Hopping, Level, Magnetization, anderson
+
+GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def goldens():
+    with open(os.path.join(GOLDEN_DIR, "reference_goldens.json")) as f:
+        return json.load(f)
+
+
+def cplx(pairs):
+    return np.array([complex(a, b) for a, b in pairs])
+
+
+def fermionic(n, beta):
+    return 1j * np.pi * (2 * np.asarray(n) + 1) / beta
+
+
+def bosonic(m, beta):
+    return 1j * np.pi * 2 * np.asarray(m) / beta
+
+
+def anderson_test_model():
+    """test/AndersonTest.cpp:32-74"""
+    U, h, V, eps, beta = 3.7, 0.1, 1.0, 2.3, 20.0
+    mu = 0.6 * U
+    H = (CoulombS("C", U, -mu) + Magnetization("C", h) + Level("0", -eps) + Level("1", eps)
+         + Hopping("C", "0", V) + Hopping("C", "1", V))
+    return H, beta
+
+
+def dimer_model():
+    """test/HamiltonianTest.cpp:40-42"""
+    return CoulombS("A", 1.0, -0.5) + CoulombS("B", 2.0, -1.0) + Hopping("A", "B", -1.0)
+
+
+def gf2site_model():
+    """test/GF2siteTest.cpp:35-57"""
+    return CoulombS("A", 1.0, -0.5) + CoulombS("B", 1.0, -0.5) + Hopping("A", "B", -1.0), 10.0
+
+
+def gf4site_model():
+    """test/GF4siteTest.cpp:32-59"""
+    H = (CoulombS("A", 1.0, -0.5) + CoulombS("B", 2.0, -1.1) + CoulombS("C", 3.0, -0.7) + CoulombS("D", 4.0, -1.1)
+         + Hopping("A", "B", -1.3) + Hopping("B", "C", -0.45) + Hopping("C", "D", -0.127) + Hopping("A", "D", -0.255))
+    return H, 10.0
+
+
+def anderson2pgf_model():
+    """test/Anderson2PGFTest.cpp:37-80"""
+    g = goldens()["Anderson2PGFTest"]["params"]
+    return anderson(g["levels"], g["hoppings"], g["U"], g["mu"]), g["beta"]
+
+
+def anderson2pgf_freqs(beta):
+    """(omega + Omega, w_p, omega) for p = 0..9 (Anderson2PGFTest.cpp:112-116)."""
+    Om, om = 2j * np.pi / beta, 1j * np.pi / beta
+    return np.array([[om + Om, 1j * (2 * i + 1) * np.pi / beta, om] for i in range(10)])
+
+
+def hubbard_atom(U, mu, h=0.0):
+    H = CoulombS("A", U, -mu)
+    if h:
+        H += Magnetization("A", h)
+    return H
+
+
+def mode_terms(H):
+    return H.mode_terms()
+
+
+def grid_triplets(nf, nb, beta):
+    """(nu + w3, w2, w3) triplets (quantum_model.cpp:220-231 layout)."""
+    from pomerol_amd.workloads import g2_grid
+    return g2_grid(nf, nb, beta)
+
+
+def rel_err(x, ref):
+    x, ref = np.asarray(x), np.asarray(ref)
+    return float(np.max(np.abs(x - ref)) / max(np.max(np.abs(ref)), 1e-300))

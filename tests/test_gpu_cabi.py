@@ -1,0 +1,150 @@
+"""Direct calls through the C ABI (include/pomx_c.h) with ctypes, checked against numpy."""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+c_int64_p = ctypes.POINTER(ctypes.c_int64)
+
+
+class Ctx:
+    def __init__(self):
+        import pomerol_amd
+
+        self.lib = ctypes.CDLL(pomerol_amd.POMX_LIB)
+        self.lib.pomx_last_error.restype = ctypes.c_char_p
+        self.ctx = ctypes.c_void_p()
+        assert self.lib.pomx_ctx_create(0, ctypes.byref(self.ctx)) == 0
+        self.bufs = []
+
+    def ok(self, rc):
+        assert rc == 0, self.lib.pomx_last_error(self.ctx).decode()
+
+    def put(self, a):
+        a = np.ascontiguousarray(a)
+        p = ctypes.c_void_p()
+        self.ok(self.lib.pomx_malloc(self.ctx, ctypes.c_size_t(max(a.nbytes, 8)), ctypes.byref(p)))
+        self.ok(self.lib.pomx_h2d(self.ctx, p, a.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(a.nbytes)))
+        self.bufs.append(p)
+        return p
+
+    def get(self, p, like):
+        out = np.empty_like(like)
+        self.ok(self.lib.pomx_d2h(self.ctx, out.ctypes.data_as(ctypes.c_void_p), p, ctypes.c_size_t(out.nbytes)))
+        return out
+
+    def close(self):
+        for p in self.bufs:
+            self.lib.pomx_free(self.ctx, p)
+        self.lib.pomx_ctx_destroy(self.ctx)
+
+
+@pytest.fixture
+def cabi():
+    c = Ctx()
+    yield c
+    c.close()
+
+
+def sym_with_spectrum(rng, n, degenerate):
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    if degenerate:
+        lam = np.repeat(rng.uniform(-3, 3, (n + 2) // 3), 3)[:n]
+    else:
+        lam = rng.uniform(-3, 3, n)
+    A = (Q * lam) @ Q.T
+    return (A + A.T) / 2
+
+
+@pytest.mark.parametrize("degenerate", [False, True])
+def test_eigh_batched(cabi, degenerate):
+    rng = np.random.default_rng(7)
+    sizes = [1, 2, 3, 5, 17, 33, 64, 65, 130, 257, 400]
+    mats = [sym_with_spectrum(rng, n, degenerate) for n in sizes]
+    print("[cabi] eigh sizes", sizes, "degenerate", degenerate, flush=True)
+    mo = np.cumsum([0] + [n * n for n in sizes])[:-1].astype(np.int64)
+    vo = np.cumsum([0] + sizes)[:-1].astype(np.int64)
+    flat = np.concatenate([m.ravel() for m in mats])
+    d_m = cabi.put(flat)
+    d_e = cabi.put(np.zeros(sum(sizes)))
+    n_arr = np.array(sizes, dtype=np.int32)
+    cabi.ok(cabi.lib.pomx_eigh_batched(cabi.ctx, len(sizes), n_arr.ctypes.data_as(ctypes.c_void_p),
+                                       mo.ctypes.data_as(ctypes.c_void_p), vo.ctypes.data_as(ctypes.c_void_p), d_m, d_e))
+    V_all = cabi.get(d_m, flat)
+    E_all = cabi.get(d_e, np.zeros(sum(sizes)))
+    for k, n in enumerate(sizes):
+        A = mats[k]
+        V = V_all[mo[k]:mo[k] + n * n].reshape(n, n)
+        E = E_all[vo[k]:vo[k] + n]
+        assert np.abs(E - np.linalg.eigvalsh(A)).max() < 1e-12 * max(1, np.abs(E).max())
+        assert np.abs(A @ V - V * E).max() < 1e-12 * max(1, np.abs(E).max())
+        assert np.abs(V.T @ V - np.eye(n)).max() < 1e-12
+
+
+def test_boltzmann(cabi):
+    rng = np.random.default_rng(3)
+    E = rng.uniform(-5, 5, 5000)
+    d_E, d_w = cabi.put(E), cabi.put(np.zeros_like(E))
+    E0, Z = ctypes.c_double(), ctypes.c_double()
+    cabi.ok(cabi.lib.pomx_boltzmann(cabi.ctx, ctypes.c_int64(len(E)), d_E, ctypes.c_double(7.0), d_w,
+                                    ctypes.byref(E0), ctypes.byref(Z)))
+    w = cabi.get(d_w, E)
+    ref = np.exp(-7.0 * (E - E.min()))
+    assert E0.value == E.min()
+    assert abs(Z.value - ref.sum()) < 1e-12 * ref.sum()
+    assert np.abs(w - ref / ref.sum()).max() < 1e-15
+
+
+class RotJob(ctypes.Structure):
+    _fields_ = [("n_to", ctypes.c_int), ("n_from", ctypes.c_int), ("u_to_off", ctypes.c_int64),
+                ("u_from_off", ctypes.c_int64), ("src_off", ctypes.c_int64), ("out_off", ctypes.c_int64)]
+
+
+def test_rotate_batched(cabi):
+    rng = np.random.default_rng(11)
+    shapes = [(1, 1), (3, 5), (16, 16), (70, 33), (129, 200), (400, 375)]
+    U, src, jobs, refs = [], [], [], []
+    uoff = soff = ooff = 0
+    for nt, nf in shapes:
+        Ut, Uf = rng.standard_normal((nt, nt)), rng.standard_normal((nf, nf))
+        s = np.zeros(nt, dtype=np.int32)
+        pre = rng.choice(nf, size=min(nt, nf), replace=False)
+        tgt = rng.choice(nt, size=len(pre), replace=False)
+        for a, t in zip(pre, tgt):
+            s[t] = (a + 1) * (1 if rng.random() < 0.5 else -1)
+        X = np.zeros((nt, nf))
+        for t in range(nt):
+            if s[t]:
+                X[t] = np.sign(s[t]) * Uf[abs(s[t]) - 1]
+        refs.append((ooff, Ut.T @ X))
+        jobs.append(RotJob(nt, nf, uoff, uoff + nt * nt, soff, ooff))
+        U += [Ut.ravel(), Uf.ravel()]
+        src.append(s)
+        uoff += nt * nt + nf * nf
+        soff += nt
+        ooff += nt * nf
+    d_U, d_s, d_o = cabi.put(np.concatenate(U)), cabi.put(np.concatenate(src)), cabi.put(np.zeros(ooff))
+    arr = (RotJob * len(jobs))(*jobs)
+    cabi.ok(cabi.lib.pomx_rotate_batched(cabi.ctx, len(jobs), arr, d_U, d_s, d_o, ctypes.c_double(1e-20)))
+    out = cabi.get(d_o, np.zeros(ooff))
+    for (off, ref), (nt, nf) in zip(refs, shapes):
+        got = out[off:off + nt * nf].reshape(nt, nf)
+        assert np.abs(got - ref).max() < 1e-12 * max(1, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_pair_eval(cabi, kind):
+    rng = np.random.default_rng(5)
+    n, nz = 100000, 300
+    R, P = rng.standard_normal(n), rng.uniform(-4, 4, n)
+    z = 1j * np.pi * (2 * np.arange(nz) + 1) / 13.0
+    zi = np.stack([z.real, z.imag], 1).ravel()
+    d_R, d_P, d_z = cabi.put(R), cabi.put(P), cabi.put(zi)
+    d_acc = cabi.put(np.zeros(2 * nz))
+    cabi.ok(cabi.lib.pomx_pair_eval(cabi.ctx, kind, ctypes.c_int64(n), d_R, d_P, nz, d_z, d_acc))
+    acc = cabi.get(d_acc, np.zeros(2 * nz))
+    got = acc[0::2] + 1j * acc[1::2]
+    ref = (R[None, :] / (z[:, None] - P[None, :])).sum(1) * (1 if kind == 0 else -1)
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()

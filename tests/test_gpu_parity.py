@@ -1,0 +1,195 @@
+"""GPU path (C++ API -> C ABI -> gfx950 kernels) against the CPU oracle on the same inputs."""
+import numpy as np
+import pytest
+from helpers import (anderson2pgf_freqs, anderson2pgf_model, anderson_test_model, bosonic, cplx, dimer_model, fermionic,
+                     gf2site_model, gf4site_model, goldens, hubbard_atom, rel_err)
+
+from pomerol_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def build(pa, oracle, H, beta, keep_hmat=True):
+    print(f"[gpu-parity] model with {len(H.indices())} modes, beta={beta}", flush=True)
+    e = pa.ED(W.terms(H), beta, keep_hmat)
+    o = oracle.ED(len(H.indices()), H.mode_terms(), beta)
+    return e, o
+
+
+def inject(e, o):
+    """Oracle copy fed the GPU's spectrum: isolates the term pipelines from eigensolver roundoff."""
+    for b in range(e.n_blocks()):
+        o.set_spectrum(b, e.evals(b), e.evecs(b).ravel(), e.weights(b))
+    return o
+
+
+MODELS = {
+    "dimer": lambda: (dimer_model(), 1.0),
+    "anderson_test": anderson_test_model,
+    "gf4site": gf4site_model,
+    "anderson2pgf": anderson2pgf_model,
+    "cfg1": lambda: (W.cfg1().expr(), W.cfg1().beta),
+}
+
+
+@pytest.mark.parametrize("name", list(MODELS))
+def test_sectors_and_eigensystem(pa, oracle, name):
+    H, beta = MODELS[name]()
+    e, o = build(pa, oracle, H, beta)
+    assert e.blocks() == o.blocks()  # bit-exact sector partition and state order
+    assert abs(e.E0 - o.E0) < 1e-12
+    for b in range(e.n_blocks()):
+        hm, V, w = e.hmat(b), e.evecs(b), e.evals(b)
+        # same sparsity; values are sums of the same inputs (order of the diagonal terms may differ)
+        assert np.array_equal(hm != 0, o.hmat(b) != 0)
+        assert np.abs(hm - o.hmat(b)).max() < 1e-14 * max(1.0, np.abs(hm).max())
+        assert np.all(np.diff(w) >= 0)
+        scale = max(1.0, np.abs(w).max())
+        assert np.abs(w - o.evals(b)).max() < 1e-12 * scale
+        assert np.abs(hm @ V - V * w).max() < 1e-12 * scale
+        assert np.abs(V.T @ V - np.eye(len(w))).max() < 1e-12
+        assert rel_err(e.weights(b), o.weights(b)) < 1e-10 if o.weights(b).max() > 1e-300 else True
+
+
+@pytest.mark.parametrize("name", ["anderson_test", "gf4site", "cfg1"])
+def test_rotation_matches_reference_formula(pa, oracle, name):
+    """O = U_L^T (M U_R) computed on the GPU equals the same product in numpy (GPU eigenvectors),
+    and the connected block pairs equal the oracle's find_connections."""
+    H, beta = MODELS[name]()
+    e, o = build(pa, oracle, H, beta)
+    blocks = e.blocks()
+    n = len(H.indices())
+    for i in range(n):
+        gparts = e.field_operator(i, True)
+        assert [(p["from"], p["to"]) for p in gparts] == [(p["from"], p["to"]) for p in o.rotate([(True, i)])]
+        for p in gparts:
+            fb, tb = p["from"], p["to"]
+            Uf, Ut = e.evecs(fb), e.evecs(tb)
+            X = np.zeros((len(blocks[tb]), len(blocks[fb])))
+            pos = {s: k for k, s in enumerate(blocks[tb])}
+            for a, s in enumerate(blocks[fb]):
+                if (s >> i) & 1:
+                    continue
+                sign = -1.0 if bin(s & ((1 << i) - 1)).count("1") % 2 else 1.0
+                X[pos[s | (1 << i)], :] += sign * Uf[a, :]
+            ref = Ut.T @ X
+            ref[np.abs(ref) <= 1e-20] = 0.0
+            assert np.abs(p["m"] - ref).max() < 1e-13
+        cparts = e.field_operator(i, False)  # adjoint view
+        cmap = {(p["from"], p["to"]): p["m"] for p in cparts}
+        for p in gparts:
+            assert np.array_equal(cmap[(p["to"], p["from"])], p["m"].T)
+
+
+@pytest.mark.parametrize("name", list(MODELS))
+def test_greens_function(pa, oracle, name):
+    H, beta = MODELS[name]()
+    e, o = build(pa, oracle, H, beta)
+    z = fermionic(np.arange(-8, 64), beta)
+    oi = inject(e, oracle.ED(len(H.indices()), H.mode_terms(), beta))
+    for i in range(min(4, len(H.indices()))):
+        g = e.gf(i, i, z, 1e-8, 1e-8)
+        ref = o.gf(i, i, z, 1e-8, 1e-8)
+        assert rel_err(g["values"], ref["values"]) < 1e-10
+        same = oi.gf(i, i, z, 1e-8, 1e-8)
+        assert g["n_parts"] == same["n_parts"] and g["n_terms"] == same["n_terms"]
+        assert rel_err(g["values"], same["values"]) < 1e-12
+
+
+def test_goldens_on_gpu(pa, oracle):
+    g = goldens()
+    H, beta = anderson_test_model()
+    e = pa.ED(W.terms(H), beta)
+    idx = H.indices()
+    z = fermionic(np.arange(10), beta)
+    for key, spin in (("G_up", 1), ("G_down", 0)):
+        i = idx.index(("C", 0, spin))
+        assert np.abs(e.gf(i, i, z)["values"] - cplx(g["AndersonTest"][key])).max() < 1e-8
+        # the per-frequency operator()(long n) path of GreensFunction
+        assert np.abs(e.gf_pointwise(i, i, list(range(10))) - cplx(g["AndersonTest"][key])).max() < 1e-8
+    H, beta = anderson2pgf_model()
+    e = pa.ED(W.terms(H), beta)
+    idx = H.indices()
+    p = g["Anderson2PGFTest"]["params"]
+    for spin in (1, 0):
+        i = idx.index(("C", 0, spin))
+        r = e.g2(i, i, i, i, anderson2pgf_freqs(beta), p["pole_resolution"], p["coefficient_tolerance"], None, True)
+        ref = np.array(g["Anderson2PGFTest"]["chi_uuuu"])
+        assert np.abs(r["values"] - ref).max() < 1e-6
+        assert np.abs(r["values_lazy"] - ref).max() < 1e-6  # TwoParticleGF::operator() on kept terms
+    e = pa.ED(W.terms(dimer_model()), 1.0)
+    assert abs(e.E0 - g["HamiltonianTest"]["E0"]) < 1e-7
+
+
+def test_hubbard_atom_closed_forms_on_gpu(pa):
+    U, mu, beta = 1.0, 0.4, 10.0
+    e = pa.ED(W.terms(hubbard_atom(U, mu)), beta)
+    n = np.arange(-100, 100)
+    z = fermionic(n, beta)
+    w0, w1, w2 = 1.0, np.exp(beta * mu), np.exp(-beta * (-2 * mu + U))
+    Z = w0 + 2 * w1 + w2
+    ref = (w0 + w1) / Z / (z + mu) + (w1 + w2) / Z / (z + mu - U)
+    assert np.abs(e.gf(0, 0, z)["values"] - ref).max() < 1e-14
+    assert np.abs(e.gf(0, 1, z)["values"]).max() < 1e-14
+
+
+def test_susceptibility_on_gpu(pa, oracle):
+    U, mu, h, beta = 1.0, 0.4, 0.01, 10.0
+    H = hubbard_atom(U, mu, -h)
+    e, o = build(pa, oracle, H, beta)
+    z = bosonic(np.arange(20), beta)
+    up, dn = 1, 0
+    ops = {"s+": [(True, up), (False, dn)], "s-": [(True, dn), (False, up)], "nu": [(True, up), (False, up)],
+           "nd": [(True, dn), (False, dn)]}
+    for a, b in [("s+", "s-"), ("nu", "nu"), ("nu", "nd")]:
+        v = e.chi(ops[a], ops[b], z, 1e-8, 1e-8, True)
+        r = o.chi(ops[a], ops[b], z, 1e-8, 1e-8, True)
+        assert np.abs(v["values"] - r["values"]).max() < 1e-14 and v["n_terms"] == r["n_terms"]
+    assert abs(e.ensemble_average(ops["nu"]) - o.ensemble_average(ops["nu"])) < 1e-14
+    # a bath model: spin and charge susceptibilities
+    H, beta = anderson_test_model()
+    e, o = build(pa, oracle, H, beta)
+    idx = H.indices()
+    u, d = idx.index(("C", 0, 1)), idx.index(("C", 0, 0))
+    for A, B in [([(True, u), (False, d)], [(True, d), (False, u)]), ([(True, u), (False, u)], [(True, d), (False, d)])]:
+        v = e.chi(A, B, z, 1e-8, 1e-8, True)
+        r = o.chi(A, B, z, 1e-8, 1e-8, True)
+        assert rel_err(v["values"], r["values"]) < 1e-10
+
+
+@pytest.mark.parametrize("name", ["anderson2pgf", "cfg1"])
+@pytest.mark.parametrize("comp", ["uuuu", "udud", "duud"])
+def test_two_particle_gf(pa, oracle, name, comp):
+    H, beta = MODELS[name]()
+    e, o = build(pa, oracle, H, beta)
+    idx = H.indices()
+    u, d = idx.index(("C", 0, 1)), idx.index(("C", 0, 0))
+    ijkl = {"uuuu": (u, u, u, u), "udud": (u, d, u, d), "duud": (d, u, u, d)}[comp]
+    zs = W.g2_grid(8, 2, beta) if name == "cfg1" else W.g2_grid(6, 3, beta)
+    r = e.g2(*ijkl, zs, 1e-8, 1e-16)
+    ref = o.g2(*ijkl, zs.ravel(), 1e-8, 1e-16)
+    assert r["n_parts"] == ref["n_parts"] and list(r["part_blocks"]) == list(ref["part_blocks"])
+    assert rel_err(r["values"], ref["values"]) < 1e-10
+    oi = inject(e, oracle.ED(len(idx), H.mode_terms(), beta))
+    same = oi.g2(*ijkl, zs.ravel(), 1e-8, 1e-16)
+    assert rel_err(r["values"], same["values"]) < 1e-12
+    assert list(r["part_nonres"]) == list(same["part_nonres"])  # non-resonant terms: exact
+    # resonant terms: the TermList erasure threshold tol/(size+1) depends on the insertion
+    # history; counts agree to a few terms in 1e6 (the differing terms have |coeff| < 1e-15)
+    assert abs(r["n_res"] - same["n_res"]) <= max(2, 2e-5 * same["n_res"])
+
+
+def test_2pgf_container_crossing_images(pa):
+    """IndexContainer4 images (IndexContainer4.hpp:190-237) equal the directly computed elements:
+    fermionic antisymmetry chi_{jikl}(w1,w2;w3) = -chi_{ijkl}(w2,w1;w3) etc."""
+    H, beta = anderson2pgf_model()
+    e = pa.ED(W.terms(H), beta)
+    idx = H.indices()
+    u, d = idx.index(("C", 0, 1)), idx.index(("C", 0, 0))
+    ns = [(a, b, c) for a in range(-2, 2) for b in range(-2, 2) for c in range(-2, 2)]
+    images = [(d, u, d, u), (u, d, d, u), (d, u, u, d), (u, d, u, d)]
+    out = e.g2_container([(d, u, d, u)], ns, 1e-8, 1e-16, images)
+    zs = np.array([[fermionic(a, beta), fermionic(b, beta), fermionic(c, beta)] for a, b, c in ns])
+    for ijkl in images:
+        direct = e.g2(*ijkl, zs, 1e-8, 1e-16)["values"]
+        assert np.abs(out[ijkl] - direct).max() < 1e-11 * max(1.0, np.abs(direct).max()), ijkl
