@@ -8,9 +8,10 @@ PoleResolution 1e-8), 2PGF <c_up c_dn c^+_up c^+_dn> on the 32 x 32 x 8 Matsubar
 
 One step = one pass of the TwoParticleGF hot path (term generation, pole merge, Lehmann sum on
 all 8192 triplets, TwoParticleGFPart::compute + frequency fill) over a fixed batch of parts of
-that component.  The full component holds ~1e11 merged terms (hours of GPU time), so a step is a
-deterministic sample of its parts (see --batch); the metric counts exactly the merged terms the
-timed region processes, times 8192.
+that component.  The full component holds 206 parts and 6.5e9 merged terms (5e13 terms x
+frequencies), so a step is a deterministic sample of its parts (every --stride-th part, which
+includes the 400-state blocks); the metric counts exactly the merged terms the timed region
+processes, times 8192.
 
 Setup (eigensolve, rotation, part enumeration) is outside the timed region; inputs are resident
 in HBM.  Prints one JSON line.
@@ -35,18 +36,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def select_parts(parts, budget_quads: float):
-    """Deterministic batch: parts in prepare() order, every part whose quadruple volume
-    n1*n2*n3*n4 is at most budget_quads, until the sum reaches 2*budget_quads."""
-    chosen, acc = [], 0.0
-    for k, p in enumerate(parts):
-        vol = float(p[5]) * p[6] * p[7] * p[8]
-        if vol <= budget_quads:
-            chosen.append(k)
-            acc += vol
-            if acc >= 2 * budget_quads:
-                break
-    return chosen
+def select_parts(parts, stride: int):
+    """Deterministic batch: every stride-th part in TwoParticleGF::prepare order (a mix of the
+    component's small and 400-state blocks)."""
+    return list(range(0, len(parts), stride))
+
+
+def cpu_sample(batch, counts, budget_terms: float):
+    """Parts of the batch for the CPU baseline: smallest non-empty parts first until the merged
+    term count reaches budget_terms (about 10-30 s of oracle work at 8192 frequencies)."""
+    order = sorted((c, k) for k, c in zip(batch, counts) if c > 0)
+    out, acc = [], 0
+    for c, k in order:
+        if acc + c > budget_terms and out:
+            break
+        out.append(k)
+        acc += c
+    return sorted(out)
 
 
 def cpu_baseline(e, H, beta, zs, ijkl, parts, sample_parts):
@@ -71,7 +77,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=float, default=2.0e8, help="quadruple-volume budget of the part sample")
+    ap.add_argument("--stride", type=int, default=8, help="every stride-th part of the component forms a step")
+    ap.add_argument("--cpu-terms", type=float, default=1.5e6, help="merged-term budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--survey", action="store_true", help="print per-part term counts (no frequencies)")
     args = ap.parse_args()
@@ -106,7 +113,7 @@ def main():
                 f"{r['n_nonres'] + r['n_res']} ({time.perf_counter() - t:.2f}s)")
         return
 
-    batch = select_parts(parts, args.batch)
+    batch = select_parts(parts, args.stride)
     log(f"[bench] batch: {len(batch)} parts {batch}")
 
     def step():
@@ -121,6 +128,7 @@ def main():
         r = step()
         terms_nr += r["n_nonres"]
         terms_r += r["n_res"]
+        log(f"[bench] step: {r['n_nonres'] + r['n_res']} merged terms, {r['seconds']:.3f}s")
     wall = time.perf_counter() - t0
     eval_ms, launches = pomerol_amd.eval_timing(0)
     nfreq = len(zs)
@@ -131,7 +139,9 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        small = [k for k in batch if float(parts[k][5]) * parts[k][6] * parts[k][7] * parts[k][8] <= 2e6][:6]
+        counts = [a + b for a, b in zip(r["part_nonres"], r["part_res"])]
+        small = cpu_sample(batch, counts, args.cpu_terms)
+        log(f"[bench] cpu sample parts {small}")
         v, secs, terms = cpu_baseline(e, H, c.beta, zs, ijkl, parts, small)
         cpu = {"value": v, "unit": "terms*freqs/s", "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
                "kind": "port",

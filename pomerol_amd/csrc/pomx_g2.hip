@@ -67,39 +67,34 @@ __global__ void rank_scatter_kernel(int64_t n, const int64_t* __restrict__ head,
 // ---- 3. term counting / emission ------------------------------------------------------------
 // The reference's per-quadruple logic (TwoParticleGFPart.cpp:241-297): weights summed in the
 // order w1+w2+w3+w4, matrix element ((O1*O2)*O3)*CX4*sign, four candidate terms with their
-// own coefficient thresholds.
-struct Quad {
-    double C, P1, P2, P3, wi, wj, wk, wl;
-};
+// own coefficient thresholds.  One wave per (i1,i2,i3) row with the lanes over i4, reading the
+// contiguous copies O3c[i3][i4] and C4t[i1][i4] (coalesced).
 
-template <class F>
-__device__ inline void for_each_term(G2Dev const& D, int i1, int i2, int i3, double ab, double s3, F&& f) {
-    for(int i4 = 0; i4 < D.n4; ++i4) {
-        double c = mat(D.O3, i3, i4);
-        double d = mat(D.CX4, i4, i1);
-        if(c == 0.0 || d == 0.0)
-            continue;
-        double wl = D.w4[i4];
-        if(!(s3 + wl > D.tol))
-            continue;
-        double C = ab * c * d * D.sign;
-        double wi = D.w1[i1], wj = D.w2[i2], wk = D.w3[i3];
-        double tol = D.tol;
-        // kind 0: non-resonant z2, 1: non-resonant z4, 2: resonant z2z3, 3: resonant z1z2
-        if(fabs(C * (wj + wk)) > tol)
-            f(0, i4);
-        if(fabs(C * (wi + wl)) > tol)
-            f(1, i4);
-        if(fabs(C * D.beta * wj) > tol || fabs(C * (wj - wl)) > tol)
-            f(2, i4);
-        if(fabs(C * D.beta * wi) > tol || fabs(C * (wk - wi)) > tol)
-            f(3, i4);
-    }
+// bit k set: term of kind k (0 non-resonant z2, 1 non-resonant z4, 2 resonant z2z3,
+// 3 resonant z1z2) is emitted for this quadruple
+__device__ inline unsigned kinds_of(G2Dev const& D, double ab, double s3, double wi, double wj, double wk, double c,
+                                    double d, double wl) {
+    if(c == 0.0 || d == 0.0)
+        return 0u;
+    if(!(s3 + wl > D.tol))
+        return 0u;
+    double const C = ab * c * d * D.sign;
+    double const tol = D.tol;
+    unsigned m = 0u;
+    if(fabs(C * (wj + wk)) > tol)
+        m |= 1u;
+    if(fabs(C * (wi + wl)) > tol)
+        m |= 2u;
+    if(fabs(C * D.beta * wj) > tol || fabs(C * (wj - wl)) > tol)
+        m |= 4u;
+    if(fabs(C * D.beta * wi) > tol || fabs(C * (wk - wi)) > tol)
+        m |= 8u;
+    return m;
 }
 
 struct RowCtx {
     int i1, i2, i3;
-    double ab, s3;
+    double ab, s3, wi, wj, wk;
     bool active;
 };
 
@@ -111,21 +106,62 @@ __device__ inline RowCtx row_ctx(G2Dev const& D, const uint64_t* __restrict__ pa
     r.i3 = i3;
     double a = mat(D.O1, r.i1, r.i2), b = mat(D.O2, r.i2, i3);
     r.ab = a * b;
-    r.s3 = D.w1[r.i1] + D.w2[r.i2] + D.w3[i3];
+    r.wi = D.w1[r.i1];
+    r.wj = D.w2[r.i2];
+    r.wk = D.w3[i3];
+    r.s3 = r.wi + r.wj + r.wk;
     // w4 is non-increasing (ascending energies), so w4[0] bounds every i4
     r.active = (a != 0.0 && b != 0.0) && (r.s3 + D.w4[0] > D.tol);
     return r;
 }
 
-__global__ void g2_count_kernel(G2Dev D, const uint64_t* __restrict__ pairs, int64_t nrows, int* __restrict__ cnt) {
-    int64_t row = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if(row >= nrows)
-        return;
-    RowCtx r = row_ctx(D, pairs, row / D.n3, int(row % D.n3));
-    int c = 0;
-    if(r.active)
-        for_each_term(D, r.i1, r.i2, r.i3, r.ab, r.s3, [&](int, int) { ++c; });
-    cnt[row] = c;
+__device__ inline int wave_sum_int(int v) {
+    for(int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ inline int wave_excl_scan_int(int v, int lane, int& total) {
+    int x = v;
+    for(int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if(lane >= o)
+            x += y;
+    }
+    total = __shfl(x, 63, 64);
+    return x - v;
+}
+
+// contiguous copies: O3c[i3][i4] = O3(i3, i4), C4t[i1][i4] = CX4(i4, i1)
+__global__ void g2_stage_kernel(G2Dev D, double* __restrict__ O3c, double* __restrict__ C4t) {
+    int64_t g = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    int64_t n34 = int64_t(D.n3) * D.n4, n14 = int64_t(D.n1) * D.n4;
+    if(g < n34)
+        O3c[g] = mat(D.O3, int(g / D.n4), int(g % D.n4));
+    else if(g < n34 + n14) {
+        int64_t h = g - n34;
+        C4t[h] = mat(D.CX4, int(h % D.n4), int(h / D.n4));
+    }
+}
+
+__global__ void __launch_bounds__(256) g2_count_kernel(G2Dev D, const double* __restrict__ O3c,
+                                                       const double* __restrict__ C4t, const uint64_t* __restrict__ pairs,
+                                                       int64_t nrows, int* __restrict__ cnt) {
+    int const lane = threadIdx.x & 63;
+    int64_t const nwaves = int64_t(gridDim.x) * (blockDim.x >> 6);
+    for(int64_t row = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6; row < nrows; row += nwaves) {
+        RowCtx r = row_ctx(D, pairs, row / D.n3, int(row % D.n3));
+        int c = 0;
+        if(r.active) {
+            const double* o3 = O3c + int64_t(r.i3) * D.n4;
+            const double* c4 = C4t + int64_t(r.i1) * D.n4;
+            for(int i4 = lane; i4 < D.n4; i4 += 64)
+                c += __popc(kinds_of(D, r.ab, r.s3, r.wi, r.wj, r.wk, o3[i4], c4[i4], D.w4[i4]));
+        }
+        c = wave_sum_int(c);
+        if(lane == 0)
+            cnt[row] = c;
+    }
 }
 
 __global__ void pair_sum_kernel(int64_t npairs, int n3, const int* __restrict__ cnt, int64_t* __restrict__ out) {
@@ -144,29 +180,46 @@ __global__ void widen_kernel(int64_t n, const int* __restrict__ in, int64_t* __r
         out[k] = in[k];
 }
 
-__global__ void g2_emit_kernel(G2Dev D, const uint64_t* __restrict__ pairs, int64_t row0, int64_t nrows, int64_t ntot,
-                               const int64_t* __restrict__ off, const uint32_t* __restrict__ r12,
-                               const uint32_t* __restrict__ r23, const uint32_t* __restrict__ r34,
-                               uint64_t* __restrict__ keys, uint64_t* __restrict__ vals) {
-    int64_t lr = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if(lr >= nrows)
-        return;
-    int64_t row = row0 + lr;
-    RowCtx r = row_ctx(D, pairs, row / D.n3, int(row % D.n3));
-    if(!r.active)
-        return;
-    int64_t at = off[lr];
-    int64_t const end = (lr + 1 < nrows) ? off[lr + 1] : ntot; // guard: never write past this row's slots
-    uint64_t k12 = uint64_t(r12[int64_t(r.i1) * D.n2 + r.i2]) << 42;
-    uint64_t k23 = uint64_t(r23[int64_t(r.i2) * D.n3 + r.i3]) << 21;
-    uint64_t base = uint64_t(r.i1) | (uint64_t(r.i2) << 16) | (uint64_t(r.i3) << 32);
-    for_each_term(D, r.i1, r.i2, r.i3, r.ab, r.s3, [&](int kind, int i4) {
-        if(at >= end)
-            return;
-        keys[at] = (uint64_t(kind) << 62) | k12 | k23 | uint64_t(r34[int64_t(r.i3) * D.n4 + i4]);
-        vals[at] = base | (uint64_t(i4) << 48);
-        ++at;
-    });
+__global__ void __launch_bounds__(256) g2_emit_kernel(G2Dev D, const double* __restrict__ O3c,
+                                                      const double* __restrict__ C4t, const uint64_t* __restrict__ pairs,
+                                                      int64_t row0, int64_t nrows, int64_t ntot,
+                                                      const int64_t* __restrict__ off, const uint32_t* __restrict__ r12,
+                                                      const uint32_t* __restrict__ r23, const uint32_t* __restrict__ r34,
+                                                      uint64_t* __restrict__ keys, uint64_t* __restrict__ vals) {
+    int const lane = threadIdx.x & 63;
+    int64_t const nwaves = int64_t(gridDim.x) * (blockDim.x >> 6);
+    for(int64_t lr = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6; lr < nrows; lr += nwaves) {
+        int64_t row = row0 + lr;
+        RowCtx r = row_ctx(D, pairs, row / D.n3, int(row % D.n3));
+        if(!r.active)
+            continue;
+        int64_t at = off[lr];
+        int64_t const end = (lr + 1 < nrows) ? off[lr + 1] : ntot; // never write past this row's slots
+        uint64_t const k12 = uint64_t(r12[int64_t(r.i1) * D.n2 + r.i2]) << 42;
+        uint64_t const k23 = uint64_t(r23[int64_t(r.i2) * D.n3 + r.i3]) << 21;
+        uint64_t const base = uint64_t(r.i1) | (uint64_t(r.i2) << 16) | (uint64_t(r.i3) << 32);
+        const double* o3 = O3c + int64_t(r.i3) * D.n4;
+        const double* c4 = C4t + int64_t(r.i1) * D.n4;
+        const uint32_t* r34row = r34 + int64_t(r.i3) * D.n4;
+        for(int b4 = 0; b4 < D.n4; b4 += 64) {
+            int const i4 = b4 + lane;
+            unsigned m = 0u;
+            if(i4 < D.n4)
+                m = kinds_of(D, r.ab, r.s3, r.wi, r.wj, r.wk, o3[i4], c4[i4], D.w4[i4]);
+            int total;
+            int64_t p = at + wave_excl_scan_int(__popc(m), lane, total);
+            uint64_t const k34 = (i4 < D.n4) ? uint64_t(r34row[i4]) : 0;
+            for(int kind = 0; kind < 4; ++kind)
+                if((m >> kind) & 1u) {
+                    if(p < end) {
+                        keys[p] = (uint64_t(kind) << 62) | k12 | k23 | k34;
+                        vals[p] = base | (uint64_t(i4) << 48);
+                    }
+                    ++p;
+                }
+            at += total;
+        }
+    }
 }
 
 __global__ void heads_kernel(int64_t n, const uint64_t* __restrict__ keys, int64_t* __restrict__ head) {
@@ -265,18 +318,40 @@ __global__ void g2_compact_kernel(int64_t ngroups, const int64_t* __restrict__ l
     }
 }
 
-__global__ void kind_count_kernel(int64_t ngroups, const int64_t* __restrict__ live, const int64_t* __restrict__ start,
-                                  const uint64_t* __restrict__ keys, unsigned long long* __restrict__ cnt) {
-    int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if(gi < ngroups && live[gi])
-        atomicAdd(&cnt[keys[start[gi]] >> 62], 1ull);
+// Live groups per kind.  Groups are sorted by key and the kind sits in the top key bits, so the
+// groups of one kind are contiguous: binary-search the kind boundaries and difference the
+// exclusive scan of the live flags (one thread; no atomics).
+__global__ void kind_count_kernel(int64_t ngroups, const int64_t* __restrict__ live, const int64_t* __restrict__ lpos,
+                                  const int64_t* __restrict__ start, const uint64_t* __restrict__ keys,
+                                  int64_t* __restrict__ cnt) {
+    if(blockIdx.x != 0 || threadIdx.x != 0)
+        return;
+    int64_t before[5];
+    for(int k = 0; k <= 4; ++k) {
+        int64_t lo = 0, hi = ngroups; // first group with kind >= k
+        while(lo < hi) {
+            int64_t mid = (lo + hi) >> 1;
+            if(int(keys[start[mid]] >> 62) >= k)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        before[k] = (lo < ngroups) ? lpos[lo] : (lpos[ngroups - 1] + live[ngroups - 1]);
+    }
+    for(int k = 0; k < 4; ++k)
+        cnt[k] = before[k + 1] - before[k];
 }
 
 // ---- 4. evaluation --------------------------------------------------------------------------
+// One workgroup = 256 threads x kFPT frequency triplets, one slice of the term list.  Terms are
+// staged through LDS 256 at a time (with their per-term constants precomputed at staging);
+// every wave then reads each term with a uniform LDS address and evaluates it for all its
+// lanes' frequencies.  Partial sums per slice are reduced in a fixed order (deterministic).
 constexpr int kChunk = 256;
-constexpr int kFPT = 2; // frequencies per thread
+constexpr int kFPT = 4; // frequencies per thread
 
 __device__ inline double rcp_nr(double x) {
+    // hardware reciprocal estimate refined by two Newton-Raphson steps (full FP64 accuracy)
     double r = __builtin_amdgcn_rcp(x);
     double e = fma(-x, r, 1.0);
     r = fma(r, e, r);
@@ -304,58 +379,139 @@ __device__ inline Freq load_freq(const double* __restrict__ fz, int f, int perm)
     return q;
 }
 
+// Stage term (fields of the SoA term list) into LDS slots.
+//  general path: NR (C, P1, P2, P3); R (Rc, Nc, P1, P2, P3)
+//  imaginary-frequency path (every z purely imaginary, z_k = i w_k, a = -P1, c = -P3):
+//    kind 0: C, a, b=-P2, c, a*b         kind 1: C, a, b'=-P1-P2-P3, c, a*b'
+//    kind 2: Rc, Nc, a, c, a*c, e=-P2-P3, e*e     kind 3: same with e=-P1-P2
+template <int KIND, bool IMAG> struct Stage {
+    static constexpr int NF = IMAG ? (KIND < 2 ? 5 : 7) : (KIND < 2 ? 4 : 5);
+    __device__ static inline void load(const double* __restrict__ T, int64_t cap, int64_t t, double (*sT)[kChunk],
+                                       int slot) {
+        if(!IMAG) {
+#pragma unroll
+            for(int k = 0; k < NF; ++k)
+                sT[k][slot] = T[k * cap + t];
+            return;
+        }
+        if(KIND < 2) {
+            double C = T[t], P1 = T[cap + t], P2 = T[2 * cap + t], P3 = T[3 * cap + t];
+            double a = -P1, c = -P3;
+            double b = (KIND == 0) ? -P2 : ((-P1) - P2) - P3;
+            sT[0][slot] = C;
+            sT[1][slot] = a;
+            sT[2][slot] = b;
+            sT[3][slot] = c;
+            sT[4][slot] = a * b;
+        } else {
+            double Rc = T[t], Nc = T[cap + t], P1 = T[2 * cap + t], P2 = T[3 * cap + t], P3 = T[4 * cap + t];
+            double a = -P1, c = -P3;
+            double e = (KIND == 3) ? (-P1) - P2 : (-P2) - P3;
+            sT[0][slot] = Rc;
+            sT[1][slot] = Nc;
+            sT[2][slot] = a;
+            sT[3][slot] = c;
+            sT[4][slot] = a * c;
+            sT[5][slot] = e;
+            sT[6][slot] = e * e;
+        }
+    }
+};
+
 // KIND 0/1: NonResonantTerm::operator() (TwoParticleGFPart.hpp:475-479)
 // KIND 2/3: ResonantTerm::operator() (TwoParticleGFPart.hpp:481-495), resonance |D| <= res
-template <int KIND>
+template <int KIND, bool IMAG>
 __global__ void __launch_bounds__(256) g2_eval_kernel(int64_t nterms, int64_t slice_len, const double* __restrict__ T,
                                                       int64_t cap, int nfreq, const double* __restrict__ fz, int perm,
                                                       double res, double* __restrict__ partial) {
-    constexpr int NF = (KIND < 2) ? 4 : 5;
-    __shared__ double sT[NF][kChunk];
+    using S = Stage<KIND, IMAG>;
+    __shared__ double sT[S::NF][kChunk];
     int const fbase = (blockIdx.x * blockDim.x + threadIdx.x) * kFPT;
     int64_t const t0 = int64_t(blockIdx.y) * slice_len;
     int64_t const t1 = min(nterms, t0 + slice_len);
-    Freq q[kFPT];
-    double s12r[kFPT], s12i[kFPT], s23r[kFPT], s23i[kFPT], s123r[kFPT], s123i[kFPT];
+    double const res2 = res * res;
+    // per-frequency constants
+    double x[kFPT], y[kFPT], w[kFPT], fa[kFPT], fb[kFPT], fc[kFPT], fd[kFPT];
+    double zr1[kFPT], zr2[kFPT], zr3[kFPT];
     double ar[kFPT], ai[kFPT];
 #pragma unroll
     for(int u = 0; u < kFPT; ++u) {
-        int f = min(fbase + u, nfreq - 1);
-        q[u] = load_freq(fz, f, perm);
-        s12r[u] = q[u].z1r + q[u].z2r;
-        s12i[u] = q[u].z1i + q[u].z2i;
-        s23r[u] = q[u].z2r + q[u].z3r;
-        s23i[u] = q[u].z2i + q[u].z3i;
-        s123r[u] = s12r[u] + q[u].z3r;
-        s123i[u] = s12i[u] + q[u].z3i;
+        Freq q = load_freq(fz, min(fbase + u, nfreq - 1), perm);
+        x[u] = q.z1i;
+        y[u] = q.z2i;
+        w[u] = q.z3i;
+        zr1[u] = q.z1r;
+        zr2[u] = q.z2r;
+        zr3[u] = q.z3r;
+        if(IMAG) {
+            if(KIND == 0) {
+                fa[u] = x[u] * y[u];
+            } else if(KIND == 1) {
+                fb[u] = (x[u] + y[u]) + w[u]; // Im(z1+z2+z3)
+                fa[u] = x[u] * fb[u];
+            } else {
+                fa[u] = x[u] * w[u];
+                fb[u] = (KIND == 3) ? x[u] + y[u] : y[u] + w[u];
+                fc[u] = fb[u] * fb[u];
+            }
+        } else {
+            fa[u] = q.z1r + q.z2r;                 // Re(z1+z2)
+            fb[u] = q.z1i + q.z2i;                 // Im(z1+z2)
+            fc[u] = q.z2r + q.z3r;                 // Re(z2+z3)
+            fd[u] = q.z2i + q.z3i;                 // Im(z2+z3)
+        }
         ar[u] = 0.0;
         ai[u] = 0.0;
     }
-    double const res2 = res * res;
     for(int64_t c0 = t0; c0 < t1; c0 += kChunk) {
-        int cnt = int(min<int64_t>(kChunk, t1 - c0));
+        int const cnt = int(min<int64_t>(kChunk, t1 - c0));
         __syncthreads();
-        if(threadIdx.x < cnt) {
-#pragma unroll
-            for(int k = 0; k < NF; ++k)
-                sT[k][threadIdx.x] = T[k * cap + c0 + threadIdx.x];
-        }
+        if(threadIdx.x < cnt)
+            S::load(T, cap, c0 + threadIdx.x, sT, threadIdx.x);
         __syncthreads();
+#pragma unroll 2
         for(int k = 0; k < cnt; ++k) {
-            double const P1 = sT[NF - 3][k], P2 = sT[NF - 2][k], P3 = sT[NF - 1][k];
-            if(KIND < 2) {
-                double const C = sT[0][k];
+            if(IMAG && KIND < 2) {
+                double const C = sT[0][k], a = sT[1][k], b = sT[2][k], c = sT[3][k], ab = sT[4][k];
 #pragma unroll
                 for(int u = 0; u < kFPT; ++u) {
-                    double d1r = q[u].z1r - P1, d1i = q[u].z1i;
-                    double d3r = q[u].z3r - P3, d3i = q[u].z3i;
+                    double const yy = (KIND == 0) ? y[u] : fb[u];
+                    double const X = ab - fa[u];             // Re((a+ix)(b+iy))
+                    double const Y = fma(a, yy, b * x[u]);   // Im
+                    double const Dr = fma(X, c, -Y * w[u]);
+                    double const Di = fma(X, w[u], Y * c);
+                    double const s = C * rcp_nr(fma(Dr, Dr, Di * Di));
+                    ar[u] = fma(s, Dr, ar[u]);
+                    ai[u] = fma(-s, Di, ai[u]);
+                }
+            } else if(IMAG) {
+                double const Rc = sT[0][k], Nc = sT[1][k], a = sT[2][k], c = sT[3][k], ac = sT[4][k], e = sT[5][k],
+                             e2 = sT[6][k];
+#pragma unroll
+                for(int u = 0; u < kFPT; ++u) {
+                    double const Qr = ac - fa[u];              // Re((a+ix)(c+iw))
+                    double const Qi = fma(a, w[u], c * x[u]);  // Im
+                    bool const resonant = e2 + fc[u] <= res2;  // |E|^2, E = e + i fb
+                    double const Gr = resonant ? Qr : fma(e, Qr, -fb[u] * Qi);
+                    double const Gi = resonant ? Qi : fma(e, Qi, fb[u] * Qr);
+                    double const K = resonant ? Rc : Nc;
+                    double const s = K * rcp_nr(fma(Gr, Gr, Gi * Gi));
+                    ar[u] = fma(s, Gr, ar[u]);
+                    ai[u] = fma(-s, Gi, ai[u]);
+                }
+            } else if(KIND < 2) {
+                double const C = sT[0][k], P1 = sT[1][k], P2 = sT[2][k], P3 = sT[3][k];
+#pragma unroll
+                for(int u = 0; u < kFPT; ++u) {
+                    double d1r = zr1[u] - P1, d1i = x[u];
+                    double d3r = zr3[u] - P3, d3i = w[u];
                     double d2r, d2i;
                     if(KIND == 0) {
-                        d2r = q[u].z2r - P2;
-                        d2i = q[u].z2i;
+                        d2r = zr2[u] - P2;
+                        d2i = y[u];
                     } else {
-                        d2r = s123r[u] - P1 - P2 - P3;
-                        d2i = s123i[u];
+                        d2r = fa[u] + zr3[u] - P1 - P2 - P3;
+                        d2i = fb[u] + w[u];
                     }
                     double xr = d1r * d2r - d1i * d2i, xi = d1r * d2i + d1i * d2r;
                     double Dr = xr * d3r - xi * d3i, Di = xr * d3i + xi * d3r;
@@ -364,22 +520,21 @@ __global__ void __launch_bounds__(256) g2_eval_kernel(int64_t nterms, int64_t sl
                     ai[u] = fma(-s, Di, ai[u]);
                 }
             } else {
-                double const Rc = sT[0][k], Nc = sT[1][k];
+                double const Rc = sT[0][k], Nc = sT[1][k], P1 = sT[2][k], P2 = sT[3][k], P3 = sT[4][k];
 #pragma unroll
                 for(int u = 0; u < kFPT; ++u) {
-                    double d1r = q[u].z1r - P1, d1i = q[u].z1i;
-                    double d3r = q[u].z3r - P3, d3i = q[u].z3i;
+                    double d1r = zr1[u] - P1, d1i = x[u];
+                    double d3r = zr3[u] - P3, d3i = w[u];
                     double Qr = d1r * d3r - d1i * d3i, Qi = d1r * d3i + d1i * d3r;
                     double Er, Ei;
                     if(KIND == 3) {
-                        Er = s12r[u] - P1 - P2;
-                        Ei = s12i[u];
+                        Er = fa[u] - P1 - P2;
+                        Ei = fb[u];
                     } else {
-                        Er = s23r[u] - P2 - P3;
-                        Ei = s23i[u];
+                        Er = fc[u] - P2 - P3;
+                        Ei = fd[u];
                     }
                     bool const resonant = fma(Er, Er, Ei * Ei) <= res2;
-                    // value = (resonant ? Rc : Nc / E) / Q = K / G
                     double Gr = resonant ? Qr : (Er * Qr - Ei * Qi);
                     double Gi = resonant ? Qi : (Er * Qi + Ei * Qr);
                     double K = resonant ? Rc : Nc;
@@ -414,10 +569,29 @@ __global__ void g2_eval_reduce_kernel(int nfreq, int nslices, const double* __re
     acc[2 * f + 1] += si;
 }
 
+// 1 if some frequency has a non-zero real part
+__global__ void any_real_kernel(int n, const double* __restrict__ fz, int* __restrict__ flag) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if(k < n && fz[2 * k] != 0.0)
+        flag[0] = 1;
+}
+
 struct TermSpan {
     const double* base; // SoA base (NF arrays of stride cap)
     int64_t cap, begin, count;
 };
+
+template <bool IMAG>
+void launch_kind(pomx_ctx* ctx, int k, dim3 grid, TermSpan const& s, int64_t slice, int nfreq, const double* d_freq,
+                 int perm, double res, double* partial) {
+    const double* T = s.base + s.begin;
+    switch(k) {
+    case 0: g2_eval_kernel<0, IMAG><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, nfreq, d_freq, perm, res, partial); break;
+    case 1: g2_eval_kernel<1, IMAG><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, nfreq, d_freq, perm, res, partial); break;
+    case 2: g2_eval_kernel<2, IMAG><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, nfreq, d_freq, perm, res, partial); break;
+    default: g2_eval_kernel<3, IMAG><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, nfreq, d_freq, perm, res, partial); break;
+    }
+}
 
 int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res, int nfreq, const double* d_freq,
                    double* d_acc) {
@@ -426,6 +600,14 @@ int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res,
         total += spans[k].count;
     if(total == 0 || nfreq == 0)
         return POMX_OK;
+    WsBuf<int> flag(ctx, 1);
+    int rc = pomx_fill_zero(ctx, flag.p, sizeof(int));
+    if(rc)
+        return rc;
+    any_real_kernel<<<grid_for(3 * nfreq), kBlock, 0, ctx->stream>>>(3 * nfreq, d_freq, flag);
+    int has_real = 0;
+    POMX_HIP(ctx, hipMemcpyAsync(&has_real, flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     int ftiles = (nfreq + 256 * kFPT - 1) / (256 * kFPT);
     int64_t want = std::max<int64_t>(1, 4096 / ftiles);
     int64_t slice = std::max<int64_t>(kChunk, (total + want - 1) / want);
@@ -434,23 +616,18 @@ int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res,
     for(int k = 0; k < 4; ++k)
         nslices = std::max<int64_t>(nslices, (spans[k].count + slice - 1) / slice);
     WsBuf<double> partial(ctx, size_t(nslices) * nfreq * 2);
-    int rc = pomx_fill_zero(ctx, partial.p, sizeof(double) * size_t(nslices) * nfreq * 2);
-    if(rc)
+    if((rc = pomx_fill_zero(ctx, partial.p, sizeof(double) * size_t(nslices) * nfreq * 2)))
         return rc;
     eval_timer_begin(ctx);
     for(int k = 0; k < 4; ++k) {
         TermSpan const& s = spans[k];
         if(!s.count)
             continue;
-        int ns = int((s.count + slice - 1) / slice);
-        dim3 grid(ftiles, ns);
-        const double* T = s.base + s.begin;
-        switch(k) {
-        case 0: g2_eval_kernel<0><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, nfreq, d_freq, perm, res, partial); break;
-        case 1: g2_eval_kernel<1><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, nfreq, d_freq, perm, res, partial); break;
-        case 2: g2_eval_kernel<2><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, nfreq, d_freq, perm, res, partial); break;
-        default: g2_eval_kernel<3><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, nfreq, d_freq, perm, res, partial); break;
-        }
+        dim3 grid(ftiles, unsigned((s.count + slice - 1) / slice));
+        if(has_real)
+            launch_kind<false>(ctx, k, grid, s, slice, nfreq, d_freq, perm, res, partial);
+        else
+            launch_kind<true>(ctx, k, grid, s, slice, nfreq, d_freq, perm, res, partial);
     }
     g2_eval_reduce_kernel<<<grid_for(nfreq), kBlock, 0, ctx->stream>>>(nfreq, nslices, partial, d_acc);
     POMX_CHECK_LAUNCH(ctx);
@@ -547,8 +724,11 @@ extern "C" int pomx_g2_part_compute(pomx_ctx* ctx, const pomx_g2_part* part, dou
     // 2. count terms per row, per pair, and cut chunks at r12 group boundaries
     int64_t const npairs = int64_t(P.n1) * P.n2;
     int64_t const nrows = npairs * P.n3;
+    WsBuf<double> O3c(ctx, int64_t(P.n3) * P.n4), C4t(ctx, int64_t(P.n1) * P.n4);
+    g2_stage_kernel<<<grid_for(int64_t(P.n3) * P.n4 + int64_t(P.n1) * P.n4), kBlock, 0, ctx->stream>>>(D, O3c, C4t);
     WsBuf<int> rowcnt(ctx, nrows);
-    g2_count_kernel<<<grid_for(nrows), kBlock, 0, ctx->stream>>>(D, pairs, nrows, rowcnt);
+    unsigned const row_blocks = unsigned(std::min<int64_t>((nrows + 3) / 4, 1 << 16)); // 4 waves per block
+    g2_count_kernel<<<row_blocks, kBlock, 0, ctx->stream>>>(D, O3c, C4t, pairs, nrows, rowcnt);
     WsBuf<int64_t> paircnt(ctx, npairs);
     pair_sum_kernel<<<grid_for(npairs), kBlock, 0, ctx->stream>>>(npairs, P.n3, rowcnt, paircnt);
     POMX_CHECK_LAUNCH(ctx);
@@ -599,7 +779,8 @@ extern "C" int pomx_g2_part_compute(pomx_ctx* ctx, const pomx_g2_part* part, dou
         if((rc = pomx_exclusive_scan(ctx, nrow, wide, off)))
             return rc;
         WsBuf<uint64_t> k0(ctx, n), k1(ctx, n), v0(ctx, n), v1(ctx, n);
-        g2_emit_kernel<<<grid_for(nrow), kBlock, 0, ctx->stream>>>(D, pairs, row0, nrow, n, off, r12, r23, r34, k0, v0);
+        unsigned const eblocks = unsigned(std::min<int64_t>((nrow + 3) / 4, 1 << 16));
+        g2_emit_kernel<<<eblocks, kBlock, 0, ctx->stream>>>(D, O3c, C4t, pairs, row0, nrow, n, off, r12, r23, r34, k0, v0);
         POMX_CHECK_LAUNCH(ctx);
         if((rc = pomx_sort_pairs(ctx, n, k0, k1, v0, v1, 0, 64)))
             return rc;
@@ -618,13 +799,11 @@ extern "C" int pomx_g2_part_compute(pomx_ctx* ctx, const pomx_g2_part* part, dou
         // number of groups in this chunk
         double thr = tol / double(ng + 1);
         g2_reduce_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, n, ng, start, k1, v1, thr, g, live);
-        WsBuf<unsigned long long> kc(ctx, 4);
-        if((rc = pomx_fill_zero(ctx, kc.p, 4 * sizeof(unsigned long long))))
-            return rc;
-        kind_count_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(ng, live, start, k1, kc);
+        WsBuf<int64_t> kc(ctx, 4);
         if((rc = pomx_exclusive_scan(ctx, ng, live, lpos)))
             return rc;
-        unsigned long long hk[4];
+        kind_count_kernel<<<1, 64, 0, ctx->stream>>>(ng, live, lpos, start, k1, kc);
+        int64_t hk[4];
         POMX_HIP(ctx, hipMemcpyAsync(hk, kc.p, sizeof(hk), hipMemcpyDeviceToHost, ctx->stream));
         POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
         int64_t c[4] = {int64_t(hk[0]), int64_t(hk[1]), int64_t(hk[2]), int64_t(hk[3])};

@@ -172,28 +172,57 @@ __global__ void pair_reduce_kernel(int kind, const PairPartDev* __restrict__ par
     live[gi] = alive ? 1 : 0;
 }
 
-__global__ void part_group_count_kernel(int64_t ngroups, const int64_t* __restrict__ start,
-                                        const uint64_t* __restrict__ keys, unsigned long long* __restrict__ cnt) {
-    int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if(gi >= ngroups)
+// Group ranges per part: groups are sorted by key and the part is the top key field, so
+// first_group[p] (first group of part >= p) follows from a binary search; no atomics.
+__device__ inline int64_t first_group_of(int64_t ngroups, const int64_t* start, const uint64_t* keys, uint64_t part) {
+    int64_t lo = 0, hi = ngroups;
+    while(lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if((keys[start[mid]] >> kPartShift) >= part)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+// groups of each part that are pole terms (the zero-pole group of chi is excluded)
+__global__ void part_group_count_kernel(int nparts, int64_t ngroups, const int64_t* __restrict__ start,
+                                        const uint64_t* __restrict__ keys, int64_t* __restrict__ first,
+                                        int64_t* __restrict__ cnt) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if(p > nparts)
         return;
-    uint64_t key = keys[start[gi]];
-    if((key & kBinMask) != kZeroPoleBin)
-        atomicAdd(&cnt[key >> kPartShift], 1ull);
+    int64_t f = first_group_of(ngroups, start, keys, uint64_t(p));
+    first[p] = f;
+    if(p < nparts) {
+        int64_t g = first_group_of(ngroups, start, keys, uint64_t(p) + 1) - f;
+        if(g > 0 && (keys[start[f + g - 1]] & kBinMask) == kZeroPoleBin)
+            --g;
+        cnt[p] = g;
+    }
+}
+
+// live terms of each part (zero-pole groups are never live)
+__global__ void part_live_count_kernel(int nparts, int64_t ngroups, const int64_t* __restrict__ first,
+                                       const int64_t* __restrict__ live, const int64_t* __restrict__ lpos,
+                                       int64_t* __restrict__ cnt) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if(p >= nparts)
+        return;
+    auto before = [&](int64_t g) { return g < ngroups ? lpos[g] : lpos[ngroups - 1] + live[ngroups - 1]; };
+    cnt[p] = before(first[p + 1]) - before(first[p]);
 }
 
 __global__ void compact_kernel(int64_t ngroups, const int64_t* __restrict__ live, const int64_t* __restrict__ pos,
                                const double* __restrict__ gR, const double* __restrict__ gP,
-                               const int64_t* __restrict__ start, const uint64_t* __restrict__ keys,
-                               double* __restrict__ outR, double* __restrict__ outP,
-                               unsigned long long* __restrict__ part_count) {
+                               double* __restrict__ outR, double* __restrict__ outP) {
     int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if(gi >= ngroups || !live[gi])
         return;
     int64_t at = pos[gi];
     outR[at] = gR[gi];
     outP[at] = gP[gi];
-    atomicAdd(&part_count[keys[start[gi]] >> kPartShift], 1ull);
 }
 
 // ---- K9: acc[w] += sgn * sum_t R_t / (z_w - P_t) --------------------------------------------
@@ -305,7 +334,7 @@ extern "C" int pomx_pair_terms(pomx_ctx* ctx, int kind, int nparts, const pomx_p
     POMX_HIP(ctx, hipMemcpyAsync(&last_pos, pos.p + ncand - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
     POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     int64_t n = last_pos + last_flag;
-    std::vector<unsigned long long> cnt(nparts, 0);
+    std::vector<int64_t> cnt(nparts, 0);
     std::vector<double> zp(nparts, 0.0);
     double *outR = nullptr, *outP = nullptr;
     if(n > 0) {
@@ -327,14 +356,12 @@ extern "C" int pomx_pair_terms(pomx_ctx* ctx, int kind, int nparts, const pomx_p
         POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
         int64_t ngroups = lh + lg;
         WsBuf<int64_t> start(ctx, ngroups), live(ctx, ngroups), lpos(ctx, ngroups), pgroups(ctx, nparts);
+        WsBuf<int64_t> first(ctx, nparts + 1), d_cnt(ctx, nparts);
         WsBuf<double> gR(ctx, ngroups), gP(ctx, ngroups), d_zp(ctx, nparts);
-        WsBuf<unsigned long long> d_cnt(ctx, nparts);
         group_start_kernel<<<grid_for(n), kBlock, 0, ctx->stream>>>(n, head, gid, start);
-        if((rc = pomx_fill_zero(ctx, pgroups.p, sizeof(int64_t) * nparts)) || (rc = pomx_fill_zero(ctx, d_zp.p, sizeof(double) * nparts)) ||
-           (rc = pomx_fill_zero(ctx, d_cnt.p, sizeof(unsigned long long) * nparts)))
+        if((rc = pomx_fill_zero(ctx, d_zp.p, sizeof(double) * nparts)))
             return rc;
-        part_group_count_kernel<<<grid_for(ngroups), kBlock, 0, ctx->stream>>>(
-            ngroups, start, k1, reinterpret_cast<unsigned long long*>(pgroups.p));
+        part_group_count_kernel<<<grid_for(nparts + 1), kBlock, 0, ctx->stream>>>(nparts, ngroups, start, k1, first, pgroups);
         pair_reduce_kernel<<<grid_for(ngroups), kBlock, 0, ctx->stream>>>(kind, d_parts, nparts, n, ngroups, start, k1, v1,
                                                                            pgroups, res, tol, gR, gP, live, d_zp);
         if((rc = pomx_exclusive_scan(ctx, ngroups, live, lpos)))
@@ -346,9 +373,10 @@ extern "C" int pomx_pair_terms(pomx_ctx* ctx, int kind, int nparts, const pomx_p
         int64_t nlive = ll + lp;
         POMX_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&outR), sizeof(double) * std::max<int64_t>(nlive, 1)));
         POMX_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&outP), sizeof(double) * std::max<int64_t>(nlive, 1)));
-        compact_kernel<<<grid_for(ngroups), kBlock, 0, ctx->stream>>>(ngroups, live, lpos, gR, gP, start, k1, outR, outP, d_cnt);
+        compact_kernel<<<grid_for(ngroups), kBlock, 0, ctx->stream>>>(ngroups, live, lpos, gR, gP, outR, outP);
+        part_live_count_kernel<<<grid_for(nparts), kBlock, 0, ctx->stream>>>(nparts, ngroups, first, live, lpos, d_cnt);
         POMX_CHECK_LAUNCH(ctx);
-        POMX_HIP(ctx, hipMemcpyAsync(cnt.data(), d_cnt.p, sizeof(unsigned long long) * nparts, hipMemcpyDeviceToHost, ctx->stream));
+        POMX_HIP(ctx, hipMemcpyAsync(cnt.data(), d_cnt.p, sizeof(int64_t) * nparts, hipMemcpyDeviceToHost, ctx->stream));
         POMX_HIP(ctx, hipMemcpyAsync(zp.data(), d_zp.p, sizeof(double) * nparts, hipMemcpyDeviceToHost, ctx->stream));
         int herr = 0;
         POMX_HIP(ctx, hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
