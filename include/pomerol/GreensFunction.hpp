@@ -101,10 +101,14 @@ public:
     RealType PoleResolution = 1e-8;
     RealType CoefficientTolerance = 1e-8;
 
+    /// Anomalous: elements <c_i c_j> (two annihilators) instead of <c_i c^+_j> (GFContainer.cpp:34-48).
+    bool const Anomalous;
+
     template <typename... IndexTypes>
     GFContainer(IndexClassification<IndexTypes...> const& IndexInfo, StatesClassification const& S, Hamiltonian const& H,
-                DensityMatrix const& DM, FieldOperatorContainer const& Ops)
-        : Thermal(DM.beta), NumIndices(IndexInfo.getIndexSize()), S(S), H(H), DM(DM), Operators(Ops) {}
+                DensityMatrix const& DM, FieldOperatorContainer const& Ops, bool Anomalous = false)
+        : Thermal(DM.beta), NumIndices(IndexInfo.getIndexSize()), S(S), H(H), DM(DM), Operators(Ops),
+          Anomalous(Anomalous) {}
 
     void prepareAll(std::set<IndexCombination2> const& Indices = {});
     void computeAll();

@@ -140,6 +140,50 @@ int pomx_g2_part_compute(pomx_ctx* ctx, const pomx_g2_part* part, double beta, d
 int pomx_g2_terms_eval(pomx_ctx* ctx, const pomx_g2_terms* terms, int perm, double pole_res, int nfreq,
                        const double* d_freq, double* d_acc);
 
+/* ---- batched full-ED + G(i w_n) solves (no counterpart in the reference) ---------------
+ * Many independent Hamiltonians with uneven sector sizes solved together: block assembly,
+ * eigensolve (persistent workgroups pulling blocks from a size-sorted queue), Boltzmann weights
+ * per instance, rotation of c^+ for one index, and G(i w_n) = sum R/(i w_n - P) on the first
+ * nfreq fermionic Matsubara frequencies of each instance, fused (terms are generated and
+ * evaluated on the fly, with the reference's per-term thresholds |R| > coeff_tol and no
+ * pole merging).  The whole solve is captured once in a hipGraph and replayed.  All
+ * descriptor arrays are host arrays copied at creation; offsets count doubles. */
+typedef struct {
+    int ninst;   /* instances */
+    int nfreq;   /* Matsubara frequencies per instance, n = 0..nfreq-1 */
+    int nblocks; /* symmetry blocks of all instances */
+    const int* block_n;                /* size of each block */
+    const int64_t* block_mat_off;      /* block matrix offset in the matrix arena */
+    const int64_t* block_state_off;    /* first state of the block in the state arrays */
+    int64_t mat_len, nstates;          /* arena sizes */
+    int64_t nnz;                       /* Hamiltonian non-zeros (unique positions) */
+    const int64_t* h_pos;              /* position in the matrix arena */
+    const double* h_val;
+    const double* inst_beta;           /* inverse temperature per instance */
+    const int64_t* inst_state_off;     /* states of an instance are contiguous */
+    const int64_t* inst_nstates;
+    int njobs;                         /* connected (from -> to) block pairs of c^+ */
+    const pomx_rot_job* jobs;          /* u_*_off index the matrix arena, out_off the operator arena */
+    const int* job_inst;
+    const int64_t* job_from_state_off; /* outer block (G: <out|c|in> with out = from block of c^+) */
+    const int64_t* job_to_state_off;
+    const int* src;                    /* gather maps (pomx_rotate_batched convention) */
+    int64_t src_len, ops_len;
+    double coeff_tol;
+} pomx_batch_desc;
+
+typedef struct pomx_batch pomx_batch;
+int pomx_batch_create(pomx_ctx* ctx, const pomx_batch_desc* desc, pomx_batch** batch);
+/* One batched solve; use_graph != 0 replays the captured hipGraph (captured on first use). */
+int pomx_batch_solve(pomx_ctx* ctx, pomx_batch* batch, int use_graph);
+/* h_G: ninst * nfreq interleaved complex; h_E0: ninst ground energies (either may be NULL). */
+int pomx_batch_read(pomx_ctx* ctx, pomx_batch* batch, double* h_G, double* h_E0);
+/* Autotuner: times the candidate workgroup sizes per block-size bucket and the G work-item
+ * sizes on this batch, keeps the fastest and recaptures the graph.  A one-line report of the
+ * choices is written to h_report (may be NULL). */
+int pomx_batch_tune(pomx_ctx* ctx, pomx_batch* batch, char* h_report, size_t report_len);
+void pomx_batch_destroy(pomx_ctx* ctx, pomx_batch* batch);
+
 /* ---- timing of the dominant kernel (HIP events on the context stream) ---------------- */
 int pomx_eval_timing(pomx_ctx* ctx, int enable, double* h_total_ms, int64_t* h_launches);
 

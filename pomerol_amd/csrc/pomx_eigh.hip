@@ -19,12 +19,6 @@
 
 namespace {
 
-struct EighJob {
-    int n;
-    int pad;
-    int64_t mat_off, val_off, ws_off;
-};
-
 constexpr int kMaxLdsN = 2048;
 
 __device__ inline double wave_sum(double v) {
@@ -48,11 +42,11 @@ __device__ inline double block_sum(double v, double* red) {
     return s;
 }
 
-__global__ void __launch_bounds__(256) eigh_kernel(const EighJob* __restrict__ jobs, double* __restrict__ mat,
-                                                   double* __restrict__ eval, double* __restrict__ ws,
-                                                   int* __restrict__ status) {
+// One block, one workgroup (128..512 threads).  status_slot receives 1 (QL did not converge) or
+// 2 (NaN eigenvalue); it is left untouched on success.
+__device__ void eigh_one(EighJob const J, double* __restrict__ mat, double* __restrict__ eval, double* __restrict__ ws,
+                         int* __restrict__ status_slot) {
     extern __shared__ double sh[];
-    EighJob const J = jobs[blockIdx.x];
     int const n = J.n;
     int const tid = threadIdx.x, nt = blockDim.x;
     double* __restrict__ A = mat + J.mat_off;
@@ -252,10 +246,56 @@ __global__ void __launch_bounds__(256) eigh_kernel(const EighJob* __restrict__ j
     }
     __syncthreads();
     if(tid == 0 && ctl[3] != 0)
-        status[blockIdx.x] = ctl[3];
+        *status_slot = ctl[3];
+}
+
+// One workgroup per block (grid = number of blocks), or PERSISTENT workgroups that pull block
+// indices from *counter until the list is exhausted (every workgroup reaches the exit test).
+template <bool PERSISTENT>
+__global__ void __launch_bounds__(512) eigh_kernel(const EighJob* __restrict__ jobs, int njobs, int* __restrict__ counter,
+                                                   double* __restrict__ mat, double* __restrict__ eval,
+                                                   double* __restrict__ ws, int* __restrict__ status) {
+    if(!PERSISTENT) {
+        eigh_one(jobs[blockIdx.x], mat, eval, ws, status + blockIdx.x);
+        return;
+    }
+    __shared__ int next;
+    while(true) {
+        __syncthreads();
+        if(threadIdx.x == 0)
+            next = atomicAdd(counter, 1);
+        __syncthreads();
+        int const j = next;
+        if(j >= njobs)
+            break;
+        eigh_one(jobs[j], mat, eval, ws, status + j);
+    }
 }
 
 } // namespace
+
+size_t eigh_lds_bytes(int nmax) { return (size_t(6) * nmax + 8) * sizeof(double) + 16 * sizeof(int); }
+int64_t eigh_ws_doubles(int n) { return n > 1 ? int64_t(n) * n + n : 0; }
+int eigh_max_n() { return kMaxLdsN; }
+
+int launch_eigh(pomx_ctx* ctx, const EighJob* d_jobs, int njobs, int nmax, int block_threads, int persistent_grid,
+                int* counter, double* d_mat, double* d_eval, double* d_ws, int* d_status) {
+    if(njobs <= 0)
+        return POMX_OK;
+    size_t lds = eigh_lds_bytes(nmax);
+    if(persistent_grid > 0) {
+        POMX_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(eigh_kernel<true>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+        eigh_kernel<true><<<persistent_grid, block_threads, lds, ctx->stream>>>(d_jobs, njobs, counter, d_mat, d_eval, d_ws,
+                                                                                 d_status);
+    } else {
+        POMX_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(eigh_kernel<false>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+        eigh_kernel<false><<<njobs, block_threads, lds, ctx->stream>>>(d_jobs, njobs, counter, d_mat, d_eval, d_ws, d_status);
+    }
+    POMX_CHECK_LAUNCH(ctx);
+    return POMX_OK;
+}
 
 extern "C" int pomx_eigh_batched(pomx_ctx* ctx, int nblocks, const int* h_n, const int64_t* h_mat_off,
                                  const int64_t* h_val_off, double* d_mat, double* d_eval) {
@@ -280,8 +320,7 @@ extern "C" int pomx_eigh_batched(pomx_ctx* ctx, int nblocks, const int* h_n, con
                              "pomx_eigh_batched: block of size " + std::to_string(n) + " exceeds " +
                                  std::to_string(kMaxLdsN));
         jobs[k] = EighJob{n, 0, h_mat_off[b], h_val_off[b], ws_total};
-        if(n > 1)
-            ws_total += int64_t(n) * n + n;
+        ws_total += eigh_ws_doubles(n);
         nmax = std::max(nmax, n);
     }
     WsBuf<EighJob> d_jobs(ctx, nblocks);
@@ -293,11 +332,8 @@ extern "C" int pomx_eigh_batched(pomx_ctx* ctx, int nblocks, const int* h_n, con
     int rc = pomx_fill_zero(ctx, d_status.p, sizeof(int) * nblocks);
     if(rc)
         return rc;
-    size_t lds = (size_t(6) * nmax + 8) * sizeof(double) + 16 * sizeof(int);
-    POMX_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(eigh_kernel),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    eigh_kernel<<<nblocks, 256, lds, ctx->stream>>>(d_jobs, d_mat, d_eval, d_ws, d_status);
-    POMX_CHECK_LAUNCH(ctx);
+    if((rc = launch_eigh(ctx, d_jobs, nblocks, nmax, 256, 0, nullptr, d_mat, d_eval, d_ws, d_status)))
+        return rc;
     std::vector<int> st(nblocks);
     POMX_HIP(ctx, hipMemcpyAsync(st.data(), d_status.p, sizeof(int) * nblocks, hipMemcpyDeviceToHost, ctx->stream));
     POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -92,7 +92,23 @@ inline unsigned grid_for(int64_t n, int per_block = kBlock) {
     return unsigned(g);
 }
 
-// Matsubara-style complex helpers (device)
-struct dcplx {
-    double re, im;
+// ---- shared device routines (used by the plain and the batched paths) ---------------------
+struct EighJob {
+    int n;
+    int pad;
+    int64_t mat_off, val_off, ws_off; // doubles: block matrix, eigenvalues, workspace (n*n + n)
 };
+size_t eigh_lds_bytes(int nmax);
+int64_t eigh_ws_doubles(int n);
+int eigh_max_n();
+// Launch the per-block eigensolver on device-resident jobs, without host synchronisation.
+// persistent_grid > 0: that many workgroups pull jobs from *counter (zeroed by the caller).
+int launch_eigh(pomx_ctx* ctx, const EighJob* d_jobs, int njobs, int nmax, int block_threads, int persistent_grid,
+                int* counter, double* d_mat, double* d_eval, double* d_ws, int* d_status);
+
+struct RotTile {
+    int job, i0, j0, pad;
+};
+// Launch the rotation GEMM over a device-resident tile list (no host synchronisation).
+int launch_rotate(pomx_ctx* ctx, const RotTile* d_tiles, int ntiles, const pomx_rot_job* d_jobs, const double* d_U,
+                  const int* d_src, double* d_out, double cutoff);

@@ -12,10 +12,6 @@ namespace {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-struct RotTile {
-    int job, i0, j0, pad;
-};
-
 __global__ void __launch_bounds__(256) rotate_kernel(const RotTile* __restrict__ tiles, const pomx_rot_job* __restrict__ jobs,
                                                      const double* __restrict__ U, const int* __restrict__ src,
                                                      double* __restrict__ out, double cutoff) {
@@ -85,6 +81,15 @@ __global__ void __launch_bounds__(256) rotate_kernel(const RotTile* __restrict__
 
 } // namespace
 
+int launch_rotate(pomx_ctx* ctx, const RotTile* d_tiles, int ntiles, const pomx_rot_job* d_jobs, const double* d_U,
+                  const int* d_src, double* d_out, double cutoff) {
+    if(ntiles <= 0)
+        return POMX_OK;
+    rotate_kernel<<<unsigned(ntiles), 256, 0, ctx->stream>>>(d_tiles, d_jobs, d_U, d_src, d_out, cutoff);
+    POMX_CHECK_LAUNCH(ctx);
+    return POMX_OK;
+}
+
 extern "C" int pomx_rotate_batched(pomx_ctx* ctx, int njobs, const pomx_rot_job* h_jobs, const double* d_U,
                                    const int* d_src, double* d_out, double cutoff) {
     POMX_REQUIRE(ctx, njobs >= 0 && (njobs == 0 || (h_jobs && d_U && d_src && d_out)), "pomx_rotate_batched: bad arguments");
@@ -103,8 +108,9 @@ extern "C" int pomx_rotate_batched(pomx_ctx* ctx, int njobs, const pomx_rot_job*
         return pomx_fail(ctx, POMX_ERR_HIP, "pomx_rotate_batched: out of device memory");
     POMX_HIP(ctx, hipMemcpyAsync(d_tiles.p, tiles.data(), sizeof(RotTile) * tiles.size(), hipMemcpyHostToDevice, ctx->stream));
     POMX_HIP(ctx, hipMemcpyAsync(d_jobs.p, h_jobs, sizeof(pomx_rot_job) * njobs, hipMemcpyHostToDevice, ctx->stream));
-    rotate_kernel<<<unsigned(tiles.size()), 256, 0, ctx->stream>>>(d_tiles, d_jobs, d_U, d_src, d_out, cutoff);
-    POMX_CHECK_LAUNCH(ctx);
+    int rc = launch_rotate(ctx, d_tiles, int(tiles.size()), d_jobs, d_U, d_src, d_out, cutoff);
+    if(rc)
+        return rc;
     POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return POMX_OK;
 }

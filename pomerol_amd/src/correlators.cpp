@@ -183,8 +183,9 @@ void GFContainer::computeAll() {
 GreensFunction& GFContainer::operator()(IndexCombination2 const& Indices) {
     auto it = ElementsMap.find(Indices);
     if(it == ElementsMap.end()) {
-        auto g = std::make_shared<GreensFunction>(S, H, Operators.getAnnihilationOperator(Indices.Index1),
-                                                  Operators.getCreationOperator(Indices.Index2), DM);
+        FieldOperator const& F2 = Anomalous ? static_cast<FieldOperator const&>(Operators.getAnnihilationOperator(Indices.Index2))
+                                            : static_cast<FieldOperator const&>(Operators.getCreationOperator(Indices.Index2));
+        auto g = std::make_shared<GreensFunction>(S, H, Operators.getAnnihilationOperator(Indices.Index1), F2, DM);
         it = ElementsMap.emplace(Indices, g).first;
     }
     return *it->second;

@@ -137,6 +137,33 @@ PYBIND11_MODULE(_pomerol, m) {
         }
     });
 
+    py::class_<BatchSolver>(m, "Batch")
+        .def(py::init([](py::list models, std::vector<double> betas, py::tuple idx, int nfreq, double tol) {
+                 std::vector<Expr> Hs;
+                 for(auto t : models)
+                     Hs.push_back(build_expr(t.cast<py::list>()));
+                 BatchSolver::Index index(idx[0].cast<std::string>(), idx[1].cast<unsigned short>(),
+                                          LatticePresets::spin(idx[2].cast<int>()));
+                 return new BatchSolver(std::move(Hs), std::move(betas), index, nfreq, tol);
+             }),
+             py::arg("models"), py::arg("betas"), py::arg("index"), py::arg("nfreq"), py::arg("coeff_tol") = 1e-8)
+        .def("prepare", &BatchSolver::prepare)
+        .def("solve", &BatchSolver::solve, py::arg("use_graph") = true)
+        .def("autotune", &BatchSolver::autotune)
+        .def("size", &BatchSolver::size)
+        .def("n_structures", &BatchSolver::numStructures)
+        .def("n_blocks", &BatchSolver::numBlocks)
+        .def("E0", [](BatchSolver& b) { return to_np(b.getGroundEnergies()); })
+        .def("G", [](BatchSolver& b) {
+            auto g = b.getG();
+            py::array_t<std::complex<double>> a({g.size(), g.empty() ? 0 : g[0].size()});
+            auto r = a.mutable_unchecked<2>();
+            for(std::size_t k = 0; k < g.size(); ++k)
+                for(std::size_t n = 0; n < g[k].size(); ++n)
+                    r(k, n) = g[k][n];
+            return a;
+        });
+
     py::class_<ED>(m, "ED")
         .def(py::init<py::list, double, bool>(), py::arg("terms"), py::arg("beta"), py::arg("keep_hmat") = false)
         .def_readonly("t_eigh", &ED::t_eigh)
