@@ -190,7 +190,7 @@ def main():
                "sample": f"oracle TwoParticleGFPart compute + fill of {len(small)} config-2 parts ({terms} merged "
                          f"terms x {nfreq} triplets, {secs:.1f}s); generation serial, fill OpenMP"}
 
-    batch = run_batch(args) if args.batch_inst > 0 else None
+    batched = run_batch(args) if args.batch_inst > 0 else None
 
     out = {
         "metric": "2PGF terms*freqs/s",
@@ -213,7 +213,7 @@ def main():
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
                      "kernel": "g2_eval_kernel (FP64 Lehmann sum)", "eval_ms": eval_ms, "eval_launches": launches},
         "cpu_baseline": cpu,
-        "batch": batch,
+        "batch": batched,
     }
     print(json.dumps(out), flush=True)
 

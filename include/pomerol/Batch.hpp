@@ -29,6 +29,10 @@ public:
     /// G_ii(i w_n), n = 0..nfreq-1 (nfreq <= 128), of every Hamiltonian, i = index.
     BatchSolver(std::vector<Expr> Hs, std::vector<RealType> betas, Index index, int nfreq,
                 RealType CoefficientTolerance = 1e-8);
+    /// Parametric instances: H_k = sum_p theta[k][p] * Ops[k][p].  The sector structure is that
+    /// of sum_p Ops[k][p] (every monomial present), so it does not change with theta.
+    BatchSolver(std::vector<std::vector<Expr>> Ops, std::vector<std::vector<RealType>> theta,
+                std::vector<RealType> betas, Index index, int nfreq, RealType CoefficientTolerance = 1e-8);
     ~BatchSolver();
     BatchSolver(BatchSolver const&) = delete;
     BatchSolver& operator=(BatchSolver const&) = delete;
@@ -40,14 +44,28 @@ public:
     /// Autotune workgroup and tile sizes on this batch; returns a report of the choices.
     std::string autotune();
     std::vector<std::vector<ComplexType>> getG() const;
+    /// New parameters for the next solve (same shape as the constructor's theta).
+    void setParameters(std::vector<std::vector<RealType>> const& theta);
+    /// Adjoint of the last solve: Gbar[k][n] = dL/dRe G_k(i w_n) + i dL/dIm G_k(i w_n);
+    /// returns dL/dtheta[k][p].
+    std::vector<std::vector<RealType>> gradient(std::vector<std::vector<ComplexType>> const& Gbar);
     std::vector<RealType> getGroundEnergies() const;
-    std::size_t size() const { return Hs.size(); }
+    std::size_t size() const { return Ops.size(); }
     std::size_t numStructures() const { return structures.size(); }
     int64_t numBlocks() const { return nblocks; }
 
 private:
     struct Structure;
-    std::vector<Expr> Hs;
+    struct Entry {      // theta[k][param] * val contributes to Hamiltonian value u
+        int64_t u;
+        int param;
+        RealType val;
+    };
+    std::vector<std::vector<Expr>> Ops;
+    std::vector<std::vector<RealType>> theta;
+    std::vector<std::vector<Entry>> entries; // per instance, sorted by u
+    std::vector<RealType> values;            // current Hamiltonian values (all instances)
+    void assemble_values();
     std::vector<RealType> betas;
     Index index;
     int nfreq;

@@ -140,6 +140,22 @@ int pomx_g2_part_compute(pomx_ctx* ctx, const pomx_g2_part* part, double beta, d
 int pomx_g2_terms_eval(pomx_ctx* ctx, const pomx_g2_terms* terms, int perm, double pole_res, int nfreq,
                        const double* d_freq, double* d_acc);
 
+/* ---- batched FP64 GEMM (matrix cores) --------------------------------------------------
+ * C = alpha * op(A) op(B) + beta * C for independent jobs; op(X) = X or X^T (trans_* != 0).
+ * Operands are row-major at the given offsets (doubles) into d_A, d_B, d_C, with row strides
+ * lda/ldb/ldc.  op(A) is m x k, op(B) is k x n.  Jobs must not write overlapping parts of C.
+ * No counterpart in the reference: the dense products of Eigen that this library needs
+ * (back-transformations, the adjoint's U^T Ubar and U K) go through this entry. */
+typedef struct {
+    int m, n, k;
+    int trans_a, trans_b;
+    int lda, ldb, ldc;
+    int64_t a_off, b_off, c_off;
+    double alpha, beta;
+} pomx_gemm_job;
+int pomx_gemm_batched(pomx_ctx* ctx, int njobs, const pomx_gemm_job* h_jobs, const double* d_A, const double* d_B,
+                      double* d_C);
+
 /* ---- batched full-ED + G(i w_n) solves (no counterpart in the reference) ---------------
  * Many independent Hamiltonians with uneven sector sizes solved together: block assembly,
  * eigensolve (persistent workgroups pulling blocks from a size-sorted queue), Boltzmann weights
@@ -183,6 +199,16 @@ int pomx_batch_read(pomx_ctx* ctx, pomx_batch* batch, double* h_G, double* h_E0)
  * choices is written to h_report (may be NULL). */
 int pomx_batch_tune(pomx_ctx* ctx, pomx_batch* batch, char* h_report, size_t report_len);
 void pomx_batch_destroy(pomx_ctx* ctx, pomx_batch* batch);
+/* Replace the Hamiltonian values (nnz doubles, in the order of desc->h_val) of every instance;
+ * the next pomx_batch_solve uses them (the captured graph reads the same device buffer). */
+int pomx_batch_set_values(pomx_ctx* ctx, pomx_batch* batch, const double* h_val);
+/* Adjoint of the last pomx_batch_solve (no counterpart in the reference).  h_Gbar holds, per
+ * instance and frequency, dL/dRe G + i dL/dIm G (ninst * nfreq interleaved complex); on return
+ * h_dval[u] = dL/d(h_val[u]) for every Hamiltonian entry u.  Chain: Lehmann sums -> weights and
+ * rotated c^+ -> rotation (recomputed from the eigenvectors) -> eigen-decomposition adjoint
+ * (pairs inside a degenerate eigenspace, |E_i - E_j| <= 1e-10 max(1,|E_i|,|E_j|), carry no
+ * gradient) -> Hamiltonian entries.  Terms dropped by |R| <= coeff_tol carry no gradient. */
+int pomx_batch_adjoint(pomx_ctx* ctx, pomx_batch* batch, const double* h_Gbar, double* h_dval);
 
 /* ---- timing of the dominant kernel (HIP events on the context stream) ---------------- */
 int pomx_eval_timing(pomx_ctx* ctx, int enable, double* h_total_ms, int64_t* h_launches);

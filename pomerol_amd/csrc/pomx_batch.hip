@@ -11,7 +11,7 @@
 //      accumulate R / (i w_n - P) for the instance's frequencies into per-item partials (K6+K9)
 //   6. per-instance reduction of the partials in item order (deterministic)
 // The autotuner picks the workgroup size of each eigensolver bucket and the G work-item size.
-#include "pomx_internal.hpp"
+#include "pomx_batch.hpp"
 
 #include <algorithm>
 #include <cfloat>
@@ -19,20 +19,6 @@
 #include <numeric>
 
 namespace {
-
-constexpr int kBuckets = 4;
-constexpr int kBucketMax[kBuckets] = {32, 128, 512, 1 << 30};
-constexpr int kGThreads = 256;
-
-struct GItem {
-    int job, inst;
-    int64_t t0, t1; // term range, term t = i_out * n_in + i_in
-};
-
-struct JobInfo {
-    int n_out, n_in, inst, pad;
-    int64_t ops_off, out_state, in_state;
-};
 
 __global__ void scatter_h_kernel(int64_t nnz, const int64_t* __restrict__ pos, const double* __restrict__ val,
                                  double* __restrict__ mat) {
@@ -172,33 +158,6 @@ template <class T> T* dev_copy(pomx_ctx* ctx, const T* h, size_t n, int& rc) {
 }
 
 } // namespace
-
-struct pomx_batch {
-    int ninst = 0, nfreq = 0, nblocks = 0, njobs = 0, ntiles = 0, nitems = 0;
-    int64_t mat_len = 0, nstates = 0, nnz = 0, ws_len = 0;
-    double tol = 0;
-    // device data
-    double *mat = nullptr, *eval = nullptr, *w = nullptr, *ws = nullptr, *h_val = nullptr, *beta = nullptr;
-    double *ops = nullptr, *partial = nullptr, *G = nullptr, *E0 = nullptr;
-    int64_t *h_pos = nullptr, *inst_off = nullptr, *inst_cnt = nullptr, *item_off = nullptr;
-    int *src = nullptr, *counters = nullptr, *status = nullptr;
-    EighJob* eigh_jobs = nullptr;
-    pomx_rot_job* jobs = nullptr;
-    RotTile* tiles = nullptr;
-    JobInfo* jinfo = nullptr;
-    GItem* items = nullptr;
-    // host copies needed to rebuild work lists
-    std::vector<JobInfo> h_jinfo;
-    std::vector<int64_t> inst_first_job; // jobs sorted by instance: first job of each instance
-    int bucket_begin[kBuckets + 1] = {0, 0, 0, 0, 0}; // ranges of eigh_jobs (sorted by size, descending)
-    int bucket_nmax[kBuckets] = {1, 1, 1, 1};
-    // tuning
-    int eigh_threads[kBuckets] = {64, 128, 256, 256};
-    int eigh_grid = 0;
-    int64_t terms_per_item = 4096;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-};
 
 namespace {
 
@@ -370,6 +329,10 @@ extern "C" int pomx_batch_create(pomx_ctx* ctx, const pomx_batch_desc* d, pomx_b
     b->jobs = dev_copy(ctx, d->jobs, size_t(d->njobs), rc);
     b->tiles = dev_copy(ctx, tiles.data(), tiles.size(), rc);
     b->jinfo = dev_copy(ctx, b->h_jinfo.data(), b->h_jinfo.size(), rc);
+    b->h_eigh = ej;
+    b->h_jobs.assign(d->jobs, d->jobs + d->njobs);
+    b->h_src.assign(d->src, d->src + d->src_len);
+    b->h_pos_host.assign(d->h_pos, d->h_pos + d->nnz);
     if(!rc)
         rc = build_items(ctx, b);
     hipDeviceProp_t prop;
@@ -503,6 +466,7 @@ extern "C" void pomx_batch_destroy(pomx_ctx* ctx, pomx_batch* b) {
     if(ctx)
         (void)hipStreamSynchronize(ctx->stream);
     drop_graph(b);
+    adjoint_free(b);
     free_batch(b);
     delete b;
 }

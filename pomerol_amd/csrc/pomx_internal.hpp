@@ -112,3 +112,13 @@ struct RotTile {
 // Launch the rotation GEMM over a device-resident tile list (no host synchronisation).
 int launch_rotate(pomx_ctx* ctx, const RotTile* d_tiles, int ntiles, const pomx_rot_job* d_jobs, const double* d_U,
                   const int* d_src, double* d_out, double cutoff);
+
+struct GemmTile {
+    int job, i0, j0, pad;
+};
+// 64 x 64 output tiles of every job, in job order.
+void gemm_tiles(const pomx_gemm_job* jobs, int njobs, std::vector<GemmTile>& tiles);
+bool gemm_job_ok(pomx_gemm_job const& j);
+// Launch the batched MFMA GEMM over device-resident tiles whose jobs share (trans_a, trans_b).
+int launch_gemm(pomx_ctx* ctx, int trans_a, int trans_b, const GemmTile* d_tiles, int ntiles,
+                const pomx_gemm_job* d_jobs, const double* d_A, const double* d_B, double* d_C);

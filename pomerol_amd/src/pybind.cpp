@@ -147,6 +147,37 @@ PYBIND11_MODULE(_pomerol, m) {
                  return new BatchSolver(std::move(Hs), std::move(betas), index, nfreq, tol);
              }),
              py::arg("models"), py::arg("betas"), py::arg("index"), py::arg("nfreq"), py::arg("coeff_tol") = 1e-8)
+        .def(py::init([](py::list param_models, std::vector<std::vector<double>> theta, std::vector<double> betas,
+                         py::tuple idx, int nfreq, double tol) {
+                 std::vector<std::vector<Expr>> ops;
+                 for(auto inst : param_models) {
+                     ops.emplace_back();
+                     for(auto t : inst.cast<py::list>())
+                         ops.back().push_back(build_expr(t.cast<py::list>()));
+                 }
+                 BatchSolver::Index index(idx[0].cast<std::string>(), idx[1].cast<unsigned short>(),
+                                          LatticePresets::spin(idx[2].cast<int>()));
+                 return new BatchSolver(std::move(ops), std::move(theta), std::move(betas), index, nfreq, tol);
+             }),
+             py::arg("param_models"), py::arg("theta"), py::arg("betas"), py::arg("index"), py::arg("nfreq"),
+             py::arg("coeff_tol") = 1e-8)
+        .def("set_params", &BatchSolver::setParameters)
+        .def("gradient",
+             [](BatchSolver& b, py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> gbar) {
+                 if(gbar.ndim() != 2)
+                     throw std::runtime_error("gradient: Gbar must be (instances, nfreq)");
+                 auto r = gbar.unchecked<2>();
+                 std::vector<std::vector<ComplexType>> g(std::size_t(r.shape(0)));
+                 for(std::size_t k = 0; k < g.size(); ++k)
+                     for(py::ssize_t n = 0; n < r.shape(1); ++n)
+                         g[k].push_back(r(py::ssize_t(k), n));
+                 std::vector<std::vector<double>> d;
+                 {
+                     py::gil_scoped_release rel;
+                     d = b.gradient(g);
+                 }
+                 return d;
+             })
         .def("prepare", &BatchSolver::prepare)
         .def("solve", &BatchSolver::solve, py::arg("use_graph") = true)
         .def("autotune", &BatchSolver::autotune)

@@ -74,6 +74,17 @@ def batch_configs(n: int = 512, seed: int = SEED) -> list[AndersonConfig]:
     return out
 
 
+def adjoint_configs(n: int = 64, seed: int = SEED) -> tuple[list[AndersonConfig], list[AndersonConfig]]:
+    """Bath-fit gradient workload: n Anderson instances with 5 bath sites (eps ~ U[-1,1],
+    V ~ U[0.2,0.6], U ~ U[2,4], beta = 20) and, for each, a target parameter set drawn the same
+    way from an independent seed (G_target is that target's G on the same frequencies)."""
+    rng = np.random.default_rng(seed + 7919)
+    Us = rng.uniform(2.0, 4.0, 2 * n)
+    cfgs = [AndersonConfig(5, float(Us[k]), 20.0, seed=seed + 10_000 + k) for k in range(n)]
+    targets = [AndersonConfig(5, float(Us[n + k]), 20.0, seed=seed + 20_000 + k) for k in range(n)]
+    return cfgs, targets
+
+
 def fermionic(n: np.ndarray | int, beta: float) -> np.ndarray:
     return 1j * np.pi * (2 * np.asarray(n) + 1) / beta
 

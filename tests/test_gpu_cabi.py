@@ -148,3 +148,52 @@ def test_pair_eval(cabi, kind):
     got = acc[0::2] + 1j * acc[1::2]
     ref = (R[None, :] / (z[:, None] - P[None, :])).sum(1) * (1 if kind == 0 else -1)
     assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+class GemmJob(ctypes.Structure):
+    _fields_ = [("m", ctypes.c_int), ("n", ctypes.c_int), ("k", ctypes.c_int), ("trans_a", ctypes.c_int),
+                ("trans_b", ctypes.c_int), ("lda", ctypes.c_int), ("ldb", ctypes.c_int), ("ldc", ctypes.c_int),
+                ("a_off", ctypes.c_int64), ("b_off", ctypes.c_int64), ("c_off", ctypes.c_int64),
+                ("alpha", ctypes.c_double), ("beta", ctypes.c_double)]
+
+
+def test_gemm_batched(cabi):
+    """Mixed shapes, all four transposition patterns, padded leading dimensions, alpha/beta."""
+    rng = np.random.default_rng(11)
+    shapes = [(1, 1, 1), (5, 7, 3), (64, 64, 64), (65, 33, 129), (130, 70, 17), (17, 200, 64), (3, 5, 0), (257, 129, 300)]
+    A_parts, B_parts, C_parts, jobs, refs = [], [], [], [], []
+    ao = bo = co = 0
+    for q, (m, n, k) in enumerate(shapes):
+        ta, tb = q % 2, (q // 2) % 2
+        pa_, pb_, pc_ = q % 3, (q + 1) % 3, q % 2  # row padding
+        ra, ca = (k, m) if ta else (m, k)
+        rb, cb = (n, k) if tb else (k, n)
+        A = rng.standard_normal((ra, ca + pa_))
+        B = rng.standard_normal((rb, cb + pb_))
+        C = rng.standard_normal((m, n + pc_))
+        alpha, beta = (1.0, 0.0) if q % 3 == 0 else (-0.5, 1.5)
+        opA = (A[:, :ca].T if ta else A[:, :ca])
+        opB = (B[:, :cb].T if tb else B[:, :cb])
+        ref = C.copy()
+        ref[:, :n] = alpha * (opA @ opB) + beta * C[:, :n]
+        jobs.append(GemmJob(m, n, k, ta, tb, ca + pa_, cb + pb_, n + pc_, ao, bo, co, alpha, beta))
+        A_parts.append(A.ravel())
+        B_parts.append(B.ravel())
+        C_parts.append(C.ravel())
+        refs.append((co, ref))
+        ao += A.size
+        bo += B.size
+        co += C.size
+    print("[cabi] gemm shapes", shapes, flush=True)
+    d_A = cabi.put(np.concatenate(A_parts))
+    d_B = cabi.put(np.concatenate(B_parts))
+    Cflat = np.concatenate(C_parts)
+    d_C = cabi.put(Cflat)
+    arr = (GemmJob * len(jobs))(*jobs)
+    cabi.ok(cabi.lib.pomx_gemm_batched(cabi.ctx, len(jobs), arr, d_A, d_B, d_C))
+    out = cabi.get(d_C, Cflat)
+    for off, ref in refs:
+        got = out[off:off + ref.size].reshape(ref.shape)
+        assert np.max(np.abs(got - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
+    bad = GemmJob(4, 4, 4, 0, 0, 2, 4, 4, 0, 0, 0, 1.0, 0.0)  # lda < k
+    assert cabi.lib.pomx_gemm_batched(cabi.ctx, 1, ctypes.byref(bad), d_A, d_B, d_C) == 1

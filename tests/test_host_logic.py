@@ -56,3 +56,34 @@ def test_hubbard_cluster_bonds():
     assert -0.3 <= tp <= 0.0
     H, beta, _ = W.cfg3_expr()
     assert len(H.indices()) == 18 and beta == 5.0
+
+
+def test_lehmann_g_autograd_convention():
+    """LehmannG hands the solver dL/dRe G + i dL/dIm G and returns its real gradient unchanged:
+    checked with a linear stand-in solver G = theta @ M against torch's own autograd."""
+    import torch
+
+    from pomerol_amd.adjoint import LehmannG
+
+    rng = np.random.default_rng(3)
+    M = rng.standard_normal((4, 6)) + 1j * rng.standard_normal((4, 6))
+    Gt = rng.standard_normal((2, 6)) + 1j * rng.standard_normal((2, 6))
+
+    class Linear:
+        theta = None
+
+        def forward(self, th):
+            self.theta = th.copy()
+            return th @ M
+
+        def backward(self, gbar):  # dL/dtheta = Re(conj(gbar) M^T)
+            return np.real(np.conj(gbar) @ M.T)
+
+    th0 = rng.standard_normal((2, 4))
+    a = torch.tensor(th0, requires_grad=True)
+    L = ((LehmannG.apply(a, Linear()) - torch.from_numpy(Gt)).abs() ** 2).sum()
+    L.backward()
+    b = torch.tensor(th0, requires_grad=True)
+    L2 = ((b.to(torch.complex128) @ torch.from_numpy(M) - torch.from_numpy(Gt)).abs() ** 2).sum()
+    L2.backward()
+    assert np.allclose(a.grad.numpy(), b.grad.numpy(), rtol=1e-12, atol=1e-12)
