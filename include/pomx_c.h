@@ -50,10 +50,18 @@ int pomx_assemble(pomx_ctx* ctx, int64_t nnz, const int64_t* d_pos, const double
  * Replaces HamiltonianPart::computeImpl (src/pomerol/HamiltonianPart.cpp:90-102), i.e. Eigen's
  * SelfAdjointEigenSolver per block.  Block b (size h_n[b]) is the row-major matrix at
  * d_mat + h_mat_off[b]; on return it holds the eigenvectors as columns and d_eval + h_val_off[b]
- * the eigenvalues in ascending order.  Householder tridiagonalisation, implicit QL on the
- * tridiagonal matrix, Householder back-transformation; one workgroup per block. */
+ * the eigenvalues in ascending order.  Blocks of size <= 512: one workgroup per block
+ * (Householder tridiagonalisation, implicit QL, Householder back-transformation), all such
+ * blocks concurrently.  Larger blocks, one at a time on the whole device: blocked Householder
+ * tridiagonalisation with MFMA trailing updates, divide and conquer on the tridiagonal matrix,
+ * blocked (WY) MFMA back-transformation; workspace about 5 n^2 doubles. */
 int pomx_eigh_batched(pomx_ctx* ctx, int nblocks, const int* h_n, const int64_t* h_mat_off,
                       const int64_t* h_val_off, double* d_mat, double* d_eval);
+/* Divide-and-conquer eigensolver of the symmetric tridiagonal matrix with diagonal h_d (n) and
+ * off-diagonal h_e (n-1): eigenvectors as columns of d_Z (n x n row-major, device), eigenvalues
+ * ascending into h_w (host).  The tridiagonal stage of the large-block path of
+ * pomx_eigh_batched (no separate counterpart in the reference). */
+int pomx_eigh_tridiag(pomx_ctx* ctx, int n, const double* h_d, const double* h_e, double* d_Z, double* h_w);
 
 /* ---- K5 Boltzmann weights ------------------------------------------------------------
  * Replaces DensityMatrixPart::computeUnnormalized/normalize (DensityMatrixPart.cpp:23-31) and

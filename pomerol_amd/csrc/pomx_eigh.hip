@@ -308,36 +308,46 @@ extern "C" int pomx_eigh_batched(pomx_ctx* ctx, int nblocks, const int* h_n, con
         order[b] = b;
     // largest blocks first so the longest workgroups start earliest
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return h_n[a] > h_n[b]; });
-    std::vector<EighJob> jobs(nblocks);
+    std::vector<EighJob> jobs;
+    std::vector<int> large;
     int64_t ws_total = 0;
     int nmax = 1;
     for(int k = 0; k < nblocks; ++k) {
         int b = order[k];
         int n = h_n[b];
         POMX_REQUIRE(ctx, n >= 1, "pomx_eigh_batched: block size must be >= 1");
-        if(n > kMaxLdsN)
-            return pomx_fail(ctx, POMX_ERR_RANGE,
-                             "pomx_eigh_batched: block of size " + std::to_string(n) + " exceeds " +
-                                 std::to_string(kMaxLdsN));
-        jobs[k] = EighJob{n, 0, h_mat_off[b], h_val_off[b], ws_total};
+        if(n >= kEighLargeMin) {
+            large.push_back(b);
+            continue;
+        }
+        jobs.push_back(EighJob{n, 0, h_mat_off[b], h_val_off[b], ws_total});
         ws_total += eigh_ws_doubles(n);
         nmax = std::max(nmax, n);
     }
-    WsBuf<EighJob> d_jobs(ctx, nblocks);
+    // large blocks: the whole device on one block at a time
+    for(int b : large) {
+        int rc = eigh_large(ctx, h_n[b], d_mat + h_mat_off[b], d_eval + h_val_off[b]);
+        if(rc)
+            return rc;
+    }
+    int const nj = int(jobs.size());
+    if(nj == 0)
+        return POMX_OK;
+    WsBuf<EighJob> d_jobs(ctx, nj);
     WsBuf<double> d_ws(ctx, size_t(std::max<int64_t>(ws_total, 1)));
-    WsBuf<int> d_status(ctx, nblocks);
+    WsBuf<int> d_status(ctx, nj);
     if(!d_jobs.p || !d_ws.p || !d_status.p)
         return pomx_fail(ctx, POMX_ERR_HIP, "pomx_eigh_batched: out of device memory");
-    POMX_HIP(ctx, hipMemcpyAsync(d_jobs.p, jobs.data(), sizeof(EighJob) * nblocks, hipMemcpyHostToDevice, ctx->stream));
-    int rc = pomx_fill_zero(ctx, d_status.p, sizeof(int) * nblocks);
+    POMX_HIP(ctx, hipMemcpyAsync(d_jobs.p, jobs.data(), sizeof(EighJob) * nj, hipMemcpyHostToDevice, ctx->stream));
+    int rc = pomx_fill_zero(ctx, d_status.p, sizeof(int) * nj);
     if(rc)
         return rc;
-    if((rc = launch_eigh(ctx, d_jobs, nblocks, nmax, 256, 0, nullptr, d_mat, d_eval, d_ws, d_status)))
+    if((rc = launch_eigh(ctx, d_jobs, nj, nmax, 256, 0, nullptr, d_mat, d_eval, d_ws, d_status)))
         return rc;
-    std::vector<int> st(nblocks);
-    POMX_HIP(ctx, hipMemcpyAsync(st.data(), d_status.p, sizeof(int) * nblocks, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<int> st(nj);
+    POMX_HIP(ctx, hipMemcpyAsync(st.data(), d_status.p, sizeof(int) * nj, hipMemcpyDeviceToHost, ctx->stream));
     POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    for(int k = 0; k < nblocks; ++k)
+    for(int k = 0; k < nj; ++k)
         if(st[k])
             return pomx_fail(ctx, POMX_ERR_RANGE,
                              std::string("pomx_eigh_batched: ") + (st[k] == 1 ? "QL did not converge" : "NaN eigenvalue") +

@@ -13,14 +13,12 @@ namespace {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
+// One 64 x 64 output tile (rows i0.., columns j0..) of job J.
 template <int TA, int TB>
-__global__ void __launch_bounds__(256) gemm_kernel(const GemmTile* __restrict__ tiles, const pomx_gemm_job* __restrict__ jobs,
-                                                   const double* __restrict__ A, const double* __restrict__ B,
-                                                   double* __restrict__ C) {
+__device__ inline void gemm_tile(pomx_gemm_job const& J, int const i0, int const j0, const double* __restrict__ A,
+                                 const double* __restrict__ B, double* __restrict__ C) {
     __shared__ double As[16][64 + 4];
     __shared__ double Bs[16][64 + 4];
-    GemmTile const T = tiles[blockIdx.x];
-    pomx_gemm_job const J = jobs[T.job];
     int const M = J.m, N = J.n, K = J.k;
     const double* __restrict__ a = A + J.a_off;
     const double* __restrict__ b = B + J.b_off;
@@ -50,9 +48,9 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmTile* __restrict__ 
     auto load = [&](int t0, double* ra, double* rb) {
 #pragma unroll
         for(int r = 0; r < 4; ++r) {
-            int t = t0 + ak[r], i = T.i0 + ai[r];
+            int t = t0 + ak[r], i = i0 + ai[r];
             ra[r] = (t < K && i < M) ? (TA == 0 ? a[int64_t(i) * J.lda + t] : a[int64_t(t) * J.lda + i]) : 0.0;
-            int tb = t0 + bk[r], j = T.j0 + bj[r];
+            int tb = t0 + bk[r], j = j0 + bj[r];
             rb[r] = (tb < K && j < N) ? (TB == 0 ? b[int64_t(tb) * J.ldb + j] : b[int64_t(j) * J.ldb + tb]) : 0.0;
         }
     };
@@ -93,14 +91,29 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmTile* __restrict__ 
 #pragma unroll
             for(int r = 0; r < 4; ++r) {
                 // v_mfma_f64_16x16x4f64 result layout: D[i][j] in lane j + 16*(i%4), register i/4
-                int i = T.i0 + wm + mt * 16 + (lane >> 4) + 4 * r;
-                int j = T.j0 + wn + nt * 16 + (lane & 15);
+                int i = i0 + wm + mt * 16 + (lane >> 4) + 4 * r;
+                int j = j0 + wn + nt * 16 + (lane & 15);
                 if(i < M && j < N) {
                     double* p = c + int64_t(i) * J.ldc + j;
                     double v = alpha * acc[mt][nt][r];
                     *p = (beta == 0.0) ? v : fma(beta, *p, v);
                 }
             }
+}
+
+template <int TA, int TB>
+__global__ void __launch_bounds__(256) gemm_kernel(const GemmTile* __restrict__ tiles, const pomx_gemm_job* __restrict__ jobs,
+                                                   const double* __restrict__ A, const double* __restrict__ B,
+                                                   double* __restrict__ C) {
+    GemmTile const T = tiles[blockIdx.x];
+    gemm_tile<TA, TB>(jobs[T.job], T.i0, T.j0, A, B, C);
+}
+
+// One job; the tile is (blockIdx.y, blockIdx.x), so nothing needs to be uploaded per launch.
+template <int TA, int TB>
+__global__ void __launch_bounds__(256) gemm1_kernel(pomx_gemm_job const J, const double* __restrict__ A,
+                                                    const double* __restrict__ B, double* __restrict__ C) {
+    gemm_tile<TA, TB>(J, int(blockIdx.y) * 64, int(blockIdx.x) * 64, A, B, C);
 }
 
 } // namespace
@@ -125,6 +138,24 @@ int launch_gemm(pomx_ctx* ctx, int trans_a, int trans_b, const GemmTile* d_tiles
         gemm_kernel<1, 0><<<g, 256, 0, ctx->stream>>>(d_tiles, d_jobs, d_A, d_B, d_C);
     else
         gemm_kernel<1, 1><<<g, 256, 0, ctx->stream>>>(d_tiles, d_jobs, d_A, d_B, d_C);
+    POMX_CHECK_LAUNCH(ctx);
+    return POMX_OK;
+}
+
+int launch_gemm1(pomx_ctx* ctx, pomx_gemm_job const& J, const double* d_A, const double* d_B, double* d_C) {
+    if(J.m <= 0 || J.n <= 0)
+        return POMX_OK;
+    if((J.m + 63) / 64 > 65535)
+        return pomx_fail(ctx, POMX_ERR_RANGE, "launch_gemm1: more than 65535 row tiles");
+    dim3 g(unsigned((J.n + 63) / 64), unsigned((J.m + 63) / 64));
+    if(!J.trans_a && !J.trans_b)
+        gemm1_kernel<0, 0><<<g, 256, 0, ctx->stream>>>(J, d_A, d_B, d_C);
+    else if(!J.trans_a && J.trans_b)
+        gemm1_kernel<0, 1><<<g, 256, 0, ctx->stream>>>(J, d_A, d_B, d_C);
+    else if(J.trans_a && !J.trans_b)
+        gemm1_kernel<1, 0><<<g, 256, 0, ctx->stream>>>(J, d_A, d_B, d_C);
+    else
+        gemm1_kernel<1, 1><<<g, 256, 0, ctx->stream>>>(J, d_A, d_B, d_C);
     POMX_CHECK_LAUNCH(ctx);
     return POMX_OK;
 }

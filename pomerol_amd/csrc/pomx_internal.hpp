@@ -122,3 +122,13 @@ bool gemm_job_ok(pomx_gemm_job const& j);
 // Launch the batched MFMA GEMM over device-resident tiles whose jobs share (trans_a, trans_b).
 int launch_gemm(pomx_ctx* ctx, int trans_a, int trans_b, const GemmTile* d_tiles, int ntiles,
                 const pomx_gemm_job* d_jobs, const double* d_A, const double* d_B, double* d_C);
+// Launch one GEMM job (tile from the block index, no device descriptors).
+int launch_gemm1(pomx_ctx* ctx, pomx_gemm_job const& J, const double* d_A, const double* d_B, double* d_C);
+
+// Dense symmetric eigensolver for one large block (whole device): blocked Householder
+// tridiagonalisation with MFMA trailing updates, divide-and-conquer on the tridiagonal matrix,
+// blocked (WY) back-transformation on MFMA.  d_A (n x n row-major) is overwritten by the
+// eigenvectors as columns; eigenvalues ascending into d_w.
+int eigh_large(pomx_ctx* ctx, int n, double* d_A, double* d_w);
+// Blocks larger than this go to eigh_large in pomx_eigh_batched.
+constexpr int kEighLargeMin = 513;
