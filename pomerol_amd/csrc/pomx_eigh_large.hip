@@ -54,44 +54,94 @@ __device__ inline double block_sum(double v, double* red) {
 }
 
 // ---- 1. tridiagonalisation ---------------------------------------------------------------
+//
+// Panel state: V = Vall[:, j0:j0+kNB] (reflectors, unit entry stored), Wp[:, p] = w_p without
+// its last term, a2[p] = -tau/2 (w_p . v_p); the true w_p is Wp[:, p] + a2[p] V[:, p] (folded
+// in by every reader, and into Wp once per panel before the trailing update).  Device-wide
+// reductions finish in the last workgroup to arrive (counter ticket, device-scope fences), so
+// a reflector costs three launches and no single-workgroup pass over the matrix.
 
-// One workgroup: bring row j up to date with the panel's i previous reflectors
-// (A[j, j:] -= V[j,:] W[j:,:]^T + W[j,:] V[j:,:]^T), then form its Householder reflector.
-__global__ void __launch_bounds__(1024) trd_row_kernel(int n, int j, int j0, int i, double* __restrict__ A,
-                                                       double* __restrict__ Vall, const double* __restrict__ Wp,
-                                                       double* __restrict__ vbuf, double* __restrict__ d,
-                                                       double* __restrict__ e, double* __restrict__ tau) {
-    __shared__ double red[16];
-    __shared__ double vj[kNB], wj[kNB];
-    int const tid = threadIdx.x, nt = blockDim.x;
+struct TrdScal {
+    double scale;   // v = x * scale (x = updated row tail), v[0] = 1
+    double t12[64]; // t1 = W~^T v (p < 32), t2 = V^T v (32 + p)
+};
+
+// Arrive at the ticket counter; true in the last workgroup (which then sees every other
+// workgroup's stores) and the counter is rearmed for the next launch.
+__device__ inline bool last_block(unsigned* counter) {
+    __shared__ bool last;
+    __threadfence();
+    __syncthreads();
+    if(threadIdx.x == 0) {
+        unsigned t = atomicAdd(counter, 1u);
+        last = (t == gridDim.x - 1);
+        if(last)
+            atomicExch(counter, 0u);
+    }
+    __syncthreads();
+    if(last)
+        __threadfence();
+    return last;
+}
+
+// Row j brought up to date with the panel's first i reflectors (columns across the grid);
+// the last workgroup forms the reflector of the row tail and writes v (vbuf and Vall column j).
+__global__ void __launch_bounds__(256) trd_row_kernel(int n, int j, int j0, int i, double* __restrict__ A,
+                                                      double* __restrict__ Vall, const double* __restrict__ Wp,
+                                                      const double* __restrict__ a2, double* __restrict__ vbuf,
+                                                      double* __restrict__ d, double* __restrict__ e,
+                                                      double* __restrict__ tau, double* __restrict__ rowpart,
+                                                      unsigned* __restrict__ counter) {
+    __shared__ double red[4];
+    __shared__ double vj[kNB], wj[kNB], av[kNB];
+    int const tid = threadIdx.x;
     if(tid < i) {
-        vj[tid] = Vall[int64_t(j) * n + j0 + tid];
-        wj[tid] = Wp[int64_t(j) * kNB + tid];
+        double const v = Vall[int64_t(j) * n + j0 + tid];
+        vj[tid] = v;
+        av[tid] = a2[tid];
+        wj[tid] = Wp[int64_t(j) * kNB + tid] + a2[tid] * v;
     }
     __syncthreads();
     double* __restrict__ row = A + int64_t(j) * n;
-    for(int c = j + tid; c < n; c += nt) {
+    double s = 0.0;
+    for(int c = j + blockIdx.x * 256 + tid; c < n; c += gridDim.x * 256) {
         double acc = row[c];
         const double* wc = Wp + int64_t(c) * kNB;
         const double* vc = Vall + int64_t(c) * n + j0;
-        for(int p = 0; p < i; ++p)
-            acc -= vj[p] * wc[p] + wj[p] * vc[p];
+        for(int p = 0; p < i; ++p) {
+            double const v = vc[p];
+            acc -= vj[p] * fma(av[p], v, wc[p]) + wj[p] * v;
+        }
         row[c] = acc;
+        if(c >= j + 2)
+            s = fma(acc, acc, s);
     }
+    s = wave_sum(s);
+    if((tid & 63) == 0)
+        red[tid >> 6] = s;
     __syncthreads();
+    if(tid == 0)
+        rowpart[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    if(!last_block(counter))
+        return;
+    double sum = 0.0;
+    for(int b = tid; b < int(gridDim.x); b += 256)
+        sum += rowpart[b];
+    sum = wave_sum(sum);
+    __syncthreads();
+    if((tid & 63) == 0)
+        red[tid >> 6] = sum;
+    __syncthreads();
+    double const ss = red[0] + red[1] + red[2] + red[3];
     int const m = n - j - 1;
-    double s = 0.0;
-    for(int c = j + 2 + tid; c < n; c += nt)
-        s += row[c] * row[c];
-    s = block_sum(s, red);
     double const alpha = row[j + 1];
     double t = 0.0, beta = alpha, scale = 0.0;
-    if(s > 0.0) {
-        beta = -copysign(sqrt(alpha * alpha + s), alpha);
+    if(ss > 0.0) {
+        beta = -copysign(sqrt(alpha * alpha + ss), alpha);
         t = (beta - alpha) / beta;
         scale = 1.0 / (alpha - beta);
     }
-    for(int c = tid; c < m; c += nt) {
+    for(int c = tid; c < m; c += 256) {
         double v = (c == 0) ? 1.0 : (t != 0.0 ? row[j + 1 + c] * scale : 0.0);
         vbuf[c] = v;
         Vall[int64_t(j + 1 + c) * n + j] = v;
@@ -103,34 +153,38 @@ __global__ void __launch_bounds__(1024) trd_row_kernel(int n, int j, int j0, int
     }
 }
 
-// w_raw = A22 v (A22 = rows/columns j+1.. at the start of the panel), one wavefront per row, and
-// per-workgroup partial sums of W^T v and V^T v over a contiguous chunk of rows.
+// w_raw = A22 v (A22 = rows/columns j+1.. at the start of the panel), one wavefront per row,
+// and partial sums of Wp^T v, V^T v over a chunk of rows; the last workgroup reduces them to
+// t1 = W~^T v = Wp^T v + a2 o (V^T v) and t2 = V^T v.
 __global__ void __launch_bounds__(256) trd_symv_kernel(int n, int j, int j0, int i, const double* __restrict__ A,
                                                        const double* __restrict__ vbuf, const double* __restrict__ Vall,
-                                                       const double* __restrict__ Wp, double* __restrict__ wraw,
-                                                       double* __restrict__ part) {
+                                                       const double* __restrict__ Wp, const double* __restrict__ a2,
+                                                       double* __restrict__ wraw, double* __restrict__ part,
+                                                       TrdScal* __restrict__ sc, unsigned* __restrict__ counter) {
     __shared__ double red[4][64];
     int const m = n - j - 1, base = j + 1;
     int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for(int r = blockIdx.x * 4 + wave; r < m; r += gridDim.x * 4) {
         const double* __restrict__ a = A + int64_t(base + r) * n + base;
-        double s0 = 0.0, s1 = 0.0;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
         int c = lane;
-        for(; c + 64 < m; c += 128) {
+        for(; c + 192 < m; c += 256) {
             s0 = fma(a[c], vbuf[c], s0);
             s1 = fma(a[c + 64], vbuf[c + 64], s1);
+            s2 = fma(a[c + 128], vbuf[c + 128], s2);
+            s3 = fma(a[c + 192], vbuf[c + 192], s3);
         }
-        if(c < m)
+        for(; c < m; c += 64)
             s0 = fma(a[c], vbuf[c], s0);
-        double s = wave_sum(s0 + s1);
+        double s = wave_sum((s0 + s1) + (s2 + s3));
         if(lane == 0)
             wraw[r] = s;
     }
     if(i == 0)
-        return;
+        return; // uniform over the grid: no reduction needed
     int const chunk = (m + gridDim.x - 1) / gridDim.x;
     int const r0 = blockIdx.x * chunk, r1 = min(m, r0 + chunk);
-    int const p = lane; // p < 32: (W^T v)_p, 32 <= p < 64: (V^T v)_{p-32}
+    int const p = lane; // p < 32: (Wp^T v)_p, 32 <= p < 64: (V^T v)_{p-32}
     double acc = 0.0;
     if((p & 31) < i)
         for(int r = r0 + wave; r < r1; r += 4) {
@@ -141,42 +195,85 @@ __global__ void __launch_bounds__(256) trd_symv_kernel(int n, int j, int j0, int
     __syncthreads();
     if(wave == 0)
         part[int64_t(blockIdx.x) * 64 + lane] = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
+    if(!last_block(counter))
+        return;
+    // 256 threads: 4 groups of 64 sum strided workgroups, then combine
+    double t = 0.0;
+    for(int b = wave; b < int(gridDim.x); b += 4)
+        t += part[int64_t(b) * 64 + lane];
+    __syncthreads();
+    red[wave][lane] = t;
+    __syncthreads();
+    if(wave == 0) {
+        double v = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
+        red[0][lane] = v;
+    }
+    __syncthreads();
+    if(threadIdx.x < 32) {
+        int const q = threadIdx.x;
+        double const t2 = red[0][32 + q];
+        sc->t12[q] = (q < i) ? red[0][q] + a2[q] * t2 : 0.0;
+        sc->t12[32 + q] = (q < i) ? t2 : 0.0;
+    }
 }
 
-// One workgroup: w = tau (w_raw - V t1 - W t2) with t1 = W^T v, t2 = V^T v (reduced from the
-// partials), then w += -tau/2 (w.v) v, stored as column i of the panel's W.
-__global__ void __launch_bounds__(1024) trd_finish_kernel(int n, int j, int j0, int i, int nparts,
-                                                          const double* __restrict__ part, const double* __restrict__ wraw,
-                                                          const double* __restrict__ vbuf, const double* __restrict__ Vall,
-                                                          double* __restrict__ Wp, const double* __restrict__ tau) {
-    __shared__ double t12[64];
-    __shared__ double red[16];
-    int const tid = threadIdx.x, nt = blockDim.x;
+// w = tau (w_raw - V t1 - W~ t2) stored (without its last term) as column i of Wp; the last
+// workgroup finishes a2[i] = -tau/2 (w . v).
+__global__ void __launch_bounds__(256) trd_w_kernel(int n, int j, int j0, int i, const double* __restrict__ wraw,
+                                                    const double* __restrict__ vbuf, const double* __restrict__ Vall,
+                                                    double* __restrict__ Wp, double* __restrict__ a2,
+                                                    const double* __restrict__ tau, const TrdScal* __restrict__ sc,
+                                                    double* __restrict__ wpart, unsigned* __restrict__ counter) {
+    __shared__ double ca[kNB], cb[kNB];
+    __shared__ double red[4];
+    int const tid = threadIdx.x;
     int const m = n - j - 1, base = j + 1;
     double const tj = tau[j];
-    if(tid < 64) {
-        double s = 0.0;
-        if((tid & 31) < i)
-            for(int b = 0; b < nparts; ++b)
-                s += part[int64_t(b) * 64 + tid];
-        t12[tid] = s;
+    if(tid < i) {
+        double const t1 = sc->t12[tid], t2 = sc->t12[32 + tid];
+        ca[tid] = t1 + a2[tid] * t2; // coefficient of V[r][p] (t1 plus the deferred a2 V t2)
+        cb[tid] = t2;                // coefficient of Wp[r][p]
     }
     __syncthreads();
     double dot = 0.0;
-    for(int r = tid; r < m; r += nt) {
+    for(int r = blockIdx.x * 256 + tid; r < m; r += gridDim.x * 256) {
         double w = wraw[r];
         const double* vr = Vall + int64_t(base + r) * n + j0;
         const double* wr = Wp + int64_t(base + r) * kNB;
         for(int p = 0; p < i; ++p)
-            w -= vr[p] * t12[p] + wr[p] * t12[32 + p];
+            w -= vr[p] * ca[p] + wr[p] * cb[p];
         w *= tj;
         Wp[int64_t(base + r) * kNB + i] = w;
         dot = fma(w, vbuf[r], dot);
     }
-    dot = block_sum(dot, red); // the barrier inside also orders the stores above for this workgroup
-    double const a2 = -0.5 * tj * dot;
-    for(int r = tid; r < m; r += nt)
-        Wp[int64_t(base + r) * kNB + i] += a2 * vbuf[r];
+    dot = wave_sum(dot);
+    if((tid & 63) == 0)
+        red[tid >> 6] = dot;
+    __syncthreads();
+    if(tid == 0)
+        wpart[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    if(!last_block(counter))
+        return;
+    double sum = 0.0;
+    for(int b = tid; b < int(gridDim.x); b += 256)
+        sum += wpart[b];
+    sum = wave_sum(sum);
+    __syncthreads();
+    if((tid & 63) == 0)
+        red[tid >> 6] = sum;
+    __syncthreads();
+    if(tid == 0)
+        a2[i] = -0.5 * tj * (red[0] + red[1] + red[2] + red[3]);
+}
+
+// End of a panel: Wp[r][p] += a2[p] V[r][p] for the rows of the trailing update.
+__global__ void trd_fold_kernel(int n, int j0, int nb, int r0, const double* __restrict__ Vall, double* __restrict__ Wp,
+                                const double* __restrict__ a2) {
+    int64_t const cnt = int64_t(n - r0) * nb;
+    for(int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < cnt; q += int64_t(gridDim.x) * blockDim.x) {
+        int const r = r0 + int(q / nb), p = int(q % nb);
+        Wp[int64_t(r) * kNB + p] += a2[p] * Vall[int64_t(r) * n + j0 + p];
+    }
 }
 
 __global__ void copy_diag_kernel(int n, const double* __restrict__ A, double* __restrict__ d) {
@@ -244,8 +341,10 @@ __global__ void dc_rot_kernel(const DcMerge* __restrict__ mg, const DcRot* __res
 
 // Root j of 1 + rho sum_i z_i^2/(d_i - lambda) = 0 (d strictly increasing, k >= 1), as
 // lambda = d[org] + tau with org the nearer pole, by safeguarded Newton inside a bracket.
-__device__ void secular_root(int k, const double* __restrict__ d, const double* __restrict__ z, double rho, int j,
-                             int* org, double* tau_out) {
+// One wavefront per root: the lanes split the sums, every lane runs the same iteration.
+__device__ void secular_root_wave(int k, const double* __restrict__ d, const double* __restrict__ z, double rho, int j,
+                                  int* org, double* tau_out) {
+    int const lane = threadIdx.x & 63;
     if(k == 1) {
         *org = 0;
         *tau_out = rho * z[0] * z[0];
@@ -255,9 +354,10 @@ __device__ void secular_root(int k, const double* __restrict__ d, const double* 
     double lo, hi;
     if(j < k - 1) {
         double const mid = 0.5 * (d[j + 1] - d[j]);
-        double f = 1.0;
-        for(int i = 0; i < k; ++i)
-            f += rho * z[i] * z[i] / ((d[i] - d[j]) - mid);
+        double f = 0.0;
+        for(int i = lane; i < k; i += 64)
+            f += z[i] * z[i] / ((d[i] - d[j]) - mid);
+        f = 1.0 + rho * wave_sum(f);
         if(f >= 0.0) {
             o = j;
             lo = 0.0;
@@ -269,22 +369,24 @@ __device__ void secular_root(int k, const double* __restrict__ d, const double* 
         }
     } else {
         double zz = 0.0;
-        for(int i = 0; i < k; ++i)
+        for(int i = lane; i < k; i += 64)
             zz += z[i] * z[i];
         o = k - 1;
         lo = 0.0;
-        hi = rho * zz;
+        hi = rho * wave_sum(zz);
     }
     double const dorg = d[o];
     double tau = 0.5 * (lo + hi);
     for(int it = 0; it < 400; ++it) {
-        double g = 1.0, gp = 0.0;
-        for(int i = 0; i < k; ++i) {
+        double g = 0.0, gp = 0.0;
+        for(int i = lane; i < k; i += 64) {
             double const r = (d[i] - dorg) - tau;
             double const q = z[i] / r;
-            g = fma(rho * z[i], q, g); // rho z_i^2 / r
-            gp = fma(rho * q, q, gp);  // rho z_i^2 / r^2
+            g = fma(z[i], q, g); // z_i^2 / r
+            gp = fma(q, q, gp);  // z_i^2 / r^2
         }
+        g = 1.0 + rho * wave_sum(g);
+        gp = rho * wave_sum(gp);
         if(g == 0.0)
             break;
         if(g < 0.0)
@@ -304,15 +406,19 @@ __device__ void secular_root(int k, const double* __restrict__ d, const double* 
     *tau_out = tau;
 }
 
-__global__ void dc_secular_kernel(int nm, const DcMerge* __restrict__ mg, const double* __restrict__ dl,
-                                  const double* __restrict__ zl, int* __restrict__ org, double* __restrict__ tau) {
+__global__ void __launch_bounds__(256) dc_secular_kernel(int nm, const DcMerge* __restrict__ mg, const double* __restrict__ dl,
+                                                         const double* __restrict__ zl, int* __restrict__ org,
+                                                         double* __restrict__ tau) {
     DcMerge const M = mg[blockIdx.y];
-    for(int jj = blockIdx.x * blockDim.x + threadIdx.x; jj < M.k; jj += gridDim.x * blockDim.x) {
+    int const wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for(int jj = blockIdx.x * 4 + wave; jj < M.k; jj += gridDim.x * 4) {
         int o;
         double t;
-        secular_root(M.k, dl + M.sec_off, zl + M.sec_off, M.rho, jj, &o, &t);
-        org[M.sec_off + jj] = o;
-        tau[M.sec_off + jj] = t;
+        secular_root_wave(M.k, dl + M.sec_off, zl + M.sec_off, M.rho, jj, &o, &t);
+        if(lane == 0) {
+            org[M.sec_off + jj] = o;
+            tau[M.sec_off + jj] = t;
+        }
     }
 }
 
@@ -694,7 +800,7 @@ int dc_tridiag(pomx_ctx* ctx, int N, std::vector<double> d, std::vector<double> 
         dc_rot_kernel<<<dim3(blocks_for(maxn, 256, 64), nm), 256, 0, s>>>(d_mg, d_rot, d_Q2, N);
         dc_colgather_kernel<<<dim3(gsq, nm), 256, 0, s>>>(d_mg, d_perm, 1, d_Q2, d_Z, N);
         if(maxk > 0) {
-            dc_secular_kernel<<<dim3(blocks_for(maxk, 64, 1024), nm), 64, 0, s>>>(int(nm), d_mg, d_dl, d_zl, d_org, d_tau);
+            dc_secular_kernel<<<dim3(blocks_for(maxk, 4, 4096), nm), 256, 0, s>>>(int(nm), d_mg, d_dl, d_zl, d_org, d_tau);
             dc_zhat_kernel<<<dim3(blocks_for(maxk, 64, 1024), nm), 64, 0, s>>>(d_mg, d_dl, d_zl, d_org, d_tau, d_zh);
             dc_u_kernel<<<dim3(blocks_for(int64_t(maxk) * maxk, 256, 2048), nm), 256, 0, s>>>(d_mg, d_dl, d_zh, d_org,
                                                                                              d_tau, d_U);
@@ -757,33 +863,42 @@ int eigh_large(pomx_ctx* ctx, int n, double* d_A, double* d_w) {
     int64_t const nn = int64_t(n) * n;
     WsBuf<double> d_V(ctx, size_t(nn)), d_Z(ctx, size_t(nn)), d_Q2(ctx, size_t(nn)), d_U(ctx, size_t(nn));
     WsBuf<double> d_W(ctx, size_t(n) * kNB), d_v(ctx, size_t(n)), d_wraw(ctx, size_t(n)), d_part(ctx, size_t(kSymvGrid) * 64);
+    WsBuf<double> d_a2(ctx, kNB), d_rowpart(ctx, 256), d_wpart(ctx, 256);
+    WsBuf<TrdScal> d_sc(ctx, 1);
+    WsBuf<unsigned> d_cnt(ctx, 4);
     WsBuf<double> d_d(ctx, size_t(n)), d_e(ctx, size_t(n)), d_tau(ctx, size_t(n));
     WsBuf<double> d_S(ctx, size_t(kNB) * kNB), d_T(ctx, size_t(kNB) * kNB), d_W1(ctx, size_t(kNB) * n),
         d_W2(ctx, size_t(kNB) * n);
     for(void* p : {(void*)d_V.p, (void*)d_Z.p, (void*)d_Q2.p, (void*)d_U.p, (void*)d_W.p, (void*)d_v.p, (void*)d_wraw.p,
                    (void*)d_part.p, (void*)d_d.p, (void*)d_e.p, (void*)d_tau.p, (void*)d_S.p, (void*)d_T.p, (void*)d_W1.p,
-                   (void*)d_W2.p})
+                   (void*)d_W2.p, (void*)d_a2.p, (void*)d_rowpart.p, (void*)d_wpart.p, (void*)d_sc.p, (void*)d_cnt.p})
         if(!p)
             return pomx_fail(ctx, POMX_ERR_HIP, "eigh_large: out of device memory for a block of size " + std::to_string(n));
     fill_kernel<<<blocks_for(nn), 256, 0, s>>>(d_V, nn, 0.0);
     fill_kernel<<<blocks_for(int64_t(n) * kNB), 256, 0, s>>>(d_W, int64_t(n) * kNB, 0.0);
     fill_kernel<<<blocks_for(n), 256, 0, s>>>(d_e, n, 0.0);
     fill_kernel<<<blocks_for(n), 256, 0, s>>>(d_tau, n, 0.0);
+    fill_kernel<<<1, 64, 0, s>>>(d_a2, kNB, 0.0);
     POMX_CHECK_LAUNCH(ctx);
     int rc;
+    if((rc = pomx_fill_zero(ctx, d_cnt.p, 4 * sizeof(unsigned))))
+        return rc;
     // 1. tridiagonalisation
     for(int j0 = 0; j0 < n - 1; j0 += kNB) {
         int const nb = std::min(kNB, n - 1 - j0);
         for(int i = 0; i < nb; ++i) {
             int const j = j0 + i, m = n - j - 1;
-            trd_row_kernel<<<1, 1024, 0, s>>>(n, j, j0, i, d_A, d_V, d_W, d_v, d_d, d_e, d_tau);
-            int const grid = std::max(1, std::min(kSymvGrid, (m + 15) / 16));
-            trd_symv_kernel<<<grid, 256, 0, s>>>(n, j, j0, i, d_A, d_v, d_V, d_W, d_wraw, d_part);
-            trd_finish_kernel<<<1, 1024, 0, s>>>(n, j, j0, i, grid, d_part, d_wraw, d_v, d_V, d_W, d_tau);
+            int const g1 = std::max(1, std::min(256, (n - j + 255) / 256));
+            trd_row_kernel<<<g1, 256, 0, s>>>(n, j, j0, i, d_A, d_V, d_W, d_a2, d_v, d_d, d_e, d_tau, d_rowpart, d_cnt.p);
+            int const g2 = std::max(1, std::min(kSymvGrid, (m + 15) / 16));
+            trd_symv_kernel<<<g2, 256, 0, s>>>(n, j, j0, i, d_A, d_v, d_V, d_W, d_a2, d_wraw, d_part, d_sc.p, d_cnt.p + 1);
+            int const g3 = std::max(1, std::min(256, (m + 255) / 256));
+            trd_w_kernel<<<g3, 256, 0, s>>>(n, j, j0, i, d_wraw, d_v, d_V, d_W, d_a2, d_tau, d_sc.p, d_wpart, d_cnt.p + 2);
         }
         POMX_CHECK_LAUNCH(ctx);
         int const j1 = j0 + nb, mt = n - j1;
         if(mt > 0) { // A22 -= V2 W2^T + W2 V2^T
+            trd_fold_kernel<<<blocks_for(int64_t(mt) * nb), 256, 0, s>>>(n, j0, nb, j1, d_V, d_W, d_a2);
             pomx_gemm_job g1{mt, mt, nb, 0, 1, n, kNB, n, int64_t(j1) * n + j0, int64_t(j1) * kNB, int64_t(j1) * n + j1,
                              -1.0, 1.0};
             pomx_gemm_job g2{mt, mt, nb, 0, 1, kNB, n, n, int64_t(j1) * kNB, int64_t(j1) * n + j0, int64_t(j1) * n + j1,
