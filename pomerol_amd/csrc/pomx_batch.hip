@@ -217,6 +217,11 @@ int enqueue_solve(pomx_ctx* ctx, pomx_batch* b) {
         if(rc)
             return rc;
     }
+    if(b->big) {
+        int rc = big_plan_enqueue(ctx, b->big, b->mat, b->eval);
+        if(rc)
+            return rc;
+    }
     inst_weights_kernel<<<b->ninst, 256, 0, s>>>(b->inst_off, b->inst_cnt, b->beta, b->eval, b->w, b->E0);
     int rc = launch_rotate(ctx, b->tiles, b->ntiles, b->jobs, b->mat, b->src, b->ops, 1e-20);
     if(rc)
@@ -261,13 +266,15 @@ extern "C" int pomx_batch_create(pomx_ctx* ctx, const pomx_batch_desc* d, pomx_b
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return d->block_n[x] > d->block_n[y]; });
     std::vector<EighJob> ej;
     int64_t ws = 0;
-    // size buckets; inside a bucket the jobs keep the largest-first order
+    // size buckets of the workgroup eigensolver; inside a bucket the jobs keep the largest-first
+    // order.  Blocks of size >= kEighLargeMin go to the device-driven large-block solver.
     std::vector<std::vector<int>> byb(kBuckets);
+    std::vector<int> big;
     for(int x : order) {
         int n = d->block_n[x];
-        if(n > eigh_max_n()) {
-            delete b;
-            return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_batch_create: block larger than the eigensolver limit");
+        if(n >= kEighLargeMin) {
+            big.push_back(x);
+            continue;
         }
         int q = 0;
         while(n > kBucketMax[q])
@@ -287,6 +294,20 @@ extern "C" int pomx_batch_create(pomx_ctx* ctx, const pomx_batch_desc* d, pomx_b
     }
     b->bucket_begin[kBuckets] = at;
     b->ws_len = ws;
+    // large blocks follow the bucketed ones in the job list (the adjoint walks all blocks)
+    std::vector<int> big_n;
+    std::vector<int64_t> big_mat, big_val;
+    for(int x : big) {
+        ej.push_back(EighJob{d->block_n[x], 0, d->block_mat_off[x], d->block_state_off[x], 0});
+        big_n.push_back(d->block_n[x]);
+        big_mat.push_back(d->block_mat_off[x]);
+        big_val.push_back(d->block_state_off[x]);
+    }
+    b->nbig = int(big.size());
+    if(!big.empty() && (rc = big_plan_create(ctx, b->nbig, big_n.data(), big_mat.data(), big_val.data(), &b->big))) {
+        delete b;
+        return rc;
+    }
     // rotation tiles
     std::vector<RotTile> tiles;
     for(int q = 0; q < d->njobs; ++q)
@@ -388,27 +409,38 @@ extern "C" int pomx_batch_read(pomx_ctx* ctx, pomx_batch* b, double* h_G, double
     for(int v : st)
         if(v)
             return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_batch: eigensolver did not converge for a block");
-    return POMX_OK;
+    return big_plan_status(ctx, b->big);
 }
 
 extern "C" int pomx_batch_tune(pomx_ctx* ctx, pomx_batch* b, char* report, size_t report_len) {
     POMX_REQUIRE(ctx, b, "pomx_batch_tune: null batch");
     drop_graph(b);
+    hipStream_t s = ctx->stream;
     hipEvent_t e0 = ctx->ev0, e1 = ctx->ev1;
-    auto time_solve = [&](float& ms) -> int {
-        int rc = enqueue_solve(ctx, b); // warm
-        if(rc)
-            return rc;
-        POMX_HIP(ctx, hipEventRecord(e0, ctx->stream));
-        if((rc = enqueue_solve(ctx, b)))
-            return rc;
-        POMX_HIP(ctx, hipEventRecord(e1, ctx->stream));
+    int rc = enqueue_solve(ctx, b); // valid spectrum, rotated operators and weights for the G timing
+    if(rc)
+        return rc;
+    // one bucket's eigensolve on freshly assembled matrices, timed alone
+    auto time_bucket = [&](int q, float& ms) -> int {
+        int const nj = b->bucket_begin[q + 1] - b->bucket_begin[q];
+        for(int rep = 0; rep < 2; ++rep) {
+            zero_kernel<<<unsigned(std::min<int64_t>((b->mat_len + 255) / 256, 65536)), 256, 0, s>>>(b->mat, b->mat_len);
+            if(b->nnz)
+                scatter_h_kernel<<<grid_for(b->nnz), kBlock, 0, s>>>(b->nnz, b->h_pos, b->h_val, b->mat);
+            zero_int_kernel<<<grid_for(kBuckets + b->nblocks), kBlock, 0, s>>>(b->counters, kBuckets + b->nblocks);
+            POMX_HIP(ctx, hipEventRecord(e0, s));
+            int r = launch_eigh(ctx, b->eigh_jobs + b->bucket_begin[q], nj, b->bucket_nmax[q], b->eigh_threads[q],
+                                std::min(nj, b->eigh_grid), b->counters + q, b->mat, b->eval, b->ws,
+                                b->status + b->bucket_begin[q]);
+            if(r)
+                return r;
+            POMX_HIP(ctx, hipEventRecord(e1, s));
+        }
         POMX_HIP(ctx, hipEventSynchronize(e1));
         POMX_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
         return POMX_OK;
     };
     std::string rep;
-    // eigensolver workgroup size per bucket (coordinate search, largest bucket first)
     int const cand[3] = {128, 256, 512};
     for(int q = kBuckets - 1; q >= 0; --q) {
         if(b->bucket_begin[q + 1] == b->bucket_begin[q])
@@ -420,8 +452,7 @@ extern "C" int pomx_batch_tune(pomx_ctx* ctx, pomx_batch* b, char* report, size_
                 continue;
             b->eigh_threads[q] = c;
             float ms = 0;
-            int rc = time_solve(ms);
-            if(rc)
+            if((rc = time_bucket(q, ms)))
                 return rc;
             if(ms < best) {
                 best = ms;
@@ -431,28 +462,40 @@ extern "C" int pomx_batch_tune(pomx_ctx* ctx, pomx_batch* b, char* report, size_
         b->eigh_threads[q] = bt;
         rep += "eigh[n<=" + std::to_string(std::min(kBucketMax[q], b->bucket_nmax[q])) + "]=" + std::to_string(bt) + " ";
     }
-    // G work-item size
+    if((rc = enqueue_solve(ctx, b))) // restore a consistent state with the chosen sizes
+        return rc;
+    // G work-item size: the fused kernel and its reduction, timed alone
     int64_t const gc[3] = {1024, 4096, 16384};
     float best = FLT_MAX;
     int64_t bg = b->terms_per_item;
     for(int64_t c : gc) {
         b->terms_per_item = c;
-        int rc = build_items(ctx, b);
-        if(rc)
+        if((rc = build_items(ctx, b)))
             return rc;
         float ms = 0;
-        if((rc = time_solve(ms)))
-            return rc;
+        for(int r = 0; r < 2; ++r) {
+            POMX_HIP(ctx, hipEventRecord(e0, s));
+            if(b->nitems)
+                fused_g_kernel<<<b->nitems, kGThreads, 0, s>>>(b->items, b->jinfo, b->ops, b->eval, b->w, b->beta, b->nfreq,
+                                                               b->tol, b->partial);
+            g_reduce_kernel<<<grid_for(int64_t(b->ninst) * b->nfreq), kBlock, 0, s>>>(b->ninst, b->nfreq, b->item_off,
+                                                                                        b->partial, b->G);
+            POMX_CHECK_LAUNCH(ctx);
+            POMX_HIP(ctx, hipEventRecord(e1, s));
+        }
+        POMX_HIP(ctx, hipEventSynchronize(e1));
+        POMX_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
         if(ms < best) {
             best = ms;
             bg = c;
         }
     }
     b->terms_per_item = bg;
-    int rc = build_items(ctx, b);
-    if(rc)
+    if((rc = build_items(ctx, b)))
         return rc;
-    rep += "g_terms_per_item=" + std::to_string(bg) + " solve_ms=" + std::to_string(best);
+    rep += "g_terms_per_item=" + std::to_string(bg) + " g_ms=" + std::to_string(best);
+    if(b->nbig)
+        rep += " large_blocks=" + std::to_string(b->nbig);
     if(report && report_len) {
         std::strncpy(report, rep.c_str(), report_len - 1);
         report[report_len - 1] = 0;
@@ -467,6 +510,7 @@ extern "C" void pomx_batch_destroy(pomx_ctx* ctx, pomx_batch* b) {
         (void)hipStreamSynchronize(ctx->stream);
     drop_graph(b);
     adjoint_free(b);
+    big_plan_destroy(b->big);
     free_batch(b);
     delete b;
 }

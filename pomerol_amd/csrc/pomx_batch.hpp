@@ -51,6 +51,8 @@ struct pomx_batch {
     std::vector<int> h_src;
     std::vector<int64_t> h_pos_host;      // Hamiltonian positions (matrix arena)
     AdjointState* adj = nullptr;          // adjoint buffers, allocated on first use
+    BigPlan* big = nullptr;               // blocks of size >= kEighLargeMin (device-driven solver)
+    int nbig = 0;
     int bucket_begin[kBuckets + 1] = {0, 0, 0, 0, 0}; // ranges of eigh_jobs (sorted by size, descending)
     int bucket_nmax[kBuckets] = {1, 1, 1, 1};
     // tuning

@@ -324,11 +324,38 @@ extern "C" int pomx_eigh_batched(pomx_ctx* ctx, int nblocks, const int* h_n, con
         ws_total += eigh_ws_doubles(n);
         nmax = std::max(nmax, n);
     }
-    // large blocks: the whole device on one block at a time
-    for(int b : large) {
-        int rc = eigh_large(ctx, h_n[b], d_mat + h_mat_off[b], d_eval + h_val_off[b]);
+    // large blocks: device-driven solver on all of them together, in groups whose scratch fits
+    // a fixed budget (the largest blocks first)
+    size_t const budget = size_t(64) << 30;
+    for(size_t g0 = 0; g0 < large.size();) {
+        size_t g1 = g0;
+        size_t bytes = 0;
+        std::vector<int> gn;
+        std::vector<int64_t> gm, gv;
+        while(g1 < large.size()) {
+            size_t const nb = size_t(h_n[large[g1]]);
+            size_t const need = sizeof(double) * (4 * nb * nb + 128 * nb + 80000);
+            if(!gn.empty() && bytes + need > budget)
+                break;
+            bytes += need;
+            gn.push_back(h_n[large[g1]]);
+            gm.push_back(h_mat_off[large[g1]]);
+            gv.push_back(h_val_off[large[g1]]);
+            ++g1;
+        }
+        BigPlan* plan = nullptr;
+        int rc = big_plan_create(ctx, int(gn.size()), gn.data(), gm.data(), gv.data(), &plan);
         if(rc)
             return rc;
+        rc = big_plan_enqueue(ctx, plan, d_mat, d_eval);
+        if(!rc && hipStreamSynchronize(ctx->stream) != hipSuccess)
+            rc = pomx_fail(ctx, POMX_ERR_HIP, "pomx_eigh_batched: large-block solve failed");
+        if(!rc)
+            rc = big_plan_status(ctx, plan);
+        big_plan_destroy(plan);
+        if(rc)
+            return rc;
+        g0 = g1;
     }
     int const nj = int(jobs.size());
     if(nj == 0)

@@ -109,6 +109,22 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmTile* __restrict__ 
     gemm_tile<TA, TB>(jobs[T.job], T.i0, T.j0, A, B, C);
 }
 
+// Job blockIdx.y, tile blockIdx.x (row-major over the job's 64 x 64 tiles).  Tiles outside the
+// job exit at once, so the launch can be sized for the largest job and the job sizes may be
+// written on the device by an earlier kernel.
+template <int TA, int TB>
+__global__ void __launch_bounds__(256) gemmj_kernel(const pomx_gemm_job* __restrict__ jobs, const double* __restrict__ A,
+                                                    const double* __restrict__ B, double* __restrict__ C) {
+    pomx_gemm_job const J = jobs[blockIdx.y];
+    int const tn = (J.n + 63) / 64;
+    if(tn == 0)
+        return;
+    int const i0 = int(blockIdx.x / unsigned(tn)) * 64, j0 = int(blockIdx.x % unsigned(tn)) * 64;
+    if(i0 >= J.m)
+        return;
+    gemm_tile<TA, TB>(J, i0, j0, A, B, C);
+}
+
 // One job; the tile is (blockIdx.y, blockIdx.x), so nothing needs to be uploaded per launch.
 template <int TA, int TB>
 __global__ void __launch_bounds__(256) gemm1_kernel(pomx_gemm_job const J, const double* __restrict__ A,
@@ -156,6 +172,25 @@ int launch_gemm1(pomx_ctx* ctx, pomx_gemm_job const& J, const double* d_A, const
         gemm1_kernel<1, 0><<<g, 256, 0, ctx->stream>>>(J, d_A, d_B, d_C);
     else
         gemm1_kernel<1, 1><<<g, 256, 0, ctx->stream>>>(J, d_A, d_B, d_C);
+    POMX_CHECK_LAUNCH(ctx);
+    return POMX_OK;
+}
+
+int launch_gemm_jobs(pomx_ctx* ctx, int trans_a, int trans_b, const pomx_gemm_job* d_jobs, int njobs, int64_t max_tiles,
+                     const double* d_A, const double* d_B, double* d_C) {
+    if(njobs <= 0 || max_tiles <= 0)
+        return POMX_OK;
+    if(njobs > 65535)
+        return pomx_fail(ctx, POMX_ERR_RANGE, "launch_gemm_jobs: more than 65535 jobs in one launch");
+    dim3 const g{unsigned(max_tiles), unsigned(njobs), 1u};
+    if(!trans_a && !trans_b)
+        gemmj_kernel<0, 0><<<g, 256, 0, ctx->stream>>>(d_jobs, d_A, d_B, d_C);
+    else if(!trans_a && trans_b)
+        gemmj_kernel<0, 1><<<g, 256, 0, ctx->stream>>>(d_jobs, d_A, d_B, d_C);
+    else if(trans_a && !trans_b)
+        gemmj_kernel<1, 0><<<g, 256, 0, ctx->stream>>>(d_jobs, d_A, d_B, d_C);
+    else
+        gemmj_kernel<1, 1><<<g, 256, 0, ctx->stream>>>(d_jobs, d_A, d_B, d_C);
     POMX_CHECK_LAUNCH(ctx);
     return POMX_OK;
 }

@@ -125,10 +125,24 @@ int launch_gemm(pomx_ctx* ctx, int trans_a, int trans_b, const GemmTile* d_tiles
 // Launch one GEMM job (tile from the block index, no device descriptors).
 int launch_gemm1(pomx_ctx* ctx, pomx_gemm_job const& J, const double* d_A, const double* d_B, double* d_C);
 
-// Dense symmetric eigensolver for one large block (whole device): blocked Householder
-// tridiagonalisation with MFMA trailing updates, divide-and-conquer on the tridiagonal matrix,
-// blocked (WY) back-transformation on MFMA.  d_A (n x n row-major) is overwritten by the
-// eigenvectors as columns; eigenvalues ascending into d_w.
-int eigh_large(pomx_ctx* ctx, int n, double* d_A, double* d_w);
-// Blocks larger than this go to eigh_large in pomx_eigh_batched.
+// Batched GEMM, job blockIdx.y, tiles along x (max_tiles = largest job's 64 x 64 tile count);
+// job sizes may be written on the device before the launch.
+int launch_gemm_jobs(pomx_ctx* ctx, int trans_a, int trans_b, const pomx_gemm_job* d_jobs, int njobs, int64_t max_tiles,
+                     const double* d_A, const double* d_B, double* d_C);
+
+// Device-driven symmetric eigensolver for a fixed set of blocks, all solved together: blocked
+// Householder tridiagonalisation with MFMA trailing updates, divide and conquer on the
+// tridiagonal matrices (deflation on the device), blocked (WY) MFMA back-transformation.  The
+// launch sequence depends only on the block sizes, so it can be captured in a hipGraph.
+struct BigPlan;
+// Blocks: size n[b], row-major matrix at d_mat + mat_off[b] (overwritten by the eigenvectors as
+// columns), eigenvalues ascending to d_eval + val_off[b].
+int big_plan_create(pomx_ctx* ctx, int nblk, const int* n, const int64_t* mat_off, const int64_t* val_off, BigPlan** plan);
+// Enqueue the whole solve on the context stream (no host synchronisation).  Non-convergence of a
+// leaf eigensolve is flagged in the plan's status (big_plan_status).
+int big_plan_enqueue(pomx_ctx* ctx, BigPlan* plan, double* d_mat, double* d_eval);
+int big_plan_status(pomx_ctx* ctx, BigPlan* plan); // after the stream is synchronised
+size_t big_plan_bytes(BigPlan const* plan);
+void big_plan_destroy(BigPlan* plan);
+// Blocks of at least this size go to the device-driven large-block solver in pomx_eigh_batched.
 constexpr int kEighLargeMin = 513;
