@@ -96,19 +96,28 @@ def run_batch(args):
     batched = len(cfgs) / secs
     log(f"[bench] batch: {len(cfgs)} instances, {b.n_blocks()} blocks, setup {setup:.1f}s, {secs * 1e3:.1f} ms/solve-batch, "
         f"tune: {tune}")
-    # plain path on a sample (same instances, in order)
-    ns = min(args.plain_sample, len(cfgs))
-    t0 = time.perf_counter()
-    for k in range(ns):
-        e = pomerol_amd.ED(terms[k], betas[k])
-        i = [tuple(t) for t in e.indices()].index(("C", 0, 1))
-        e.gf(i, i, W.g1_grid(128, betas[k]), 1e-8, 1e-8)
-    plain_secs = time.perf_counter() - t0
-    # the sample's share of the batch work: scale by the mix of bath sizes actually sampled
-    plain = ns / plain_secs
+    # plain path: a few instances of every bath size, weighted by the batch's own mix of sizes
+    per_size = {}
+    for k, c in enumerate(cfgs):
+        per_size.setdefault(c.n_bath, [])
+        if len(per_size[c.n_bath]) < args.plain_sample:
+            per_size[c.n_bath].append(k)
+    t_size = {}
+    for nb, ks in sorted(per_size.items()):
+        t0 = time.perf_counter()
+        for k in ks:
+            e = pomerol_amd.ED(terms[k], betas[k])
+            i = [tuple(t) for t in e.indices()].index(("C", 0, 1))
+            e.gf(i, i, W.g1_grid(128, betas[k]), 1e-8, 1e-8)
+        t_size[nb] = (time.perf_counter() - t0) / len(ks)
+    mix = {nb: sum(c.n_bath == nb for c in cfgs) / len(cfgs) for nb in t_size}
+    plain_secs = sum(mix[nb] * t_size[nb] for nb in t_size)
+    plain = 1.0 / plain_secs
+    ns = sum(len(v) for v in per_size.values())
+    log(f"[bench] plain path s/solve by bath size: { {nb: round(t, 4) for nb, t in t_size.items()} }, mix {mix}")
     return {"instances": len(cfgs), "freqs": 128, "solves_per_s": batched, "ms_per_batch": secs * 1e3,
             "plain_sample": ns, "plain_solves_per_s": plain, "speedup_vs_plain": batched / plain,
-            "bath_sizes_sampled": [c.n_bath for c in cfgs[:ns]], "autotune": tune, "graph": True}
+            "plain_s_per_solve_by_bath": t_size, "bath_mix": mix, "autotune": tune, "graph": True}
 
 
 def main():
@@ -122,7 +131,7 @@ def main():
     ap.add_argument("--survey", action="store_true", help="print per-part term counts (no frequencies)")
     ap.add_argument("--batch-inst", type=int, default=512, help="instances of the batched ED+G solve (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=3)
-    ap.add_argument("--plain-sample", type=int, default=16, help="instances timed on the plain per-solve path")
+    ap.add_argument("--plain-sample", type=int, default=4, help="instances per bath size timed on the plain per-solve path")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path's multi-GPU "

@@ -50,7 +50,7 @@ int pomx_assemble(pomx_ctx* ctx, int64_t nnz, const int64_t* d_pos, const double
  * Replaces HamiltonianPart::computeImpl (src/pomerol/HamiltonianPart.cpp:90-102), i.e. Eigen's
  * SelfAdjointEigenSolver per block.  Block b (size h_n[b]) is the row-major matrix at
  * d_mat + h_mat_off[b]; on return it holds the eigenvectors as columns and d_eval + h_val_off[b]
- * the eigenvalues in ascending order.  Blocks of size <= 512: one workgroup per block
+ * the eigenvalues in ascending order.  Blocks of size <= 128: one workgroup per block
  * (Householder tridiagonalisation, implicit QL, Householder back-transformation), all such
  * blocks concurrently.  Larger blocks, one at a time on the whole device: blocked Householder
  * tridiagonalisation with MFMA trailing updates, divide and conquer on the tridiagonal matrix,

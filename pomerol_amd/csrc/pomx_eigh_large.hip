@@ -31,6 +31,7 @@ namespace {
 constexpr int kNB = 32;   // reflectors per panel
 constexpr int kLeaf = 64; // largest divide-and-conquer leaf
 constexpr int kMaxSymvGrid = 1024;
+constexpr int kLdsV = 16384; // longest reflector staged in LDS by the matrix-vector kernel
 
 struct BigBlk {
     int n, gsym;              // block size; symv grid width (partial sums per block)
@@ -143,44 +144,27 @@ __device__ inline double wg_max(double v, double* red) {
     return s;
 }
 
-// Arrive at a ticket counter; true in the last workgroup of the grid row (which then sees every
-// other workgroup's stores) and the counter is rearmed for the next launch.
-__device__ inline bool last_block(unsigned* counter) {
-    __shared__ bool last;
-    __threadfence();
-    __syncthreads();
-    if(threadIdx.x == 0) {
-        unsigned t = atomicAdd(counter, 1u);
-        last = (t == gridDim.x - 1);
-        if(last)
-            atomicExch(counter, 0u);
-    }
-    __syncthreads();
-    if(last)
-        __threadfence();
-    return last;
-}
-
 // ---- 1. tridiagonalisation -------------------------------------------------------------------
 //
 // Panel state per block: V = Vall[:, j0:j0+kNB] (reflectors, unit entry stored), W[:, p] = w_p
 // without its last term, a2[p] = -tau/2 (w_p . v_p); the true w_p = W[:, p] + a2[p] V[:, p] is
-// folded in by every reader, and into W once per panel before the trailing update.
+// folded in by every reader, and into W once per panel before the trailing update.  Each
+// device-wide sum is written as per-workgroup partials and finished by a one-workgroup kernel per
+// block (the kernel boundary orders the partials; no device-scope fences inside kernels).
 
-// Row j brought up to date with the panel's first i reflectors (columns across the grid); the
-// last workgroup forms the reflector of the row tail and writes v (vbuf and Vall column j).
+// Row j brought up to date with the panel's first i reflectors (columns across the grid), and
+// partial sums of the squared row tail beyond j+1.
 __global__ void __launch_bounds__(256) trd_row_kernel(const BigBlk* __restrict__ blks, int j, int j0, int i,
-                                                      double* __restrict__ Aar, double* __restrict__ D, int* __restrict__ I) {
+                                                      double* __restrict__ Aar, double* __restrict__ D) {
     __shared__ double red[4];
     __shared__ double vj[kNB], wj[kNB], av[kNB];
     BigBlk const B = blks[blockIdx.y];
     int const n = B.n, tid = threadIdx.x;
     DOff const o = doff(n, B.gsym);
     double* __restrict__ base = D + B.dbase;
-    double* __restrict__ Vall = base + o.V;
+    const double* __restrict__ Vall = base + o.V;
     const double* __restrict__ Wp = base + o.W;
     const double* __restrict__ a2 = base + o.a2;
-    unsigned* counter = reinterpret_cast<unsigned*>(I + B.ibase + ioff(n).cnt);
     if(tid < i) {
         double const v = Vall[int64_t(j) * n + j0 + tid];
         vj[tid] = v;
@@ -208,10 +192,20 @@ __global__ void __launch_bounds__(256) trd_row_kernel(const BigBlk* __restrict__
     __syncthreads();
     if(tid == 0)
         base[o.rowpart + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
-    if(!last_block(counter))
-        return;
+}
+
+// One workgroup per block: the Householder reflector of the updated row tail; v to vbuf and to
+// column j of Vall, d[j], e[j], tau[j].
+__global__ void __launch_bounds__(256) trd_refl_kernel(const BigBlk* __restrict__ blks, int j, int nparts,
+                                                       const double* __restrict__ Aar, double* __restrict__ D) {
+    __shared__ double red[4];
+    BigBlk const B = blks[blockIdx.y];
+    int const n = B.n, tid = threadIdx.x;
+    DOff const o = doff(n, B.gsym);
+    double* __restrict__ base = D + B.dbase;
+    const double* __restrict__ row = Aar + B.mat_off + int64_t(j) * n;
     double sum = 0.0;
-    for(int b = tid; b < int(gridDim.x); b += 256)
+    for(int b = tid; b < nparts; b += 256)
         sum += base[o.rowpart + b];
     double const ss = wg_sum(sum, red);
     int const m = n - j - 1;
@@ -225,7 +219,7 @@ __global__ void __launch_bounds__(256) trd_row_kernel(const BigBlk* __restrict__
     for(int c = tid; c < m; c += 256) {
         double v = (c == 0) ? 1.0 : (t != 0.0 ? row[j + 1 + c] * scale : 0.0);
         base[o.vbuf + c] = v;
-        Vall[int64_t(j + 1 + c) * n + j] = v;
+        base[o.V + int64_t(j + 1 + c) * n + j] = v;
     }
     if(tid == 0) {
         base[o.d + j] = row[j];
@@ -234,40 +228,52 @@ __global__ void __launch_bounds__(256) trd_row_kernel(const BigBlk* __restrict__
     }
 }
 
-// w_raw = A22 v, one wavefront per row, plus partial sums of W^T v and V^T v over a chunk of
-// rows; the last workgroup reduces them to t1 = (true W)^T v and t2 = V^T v.
+// w_raw = A22 v (v staged in LDS when it fits), one wavefront per row with eight loads in flight,
+// plus partial sums of W^T v and V^T v over a chunk of rows.
+template <bool LDSV>
 __global__ void __launch_bounds__(256) trd_symv_kernel(const BigBlk* __restrict__ blks, int j, int j0, int i,
-                                                       const double* __restrict__ Aar, double* __restrict__ D,
-                                                       int* __restrict__ I) {
+                                                       const double* __restrict__ Aar, double* __restrict__ D) {
+    extern __shared__ double vs[];
     __shared__ double red[4][64];
     BigBlk const B = blks[blockIdx.y];
     int const n = B.n;
     DOff const o = doff(n, B.gsym);
     double* __restrict__ base = D + B.dbase;
-    const double* __restrict__ vbuf = base + o.vbuf;
     const double* __restrict__ Vall = base + o.V;
     const double* __restrict__ Wp = base + o.W;
-    unsigned* counter = reinterpret_cast<unsigned*>(I + B.ibase + ioff(n).cnt) + 1;
     int const m = n - j - 1, jb = j + 1;
     int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const double* __restrict__ v = base + o.vbuf;
+    if(LDSV) {
+        for(int c = threadIdx.x; c < m; c += 256)
+            vs[c] = v[c];
+        __syncthreads();
+        v = vs;
+    }
     for(int r = blockIdx.x * 4 + wave; r < m; r += gridDim.x * 4) {
         const double* __restrict__ a = Aar + B.mat_off + int64_t(jb + r) * n + jb;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, s5 = 0.0, s6 = 0.0, s7 = 0.0;
         int c = lane;
-        for(; c + 192 < m; c += 256) {
-            s0 = fma(a[c], vbuf[c], s0);
-            s1 = fma(a[c + 64], vbuf[c + 64], s1);
-            s2 = fma(a[c + 128], vbuf[c + 128], s2);
-            s3 = fma(a[c + 192], vbuf[c + 192], s3);
+        for(; c + 448 < m; c += 512) {
+            double const a0 = a[c], a1 = a[c + 64], a2v = a[c + 128], a3 = a[c + 192];
+            double const a4 = a[c + 256], a5 = a[c + 320], a6 = a[c + 384], a7 = a[c + 448];
+            s0 = fma(a0, v[c], s0);
+            s1 = fma(a1, v[c + 64], s1);
+            s2 = fma(a2v, v[c + 128], s2);
+            s3 = fma(a3, v[c + 192], s3);
+            s4 = fma(a4, v[c + 256], s4);
+            s5 = fma(a5, v[c + 320], s5);
+            s6 = fma(a6, v[c + 384], s6);
+            s7 = fma(a7, v[c + 448], s7);
         }
         for(; c < m; c += 64)
-            s0 = fma(a[c], vbuf[c], s0);
-        double const s = wave_sum((s0 + s1) + (s2 + s3));
+            s0 = fma(a[c], v[c], s0);
+        double const s = wave_sum(((s0 + s1) + (s2 + s3)) + ((s4 + s5) + (s6 + s7)));
         if(lane == 0)
             base[o.wraw + r] = s;
     }
     if(i == 0)
-        return; // uniform: no reduction at the first reflector of a panel
+        return;
     int const chunk = (m + gridDim.x - 1) / gridDim.x;
     int const r0 = blockIdx.x * chunk, r1 = min(m, r0 + chunk);
     int const p = lane; // p < 32: (W^T v)_p, 32 <= p < 64: (V^T v)_{p-32}
@@ -275,18 +281,25 @@ __global__ void __launch_bounds__(256) trd_symv_kernel(const BigBlk* __restrict_
     if((p & 31) < i)
         for(int r = r0 + wave; r < r1; r += 4) {
             double x = (p < 32) ? Wp[int64_t(jb + r) * kNB + p] : Vall[int64_t(jb + r) * n + j0 + (p - 32)];
-            acc = fma(x, vbuf[r], acc);
+            acc = fma(x, v[r], acc);
         }
     red[wave][lane] = acc;
     __syncthreads();
     if(wave == 0)
         base[o.part + int64_t(blockIdx.x) * 64 + lane] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
-    if(!last_block(counter))
-        return;
+}
+
+// One workgroup per block: t1 = (true W)^T v and t2 = V^T v from the partials.
+__global__ void __launch_bounds__(256) trd_t_kernel(const BigBlk* __restrict__ blks, int i, int nparts,
+                                                    double* __restrict__ D) {
+    __shared__ double red[4][64];
+    BigBlk const B = blks[blockIdx.y];
+    DOff const o = doff(B.n, B.gsym);
+    double* __restrict__ base = D + B.dbase;
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double t = 0.0;
-    for(int b = wave; b < int(gridDim.x); b += 4)
+    for(int b = wave; b < nparts; b += 4)
         t += base[o.part + int64_t(b) * 64 + lane];
-    __syncthreads();
     red[wave][lane] = t;
     __syncthreads();
     if(threadIdx.x < 32) {
@@ -298,10 +311,10 @@ __global__ void __launch_bounds__(256) trd_symv_kernel(const BigBlk* __restrict_
     }
 }
 
-// w = tau (w_raw - V t1 - (true W) t2) stored without its last term as column i of W; the last
-// workgroup finishes a2[i] = -tau/2 (w . v).
+// w = tau (w_raw - V t1 - (true W) t2) stored without its last term as column i of W, plus
+// partial sums of w . v.
 __global__ void __launch_bounds__(256) trd_w_kernel(const BigBlk* __restrict__ blks, int j, int j0, int i,
-                                                    double* __restrict__ D, int* __restrict__ I) {
+                                                    double* __restrict__ D) {
     __shared__ double ca[kNB], cb[kNB];
     __shared__ double red[4];
     BigBlk const B = blks[blockIdx.y];
@@ -310,7 +323,6 @@ __global__ void __launch_bounds__(256) trd_w_kernel(const BigBlk* __restrict__ b
     double* __restrict__ base = D + B.dbase;
     const double* __restrict__ Vall = base + o.V;
     double* __restrict__ Wp = base + o.W;
-    unsigned* counter = reinterpret_cast<unsigned*>(I + B.ibase + ioff(n).cnt) + 2;
     int const m = n - j - 1, jb = j + 1;
     double const tj = base[o.tau + j];
     if(tid < i) {
@@ -336,14 +348,21 @@ __global__ void __launch_bounds__(256) trd_w_kernel(const BigBlk* __restrict__ b
     __syncthreads();
     if(tid == 0)
         base[o.wpart + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
-    if(!last_block(counter))
-        return;
+}
+
+// One workgroup per block: a2[i] = -tau/2 (w . v) from the partials.
+__global__ void __launch_bounds__(256) trd_a2_kernel(const BigBlk* __restrict__ blks, int j, int i, int nparts,
+                                                     double* __restrict__ D) {
+    __shared__ double red[4];
+    BigBlk const B = blks[blockIdx.y];
+    DOff const o = doff(B.n, B.gsym);
+    double* __restrict__ base = D + B.dbase;
     double sum = 0.0;
-    for(int b = tid; b < int(gridDim.x); b += 256)
+    for(int b = threadIdx.x; b < nparts; b += 256)
         sum += base[o.wpart + b];
     double const tot = wg_sum(sum, red);
-    if(tid == 0)
-        base[o.a2 + i] = -0.5 * tj * tot;
+    if(threadIdx.x == 0)
+        base[o.a2 + i] = -0.5 * base[o.tau + j] * tot;
 }
 
 // End of a panel (blocks listed in ids, panel ending at reflector j): W[r][p] += a2[p] V[r][p]
@@ -862,6 +881,13 @@ unsigned gx_for(int64_t work, int per = 256, int cap = 2048) {
     return unsigned(std::max<int64_t>(1, std::min<int64_t>((work + per - 1) / per, cap)));
 }
 
+// Grid width of a launch over ny blocks/nodes: enough workgroups for the work of one row, but
+// about 32k workgroups in all (every batched kernel loops over its row's work).
+unsigned gx_rows(int64_t work, int per, int ny) {
+    int64_t const cap = std::max<int64_t>(1, (32768 + ny - 1) / std::max(ny, 1));
+    return unsigned(std::max<int64_t>(1, std::min<int64_t>((work + per - 1) / per, cap)));
+}
+
 template <class T> T* dupload(std::vector<T> const& h, hipStream_t s, bool& ok) {
     void* p = nullptr;
     if(hipMalloc(&p, sizeof(T) * std::max<size_t>(h.size(), 1)) != hipSuccess) {
@@ -1118,7 +1144,10 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
     zero_ints_kernel<<<gx_for(std::max(P->nleaves, 1)), 256, 0, s>>>(P->status, std::max(P->nleaves, 1));
     // 1. reduction to tridiagonal form
     if(!P->tri_only) {
-        fill_blocks_kernel<<<dim3(gx_for(int64_t(P->nmax) * P->nmax), unsigned(nb)), 256, 0, s>>>(P->blk, P->D, 1, 0.0);
+        POMX_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(trd_symv_kernel<true>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(double) * kLdsV)));
+        fill_blocks_kernel<<<dim3(gx_rows(int64_t(P->nmax) * P->nmax, 256, nb), unsigned(nb)), 256, 0, s>>>(P->blk, P->D, 1,
+                                                                                                        0.0);
         size_t e = 0;
         int nact = nb;
         int const n0 = P->h_blk[0].n;
@@ -1126,15 +1155,24 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
             while(nact > 0 && P->h_blk[size_t(nact - 1)].n - 2 < j)
                 --nact;
             int const j0 = (j / kNB) * kNB, i = j - j0, m = n0 - j - 1;
-            unsigned const g1 = unsigned(std::max(1, std::min(256, (n0 - j + 255) / 256)));
-            unsigned const g2 = unsigned(std::max(1, std::min(P->gsym, (m + 15) / 16)));
-            unsigned const g3 = unsigned(std::max(1, std::min(256, (m + 255) / 256)));
-            trd_row_kernel<<<dim3(g1, unsigned(nact)), 256, 0, s>>>(P->blk, j, j0, i, Aar, P->D, P->I);
-            trd_symv_kernel<<<dim3(g2, unsigned(nact)), 256, 0, s>>>(P->blk, j, j0, i, Aar, P->D, P->I);
-            trd_w_kernel<<<dim3(g3, unsigned(nact)), 256, 0, s>>>(P->blk, j, j0, i, P->D, P->I);
+            unsigned const g1 = std::min(256u, gx_rows(n0 - j, 256, nact));
+            unsigned const g2 = std::min(unsigned(P->gsym), gx_rows(m, 16, nact));
+            unsigned const g3 = std::min(256u, gx_rows(m, 256, nact));
+            dim3 const one(1, unsigned(nact));
+            trd_row_kernel<<<dim3(g1, unsigned(nact)), 256, 0, s>>>(P->blk, j, j0, i, Aar, P->D);
+            trd_refl_kernel<<<one, 256, 0, s>>>(P->blk, j, int(g1), Aar, P->D);
+            if(m <= kLdsV)
+                trd_symv_kernel<true><<<dim3(g2, unsigned(nact)), 256, sizeof(double) * size_t(m), s>>>(P->blk, j, j0, i, Aar,
+                                                                                                       P->D);
+            else
+                trd_symv_kernel<false><<<dim3(g2, unsigned(nact)), 256, 0, s>>>(P->blk, j, j0, i, Aar, P->D);
+            if(i > 0)
+                trd_t_kernel<<<one, 256, 0, s>>>(P->blk, i, int(g2), P->D);
+            trd_w_kernel<<<dim3(g3, unsigned(nact)), 256, 0, s>>>(P->blk, j, j0, i, P->D);
+            trd_a2_kernel<<<one, 256, 0, s>>>(P->blk, j, i, int(g3), P->D);
             if(e < P->ends.size() && P->ends[e].j == j) {
                 BigPlan::PanelEnd const& E = P->ends[e++];
-                trd_fold_kernel<<<dim3(gx_for(int64_t(std::max(m, 1)) * kNB, 256, 512), unsigned(E.nblk)), 256, 0, s>>>(
+                trd_fold_kernel<<<dim3(gx_rows(int64_t(std::max(m, 1)) * kNB, 256, E.nblk), unsigned(E.nblk)), 256, 0, s>>>(
                     P->blk, P->ids + E.ids_off, j, P->D);
                 if((rc = launch_gemm_jobs(ctx, 0, 1, P->gjobs + E.jobs_off, E.nblk, E.max_tiles, P->D, P->D, Aar)) ||
                    (rc = launch_gemm_jobs(ctx, 0, 1, P->gjobs + E.jobs_off + E.nblk, E.nblk, E.max_tiles, P->D, P->D, Aar)))
@@ -1148,7 +1186,7 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
     dc_leaf_build_kernel<<<unsigned(P->nleaves), 256, 0, s>>>(P->blk, P->leaves, P->D, P->leaf_arena);
     if((rc = launch_eigh(ctx, P->leaf_jobs, P->nleaves, kLeaf, 64, 0, nullptr, P->leaf_arena, P->D, P->leaf_ws, P->status)))
         return rc;
-    fill_blocks_kernel<<<dim3(gx_for(int64_t(P->nmax) * P->nmax), unsigned(nb)), 256, 0, s>>>(P->blk, P->D, 0, 0.0);
+    fill_blocks_kernel<<<dim3(gx_rows(int64_t(P->nmax) * P->nmax, 256, nb), unsigned(nb)), 256, 0, s>>>(P->blk, P->D, 0, 0.0);
     dc_leaf_place_kernel<<<unsigned(P->nleaves), 256, 0, s>>>(P->blk, P->leaves, P->leaf_arena, P->D);
     POMX_CHECK_LAUNCH(ctx);
     // merges, level by level
@@ -1156,9 +1194,9 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
         DcNode const* nd = P->nodes + L.node_off;
         DcState* st = P->states + L.node_off;
         pomx_gemm_job* jb = P->dc_jobs + L.node_off;
-        dim3 const gsq(gx_for(int64_t(L.nmax) * L.nmax), unsigned(L.nnodes));
-        dim3 const grow(gx_for(L.nmax), unsigned(L.nnodes));
-        dim3 const gwave(gx_for(L.nmax, 4, 4096), unsigned(L.nnodes));
+        dim3 const gsq(gx_rows(int64_t(L.nmax) * L.nmax, 256, L.nnodes), unsigned(L.nnodes));
+        dim3 const grow(gx_rows(L.nmax, 256, L.nnodes), unsigned(L.nnodes));
+        dim3 const gwave(gx_rows(L.nmax, 4, L.nnodes), unsigned(L.nnodes));
         dc_deflate_kernel<<<dim3(1, unsigned(L.nnodes)), 256, 0, s>>>(P->blk, nd, st, jb, P->D, P->I, P->R);
         dc_gather_kernel<<<gsq, 256, 0, s>>>(P->blk, nd, 0, P->D, P->I);
         dc_rot_kernel<<<grow, 256, 0, s>>>(P->blk, nd, st, P->D, P->R);
@@ -1179,7 +1217,7 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
     }
     // 3. back-transformation, eigenvalues
     if(!P->tri_only) {
-        copy_z_kernel<<<dim3(gx_for(int64_t(P->nmax) * P->nmax), unsigned(nb)), 256, 0, s>>>(P->blk, P->D, Aar);
+        copy_z_kernel<<<dim3(gx_rows(int64_t(P->nmax) * P->nmax, 256, nb), unsigned(nb)), 256, 0, s>>>(P->blk, P->D, Aar);
         int64_t const tn = (P->nmax + 63) / 64;
         for(auto const& T : P->bt) {
             pomx_gemm_job const* J = P->gjobs + T.jobs_off;
@@ -1193,7 +1231,7 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
                (rc = launch_gemm_jobs(ctx, 0, 0, J + 3 * T.nblk, T.nblk, tr * tn, P->D, P->D, Aar)))
                 return rc;
         }
-        evals_out_kernel<<<dim3(gx_for(P->nmax), unsigned(nb)), 256, 0, s>>>(P->blk, P->D, eval);
+        evals_out_kernel<<<dim3(gx_rows(P->nmax, 256, nb), unsigned(nb)), 256, 0, s>>>(P->blk, P->D, eval);
         POMX_CHECK_LAUNCH(ctx);
     }
     return POMX_OK;

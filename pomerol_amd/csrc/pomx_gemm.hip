@@ -116,13 +116,12 @@ template <int TA, int TB>
 __global__ void __launch_bounds__(256) gemmj_kernel(const pomx_gemm_job* __restrict__ jobs, const double* __restrict__ A,
                                                     const double* __restrict__ B, double* __restrict__ C) {
     pomx_gemm_job const J = jobs[blockIdx.y];
-    int const tn = (J.n + 63) / 64;
-    if(tn == 0)
-        return;
-    int const i0 = int(blockIdx.x / unsigned(tn)) * 64, j0 = int(blockIdx.x % unsigned(tn)) * 64;
-    if(i0 >= J.m)
-        return;
-    gemm_tile<TA, TB>(J, i0, j0, A, B, C);
+    int const tn = (J.n + 63) / 64, tm = (J.m + 63) / 64;
+    // grid-stride over the job's tiles (uniform per workgroup: the barriers inside stay uniform)
+    for(int t = blockIdx.x; t < tm * tn; t += gridDim.x) {
+        __syncthreads();
+        gemm_tile<TA, TB>(J, (t / tn) * 64, (t % tn) * 64, A, B, C);
+    }
 }
 
 // One job; the tile is (blockIdx.y, blockIdx.x), so nothing needs to be uploaded per launch.
@@ -182,7 +181,10 @@ int launch_gemm_jobs(pomx_ctx* ctx, int trans_a, int trans_b, const pomx_gemm_jo
         return POMX_OK;
     if(njobs > 65535)
         return pomx_fail(ctx, POMX_ERR_RANGE, "launch_gemm_jobs: more than 65535 jobs in one launch");
-    dim3 const g{unsigned(max_tiles), unsigned(njobs), 1u};
+    // about 32k workgroups in all: many small jobs loop over their tiles instead of launching
+    // one workgroup per tile
+    int64_t const gx = std::max<int64_t>(1, std::min<int64_t>(max_tiles, (32768 + njobs - 1) / njobs));
+    dim3 const g{unsigned(gx), unsigned(njobs), 1u};
     if(!trans_a && !trans_b)
         gemmj_kernel<0, 0><<<g, 256, 0, ctx->stream>>>(d_jobs, d_A, d_B, d_C);
     else if(!trans_a && trans_b)

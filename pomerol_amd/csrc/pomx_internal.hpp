@@ -145,4 +145,4 @@ int big_plan_status(pomx_ctx* ctx, BigPlan* plan); // after the stream is synchr
 size_t big_plan_bytes(BigPlan const* plan);
 void big_plan_destroy(BigPlan* plan);
 // Blocks of at least this size go to the device-driven large-block solver in pomx_eigh_batched.
-constexpr int kEighLargeMin = 513;
+constexpr int kEighLargeMin = 129;
