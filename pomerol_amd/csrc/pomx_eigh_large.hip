@@ -31,7 +31,9 @@ namespace {
 constexpr int kNB = 32;   // reflectors per panel
 constexpr int kLeaf = 64; // largest divide-and-conquer leaf
 constexpr int kMaxSymvGrid = 1024;
-constexpr int kLdsV = 16384; // longest reflector staged in LDS by the matrix-vector kernel
+constexpr int kLdsV = 16384;   // longest reflector staged in LDS by the matrix-vector kernel
+constexpr int kSymTile = 64;   // rows per tile of the symmetric matrix-vector product
+constexpr int kSymChunk = 512; // columns per work item of the symmetric matrix-vector product
 
 struct BigBlk {
     int n, gsym;              // block size; symv grid width (partial sums per block)
@@ -42,7 +44,7 @@ struct BigBlk {
 // Per-block scratch layout (doubles from dbase).
 struct DOff {
     int64_t V, Z, Q2, U, W, W1, W2, vbuf, wraw, d, e, tau, lam, zs, dsrt, dl, zl, rt, zh, vals, a2, S, T, rowpart, wpart,
-        scal, part, end;
+        scal, part, rp, cp, end;
 };
 __host__ __device__ inline DOff doff(int n, int gsym) {
     DOff o;
@@ -75,6 +77,9 @@ __host__ __device__ inline DOff doff(int n, int gsym) {
     o.wpart = at, at += 256;
     o.scal = at, at += 64;
     o.part = at, at += int64_t(gsym) * 64;
+    int64_t const nt = (n + kSymTile - 1) / kSymTile, nc = (n + kSymChunk - 1) / kSymChunk;
+    o.rp = at, at += nt * nc * kSymTile; // symmetric product: row partials per (row tile, chunk)
+    o.cp = at, at += nt * int64_t(n);    // column partials per row tile
     o.end = at;
     return o;
 }
@@ -228,19 +233,19 @@ __global__ void __launch_bounds__(256) trd_refl_kernel(const BigBlk* __restrict_
     }
 }
 
-// w_raw = A22 v (v staged in LDS when it fits), one wavefront per row with eight loads in flight,
-// plus partial sums of W^T v and V^T v over a chunk of rows.
+// w_raw = A22 v from the upper triangle of A22 only (A22 symmetric; its lower triangle is never
+// updated): work item (row tile t, column chunk k) covers rows [64t, 64t+64) and columns
+// [64t + 512k, +512) on or above the diagonal; it adds A[r][c] v[c] to the row partial of r and
+// A[r][c] v[r] (c > r) to the column partial of c.  v is staged in LDS when it fits.
 template <bool LDSV>
-__global__ void __launch_bounds__(256) trd_symv_kernel(const BigBlk* __restrict__ blks, int j, int j0, int i,
-                                                       const double* __restrict__ Aar, double* __restrict__ D) {
+__global__ void __launch_bounds__(256) trd_symv_kernel(const BigBlk* __restrict__ blks, int j, const double* __restrict__ Aar,
+                                                       double* __restrict__ D) {
     extern __shared__ double vs[];
-    __shared__ double red[4][64];
+    __shared__ double cred[4][kSymChunk];
     BigBlk const B = blks[blockIdx.y];
     int const n = B.n;
     DOff const o = doff(n, B.gsym);
     double* __restrict__ base = D + B.dbase;
-    const double* __restrict__ Vall = base + o.V;
-    const double* __restrict__ Wp = base + o.W;
     int const m = n - j - 1, jb = j + 1;
     int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const double* __restrict__ v = base + o.vbuf;
@@ -250,38 +255,91 @@ __global__ void __launch_bounds__(256) trd_symv_kernel(const BigBlk* __restrict_
         __syncthreads();
         v = vs;
     }
-    for(int r = blockIdx.x * 4 + wave; r < m; r += gridDim.x * 4) {
-        const double* __restrict__ a = Aar + B.mat_off + int64_t(jb + r) * n + jb;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, s5 = 0.0, s6 = 0.0, s7 = 0.0;
-        int c = lane;
-        for(; c + 448 < m; c += 512) {
-            double const a0 = a[c], a1 = a[c + 64], a2v = a[c + 128], a3 = a[c + 192];
-            double const a4 = a[c + 256], a5 = a[c + 320], a6 = a[c + 384], a7 = a[c + 448];
-            s0 = fma(a0, v[c], s0);
-            s1 = fma(a1, v[c + 64], s1);
-            s2 = fma(a2v, v[c + 128], s2);
-            s3 = fma(a3, v[c + 192], s3);
-            s4 = fma(a4, v[c + 256], s4);
-            s5 = fma(a5, v[c + 320], s5);
-            s6 = fma(a6, v[c + 384], s6);
-            s7 = fma(a7, v[c + 448], s7);
+    int const nt = (m + kSymTile - 1) / kSymTile, ncmax = (m + kSymChunk - 1) / kSymChunk;
+    int const ncb = (n + kSymChunk - 1) / kSymChunk; // row-partial stride of this block
+    for(int q = blockIdx.x; q < nt * ncmax; q += gridDim.x) {
+        int const t = q / ncmax, k = q % ncmax;
+        int const R0 = t * kSymTile, C0 = R0 + k * kSymChunk;
+        if(C0 >= m)
+            continue; // uniform over the workgroup
+        double cacc[kSymChunk / 64];
+#pragma unroll
+        for(int u = 0; u < kSymChunk / 64; ++u)
+            cacc[u] = 0.0;
+        for(int rr = wave; rr < kSymTile; rr += 4) {
+            int const r = R0 + rr;
+            if(r >= m)
+                break;
+            const double* __restrict__ a = Aar + B.mat_off + int64_t(jb + r) * n + jb;
+            double const vr = v[r];
+            double x[kSymChunk / 64];
+#pragma unroll
+            for(int u = 0; u < kSymChunk / 64; ++u) {
+                int const c = C0 + lane + 64 * u;
+                x[u] = (c < m && c >= r) ? a[c] : 0.0;
+            }
+            double rs = 0.0;
+#pragma unroll
+            for(int u = 0; u < kSymChunk / 64; ++u) {
+                int const c = C0 + lane + 64 * u;
+                if(c < m) {
+                    rs = fma(x[u], v[c], rs);
+                    if(c > r)
+                        cacc[u] = fma(x[u], vr, cacc[u]);
+                }
+            }
+            rs = wave_sum(rs);
+            if(lane == 0)
+                base[o.rp + (int64_t(t) * ncb + k) * kSymTile + rr] = rs;
         }
-        for(; c < m; c += 64)
-            s0 = fma(a[c], v[c], s0);
-        double const s = wave_sum(((s0 + s1) + (s2 + s3)) + ((s4 + s5) + (s6 + s7)));
-        if(lane == 0)
-            base[o.wraw + r] = s;
+#pragma unroll
+        for(int u = 0; u < kSymChunk / 64; ++u)
+            cred[wave][lane + 64 * u] = cacc[u];
+        __syncthreads();
+        for(int c = threadIdx.x; c < kSymChunk; c += 256) {
+            int const cc = C0 + c;
+            if(cc < m)
+                base[o.cp + int64_t(t) * n + cc] = (cred[0][c] + cred[1][c]) + (cred[2][c] + cred[3][c]);
+        }
+        __syncthreads();
+    }
+}
+
+// w_raw[x] = sum of the row partials of row x and the column partials of x from every row tile
+// at or above it; plus partial sums of W^T v and V^T v over a chunk of rows.
+__global__ void __launch_bounds__(256) trd_symv_reduce_kernel(const BigBlk* __restrict__ blks, int j, int j0, int i,
+                                                              double* __restrict__ D) {
+    __shared__ double red[4][64];
+    BigBlk const B = blks[blockIdx.y];
+    int const n = B.n;
+    DOff const o = doff(n, B.gsym);
+    double* __restrict__ base = D + B.dbase;
+    const double* __restrict__ Vall = base + o.V;
+    const double* __restrict__ Wp = base + o.W;
+    const double* __restrict__ v = base + o.vbuf;
+    int const m = n - j - 1, jb = j + 1;
+    int const ncmax = (m + kSymChunk - 1) / kSymChunk, ncb = (n + kSymChunk - 1) / kSymChunk;
+    for(int x = blockIdx.x * 256 + threadIdx.x; x < m; x += gridDim.x * 256) {
+        int const t = x / kSymTile;
+        int const nck = (m - t * kSymTile + kSymChunk - 1) / kSymChunk; // chunks of row tile t
+        double s = 0.0;
+        for(int k = 0; k < nck && k < ncmax; ++k)
+            s += base[o.rp + (int64_t(t) * ncb + k) * kSymTile + (x - t * kSymTile)];
+        for(int tt = 0; tt <= t; ++tt)
+            s += base[o.cp + int64_t(tt) * n + x];
+        base[o.wraw + x] = s;
     }
     if(i == 0)
         return;
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int const chunk = (m + gridDim.x - 1) / gridDim.x;
     int const r0 = blockIdx.x * chunk, r1 = min(m, r0 + chunk);
     int const p = lane; // p < 32: (W^T v)_p, 32 <= p < 64: (V^T v)_{p-32}
     double acc = 0.0;
     if((p & 31) < i)
         for(int r = r0 + wave; r < r1; r += 4) {
-            double x = (p < 32) ? Wp[int64_t(jb + r) * kNB + p] : Vall[int64_t(jb + r) * n + j0 + (p - 32)];
-            acc = fma(x, v[r], acc);
+            double xx = (p < 32) ? Wp[int64_t(jb + r) * kNB + p] : Vall[int64_t(jb + r) * n + j0 + (p - 32)];
+            acc = fma(xx, v[r], acc);
         }
     red[wave][lane] = acc;
     __syncthreads();
@@ -1156,16 +1214,18 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
                 --nact;
             int const j0 = (j / kNB) * kNB, i = j - j0, m = n0 - j - 1;
             unsigned const g1 = std::min(256u, gx_rows(n0 - j, 256, nact));
-            unsigned const g2 = std::min(unsigned(P->gsym), gx_rows(m, 16, nact));
+            unsigned const g2 = std::min(unsigned(P->gsym), gx_rows(m, 256, nact));
             unsigned const g3 = std::min(256u, gx_rows(m, 256, nact));
             dim3 const one(1, unsigned(nact));
             trd_row_kernel<<<dim3(g1, unsigned(nact)), 256, 0, s>>>(P->blk, j, j0, i, Aar, P->D);
             trd_refl_kernel<<<one, 256, 0, s>>>(P->blk, j, int(g1), Aar, P->D);
+            int64_t const items = int64_t((m + kSymTile - 1) / kSymTile) * ((m + kSymChunk - 1) / kSymChunk);
+            unsigned const gs = gx_rows(items, 1, nact);
             if(m <= kLdsV)
-                trd_symv_kernel<true><<<dim3(g2, unsigned(nact)), 256, sizeof(double) * size_t(m), s>>>(P->blk, j, j0, i, Aar,
-                                                                                                       P->D);
+                trd_symv_kernel<true><<<dim3(gs, unsigned(nact)), 256, sizeof(double) * size_t(m), s>>>(P->blk, j, Aar, P->D);
             else
-                trd_symv_kernel<false><<<dim3(g2, unsigned(nact)), 256, 0, s>>>(P->blk, j, j0, i, Aar, P->D);
+                trd_symv_kernel<false><<<dim3(gs, unsigned(nact)), 256, 0, s>>>(P->blk, j, Aar, P->D);
+            trd_symv_reduce_kernel<<<dim3(g2, unsigned(nact)), 256, 0, s>>>(P->blk, j, j0, i, P->D);
             if(i > 0)
                 trd_t_kernel<<<one, 256, 0, s>>>(P->blk, i, int(g2), P->D);
             trd_w_kernel<<<dim3(g3, unsigned(nact)), 256, 0, s>>>(P->blk, j, j0, i, P->D);
@@ -1174,8 +1234,9 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
                 BigPlan::PanelEnd const& E = P->ends[e++];
                 trd_fold_kernel<<<dim3(gx_rows(int64_t(std::max(m, 1)) * kNB, 256, E.nblk), unsigned(E.nblk)), 256, 0, s>>>(
                     P->blk, P->ids + E.ids_off, j, P->D);
-                if((rc = launch_gemm_jobs(ctx, 0, 1, P->gjobs + E.jobs_off, E.nblk, E.max_tiles, P->D, P->D, Aar)) ||
-                   (rc = launch_gemm_jobs(ctx, 0, 1, P->gjobs + E.jobs_off + E.nblk, E.nblk, E.max_tiles, P->D, P->D, Aar)))
+                if((rc = launch_gemm_jobs(ctx, 0, 1, P->gjobs + E.jobs_off, E.nblk, E.max_tiles, P->D, P->D, Aar, true)) ||
+                   (rc = launch_gemm_jobs(ctx, 0, 1, P->gjobs + E.jobs_off + E.nblk, E.nblk, E.max_tiles, P->D, P->D, Aar,
+                                          true)))
                     return rc;
             }
         }

@@ -112,15 +112,19 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmTile* __restrict__ 
 // Job blockIdx.y, tile blockIdx.x (row-major over the job's 64 x 64 tiles).  Tiles outside the
 // job exit at once, so the launch can be sized for the largest job and the job sizes may be
 // written on the device by an earlier kernel.
-template <int TA, int TB>
+template <int TA, int TB, bool UPPER>
 __global__ void __launch_bounds__(256) gemmj_kernel(const pomx_gemm_job* __restrict__ jobs, const double* __restrict__ A,
                                                     const double* __restrict__ B, double* __restrict__ C) {
     pomx_gemm_job const J = jobs[blockIdx.y];
     int const tn = (J.n + 63) / 64, tm = (J.m + 63) / 64;
-    // grid-stride over the job's tiles (uniform per workgroup: the barriers inside stay uniform)
+    // grid-stride over the job's tiles (uniform per workgroup: the barriers inside stay uniform);
+    // UPPER: only tiles that touch the upper triangle (column >= row) of the output
     for(int t = blockIdx.x; t < tm * tn; t += gridDim.x) {
+        int const i0 = (t / tn) * 64, j0 = (t % tn) * 64;
+        if(UPPER && j0 + 63 < i0)
+            continue;
         __syncthreads();
-        gemm_tile<TA, TB>(J, (t / tn) * 64, (t % tn) * 64, A, B, C);
+        gemm_tile<TA, TB>(J, i0, j0, A, B, C);
     }
 }
 
@@ -176,7 +180,7 @@ int launch_gemm1(pomx_ctx* ctx, pomx_gemm_job const& J, const double* d_A, const
 }
 
 int launch_gemm_jobs(pomx_ctx* ctx, int trans_a, int trans_b, const pomx_gemm_job* d_jobs, int njobs, int64_t max_tiles,
-                     const double* d_A, const double* d_B, double* d_C) {
+                     const double* d_A, const double* d_B, double* d_C, bool upper) {
     if(njobs <= 0 || max_tiles <= 0)
         return POMX_OK;
     if(njobs > 65535)
@@ -185,14 +189,18 @@ int launch_gemm_jobs(pomx_ctx* ctx, int trans_a, int trans_b, const pomx_gemm_jo
     // one workgroup per tile
     int64_t const gx = std::max<int64_t>(1, std::min<int64_t>(max_tiles, (32768 + njobs - 1) / njobs));
     dim3 const g{unsigned(gx), unsigned(njobs), 1u};
-    if(!trans_a && !trans_b)
-        gemmj_kernel<0, 0><<<g, 256, 0, ctx->stream>>>(d_jobs, d_A, d_B, d_C);
+    if(upper && !trans_a && trans_b)
+        gemmj_kernel<0, 1, true><<<g, 256, 0, ctx->stream>>>(d_jobs, d_A, d_B, d_C);
+    else if(upper)
+        return pomx_fail(ctx, POMX_ERR_ARG, "launch_gemm_jobs: upper-triangle output only for op(A) op(B)^T");
+    else if(!trans_a && !trans_b)
+        gemmj_kernel<0, 0, false><<<g, 256, 0, ctx->stream>>>(d_jobs, d_A, d_B, d_C);
     else if(!trans_a && trans_b)
-        gemmj_kernel<0, 1><<<g, 256, 0, ctx->stream>>>(d_jobs, d_A, d_B, d_C);
+        gemmj_kernel<0, 1, false><<<g, 256, 0, ctx->stream>>>(d_jobs, d_A, d_B, d_C);
     else if(trans_a && !trans_b)
-        gemmj_kernel<1, 0><<<g, 256, 0, ctx->stream>>>(d_jobs, d_A, d_B, d_C);
+        gemmj_kernel<1, 0, false><<<g, 256, 0, ctx->stream>>>(d_jobs, d_A, d_B, d_C);
     else
-        gemmj_kernel<1, 1><<<g, 256, 0, ctx->stream>>>(d_jobs, d_A, d_B, d_C);
+        gemmj_kernel<1, 1, false><<<g, 256, 0, ctx->stream>>>(d_jobs, d_A, d_B, d_C);
     POMX_CHECK_LAUNCH(ctx);
     return POMX_OK;
 }
