@@ -43,7 +43,7 @@ struct BigBlk {
 
 // Per-block scratch layout (doubles from dbase).
 struct DOff {
-    int64_t V, Z, Q2, U, W, W1, W2, vbuf, wraw, d, e, tau, lam, zs, dsrt, dl, zl, rt, zh, vals, a2, S, T, rowpart, wpart,
+    int64_t V, Z, Q2, U, W, Vt, W1, W2, vbuf, wraw, d, e, tau, lam, zs, dsrt, dl, zl, rt, zh, vals, a2, S, T, rowpart, wpart,
         scal, part, rp, cp, end;
 };
 __host__ __device__ inline DOff doff(int n, int gsym) {
@@ -54,7 +54,8 @@ __host__ __device__ inline DOff doff(int n, int gsym) {
     o.Z = at, at += nn;
     o.Q2 = at, at += nn;
     o.U = at, at += nn;
-    o.W = at, at += int64_t(n) * kNB;
+    o.W = at, at += int64_t(n) * kNB;  // panel w's, reflector-major: W[p][r] at p*n + r
+    o.Vt = at, at += int64_t(n) * kNB; // panel reflectors, reflector-major copy of V
     o.W1 = at, at += int64_t(n) * kNB;
     o.W2 = at, at += int64_t(n) * kNB;
     o.vbuf = at, at += n;
@@ -167,25 +168,23 @@ __global__ void __launch_bounds__(256) trd_row_kernel(const BigBlk* __restrict__
     int const n = B.n, tid = threadIdx.x;
     DOff const o = doff(n, B.gsym);
     double* __restrict__ base = D + B.dbase;
-    const double* __restrict__ Vall = base + o.V;
-    const double* __restrict__ Wp = base + o.W;
+    const double* __restrict__ Vt = base + o.Vt;
+    const double* __restrict__ Wt = base + o.W;
     const double* __restrict__ a2 = base + o.a2;
     if(tid < i) {
-        double const v = Vall[int64_t(j) * n + j0 + tid];
+        double const v = Vt[int64_t(tid) * n + j];
         vj[tid] = v;
         av[tid] = a2[tid];
-        wj[tid] = Wp[int64_t(j) * kNB + tid] + a2[tid] * v;
+        wj[tid] = Wt[int64_t(tid) * n + j] + a2[tid] * v;
     }
     __syncthreads();
     double* __restrict__ row = Aar + B.mat_off + int64_t(j) * n;
     double s = 0.0;
     for(int c = j + blockIdx.x * 256 + tid; c < n; c += gridDim.x * 256) {
         double acc = row[c];
-        const double* wc = Wp + int64_t(c) * kNB;
-        const double* vc = Vall + int64_t(c) * n + j0;
         for(int p = 0; p < i; ++p) {
-            double const v = vc[p];
-            acc -= vj[p] * fma(av[p], v, wc[p]) + wj[p] * v;
+            double const v = Vt[int64_t(p) * n + c];
+            acc -= vj[p] * fma(av[p], v, Wt[int64_t(p) * n + c]) + wj[p] * v;
         }
         row[c] = acc;
         if(c >= j + 2)
@@ -201,7 +200,7 @@ __global__ void __launch_bounds__(256) trd_row_kernel(const BigBlk* __restrict__
 
 // One workgroup per block: the Householder reflector of the updated row tail; v to vbuf and to
 // column j of Vall, d[j], e[j], tau[j].
-__global__ void __launch_bounds__(256) trd_refl_kernel(const BigBlk* __restrict__ blks, int j, int nparts,
+__global__ void __launch_bounds__(256) trd_refl_kernel(const BigBlk* __restrict__ blks, int j, int i, int nparts,
                                                        const double* __restrict__ Aar, double* __restrict__ D) {
     __shared__ double red[4];
     BigBlk const B = blks[blockIdx.y];
@@ -225,7 +224,10 @@ __global__ void __launch_bounds__(256) trd_refl_kernel(const BigBlk* __restrict_
         double v = (c == 0) ? 1.0 : (t != 0.0 ? row[j + 1 + c] * scale : 0.0);
         base[o.vbuf + c] = v;
         base[o.V + int64_t(j + 1 + c) * n + j] = v;
+        base[o.Vt + int64_t(i) * n + j + 1 + c] = v;
     }
+    if(tid == 0)
+        base[o.Vt + int64_t(i) * n + j] = 0.0; // v has no component j
     if(tid == 0) {
         base[o.d + j] = row[j];
         base[o.e + j] = beta;
@@ -314,8 +316,8 @@ __global__ void __launch_bounds__(256) trd_symv_reduce_kernel(const BigBlk* __re
     int const n = B.n;
     DOff const o = doff(n, B.gsym);
     double* __restrict__ base = D + B.dbase;
-    const double* __restrict__ Vall = base + o.V;
-    const double* __restrict__ Wp = base + o.W;
+    const double* __restrict__ Vt = base + o.Vt;
+    const double* __restrict__ Wt = base + o.W;
     const double* __restrict__ v = base + o.vbuf;
     int const m = n - j - 1, jb = j + 1;
     int const ncmax = (m + kSymChunk - 1) / kSymChunk, ncb = (n + kSymChunk - 1) / kSymChunk;
@@ -331,20 +333,25 @@ __global__ void __launch_bounds__(256) trd_symv_reduce_kernel(const BigBlk* __re
     }
     if(i == 0)
         return;
+    // partial dots over a chunk of rows: wavefront per (matrix, p), lanes along the rows
     int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int const chunk = (m + gridDim.x - 1) / gridDim.x;
     int const r0 = blockIdx.x * chunk, r1 = min(m, r0 + chunk);
-    int const p = lane; // p < 32: (W^T v)_p, 32 <= p < 64: (V^T v)_{p-32}
-    double acc = 0.0;
-    if((p & 31) < i)
-        for(int r = r0 + wave; r < r1; r += 4) {
-            double xx = (p < 32) ? Wp[int64_t(jb + r) * kNB + p] : Vall[int64_t(jb + r) * n + j0 + (p - 32)];
-            acc = fma(xx, v[r], acc);
+    for(int q = wave; q < 64; q += 4) { // q < 32: (W^T v)_q, 32 <= q < 64: (V^T v)_{q-32}
+        int const p = q & 31;
+        double acc = 0.0;
+        if(p < i) {
+            const double* x = (q < 32 ? Wt : Vt) + int64_t(p) * n + jb;
+            for(int r = r0 + lane; r < r1; r += 64)
+                acc = fma(x[r], v[r], acc);
         }
-    red[wave][lane] = acc;
+        acc = wave_sum(acc);
+        if(lane == 0)
+            red[0][q] = acc;
+    }
     __syncthreads();
-    if(wave == 0)
-        base[o.part + int64_t(blockIdx.x) * 64 + lane] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    if(threadIdx.x < 64)
+        base[o.part + int64_t(blockIdx.x) * 64 + threadIdx.x] = red[0][threadIdx.x];
 }
 
 // One workgroup per block: t1 = (true W)^T v and t2 = V^T v from the partials.
@@ -379,8 +386,8 @@ __global__ void __launch_bounds__(256) trd_w_kernel(const BigBlk* __restrict__ b
     int const n = B.n, tid = threadIdx.x;
     DOff const o = doff(n, B.gsym);
     double* __restrict__ base = D + B.dbase;
-    const double* __restrict__ Vall = base + o.V;
-    double* __restrict__ Wp = base + o.W;
+    const double* __restrict__ Vt = base + o.Vt;
+    double* __restrict__ Wt = base + o.W;
     int const m = n - j - 1, jb = j + 1;
     double const tj = base[o.tau + j];
     if(tid < i) {
@@ -392,12 +399,10 @@ __global__ void __launch_bounds__(256) trd_w_kernel(const BigBlk* __restrict__ b
     double dot = 0.0;
     for(int r = blockIdx.x * 256 + tid; r < m; r += gridDim.x * 256) {
         double w = base[o.wraw + r];
-        const double* vr = Vall + int64_t(jb + r) * n + j0;
-        const double* wr = Wp + int64_t(jb + r) * kNB;
         for(int p = 0; p < i; ++p)
-            w -= vr[p] * ca[p] + wr[p] * cb[p];
+            w -= Vt[int64_t(p) * n + jb + r] * ca[p] + Wt[int64_t(p) * n + jb + r] * cb[p];
         w *= tj;
-        Wp[int64_t(jb + r) * kNB + i] = w;
+        Wt[int64_t(i) * n + jb + r] = w;
         dot = fma(w, base[o.vbuf + r], dot);
     }
     dot = wave_sum(dot);
@@ -423,7 +428,7 @@ __global__ void __launch_bounds__(256) trd_a2_kernel(const BigBlk* __restrict__ 
         base[o.a2 + i] = -0.5 * base[o.tau + j] * tot;
 }
 
-// End of a panel (blocks listed in ids, panel ending at reflector j): W[r][p] += a2[p] V[r][p]
+// End of a panel (blocks listed in ids, panel ending at reflector j): W[p][r] += a2[p] V[p][r]
 // for the rows of the trailing update.
 __global__ void trd_fold_kernel(const BigBlk* __restrict__ blks, const int* __restrict__ ids, int j, double* __restrict__ D) {
     BigBlk const B = blks[ids[blockIdx.y]];
@@ -433,8 +438,8 @@ __global__ void trd_fold_kernel(const BigBlk* __restrict__ blks, const int* __re
     int const j0 = (j / kNB) * kNB, nb = j - j0 + 1, r0 = j + 1;
     int64_t const cnt = int64_t(n - r0) * nb;
     for(int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < cnt; q += int64_t(gridDim.x) * blockDim.x) {
-        int const r = r0 + int(q / nb), p = int(q % nb);
-        base[o.W + int64_t(r) * kNB + p] += base[o.a2 + p] * base[o.V + int64_t(r) * n + j0 + p];
+        int const p = int(q / (n - r0)), r = r0 + int(q % (n - r0));
+        base[o.W + int64_t(p) * n + r] += base[o.a2 + p] * base[o.Vt + int64_t(p) * n + r];
     }
 }
 
@@ -809,15 +814,15 @@ __global__ void __launch_bounds__(256) dc_colnorm_kernel(const BigBlk* __restric
     BigBlk const B = blks[N.blk];
     int const NB = B.n;
     DOff const od = doff(NB, B.gsym);
-    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for(int jj = blockIdx.x * 4 + wave; jj < k; jj += gridDim.x * 4) {
-        double* u = D + B.dbase + od.U + int64_t(N.o) * NB + N.o + jj;
+    double* U = D + B.dbase + od.U + int64_t(N.o) * NB + N.o;
+    // one thread per column: consecutive threads read consecutive columns of a row
+    for(int jj = blockIdx.x * blockDim.x + threadIdx.x; jj < k; jj += gridDim.x * blockDim.x) {
         double s = 0.0;
-        for(int i = lane; i < k; i += 64)
-            s = fma(u[int64_t(i) * NB], u[int64_t(i) * NB], s);
-        double const inv = 1.0 / sqrt(wave_sum(s));
-        for(int i = lane; i < k; i += 64)
-            u[int64_t(i) * NB] *= inv;
+        for(int i = 0; i < k; ++i)
+            s = fma(U[int64_t(i) * NB + jj], U[int64_t(i) * NB + jj], s);
+        double const inv = 1.0 / sqrt(s);
+        for(int i = 0; i < k; ++i)
+            U[int64_t(i) * NB + jj] *= inv;
     }
 }
 
@@ -1062,21 +1067,21 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
             BigPlan::PanelEnd E{j, int(L.size()), 0, int64_t(ids.size()), int64_t(gj.size())};
             int const j0 = (j / kNB) * kNB, nb = j - j0 + 1, j1 = j + 1;
             int mtmax = 0;
-            for(int b : L) { // A22 -= V2 W2^T
+            for(int b : L) { // A22 -= V2 W2^T  (V2^T, W2^T are rows j1.. of the reflector-major panels)
                 BigBlk const& B = P->h_blk[size_t(b)];
                 int const n = B.n, mt = n - j1;
                 DOff const o = doff(n, B.gsym);
                 ids.push_back(b);
                 mtmax = std::max(mtmax, mt);
-                gj.push_back(pomx_gemm_job{mt, mt, nb, 0, 1, n, kNB, n, B.dbase + o.V + int64_t(j1) * n + j0,
-                                           B.dbase + o.W + int64_t(j1) * kNB, B.mat_off + int64_t(j1) * n + j1, -1.0, 1.0});
+                gj.push_back(pomx_gemm_job{mt, mt, nb, 1, 0, n, n, n, B.dbase + o.Vt + j1, B.dbase + o.W + j1,
+                                           B.mat_off + int64_t(j1) * n + j1, -1.0, 1.0});
             }
             for(int b : L) { // A22 -= W2 V2^T
                 BigBlk const& B = P->h_blk[size_t(b)];
                 int const n = B.n, mt = n - j1;
                 DOff const o = doff(n, B.gsym);
-                gj.push_back(pomx_gemm_job{mt, mt, nb, 0, 1, kNB, n, n, B.dbase + o.W + int64_t(j1) * kNB,
-                                           B.dbase + o.V + int64_t(j1) * n + j0, B.mat_off + int64_t(j1) * n + j1, -1.0, 1.0});
+                gj.push_back(pomx_gemm_job{mt, mt, nb, 1, 0, n, n, n, B.dbase + o.W + j1, B.dbase + o.Vt + j1,
+                                           B.mat_off + int64_t(j1) * n + j1, -1.0, 1.0});
             }
             int64_t const t = (mtmax + 63) / 64;
             E.max_tiles = int(t * t);
@@ -1218,7 +1223,7 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
             unsigned const g3 = std::min(256u, gx_rows(m, 256, nact));
             dim3 const one(1, unsigned(nact));
             trd_row_kernel<<<dim3(g1, unsigned(nact)), 256, 0, s>>>(P->blk, j, j0, i, Aar, P->D);
-            trd_refl_kernel<<<one, 256, 0, s>>>(P->blk, j, int(g1), Aar, P->D);
+            trd_refl_kernel<<<one, 256, 0, s>>>(P->blk, j, i, int(g1), Aar, P->D);
             int64_t const items = int64_t((m + kSymTile - 1) / kSymTile) * ((m + kSymChunk - 1) / kSymChunk);
             unsigned const gs = gx_rows(items, 1, nact);
             if(m <= kLdsV)
@@ -1234,8 +1239,8 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
                 BigPlan::PanelEnd const& E = P->ends[e++];
                 trd_fold_kernel<<<dim3(gx_rows(int64_t(std::max(m, 1)) * kNB, 256, E.nblk), unsigned(E.nblk)), 256, 0, s>>>(
                     P->blk, P->ids + E.ids_off, j, P->D);
-                if((rc = launch_gemm_jobs(ctx, 0, 1, P->gjobs + E.jobs_off, E.nblk, E.max_tiles, P->D, P->D, Aar, true)) ||
-                   (rc = launch_gemm_jobs(ctx, 0, 1, P->gjobs + E.jobs_off + E.nblk, E.nblk, E.max_tiles, P->D, P->D, Aar,
+                if((rc = launch_gemm_jobs(ctx, 1, 0, P->gjobs + E.jobs_off, E.nblk, E.max_tiles, P->D, P->D, Aar, true)) ||
+                   (rc = launch_gemm_jobs(ctx, 1, 0, P->gjobs + E.jobs_off + E.nblk, E.nblk, E.max_tiles, P->D, P->D, Aar,
                                           true)))
                     return rc;
             }
@@ -1265,7 +1270,7 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
         dc_secular_kernel<<<gwave, 256, 0, s>>>(P->blk, nd, st, P->D, P->I);
         dc_zhat_kernel<<<grow, 256, 0, s>>>(P->blk, nd, st, P->D, P->I);
         dc_u_kernel<<<gsq, 256, 0, s>>>(P->blk, nd, st, P->D, P->I);
-        dc_colnorm_kernel<<<gwave, 256, 0, s>>>(P->blk, nd, st, P->D);
+        dc_colnorm_kernel<<<grow, 256, 0, s>>>(P->blk, nd, st, P->D);
         POMX_CHECK_LAUNCH(ctx);
         int64_t const t = (L.nmax + 63) / 64;
         if((rc = launch_gemm_jobs(ctx, 0, 0, jb, L.nnodes, t * t, P->D, P->D, P->D)))

@@ -189,10 +189,10 @@ int launch_gemm_jobs(pomx_ctx* ctx, int trans_a, int trans_b, const pomx_gemm_jo
     // one workgroup per tile
     int64_t const gx = std::max<int64_t>(1, std::min<int64_t>(max_tiles, (32768 + njobs - 1) / njobs));
     dim3 const g{unsigned(gx), unsigned(njobs), 1u};
-    if(upper && !trans_a && trans_b)
-        gemmj_kernel<0, 1, true><<<g, 256, 0, ctx->stream>>>(d_jobs, d_A, d_B, d_C);
+    if(upper && trans_a && !trans_b)
+        gemmj_kernel<1, 0, true><<<g, 256, 0, ctx->stream>>>(d_jobs, d_A, d_B, d_C);
     else if(upper)
-        return pomx_fail(ctx, POMX_ERR_ARG, "launch_gemm_jobs: upper-triangle output only for op(A) op(B)^T");
+        return pomx_fail(ctx, POMX_ERR_ARG, "launch_gemm_jobs: upper-triangle output only for op(A)^T op(B)");
     else if(!trans_a && !trans_b)
         gemmj_kernel<0, 0, false><<<g, 256, 0, ctx->stream>>>(d_jobs, d_A, d_B, d_C);
     else if(!trans_a && trans_b)

@@ -128,7 +128,7 @@ int launch_gemm1(pomx_ctx* ctx, pomx_gemm_job const& J, const double* d_A, const
 // Batched GEMM, job blockIdx.y, tiles along x (max_tiles = largest job's 64 x 64 tile count);
 // job sizes may be written on the device before the launch.
 // upper: only output tiles touching the upper triangle (for symmetric updates of square outputs
-// that start on the diagonal; NT only).
+// that start on the diagonal; TN only).
 int launch_gemm_jobs(pomx_ctx* ctx, int trans_a, int trans_b, const pomx_gemm_job* d_jobs, int njobs, int64_t max_tiles,
                      const double* d_A, const double* d_B, double* d_C, bool upper = false);
 
