@@ -13,7 +13,9 @@
 #include <pybind11/stl.h>
 
 #include <chrono>
+#include <map>
 #include <memory>
+#include <set>
 
 namespace py = pybind11;
 using namespace Pomerol;
@@ -82,15 +84,31 @@ struct ED {
         rho->compute();
     }
 
+    std::set<ParticleIndex> field_subset; // empty: every index (FieldOperatorContainer's default)
+
     FieldOperatorContainer& field_ops() {
         if(!ops) {
             auto t0 = std::chrono::steady_clock::now();
-            ops.reset(new FieldOperatorContainer(*info, *HS, *S, *H));
+            ops.reset(new FieldOperatorContainer(*info, *HS, *S, *H, field_subset));
             ops->prepareAll(*HS);
             ops->computeAll();
             t_rot = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         }
         return *ops;
+    }
+
+    // Rotated monomial operators, computed once per session (susceptibilities share them).
+    std::map<std::string, std::unique_ptr<MonomialOperator>> mono_cache;
+    MonomialOperator& mono(py::list ops_) {
+        std::string key = py::str(ops_);
+        auto it = mono_cache.find(key);
+        if(it == mono_cache.end()) {
+            auto op = std::make_unique<MonomialOperator>(mono_expr(ops_), *HS, *S, *H);
+            op->prepare(*HS);
+            op->compute();
+            it = mono_cache.emplace(key, std::move(op)).first;
+        }
+        return *it->second;
     }
 
     Expr mono_expr(py::list ops_) {
@@ -198,7 +216,6 @@ PYBIND11_MODULE(_pomerol, m) {
     py::class_<ED>(m, "ED")
         .def(py::init<py::list, double, bool>(), py::arg("terms"), py::arg("beta"), py::arg("keep_hmat") = false)
         .def_readonly("t_eigh", &ED::t_eigh)
-        .def_readonly("t_rot", &ED::t_rot)
         .def_property_readonly("E0", [](ED& e) { return e.H->getGroundEnergy(); })
         .def_property_readonly("Z", [](ED& e) { return e.rho->getPartitionFunction(); })
         .def("indices", [](ED& e) {
@@ -227,6 +244,14 @@ PYBIND11_MODULE(_pomerol, m) {
             return to_np(std::vector<double>(w.begin(), w.end()));
         })
         .def("truncate_blocks", [](ED& e, double tol) { e.rho->truncateBlocks(tol, false); })
+        .def("use_indices", [](ED& e, std::vector<int> idx) {
+            if(e.ops)
+                throw std::runtime_error("use_indices: field operators already computed");
+            e.field_subset.clear();
+            for(int i : idx)
+                e.field_subset.insert(ParticleIndex(i));
+        })
+        .def_readonly("t_rot", &ED::t_rot)
         .def("retained", [](ED& e) {
             std::vector<bool> r;
             for(BlockNumber b = 0; b < e.S->getNumberOfBlocks(); ++b)
@@ -294,10 +319,8 @@ PYBIND11_MODULE(_pomerol, m) {
             return to_np(v);
         })
         .def("chi", [](ED& e, py::list A, py::list B, py::array_t<std::complex<double>> z, double pr, double ct, bool sub) {
-            MonomialOperator MA(e.mono_expr(A), *e.HS, *e.S, *e.H), MB(e.mono_expr(B), *e.HS, *e.S, *e.H);
-            MA.prepare(*e.HS);
-            MB.prepare(*e.HS);
-            MonomialOperator::computeBatch({&MA, &MB}, 1e-8);
+            MonomialOperator& MA = e.mono(A);
+            MonomialOperator& MB = e.mono(B);
             Susceptibility X(*e.S, *e.H, MA, MB, *e.rho);
             X.PoleResolution = pr;
             X.CoefficientTolerance = ct;

@@ -84,3 +84,46 @@ def grid_triplets(nf, nb, beta):
 def rel_err(x, ref):
     x, ref = np.asarray(x), np.asarray(ref)
     return float(np.max(np.abs(x - ref)) / max(np.max(np.abs(ref)), 1e-300))
+
+
+def sector_hamiltonian(terms, indices, n_up, n_dn):
+    """Sparse Hamiltonian of one (N_up, N_dn) sector from a term list, vectorised over the sector's
+    Fock states (bit i = mode i, c/c^dag sign (-1)^(occupied modes below i)); an independent check
+    of the block eigensolver through scipy's Lanczos."""
+    import itertools
+
+    import scipy.sparse as sp
+
+    mode = {tuple(t): k for k, t in enumerate(indices)}
+    up = [k for k, t in enumerate(indices) if t[2] == 1]
+    dn = [k for k, t in enumerate(indices) if t[2] == 0]
+    states = []
+    for cu in itertools.combinations(up, n_up):
+        for cd in itertools.combinations(dn, n_dn):
+            states.append(sum(1 << k for k in cu + cd))
+    states = np.array(sorted(states), dtype=np.int64)
+    rows, cols, vals = [], [], []
+    for coeff, mono in terms:
+        t = states.copy()
+        sign = np.ones(len(states))
+        ok = np.ones(len(states), dtype=bool)
+        for dag, idx in reversed(mono):
+            k = mode[tuple(idx)]
+            bit = np.int64(1) << k
+            occ = (t & bit) != 0
+            ok &= occ != dag
+            below = t & (bit - 1)
+            par = np.zeros(len(states), dtype=np.int64)
+            x = below.copy()
+            while np.any(x):
+                par ^= x & 1
+                x >>= 1
+            sign = np.where(par == 1, -sign, sign)
+            t = t ^ bit
+        tgt = np.searchsorted(states, t[ok])
+        assert np.all(states[tgt] == t[ok]), "term leaves the sector"
+        rows.append(tgt)
+        cols.append(np.nonzero(ok)[0])
+        vals.append(coeff * sign[ok])
+    H = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(len(states),) * 2)
+    return H, states
