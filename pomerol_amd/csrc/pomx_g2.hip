@@ -348,15 +348,14 @@ __global__ void kind_count_kernel(int64_t ngroups, const int64_t* __restrict__ l
 // every wave then reads each term with a uniform LDS address and evaluates it for all its
 // lanes' frequencies.  Partial sums per slice are reduced in a fixed order (deterministic).
 constexpr int kChunk = 256;
-constexpr int kFPT = 4; // frequencies per thread
+constexpr int kFPT = 8; // frequencies per thread (8 independent accumulator pairs per lane)
 
 __device__ inline double rcp_nr(double x) {
-    // hardware reciprocal estimate refined by two Newton-Raphson steps (full FP64 accuracy)
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
+    // v_rcp_f64 estimate (relative error <= 2^-24.4 on gfx950, measured) refined by one
+    // Newton-Raphson step: relative error <= ~2.2e-15, far below the 1e-10 bar of the summed
+    // correlators, for one FMA pair less per term and frequency
+    double const r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
 }
 
 __constant__ int c_perm3[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
@@ -487,17 +486,32 @@ __global__ void __launch_bounds__(256) g2_eval_kernel(int64_t nterms, int64_t sl
             } else if(IMAG) {
                 double const Rc = sT[0][k], Nc = sT[1][k], a = sT[2][k], c = sT[3][k], ac = sT[4][k], e = sT[5][k],
                              e2 = sT[6][k];
+                if(e2 > res2) {
+                    // |E|^2 = e^2 + Im(E)^2 > res^2 at every frequency: no resonance for this term
+                    // (the common case; the branch is uniform because every lane reads term k)
 #pragma unroll
-                for(int u = 0; u < kFPT; ++u) {
-                    double const Qr = ac - fa[u];              // Re((a+ix)(c+iw))
-                    double const Qi = fma(a, w[u], c * x[u]);  // Im
-                    bool const resonant = e2 + fc[u] <= res2;  // |E|^2, E = e + i fb
-                    double const Gr = resonant ? Qr : fma(e, Qr, -fb[u] * Qi);
-                    double const Gi = resonant ? Qi : fma(e, Qi, fb[u] * Qr);
-                    double const K = resonant ? Rc : Nc;
-                    double const s = K * rcp_nr(fma(Gr, Gr, Gi * Gi));
-                    ar[u] = fma(s, Gr, ar[u]);
-                    ai[u] = fma(-s, Gi, ai[u]);
+                    for(int u = 0; u < kFPT; ++u) {
+                        double const Qr = ac - fa[u];             // Re((a+ix)(c+iw))
+                        double const Qi = fma(a, w[u], c * x[u]); // Im
+                        double const Gr = fma(e, Qr, -fb[u] * Qi);
+                        double const Gi = fma(e, Qi, fb[u] * Qr);
+                        double const s = Nc * rcp_nr(fma(Gr, Gr, Gi * Gi));
+                        ar[u] = fma(s, Gr, ar[u]);
+                        ai[u] = fma(-s, Gi, ai[u]);
+                    }
+                } else {
+#pragma unroll
+                    for(int u = 0; u < kFPT; ++u) {
+                        double const Qr = ac - fa[u];
+                        double const Qi = fma(a, w[u], c * x[u]);
+                        bool const resonant = e2 + fc[u] <= res2; // |E|^2, E = e + i fb
+                        double const Gr = resonant ? Qr : fma(e, Qr, -fb[u] * Qi);
+                        double const Gi = resonant ? Qi : fma(e, Qi, fb[u] * Qr);
+                        double const K = resonant ? Rc : Nc;
+                        double const s = K * rcp_nr(fma(Gr, Gr, Gi * Gi));
+                        ar[u] = fma(s, Gr, ar[u]);
+                        ai[u] = fma(-s, Gi, ai[u]);
+                    }
                 }
             } else if(KIND < 2) {
                 double const C = sT[0][k], P1 = sT[1][k], P2 = sT[2][k], P3 = sT[3][k];
