@@ -348,7 +348,7 @@ __global__ void kind_count_kernel(int64_t ngroups, const int64_t* __restrict__ l
 // every wave then reads each term with a uniform LDS address and evaluates it for all its
 // lanes' frequencies.  Partial sums per slice are reduced in a fixed order (deterministic).
 constexpr int kChunk = 256;
-constexpr int kFPT = 8; // frequencies per thread (8 independent accumulator pairs per lane)
+constexpr int kFPT = 8; // frequencies per thread (8 independent accumulator pairs per lane; multiple of 4)
 
 __device__ inline double rcp_nr(double x) {
     // v_rcp_f64 estimate (relative error <= 2^-24.4 on gfx950, measured) refined by one
@@ -472,16 +472,30 @@ __global__ void __launch_bounds__(256) g2_eval_kernel(int64_t nterms, int64_t sl
         for(int k = 0; k < cnt; ++k) {
             if(IMAG && KIND < 2) {
                 double const C = sT[0][k], a = sT[1][k], b = sT[2][k], c = sT[3][k], ab = sT[4][k];
+                // four frequencies share one reciprocal: with r = 1/(q0 q1 q2 q3),
+                // 1/q0 = q1 (q2 q3) r and so on (|D|^2 lies within ~1e-20..1e10 here, so the
+                // product of four stays far inside the double range)
 #pragma unroll
-                for(int u = 0; u < kFPT; ++u) {
-                    double const yy = (KIND == 0) ? y[u] : fb[u];
-                    double const X = ab - fa[u];             // Re((a+ix)(b+iy))
-                    double const Y = fma(a, yy, b * x[u]);   // Im
-                    double const Dr = fma(X, c, -Y * w[u]);
-                    double const Di = fma(X, w[u], Y * c);
-                    double const s = C * rcp_nr(fma(Dr, Dr, Di * Di));
-                    ar[u] = fma(s, Dr, ar[u]);
-                    ai[u] = fma(-s, Di, ai[u]);
+                for(int u = 0; u < kFPT; u += 4) {
+                    double Dr[4], Di[4], q[4];
+#pragma unroll
+                    for(int h = 0; h < 4; ++h) {
+                        double const yy = (KIND == 0) ? y[u + h] : fb[u + h];
+                        double const X = ab - fa[u + h];           // Re((a+ix)(b+iy))
+                        double const Y = fma(a, yy, b * x[u + h]); // Im
+                        Dr[h] = fma(X, c, -Y * w[u + h]);
+                        Di[h] = fma(X, w[u + h], Y * c);
+                        q[h] = fma(Dr[h], Dr[h], Di[h] * Di[h]);
+                    }
+                    double const q01 = q[0] * q[1], q23 = q[2] * q[3];
+                    double const r = C * rcp_nr(q01 * q23);
+                    double const r01 = q23 * r, r23 = q01 * r;
+                    double const s[4] = {q[1] * r01, q[0] * r01, q[3] * r23, q[2] * r23};
+#pragma unroll
+                    for(int h = 0; h < 4; ++h) {
+                        ar[u + h] = fma(s[h], Dr[h], ar[u + h]);
+                        ai[u + h] = fma(-s[h], Di[h], ai[u + h]);
+                    }
                 }
             } else if(IMAG) {
                 double const Rc = sT[0][k], Nc = sT[1][k], a = sT[2][k], c = sT[3][k], ac = sT[4][k], e = sT[5][k],
@@ -490,14 +504,25 @@ __global__ void __launch_bounds__(256) g2_eval_kernel(int64_t nterms, int64_t sl
                     // |E|^2 = e^2 + Im(E)^2 > res^2 at every frequency: no resonance for this term
                     // (the common case; the branch is uniform because every lane reads term k)
 #pragma unroll
-                    for(int u = 0; u < kFPT; ++u) {
-                        double const Qr = ac - fa[u];             // Re((a+ix)(c+iw))
-                        double const Qi = fma(a, w[u], c * x[u]); // Im
-                        double const Gr = fma(e, Qr, -fb[u] * Qi);
-                        double const Gi = fma(e, Qi, fb[u] * Qr);
-                        double const s = Nc * rcp_nr(fma(Gr, Gr, Gi * Gi));
-                        ar[u] = fma(s, Gr, ar[u]);
-                        ai[u] = fma(-s, Gi, ai[u]);
+                    for(int u = 0; u < kFPT; u += 4) { // four frequencies per reciprocal, as above
+                        double Gr[4], Gi[4], q[4];
+#pragma unroll
+                        for(int h = 0; h < 4; ++h) {
+                            double const Qr = ac - fa[u + h];                 // Re((a+ix)(c+iw))
+                            double const Qi = fma(a, w[u + h], c * x[u + h]); // Im
+                            Gr[h] = fma(e, Qr, -fb[u + h] * Qi);
+                            Gi[h] = fma(e, Qi, fb[u + h] * Qr);
+                            q[h] = fma(Gr[h], Gr[h], Gi[h] * Gi[h]);
+                        }
+                        double const q01 = q[0] * q[1], q23 = q[2] * q[3];
+                        double const r = Nc * rcp_nr(q01 * q23);
+                        double const r01 = q23 * r, r23 = q01 * r;
+                        double const sv[4] = {q[1] * r01, q[0] * r01, q[3] * r23, q[2] * r23};
+#pragma unroll
+                        for(int h = 0; h < 4; ++h) {
+                            ar[u + h] = fma(sv[h], Gr[h], ar[u + h]);
+                            ai[u + h] = fma(-sv[h], Gi[h], ai[u + h]);
+                        }
                     }
                 } else {
 #pragma unroll
