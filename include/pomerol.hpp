@@ -10,3 +10,4 @@
 #include "pomerol/GreensFunction.hpp"
 #include "pomerol/TwoParticleGF.hpp"
 #include "pomerol/Batch.hpp"
+#include "pomerol/Tangent.hpp"

@@ -19,6 +19,8 @@
 
 namespace Pomerol {
 
+class SpectrumTangent;
+
 using FreqTuple3 = std::tuple<ComplexType, ComplexType, ComplexType>;
 using FreqVec3 = std::vector<FreqTuple3>;
 
@@ -67,6 +69,7 @@ private:
 
     pomx_g2_part devicePart() const;
     void run(int nfreq, double const* d_freq, double* d_acc, bool keep);
+    void runTangent(SpectrumTangent& T, int nfreq, double const* d_freq, double* d_acc, double* d_dacc);
 
 public:
     TwoParticleGFPart(MonomialOperatorPart const& O1, MonomialOperatorPart const& O2, MonomialOperatorPart const& O3,
@@ -122,6 +125,11 @@ public:
     std::vector<ComplexType> compute(bool clear = false, FreqVec3 const& freqs = {}, MPI_Comm const& comm = MPI_COMM_WORLD);
     /// As compute(), restricted to the listed parts (bounded samples of very large workloads).
     std::vector<ComplexType> computeParts(std::vector<std::size_t> const& which, bool clear, FreqVec3 const& freqs);
+    /// Derivative along the direction of T (computed) at the triplets, from the terms compute()
+    /// keeps, with their thresholds and merge groups held fixed; *values (if given) receives the
+    /// value from the same terms.  No counterpart in the reference.
+    std::vector<ComplexType> computeTangent(SpectrumTangent& T, FreqVec3 const& freqs,
+                                            std::vector<ComplexType>* values = nullptr);
     ParticleIndex getIndex(std::size_t Position) const;
     ComplexType operator()(ComplexType z1, ComplexType z2, ComplexType z3) const;
     ComplexType operator()(long MatsubaraNumber1, long MatsubaraNumber2, long MatsubaraNumber3) const;

@@ -218,6 +218,36 @@ int pomx_batch_set_values(pomx_ctx* ctx, pomx_batch* batch, const double* h_val)
  * gradient) -> Hamiltonian entries.  Terms dropped by |R| <= coeff_tol carry no gradient. */
 int pomx_batch_adjoint(pomx_ctx* ctx, pomx_batch* batch, const double* h_Gbar, double* h_dval);
 
+/* ---- tangents: first-order response along a Hamiltonian direction ------------------------
+ * No counterpart in the reference (forward-mode differentiation of the correlators with respect
+ * to one Hamiltonian parameter t, H(t) = H + t dH at t = 0).
+ *
+ * pomx_eigh_tangent: for blocks already solved by pomx_eigh_batched (eigenvectors as columns of
+ * d_U, eigenvalues in d_E) and the direction dH assembled in the same arena layout (d_dH),
+ *   A = U^T dH U,  d_dE[i] = A_ii,  d_K = K with K_ij = A_ij / (E_j - E_i) for i != j
+ * (dU = U K; K is antisymmetric, zero on the diagonal and between degenerate levels,
+ * |E_i - E_j| <= 1e-10 max(1,|E_i|,|E_j|)).  d_K uses the arena layout of d_U.  A rotated operator
+ * O = U_L^T M U_R then moves as dO = O K_R - K_L O (pomx_gemm_batched). */
+int pomx_eigh_tangent(pomx_ctx* ctx, int nblocks, const int* h_n, const int64_t* h_mat_off, const int64_t* h_val_off,
+                      const double* d_U, const double* d_E, const double* d_dH, double* d_K, double* d_dE);
+/* Boltzmann weights w = exp(-beta E)/Z (pomx_boltzmann) move as dw = -beta w (dE - sum w dE). */
+int pomx_boltzmann_tangent(pomx_ctx* ctx, int64_t nstates, const double* d_w, const double* d_dE, double beta,
+                           double* d_dw);
+/* Tangent of one two-particle GF part: the derivatives of its rotated operators, eigenvalues and
+ * weights (same layouts as the part's own). */
+typedef struct {
+    pomx_view dO1, dO2, dO3, dCX4;
+    const double *dE1, *dE2, *dE3, *dE4;
+    const double *dw1, *dw2, *dw3, *dw4;
+} pomx_g2_tangent;
+/* Value and derivative of a two-particle GF part at the triplets d_freq: the terms are generated,
+ * thresholded and merged exactly as by pomx_g2_part_compute (the kept set and the groups are
+ * held fixed), each merged group carrying the derivative of its summed coefficients and of its
+ * mean poles; d_acc += value, d_dacc += derivative (nfreq complex each). */
+int pomx_g2_part_tangent(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* tangent, double beta,
+                         double pole_res, double coeff_tol, int nfreq, const double* d_freq, double* d_acc,
+                         double* d_dacc, int64_t* h_counts, int64_t* h_raw_terms);
+
 /* ---- timing of the dominant kernel (HIP events on the context stream) ---------------- */
 int pomx_eval_timing(pomx_ctx* ctx, int enable, double* h_total_ms, int64_t* h_launches);
 

@@ -31,6 +31,13 @@ struct G2Dev {
     double sign, beta, tol;
 };
 
+// derivatives of a part's inputs along a Hamiltonian direction (pomx_g2_tangent)
+struct G2Tan {
+    pomx_view dO1, dO2, dO3, dCX4;
+    const double *dE1, *dE2, *dE3, *dE4;
+    const double *dw1, *dw2, *dw3, *dw4;
+};
+
 __device__ inline double mat(pomx_view const& v, int r, int c) { return v.base[r * v.rs + c * v.cs]; }
 
 // ---- 1. rank tables -------------------------------------------------------------------------
@@ -315,6 +322,122 @@ __global__ void g2_compact_kernel(int64_t ngroups, const int64_t* __restrict__ l
         rs[2 * r_cap + at] = g[gi * 5 + 2];
         rs[3 * r_cap + at] = g[gi * 5 + 3];
         rs[4 * r_cap + at] = g[gi * 5 + 4];
+    }
+}
+
+// Tangent of g2_reduce_kernel: the same groups, sums, erasures and pole means (computed in the
+// same order, so the values match the forward path bit for bit), each group also carrying the
+// derivative of its summed coefficients and of its mean poles.  g holds 10 doubles per group:
+// c0, c1, P1, P2, P3, dc0, dc1, dP1, dP2, dP3.
+__global__ void g2_reduce_tangent_kernel(G2Dev D, G2Tan T, int64_t n, int64_t ngroups, const int64_t* __restrict__ start,
+                                         const uint64_t* __restrict__ keys, const uint64_t* __restrict__ vals, double thr,
+                                         double* __restrict__ g, int64_t* __restrict__ live) {
+    int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(gi >= ngroups)
+        return;
+    int64_t s = start[gi], e = (gi + 1 < ngroups) ? start[gi + 1] : n;
+    int kind = int(keys[s] >> 62);
+    bool alive = false;
+    double c0 = 0, c1 = 0, p1 = 0, p2 = 0, p3 = 0;
+    double dc0 = 0, dc1 = 0, dp1 = 0, dp2 = 0, dp3 = 0;
+    long cnt = 0;
+    for(int64_t k = s; k < e; ++k) {
+        uint64_t v = vals[k];
+        int i1 = int(v & 0xffff), i2 = int((v >> 16) & 0xffff), i3 = int((v >> 32) & 0xffff), i4 = int(v >> 48);
+        double const o1 = mat(D.O1, i1, i2), o2 = mat(D.O2, i2, i3), o3 = mat(D.O3, i3, i4), o4 = mat(D.CX4, i4, i1);
+        double const C = o1 * o2 * o3 * o4 * D.sign;
+        double const dC = (mat(T.dO1, i1, i2) * o2 * o3 * o4 + o1 * mat(T.dO2, i2, i3) * o3 * o4 +
+                           o1 * o2 * mat(T.dO3, i3, i4) * o4 + o1 * o2 * o3 * mat(T.dCX4, i4, i1)) *
+                          D.sign;
+        double wi = D.w1[i1], wj = D.w2[i2], wk = D.w3[i3], wl = D.w4[i4];
+        double dwi = T.dw1[i1], dwj = T.dw2[i2], dwk = T.dw3[i3], dwl = T.dw4[i4];
+        double a, b = 0.0, da, db = 0.0;
+        if(kind == 0) {
+            a = -C * (wj + wk);
+            da = -(dC * (wj + wk) + C * (dwj + dwk));
+        } else if(kind == 1) {
+            a = C * (wi + wl);
+            da = dC * (wi + wl) + C * (dwi + dwl);
+        } else if(kind == 2) {
+            a = -C * D.beta * wj;
+            b = C * (wj - wl);
+            da = -D.beta * (dC * wj + C * dwj);
+            db = dC * (wj - wl) + C * (dwj - dwl);
+        } else {
+            a = C * D.beta * wi;
+            b = C * (wk - wi);
+            da = D.beta * (dC * wi + C * dwi);
+            db = dC * (wk - wi) + C * (dwk - dwi);
+        }
+        double P1 = D.E2[i2] - D.E1[i1], P2 = D.E3[i3] - D.E2[i2], P3 = D.E4[i4] - D.E3[i3];
+        double dP1 = T.dE2[i2] - T.dE1[i1], dP2 = T.dE3[i3] - T.dE2[i2], dP3 = T.dE4[i4] - T.dE3[i3];
+        if(!alive) {
+            alive = true;
+            c0 = a;
+            c1 = b;
+            p1 = P1;
+            p2 = P2;
+            p3 = P3;
+            dc0 = da;
+            dc1 = db;
+            dp1 = dP1;
+            dp2 = dP2;
+            dp3 = dP3;
+            cnt = 1;
+        } else {
+            c0 += a;
+            c1 += b;
+            p1 += P1;
+            p2 += P2;
+            p3 += P3;
+            dc0 += da;
+            dc1 += db;
+            dp1 += dP1;
+            dp2 += dP2;
+            dp3 += dP3;
+            ++cnt;
+            if(fabs(c0) <= thr && fabs(c1) <= thr)
+                alive = false;
+        }
+    }
+    double inv = 1.0 / double(cnt);
+    double* o = g + gi * 10;
+    o[0] = c0;
+    o[1] = c1;
+    o[2] = p1 * inv;
+    o[3] = p2 * inv;
+    o[4] = p3 * inv;
+    o[5] = dc0;
+    o[6] = dc1;
+    o[7] = dp1 * inv;
+    o[8] = dp2 * inv;
+    o[9] = dp3 * inv;
+    live[gi] = alive ? 1 : 0;
+}
+
+// Live tangent groups into kind-segmented SoA lists: NR (c, P1, P2, P3, dc, dP1, dP2, dP3),
+// R (Rc, Nc, P1, P2, P3, dRc, dNc, dP1, dP2, dP3).
+__global__ void g2_compact_tangent_kernel(int64_t ngroups, const int64_t* __restrict__ live,
+                                          const int64_t* __restrict__ lpos, const double* __restrict__ g,
+                                          const int64_t* __restrict__ start, const uint64_t* __restrict__ keys,
+                                          int64_t nr_cap, int64_t r_cap, const int64_t* __restrict__ kind_first,
+                                          double* __restrict__ nr, double* __restrict__ rs) {
+    int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(gi >= ngroups || !live[gi])
+        return;
+    int kind = int(keys[start[gi]] >> 62);
+    int64_t at = lpos[gi];
+    const double* x = g + gi * 10;
+    if(kind < 2) {
+        int const f[8] = {0, 2, 3, 4, 5, 7, 8, 9};
+#pragma unroll
+        for(int q = 0; q < 8; ++q)
+            nr[q * nr_cap + at] = x[f[q]];
+    } else {
+        at -= kind_first[2];
+#pragma unroll
+        for(int q = 0; q < 10; ++q)
+            rs[q * r_cap + at] = x[q];
     }
 }
 
@@ -608,6 +731,136 @@ __global__ void g2_eval_reduce_kernel(int nfreq, int nslices, const double* __re
     acc[2 * f + 1] += si;
 }
 
+// ---- tangent evaluation ---------------------------------------------------------------------
+// Value f and derivative df of every merged term at every triplet, with the poles' derivatives
+// entering through d(1/(z - P)) = dP / (z - P)^2:
+//   NR:  f = c g, g = 1/(D1 D2 D3),  df = g (dc + c (dP1/D1 + dP2'/D2 + dP3/D3)),
+//        D2 = z2 - P2 (dP2' = dP2) or z1 + z2 + z3 - P1 - P2 - P3 (dP2' = dP1 + dP2 + dP3);
+//   R:   D = z2 + z3 - P2 - P3 (or z1 + z2 - P1 - P2), q = 1/(D1 D3); if |D| <= res
+//        f = Rc q, df = q (dRc + Rc (dP1/D1 + dP3/D3)), else
+//        f = Nc q/D, df = (q/D) (dNc + Nc (dP1/D1 + dP3/D3 + dPD/D)).
+// One thread per kJF triplets, terms staged through LDS, per-slice partials (value, derivative)
+// reduced in a fixed order.
+constexpr int kJF = 4;
+
+struct Cx {
+    double r, i;
+};
+__device__ inline Cx cmul(Cx a, Cx b) { return Cx{a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+__device__ inline Cx crcp(double r, double i) {
+    double const s = 1.0 / (r * r + i * i);
+    return Cx{r * s, -i * s};
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(256) g2_tangent_eval_kernel(int64_t nterms, int64_t slice_len,
+                                                              const double* __restrict__ T, int64_t cap, int nfreq,
+                                                              const double* __restrict__ fz, int perm, double res,
+                                                              double* __restrict__ partial) {
+    constexpr int NF = KIND < 2 ? 8 : 10;
+    __shared__ double sT[NF][kChunk];
+    int const fbase = (blockIdx.x * blockDim.x + threadIdx.x) * kJF;
+    int64_t const t0 = int64_t(blockIdx.y) * slice_len;
+    int64_t const t1 = min(nterms, t0 + slice_len);
+    double const res2 = res * res;
+    Freq q[kJF];
+    double acc[kJF][4];
+#pragma unroll
+    for(int u = 0; u < kJF; ++u) {
+        q[u] = load_freq(fz, min(fbase + u, nfreq - 1), perm);
+#pragma unroll
+        for(int h = 0; h < 4; ++h)
+            acc[u][h] = 0.0;
+    }
+    for(int64_t c0 = t0; c0 < t1; c0 += kChunk) {
+        int const cnt = int(min<int64_t>(kChunk, t1 - c0));
+        __syncthreads();
+        if(int(threadIdx.x) < cnt)
+#pragma unroll
+            for(int f = 0; f < NF; ++f)
+                sT[f][threadIdx.x] = T[f * cap + c0 + threadIdx.x];
+        __syncthreads();
+        for(int k = 0; k < cnt; ++k) {
+            if(KIND < 2) {
+                double const C = sT[0][k], P1 = sT[1][k], P2 = sT[2][k], P3 = sT[3][k];
+                double const dC = sT[4][k], dP1 = sT[5][k], dP2 = sT[6][k], dP3 = sT[7][k];
+                double const dQ2 = (KIND == 0) ? dP2 : (dP1 + dP2) + dP3;
+#pragma unroll
+                for(int u = 0; u < kJF; ++u) {
+                    Freq const& z = q[u];
+                    Cx const i1 = crcp(z.z1r - P1, z.z1i);
+                    Cx const i3 = crcp(z.z3r - P3, z.z3i);
+                    Cx const i2 = (KIND == 0) ? crcp(z.z2r - P2, z.z2i)
+                                              : crcp(z.z1r + z.z2r + z.z3r - P1 - P2 - P3, z.z1i + z.z2i + z.z3i);
+                    Cx const gg = cmul(cmul(i1, i2), i3);
+                    // s = dc + c (dP1 i1 + dQ2 i2 + dP3 i3)
+                    double const sr = fma(C, fma(dP1, i1.r, fma(dQ2, i2.r, dP3 * i3.r)), dC);
+                    double const si = C * fma(dP1, i1.i, fma(dQ2, i2.i, dP3 * i3.i));
+                    acc[u][0] = fma(C, gg.r, acc[u][0]);
+                    acc[u][1] = fma(C, gg.i, acc[u][1]);
+                    acc[u][2] += gg.r * sr - gg.i * si;
+                    acc[u][3] += gg.r * si + gg.i * sr;
+                }
+            } else {
+                double const Rc = sT[0][k], Nc = sT[1][k], P1 = sT[2][k], P2 = sT[3][k], P3 = sT[4][k];
+                double const dRc = sT[5][k], dNc = sT[6][k], dP1 = sT[7][k], dP2 = sT[8][k], dP3 = sT[9][k];
+                double const dPD = (KIND == 3) ? dP1 + dP2 : dP2 + dP3;
+#pragma unroll
+                for(int u = 0; u < kJF; ++u) {
+                    Freq const& z = q[u];
+                    Cx const i1 = crcp(z.z1r - P1, z.z1i);
+                    Cx const i3 = crcp(z.z3r - P3, z.z3i);
+                    double const Er = (KIND == 3) ? z.z1r + z.z2r - P1 - P2 : z.z2r + z.z3r - P2 - P3;
+                    double const Ei = (KIND == 3) ? z.z1i + z.z2i : z.z2i + z.z3i;
+                    Cx gg = cmul(i1, i3);
+                    double K, dK, xr = fma(dP1, i1.r, dP3 * i3.r), xi = fma(dP1, i1.i, dP3 * i3.i);
+                    if(fma(Er, Er, Ei * Ei) <= res2) {
+                        K = Rc;
+                        dK = dRc;
+                    } else {
+                        Cx const iE = crcp(Er, Ei);
+                        gg = cmul(gg, iE);
+                        K = Nc;
+                        dK = dNc;
+                        xr = fma(dPD, iE.r, xr);
+                        xi = fma(dPD, iE.i, xi);
+                    }
+                    double const sr = fma(K, xr, dK), si = K * xi;
+                    acc[u][0] = fma(K, gg.r, acc[u][0]);
+                    acc[u][1] = fma(K, gg.i, acc[u][1]);
+                    acc[u][2] += gg.r * sr - gg.i * si;
+                    acc[u][3] += gg.r * si + gg.i * sr;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for(int u = 0; u < kJF; ++u) {
+        int f = fbase + u;
+        if(f < nfreq) {
+            int64_t o = (int64_t(blockIdx.y) * nfreq + f) * 4;
+#pragma unroll
+            for(int h = 0; h < 4; ++h)
+                partial[o + h] += acc[u][h];
+        }
+    }
+}
+
+__global__ void g2_tangent_reduce_kernel(int nfreq, int nslices, const double* __restrict__ partial,
+                                         double* __restrict__ acc, double* __restrict__ dacc) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if(f >= nfreq)
+        return;
+    double s[4] = {0, 0, 0, 0};
+    for(int sl = 0; sl < nslices; ++sl)
+        for(int h = 0; h < 4; ++h)
+            s[h] += partial[(int64_t(sl) * nfreq + f) * 4 + h];
+    acc[2 * f] += s[0];
+    acc[2 * f + 1] += s[1];
+    dacc[2 * f] += s[2];
+    dacc[2 * f + 1] += s[3];
+}
+
 // 1 if some frequency has a non-zero real part
 __global__ void any_real_kernel(int n, const double* __restrict__ fz, int* __restrict__ flag) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -674,6 +927,42 @@ int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res,
     return POMX_OK;
 }
 
+int launch_g2_tangent_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res, int nfreq,
+                           const double* d_freq, double* d_acc, double* d_dacc) {
+    int64_t total = 0;
+    for(int k = 0; k < 4; ++k)
+        total += spans[k].count;
+    if(total == 0 || nfreq == 0)
+        return POMX_OK;
+    int ftiles = (nfreq + 256 * kJF - 1) / (256 * kJF);
+    int64_t want = std::max<int64_t>(1, 4096 / ftiles);
+    int64_t slice = std::max<int64_t>(kChunk, (total + want - 1) / want);
+    slice = (slice + kChunk - 1) / kChunk * kChunk;
+    int nslices = 1;
+    for(int k = 0; k < 4; ++k)
+        nslices = std::max<int64_t>(nslices, (spans[k].count + slice - 1) / slice);
+    WsBuf<double> partial(ctx, size_t(nslices) * nfreq * 4);
+    int rc = pomx_fill_zero(ctx, partial.p, sizeof(double) * size_t(nslices) * nfreq * 4);
+    if(rc)
+        return rc;
+    for(int k = 0; k < 4; ++k) {
+        TermSpan const& s = spans[k];
+        if(!s.count)
+            continue;
+        dim3 grid(ftiles, unsigned((s.count + slice - 1) / slice));
+        const double* T = s.base + s.begin;
+        switch(k) {
+        case 0: g2_tangent_eval_kernel<0><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, nfreq, d_freq, perm, res, partial); break;
+        case 1: g2_tangent_eval_kernel<1><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, nfreq, d_freq, perm, res, partial); break;
+        case 2: g2_tangent_eval_kernel<2><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, nfreq, d_freq, perm, res, partial); break;
+        default: g2_tangent_eval_kernel<3><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, nfreq, d_freq, perm, res, partial); break;
+        }
+    }
+    g2_tangent_reduce_kernel<<<grid_for(nfreq), kBlock, 0, ctx->stream>>>(nfreq, nslices, partial, d_acc, d_dacc);
+    POMX_CHECK_LAUNCH(ctx);
+    return POMX_OK;
+}
+
 int read_i64(pomx_ctx* ctx, const int64_t* d, int64_t* h) {
     POMX_HIP(ctx, hipMemcpyAsync(h, d, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
     POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -708,6 +997,42 @@ int rank_table(pomx_ctx* ctx, int nx, int ny, const double* Ex, const double* Ey
     return POMX_OK;
 }
 
+// Tangent reduction, compaction and evaluation of one chunk's sorted groups.
+int g2_chunk_tangent(pomx_ctx* ctx, G2Dev const& D, pomx_g2_tangent const& t, int64_t n, int64_t ng,
+                     const int64_t* start, const uint64_t* keys, const uint64_t* vals, double thr, int perm, double res,
+                     int nfreq, const double* d_freq, double* d_acc, double* d_dacc, int64_t counts[4]) {
+    G2Tan T{t.dO1, t.dO2, t.dO3, t.dCX4, t.dE1, t.dE2, t.dE3, t.dE4, t.dw1, t.dw2, t.dw3, t.dw4};
+    WsBuf<int64_t> live(ctx, ng), lpos(ctx, ng), kc(ctx, 4), d_kf(ctx, 4);
+    WsBuf<double> g(ctx, size_t(ng) * 10);
+    if(!live.p || !lpos.p || !g.p)
+        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_tangent: out of device memory");
+    g2_reduce_tangent_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, T, n, ng, start, keys, vals, thr, g, live);
+    int rc = pomx_exclusive_scan(ctx, ng, live, lpos);
+    if(rc)
+        return rc;
+    kind_count_kernel<<<1, 64, 0, ctx->stream>>>(ng, live, lpos, start, keys, kc);
+    int64_t c[4];
+    POMX_HIP(ctx, hipMemcpyAsync(c, kc.p, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+    POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int64_t const nnr = std::max<int64_t>(c[0] + c[1], 1), nr = std::max<int64_t>(c[2] + c[3], 1);
+    int64_t kf[4] = {0, c[0], c[0] + c[1], c[0] + c[1] + c[2]};
+    POMX_HIP(ctx, hipMemcpyAsync(d_kf.p, kf, sizeof(kf), hipMemcpyHostToDevice, ctx->stream));
+    WsBuf<double> nrbuf(ctx, size_t(8 * nnr)), rbuf(ctx, size_t(10 * nr));
+    if(!nrbuf.p || !rbuf.p)
+        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_tangent: out of device memory");
+    g2_compact_tangent_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(ng, live, lpos, g, start, keys, nnr, nr, d_kf,
+                                                                         nrbuf, rbuf);
+    POMX_CHECK_LAUNCH(ctx);
+    for(int k = 0; k < 4; ++k)
+        counts[k] += c[k];
+    TermSpan spans[4] = {{nrbuf, nnr, 0, c[0]}, {nrbuf, nnr, c[0], c[1]}, {rbuf, nr, 0, c[2]}, {rbuf, nr, c[2], c[3]}};
+    if((rc = launch_g2_tangent_eval(ctx, spans, perm, res, nfreq, d_freq, d_acc, d_dacc)))
+        return rc;
+    // kf lives on this stack frame: the copy that reads it must finish before the return
+    POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return POMX_OK;
+}
+
 // Chunk budget in terms (keys + values + sort buffers: 32 bytes per term).
 constexpr int64_t kChunkTerms = int64_t(1) << 27;
 
@@ -728,11 +1053,15 @@ extern "C" int pomx_g2_terms_eval(pomx_ctx* ctx, const pomx_g2_terms* t, int per
     return POMX_OK;
 }
 
-extern "C" int pomx_g2_part_compute(pomx_ctx* ctx, const pomx_g2_part* part, double beta, double res, double tol,
-                                    int nfreq, const double* d_freq, double* d_acc, pomx_g2_terms* keep,
-                                    int64_t* h_counts, int64_t* h_raw_terms) {
+namespace {
+
+// Shared by the value path (tan == NULL) and the tangent path (keep must be NULL).
+int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* tan, double beta, double res, double tol,
+                int nfreq, const double* d_freq, double* d_acc, double* d_dacc, pomx_g2_terms* keep, int64_t* h_counts,
+                int64_t* h_raw_terms) {
     POMX_REQUIRE(ctx, part && res > 0.0 && nfreq >= 0 && (nfreq == 0 || (d_freq && d_acc)),
                  "pomx_g2_part_compute: bad arguments");
+    POMX_REQUIRE(ctx, !tan || (!keep && (nfreq == 0 || d_dacc)), "pomx_g2_part_tangent: bad arguments");
     POMX_REQUIRE(ctx, part->perm >= 0 && part->perm < 6, "pomx_g2_part_compute: bad permutation");
     pomx_g2_part const& P = *part;
     POMX_REQUIRE(ctx, P.n1 >= 1 && P.n2 >= 1 && P.n3 >= 1 && P.n4 >= 1, "pomx_g2_part_compute: empty block");
@@ -837,6 +1166,12 @@ extern "C" int pomx_g2_part_compute(pomx_ctx* ctx, const pomx_g2_part* part, dou
         // TermList erases a merged term when |coeff| <= tol/(size+1); size is taken as the
         // number of groups in this chunk
         double thr = tol / double(ng + 1);
+        if(tan) {
+            if((rc = g2_chunk_tangent(ctx, D, *tan, n, ng, start, k1, v1, thr, P.perm, res, nfreq, d_freq, d_acc, d_dacc,
+                                      counts)))
+                return rc;
+            continue;
+        }
         g2_reduce_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, n, ng, start, k1, v1, thr, g, live);
         WsBuf<int64_t> kc(ctx, 4);
         if((rc = pomx_exclusive_scan(ctx, ng, live, lpos)))
@@ -916,4 +1251,19 @@ extern "C" int pomx_g2_part_compute(pomx_ctx* ctx, const pomx_g2_part* part, dou
     if(h_raw_terms)
         *h_raw_terms = quads_total;
     return POMX_OK;
+}
+
+} // namespace
+
+extern "C" int pomx_g2_part_compute(pomx_ctx* ctx, const pomx_g2_part* part, double beta, double res, double tol,
+                                    int nfreq, const double* d_freq, double* d_acc, pomx_g2_terms* keep,
+                                    int64_t* h_counts, int64_t* h_raw_terms) {
+    return g2_part_run(ctx, part, nullptr, beta, res, tol, nfreq, d_freq, d_acc, nullptr, keep, h_counts, h_raw_terms);
+}
+
+extern "C" int pomx_g2_part_tangent(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* tangent, double beta,
+                                    double res, double tol, int nfreq, const double* d_freq, double* d_acc,
+                                    double* d_dacc, int64_t* h_counts, int64_t* h_raw_terms) {
+    POMX_REQUIRE(ctx, tangent != nullptr, "pomx_g2_part_tangent: no tangent");
+    return g2_part_run(ctx, part, tangent, beta, res, tol, nfreq, d_freq, d_acc, d_dacc, nullptr, h_counts, h_raw_terms);
 }

@@ -1,5 +1,6 @@
 // Green's function, susceptibility and two-particle GF on top of the device term pipelines.
 #include "pomerol/TwoParticleGF.hpp"
+#include "pomerol/Tangent.hpp"
 
 #include <algorithm>
 #include <limits>
@@ -381,6 +382,26 @@ void TwoParticleGFPart::run(int nfreq, double const* d_freq, double* d_acc, bool
     setStatus(keep ? Computed : Constructed);
 }
 
+void TwoParticleGFPart::runTangent(SpectrumTangent& T, int nfreq, double const* d_freq, double* d_acc, double* d_dacc) {
+    pomx_g2_part d = devicePart();
+    pomx_g2_tangent t;
+    t.dO1 = T.operatorTangent(O1);
+    t.dO2 = T.operatorTangent(O2);
+    t.dO3 = T.operatorTangent(O3);
+    t.dCX4 = T.operatorTangent(CX4);
+    t.dE1 = T.deviceEigenvalueTangent(Hpart1.getBlockNumber());
+    t.dE2 = T.deviceEigenvalueTangent(Hpart2.getBlockNumber());
+    t.dE3 = T.deviceEigenvalueTangent(Hpart3.getBlockNumber());
+    t.dE4 = T.deviceEigenvalueTangent(Hpart4.getBlockNumber());
+    t.dw1 = T.deviceWeightTangent(Hpart1.getBlockNumber());
+    t.dw2 = T.deviceWeightTangent(Hpart2.getBlockNumber());
+    t.dw3 = T.deviceWeightTangent(Hpart3.getBlockNumber());
+    t.dw4 = T.deviceWeightTangent(Hpart4.getBlockNumber());
+    detail::check(pomx_g2_part_tangent(detail::gpu(), &d, &t, beta, PoleResolution, CoefficientTolerance, nfreq, d_freq,
+                                       d_acc, d_dacc, counts, &raw_terms),
+                  "two-particle GF part tangent");
+}
+
 void TwoParticleGFPart::compute() {
     if(getStatus() >= Computed)
         return;
@@ -539,6 +560,39 @@ std::vector<ComplexType> TwoParticleGF::computeParts(std::vector<std::size_t> co
             m_data[std::size_t(w)] = ComplexType(h[std::size_t(2 * w)], h[std::size_t(2 * w + 1)]);
     }
     return m_data;
+}
+
+std::vector<ComplexType> TwoParticleGF::computeTangent(SpectrumTangent& T, FreqVec3 const& freqs,
+                                                       std::vector<ComplexType>* values) {
+    if(getStatus() < Prepared)
+        throw StatusMismatch("TwoParticleGF is not prepared yet.");
+    std::vector<ComplexType> d_data(freqs.size(), 0.0);
+    if(values)
+        values->assign(freqs.size(), 0.0);
+    if(Vanishing || freqs.empty())
+        return d_data;
+    int nf = int(freqs.size());
+    std::vector<double> h(std::size_t(6 * nf));
+    for(int w = 0; w < nf; ++w) {
+        ComplexType z[3] = {std::get<0>(freqs[w]), std::get<1>(freqs[w]), std::get<2>(freqs[w])};
+        for(int k = 0; k < 3; ++k) {
+            h[std::size_t(6 * w + 2 * k)] = z[k].real();
+            h[std::size_t(6 * w + 2 * k + 1)] = z[k].imag();
+        }
+    }
+    detail::DeviceArray<double> df(h.size()), acc(std::size_t(2 * nf)), dacc(std::size_t(2 * nf));
+    df.upload(h);
+    acc.zero();
+    dacc.zero();
+    for(auto& p : parts)
+        p.runTangent(T, nf, df.data(), acc.data(), dacc.data());
+    auto ha = acc.download(), hd = dacc.download();
+    for(int w = 0; w < nf; ++w) {
+        d_data[std::size_t(w)] = ComplexType(hd[std::size_t(2 * w)], hd[std::size_t(2 * w + 1)]);
+        if(values)
+            (*values)[std::size_t(w)] = ComplexType(ha[std::size_t(2 * w)], ha[std::size_t(2 * w + 1)]);
+    }
+    return d_data;
 }
 
 // TwoParticleGF.cpp:161-204 (one GPU: the MPI job farm and Allreduce reduce to a loop)

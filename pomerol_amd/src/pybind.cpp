@@ -399,6 +399,48 @@ PYBIND11_MODULE(_pomerol, m) {
             return d;
         }, py::arg("i"), py::arg("j"), py::arg("k"), py::arg("l"), py::arg("zs"), py::arg("pole_res") = 1e-8,
            py::arg("coeff_tol") = 1e-16, py::arg("parts") = py::none(), py::arg("keep") = false)
+        .def("spectrum_tangent", [](ED& e, py::list direction) {
+            // per block: (dE, dw) along the Hamiltonian direction
+            SpectrumTangent T(*e.S, *e.H, *e.rho);
+            T.prepare(build_expr(direction), *e.HS);
+            T.compute();
+            py::list out;
+            for(BlockNumber b = 0; b < e.S->getNumberOfBlocks(); ++b)
+                out.append(py::make_tuple(to_np(T.getEigenValueTangents(b)), to_np(T.getWeightTangents(b))));
+            return out;
+        })
+        .def("g2_tangent", [](ED& e, int i, int j, int k, int l, py::array_t<std::complex<double>> zs, py::list direction,
+                              double pr, double ct) {
+            auto& ops = e.field_ops();
+            TwoParticleGF G2(*e.S, *e.H, ops.getAnnihilationOperator(ParticleIndex(i)), ops.getAnnihilationOperator(ParticleIndex(j)),
+                             ops.getCreationOperator(ParticleIndex(k)), ops.getCreationOperator(ParticleIndex(l)), *e.rho);
+            G2.PoleResolution = pr;
+            G2.CoefficientTolerance = ct;
+            G2.prepare();
+            FreqVec3 f = to_freqs(zs);
+            auto t0 = std::chrono::steady_clock::now();
+            SpectrumTangent T(*e.S, *e.H, *e.rho);
+            T.prepare(build_expr(direction), *e.HS);
+            T.compute();
+            std::vector<ComplexType> vals;
+            std::vector<ComplexType> der = G2.computeTangent(T, f, &vals);
+            double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            py::dict d;
+            d["values"] = to_np(vals);
+            d["tangent"] = to_np(der);
+            d["seconds"] = secs;
+            int64_t nnr = 0, nr = 0, raw = 0;
+            for(auto const& p : G2.getParts()) {
+                nnr += int64_t(p.getNumNonResonantTerms());
+                nr += int64_t(p.getNumResonantTerms());
+                raw += p.getNumRawTerms();
+            }
+            d["n_nonres"] = nnr;
+            d["n_res"] = nr;
+            d["n_raw"] = raw;
+            return d;
+        }, py::arg("i"), py::arg("j"), py::arg("k"), py::arg("l"), py::arg("zs"), py::arg("direction"),
+           py::arg("pole_res") = 1e-8, py::arg("coeff_tol") = 1e-16)
         .def("g2_parts", [](ED& e, int i, int j, int k, int l) {
             // part list of TwoParticleGF::prepare: (S1, S2, S3, S4, permutation, n1, n2, n3, n4)
             auto& ops = e.field_ops();
