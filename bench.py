@@ -30,6 +30,11 @@ sys.path.insert(0, ROOT)
 FLOP_NONRES = 28  # complex division 11, products 12, pole subtractions 3, accumulate 2
 FLOP_RES = 36
 PEAK_FP64_TFLOPS = 78.6
+# HBM bytes per evaluation-kernel dispatch of the default step (--stride 8), measured with
+# rocprofv3 --pmc: FETCH_SIZE x 2 + WRITE_SIZE x 1 (the factors are what the same counters read
+# for a copy kernel of known size); the counters cannot be read from inside this process.
+EVAL_TRAFFIC_BYTES = 1.841e9
+DEFAULT_STRIDE = 8
 
 
 def log(*a):
@@ -125,7 +130,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--stride", type=int, default=8, help="every stride-th part of the component forms a step")
+    ap.add_argument("--stride", type=int, default=DEFAULT_STRIDE, help="every stride-th part of the component forms a step")
     ap.add_argument("--cpu-terms", type=float, default=1.5e6, help="merged-term budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--survey", action="store_true", help="print per-part term counts (no frequencies)")
@@ -135,7 +140,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path's multi-GPU "
-              "sharding is described in DESIGN.md and not built", file=sys.stderr)
+              "sharding is not built", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -219,7 +224,9 @@ def main():
                    "parts_per_step": len(batch), "merged_terms_per_step": (terms_nr + terms_r) // args.steps,
                    "freq_triplets": nfreq},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": EVAL_TRAFFIC_BYTES if args.stride == DEFAULT_STRIDE else None,
+                     "traffic_unit": "bytes per launch (rocprofv3 PMC FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
                      "kernel": "g2_eval_kernel (FP64 Lehmann sum)", "eval_ms": eval_ms, "eval_launches": launches},
         "cpu_baseline": cpu,
         "batch": batched,

@@ -1,5 +1,6 @@
 """Bath-parameter gradients of G(i w_n): the batched solver's hand-written adjoint behind a
-``torch.autograd.Function``.
+``torch.autograd.Function``; and the U-derivative of the up-down two-particle GF (forward-mode
+tangent, ``TwoParticleU``).
 
 A parametric batch holds instances ``H_k = sum_p theta[k, p] * H_p`` (one operator list per
 instance).  ``LehmannG.apply(theta, solver)`` returns G_k(i w_n) for n = 0..nfreq-1 as a complex128
@@ -16,7 +17,7 @@ import numpy as np
 import torch
 
 from . import _pomerol
-from .presets import CoulombS, Hopping, Level
+from .presets import CoulombS, Hopping, Level, anderson
 from .workloads import terms
 
 
@@ -86,3 +87,48 @@ class LehmannG(torch.autograd.Function):
 def bath_fit_loss(G: torch.Tensor, G_target: torch.Tensor) -> torch.Tensor:
     """sum_n |G(i w_n) - G_target(i w_n)|^2, per instance summed over the batch."""
     return ((G - G_target).abs() ** 2).sum()
+
+
+class AndersonTwoParticle:
+    """chi_{u d u d}(w1, w2; w3) = <c_up c_dn c+_up c+_dn> of the Anderson impurity with bath
+    (eps, V) at inverse temperature beta on a triplet list, as a function of U (mu = U/2), with
+    dchi/dU from the device tangent (TwoParticleGF::computeTangent along
+    dH/dU = n_up n_dn - (n_up + n_dn)/2)."""
+
+    def __init__(self, eps, V, beta: float, triplets, pole_res: float = 1e-8, coeff_tol: float = 1e-16,
+                 imp: str = "C"):
+        self.eps = np.asarray(eps, dtype=float)
+        self.V = np.asarray(V, dtype=float)
+        self.beta = float(beta)
+        self.triplets = np.ascontiguousarray(triplets, dtype=complex)
+        self.pole_res = float(pole_res)
+        self.coeff_tol = float(coeff_tol)
+        self.imp = imp
+        self.direction = terms(CoulombS(imp, 1.0, -0.5))
+
+    def value_and_tangent(self, U: float):
+        H = anderson(self.eps, self.V, U, U / 2, imp=self.imp)
+        idx = H.indices()
+        u, d = idx.index((self.imp, 0, 1)), idx.index((self.imp, 0, 0))
+        e = _pomerol.ED(terms(H), self.beta)
+        r = e.g2_tangent(u, d, u, d, self.triplets, self.direction, self.pole_res, self.coeff_tol)
+        return np.asarray(r["values"]), np.asarray(r["tangent"])
+
+
+class TwoParticleU(torch.autograd.Function):
+    """chi = TwoParticleU.apply(U, model): U a 0-d float64 tensor, model an AndersonTwoParticle,
+    chi complex128 over the model's triplets.  The forward pass evaluates the value and the tangent
+    dchi/dU together (one parameter, so forward mode is the cheap direction); backward returns
+    Re sum conj(grad_chi) dchi/dU, with grad_chi = dL/dRe chi + i dL/dIm chi."""
+
+    @staticmethod
+    def forward(ctx, U: torch.Tensor, model: AndersonTwoParticle) -> torch.Tensor:
+        chi, dchi = model.value_and_tangent(float(U.detach().cpu()))
+        ctx.dchi = torch.from_numpy(dchi)
+        ctx.shape, ctx.dtype, ctx.device = U.shape, U.dtype, U.device
+        return torch.from_numpy(chi).to(U.device)
+
+    @staticmethod
+    def backward(ctx, grad_chi: torch.Tensor):
+        g = torch.real(torch.sum(torch.conj(grad_chi.detach().cpu()) * ctx.dchi))
+        return g.to(dtype=ctx.dtype, device=ctx.device).reshape(ctx.shape), None

@@ -90,6 +90,29 @@ def test_g2_tangent_matches_oracle_finite_differences(pa, oracle):
     _check("1+2 bath vs oracle FD", r["tangent"], (vals[0] - vals[1]) / (2 * H_STEP))
 
 
+def test_two_particle_u_autograd(pa):
+    """torch.autograd.Function of the tangent: dL/dU of L = sum |chi(U) - chi_t|^2 against central
+    differences of L from the device 2PGF."""
+    import torch
+
+    from pomerol_amd.adjoint import AndersonTwoParticle, TwoParticleU
+
+    cfg = W.AndersonConfig(2, 2.5, 8.0, seed=W.SEED + 502)
+    zs = W.g2_grid(8, 3, cfg.beta)
+    model = AndersonTwoParticle(cfg.eps, cfg.V, cfg.beta, zs, 1e-8, 0.0)
+    target = torch.from_numpy(model.value_and_tangent(3.0)[0])
+    U = torch.tensor(cfg.U, dtype=torch.float64, requires_grad=True)
+    L = ((TwoParticleU.apply(U, model) - target).abs() ** 2).sum()
+    L.backward()
+
+    def loss(u):
+        return float(np.sum(np.abs(model.value_and_tangent(u)[0] - target.numpy()) ** 2))
+
+    fd = (loss(cfg.U + H_STEP) - loss(cfg.U - H_STEP)) / (2 * H_STEP)
+    print(f"[g2-tangent] autograd dL/dU {float(U.grad):.10e}, FD {fd:.10e}", flush=True)
+    assert abs(float(U.grad) - fd) <= 1e-6 * max(1.0, abs(fd))
+
+
 def test_g2_tangent_config_1plus3_bath(pa):
     cfg = W.cfg1()
     zs = W.g2_grid(16, 4, cfg.beta)
