@@ -220,7 +220,7 @@ __global__ void zero_kernel(double* __restrict__ p, int64_t n) {
 unsigned zero_grid(int64_t n) { return unsigned(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536))); }
 
 template <class T> int upload(pomx_ctx* ctx, T*& d, std::vector<T> const& h) {
-    if(hipMalloc(reinterpret_cast<void**>(&d), sizeof(T) * std::max<size_t>(h.size(), 1)) != hipSuccess)
+    if(device_malloc(ctx, reinterpret_cast<void**>(&d), sizeof(T) * std::max<size_t>(h.size(), 1)) != hipSuccess)
         return pomx_fail(ctx, POMX_ERR_HIP, "pomx_batch_adjoint: out of device memory");
     if(!h.empty())
         POMX_HIP(ctx, hipMemcpyAsync(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, ctx->stream));
@@ -228,7 +228,7 @@ template <class T> int upload(pomx_ctx* ctx, T*& d, std::vector<T> const& h) {
 }
 
 int alloc_doubles(pomx_ctx* ctx, double*& d, int64_t n) {
-    if(hipMalloc(reinterpret_cast<void**>(&d), sizeof(double) * size_t(std::max<int64_t>(n, 1))) != hipSuccess)
+    if(device_malloc(ctx, reinterpret_cast<void**>(&d), sizeof(double) * size_t(std::max<int64_t>(n, 1))) != hipSuccess)
         return pomx_fail(ctx, POMX_ERR_HIP, "pomx_batch_adjoint: out of device memory");
     return POMX_OK;
 }

@@ -148,7 +148,7 @@ __global__ void g_reduce_kernel(int ninst, int nfreq, const int64_t* __restrict_
 
 template <class T> T* dev_copy(pomx_ctx* ctx, const T* h, size_t n, int& rc) {
     void* p = nullptr;
-    if(hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) {
+    if(device_malloc(ctx, &p, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) {
         rc = pomx_fail(ctx, POMX_ERR_HIP, "pomx_batch_create: out of device memory");
         return nullptr;
     }

@@ -13,6 +13,27 @@ int pomx_fail(pomx_ctx* ctx, int code, std::string const& msg) {
     return code;
 }
 
+// Return every cached workspace block to the device (after the work that used them).
+static void ws_release_cached(pomx_ctx* ctx) {
+    hipStreamSynchronize(ctx->stream);
+    for(auto& kv : ctx->free_blocks)
+        hipFree(kv.second);
+    ctx->free_blocks.clear();
+}
+
+hipError_t device_malloc(pomx_ctx* ctx, void** p, size_t bytes) {
+    if(hipMalloc(p, bytes) == hipSuccess)
+        return hipSuccess;
+    (void)hipGetLastError();
+    ws_release_cached(ctx);
+    hipError_t e = hipMalloc(p, bytes);
+    if(e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+    }
+    return e;
+}
+
 void* ws_get(pomx_ctx* ctx, size_t bytes) {
     bytes = (bytes + 255) & ~size_t(255);
     auto it = ctx->free_blocks.lower_bound(bytes);
@@ -24,15 +45,8 @@ void* ws_get(pomx_ctx* ctx, size_t bytes) {
         return p;
     }
     void* p = nullptr;
-    if(hipMalloc(&p, bytes) != hipSuccess) {
-        // release cached blocks and retry once
-        hipStreamSynchronize(ctx->stream);
-        for(auto& kv : ctx->free_blocks)
-            hipFree(kv.second);
-        ctx->free_blocks.clear();
-        if(hipMalloc(&p, bytes) != hipSuccess)
-            return nullptr;
-    }
+    if(device_malloc(ctx, &p, bytes) != hipSuccess)
+        return nullptr;
     ctx->live_blocks[p] = bytes;
     return p;
 }
@@ -92,7 +106,9 @@ int pomx_malloc(pomx_ctx* ctx, size_t bytes, void** d_ptr) {
     *d_ptr = nullptr;
     if(bytes == 0)
         bytes = 8;
-    POMX_HIP(ctx, hipMalloc(d_ptr, bytes));
+    hipError_t e = device_malloc(ctx, d_ptr, bytes);
+    if(e != hipSuccess)
+        return pomx_fail(ctx, POMX_ERR_HIP, std::string("pomx_malloc: ") + hipGetErrorString(e));
     return POMX_OK;
 }
 

@@ -951,9 +951,9 @@ unsigned gx_rows(int64_t work, int per, int ny) {
     return unsigned(std::max<int64_t>(1, std::min<int64_t>((work + per - 1) / per, cap)));
 }
 
-template <class T> T* dupload(std::vector<T> const& h, hipStream_t s, bool& ok) {
+template <class T> T* dupload(pomx_ctx* ctx, std::vector<T> const& h, hipStream_t s, bool& ok) {
     void* p = nullptr;
-    if(hipMalloc(&p, sizeof(T) * std::max<size_t>(h.size(), 1)) != hipSuccess) {
+    if(device_malloc(ctx, &p, sizeof(T) * std::max<size_t>(h.size(), 1)) != hipSuccess) {
         ok = false;
         return nullptr;
     }
@@ -1168,18 +1168,18 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
         }
     }
     bool ok = true;
-    P->blk = dupload(P->h_blk, s, ok);
-    P->ids = dupload(ids, s, ok);
-    P->gjobs = dupload(gj, s, ok);
-    P->leaves = dupload(leaves, s, ok);
-    P->leaf_jobs = dupload(ljobs, s, ok);
-    P->nodes = dupload(nodes, s, ok);
+    P->blk = dupload(ctx, P->h_blk, s, ok);
+    P->ids = dupload(ctx, ids, s, ok);
+    P->gjobs = dupload(ctx, gj, s, ok);
+    P->leaves = dupload(ctx, leaves, s, ok);
+    P->leaf_jobs = dupload(ctx, ljobs, s, ok);
+    P->nodes = dupload(ctx, nodes, s, ok);
     std::vector<DcState> st0(nodes.size(), DcState{0, 0, 0.0}); // host sources live until the sync below
     std::vector<pomx_gemm_job> jb0(nodes.size());
-    P->states = dupload(st0, s, ok);
-    P->dc_jobs = dupload(jb0, s, ok);
+    P->states = dupload(ctx, st0, s, ok);
+    P->dc_jobs = dupload(ctx, jb0, s, ok);
     auto alloc = [&](void** q, size_t bytes) {
-        if(ok && hipMalloc(q, std::max<size_t>(bytes, 8)) != hipSuccess)
+        if(ok && device_malloc(ctx, q, std::max<size_t>(bytes, 8)) != hipSuccess)
             ok = false;
         P->bytes += bytes;
     };

@@ -1213,7 +1213,7 @@ int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* 
     if(keep) {
         int64_t nnr = counts[0] + counts[1], nr = counts[2] + counts[3];
         size_t bytes = sizeof(double) * size_t(4 * nnr + 5 * nr + 1);
-        POMX_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&keep->data), bytes));
+        POMX_HIP(ctx, device_malloc(ctx, reinterpret_cast<void**>(&keep->data), bytes));
         double* NR = keep->data;
         double* R = keep->data + 4 * nnr;
         // kind-major concatenation of the chunks

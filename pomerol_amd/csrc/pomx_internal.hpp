@@ -71,6 +71,11 @@ template <class T> struct WsBuf {
     operator T*() const { return p; }
 };
 
+// hipMalloc, retried once after the cached workspace blocks are released.  A failed attempt
+// leaves its error as the runtime's last error, which a later hipGetLastError() launch check
+// would report: it is consumed here.  Every device allocation of the library goes through it.
+hipError_t device_malloc(pomx_ctx* ctx, void** p, size_t bytes);
+
 // Stable LSD radix sort of (uint64 key, uint64 value) pairs on the context stream.
 // keys_in/vals_in are consumed; sorted output lands in keys_out/vals_out.
 int pomx_sort_pairs(pomx_ctx* ctx, int64_t n, uint64_t* keys_in, uint64_t* keys_out, uint64_t* vals_in,

@@ -312,8 +312,8 @@ extern "C" int pomx_pair_terms(pomx_ctx* ctx, int kind, int nparts, const pomx_p
     if(ncand == 0) {
         void* a = nullptr;
         void* b = nullptr;
-        POMX_HIP(ctx, hipMalloc(&a, 8));
-        POMX_HIP(ctx, hipMalloc(&b, 8));
+        POMX_HIP(ctx, device_malloc(ctx, &a, 8));
+        POMX_HIP(ctx, device_malloc(ctx, &b, 8));
         *d_residue = static_cast<double*>(a);
         *d_pole = static_cast<double*>(b);
         return POMX_OK;
@@ -371,8 +371,8 @@ extern "C" int pomx_pair_terms(pomx_ctx* ctx, int kind, int nparts, const pomx_p
         POMX_HIP(ctx, hipMemcpyAsync(&lp, lpos.p + ngroups - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
         POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
         int64_t nlive = ll + lp;
-        POMX_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&outR), sizeof(double) * std::max<int64_t>(nlive, 1)));
-        POMX_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&outP), sizeof(double) * std::max<int64_t>(nlive, 1)));
+        POMX_HIP(ctx, device_malloc(ctx, reinterpret_cast<void**>(&outR), sizeof(double) * std::max<int64_t>(nlive, 1)));
+        POMX_HIP(ctx, device_malloc(ctx, reinterpret_cast<void**>(&outP), sizeof(double) * std::max<int64_t>(nlive, 1)));
         compact_kernel<<<grid_for(ngroups), kBlock, 0, ctx->stream>>>(ngroups, live, lpos, gR, gP, outR, outP);
         part_live_count_kernel<<<grid_for(nparts), kBlock, 0, ctx->stream>>>(nparts, ngroups, first, live, lpos, d_cnt);
         POMX_CHECK_LAUNCH(ctx);
@@ -387,8 +387,8 @@ extern "C" int pomx_pair_terms(pomx_ctx* ctx, int kind, int nparts, const pomx_p
             return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_pair_terms: |pole/pole_res| exceeds 2^39");
         }
     } else {
-        POMX_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&outR), 8));
-        POMX_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&outP), 8));
+        POMX_HIP(ctx, device_malloc(ctx, reinterpret_cast<void**>(&outR), 8));
+        POMX_HIP(ctx, device_malloc(ctx, reinterpret_cast<void**>(&outP), 8));
     }
     for(int p = 0; p < nparts; ++p) {
         h_count[p] = int64_t(cnt[p]);
