@@ -125,6 +125,94 @@ def run_batch(args):
             "plain_s_per_solve_by_bath": t_size, "bath_mix": mix, "autotune": tune, "graph": True}
 
 
+def run_dense(args):
+    """FP64 eigensolve and GEMM rates through the C ABI on dense device matrices.
+
+    * eigensolve (pomx_eigh_batched, large-block path): one GOE matrix of size --eigh-n (default
+      15876, the largest block of the 3x3 Hubbard workload; a random spectrum deflates less than
+      the cluster's degenerate one), one warm-up solve then one timed solve; algorithmic work
+      (10/3) n^3 (tridiagonalisation 4/3 n^3 + back-transformation 2 n^3, divide-and-conquer
+      merges not counted); residual and orthogonality checked on 32 eigenvectors;
+    * GEMM (pomx_gemm_batched): C = A B with n = --gemm-n, 2 n^3 FLOP, best of 3.
+    Matrices are generated and held by torch (device memory only)."""
+    import ctypes
+
+    import torch
+
+    import pomerol_amd
+
+    lib = ctypes.CDLL(pomerol_amd.POMX_LIB)
+    lib.pomx_last_error.restype = ctypes.c_char_p
+    ctx = ctypes.c_void_p()
+    if lib.pomx_ctx_create(0, ctypes.byref(ctx)) != 0:
+        raise RuntimeError("pomx_ctx_create failed")
+
+    def check(rc):
+        if rc != 0:
+            raise RuntimeError(lib.pomx_last_error(ctx).decode())
+
+    out = {}
+    dev = torch.device("cuda")
+    gen = torch.Generator(device=dev)
+    n = args.eigh_n
+    if n > 0:
+        gen.manual_seed(12345)
+        A0 = torch.randn(n, n, dtype=torch.float64, device=dev, generator=gen)
+        A0 = (A0 + A0.T) * 0.5
+        A = torch.empty_like(A0)
+        w = torch.empty(n, dtype=torch.float64, device=dev)
+        sizes = (ctypes.c_int * 1)(n)
+        zero = (ctypes.c_int64 * 1)(0)
+        secs = []
+        for _ in range(2):
+            A.copy_(A0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            check(lib.pomx_eigh_batched(ctx, 1, sizes, zero, zero, ctypes.c_void_p(A.data_ptr()),
+                                        ctypes.c_void_p(w.data_ptr())))
+            secs.append(time.perf_counter() - t0)
+        V = A[:, ::max(1, n // 32)][:, :32]  # eigenvectors are the columns
+        wk = w[::max(1, n // 32)][:32]
+        scale = float(w.abs().max())
+        res = float((A0 @ V - V * wk).abs().max()) / scale
+        orth = float((V.T @ V - torch.eye(V.shape[1], dtype=torch.float64, device=dev)).abs().max())
+        flops = 10.0 / 3.0 * n ** 3
+        out["eigensolve"] = {"n": n, "matrix": "GOE (dense random symmetric)", "seconds": secs[-1],
+                             "warmup_seconds": secs[0], "tflops_algorithmic": flops / secs[-1] / 1e12,
+                             "flop_model": "(10/3) n^3", "residual": res, "orthogonality": orth}
+        log(f"[bench] eigensolve n={n}: {secs[-1]:.3f}s ({flops / secs[-1] / 1e12:.2f} TFLOP/s), residual {res:.1e}, "
+            f"orthogonality {orth:.1e}")
+        del A0, A, V
+        torch.cuda.empty_cache()
+    n = args.gemm_n
+    if n > 0:
+        gen.manual_seed(54321)
+        X = torch.randn(2, n, n, dtype=torch.float64, device=dev, generator=gen)
+        C = torch.empty(n, n, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+
+        class Job(ctypes.Structure):
+            _fields_ = [("m", ctypes.c_int), ("n", ctypes.c_int), ("k", ctypes.c_int), ("trans_a", ctypes.c_int),
+                        ("trans_b", ctypes.c_int), ("lda", ctypes.c_int), ("ldb", ctypes.c_int), ("ldc", ctypes.c_int),
+                        ("a_off", ctypes.c_int64), ("b_off", ctypes.c_int64), ("c_off", ctypes.c_int64),
+                        ("alpha", ctypes.c_double), ("beta", ctypes.c_double)]
+
+        job = Job(n, n, n, 0, 0, n, n, n, 0, n * n, 0, 1.0, 0.0)
+        best = float("inf")
+        for _ in range(4):
+            t0 = time.perf_counter()
+            check(lib.pomx_gemm_batched(ctx, 1, ctypes.byref(job), ctypes.c_void_p(X.data_ptr()),
+                                        ctypes.c_void_p(X.data_ptr()), ctypes.c_void_p(C.data_ptr())))
+            best = min(best, time.perf_counter() - t0)
+        err = float((C[:64] - X[0, :64] @ X[1]).abs().max()) / float(C[:64].abs().max())
+        out["gemm"] = {"n": n, "seconds": best, "tflops": 2.0 * n ** 3 / best / 1e12, "rel_err_vs_torch": err}
+        log(f"[bench] gemm n={n}: {best * 1e3:.1f} ms ({2.0 * n ** 3 / best / 1e12:.2f} TFLOP/s), rel err {err:.1e}")
+        del X, C
+        torch.cuda.empty_cache()
+    lib.pomx_ctx_destroy(ctx)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,11 +225,18 @@ def main():
     ap.add_argument("--batch-inst", type=int, default=512, help="instances of the batched ED+G solve (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--plain-sample", type=int, default=4, help="instances per bath size timed on the plain per-solve path")
+    ap.add_argument("--eigh-n", type=int, default=15876, help="dense eigensolve size (0: skip)")
+    ap.add_argument("--gemm-n", type=int, default=8192, help="dense GEMM size (0: skip)")
+    ap.add_argument("--dense-only", action="store_true", help="only the dense eigensolve and GEMM legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path's multi-GPU "
               "sharding is not built", file=sys.stderr)
         sys.exit(2)
+
+    if args.dense_only:
+        print(json.dumps({"dense": run_dense(args)}), flush=True)
+        return
 
     import numpy as np
 
@@ -205,6 +300,7 @@ def main():
                          f"terms x {nfreq} triplets, {secs:.1f}s); generation serial, fill OpenMP"}
 
     batched = run_batch(args) if args.batch_inst > 0 else None
+    dense = run_dense(args) if (args.eigh_n > 0 or args.gemm_n > 0) else None
 
     out = {
         "metric": "2PGF terms*freqs/s",
@@ -230,6 +326,7 @@ def main():
                      "kernel": "g2_eval_kernel (FP64 Lehmann sum)", "eval_ms": eval_ms, "eval_launches": launches},
         "cpu_baseline": cpu,
         "batch": batched,
+        "dense": dense,
     }
     print(json.dumps(out), flush=True)
 

@@ -31,7 +31,6 @@ namespace {
 constexpr int kNB = 32;   // reflectors per panel
 constexpr int kLeaf = 64; // largest divide-and-conquer leaf
 constexpr int kMaxSymvGrid = 1024;
-constexpr int kLdsV = 16384;   // longest reflector staged in LDS by the matrix-vector kernel
 constexpr int kSymTile = 64;   // rows per tile of the symmetric matrix-vector product
 constexpr int kSymChunk = 512; // columns per work item of the symmetric matrix-vector product
 
@@ -238,11 +237,12 @@ __global__ void __launch_bounds__(256) trd_refl_kernel(const BigBlk* __restrict_
 // w_raw = A22 v from the upper triangle of A22 only (A22 symmetric; its lower triangle is never
 // updated): work item (row tile t, column chunk k) covers rows [64t, 64t+64) and columns
 // [64t + 512k, +512) on or above the diagonal; it adds A[r][c] v[c] to the row partial of r and
-// A[r][c] v[r] (c > r) to the column partial of c.  v is staged in LDS when it fits.
-template <bool LDSV>
+// A[r][c] v[r] (c > r) to the column partial of c.  Only the item's 512 entries of v are staged
+// in LDS (about 20 KB per workgroup, so several workgroups share a CU and keep enough loads in
+// flight); each wave streams two 4 KB row segments at a time.
 __global__ void __launch_bounds__(256) trd_symv_kernel(const BigBlk* __restrict__ blks, int j, const double* __restrict__ Aar,
                                                        double* __restrict__ D) {
-    extern __shared__ double vs[];
+    __shared__ double vc[kSymChunk];
     __shared__ double cred[4][kSymChunk];
     BigBlk const B = blks[blockIdx.y];
     int const n = B.n;
@@ -251,51 +251,60 @@ __global__ void __launch_bounds__(256) trd_symv_kernel(const BigBlk* __restrict_
     int const m = n - j - 1, jb = j + 1;
     int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const double* __restrict__ v = base + o.vbuf;
-    if(LDSV) {
-        for(int c = threadIdx.x; c < m; c += 256)
-            vs[c] = v[c];
-        __syncthreads();
-        v = vs;
-    }
     int const nt = (m + kSymTile - 1) / kSymTile, ncmax = (m + kSymChunk - 1) / kSymChunk;
     int const ncb = (n + kSymChunk - 1) / kSymChunk; // row-partial stride of this block
+    constexpr int U = kSymChunk / 64;
     for(int q = blockIdx.x; q < nt * ncmax; q += gridDim.x) {
         int const t = q / ncmax, k = q % ncmax;
         int const R0 = t * kSymTile, C0 = R0 + k * kSymChunk;
         if(C0 >= m)
             continue; // uniform over the workgroup
-        double cacc[kSymChunk / 64];
+        __syncthreads(); // the previous item's readers of vc and cred are done
+        for(int c = threadIdx.x; c < kSymChunk; c += 256)
+            vc[c] = (C0 + c < m) ? v[C0 + c] : 0.0;
+        __syncthreads();
+        double cacc[U];
 #pragma unroll
-        for(int u = 0; u < kSymChunk / 64; ++u)
+        for(int u = 0; u < U; ++u)
             cacc[u] = 0.0;
-        for(int rr = wave; rr < kSymTile; rr += 4) {
-            int const r = R0 + rr;
-            if(r >= m)
+        // rows wave + 8 h and wave + 8 h + 4 of the tile, h = 0..7
+        for(int h = 0; h < kSymTile / 8; ++h) {
+            int const ra = R0 + wave + 8 * h, rb = ra + 4;
+            if(ra >= m)
                 break;
-            const double* __restrict__ a = Aar + B.mat_off + int64_t(jb + r) * n + jb;
-            double const vr = v[r];
-            double x[kSymChunk / 64];
+            bool const hb = rb < m;
+            const double* __restrict__ a = Aar + B.mat_off + int64_t(jb + ra) * n + jb;
+            const double* __restrict__ b = a + int64_t(4) * n;
+            double xa[U], xb[U];
 #pragma unroll
-            for(int u = 0; u < kSymChunk / 64; ++u) {
+            for(int u = 0; u < U; ++u) {
                 int const c = C0 + lane + 64 * u;
-                x[u] = (c < m && c >= r) ? a[c] : 0.0;
+                xa[u] = (c < m && c >= ra) ? a[c] : 0.0;
+                xb[u] = (hb && c < m && c >= rb) ? b[c] : 0.0;
             }
-            double rs = 0.0;
+            double const va = v[ra], vb = hb ? v[rb] : 0.0;
+            double sa = 0.0, sb = 0.0;
 #pragma unroll
-            for(int u = 0; u < kSymChunk / 64; ++u) {
+            for(int u = 0; u < U; ++u) {
                 int const c = C0 + lane + 64 * u;
-                if(c < m) {
-                    rs = fma(x[u], v[c], rs);
-                    if(c > r)
-                        cacc[u] = fma(x[u], vr, cacc[u]);
-                }
+                double const vv = vc[lane + 64 * u];
+                sa = fma(xa[u], vv, sa);
+                sb = fma(xb[u], vv, sb);
+                // column parts (c > r); the diagonal element counts once, in the row part
+                cacc[u] = fma(c > ra ? xa[u] : 0.0, va, cacc[u]);
+                cacc[u] = fma(c > rb ? xb[u] : 0.0, vb, cacc[u]);
             }
-            rs = wave_sum(rs);
-            if(lane == 0)
-                base[o.rp + (int64_t(t) * ncb + k) * kSymTile + rr] = rs;
+            sa = wave_sum(sa);
+            sb = wave_sum(sb);
+            if(lane == 0) {
+                double* rp = base + o.rp + (int64_t(t) * ncb + k) * kSymTile;
+                rp[ra - R0] = sa;
+                if(hb)
+                    rp[rb - R0] = sb;
+            }
         }
 #pragma unroll
-        for(int u = 0; u < kSymChunk / 64; ++u)
+        for(int u = 0; u < U; ++u)
             cred[wave][lane + 64 * u] = cacc[u];
         __syncthreads();
         for(int c = threadIdx.x; c < kSymChunk; c += 256) {
@@ -303,7 +312,6 @@ __global__ void __launch_bounds__(256) trd_symv_kernel(const BigBlk* __restrict_
             if(cc < m)
                 base[o.cp + int64_t(t) * n + cc] = (cred[0][c] + cred[1][c]) + (cred[2][c] + cred[3][c]);
         }
-        __syncthreads();
     }
 }
 
@@ -1207,8 +1215,6 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
     zero_ints_kernel<<<gx_for(std::max(P->nleaves, 1)), 256, 0, s>>>(P->status, std::max(P->nleaves, 1));
     // 1. reduction to tridiagonal form
     if(!P->tri_only) {
-        POMX_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(trd_symv_kernel<true>),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(double) * kLdsV)));
         fill_blocks_kernel<<<dim3(gx_rows(int64_t(P->nmax) * P->nmax, 256, nb), unsigned(nb)), 256, 0, s>>>(P->blk, P->D, 1,
                                                                                                         0.0);
         size_t e = 0;
@@ -1226,10 +1232,7 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
             trd_refl_kernel<<<one, 256, 0, s>>>(P->blk, j, i, int(g1), Aar, P->D);
             int64_t const items = int64_t((m + kSymTile - 1) / kSymTile) * ((m + kSymChunk - 1) / kSymChunk);
             unsigned const gs = gx_rows(items, 1, nact);
-            if(m <= kLdsV)
-                trd_symv_kernel<true><<<dim3(gs, unsigned(nact)), 256, sizeof(double) * size_t(m), s>>>(P->blk, j, Aar, P->D);
-            else
-                trd_symv_kernel<false><<<dim3(gs, unsigned(nact)), 256, 0, s>>>(P->blk, j, Aar, P->D);
+            trd_symv_kernel<<<dim3(gs, unsigned(nact)), 256, 0, s>>>(P->blk, j, Aar, P->D);
             trd_symv_reduce_kernel<<<dim3(g2, unsigned(nact)), 256, 0, s>>>(P->blk, j, j0, i, P->D);
             if(i > 0)
                 trd_t_kernel<<<one, 256, 0, s>>>(P->blk, i, int(g2), P->D);
