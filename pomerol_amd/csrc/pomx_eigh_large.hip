@@ -329,20 +329,29 @@ __global__ void __launch_bounds__(256) trd_symv_reduce_kernel(const BigBlk* __re
     const double* __restrict__ v = base + o.vbuf;
     int const m = n - j - 1, jb = j + 1;
     int const ncmax = (m + kSymChunk - 1) / kSymChunk, ncb = (n + kSymChunk - 1) / kSymChunk;
-    for(int x = blockIdx.x * 256 + threadIdx.x; x < m; x += gridDim.x * 256) {
-        int const t = x / kSymTile;
-        int const nck = (m - t * kSymTile + kSymChunk - 1) / kSymChunk; // chunks of row tile t
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // one 64-row slab (= row tile g) per step; the four waves split its row partials (chunks k)
+    // and column partials (row tiles tt <= g) and their sums are added in a fixed order
+    int const nslab = (m + kSymTile - 1) / kSymTile;
+    for(int g = blockIdx.x; g < nslab; g += gridDim.x) {
+        int const x = g * kSymTile + lane;
+        int const nck = (m - g * kSymTile + kSymChunk - 1) / kSymChunk; // chunks of row tile g
         double s = 0.0;
-        for(int k = 0; k < nck && k < ncmax; ++k)
-            s += base[o.rp + (int64_t(t) * ncb + k) * kSymTile + (x - t * kSymTile)];
-        for(int tt = 0; tt <= t; ++tt)
-            s += base[o.cp + int64_t(tt) * n + x];
-        base[o.wraw + x] = s;
+        if(x < m) {
+            for(int k = wave; k < nck && k < ncmax; k += 4)
+                s += base[o.rp + (int64_t(g) * ncb + k) * kSymTile + lane];
+            for(int tt = wave; tt <= g; tt += 4)
+                s += base[o.cp + int64_t(tt) * n + x];
+        }
+        red[wave][lane] = s;
+        __syncthreads();
+        if(wave == 0 && x < m)
+            base[o.wraw + x] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+        __syncthreads();
     }
     if(i == 0)
         return;
     // partial dots over a chunk of rows: wavefront per (matrix, p), lanes along the rows
-    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int const chunk = (m + gridDim.x - 1) / gridDim.x;
     int const r0 = blockIdx.x * chunk, r1 = min(m, r0 + chunk);
     for(int q = wave; q < 64; q += 4) { // q < 32: (W^T v)_q, 32 <= q < 64: (V^T v)_{q-32}
@@ -1225,7 +1234,7 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
                 --nact;
             int const j0 = (j / kNB) * kNB, i = j - j0, m = n0 - j - 1;
             unsigned const g1 = std::min(256u, gx_rows(n0 - j, 256, nact));
-            unsigned const g2 = std::min(unsigned(P->gsym), gx_rows(m, 256, nact));
+            unsigned const g2 = std::min(unsigned(P->gsym), gx_rows(m, kSymTile, nact));
             unsigned const g3 = std::min(256u, gx_rows(m, 256, nact));
             dim3 const one(1, unsigned(nact));
             trd_row_kernel<<<dim3(g1, unsigned(nact)), 256, 0, s>>>(P->blk, j, j0, i, Aar, P->D);
