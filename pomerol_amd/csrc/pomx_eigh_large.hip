@@ -7,16 +7,18 @@
 // permutations, GEMM sizes) lives in device memory, so a plan can be replayed inside a hipGraph.
 //   1. Blocked Householder tridiagonalisation T = Q^T A Q, panels of kNB reflectors.  Inside a
 //      panel the trailing matrix is not updated; w = tau (A22 v - V W^T v - W V^T v) comes from
-//      one device-wide matrix-vector product over A22 (the HBM-bound part); reductions finish in
-//      the last workgroup to arrive.  After each panel, A22 -= V W^T + W V^T as MFMA GEMMs.
+//      one device-wide matrix-vector product over the upper triangle of A22 (the HBM-bound
+//      part) with its partials reduced by small follow-up kernels.  After each panel,
+//      A22 -= V W^T + W V^T as MFMA GEMMs on the upper tiles.
 //   2. Divide and conquer on (d, e): leaves of at most kLeaf rows (workgroup QL eigensolver),
 //      merged pairwise bottom-up.  A merge diagonalises D + rho z z^T: one workgroup merges the
 //      two sorted child spectra, normalises z and deflates (negligible z_i; close pairs by a
 //      Givens rotation), one wavefront per secular root with the origin at the nearer pole, the
 //      Gu-Eisenstat recomputed z for orthogonal eigenvectors, the eigenvector update as an MFMA
 //      GEMM whose size the deflation kernel writes, and a rank sort of the merged spectrum.
-//   3. Back-transformation X = Q Z, panels in reverse, each I - V T V^T (forward triangular
-//      factor) with MFMA GEMMs.
+//   3. Back-transformation X = Q Z, blocks of kBT reflectors (four panels) in reverse, each
+//      I - V T V^T (forward triangular factor) with MFMA GEMMs; the wide blocks read and write
+//      Z a quarter as often as panel-sized ones would.
 // Output as the small-block path: eigenvectors as columns of the row-major block, eigenvalues
 // ascending.
 #include "pomx_internal.hpp"
@@ -31,6 +33,7 @@ namespace {
 constexpr int kNB = 32;   // reflectors per panel
 constexpr int kLeaf = 64; // largest divide-and-conquer leaf
 constexpr int kMaxSymvGrid = 1024;
+constexpr int kBT = 128;       // reflectors per block of the back-transformation (4 panels)
 constexpr int kSymTile = 64;   // rows per tile of the symmetric matrix-vector product
 constexpr int kSymChunk = 512; // columns per work item of the symmetric matrix-vector product
 
@@ -55,8 +58,8 @@ __host__ __device__ inline DOff doff(int n, int gsym) {
     o.U = at, at += nn;
     o.W = at, at += int64_t(n) * kNB;  // panel w's, reflector-major: W[p][r] at p*n + r
     o.Vt = at, at += int64_t(n) * kNB; // panel reflectors, reflector-major copy of V
-    o.W1 = at, at += int64_t(n) * kNB;
-    o.W2 = at, at += int64_t(n) * kNB;
+    o.W1 = at, at += int64_t(n) * kBT;
+    o.W2 = at, at += int64_t(n) * kBT;
     o.vbuf = at, at += n;
     o.wraw = at, at += n;
     o.d = at, at += n;
@@ -71,8 +74,8 @@ __host__ __device__ inline DOff doff(int n, int gsym) {
     o.zh = at, at += n;
     o.vals = at, at += n;
     o.a2 = at, at += kNB;
-    o.S = at, at += kNB * kNB;
-    o.T = at, at += kNB * kNB;
+    o.S = at, at += kBT * kBT;
+    o.T = at, at += kBT * kBT;
     o.rowpart = at, at += 256;
     o.wpart = at, at += 256;
     o.scal = at, at += 64;
@@ -911,19 +914,20 @@ __global__ void copy_z_kernel(const BigBlk* __restrict__ blks, const double* __r
         Aar[B.mat_off + q] = D[B.dbase + od.Z + q];
 }
 
-// T (nb x nb, upper) of I - V T V^T = H_j0 ... H_{j0+nb-1} from S = V^T V and tau, one workgroup per
-// block of the panel.
-__global__ void wy_t_kernel(const BigBlk* __restrict__ blks, const int* __restrict__ ids, int j0, double* __restrict__ D) {
-    __shared__ double sT[kNB][kNB];
-    __shared__ double col[kNB];
+// T (nb x nb, upper, nb <= kBT) of I - V T V^T = H_j0 ... H_{j0+nb-1} from S = V^T V and tau, one
+// workgroup of kBT threads per block: column i of T is -tau_i T[:i, :i] S[:i, i], T_ii = tau_i
+// (thread r owns row r; T is staged in dynamic LDS, kBT^2 doubles).
+__global__ void __launch_bounds__(kBT) wy_t_kernel(const BigBlk* __restrict__ blks, const int* __restrict__ ids, int j0,
+                                                   double* __restrict__ D) {
+    extern __shared__ double sT[]; // kBT x kBT
+    __shared__ double col[kBT];
     BigBlk const B = blks[ids[blockIdx.x]];
     DOff const od = doff(B.n, B.gsym);
     double* base = D + B.dbase;
-    int const nb = min(kNB, B.n - 1 - j0);
+    int const nb = min(kBT, B.n - 1 - j0);
     int const r = threadIdx.x;
-    if(r < kNB)
-        for(int c = 0; c < kNB; ++c)
-            sT[r][c] = 0.0;
+    for(int c = 0; c < kBT; ++c)
+        sT[r * kBT + c] = 0.0;
     __syncthreads();
     for(int i = 0; i < nb; ++i) {
         double const ti = base[od.tau + j0 + i];
@@ -933,16 +937,16 @@ __global__ void wy_t_kernel(const BigBlk* __restrict__ blks, const int* __restri
         if(r < i) {
             double acc = 0.0;
             for(int q = r; q < i; ++q)
-                acc += sT[r][q] * col[q];
-            sT[r][i] = acc;
+                acc = fma(sT[r * kBT + q], col[q], acc);
+            sT[r * kBT + i] = acc;
         }
         if(r == i)
-            sT[i][i] = ti;
+            sT[i * kBT + i] = ti;
         __syncthreads();
     }
     if(r < nb)
         for(int c = 0; c < nb; ++c)
-            base[od.T + r * nb + c] = sT[r][c];
+            base[od.T + r * nb + c] = sT[r * kBT + c];
 }
 
 __global__ void evals_out_kernel(const BigBlk* __restrict__ blks, const double* __restrict__ D, double* __restrict__ eval) {
@@ -1151,8 +1155,8 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
     P->nleaves = int(leaves.size());
     // back-transformation panels, last to first
     if(!tri_only) {
-        int const last = ((P->nmax - 2) / kNB) * kNB;
-        for(int j0 = last; j0 >= 0; j0 -= kNB) {
+        int const last = ((P->nmax - 2) / kBT) * kBT;
+        for(int j0 = last; j0 >= 0; j0 -= kBT) {
             std::vector<int> L;
             for(int b = 0; b < nblk; ++b)
                 if(j0 < P->h_blk[size_t(b)].n - 1)
@@ -1163,7 +1167,7 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
             for(int kind = 0; kind < 4; ++kind) // jobs grouped by kind: S, W1, W2, X
                 for(int b : L) {
                     BigBlk const& B = P->h_blk[size_t(b)];
-                    int const n = B.n, nb = std::min(kNB, n - 1 - j0), rows = n - j0 - 1;
+                    int const n = B.n, nb = std::min(kBT, n - 1 - j0), rows = n - j0 - 1;
                     DOff const o = doff(n, B.gsym);
                     int64_t const voff = B.dbase + o.V + int64_t(j0 + 1) * n + j0;
                     if(kind == 0) {
@@ -1295,6 +1299,8 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
     }
     // 3. back-transformation, eigenvalues
     if(!P->tri_only) {
+        POMX_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(wy_t_kernel),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(double) * kBT * kBT)));
         copy_z_kernel<<<dim3(gx_rows(int64_t(P->nmax) * P->nmax, 256, nb), unsigned(nb)), 256, 0, s>>>(P->blk, P->D, Aar);
         int64_t const tn = (P->nmax + 63) / 64;
         for(auto const& T : P->bt) {
@@ -1303,7 +1309,7 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
             int64_t const tr = (T.rows_max + 63) / 64;
             if((rc = launch_gemm_jobs(ctx, 1, 0, J, T.nblk, tb * tb, P->D, P->D, P->D)))
                 return rc;
-            wy_t_kernel<<<unsigned(T.nblk), 64, 0, s>>>(P->blk, P->ids + T.ids_off, T.j0, P->D);
+            wy_t_kernel<<<unsigned(T.nblk), kBT, sizeof(double) * kBT * kBT, s>>>(P->blk, P->ids + T.ids_off, T.j0, P->D);
             if((rc = launch_gemm_jobs(ctx, 1, 0, J + T.nblk, T.nblk, tb * tn, P->D, Aar, P->D)) ||
                (rc = launch_gemm_jobs(ctx, 0, 0, J + 2 * T.nblk, T.nblk, tb * tn, P->D, P->D, P->D)) ||
                (rc = launch_gemm_jobs(ctx, 0, 0, J + 3 * T.nblk, T.nblk, tr * tn, P->D, P->D, Aar)))
