@@ -915,38 +915,36 @@ __global__ void copy_z_kernel(const BigBlk* __restrict__ blks, const double* __r
 }
 
 // T (nb x nb, upper, nb <= kBT) of I - V T V^T = H_j0 ... H_{j0+nb-1} from S = V^T V and tau, one
-// workgroup of kBT threads per block: column i of T is -tau_i T[:i, :i] S[:i, i], T_ii = tau_i
-// (thread r owns row r; T is staged in dynamic LDS, kBT^2 doubles).
+// workgroup of kBT threads per block: column i of T is -tau_i T[:i, :i] S[:i, i], T_ii = tau_i.
+// Thread r owns row r of T and builds it in place in the block's scratch (it only reads back
+// its own writes); the column factor is shared through LDS.
 __global__ void __launch_bounds__(kBT) wy_t_kernel(const BigBlk* __restrict__ blks, const int* __restrict__ ids, int j0,
                                                    double* __restrict__ D) {
-    extern __shared__ double sT[]; // kBT x kBT
     __shared__ double col[kBT];
     BigBlk const B = blks[ids[blockIdx.x]];
     DOff const od = doff(B.n, B.gsym);
     double* base = D + B.dbase;
     int const nb = min(kBT, B.n - 1 - j0);
     int const r = threadIdx.x;
-    for(int c = 0; c < kBT; ++c)
-        sT[r * kBT + c] = 0.0;
-    __syncthreads();
+    double* __restrict__ Tr = base + od.T + int64_t(r) * nb; // row r (valid for r < nb)
+    if(r < nb)
+        for(int c = 0; c < nb; ++c)
+            Tr[c] = 0.0;
     for(int i = 0; i < nb; ++i) {
         double const ti = base[od.tau + j0 + i];
+        __syncthreads(); // readers of the previous column factor are done
         if(r < i)
             col[r] = -ti * base[od.S + r * nb + i];
         __syncthreads();
         if(r < i) {
             double acc = 0.0;
             for(int q = r; q < i; ++q)
-                acc = fma(sT[r * kBT + q], col[q], acc);
-            sT[r * kBT + i] = acc;
+                acc = fma(Tr[q], col[q], acc);
+            Tr[i] = acc;
         }
         if(r == i)
-            sT[i * kBT + i] = ti;
-        __syncthreads();
+            Tr[i] = ti;
     }
-    if(r < nb)
-        for(int c = 0; c < nb; ++c)
-            base[od.T + r * nb + c] = sT[r * kBT + c];
 }
 
 __global__ void evals_out_kernel(const BigBlk* __restrict__ blks, const double* __restrict__ D, double* __restrict__ eval) {
@@ -1299,8 +1297,6 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
     }
     // 3. back-transformation, eigenvalues
     if(!P->tri_only) {
-        POMX_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(wy_t_kernel),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(double) * kBT * kBT)));
         copy_z_kernel<<<dim3(gx_rows(int64_t(P->nmax) * P->nmax, 256, nb), unsigned(nb)), 256, 0, s>>>(P->blk, P->D, Aar);
         int64_t const tn = (P->nmax + 63) / 64;
         for(auto const& T : P->bt) {
@@ -1309,7 +1305,7 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
             int64_t const tr = (T.rows_max + 63) / 64;
             if((rc = launch_gemm_jobs(ctx, 1, 0, J, T.nblk, tb * tb, P->D, P->D, P->D)))
                 return rc;
-            wy_t_kernel<<<unsigned(T.nblk), kBT, sizeof(double) * kBT * kBT, s>>>(P->blk, P->ids + T.ids_off, T.j0, P->D);
+            wy_t_kernel<<<unsigned(T.nblk), kBT, 0, s>>>(P->blk, P->ids + T.ids_off, T.j0, P->D);
             if((rc = launch_gemm_jobs(ctx, 1, 0, J + T.nblk, T.nblk, tb * tn, P->D, Aar, P->D)) ||
                (rc = launch_gemm_jobs(ctx, 0, 0, J + 2 * T.nblk, T.nblk, tb * tn, P->D, P->D, P->D)) ||
                (rc = launch_gemm_jobs(ctx, 0, 0, J + 3 * T.nblk, T.nblk, tr * tn, P->D, P->D, Aar)))
