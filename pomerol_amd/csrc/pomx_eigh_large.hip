@@ -33,6 +33,7 @@ namespace {
 constexpr int kNB = 32;   // reflectors per panel
 constexpr int kLeaf = 64; // largest divide-and-conquer leaf
 constexpr int kMaxSymvGrid = 1024;
+constexpr int kDotParts = 64; // row chunks of the panel dot products W^T v, V^T v
 constexpr int kBT = 128;       // reflectors per block of the back-transformation (4 panels)
 constexpr int kSymTile = 64;   // rows per tile of the symmetric matrix-vector product
 constexpr int kSymChunk = 512; // columns per work item of the symmetric matrix-vector product
@@ -352,10 +353,12 @@ __global__ void __launch_bounds__(256) trd_symv_reduce_kernel(const BigBlk* __re
             base[o.wraw + x] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
         __syncthreads();
     }
-    if(i == 0)
+    if(i == 0 || int(blockIdx.x) >= kDotParts)
         return;
-    // partial dots over a chunk of rows: wavefront per (matrix, p), lanes along the rows
-    int const chunk = (m + gridDim.x - 1) / gridDim.x;
+    // partial dots over a chunk of rows (the first kDotParts workgroups): wavefront per
+    // (matrix, p), lanes along the rows
+    int const nparts = min(int(gridDim.x), kDotParts);
+    int const chunk = (m + nparts - 1) / nparts;
     int const r0 = blockIdx.x * chunk, r1 = min(m, r0 + chunk);
     for(int q = wave; q < 64; q += 4) { // q < 32: (W^T v)_q, 32 <= q < 64: (V^T v)_{q-32}
         int const p = q & 31;
@@ -937,10 +940,18 @@ __global__ void __launch_bounds__(kBT) wy_t_kernel(const BigBlk* __restrict__ bl
             col[r] = -ti * base[od.S + r * nb + i];
         __syncthreads();
         if(r < i) {
-            double acc = 0.0;
-            for(int q = r; q < i; ++q)
-                acc = fma(Tr[q], col[q], acc);
-            Tr[i] = acc;
+            // four independent chains (loads in flight), added in a fixed order
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+            int q = r;
+            for(; q + 3 < i; q += 4) {
+                a0 = fma(Tr[q], col[q], a0);
+                a1 = fma(Tr[q + 1], col[q + 1], a1);
+                a2 = fma(Tr[q + 2], col[q + 2], a2);
+                a3 = fma(Tr[q + 3], col[q + 3], a3);
+            }
+            for(; q < i; ++q)
+                a0 = fma(Tr[q], col[q], a0);
+            Tr[i] = (a0 + a1) + (a2 + a3);
         }
         if(r == i)
             Tr[i] = ti;
@@ -1246,7 +1257,7 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
             trd_symv_kernel<<<dim3(gs, unsigned(nact)), 256, 0, s>>>(P->blk, j, Aar, P->D);
             trd_symv_reduce_kernel<<<dim3(g2, unsigned(nact)), 256, 0, s>>>(P->blk, j, j0, i, P->D);
             if(i > 0)
-                trd_t_kernel<<<one, 256, 0, s>>>(P->blk, i, int(g2), P->D);
+                trd_t_kernel<<<one, 256, 0, s>>>(P->blk, i, int(std::min(g2, unsigned(kDotParts))), P->D);
             trd_w_kernel<<<dim3(g3, unsigned(nact)), 256, 0, s>>>(P->blk, j, j0, i, P->D);
             trd_a2_kernel<<<one, 256, 0, s>>>(P->blk, j, i, int(g3), P->D);
             if(e < P->ends.size() && P->ends[e].j == j) {
