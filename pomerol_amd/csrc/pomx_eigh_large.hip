@@ -46,7 +46,7 @@ struct BigBlk {
 
 // Per-block scratch layout (doubles from dbase).
 struct DOff {
-    int64_t V, Z, Q2, U, W, Vt, W1, W2, vbuf, wraw, d, e, tau, lam, zs, dsrt, dl, zl, rt, zh, vals, a2, S, T, rowpart, wpart,
+    int64_t V, Z, Q2, U, W, Vt, W3, W1, W2, vbuf, wraw, d, e, tau, lam, zs, dsrt, dl, zl, rt, zh, vals, a2, S, T, rowpart, wpart,
         scal, part, rp, cp, end;
 };
 __host__ __device__ inline DOff doff(int n, int gsym) {
@@ -59,6 +59,7 @@ __host__ __device__ inline DOff doff(int n, int gsym) {
     o.U = at, at += nn;
     o.W = at, at += int64_t(n) * kNB;  // panel w's, reflector-major: W[p][r] at p*n + r
     o.Vt = at, at += int64_t(n) * kNB; // panel reflectors, reflector-major copy of V
+    o.W3 = at, at += int64_t(n) * kNB; // copy of W after Vt: [W; Vt] and [Vt; W3] are contiguous
     o.W1 = at, at += int64_t(n) * kBT;
     o.W2 = at, at += int64_t(n) * kBT;
     o.vbuf = at, at += n;
@@ -454,15 +455,24 @@ __global__ void __launch_bounds__(256) trd_a2_kernel(const BigBlk* __restrict__ 
 // End of a panel (blocks listed in ids, panel ending at reflector j): W[p][r] += a2[p] V[p][r]
 // for the rows of the trailing update.
 __global__ void trd_fold_kernel(const BigBlk* __restrict__ blks, const int* __restrict__ ids, int j, double* __restrict__ D) {
+    // W += a2 V over the trailing rows, copied to W3; rows beyond a short last panel are zeroed
+    // (in W, W3 and Vt) so the trailing update can always take kNB reflectors
     BigBlk const B = blks[ids[blockIdx.y]];
     int const n = B.n;
     DOff const o = doff(n, B.gsym);
     double* __restrict__ base = D + B.dbase;
     int const j0 = (j / kNB) * kNB, nb = j - j0 + 1, r0 = j + 1;
-    int64_t const cnt = int64_t(n - r0) * nb;
+    int64_t const cnt = int64_t(n - r0) * kNB;
     for(int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < cnt; q += int64_t(gridDim.x) * blockDim.x) {
         int const p = int(q / (n - r0)), r = r0 + int(q % (n - r0));
-        base[o.W + int64_t(p) * n + r] += base[o.a2 + p] * base[o.Vt + int64_t(p) * n + r];
+        int64_t const at = int64_t(p) * n + r;
+        double w = 0.0;
+        if(p < nb)
+            w = base[o.W + at] + base[o.a2 + p] * base[o.Vt + at];
+        else
+            base[o.Vt + at] = 0.0;
+        base[o.W + at] = w;
+        base[o.W3 + at] = w;
     }
 }
 
@@ -1097,20 +1107,13 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
             BigPlan::PanelEnd E{j, int(L.size()), 0, int64_t(ids.size()), int64_t(gj.size())};
             int const j0 = (j / kNB) * kNB, nb = j - j0 + 1, j1 = j + 1;
             int mtmax = 0;
-            for(int b : L) { // A22 -= V2 W2^T  (V2^T, W2^T are rows j1.. of the reflector-major panels)
+            for(int b : L) { // A22 -= [V2 W2] [W2 V2]^T as one product over 2 kNB reflector rows
                 BigBlk const& B = P->h_blk[size_t(b)];
                 int const n = B.n, mt = n - j1;
                 DOff const o = doff(n, B.gsym);
                 ids.push_back(b);
                 mtmax = std::max(mtmax, mt);
-                gj.push_back(pomx_gemm_job{mt, mt, nb, 1, 0, n, n, n, B.dbase + o.Vt + j1, B.dbase + o.W + j1,
-                                           B.mat_off + int64_t(j1) * n + j1, -1.0, 1.0});
-            }
-            for(int b : L) { // A22 -= W2 V2^T
-                BigBlk const& B = P->h_blk[size_t(b)];
-                int const n = B.n, mt = n - j1;
-                DOff const o = doff(n, B.gsym);
-                gj.push_back(pomx_gemm_job{mt, mt, nb, 1, 0, n, n, n, B.dbase + o.W + j1, B.dbase + o.Vt + j1,
+                gj.push_back(pomx_gemm_job{mt, mt, 2 * kNB, 1, 0, n, n, n, B.dbase + o.Vt + j1, B.dbase + o.W + j1,
                                            B.mat_off + int64_t(j1) * n + j1, -1.0, 1.0});
             }
             int64_t const t = (mtmax + 63) / 64;
@@ -1264,9 +1267,7 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
                 BigPlan::PanelEnd const& E = P->ends[e++];
                 trd_fold_kernel<<<dim3(gx_rows(int64_t(std::max(m, 1)) * kNB, 256, E.nblk), unsigned(E.nblk)), 256, 0, s>>>(
                     P->blk, P->ids + E.ids_off, j, P->D);
-                if((rc = launch_gemm_jobs(ctx, 1, 0, P->gjobs + E.jobs_off, E.nblk, E.max_tiles, P->D, P->D, Aar, true)) ||
-                   (rc = launch_gemm_jobs(ctx, 1, 0, P->gjobs + E.jobs_off + E.nblk, E.nblk, E.max_tiles, P->D, P->D, Aar,
-                                          true)))
+                if((rc = launch_gemm_jobs(ctx, 1, 0, P->gjobs + E.jobs_off, E.nblk, E.max_tiles, P->D, P->D, Aar, true)))
                     return rc;
             }
         }
