@@ -133,11 +133,11 @@ def run_dense(args):
       the cluster's degenerate one), one warm-up solve then one timed solve; algorithmic work
       (10/3) n^3 (tridiagonalisation 4/3 n^3 + back-transformation 2 n^3, divide-and-conquer
       merges not counted); residual and orthogonality checked on 32 eigenvectors;
-    * GEMM (pomx_gemm_batched): C = A B with n = --gemm-n, 2 n^3 FLOP, best of 3.
-    Matrices are generated and held by torch (device memory only)."""
+    * GEMM (pomx_gemm_batched): C = A B with n = --gemm-n, 2 n^3 FLOP, best of 4.
+    Matrices are drawn on the host (numpy) and uploaded; the timed calls start with them in HBM."""
     import ctypes
 
-    import torch
+    import numpy as np
 
     import pomerol_amd
 
@@ -151,45 +151,59 @@ def run_dense(args):
         if rc != 0:
             raise RuntimeError(lib.pomx_last_error(ctx).decode())
 
+    def alloc(nbytes):
+        p = ctypes.c_void_p()
+        check(lib.pomx_malloc(ctx, ctypes.c_size_t(nbytes), ctypes.byref(p)))
+        return p
+
+    def h2d(p, a, off=0):
+        check(lib.pomx_h2d(ctx, ctypes.c_void_p(p.value + off), a.ctypes.data_as(ctypes.c_void_p),
+                           ctypes.c_size_t(a.nbytes)))
+
+    def d2h(a, p):
+        check(lib.pomx_d2h(ctx, a.ctypes.data_as(ctypes.c_void_p), p, ctypes.c_size_t(a.nbytes)))
+
     out = {}
-    dev = torch.device("cuda")
-    gen = torch.Generator(device=dev)
     n = args.eigh_n
     if n > 0:
-        gen.manual_seed(12345)
-        A0 = torch.randn(n, n, dtype=torch.float64, device=dev, generator=gen)
-        A0 = (A0 + A0.T) * 0.5
-        A = torch.empty_like(A0)
-        w = torch.empty(n, dtype=torch.float64, device=dev)
+        rng = np.random.default_rng(12345)
+        A0 = rng.standard_normal((n, n))
+        A0 += A0.T
+        A0 *= 0.5
+        d_A, d_w = alloc(A0.nbytes), alloc(8 * n)
         sizes = (ctypes.c_int * 1)(n)
         zero = (ctypes.c_int64 * 1)(0)
         secs = []
         for _ in range(2):
-            A.copy_(A0)
-            torch.cuda.synchronize()
+            h2d(d_A, A0)
             t0 = time.perf_counter()
-            check(lib.pomx_eigh_batched(ctx, 1, sizes, zero, zero, ctypes.c_void_p(A.data_ptr()),
-                                        ctypes.c_void_p(w.data_ptr())))
+            check(lib.pomx_eigh_batched(ctx, 1, sizes, zero, zero, d_A, d_w))
             secs.append(time.perf_counter() - t0)
-        V = A[:, ::max(1, n // 32)][:, :32]  # eigenvectors are the columns
-        wk = w[::max(1, n // 32)][:32]
-        scale = float(w.abs().max())
-        res = float((A0 @ V - V * wk).abs().max()) / scale
-        orth = float((V.T @ V - torch.eye(V.shape[1], dtype=torch.float64, device=dev)).abs().max())
+        w = np.empty(n)
+        d2h(w, d_w)
+        V = np.empty((n, n))
+        d2h(V, d_A)
+        cols = np.arange(0, n, max(1, n // 32))[:32]
+        Vs, ws = V[:, cols], w[cols]
+        scale = float(np.abs(w).max())
+        res = float(np.abs(A0 @ Vs - Vs * ws).max()) / scale
+        orth = float(np.abs(Vs.T @ Vs - np.eye(len(cols))).max())
         flops = 10.0 / 3.0 * n ** 3
         out["eigensolve"] = {"n": n, "matrix": "GOE (dense random symmetric)", "seconds": secs[-1],
                              "warmup_seconds": secs[0], "tflops_algorithmic": flops / secs[-1] / 1e12,
-                             "flop_model": "(10/3) n^3", "residual": res, "orthogonality": orth}
+                             "flop_model": "(10/3) n^3", "residual": res, "orthogonality": orth,
+                             "ascending": bool(np.all(np.diff(w) >= 0))}
         log(f"[bench] eigensolve n={n}: {secs[-1]:.3f}s ({flops / secs[-1] / 1e12:.2f} TFLOP/s), residual {res:.1e}, "
             f"orthogonality {orth:.1e}")
-        del A0, A, V
-        torch.cuda.empty_cache()
+        check(lib.pomx_free(ctx, d_A))
+        check(lib.pomx_free(ctx, d_w))
+        del A0, V
     n = args.gemm_n
     if n > 0:
-        gen.manual_seed(54321)
-        X = torch.randn(2, n, n, dtype=torch.float64, device=dev, generator=gen)
-        C = torch.empty(n, n, dtype=torch.float64, device=dev)
-        torch.cuda.synchronize()
+        rng = np.random.default_rng(54321)
+        X = rng.standard_normal((2, n, n))
+        d_X, d_C = alloc(X.nbytes), alloc(8 * n * n)
+        h2d(d_X, X)
 
         class Job(ctypes.Structure):
             _fields_ = [("m", ctypes.c_int), ("n", ctypes.c_int), ("k", ctypes.c_int), ("trans_a", ctypes.c_int),
@@ -201,14 +215,16 @@ def run_dense(args):
         best = float("inf")
         for _ in range(4):
             t0 = time.perf_counter()
-            check(lib.pomx_gemm_batched(ctx, 1, ctypes.byref(job), ctypes.c_void_p(X.data_ptr()),
-                                        ctypes.c_void_p(X.data_ptr()), ctypes.c_void_p(C.data_ptr())))
+            check(lib.pomx_gemm_batched(ctx, 1, ctypes.byref(job), d_X, d_X, d_C))
             best = min(best, time.perf_counter() - t0)
-        err = float((C[:64] - X[0, :64] @ X[1]).abs().max()) / float(C[:64].abs().max())
-        out["gemm"] = {"n": n, "seconds": best, "tflops": 2.0 * n ** 3 / best / 1e12, "rel_err_vs_torch": err}
+        C = np.empty((64, n))
+        check(lib.pomx_d2h(ctx, C.ctypes.data_as(ctypes.c_void_p), d_C, ctypes.c_size_t(C.nbytes)))
+        ref = X[0, :64] @ X[1]
+        err = float(np.abs(C - ref).max()) / float(np.abs(ref).max())
+        out["gemm"] = {"n": n, "seconds": best, "tflops": 2.0 * n ** 3 / best / 1e12, "rel_err_vs_numpy": err}
         log(f"[bench] gemm n={n}: {best * 1e3:.1f} ms ({2.0 * n ** 3 / best / 1e12:.2f} TFLOP/s), rel err {err:.1e}")
-        del X, C
-        torch.cuda.empty_cache()
+        check(lib.pomx_free(ctx, d_X))
+        check(lib.pomx_free(ctx, d_C))
     lib.pomx_ctx_destroy(ctx)
     return out
 
