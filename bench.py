@@ -93,10 +93,11 @@ def run_batch(args):
     b.prepare()
     setup = time.perf_counter() - t0
     tune = b.autotune()
-    b.solve(True)  # capture + warm
+    graph = not args.batch_eager
+    b.solve(graph)  # capture + warm
     t0 = time.perf_counter()
     for _ in range(args.batch_steps):
-        b.solve(True)
+        b.solve(graph)
     secs = (time.perf_counter() - t0) / args.batch_steps
     batched = len(cfgs) / secs
     log(f"[bench] batch: {len(cfgs)} instances, {b.n_blocks()} blocks, setup {setup:.1f}s, {secs * 1e3:.1f} ms/solve-batch, "
@@ -122,7 +123,7 @@ def run_batch(args):
     log(f"[bench] plain path s/solve by bath size: { {nb: round(t, 4) for nb, t in t_size.items()} }, mix {mix}")
     return {"instances": len(cfgs), "freqs": 128, "solves_per_s": batched, "ms_per_batch": secs * 1e3,
             "plain_sample": ns, "plain_solves_per_s": plain, "speedup_vs_plain": batched / plain,
-            "plain_s_per_solve_by_bath": t_size, "bath_mix": mix, "autotune": tune, "graph": True}
+            "plain_s_per_solve_by_bath": t_size, "bath_mix": mix, "autotune": tune, "graph": graph}
 
 
 def run_dense(args):
@@ -244,6 +245,8 @@ def main():
     ap.add_argument("--eigh-n", type=int, default=15876, help="dense eigensolve size (0: skip)")
     ap.add_argument("--gemm-n", type=int, default=8192, help="dense GEMM size (0: skip)")
     ap.add_argument("--dense-only", action="store_true", help="only the dense eigensolve and GEMM legs")
+    ap.add_argument("--batch-only", action="store_true", help="only the batched-solve leg")
+    ap.add_argument("--batch-eager", action="store_true", help="batched solves launched eagerly (no graph replay)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path's multi-GPU "
@@ -252,6 +255,9 @@ def main():
 
     if args.dense_only:
         print(json.dumps({"dense": run_dense(args)}), flush=True)
+        return
+    if args.batch_only:
+        print(json.dumps({"batch": run_batch(args)}), flush=True)
         return
 
     import numpy as np

@@ -991,6 +991,14 @@ unsigned gx_rows(int64_t work, int per, int ny) {
     return unsigned(std::max<int64_t>(1, std::min<int64_t>((work + per - 1) / per, cap)));
 }
 
+// Grid width of the per-column reduction kernels over ny blocks: about 2048 workgroups in all
+// (these kernels do O(n) work per block and column, so with many blocks one workgroup per block
+// avoids paying dispatch for idle workgroups), at least 1 and at most the row's work needs.
+unsigned gx_col(int64_t work, int per, int ny) {
+    int64_t const cap = std::max<int64_t>(1, (2048 + ny - 1) / std::max(ny, 1));
+    return unsigned(std::max<int64_t>(1, std::min<int64_t>((work + per - 1) / per, cap)));
+}
+
 template <class T> T* dupload(pomx_ctx* ctx, std::vector<T> const& h, hipStream_t s, bool& ok) {
     void* p = nullptr;
     if(device_malloc(ctx, &p, sizeof(T) * std::max<size_t>(h.size(), 1)) != hipSuccess) {
@@ -1249,9 +1257,9 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
             while(nact > 0 && P->h_blk[size_t(nact - 1)].n - 2 < j)
                 --nact;
             int const j0 = (j / kNB) * kNB, i = j - j0, m = n0 - j - 1;
-            unsigned const g1 = std::min(256u, gx_rows(n0 - j, 256, nact));
-            unsigned const g2 = std::min(unsigned(P->gsym), gx_rows(m, kSymTile, nact));
-            unsigned const g3 = std::min(256u, gx_rows(m, 256, nact));
+            unsigned const g1 = std::min(256u, gx_col(n0 - j, 256, nact));
+            unsigned const g2 = std::min(unsigned(P->gsym), gx_col(m, kSymTile, nact));
+            unsigned const g3 = std::min(256u, gx_col(m, 256, nact));
             dim3 const one(1, unsigned(nact));
             trd_row_kernel<<<dim3(g1, unsigned(nact)), 256, 0, s>>>(P->blk, j, j0, i, Aar, P->D);
             trd_refl_kernel<<<one, 256, 0, s>>>(P->blk, j, i, int(g1), Aar, P->D);
