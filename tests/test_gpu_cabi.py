@@ -197,3 +197,77 @@ def test_gemm_batched(cabi):
         assert np.max(np.abs(got - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
     bad = GemmJob(4, 4, 4, 0, 0, 2, 4, 4, 0, 0, 0, 1.0, 0.0)  # lda < k
     assert cabi.lib.pomx_gemm_batched(cabi.ctx, 1, ctypes.byref(bad), d_A, d_B, d_C) == 1
+
+
+def _aligned_eigh(H, ref=None):
+    E, U = np.linalg.eigh(H)
+    if ref is not None:
+        U = U * np.sign(np.sum(U * ref, axis=0))
+    return E, U
+
+
+def test_eigh_tangent(cabi):
+    """pomx_eigh_tangent against central differences of numpy's eigendecomposition (h = 1e-6):
+    dE and dU = U K; K vanishes between exactly degenerate levels."""
+    rng = np.random.default_rng(21)
+    sizes = [1, 2, 7, 64, 130, 300]
+    h = 1e-6
+    blocks = []
+    for n in sizes:
+        H = rng.standard_normal((n, n))
+        H = (H + H.T) / 2
+        dH = rng.standard_normal((n, n))
+        blocks.append((H, (dH + dH.T) / 2))
+    # one more block with an exactly degenerate pair (eigenvalue 0.5 twice)
+    Q, _ = np.linalg.qr(rng.standard_normal((9, 9)))
+    lam = np.array([-2.0, -1.0, 0.5, 0.5, 1.0, 1.5, 2.0, 2.5, 3.0])
+    Hd = (Q * lam) @ Q.T
+    dHd = rng.standard_normal((9, 9))
+    blocks.append(((Hd + Hd.T) / 2, (dHd + dHd.T) / 2))
+    sizes.append(9)
+    mo = np.cumsum([0] + [n * n for n in sizes])[:-1].astype(np.int64)
+    vo = np.cumsum([0] + sizes)[:-1].astype(np.int64)
+    Es, Us = zip(*[_aligned_eigh(H) for H, _ in blocks])
+    d_U = cabi.put(np.concatenate([U.ravel() for U in Us]))
+    d_E = cabi.put(np.concatenate(Es))
+    d_dH = cabi.put(np.concatenate([dH.ravel() for _, dH in blocks]))
+    d_K = cabi.put(np.zeros(int(mo[-1] + sizes[-1] ** 2)))
+    d_dE = cabi.put(np.zeros(sum(sizes)))
+    n_arr = np.array(sizes, dtype=np.int32)
+    cabi.ok(cabi.lib.pomx_eigh_tangent(cabi.ctx, len(sizes), n_arr.ctypes.data_as(ctypes.c_void_p),
+                                       mo.ctypes.data_as(ctypes.c_void_p), vo.ctypes.data_as(ctypes.c_void_p), d_U, d_E,
+                                       d_dH, d_K, d_dE))
+    K_all = cabi.get(d_K, np.zeros(int(mo[-1] + sizes[-1] ** 2)))
+    dE_all = cabi.get(d_dE, np.zeros(sum(sizes)))
+    for b, n in enumerate(sizes[:-1]):
+        H, dH = blocks[b]
+        K = K_all[mo[b]:mo[b] + n * n].reshape(n, n)
+        dE = dE_all[vo[b]:vo[b] + n]
+        Ep, Up = _aligned_eigh(H + h * dH, Us[b])
+        Em, Um = _aligned_eigh(H - h * dH, Us[b])
+        fdE, fdU = (Ep - Em) / (2 * h), (Up - Um) / (2 * h)
+        errE = np.abs(dE - fdE).max()
+        errU = np.abs(Us[b] @ K - fdU).max() / max(1.0, np.abs(fdU).max())
+        print(f"[cabi] eigh tangent n={n}: |dE - FD| {errE:.1e}, |U K - dU_FD| {errU:.1e}", flush=True)
+        assert errE <= 1e-7 and errU <= 1e-6
+        assert np.abs(K + K.T).max() <= 1e-12 * max(1.0, np.abs(K).max())
+    K = K_all[mo[-1]:].reshape(9, 9)
+    assert K[2, 3] == 0.0 and K[3, 2] == 0.0  # the degenerate pair carries no rotation
+    assert K[2, 4] != 0.0 and K[1, 3] != 0.0
+
+
+def test_boltzmann_tangent(cabi):
+    rng = np.random.default_rng(23)
+    E, dE, beta, h = rng.uniform(-3, 3, 4000), rng.standard_normal(4000), 4.0, 1e-6
+
+    def weights(x):
+        w = np.exp(-beta * (x - x.min()))
+        return w / w.sum()
+
+    w = weights(E)
+    d_w, d_dE, d_dw = cabi.put(w), cabi.put(dE), cabi.put(np.zeros_like(E))
+    cabi.ok(cabi.lib.pomx_boltzmann_tangent(cabi.ctx, ctypes.c_int64(len(E)), d_w, d_dE, ctypes.c_double(beta), d_dw))
+    dw = cabi.get(d_dw, E)
+    fd = (weights(E + h * dE) - weights(E - h * dE)) / (2 * h)
+    assert np.abs(dw - fd).max() <= 1e-8 * max(1.0, np.abs(fd).max())
+    assert abs(dw.sum()) <= 1e-12
