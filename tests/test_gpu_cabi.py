@@ -160,7 +160,9 @@ class GemmJob(ctypes.Structure):
 def test_gemm_batched(cabi):
     """Mixed shapes, all four transposition patterns, padded leading dimensions, alpha/beta."""
     rng = np.random.default_rng(11)
-    shapes = [(1, 1, 1), (5, 7, 3), (64, 64, 64), (65, 33, 129), (130, 70, 17), (17, 200, 64), (3, 5, 0), (257, 129, 300)]
+    shapes = [(1, 1, 1), (5, 7, 3), (64, 64, 64), (65, 33, 129), (130, 70, 17), (17, 200, 64), (3, 5, 0), (257, 129, 300),
+              # larger jobs, every transposition pattern
+              (300, 260, 70), (513, 300, 129), (256, 385, 16), (290, 257, 1)]
     A_parts, B_parts, C_parts, jobs, refs = [], [], [], [], []
     ao = bo = co = 0
     for q, (m, n, k) in enumerate(shapes):
