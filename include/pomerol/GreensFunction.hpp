@@ -43,6 +43,8 @@ public:
                        DensityMatrixPart const& DMpartOuter, RealType PoleResolution, RealType CoefficientTolerance);
     ComplexType operator()(ComplexType z) const;
     ComplexType operator()(long MatsubaraNumber) const { return (*this)(MatsubaraSpacing * RealType(2 * MatsubaraNumber + 1)); }
+    /// G_part(tau), 0 <= tau <= beta (GreensFunctionPart.hpp:219-221).
+    ComplexType of_tau(RealType tau) const;
     std::size_t getNumTerms() const { return std::size_t(n_terms); }
     std::vector<PoleTerm> getTerms() const;
     pomx_pair_part devicePart() const;
@@ -74,6 +76,10 @@ public:
     ComplexType operator()(ComplexType z) const;
     /// G(z) at many points in one device launch.
     std::vector<ComplexType> evaluate(std::vector<ComplexType> const& z) const;
+    /// G(tau), 0 <= tau <= beta (GreensFunction.hpp:138-146).
+    ComplexType of_tau(RealType tau) const;
+    /// G(tau) at many points in one device launch.
+    std::vector<ComplexType> evaluateTau(std::vector<RealType> const& tau) const;
     bool isVanishing() const { return Vanishing; }
     std::size_t getNumParts() const { return parts.size(); }
     std::size_t getNumTerms() const { return std::size_t(n_terms); }
@@ -139,6 +145,8 @@ public:
     ComplexType operator()(ComplexType z) const;
     ComplexType operator()(long MatsubaraNumber) const { return (*this)(MatsubaraSpacing * RealType(2 * MatsubaraNumber)); }
     ComplexType getZeroPoleWeight() const { return ZeroPoleWeight; }
+    /// chi_part(tau), 0 <= tau <= beta, zero-pole weight included (SusceptibilityPart.hpp:222-224).
+    ComplexType of_tau(RealType tau) const;
     std::size_t getNumTerms() const { return std::size_t(n_terms); }
     std::vector<PoleTerm> getTerms() const;
     pomx_pair_part devicePart() const;
@@ -174,8 +182,13 @@ public:
     ComplexType operator()(long MatsubaraNumber) const { return (*this)(MatsubaraSpacing * RealType(2 * MatsubaraNumber)); }
     ComplexType operator()(ComplexType z) const;
     std::vector<ComplexType> evaluate(std::vector<ComplexType> const& z) const;
+    /// chi(tau), 0 <= tau <= beta (Susceptibility.hpp:160-170).
+    ComplexType of_tau(RealType tau) const;
+    /// chi(tau) at many points in one device launch.
+    std::vector<ComplexType> evaluateTau(std::vector<RealType> const& tau) const;
     bool isVanishing() const { return Vanishing; }
     std::size_t getNumParts() const { return parts.size(); }
+    std::vector<SusceptibilityPart> const& getParts() const { return parts; }
     std::size_t getNumTerms() const { return std::size_t(n_terms); }
 };
 

@@ -113,6 +113,15 @@ int pomx_pair_terms(pomx_ctx* ctx, int kind, int nparts, const pomx_pair_part* h
 int pomx_pair_eval(pomx_ctx* ctx, int kind, int64_t nterms, const double* d_residue, const double* d_pole, int nz,
                    const double* d_z, double* d_acc);
 
+/* Imaginary-time Lehmann sums on tau in [0, beta] (d_tau: ntau doubles; d_acc: ntau interleaved
+ * complex, accumulated, imaginary parts untouched):
+ * GF  GreensFunctionPart::Term::operator()(tau, beta) (GreensFunctionPart.cpp:28-31),
+ *     acc += sum_t -R_t e^{-tau P_t} / (1 + e^{-beta P_t});
+ * CHI SusceptibilityPart::Term::operator()(tau, beta) (SusceptibilityPart.cpp:31-35),
+ *     acc += sum_t -R_t e^{-tau P_t} / (e^{-beta P_t} - 1)  (the zero-pole weight is the caller's). */
+int pomx_pair_eval_tau(pomx_ctx* ctx, int kind, int64_t nterms, const double* d_residue, const double* d_pole, int ntau,
+                       const double* d_tau, double beta, double* d_acc);
+
 /* ---- K7 + K8 + K9 two-particle GF part -----------------------------------------------
  * Replaces TwoParticleGFPart::computeImpl/addMultiterm (TwoParticleGFPart.cpp:189-298), the
  * NonResonant/Resonant TermList merges (TwoParticleGFPart.hpp:83-142,208-268) and the

@@ -502,11 +502,25 @@ struct RTerm {
 };
 
 // GreensFunction.cpp:41-86 (parts), GreensFunctionPart.cpp:65-110 (terms), GreensFunction.hpp:123-136
-GFResult ED::gf(int i, int j, std::vector<cplx> const& z, double pole_res, double coeff_tol) const {
+// GreensFunctionPart.cpp:28-31: a term R/(z - P) at imaginary time 0 <= tau <= beta
+static cplx gf_term_tau(cplx R, double P, double tau, double beta) {
+    return P > 0 ? -R * std::exp(-tau * P) / (1.0 + std::exp(-beta * P))
+                 : -R * std::exp((beta - tau) * P) / (std::exp(beta * P) + 1.0);
+}
+
+// SusceptibilityPart.cpp:31-35: a term -R/(z - P) at imaginary time 0 <= tau <= beta
+static cplx chi_term_tau(cplx R, double P, double tau, double beta) {
+    return P > 0 ? -R * std::exp(-tau * P) / std::expm1(-beta * P)
+                 : R * std::exp((beta - tau) * P) / std::expm1(beta * P);
+}
+
+GFResult ED::gf(int i, int j, std::vector<cplx> const& z, double pole_res, double coeff_tol,
+                std::vector<double> const& tau) const {
     RotOp const& F1 = c_op(i);
     RotOp const& F2 = cdag_op(j);
     GFResult out;
     out.values.assign(z.size(), 0.0);
+    out.tau_values.assign(tau.size(), 0.0); // GreensFunction.hpp:138-146
     int nb = int(S.blocks.size());
     for(int L = 0; L < nb; ++L) { // merge-join over F1.left (ascending) and F2.right
         int R = F1.right_of_left[L];
@@ -539,6 +553,12 @@ GFResult ED::gf(int i, int j, std::vector<cplx> const& z, double pole_res, doubl
                 s += t.R / (z[w] - t.P);
             out.values[w] += s;
         }
+        for(std::size_t w = 0; w < tau.size(); ++w) {
+            cplx s = 0;
+            for(auto const& t : terms)
+                s += gf_term_tau(t.R, t.P, tau[w], beta);
+            out.tau_values[w] += s;
+        }
     }
     return out;
 }
@@ -559,10 +579,11 @@ cplx ED::ensemble_average(Mono const& A) const {
 
 // Susceptibility.cpp:42-119, SusceptibilityPart.cpp:24-127, SusceptibilityPart.hpp:216-220
 ChiResult ED::chi(Mono const& A, Mono const& B, std::vector<cplx> const& z, double pole_res, double coeff_tol,
-                  bool subtract_disconnected) const {
+                  bool subtract_disconnected, std::vector<double> const& tau) const {
     RotOp Ar = rotate(A), Br = rotate(B);
     ChiResult out;
     out.values.assign(z.size(), 0.0);
+    out.tau_values.assign(tau.size(), 0.0); // Susceptibility.hpp:160-170, SusceptibilityPart.hpp:222-224
     int nb = int(S.blocks.size());
     for(int L = 0; L < nb; ++L) {
         int R = Ar.right_of_left[L];
@@ -600,12 +621,20 @@ ChiResult ED::chi(Mono const& A, Mono const& B, std::vector<cplx> const& z, doub
                 s += zpw * beta;
             out.values[w] += s;
         }
+        for(std::size_t w = 0; w < tau.size(); ++w) {
+            cplx s = 0;
+            for(auto const& t : terms)
+                s += chi_term_tau(t.R, t.P, tau[w], beta);
+            out.tau_values[w] += s + zpw;
+        }
     }
     if(subtract_disconnected) {
         cplx aA = ensemble_average(A), aB = ensemble_average(B);
         for(std::size_t w = 0; w < z.size(); ++w)
             if(std::abs(z[w]) <= std::numeric_limits<double>::epsilon())
                 out.values[w] -= aA * aB * beta;
+        for(std::size_t w = 0; w < tau.size(); ++w)
+            out.tau_values[w] -= aA * aB;
     }
     return out;
 }

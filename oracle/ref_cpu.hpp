@@ -64,11 +64,13 @@ struct RotOp {
 
 struct GFResult {
     std::vector<cplx> values;
+    std::vector<cplx> tau_values; // G(tau) on the requested imaginary times
     long n_parts = 0, n_terms = 0;
     std::vector<double> poles; std::vector<cplx> residues; // merged terms, part by part
 };
 struct ChiResult {
     std::vector<cplx> values;
+    std::vector<cplx> tau_values; // chi(tau) on the requested imaginary times
     long n_parts = 0, n_terms = 0;
     cplx zero_pole_weight = 0;
 };
@@ -103,9 +105,10 @@ public:
     RotOp rotate(Mono const& op, double tol = 1e-8) const;  // MonomialOperatorPart.cpp:48-92
     static RotOp adjoint(RotOp const& cdag);                  // MonomialOperatorPart.cpp:94-111
 
-    GFResult gf(int i, int j, std::vector<cplx> const& z, double pole_res, double coeff_tol) const;
+    GFResult gf(int i, int j, std::vector<cplx> const& z, double pole_res, double coeff_tol,
+                std::vector<double> const& tau = {}) const;
     ChiResult chi(Mono const& A, Mono const& B, std::vector<cplx> const& z, double pole_res,
-                  double coeff_tol, bool subtract_disconnected) const;
+                  double coeff_tol, bool subtract_disconnected, std::vector<double> const& tau = {}) const;
     cplx ensemble_average(Mono const& A) const; // EnsembleAverage.cpp:27-67
     // 2PGF <c_i c_j c^dag_k c^dag_l> at the frequency triplets (z1, z2, z3) of zs (3 per point).
     G2Result g2(int i, int j, int k, int l, std::vector<cplx> const& zs, double pole_res,

@@ -82,30 +82,34 @@ PYBIND11_MODULE(pomerol_oracle, m) {
              },
              py::arg("op"), py::arg("tol") = 1e-8)
         .def("gf",
-             [](ED const& e, int i, int j, py::array_t<std::complex<double>> z, double pr, double ct) {
-                 GFResult r = e.gf(i, j, zvec(z), pr, ct);
+             [](ED const& e, int i, int j, py::array_t<std::complex<double>> z, double pr, double ct,
+                std::vector<double> tau) {
+                 GFResult r = e.gf(i, j, zvec(z), pr, ct, tau);
                  py::dict d;
                  d["values"] = arr(r.values);
+                 d["tau_values"] = arr(r.tau_values);
                  d["n_parts"] = r.n_parts;
                  d["n_terms"] = r.n_terms;
                  d["poles"] = arr(r.poles);
                  d["residues"] = arr(r.residues);
                  return d;
              },
-             py::arg("i"), py::arg("j"), py::arg("z"), py::arg("pole_res") = 1e-8, py::arg("coeff_tol") = 1e-8)
+             py::arg("i"), py::arg("j"), py::arg("z"), py::arg("pole_res") = 1e-8, py::arg("coeff_tol") = 1e-8,
+             py::arg("tau") = std::vector<double>{})
         .def("chi",
              [](ED const& e, py::list A, py::list B, py::array_t<std::complex<double>> z, double pr, double ct,
-                bool sub) {
-                 ChiResult r = e.chi(to_mono(A), to_mono(B), zvec(z), pr, ct, sub);
+                bool sub, std::vector<double> tau) {
+                 ChiResult r = e.chi(to_mono(A), to_mono(B), zvec(z), pr, ct, sub, tau);
                  py::dict d;
                  d["values"] = arr(r.values);
+                 d["tau_values"] = arr(r.tau_values);
                  d["n_parts"] = r.n_parts;
                  d["n_terms"] = r.n_terms;
                  d["zero_pole_weight"] = r.zero_pole_weight;
                  return d;
              },
              py::arg("A"), py::arg("B"), py::arg("z"), py::arg("pole_res") = 1e-8, py::arg("coeff_tol") = 1e-8,
-             py::arg("subtract_disconnected") = false)
+             py::arg("subtract_disconnected") = false, py::arg("tau") = std::vector<double>{})
         .def("ensemble_average", [](ED const& e, py::list A) { return e.ensemble_average(to_mono(A)); })
         .def("g2",
              [](ED const& e, int i, int j, int k, int l, py::array_t<std::complex<double>> zs, double pr, double ct,

@@ -288,7 +288,84 @@ __global__ void pair_eval_reduce_kernel(int nz, int nslices, double sgn, const d
     acc[2 * f + 1] += sgn * si;
 }
 
+// Imaginary-time Lehmann sums, thread per tau, terms staged through LDS; per-slice partials.
+//   GF  (GreensFunctionPart.cpp:28-31):  P > 0: -R e^{-tau P} / (1 + e^{-beta P}),
+//                                        else   -R e^{(beta - tau) P} / (e^{beta P} + 1);
+//   CHI (SusceptibilityPart.cpp:31-35):  P > 0: -R e^{-tau P} / expm1(-beta P),
+//                                        else    R e^{(beta - tau) P} / expm1(beta P).
+// Both branches are the same function; each keeps its exponents non-positive.
+template <int KIND>
+__global__ void __launch_bounds__(256) pair_tau_kernel(int64_t nterms, int64_t slice_len, const double* __restrict__ R,
+                                                       const double* __restrict__ P, int ntau,
+                                                       const double* __restrict__ tau, double beta,
+                                                       double* __restrict__ partial) {
+    __shared__ double sR[kEvalChunk], sP[kEvalChunk], sD[kEvalChunk];
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t0 = int64_t(blockIdx.y) * slice_len;
+    int64_t t1 = min(nterms, t0 + slice_len);
+    double const tf = (f < ntau) ? tau[f] : 0.0;
+    double acc = 0.0;
+    for(int64_t c0 = t0; c0 < t1; c0 += kEvalChunk) {
+        int64_t cnt = min<int64_t>(kEvalChunk, t1 - c0);
+        __syncthreads();
+        if(threadIdx.x < cnt) {
+            double const r = R[c0 + threadIdx.x], p = P[c0 + threadIdx.x];
+            sP[threadIdx.x] = p;
+            // residue over the statistical denominator, with the branch's sign
+            double d;
+            if(KIND == POMX_KIND_GF)
+                d = (p > 0) ? 1.0 + exp(-beta * p) : exp(beta * p) + 1.0;
+            else
+                d = (p > 0) ? expm1(-beta * p) : -expm1(beta * p);
+            sR[threadIdx.x] = -r / d;
+            sD[threadIdx.x] = (p > 0) ? 0.0 : beta; // exponent origin: -tau P or (beta - tau) P
+        }
+        __syncthreads();
+        for(int k = 0; k < cnt; ++k)
+            acc = fma(sR[k], exp((sD[k] - tf) * sP[k]), acc);
+    }
+    if(f < ntau)
+        partial[int64_t(blockIdx.y) * ntau + f] = acc;
+}
+
+__global__ void pair_tau_reduce_kernel(int ntau, int nslices, const double* __restrict__ partial,
+                                       double* __restrict__ acc) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if(f >= ntau)
+        return;
+    double s = 0;
+    for(int sl = 0; sl < nslices; ++sl)
+        s += partial[int64_t(sl) * ntau + f];
+    acc[2 * f] += s;
+}
+
 } // namespace
+
+extern "C" int pomx_pair_eval_tau(pomx_ctx* ctx, int kind, int64_t nterms, const double* d_residue,
+                                  const double* d_pole, int ntau, const double* d_tau, double beta, double* d_acc) {
+    POMX_REQUIRE(ctx, (kind == POMX_KIND_GF || kind == POMX_KIND_CHI) && nterms >= 0 && ntau >= 0 && beta > 0.0,
+                 "pomx_pair_eval_tau: bad arguments");
+    if(nterms == 0 || ntau == 0)
+        return POMX_OK;
+    int ftiles = (ntau + 255) / 256;
+    int64_t want = std::max<int64_t>(1, 2048 / ftiles);
+    int64_t slice = std::max<int64_t>(kEvalChunk, (nterms + want - 1) / want);
+    slice = (slice + kEvalChunk - 1) / kEvalChunk * kEvalChunk;
+    int nslices = int((nterms + slice - 1) / slice);
+    WsBuf<double> partial(ctx, size_t(nslices) * ntau);
+    if(!partial.p)
+        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_pair_eval_tau: out of device memory");
+    if(kind == POMX_KIND_GF)
+        pair_tau_kernel<POMX_KIND_GF><<<dim3(ftiles, nslices), 256, 0, ctx->stream>>>(nterms, slice, d_residue, d_pole,
+                                                                                    ntau, d_tau, beta, partial);
+    else
+        pair_tau_kernel<POMX_KIND_CHI><<<dim3(ftiles, nslices), 256, 0, ctx->stream>>>(nterms, slice, d_residue, d_pole,
+                                                                                     ntau, d_tau, beta, partial);
+    pair_tau_reduce_kernel<<<grid_for(ntau), kBlock, 0, ctx->stream>>>(ntau, nslices, partial, d_acc);
+    POMX_CHECK_LAUNCH(ctx);
+    POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return POMX_OK;
+}
 
 extern "C" int pomx_pair_terms(pomx_ctx* ctx, int kind, int nparts, const pomx_pair_part* h_parts, double res,
                                double tol, double** d_residue, double** d_pole, int64_t* h_count, double* h_zero_pole) {

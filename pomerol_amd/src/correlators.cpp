@@ -38,6 +38,25 @@ std::vector<ComplexType> pair_eval(int kind, double const* res, double const* po
     return out;
 }
 
+std::vector<ComplexType> pair_eval_tau(int kind, double const* res, double const* pole, int64_t nterms,
+                                       std::vector<RealType> const& tau, RealType beta) {
+    std::vector<ComplexType> out(tau.size(), 0.0);
+    for(RealType t : tau)
+        if(!(t >= 0 && t <= beta))
+            throw std::runtime_error("imaginary time outside [0, beta]");
+    if(nterms == 0 || tau.empty())
+        return out;
+    detail::DeviceArray<double> dt(tau.size()), acc(2 * tau.size());
+    dt.upload(std::vector<double>(tau.begin(), tau.end()));
+    acc.zero();
+    detail::check(pomx_pair_eval_tau(detail::gpu(), kind, nterms, res, pole, int(tau.size()), dt.data(), beta, acc.data()),
+                  "imaginary-time Lehmann sum");
+    std::vector<double> h = acc.download();
+    for(std::size_t k = 0; k < tau.size(); ++k)
+        out[k] = ComplexType(h[2 * k], h[2 * k + 1]);
+    return out;
+}
+
 std::vector<PoleTerm> download_terms(double const* res, double const* pole, int64_t n) {
     std::vector<double> r(static_cast<std::size_t>(n)), p(static_cast<std::size_t>(n));
     if(n) {
@@ -79,6 +98,10 @@ ComplexType GreensFunctionPart::operator()(ComplexType z) const {
 }
 
 std::vector<PoleTerm> GreensFunctionPart::getTerms() const { return download_terms(d_res, d_pole, n_terms); }
+
+ComplexType GreensFunctionPart::of_tau(RealType tau) const {
+    return pair_eval_tau(POMX_KIND_GF, d_res, d_pole, n_terms, {tau}, beta)[0];
+}
 
 GreensFunction::GreensFunction(StatesClassification const& S, Hamiltonian const& H, FieldOperator const& F1,
                                FieldOperator const& F2, DensityMatrix const& DM)
@@ -160,6 +183,16 @@ std::vector<ComplexType> GreensFunction::evaluate(std::vector<ComplexType> const
 
 ComplexType GreensFunction::operator()(ComplexType z) const { return evaluate({z})[0]; }
 
+std::vector<ComplexType> GreensFunction::evaluateTau(std::vector<RealType> const& tau) const {
+    if(Vanishing)
+        return std::vector<ComplexType>(tau.size(), 0.0);
+    if(getStatus() < Computed)
+        throw StatusMismatch("GreensFunction is not computed yet.");
+    return pair_eval_tau(POMX_KIND_GF, d_res->data(), d_pole->data(), n_terms, tau, beta);
+}
+
+ComplexType GreensFunction::of_tau(RealType tau) const { return evaluateTau({tau})[0]; }
+
 // ---- GFContainer ------------------------------------------------------------------------------
 void GFContainer::prepareAll(std::set<IndexCombination2> const& Indices) {
     std::set<IndexCombination2> II = Indices;
@@ -219,6 +252,10 @@ ComplexType SusceptibilityPart::operator()(ComplexType z) const {
 }
 
 std::vector<PoleTerm> SusceptibilityPart::getTerms() const { return download_terms(d_res, d_pole, n_terms); }
+
+ComplexType SusceptibilityPart::of_tau(RealType tau) const {
+    return pair_eval_tau(POMX_KIND_CHI, d_res, d_pole, n_terms, {tau}, beta)[0] + ZeroPoleWeight;
+}
 
 Susceptibility::Susceptibility(StatesClassification const& S, Hamiltonian const& H, MonomialOperator const& A,
                                MonomialOperator const& B, DensityMatrix const& DM)
@@ -320,6 +357,23 @@ std::vector<ComplexType> Susceptibility::evaluate(std::vector<ComplexType> const
 }
 
 ComplexType Susceptibility::operator()(ComplexType z) const { return evaluate({z})[0]; }
+
+std::vector<ComplexType> Susceptibility::evaluateTau(std::vector<RealType> const& tau) const {
+    std::vector<ComplexType> v(tau.size(), 0.0);
+    if(!Vanishing) {
+        if(getStatus() < Computed)
+            throw StatusMismatch("Susceptibility is not computed yet.");
+        v = pair_eval_tau(POMX_KIND_CHI, d_res->data(), d_pole->data(), n_terms, tau, beta);
+        for(auto& x : v)
+            x += zero_pole_total;
+    }
+    if(SubtractDisconnected)
+        for(auto& x : v)
+            x -= ave_A * ave_B;
+    return v;
+}
+
+ComplexType Susceptibility::of_tau(RealType tau) const { return evaluateTau({tau})[0]; }
 
 // ---- TwoParticleGFPart -------------------------------------------------------------------------
 TwoParticleGFPart::TwoParticleGFPart(MonomialOperatorPart const& O1, MonomialOperatorPart const& O2,

@@ -286,7 +286,8 @@ PYBIND11_MODULE(_pomerol, m) {
             }
             return out;
         })
-        .def("gf", [](ED& e, int i, int j, py::array_t<std::complex<double>> z, double pr, double ct) {
+        .def("gf", [](ED& e, int i, int j, py::array_t<std::complex<double>> z, double pr, double ct,
+                      std::vector<double> tau) {
             auto& ops = e.field_ops();
             GreensFunction G(*e.S, *e.H, ops.getAnnihilationOperator(ParticleIndex(i)), ops.getCreationOperator(ParticleIndex(j)), *e.rho);
             G.PoleResolution = pr;
@@ -306,8 +307,17 @@ PYBIND11_MODULE(_pomerol, m) {
                 }
             d["poles"] = to_np(poles);
             d["residues"] = to_np(res);
+            if(!tau.empty()) {
+                d["tau_values"] = to_np(G.evaluateTau(tau));
+                ComplexType s0 = 0; // GreensFunction::of_tau as the sum of the parts' of_tau
+                for(auto const& p : G.getParts())
+                    s0 += p.of_tau(tau[0]);
+                d["tau0_parts"] = s0;
+                d["tau0"] = G.of_tau(tau[0]);
+            }
             return d;
-        }, py::arg("i"), py::arg("j"), py::arg("z"), py::arg("pole_res") = 1e-8, py::arg("coeff_tol") = 1e-8)
+        }, py::arg("i"), py::arg("j"), py::arg("z"), py::arg("pole_res") = 1e-8, py::arg("coeff_tol") = 1e-8,
+           py::arg("tau") = std::vector<double>{})
         .def("gf_pointwise", [](ED& e, int i, int j, std::vector<long> n) {
             auto& ops = e.field_ops();
             GreensFunction G(*e.S, *e.H, ops.getAnnihilationOperator(ParticleIndex(i)), ops.getCreationOperator(ParticleIndex(j)), *e.rho);
@@ -318,7 +328,8 @@ PYBIND11_MODULE(_pomerol, m) {
                 v.push_back(G(k));
             return to_np(v);
         })
-        .def("chi", [](ED& e, py::list A, py::list B, py::array_t<std::complex<double>> z, double pr, double ct, bool sub) {
+        .def("chi", [](ED& e, py::list A, py::list B, py::array_t<std::complex<double>> z, double pr, double ct, bool sub,
+                       std::vector<double> tau) {
             MonomialOperator& MA = e.mono(A);
             MonomialOperator& MB = e.mono(B);
             Susceptibility X(*e.S, *e.H, MA, MB, *e.rho);
@@ -332,9 +343,17 @@ PYBIND11_MODULE(_pomerol, m) {
             d["values"] = to_np(X.evaluate(to_points(z)));
             d["n_parts"] = X.getNumParts();
             d["n_terms"] = X.getNumTerms();
+            if(!tau.empty()) {
+                d["tau_values"] = to_np(X.evaluateTau(tau));
+                ComplexType s0 = 0; // Susceptibility::of_tau as the sum of the parts' of_tau
+                for(auto const& p : X.getParts())
+                    s0 += p.of_tau(tau[0]);
+                d["tau0_parts"] = s0;
+                d["tau0"] = X.of_tau(tau[0]);
+            }
             return d;
         }, py::arg("A"), py::arg("B"), py::arg("z"), py::arg("pole_res") = 1e-8, py::arg("coeff_tol") = 1e-8,
-           py::arg("subtract_disconnected") = false)
+           py::arg("subtract_disconnected") = false, py::arg("tau") = std::vector<double>{})
         .def("ensemble_average", [](ED& e, py::list A) {
             MonomialOperator MA(e.mono_expr(A), *e.HS, *e.S, *e.H);
             MA.prepare(*e.HS);

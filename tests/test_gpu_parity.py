@@ -193,3 +193,28 @@ def test_2pgf_container_crossing_images(pa):
     for ijkl in images:
         direct = e.g2(*ijkl, zs, 1e-8, 1e-16)["values"]
         assert np.abs(out[ijkl] - direct).max() < 1e-11 * max(1.0, np.abs(direct).max()), ijkl
+
+
+@pytest.mark.parametrize("name", ["anderson_test", "cfg1"])
+def test_imaginary_time(pa, oracle, name):
+    """G(tau) and chi(tau) (GreensFunction::of_tau, Susceptibility::of_tau and the parts') on the
+    device against the oracle, including both ends of [0, beta]."""
+    H, beta = MODELS[name]()
+    e, o = build(pa, oracle, H, beta)
+    tau = list(np.linspace(0.0, beta, 41))
+    z = fermionic(np.arange(4), beta)
+    for i in range(2):
+        g = e.gf(i, i, z, 1e-8, 1e-8, tau)
+        ref = o.gf(i, i, z, 1e-8, 1e-8, tau)
+        assert rel_err(g["tau_values"], ref["tau_values"]) < 1e-10
+        assert abs(g["tau0_parts"] - g["tau0"]) < 1e-13 and abs(g["tau0"] - ref["tau_values"][0]) < 1e-10
+    idx = H.indices()
+    u, d = idx.index(("C", 0, 1)), idx.index(("C", 0, 0))
+    zb = bosonic(np.arange(2), beta)
+    for A, B in [([(True, u), (False, d)], [(True, d), (False, u)]), ([(True, u), (False, u)], [(True, d), (False, d)])]:
+        for sub in (False, True):
+            v = e.chi(A, B, zb, 1e-8, 1e-8, sub, tau)
+            r = o.chi(A, B, zb, 1e-8, 1e-8, sub, tau)
+            assert rel_err(v["tau_values"], r["tau_values"]) < 1e-10
+            if not sub:
+                assert abs(v["tau0_parts"] - v["tau0"]) < 1e-13

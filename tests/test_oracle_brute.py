@@ -137,3 +137,47 @@ def test_planted_sign_error_is_detected(oracle):
     bad = fe.g2(1, 0, 1, 0, zs)
     v = ed.g2(1, 0, 1, 0, zs.ravel(), 1e-8, 0.0)["values"]
     assert rel_err(v, good) < 1e-11 and rel_err(v, bad) > 1e-6
+
+
+def _gauss_legendre_grid(beta, panels=60, order=24):
+    """Composite Gauss-Legendre nodes and weights on [0, beta] (imaginary-time quadrature)."""
+    x, w = np.polynomial.legendre.leggauss(order)
+    edges = np.linspace(0.0, beta, panels + 1)
+    nodes = np.concatenate([(a + b) / 2 + (b - a) / 2 * x for a, b in zip(edges[:-1], edges[1:])])
+    weights = np.concatenate([(b - a) / 2 * w for a, b in zip(edges[:-1], edges[1:])])
+    return nodes, weights
+
+
+@pytest.mark.parametrize("model", ["dimer", "anderson2"])
+def test_gf_tau_fourier_and_sum_rule(oracle, model):
+    """G(tau) of the oracle: its Fourier integral int_0^beta e^{i w_n tau} G(tau) dtau (numerical
+    quadrature) reproduces G(i w_n), and G(0) + G(beta) = -<{c, c^+}> = -1."""
+    H, beta = (dimer_model(), 3.0) if model == "dimer" else small_anderson(2)
+    ed = oracle.ED(len(H.indices()), H.mode_terms(), beta)
+    tau, wq = _gauss_legendre_grid(beta)
+    z = fermionic(np.arange(-4, 12), beta)
+    for i in (0, 1):
+        r = ed.gf(i, i, z, 1e-8, 0.0, list(tau) + [0.0, beta])
+        g_tau, ends = r["tau_values"][:-2], r["tau_values"][-2:]
+        fourier = np.array([np.sum(wq * np.exp(zz * tau) * g_tau) for zz in z])
+        assert rel_err(fourier, r["values"]) < 1e-10
+        assert abs(ends[0] + ends[1] + 1.0) < 1e-12
+
+
+def test_chi_tau_fourier_and_symmetry(oracle):
+    """chi(tau) of the oracle: its Fourier integral reproduces chi(i nu_m) (zero-pole weight and the
+    disconnected part included), and chi(tau) = chi(beta - tau) for <n_up(tau) n_dn>."""
+    H, beta = small_anderson(2)
+    ed = oracle.ED(len(H.indices()), H.mode_terms(), beta)
+    tau, wq = _gauss_legendre_grid(beta)
+    z = bosonic(np.arange(0, 10), beta)
+    nu, nd = [(True, 1), (False, 1)], [(True, 0), (False, 0)]
+    sp, sm = [(True, 1), (False, 0)], [(True, 0), (False, 1)]
+    for A, B in [(nu, nd), (sp, sm)]:
+        for sub in (False, True):
+            r = ed.chi(A, B, z, 1e-8, 0.0, sub, list(tau))
+            fourier = np.array([np.sum(wq * np.exp(zz * tau) * r["tau_values"]) for zz in z])
+            assert rel_err(fourier, r["values"]) < 1e-10
+    t = np.linspace(0.0, beta, 9)
+    r = ed.chi(nu, nd, z[:1], 1e-8, 0.0, False, list(t))
+    assert np.abs(r["tau_values"] - r["tau_values"][::-1]).max() < 1e-12
