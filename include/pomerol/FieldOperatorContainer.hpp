@@ -1,0 +1,4 @@
+// FieldOperatorContainer (FieldOperatorContainer.hpp): the reference's header name for a class this library declares in MonomialOperator.hpp.
+#pragma once
+
+#include "MonomialOperator.hpp"

@@ -1,0 +1,4 @@
+// IndexClassification, MakeIndexClassification (IndexClassification.hpp): the reference's header name for a class this library declares in HilbertSpace.hpp.
+#pragma once
+
+#include "HilbertSpace.hpp"

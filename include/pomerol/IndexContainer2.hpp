@@ -1,0 +1,4 @@
+// IndexCombination2 and the GF container (IndexContainer2.hpp): the reference's header name for a class this library declares in GreensFunction.hpp.
+#pragma once
+
+#include "GreensFunction.hpp"

@@ -1,0 +1,4 @@
+// MonomialOperatorPart (MonomialOperatorPart.hpp): the reference's header name for a class this library declares in MonomialOperator.hpp.
+#pragma once
+
+#include "MonomialOperator.hpp"

@@ -1,0 +1,78 @@
+// A pomerol caller written against the reference's header names: compiled and linked against
+// this library by tests/test_library.py (no device is needed to build it).
+#include <pomerol/DensityMatrix.hpp>
+#include <pomerol/EnsembleAverage.hpp>
+#include <pomerol/FieldOperatorContainer.hpp>
+#include <pomerol/GFContainer.hpp>
+#include <pomerol/GreensFunction.hpp>
+#include <pomerol/Hamiltonian.hpp>
+#include <pomerol/HilbertSpace.hpp>
+#include <pomerol/IndexClassification.hpp>
+#include <pomerol/LatticePresets.hpp>
+#include <pomerol/Misc.hpp>
+#include <pomerol/StatesClassification.hpp>
+#include <pomerol/Susceptibility.hpp>
+#include <pomerol/TwoParticleGFContainer.hpp>
+
+#include <complex>
+#include <iostream>
+#include <set>
+
+using namespace Pomerol;
+
+int main() {
+    using namespace LatticePresets;
+    RealType const U = 1.0, mu = 0.4, beta = 10.0;
+    auto HExpr = CoulombS("A", U, -mu) + Level("B", 0.3) + Hopping("A", "B", 0.5);
+    auto IndexInfo = MakeIndexClassification(HExpr);
+    auto HS = MakeHilbertSpace(IndexInfo, HExpr);
+    HS.compute();
+    StatesClassification S;
+    S.compute(HS);
+    Hamiltonian H(S);
+    H.prepare(HExpr, HS, MPI_COMM_WORLD);
+    H.compute(MPI_COMM_WORLD);
+    DensityMatrix rho(S, H, beta);
+    rho.prepare();
+    rho.compute();
+    rho.truncateBlocks(1e-12, false);
+
+    FieldOperatorContainer Operators(IndexInfo, HS, S, H);
+    Operators.prepareAll(HS);
+    Operators.computeAll();
+    ParticleIndex iu = IndexInfo.getIndex("A", 0, up), id = IndexInfo.getIndex("A", 0, down);
+
+    GreensFunction GF(S, H, Operators.getAnnihilationOperator(id), Operators.getCreationOperator(id), rho);
+    GF.prepare();
+    GF.compute();
+    std::complex<double> g0 = GF(0), gt = GF.of_tau(0.5 * beta);
+
+    GFContainer G(IndexInfo, S, H, rho, Operators);
+    std::set<IndexCombination2> pairs = {IndexCombination2(iu, iu), IndexCombination2(id, id)};
+    G.prepareAll(pairs);
+    G.computeAll();
+
+    QuadraticOperator SpOp(IndexInfo, HS, S, H, iu, id), SmOp(IndexInfo, HS, S, H, id, iu);
+    SpOp.prepare(HS);
+    SpOp.compute();
+    SmOp.prepare(HS);
+    SmOp.compute();
+    Susceptibility Chi(S, H, SpOp, SmOp, rho);
+    Chi.prepare();
+    Chi.compute();
+    Chi.subtractDisconnected();
+    std::complex<double> chi0 = Chi(0), chit = Chi.of_tau(0.0);
+    EnsembleAverage Ave(SpOp, rho);
+    Ave.compute();
+
+    TwoParticleGFContainer Chi4(IndexInfo, S, H, rho, Operators);
+    Chi4.PoleResolution = 1e-8;
+    Chi4.CoefficientTolerance = 1e-16;
+    std::set<IndexCombination4> quads = {IndexCombination4(iu, id, iu, id)};
+    Chi4.prepareAll(quads);
+    Chi4.computeAll(false, FreqVec3{}, MPI_COMM_WORLD);
+    std::complex<double> x = Chi4(IndexCombination4(iu, id, iu, id))(0, 0, 0);
+
+    std::cout << g0 << gt << G(iu, iu)(3) << chi0 << chit << Ave() << x << std::endl;
+    return 0;
+}
