@@ -718,6 +718,161 @@ __global__ void __launch_bounds__(256) g2_eval_kernel(int64_t nterms, int64_t sl
     }
 }
 
+// ---- evaluation on grids of shared fixed slots ----------------------------------------------
+// When every aligned group of kFPT consecutive triplets shares z1 and z3 (the Matsubara grid of
+// the reference's programs: nu outermost, w3, then w2 innermost), a thread's kFPT frequencies
+// differ only in F2 = z2.  Under the part's permutation F2 lands in one slot S of the term, and
+// for most (kind, S) only one denominator factor V = a + i y depends on it; the others form a
+// fixed product G per term and thread.  Then
+//   term = Kc / (G V) = K conj(V) / |V|^2,  K = Kc / G  (once per term and thread),
+// so a frequency costs |V|^2, a quarter of a shared reciprocal, two FMAs for K conj(V) and two
+// for the accumulation (about 9 FP64 instructions instead of 14).  Kinds and slots:
+//   kind 0 (D1 D2 D3): any S, V = D_S;               kind 1 (D1 D2' D3): S = 1, V = D2';
+//   kind 2 (D1 D3 [E = z2+z3-P2-P3]): S = 0, V = D1 (E fixed); S = 1, V = E;
+//   kind 3 (D1 D3 [E = z1+z2-P1-P2]): S = 1, V = E; S = 2, V = D3 (E fixed).
+// Resonance (|E| <= res): with E fixed it selects the coefficient and whether E enters G; with E
+// varying, terms with (P-sum)^2 > res^2 can never resonate (uniform fast path), the others take
+// a per-frequency path.
+__host__ __device__ constexpr bool g2_structured(int kind, int s) {
+    return kind == 0 || (kind == 1 && s == 1) || (kind == 2 && s <= 1) || (kind == 3 && s >= 1);
+}
+
+struct Cplx {
+    double r, i;
+};
+__device__ inline Cplx cx_mul(Cplx a, Cplx b) { return Cplx{fma(a.r, b.r, -a.i * b.i), fma(a.r, b.i, a.i * b.r)}; }
+
+template <int KIND, int S>
+__global__ void __launch_bounds__(256) g2_eval_s_kernel(int64_t nterms, int64_t slice_len, const double* __restrict__ T,
+                                                        int64_t cap, int nfreq, const double* __restrict__ fz, int perm,
+                                                        double res, double* __restrict__ partial) {
+    constexpr int NF = KIND < 2 ? 4 : 5;           // NR (C, P1, P2, P3); R (Rc, Nc, P1, P2, P3)
+    constexpr bool EVAR = (KIND == 2 && S == 1) || (KIND == 3 && S == 1); // V = E
+    __shared__ double sT[NF][kChunk];
+    int const fbase = (blockIdx.x * blockDim.x + threadIdx.x) * kFPT;
+    int64_t const t0 = int64_t(blockIdx.y) * slice_len;
+    int64_t const t1 = min(nterms, t0 + slice_len);
+    double const res2 = res * res;
+    // imaginary parts of the fixed slots (shared by the group) and of the varying factor
+    Freq const q0 = load_freq(fz, min(fbase, nfreq - 1), perm);
+    double const x1 = q0.z1i, x2 = q0.z2i, x3 = q0.z3i;
+    double yv[kFPT], ar[kFPT], ai[kFPT];
+#pragma unroll
+    for(int u = 0; u < kFPT; ++u) {
+        Freq const q = load_freq(fz, min(fbase + u, nfreq - 1), perm);
+        if(KIND == 1)
+            yv[u] = (q.z1i + q.z2i) + q.z3i; // Im(z1 + z2 + z3)
+        else if(KIND == 2 && S == 1)
+            yv[u] = q.z2i + q.z3i; // Im E
+        else if(KIND == 3 && S == 1)
+            yv[u] = q.z1i + q.z2i; // Im E
+        else
+            yv[u] = (S == 0) ? q.z1i : (S == 1) ? q.z2i : q.z3i;
+        ar[u] = 0.0;
+        ai[u] = 0.0;
+    }
+    for(int64_t c0 = t0; c0 < t1; c0 += kChunk) {
+        int const cnt = int(min<int64_t>(kChunk, t1 - c0));
+        __syncthreads();
+        if(int(threadIdx.x) < cnt)
+#pragma unroll
+            for(int f = 0; f < NF; ++f)
+                sT[f][threadIdx.x] = T[f * cap + c0 + threadIdx.x];
+        __syncthreads();
+#pragma unroll 2
+        for(int k = 0; k < cnt; ++k) {
+            double const P1 = sT[NF - 3][k], P2 = sT[NF - 2][k], P3 = sT[NF - 1][k];
+            Cplx const D1{-P1, x1}, D3{-P3, x3};
+            double Kc, av;
+            Cplx G;
+            if(KIND == 0) {
+                Cplx const D2{-P2, x2};
+                Kc = sT[0][k];
+                av = (S == 0) ? -P1 : (S == 1) ? -P2 : -P3;
+                G = (S == 0) ? cx_mul(D2, D3) : (S == 1) ? cx_mul(D1, D3) : cx_mul(D1, D2);
+            } else if(KIND == 1) {
+                Kc = sT[0][k];
+                av = ((-P1) - P2) - P3;
+                G = cx_mul(D1, D3);
+            } else if(EVAR) {
+                Kc = sT[1][k]; // non-resonant coefficient; resonant terms are handled below
+                av = (KIND == 3) ? (-P1) - P2 : (-P2) - P3;
+                G = cx_mul(D1, D3);
+            } else {
+                // E fixed per thread: resonance selects the coefficient and whether E enters G
+                Cplx const E = (KIND == 2) ? Cplx{(-P2) - P3, x2 + x3} : Cplx{(-P1) - P2, x1 + x2};
+                Cplx const F = (KIND == 2) ? D3 : D1;
+                bool const resonant = fma(E.r, E.r, E.i * E.i) <= res2;
+                Kc = resonant ? sT[0][k] : sT[1][k];
+                av = (KIND == 2) ? -P1 : -P3;
+                G = resonant ? F : cx_mul(F, E);
+            }
+            // K = Kc / G
+            double const ig = Kc * rcp_nr(fma(G.r, G.r, G.i * G.i));
+            double const Kr = G.r * ig, Ki = -G.i * ig;
+            double const a2 = av * av, Kra = Kr * av, Kia = Ki * av;
+            if(!EVAR || a2 > res2) {
+#pragma unroll
+                for(int u = 0; u < kFPT; u += 4) {
+                    double q[4];
+#pragma unroll
+                    for(int h = 0; h < 4; ++h)
+                        q[h] = fma(yv[u + h], yv[u + h], a2); // |V|^2
+                    double const q01 = q[0] * q[1], q23 = q[2] * q[3];
+                    double const r = rcp_nr(q01 * q23);
+                    double const r01 = q23 * r, r23 = q01 * r;
+                    double const s[4] = {q[1] * r01, q[0] * r01, q[3] * r23, q[2] * r23};
+#pragma unroll
+                    for(int h = 0; h < 4; ++h) {
+                        ar[u + h] = fma(s[h], fma(Ki, yv[u + h], Kra), ar[u + h]);  // Re(K conj(V)) / |V|^2
+                        ai[u + h] = fma(s[h], fma(-Kr, yv[u + h], Kia), ai[u + h]); // Im(K conj(V)) / |V|^2
+                    }
+                }
+            } else {
+                // V = E may resonate at some frequency: Rc / G there, Nc / (G E) elsewhere
+                double const igr = sT[0][k] * rcp_nr(fma(G.r, G.r, G.i * G.i));
+                double const Rr = G.r * igr, Ri = -G.i * igr;
+#pragma unroll
+                for(int u = 0; u < kFPT; ++u) {
+                    double const qv = fma(yv[u], yv[u], a2);
+                    if(qv <= res2) {
+                        ar[u] += Rr;
+                        ai[u] += Ri;
+                    } else {
+                        double const s = rcp_nr(qv);
+                        ar[u] = fma(s, fma(Ki, yv[u], Kra), ar[u]);
+                        ai[u] = fma(s, fma(-Kr, yv[u], Kia), ai[u]);
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for(int u = 0; u < kFPT; ++u) {
+        int f = fbase + u;
+        if(f < nfreq) {
+            int64_t o = (int64_t(blockIdx.y) * nfreq + f) * 2;
+            partial[o] += ar[u];
+            partial[o + 1] += ai[u];
+        }
+    }
+}
+
+// 1 in flag[0] if some aligned group of kFPT triplets does not share z1 and z3
+__global__ void group_mismatch_kernel(int nfreq, const double* __restrict__ fz, int* __restrict__ flag) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if(f >= nfreq)
+        return;
+    int const g = f - f % kFPT;
+    bool same = true;
+    for(int c = 0; c < 2; ++c) { // z1 (doubles 0, 1) and z3 (doubles 4, 5)
+        int const o = (c == 0) ? 0 : 4;
+        same = same && fz[6 * f + o] == fz[6 * g + o] && fz[6 * f + o + 1] == fz[6 * g + o + 1];
+    }
+    if(!same)
+        flag[0] = 1;
+}
+
 __global__ void g2_eval_reduce_kernel(int nfreq, int nslices, const double* __restrict__ partial, double* __restrict__ acc) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if(f >= nfreq)
@@ -885,6 +1040,25 @@ void launch_kind(pomx_ctx* ctx, int k, dim3 grid, TermSpan const& s, int64_t sli
     }
 }
 
+void launch_kind_s(pomx_ctx* ctx, int k, int slot, dim3 grid, TermSpan const& s, int64_t slice, int nfreq,
+                   const double* d_freq, int perm, double res, double* partial) {
+    const double* T = s.base + s.begin;
+#define POMX_G2S(K, S)                                                                                                    \
+    g2_eval_s_kernel<K, S><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, nfreq, d_freq, perm, res, partial)
+    switch(k * 3 + slot) {
+    case 0: POMX_G2S(0, 0); break;
+    case 1: POMX_G2S(0, 1); break;
+    case 2: POMX_G2S(0, 2); break;
+    case 4: POMX_G2S(1, 1); break;
+    case 6: POMX_G2S(2, 0); break;
+    case 7: POMX_G2S(2, 1); break;
+    case 10: POMX_G2S(3, 1); break;
+    case 11: POMX_G2S(3, 2); break;
+    default: break; // not a structured (kind, slot): the caller launches the general kernel
+    }
+#undef POMX_G2S
+}
+
 int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res, int nfreq, const double* d_freq,
                    double* d_acc) {
     int64_t total = 0;
@@ -892,14 +1066,21 @@ int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res,
         total += spans[k].count;
     if(total == 0 || nfreq == 0)
         return POMX_OK;
-    WsBuf<int> flag(ctx, 1);
-    int rc = pomx_fill_zero(ctx, flag.p, sizeof(int));
+    WsBuf<int> flag(ctx, 2);
+    int rc = pomx_fill_zero(ctx, flag.p, 2 * sizeof(int));
     if(rc)
         return rc;
     any_real_kernel<<<grid_for(3 * nfreq), kBlock, 0, ctx->stream>>>(3 * nfreq, d_freq, flag);
-    int has_real = 0;
-    POMX_HIP(ctx, hipMemcpyAsync(&has_real, flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    group_mismatch_kernel<<<grid_for(nfreq), kBlock, 0, ctx->stream>>>(nfreq, d_freq, flag.p + 1);
+    int hflag[2] = {0, 0};
+    POMX_HIP(ctx, hipMemcpyAsync(hflag, flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int const has_real = hflag[0];
+    // purely imaginary triplets whose aligned groups share z1 and z3: the slot of F2 = z2 under
+    // this part's permutation decides which kinds take the grouped kernel
+    bool const grouped = !has_real && !hflag[1] && nfreq % kFPT == 0;
+    static constexpr int perm3[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    int const slot = (perm3[perm][0] == 1) ? 0 : (perm3[perm][1] == 1) ? 1 : 2;
     int ftiles = (nfreq + 256 * kFPT - 1) / (256 * kFPT);
     int64_t want = std::max<int64_t>(1, 4096 / ftiles);
     int64_t slice = std::max<int64_t>(kChunk, (total + want - 1) / want);
@@ -916,7 +1097,9 @@ int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res,
         if(!s.count)
             continue;
         dim3 grid(ftiles, unsigned((s.count + slice - 1) / slice));
-        if(has_real)
+        if(grouped && g2_structured(k, slot))
+            launch_kind_s(ctx, k, slot, grid, s, slice, nfreq, d_freq, perm, res, partial);
+        else if(has_real)
             launch_kind<false>(ctx, k, grid, s, slice, nfreq, d_freq, perm, res, partial);
         else
             launch_kind<true>(ctx, k, grid, s, slice, nfreq, d_freq, perm, res, partial);
