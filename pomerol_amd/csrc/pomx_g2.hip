@@ -742,24 +742,29 @@ struct Cplx {
 };
 __device__ inline Cplx cx_mul(Cplx a, Cplx b) { return Cplx{fma(a.r, b.r, -a.i * b.i), fma(a.r, b.i, a.i * b.r)}; }
 
-template <int KIND, int S>
+// STRIDED: the thread's frequencies are f = t + u * stride (u < kFPT) and they share z2 and z3
+// (grids whose outer index is the bosonic frequency): then F1 = z1 is the varying one, and S is
+// its slot.  Otherwise f = kFPT t + u, sharing z1 and z3, with F2 varying.
+template <int KIND, int S, bool STRIDED>
 __global__ void __launch_bounds__(256) g2_eval_s_kernel(int64_t nterms, int64_t slice_len, const double* __restrict__ T,
-                                                        int64_t cap, int nfreq, const double* __restrict__ fz, int perm,
-                                                        double res, double* __restrict__ partial) {
+                                                        int64_t cap, int nfreq, int stride, const double* __restrict__ fz,
+                                                        int perm, double res, double* __restrict__ partial) {
     constexpr int NF = KIND < 2 ? 4 : 5;           // NR (C, P1, P2, P3); R (Rc, Nc, P1, P2, P3)
     constexpr bool EVAR = (KIND == 2 && S == 1) || (KIND == 3 && S == 1); // V = E
     __shared__ double sT[NF][kChunk];
-    int const fbase = (blockIdx.x * blockDim.x + threadIdx.x) * kFPT;
+    int const tid = blockIdx.x * blockDim.x + threadIdx.x;
+    int const fbase = STRIDED ? tid : tid * kFPT, fstep = STRIDED ? stride : 1;
+    bool const live = STRIDED ? tid < stride : fbase < nfreq;
     int64_t const t0 = int64_t(blockIdx.y) * slice_len;
     int64_t const t1 = min(nterms, t0 + slice_len);
     double const res2 = res * res;
     // imaginary parts of the fixed slots (shared by the group) and of the varying factor
-    Freq const q0 = load_freq(fz, min(fbase, nfreq - 1), perm);
+    Freq const q0 = load_freq(fz, live ? fbase : 0, perm);
     double const x1 = q0.z1i, x2 = q0.z2i, x3 = q0.z3i;
     double yv[kFPT], ar[kFPT], ai[kFPT];
 #pragma unroll
     for(int u = 0; u < kFPT; ++u) {
-        Freq const q = load_freq(fz, min(fbase + u, nfreq - 1), perm);
+        Freq const q = load_freq(fz, live ? min(fbase + u * fstep, nfreq - 1) : 0, perm);
         if(KIND == 1)
             yv[u] = (q.z1i + q.z2i) + q.z3i; // Im(z1 + z2 + z3)
         else if(KIND == 2 && S == 1)
@@ -849,13 +854,26 @@ __global__ void __launch_bounds__(256) g2_eval_s_kernel(int64_t nterms, int64_t 
     }
 #pragma unroll
     for(int u = 0; u < kFPT; ++u) {
-        int f = fbase + u;
-        if(f < nfreq) {
+        int f = fbase + u * fstep;
+        if(live && f < nfreq) {
             int64_t o = (int64_t(blockIdx.y) * nfreq + f) * 2;
             partial[o] += ar[u];
             partial[o + 1] += ai[u];
         }
     }
+}
+
+// 1 in flag[0] if some triplet f does not share z2 and z3 with triplet f mod stride
+__global__ void stride_mismatch_kernel(int nfreq, int stride, const double* __restrict__ fz, int* __restrict__ flag) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if(f >= nfreq)
+        return;
+    int const g = f % stride;
+    bool same = true;
+    for(int o = 2; o < 6; ++o) // z2 (doubles 2, 3) and z3 (doubles 4, 5)
+        same = same && fz[6 * f + o] == fz[6 * g + o];
+    if(!same)
+        flag[0] = 1;
 }
 
 // 1 in flag[0] if some aligned group of kFPT triplets does not share z1 and z3
@@ -1040,11 +1058,13 @@ void launch_kind(pomx_ctx* ctx, int k, dim3 grid, TermSpan const& s, int64_t sli
     }
 }
 
-void launch_kind_s(pomx_ctx* ctx, int k, int slot, dim3 grid, TermSpan const& s, int64_t slice, int nfreq,
+template <bool STRIDED>
+void launch_kind_s(pomx_ctx* ctx, int k, int slot, dim3 grid, TermSpan const& s, int64_t slice, int nfreq, int stride,
                    const double* d_freq, int perm, double res, double* partial) {
     const double* T = s.base + s.begin;
 #define POMX_G2S(K, S)                                                                                                    \
-    g2_eval_s_kernel<K, S><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, nfreq, d_freq, perm, res, partial)
+    g2_eval_s_kernel<K, S, STRIDED><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, nfreq, stride, d_freq, perm, \
+                                                                     res, partial)
     switch(k * 3 + slot) {
     case 0: POMX_G2S(0, 0); break;
     case 1: POMX_G2S(0, 1); break;
@@ -1066,21 +1086,29 @@ int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res,
         total += spans[k].count;
     if(total == 0 || nfreq == 0)
         return POMX_OK;
-    WsBuf<int> flag(ctx, 2);
-    int rc = pomx_fill_zero(ctx, flag.p, 2 * sizeof(int));
+    int const stride = nfreq / kFPT;
+    bool const divisible = nfreq % kFPT == 0;
+    WsBuf<int> flag(ctx, 3);
+    int rc = pomx_fill_zero(ctx, flag.p, 3 * sizeof(int));
     if(rc)
         return rc;
     any_real_kernel<<<grid_for(3 * nfreq), kBlock, 0, ctx->stream>>>(3 * nfreq, d_freq, flag);
-    group_mismatch_kernel<<<grid_for(nfreq), kBlock, 0, ctx->stream>>>(nfreq, d_freq, flag.p + 1);
-    int hflag[2] = {0, 0};
-    POMX_HIP(ctx, hipMemcpyAsync(hflag, flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    if(divisible) {
+        group_mismatch_kernel<<<grid_for(nfreq), kBlock, 0, ctx->stream>>>(nfreq, d_freq, flag.p + 1);
+        stride_mismatch_kernel<<<grid_for(nfreq), kBlock, 0, ctx->stream>>>(nfreq, stride, d_freq, flag.p + 2);
+    }
+    int hflag[3] = {0, 0, 0};
+    POMX_HIP(ctx, hipMemcpyAsync(hflag, flag.p, 3 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     int const has_real = hflag[0];
-    // purely imaginary triplets whose aligned groups share z1 and z3: the slot of F2 = z2 under
-    // this part's permutation decides which kinds take the grouped kernel
-    bool const grouped = !has_real && !hflag[1] && nfreq % kFPT == 0;
+    // purely imaginary triplets whose aligned groups share z1 and z3 (F2 = z2 varies inside a
+    // group), or whose stride-apart triplets share z2 and z3 (F1 = z1 varies): the slot of the
+    // varying F under this part's permutation decides which kinds take the grouped kernels
+    bool const grouped = !has_real && divisible && !hflag[1];
+    bool const strided = !has_real && divisible && !hflag[2];
     static constexpr int perm3[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
-    int const slot = (perm3[perm][0] == 1) ? 0 : (perm3[perm][1] == 1) ? 1 : 2;
+    int const slot2 = (perm3[perm][0] == 1) ? 0 : (perm3[perm][1] == 1) ? 1 : 2; // slot of F2
+    int const slot1 = (perm3[perm][0] == 0) ? 0 : (perm3[perm][1] == 0) ? 1 : 2; // slot of F1
     int ftiles = (nfreq + 256 * kFPT - 1) / (256 * kFPT);
     int64_t want = std::max<int64_t>(1, 4096 / ftiles);
     int64_t slice = std::max<int64_t>(kChunk, (total + want - 1) / want);
@@ -1097,8 +1125,10 @@ int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res,
         if(!s.count)
             continue;
         dim3 grid(ftiles, unsigned((s.count + slice - 1) / slice));
-        if(grouped && g2_structured(k, slot))
-            launch_kind_s(ctx, k, slot, grid, s, slice, nfreq, d_freq, perm, res, partial);
+        if(grouped && g2_structured(k, slot2))
+            launch_kind_s<false>(ctx, k, slot2, grid, s, slice, nfreq, stride, d_freq, perm, res, partial);
+        else if(strided && g2_structured(k, slot1))
+            launch_kind_s<true>(ctx, k, slot1, grid, s, slice, nfreq, stride, d_freq, perm, res, partial);
         else if(has_real)
             launch_kind<false>(ctx, k, grid, s, slice, nfreq, d_freq, perm, res, partial);
         else

@@ -220,17 +220,19 @@ def test_imaginary_time(pa, oracle, name):
                 assert abs(v["tau0_parts"] - v["tau0"]) < 1e-13
 
 
+@pytest.mark.parametrize("grid", [(16, 4), (8, 8)])
 @pytest.mark.parametrize("comp", ["uuuu", "udud", "duud", "uddu"])
-def test_2pgf_grouped_grid_matches_general_path(pa, comp):
-    """The Matsubara grid of aligned groups sharing (z1, z3) takes the grouped evaluation kernel;
-    the same triplets in a shuffled order take the general kernel.  Both must agree."""
+def test_2pgf_grouped_grid_matches_general_path(pa, comp, grid):
+    """The Matsubara grid takes the grouped evaluation kernels (aligned groups sharing (z1, z3);
+    with 8 bosonic frequencies also triplets a stride apart sharing (z2, z3)); the same triplets
+    in a shuffled order take the general kernel.  Both must agree."""
     c = W.cfg1()
     H = c.expr()
     idx = H.indices()
     u, d = idx.index(("C", 0, 1)), idx.index(("C", 0, 0))
     ijkl = {"uuuu": (u, u, u, u), "udud": (u, d, u, d), "duud": (d, u, u, d), "uddu": (u, d, d, u)}[comp]
     e = pa.ED(W.terms(H), c.beta)
-    zs = W.g2_grid(16, 4, c.beta)
+    zs = W.g2_grid(*grid, c.beta)
     order = np.random.default_rng(3).permutation(len(zs))
     grouped = e.g2(*ijkl, zs, 1e-8, 1e-16)["values"]
     general = np.empty_like(grouped)
