@@ -33,7 +33,11 @@ PEAK_FP64_TFLOPS = 78.6
 # HBM bytes per evaluation-kernel dispatch of the default step (--stride 8), measured with
 # rocprofv3 --pmc: FETCH_SIZE x 2 + WRITE_SIZE x 1 (the factors are what the same counters read
 # for a copy kernel of known size); the counters cannot be read from inside this process.
-EVAL_TRAFFIC_BYTES = 1.841e9
+EVAL_TRAFFIC_BYTES = 1.842e9
+# Hardware FP64 FLOP per term x frequency of the default step (FMA = 2), measured with
+# rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 over its evaluation dispatches: the
+# grouped kernels evaluate fewer operations than the 28/36-FLOP direct formula the metric counts.
+EVAL_HW_FLOP_PER_TERM_FREQ = 16.26
 DEFAULT_STRIDE = 8
 
 
@@ -309,6 +313,8 @@ def main():
     value = term_freqs / wall
     flops = (FLOP_NONRES * terms_nr + FLOP_RES * terms_r) * nfreq
     achieved = flops / (eval_ms * 1e-3) / 1e12 if eval_ms > 0 else 0.0
+    hw = (EVAL_HW_FLOP_PER_TERM_FREQ * term_freqs / (eval_ms * 1e-3) / 1e12
+          if eval_ms > 0 and args.stride == DEFAULT_STRIDE else None)
 
     cpu = None
     if not args.no_cpu:
@@ -345,7 +351,11 @@ def main():
                      "frac": achieved / PEAK_FP64_TFLOPS,
                      "traffic": EVAL_TRAFFIC_BYTES if args.stride == DEFAULT_STRIDE else None,
                      "traffic_unit": "bytes per launch (rocprofv3 PMC FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-                     "kernel": "g2_eval_kernel (FP64 Lehmann sum)", "eval_ms": eval_ms, "eval_launches": launches},
+                     "kernel": "g2_eval_s_kernel / g2_eval_kernel (FP64 Lehmann sum)", "eval_ms": eval_ms,
+                     "eval_launches": launches,
+                     "flop_model": "28 per non-resonant, 36 per resonant term x frequency (direct formula)",
+                     "hw_achieved": hw, "hw_frac": hw / PEAK_FP64_TFLOPS if hw else None,
+                     "hw_flop_model": f"{EVAL_HW_FLOP_PER_TERM_FREQ} FP64 FLOP per term x frequency (PMC)"},
         "cpu_baseline": cpu,
         "batch": batched,
         "dense": dense,
