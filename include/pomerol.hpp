@@ -9,5 +9,6 @@
 #include "pomerol/MonomialOperator.hpp"
 #include "pomerol/GreensFunction.hpp"
 #include "pomerol/TwoParticleGF.hpp"
+#include "pomerol/Vertex4.hpp"
 #include "pomerol/Batch.hpp"
 #include "pomerol/Tangent.hpp"

@@ -133,6 +133,9 @@ public:
     ParticleIndex getIndex(std::size_t Position) const;
     ComplexType operator()(ComplexType z1, ComplexType z2, ComplexType z3) const;
     ComplexType operator()(long MatsubaraNumber1, long MatsubaraNumber2, long MatsubaraNumber3) const;
+    /// Values at many triplets from the kept terms (compute() with clear == false), one device
+    /// evaluation per part.
+    std::vector<ComplexType> evaluate(FreqVec3 const& freqs) const;
     bool isVanishing() const { return Vanishing; }
     std::vector<TwoParticleGFPart> const& getParts() const { return parts; }
     std::size_t getNumParts() const { return parts.size(); }

@@ -682,6 +682,35 @@ ComplexType TwoParticleGF::operator()(ComplexType z1, ComplexType z2, ComplexTyp
     return s;
 }
 
+std::vector<ComplexType> TwoParticleGF::evaluate(FreqVec3 const& freqs) const {
+    std::vector<ComplexType> out(freqs.size(), 0.0);
+    if(Vanishing || freqs.empty())
+        return out;
+    int nf = int(freqs.size());
+    std::vector<double> h(std::size_t(6 * nf));
+    for(int w = 0; w < nf; ++w) {
+        ComplexType z[3] = {std::get<0>(freqs[w]), std::get<1>(freqs[w]), std::get<2>(freqs[w])};
+        for(int k = 0; k < 3; ++k) {
+            h[std::size_t(6 * w + 2 * k)] = z[k].real();
+            h[std::size_t(6 * w + 2 * k + 1)] = z[k].imag();
+        }
+    }
+    detail::DeviceArray<double> df(h.size()), acc(std::size_t(2 * nf));
+    df.upload(h);
+    acc.zero();
+    for(auto const& p : parts) {
+        if(p.getStatus() != Computed || !p.terms)
+            throw StatusMismatch("TwoParticleGF::evaluate: the terms of a part were not kept by compute()");
+        detail::check(pomx_g2_terms_eval(detail::gpu(), p.terms.get(), p.perm_index, p.PoleResolution, nf, df.data(),
+                                         acc.data()),
+                      "two-particle GF evaluation");
+    }
+    auto ha = acc.download();
+    for(int w = 0; w < nf; ++w)
+        out[std::size_t(w)] = ComplexType(ha[std::size_t(2 * w)], ha[std::size_t(2 * w + 1)]);
+    return out;
+}
+
 ComplexType TwoParticleGF::operator()(long n1, long n2, long n3) const {
     return (*this)(MatsubaraSpacing * RealType(2 * n1 + 1), MatsubaraSpacing * RealType(2 * n2 + 1),
                    MatsubaraSpacing * RealType(2 * n3 + 1));

@@ -460,6 +460,42 @@ PYBIND11_MODULE(_pomerol, m) {
             return d;
         }, py::arg("i"), py::arg("j"), py::arg("k"), py::arg("l"), py::arg("zs"), py::arg("direction"),
            py::arg("pole_res") = 1e-8, py::arg("coeff_tol") = 1e-16)
+        .def("vertex4", [](ED& e, int i, int j, int k, int l, long n_iw, double pr, double ct,
+                           std::vector<std::tuple<long, long, long>> query) {
+            // Gamma4 of <c_i c_j c+_k c+_l> with G13 = G_ik, G24 = G_jl, G14 = G_il, G23 = G_jk
+            auto& ops = e.field_ops();
+            TwoParticleGF Chi(*e.S, *e.H, ops.getAnnihilationOperator(ParticleIndex(i)), ops.getAnnihilationOperator(ParticleIndex(j)),
+                              ops.getCreationOperator(ParticleIndex(k)), ops.getCreationOperator(ParticleIndex(l)), *e.rho);
+            Chi.PoleResolution = pr;
+            Chi.CoefficientTolerance = ct;
+            Chi.prepare();
+            Chi.compute(false);
+            auto gf = [&](int a, int b) {
+                auto G = std::make_unique<GreensFunction>(*e.S, *e.H, ops.getAnnihilationOperator(ParticleIndex(a)),
+                                                          ops.getCreationOperator(ParticleIndex(b)), *e.rho);
+                G->prepare();
+                G->compute();
+                return G;
+            };
+            auto G13 = gf(i, k), G24 = gf(j, l), G14 = gf(i, l), G23 = gf(j, k);
+            Vertex4 V(Chi, *G13, *G24, *G14, *G23);
+            auto t0 = std::chrono::steady_clock::now();
+            V.compute(n_iw);
+            double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            std::vector<ComplexType> stored, direct, chi;
+            for(auto const& q : query) {
+                long n1 = std::get<0>(q), n2 = std::get<1>(q), n3 = std::get<2>(q);
+                stored.push_back(V(n1, n2, n3));
+                direct.push_back(V.value(n1, n2, n3));
+                chi.push_back(Chi(n1, n2, n3));
+            }
+            py::dict d;
+            d["gamma"] = to_np(stored);
+            d["gamma_value"] = to_np(direct);
+            d["chi"] = to_np(chi);
+            d["seconds"] = secs;
+            return d;
+        })
         .def("g2_parts", [](ED& e, int i, int j, int k, int l) {
             // part list of TwoParticleGF::prepare: (S1, S2, S3, S4, permutation, n1, n2, n3, n4)
             auto& ops = e.field_ops();

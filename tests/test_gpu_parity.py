@@ -238,3 +238,33 @@ def test_2pgf_grouped_grid_matches_general_path(pa, comp, grid):
     general = np.empty_like(grouped)
     general[order] = e.g2(*ijkl, zs[order], 1e-8, 1e-16)["values"]
     assert rel_err(grouped, general) < 1e-12
+
+
+def test_vertex4_hubbard_atom_closed_form(pa):
+    """Vertex4 (Gamma4 = chi - chi0, MatsubaraContainer4) of a Hubbard atom at half filling
+    against the closed form Gamma_uuuu G^4 of test/Vertex4Test.cpp:52-58 (absolute 1e-10, as
+    there), for every stored triplet with |n| <= 4, plus triplets outside the container."""
+    from pomerol_amd.presets import CoulombS
+
+    U, beta, n_iw = 1.0, 40.0, 4
+    H = CoulombS("A", U, -U / 2)
+    idx = H.indices()
+    up = idx.index(("A", 0, 1))
+    e = pa.ED(W.terms(H), beta)
+    q = [(n1, n2, n3) for n1 in range(-n_iw, n_iw) for n2 in range(-n_iw, n_iw) for n3 in range(-n_iw, n_iw)]
+    q_out = [(n_iw + 1, 0, n_iw + 1), (-n_iw - 2, 1, 0)]
+    r = e.vertex4(up, up, up, up, n_iw, 1e-4, 1e-12, q + q_out)
+    w = lambda n: np.pi * (2 * n + 1) / beta
+    G = e.gf(up, up, fermionic(np.arange(-3 * n_iw, 3 * n_iw), beta), 1e-8, 1e-12)["values"]
+    g = lambda n: G[n + 3 * n_iw]
+
+    def gamma_ref(n1, n2, n3):
+        d = float(n1 == n3) - float(n2 == n3)
+        return (-beta * d * (0.5 * U) ** 2 * (1 + (0.5 * U / w(n1)) ** 2) * (1 + (0.5 * U / w(n2)) ** 2)
+                * g(n1) * g(n2) * g(n3) * g(n1 + n2 - n3))
+
+    ref = np.array([gamma_ref(*t) for t in q])
+    got = r["gamma"][:len(q)]
+    print(f"[vertex4] max |Gamma - closed form| {np.abs(got - ref).max():.2e} over {len(q)} triplets", flush=True)
+    assert np.abs(got - ref).max() < 1e-10
+    assert np.abs(r["gamma"] - r["gamma_value"]).max() < 1e-12  # container == point-by-point value()
