@@ -13,6 +13,7 @@
 #include <pomerol/StatesClassification.hpp>
 #include <pomerol/Susceptibility.hpp>
 #include <pomerol/TwoParticleGFContainer.hpp>
+#include <pomerol/Vertex4.hpp>
 
 #include <complex>
 #include <iostream>
@@ -72,6 +73,14 @@ int main() {
     Chi4.prepareAll(quads);
     Chi4.computeAll(false, FreqVec3{}, MPI_COMM_WORLD);
     std::complex<double> x = Chi4(IndexCombination4(iu, id, iu, id))(0, 0, 0);
+    GreensFunction const& Gu = G(iu, iu);
+    GreensFunction const& Gd = G(id, id);
+    Vertex4 Gamma(Chi4(IndexCombination4(iu, id, iu, id)), Gu, Gd, G(iu, id), G(id, iu));
+    std::set<IndexCombination2> mixed = {IndexCombination2(iu, id), IndexCombination2(id, iu)};
+    G.prepareAll(mixed);
+    G.computeAll();
+    Gamma.compute(2);
+    x += Gamma(0, 0, 0) - Gamma.value(0, 0, 0);
 
     std::cout << g0 << gt << G(iu, iu)(3) << chi0 << chit << Ave() << x << std::endl;
     return 0;
