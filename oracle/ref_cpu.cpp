@@ -744,4 +744,218 @@ G2Result ED::g2(int i, int j, int k, int l, std::vector<cplx> const& zs, double 
     return out;
 }
 
+// ---- 3-point susceptibility ------------------------------------------------------------------
+// ThreePointSusceptibilityPart.hpp:66-168 and .cpp:34-43: fermion-fermion term
+// C / ((z1 - P0)(z2 - P1)), merged on (bin P0, bin P1), count-weighted mean poles.
+struct FFTerm {
+    cplx C;
+    std::array<double, 2> P;
+    long W;
+    struct Hash {
+        double res;
+        std::size_t operator()(FFTerm const& t) const {
+            return mix(std::hash<int64_t>()(bin_of(t.P[0], res)), std::hash<int64_t>()(bin_of(t.P[1], res)));
+        }
+    };
+    struct Eq {
+        double res;
+        bool operator()(FFTerm const& a, FFTerm const& b) const {
+            return std::fabs(a.P[0] - b.P[0]) <= res && std::fabs(a.P[1] - b.P[1]) <= res;
+        }
+    };
+    FFTerm& operator+=(FFTerm const& o) {
+        long cw = W + o.W;
+        for(int k = 0; k < 2; ++k)
+            P[k] = (double(W) * P[k] + double(o.W) * o.P[k]) / double(cw);
+        W = cw;
+        C += o.C;
+        return *this;
+    }
+    bool negligible(double tol, std::size_t div) const { return std::abs(C) <= tol / double(div); }
+    // ThreePointSusceptibilityPart.hpp:527-529
+    cplx operator()(cplx z1, cplx z2) const { return C / ((z1 - P[0]) * (z2 - P[1])); }
+};
+
+// ThreePointSusceptibilityPart.hpp:170-280 and .cpp:85-95: fermion-boson term
+// C / ((z1 - P1)(z1 - xi z2 - P12)), merged on (xi, bin P1, bin P12).
+struct FBTerm {
+    cplx C;
+    double P1, P12;
+    int xi;
+    long W;
+    struct Hash {
+        double res;
+        std::size_t operator()(FBTerm const& t) const {
+            std::size_t h = std::size_t(t.xi + 2);
+            h = mix(h, std::hash<int64_t>()(bin_of(t.P1, res)));
+            return mix(h, std::hash<int64_t>()(bin_of(t.P12, res)));
+        }
+    };
+    struct Eq {
+        double res;
+        bool operator()(FBTerm const& a, FBTerm const& b) const {
+            return a.xi == b.xi && std::fabs(a.P1 - b.P1) <= res && std::fabs(a.P12 - b.P12) <= res;
+        }
+    };
+    FBTerm& operator+=(FBTerm const& o) {
+        long cw = W + o.W;
+        P1 = (double(W) * P1 + double(o.W) * o.P1) / double(cw);
+        P12 = (double(W) * P12 + double(o.W) * o.P12) / double(cw);
+        W = cw;
+        C += o.C;
+        return *this;
+    }
+    bool negligible(double tol, std::size_t div) const { return std::abs(C) <= tol / double(div); }
+    // ThreePointSusceptibilityPart.hpp:531-533
+    cplx operator()(cplx z1, cplx z2) const { return C / ((z1 - P1) * (z1 - double(xi) * z2 - P12)); }
+};
+
+// ThreePointSusceptibilityPart.hpp:282-370 and .cpp:141-149: resonant term, C / (z1 - P) where
+// |z1 - xi z2| <= tolerance, merged on (xi, bin P).
+struct R3Term {
+    cplx C;
+    double P;
+    int xi;
+    long W;
+    struct Hash {
+        double res;
+        std::size_t operator()(R3Term const& t) const {
+            return mix(std::size_t(t.xi + 2), std::hash<int64_t>()(bin_of(t.P, res)));
+        }
+    };
+    struct Eq {
+        double res;
+        bool operator()(R3Term const& a, R3Term const& b) const { return a.xi == b.xi && std::fabs(a.P - b.P) <= res; }
+    };
+    R3Term& operator+=(R3Term const& o) {
+        long cw = W + o.W;
+        P = (double(W) * P + double(o.W) * o.P) / double(cw);
+        W = cw;
+        C += o.C;
+        return *this;
+    }
+    bool negligible(double tol, std::size_t div) const { return std::abs(C) <= tol / double(div); }
+    // ThreePointSusceptibilityPart.hpp:535-540
+    cplx operator()(cplx z1, cplx z2, double dtol) const {
+        return (std::abs(z1 - double(xi) * z2) <= dtol) ? (C / (z1 - P)) : cplx(0.0);
+    }
+};
+
+// ThreePointSusceptibility.cpp:37-66 (operator roles per channel), :68-142 (parts),
+// ThreePointSusceptibilityPart.cpp:243-348 (terms, addMultiterm), .cpp:350-359 and
+// ThreePointSusceptibility.hpp:162-177 (evaluation).
+Chi3Result ED::chi3(int channel, int i, int j, int k, int l, std::vector<cplx> const& zs, double pole_res,
+                    double coeff_tol) const {
+    RotOp const& CX1 = cdag_op(i);
+    RotOp const& C2 = c_op(j);
+    RotOp const& CX3 = cdag_op(k);
+    RotOp const& C4 = c_op(l);
+    bool const pp = channel == 0;
+    RotOp const& F1 = CX1;
+    RotOp const& F2 = channel == 0 ? CX3 : (channel == 1 ? C2 : C4);
+    RotOp const& B1 = channel == 0 ? C2 : CX3;
+    RotOp const& B2 = channel == 2 ? C2 : C4;
+    int const xi = pp ? -1 : 1;
+    std::size_t nw = zs.size() / 2;
+    Chi3Result out;
+    out.values.assign(nw, 0.0);
+
+    // one part: <1|Fa|2> <2|Fb|3> <3|B1 B2|1>, blocks (S1, S2, S3)
+    auto run_part = [&](OpPart const* fa, OpPart const* fb, OpPart const* b1, OpPart const* b2, int S1, int S2, int S3,
+                        bool sw) {
+        int n1 = int(S.blocks[S1].size()), n2 = int(S.blocks[S2].size()), n3 = int(S.blocks[S3].size());
+        int nm = b1->n_from;
+        std::vector<double> B(std::size_t(n3) * n1, 0.0);
+        for(int r = 0; r < n3; ++r)
+            for(int m = 0; m < nm; ++m) {
+                double a = b1->m[std::size_t(r) * nm + m];
+                if(a == 0.0)
+                    continue;
+                for(int c = 0; c < n1; ++c)
+                    B[std::size_t(r) * n1 + c] += a * b2->m[std::size_t(m) * n1 + c];
+            }
+        auto const &E1 = evals[S1], &E2 = evals[S2], &E3 = evals[S3];
+        auto const &W1 = weights[S1], &W2 = weights[S2], &W3 = weights[S3];
+        double const prefactor = ((channel == 1) == sw) ? 1.0 : -1.0;
+        TermSet<FFTerm> ff(pole_res, coeff_tol);
+        TermSet<FBTerm> fbs(pole_res, coeff_tol);
+        TermSet<R3Term> rs(pole_res, coeff_tol);
+        for(int i1 = 0; i1 < n1; ++i1)
+            for(int i2 = 0; i2 < n2; ++i2) {
+                double x = fa->m[std::size_t(i1) * n2 + i2];
+                if(x == 0.0)
+                    continue;
+                for(int i3 = 0; i3 < n3; ++i3) {
+                    double y = fb->m[std::size_t(i2) * n3 + i3], b = B[std::size_t(i3) * n1 + i1];
+                    if(y == 0.0 || b == 0.0)
+                        continue;
+                    ++out.n_triples;
+                    cplx C = cplx(x * y * b);
+                    C *= prefactor;
+                    // addMultiterm (ThreePointSusceptibilityPart.cpp:298-348)
+                    double Ei = E1[i1], Ej = E2[i2], Ek = E3[i3], Wi = W1[i1], Wj = W2[i2], Wk = W3[i3];
+                    double Eij = Ei - Ej, Ejk = Ej - Ek, Eik = Ei - Ek;
+                    cplx cff = C * (Wi + Wj);
+                    if(std::abs(cff) > coeff_tol)
+                        ff.add(FFTerm{pp ? -cff : cff, {sw ? Ejk : Eij, (sw ? Eij : Ejk) * (pp ? 1.0 : -1.0)}, 1});
+                    if(std::fabs(Eik) <= pole_res) {
+                        cplx cr = -C * beta * Wi;
+                        if(std::abs(cr) > coeff_tol)
+                            rs.add(R3Term{sw ? -cr : cr, sw ? Ejk : -Ejk, xi, 1});
+                    } else {
+                        cplx cfb = C * (Wk - Wi);
+                        if(std::abs(cfb) > coeff_tol) {
+                            fbs.add(FBTerm{sw ? -cfb : cfb, sw ? Ejk : Eij, Eik, xi, 1});
+                            if(!sw)
+                                ff.add(FFTerm{pp ? -cfb : cfb, {Eij, pp ? Ejk : -Ejk}, 1});
+                        }
+                    }
+                }
+            }
+        out.n_parts++;
+        out.n_ff += long(ff.size());
+        out.n_fb += long(fbs.size());
+        out.n_res += long(rs.size());
+        out.part_ff.push_back(long(ff.size()));
+        out.part_fb.push_back(long(fbs.size()));
+        out.part_res.push_back(long(rs.size()));
+        for(int v : {S1, S2, S3, int(sw)})
+            out.part_blocks.push_back(v);
+        std::vector<FFTerm> vff(ff.begin(), ff.end());
+        std::vector<FBTerm> vfb(fbs.begin(), fbs.end());
+        std::vector<R3Term> vr(rs.begin(), rs.end());
+#pragma omp parallel for schedule(static)
+        for(long w = 0; w < long(nw); ++w) {
+            cplx z1 = zs[2 * w], z2 = zs[2 * w + 1];
+            cplx s = 0;
+            for(auto const& t : vff)
+                s += t(z1, z2);
+            for(auto const& t : vfb)
+                s += t(z1, z2);
+            for(auto const& t : vr)
+                s += t(z1, z2, pole_res);
+            out.values[w] += s;
+        }
+    };
+
+    for(auto const& b2 : B2.parts) { // ascending right block of B2
+        int B2right = b2.from, B2left = b2.to;
+        int B1left = B1.left_of_right[B2left];
+        if(B1left < 0)
+            continue;
+        OpPart const* b1 = part_from_left(B1, B1left);
+        // F1, F2, B term: <B2right|F1|F1right><F2left|F2|B1left><B1left|B1|B2left><B2left|B2|B2right>
+        int F1right = F1.right_of_left[B2right];
+        int F2left = F2.left_of_right[B1left];
+        if(F1right == F2left && F1right >= 0 && retained[B2right] && retained[F1right] && retained[B1left])
+            run_part(part_from_left(F1, B2right), part_from_left(F2, F2left), b1, &b2, B2right, F1right, B1left, false);
+        // F2, F1, B term
+        int F2right = F2.right_of_left[B2right];
+        int F1left = F1.left_of_right[B1left];
+        if(F2right == F1left && F2right >= 0 && retained[B2right] && retained[F2right] && retained[B1left])
+            run_part(part_from_left(F2, B2right), part_from_left(F1, F1left), b1, &b2, B2right, F2right, B1left, true);
+    }
+    return out;
+}
+
 } // namespace oracle

@@ -81,6 +81,13 @@ struct G2Result {
     std::vector<long> part_nonres, part_res;
     std::vector<int> part_blocks; // 5 ints per part: S1 S2 S3 S4 perm
 };
+struct Chi3Result {
+    std::vector<cplx> values;
+    long n_parts = 0, n_ff = 0, n_fb = 0, n_res = 0;
+    long n_triples = 0; // index triples with a non-zero matrix element
+    std::vector<long> part_ff, part_fb, part_res;
+    std::vector<int> part_blocks; // 4 ints per part: S1 S2 S3 swapped
+};
 
 class ED {
 public:
@@ -113,6 +120,10 @@ public:
     // 2PGF <c_i c_j c^dag_k c^dag_l> at the frequency triplets (z1, z2, z3) of zs (3 per point).
     G2Result g2(int i, int j, int k, int l, std::vector<cplx> const& zs, double pole_res,
                 double coeff_tol, int max_parts = -1, std::vector<int> const* part_subset = nullptr) const;
+    // 3-point susceptibility of (c^dag_i, c_j, c^dag_k, c_l) in channel 0 PP, 1 PH, 2 xPH at the
+    // frequency pairs (z1, z2) of zs (2 per point).
+    Chi3Result chi3(int channel, int i, int j, int k, int l, std::vector<cplx> const& zs, double pole_res,
+                    double coeff_tol) const;
 
 private:
     mutable std::vector<RotOp> cdag_cache_, c_cache_;

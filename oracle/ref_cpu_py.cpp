@@ -111,6 +111,30 @@ PYBIND11_MODULE(pomerol_oracle, m) {
              py::arg("A"), py::arg("B"), py::arg("z"), py::arg("pole_res") = 1e-8, py::arg("coeff_tol") = 1e-8,
              py::arg("subtract_disconnected") = false, py::arg("tau") = std::vector<double>{})
         .def("ensemble_average", [](ED const& e, py::list A) { return e.ensemble_average(to_mono(A)); })
+        .def("chi3",
+             [](ED const& e, int channel, int i, int j, int k, int l, py::array_t<std::complex<double>> zs, double pr,
+                double ct) {
+                 Chi3Result r;
+                 std::vector<cplx> zv = zvec(zs);
+                 {
+                     py::gil_scoped_release rel;
+                     r = e.chi3(channel, i, j, k, l, zv, pr, ct);
+                 }
+                 py::dict d;
+                 d["values"] = arr(r.values);
+                 d["n_parts"] = r.n_parts;
+                 d["n_ff"] = r.n_ff;
+                 d["n_fb"] = r.n_fb;
+                 d["n_res"] = r.n_res;
+                 d["n_triples"] = r.n_triples;
+                 d["part_ff"] = arr(r.part_ff);
+                 d["part_fb"] = arr(r.part_fb);
+                 d["part_res"] = arr(r.part_res);
+                 d["part_blocks"] = arr(r.part_blocks);
+                 return d;
+             },
+             py::arg("channel"), py::arg("i"), py::arg("j"), py::arg("k"), py::arg("l"), py::arg("zs"),
+             py::arg("pole_res") = 1e-8, py::arg("coeff_tol") = 1e-16)
         .def("g2",
              [](ED const& e, int i, int j, int k, int l, py::array_t<std::complex<double>> zs, double pr, double ct,
                 int max_parts, py::object subset) {

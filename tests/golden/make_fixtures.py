@@ -3,7 +3,8 @@
 * ``configs.json``: the seeded parameters of the benchmark workloads (pomerol_amd.workloads).
 * ``reference_goldens.json``: golden values held by the reference's own tests, transcribed as
   data with their file:line (test/AndersonTest.cpp:40-64, GF2siteTest.cpp:40-51,
-  GF4siteTest.cpp:36-46, Anderson2PGFTest.cpp:37-57, HamiltonianTest.cpp:70-75).
+  GF4siteTest.cpp:36-46, Anderson2PGFTest.cpp:37-57, HamiltonianTest.cpp:70-75,
+  3PSusc3siteTest.cpp:72-156).
 
 Run: ``python tests/golden/make_fixtures.py`` (no reference code is imported or run).
 """
@@ -62,6 +63,36 @@ GOLDENS = {
         "source": "test/HamiltonianTest.cpp:40-42,70-75",
         "E0": -2.8860009,
         "tolerance": 1e-7,
+    },
+    # chi3(n1, n2) for n1, n2 in -1..1, row n1 + 1, column n2 + 1, as [re, im]
+    "3PSusc3siteTest": {
+        "source": "test/3PSusc3siteTest.cpp:30-158",
+        "params": {"U": 4.0, "mu_over_U": 0.6, "t": 1.0, "beta": 5.0},
+        "tolerance": 1e-10,
+        "PP": {"indices": [["A", "up"], ["C", "up"], ["A", "dn"], ["C", "dn"]],
+               "chi3": [[[0.056123393380680404, -0.027886788783494966], [0.03544036019820439, 0.0],
+                         [0.008008339386258077, -0.0022610905291281]],
+                        [[0.03544036019820439, 0.0], [0.056123393380680404, 0.027886788783494966],
+                         [0.026205160724877802, 0.020851328654297216]],
+                        [[0.008008339386258187, -0.002261090529127845],
+                         [0.026205160724877885, 0.020851328654297483],
+                         [0.013637220898198006, 0.012985397056265282]]]},
+        "PH": {"indices": [["A", "up"], ["A", "up"], ["C", "dn"], ["C", "dn"]],
+               "chi3": [[[-0.03377687240880357, 0.6799848693219751], [-0.01589165211289571, 0.0],
+                         [-0.010498942766103334, -0.0019558762969712905]],
+                        [[-0.01589165211289571, 0.0], [-0.03377687240880357, -0.6799848693219751],
+                         [0.028277168735633868, 0.006544223759963227]],
+                        [[-0.010498942766103213, -0.001955876296971393],
+                         [0.02827716873563382, 0.006544223759962816],
+                         [0.03857021454863498, -0.6687150248684948]]]},
+        "xPH": {"indices": [["A", "up"], ["C", "up"], ["C", "dn"], ["A", "dn"]],
+                "chi3": [[[0.27954539611247325, -0.16431057067114357], [0.08787949653955825, 0.0],
+                          [0.025342850143558054, -0.003942925235138866]],
+                         [[0.08787949653955825, 0.0], [0.27954539611247325, 0.16431057067114357],
+                          [0.05067298632153447, 0.02536375725665837]],
+                         [[0.02534285014355848, -0.00394292523513924],
+                          [0.05067298632153436, 0.0253637572566579],
+                          [0.11815163820082586, 0.06747758349542371]]]},
     },
 }
 

@@ -127,3 +127,59 @@ def sector_hamiltonian(terms, indices, n_up, n_dn):
         vals.append(coeff * sign[ok])
     H = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(len(states),) * 2)
     return H, states
+
+
+def chi3_atom_closed_form(channel, i1, i2, U, mu, h, beta, ns):
+    """3-point susceptibility of the Hubbard atom (c^+_i1 c_i1 c^+_i2 c_i2, modes up = 1, dn = 0) on
+    the grid n1, n2 in ns, from the auxiliary function g of 3PSusc1siteTest.cpp:35-59 and the
+    channel expressions of 3PSusc1siteTest.cpp:103-212; channel 0 PP, 1 PH, 2 xPH."""
+    E = np.array([0.0, -mu - h, -mu + h, U - 2 * mu])  # |0>, |up>, |dn>, |2>
+    rho = np.exp(-beta * E)
+    rho /= rho.sum()
+
+    def g(i, j, k, w1, w2):
+        if abs(rho[k] - rho[i]) < 1e-14:
+            d = 1.0 if abs(w1 + w2) < 1e-14 else 0.0
+            return (1 / (1j * w2 + E[j] - E[k]) * (rho[i] + rho[j]) / (1j * w1 + E[i] - E[j])
+                    + beta * d * rho[i] / (1j * w2 + E[j] - E[i]))
+        return 1 / (1j * w2 + E[j] - E[k]) * ((rho[k] - rho[i]) / (1j * w1 + 1j * w2 + E[i] - E[k])
+                                               + (rho[i] + rho[j]) / (1j * w1 + E[i] - E[j]))
+
+    up = 1
+    st = lambda i: 1 if i == up else 2          # the singly occupied state of spin i
+    other = lambda i: 2 if i == up else 1
+    om = lambda n: np.pi * (2 * n + 1) / beta
+    out = []
+    for n1 in ns:
+        for n2 in ns:
+            a, b = om(n1), om(n2)
+            if channel == 0:
+                v = 0.0 if i1 == i2 else g(3, st(i2), 0, -a, -b) + g(3, st(i1), 0, -b, -a)
+            elif channel == 1:
+                if i1 == i2:
+                    v = g(st(i1), 0, st(i1), -a, b) + g(3, other(i1), 3, -a, b)
+                else:
+                    v = g(3, st(i2), 3, -a, b) - g(st(i2), 3, st(i2), a, -b)
+            else:
+                if i1 == i2:
+                    v = -g(st(i1), 0, st(i1), -a, b) - g(3, other(i1), 3, -a, b)
+                else:
+                    v = -g(st(i1), 0, other(i1), -a, b) - g(st(i1), 3, other(i1), b, -a)
+            out.append(v)
+    return np.array(out)
+
+
+def chi3_grid(ns, beta):
+    """(i w_n1, i w_n2) pairs, n1 outer, n2 inner."""
+    w = lambda n: 1j * np.pi * (2 * n + 1) / beta
+    return np.array([[w(a), w(b)] for a in ns for b in ns])
+
+
+def cluster3_model():
+    """test/3PSusc3siteTest.cpp:30-41"""
+    g = goldens()["3PSusc3siteTest"]["params"]
+    U, t = g["U"], g["t"]
+    mu = g["mu_over_U"] * U
+    H = (CoulombS("A", U, -mu) + CoulombS("B", U, -mu) + CoulombS("C", U, -mu)
+         + Hopping("A", "B", -t) + Hopping("B", "C", -t) + Hopping("C", "A", -t))
+    return H, g["beta"]

@@ -131,3 +131,49 @@ def test_susceptibility_closed_forms(oracle):
     v = ed.chi(n_up, n_dn, z, 1e-8, 1e-8, True)["values"]
     ref = np.where(np.arange(20) == 0, (w2 - (wu + w2) * (wd + w2)) * beta, 0)
     assert np.abs(v - ref).max() < 1e-14
+
+
+@pytest.mark.parametrize("h", [0.0, 0.01])
+def test_chi3_hubbard_atom_closed_form(oracle, h):
+    """3PSusc1siteTest.cpp:62-212: every channel and spin pair on n1, n2 in [-20, 20) at 1e-14."""
+    from helpers import chi3_atom_closed_form, chi3_grid
+    U, mu, beta = 1.0, 0.4, 10.0
+    ed = oracle.ED(2, hubbard_atom(U, mu, -h).mode_terms(), beta)
+    ns = np.arange(-20, 20)
+    zs = chi3_grid(ns, beta).ravel()
+    for channel in (0, 1, 2):
+        for i1 in (1, 0):
+            for i2 in (1, 0):
+                r = ed.chi3(channel, i1, i1, i2, i2, zs)
+                ref = chi3_atom_closed_form(channel, i1, i2, U, mu, h, beta, ns)
+                assert np.abs(r["values"] - ref).max() < 1e-14, (channel, i1, i2)
+                if channel == 0 and i1 == i2:
+                    assert r["n_parts"] == 0  # isVanishing()
+
+
+def test_chi3_hubbard_atom_crossing_symmetry(oracle):
+    """3PSusc1siteTest.cpp:215-250: chi3_xPH(i1, i4, i3, i2) = -chi3_PH(i1, i2, i3, i4)."""
+    from helpers import chi3_grid
+    beta = 10.0
+    ed = oracle.ED(2, hubbard_atom(1.0, 0.4, -0.01).mode_terms(), beta)
+    zs = chi3_grid(np.arange(-20, 20), beta).ravel()
+    for i1, i2, i3, i4 in np.ndindex(2, 2, 2, 2):
+        ph = ed.chi3(1, i1, i2, i3, i4, zs)["values"]
+        xph = ed.chi3(2, i1, i4, i3, i2, zs)["values"]
+        assert np.abs(xph + ph).max() < 1e-14
+
+
+def test_chi3_3site_golden(oracle):
+    """3PSusc3siteTest.cpp:72-156: PP, PH and xPH on n1, n2 in -1..1 at 1e-10."""
+    from helpers import chi3_grid, cluster3_model
+    g = goldens()["3PSusc3siteTest"]
+    H, beta = cluster3_model()
+    idx = H.indices()
+    ed = oracle.ED(len(idx), H.mode_terms(), beta)
+    zs = chi3_grid([-1, 0, 1], beta).ravel()
+    spin = {"up": 1, "dn": 0}
+    for channel, name in ((0, "PP"), (1, "PH"), (2, "xPH")):
+        ii = [idx.index((site, 0, spin[s])) for site, s in g[name]["indices"]]
+        ref = np.array([complex(a, b) for row in g[name]["chi3"] for a, b in row])
+        v = ed.chi3(channel, *ii, zs)["values"]
+        assert np.abs(v - ref).max() < g["tolerance"], name
