@@ -9,6 +9,7 @@
 #include "pomerol/MonomialOperator.hpp"
 #include "pomerol/GreensFunction.hpp"
 #include "pomerol/TwoParticleGF.hpp"
+#include "pomerol/ThreePointSusceptibility.hpp"
 #include "pomerol/Vertex4.hpp"
 #include "pomerol/Batch.hpp"
 #include "pomerol/Tangent.hpp"

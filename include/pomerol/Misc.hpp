@@ -111,6 +111,14 @@ extern std::array<Permutation4, 24> const permutations4;
 std::ostream& operator<<(std::ostream& os, Permutation3 const& p);
 std::ostream& operator<<(std::ostream& os, Permutation4 const& p);
 
+/// Channel in which a susceptibility function is defined (Misc.hpp:159-165, Misc.cpp:69-79).
+enum Channel : int {
+    PP,  ///< particle-particle
+    PH,  ///< particle-hole
+    xPH  ///< crossed particle-hole
+};
+std::ostream& operator<<(std::ostream& os, Channel channel);
+
 /// Bin index of a real number (Misc.cpp:83-86).
 std::size_t hash_binned_real(double x, double bin_size);
 

@@ -157,6 +157,41 @@ int pomx_g2_part_compute(pomx_ctx* ctx, const pomx_g2_part* part, double beta, d
 int pomx_g2_terms_eval(pomx_ctx* ctx, const pomx_g2_terms* terms, int perm, double pole_res, int nfreq,
                        const double* d_freq, double* d_acc);
 
+/* ---- K12 + K13 three-point susceptibility part -----------------------------------------
+ * Replaces ThreePointSusceptibilityPart::computeImpl/addMultiterm
+ * (ThreePointSusceptibilityPart.cpp:243-348), its NonResonantFF/NonResonantFB/Resonant
+ * TermLists (ThreePointSusceptibilityPart.hpp:66-370, .cpp:34-149) and the frequency fill
+ * (ThreePointSusceptibility.cpp:148-184, ThreePointSusceptibilityPart.cpp:350-359).
+ * The part is <S1|Fa|S2><S2|Fb|S3><S3|B1 B2|S1> (Fa = F1, Fb = F2, or swapped Fa = F2, Fb = F1).
+ * Terms are generated for every (i1,i2,i3) whose coefficient exceeds coeff_tol, merged on
+ * (kind, round(P/res) of each pole) with count-weighted pole means, and (if nfreq > 0)
+ * evaluated at the pairs d_freq (nfreq x 2 complex, (z1, z2)) into d_acc (nfreq complex,
+ * accumulated).  h_counts receives the merged term counts (see pomx_chi3_terms), h_raw_terms
+ * (may be NULL) the number of terms emitted before merging.  keep == NULL discards the terms. */
+typedef struct {
+    int n1, n2, n3, n4;       /* sizes of S1, S2, S3 and of the block S4 between B1 and B2 */
+    pomx_view Fa, Fb, B1, B2; /* <S1|Fa|S2> <S2|Fb|S3> <S3|B1|S4> <S4|B2|S1> */
+    const double *E1, *E2, *E3;
+    const double *w1, *w2, *w3;
+    int channel; /* 0 PP, 1 PH, 2 xPH (Misc.hpp Channel order) */
+    int swapped; /* SwappedFermionOps */
+} pomx_chi3_part;
+
+/* Merged terms of one part (freed with pomx_free(ctx, data)): count[0] fermion-fermion terms as
+ * SoA (coeff, P0, P1), then count[1] fermion-boson terms (coeff, P1, P12), then count[2]
+ * resonant terms (coeff, P); every field is count[k] doubles long.  xi = -1 (PP) or 1. */
+typedef struct {
+    int64_t count[3];
+    int xi;
+    double* data;
+} pomx_chi3_terms;
+
+int pomx_chi3_part_compute(pomx_ctx* ctx, const pomx_chi3_part* part, double beta, double pole_res,
+                           double coeff_tol, int nfreq, const double* d_freq, double* d_acc, pomx_chi3_terms* keep,
+                           int64_t* h_counts, int64_t* h_raw_terms);
+int pomx_chi3_terms_eval(pomx_ctx* ctx, const pomx_chi3_terms* terms, double pole_res, int nfreq,
+                         const double* d_freq, double* d_acc);
+
 /* ---- batched FP64 GEMM (matrix cores) --------------------------------------------------
  * C = alpha * op(A) op(B) + beta * C for independent jobs; op(X) = X or X^T (trans_* != 0).
  * Operands are row-major at the given offsets (doubles) into d_A, d_B, d_C, with row strides

@@ -1176,6 +1176,9 @@ int launch_g2_tangent_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, dou
     return POMX_OK;
 }
 
+} // namespace
+
+// Shared with the 3-point susceptibility pipeline (declared in pomx_internal.hpp).
 int read_i64(pomx_ctx* ctx, const int64_t* d, int64_t* h) {
     POMX_HIP(ctx, hipMemcpyAsync(h, d, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
     POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1209,6 +1212,8 @@ int rank_table(pomx_ctx* ctx, int nx, int ny, const double* Ex, const double* Ey
     }
     return POMX_OK;
 }
+
+namespace {
 
 // Tangent reduction, compaction and evaluation of one chunk's sorted groups.
 int g2_chunk_tangent(pomx_ctx* ctx, G2Dev const& D, pomx_g2_tangent const& t, int64_t n, int64_t ng,

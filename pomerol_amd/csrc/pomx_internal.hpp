@@ -83,6 +83,15 @@ int pomx_sort_pairs(pomx_ctx* ctx, int64_t n, uint64_t* keys_in, uint64_t* keys_
 // Exclusive prefix sum of int64 (out may alias in).
 int pomx_exclusive_scan(pomx_ctx* ctx, int64_t n, const int64_t* in, int64_t* out);
 
+// Copy one int64 from the device (synchronises the context stream).
+int read_i64(pomx_ctx* ctx, const int64_t* d, int64_t* h);
+// Dense ranks of round((Ey[y] - Ex[x]) / res) over all (x, y), stored at rank[x * ny + y]; equal
+// bins get equal ranks, ascending with the bin.  'sorted' (optional) receives the x * ny + y
+// indices in rank order; *nranks the number of distinct bins; d_err[0] is set to 1 where
+// |bin| >= 1e18.
+int rank_table(pomx_ctx* ctx, int nx, int ny, const double* Ex, const double* Ey, double res, WsBuf<uint32_t>& rank,
+               WsBuf<uint64_t>* sorted, int64_t* nranks, int* d_err);
+
 // Evaluation-kernel timing hooks (HIP events on the context stream).
 void eval_timer_begin(pomx_ctx* ctx);
 void eval_timer_end(pomx_ctx* ctx);

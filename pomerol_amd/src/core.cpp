@@ -31,6 +31,15 @@ std::ostream& operator<<(std::ostream& os, Permutation4 const& p) {
     return os << p.perm[0] + 1 << p.perm[1] + 1 << p.perm[2] + 1 << p.perm[3] + 1;
 }
 
+std::ostream& operator<<(std::ostream& os, Channel channel) {
+    switch(channel) {
+    case Channel::PP: return os << "PP";
+    case Channel::PH: return os << "PH";
+    case Channel::xPH: return os << "xPH";
+    }
+    return os;
+}
+
 std::size_t hash_binned_real(double x, double bin_size) {
     return std::hash<std::int64_t>()(static_cast<std::int64_t>(std::round(x / bin_size)));
 }

@@ -132,6 +132,16 @@ FreqVec3 to_freqs(py::array_t<std::complex<double>, py::array::c_style | py::arr
     return f;
 }
 
+FreqVec2 to_pairs(py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> zs) {
+    if(zs.ndim() != 2 || zs.shape(1) != 2)
+        throw std::runtime_error("frequency pairs must have shape (N, 2)");
+    FreqVec2 f;
+    auto r = zs.unchecked<2>();
+    for(py::ssize_t w = 0; w < zs.shape(0); ++w)
+        f.emplace_back(r(w, 0), r(w, 1));
+    return f;
+}
+
 std::vector<ComplexType> to_points(py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> z) {
     return std::vector<ComplexType>(z.data(), z.data() + z.size());
 }
@@ -494,6 +504,90 @@ PYBIND11_MODULE(_pomerol, m) {
             d["gamma_value"] = to_np(direct);
             d["chi"] = to_np(chi);
             d["seconds"] = secs;
+            return d;
+        })
+        .def("chi3", [](ED& e, int channel, int i, int j, int k, int l, py::array_t<std::complex<double>> zs, double pr,
+                        double ct, bool keep) {
+            if(channel < 0 || channel > 2)
+                throw std::runtime_error("channel must be 0 (PP), 1 (PH) or 2 (xPH)");
+            auto& ops = e.field_ops();
+            ThreePointSusceptibility X(Channel(channel), *e.S, *e.H, ops.getCreationOperator(ParticleIndex(i)),
+                                       ops.getAnnihilationOperator(ParticleIndex(j)),
+                                       ops.getCreationOperator(ParticleIndex(k)),
+                                       ops.getAnnihilationOperator(ParticleIndex(l)), *e.rho);
+            X.PoleResolution = pr;
+            X.CoefficientTolerance = ct;
+            X.prepare();
+            FreqVec2 f = to_pairs(zs);
+            auto t0 = std::chrono::steady_clock::now();
+            std::vector<ComplexType> vals = X.compute(!keep, f);
+            double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            py::dict d;
+            d["values"] = to_np(vals);
+            d["seconds"] = secs;
+            d["vanishing"] = X.isVanishing();
+            d["n_parts"] = X.getNumParts();
+            int64_t nff = 0, nfb = 0, nr = 0, raw = 0;
+            std::vector<int64_t> pff, pfb, pr_, pb;
+            for(auto const& p : X.getParts()) {
+                nff += int64_t(p.getNumNonResonantFFTerms());
+                nfb += int64_t(p.getNumNonResonantFBTerms());
+                nr += int64_t(p.getNumResonantTerms());
+                raw += p.getNumRawTerms();
+                pff.push_back(int64_t(p.getNumNonResonantFFTerms()));
+                pfb.push_back(int64_t(p.getNumNonResonantFBTerms()));
+                pr_.push_back(int64_t(p.getNumResonantTerms()));
+                auto b = p.getBlocks();
+                for(int x : {int(b[0]), int(b[1]), int(b[2]), int(p.getSwappedFermionOps())})
+                    pb.push_back(x);
+            }
+            d["n_ff"] = nff;
+            d["n_fb"] = nfb;
+            d["n_res"] = nr;
+            d["n_raw"] = raw;
+            d["part_ff"] = pff;
+            d["part_fb"] = pfb;
+            d["part_res"] = pr_;
+            d["part_blocks"] = pb;
+            if(keep) {
+                std::vector<ComplexType> lazy;
+                for(auto const& t : f)
+                    lazy.push_back(X(std::get<0>(t), std::get<1>(t)));
+                d["values_lazy"] = to_np(lazy);
+                d["values_bulk"] = to_np(X.evaluate(f));
+                // merged terms of the first part, through the term getters
+                if(X.getNumParts() > 0) {
+                    auto const& p = X.getParts()[0];
+                    ComplexType s = 0;
+                    ComplexType z1 = std::get<0>(f.at(0)), z2 = std::get<1>(f.at(0));
+                    for(auto const& t : p.getNonResonantFFTerms())
+                        s += t(z1, z2);
+                    for(auto const& t : p.getNonResonantFBTerms())
+                        s += t(z1, z2);
+                    for(auto const& t : p.getResonantTerms())
+                        s += t(z1, z2, pr);
+                    d["part0_from_terms"] = s;
+                    d["part0"] = p(z1, z2);
+                }
+            }
+            return d;
+        }, py::arg("channel"), py::arg("i"), py::arg("j"), py::arg("k"), py::arg("l"), py::arg("zs"),
+           py::arg("pole_res") = 1e-8, py::arg("coeff_tol") = 1e-16, py::arg("keep") = false)
+        .def("chi3_container", [](ED& e, int channel, std::vector<std::tuple<int, int, int, int>> idx,
+                                  py::array_t<std::complex<double>> zs) {
+            auto& ops = e.field_ops();
+            ThreePointSusceptibilityContainer C(Channel(channel), *e.info, *e.S, *e.H, *e.rho, ops);
+            std::set<IndexCombination4> want;
+            for(auto const& t : idx)
+                want.emplace(ParticleIndex(std::get<0>(t)), ParticleIndex(std::get<1>(t)), ParticleIndex(std::get<2>(t)),
+                             ParticleIndex(std::get<3>(t)));
+            C.prepareAll(want);
+            auto all = C.computeAll(false, to_pairs(zs));
+            py::dict d;
+            for(auto const& kv : all) {
+                auto const& I = kv.first;
+                d[py::make_tuple(int(I.Index1), int(I.Index2), int(I.Index3), int(I.Index4))] = to_np(kv.second);
+            }
             return d;
         })
         .def("g2_parts", [](ED& e, int i, int j, int k, int l) {

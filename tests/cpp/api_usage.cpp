@@ -12,6 +12,8 @@
 #include <pomerol/Misc.hpp>
 #include <pomerol/StatesClassification.hpp>
 #include <pomerol/Susceptibility.hpp>
+#include <pomerol/ThreePointSusceptibility.hpp>
+#include <pomerol/ThreePointSusceptibilityContainer.hpp>
 #include <pomerol/TwoParticleGFContainer.hpp>
 #include <pomerol/Vertex4.hpp>
 
@@ -81,6 +83,18 @@ int main() {
     G.computeAll();
     Gamma.compute(2);
     x += Gamma(0, 0, 0) - Gamma.value(0, 0, 0);
+
+    ThreePointSusceptibility Chi3(Channel::PH, S, H, Operators.getCreationOperator(iu),
+                                  Operators.getAnnihilationOperator(iu), Operators.getCreationOperator(id),
+                                  Operators.getAnnihilationOperator(id), rho);
+    Chi3.prepare();
+    Chi3.compute();
+    x += Chi3(0, 0);
+    ThreePointSusceptibilityContainer Chi3s(Channel::xPH, IndexInfo, S, H, rho, Operators);
+    Chi3s.prepareAll({IndexCombination4(iu, id, id, iu)});
+    RealType const w0 = M_PI / beta;
+    auto chi3vals = Chi3s.computeAll(false, FreqVec2{FreqTuple2(ComplexType(0, w0), ComplexType(0, -w0))});
+    x += chi3vals.begin()->second.at(0) - Chi3s(IndexCombination4(iu, id, id, iu)).element()->operator()(0, -1);
 
     std::cout << g0 << gt << G(iu, iu)(3) << chi0 << chit << Ave() << x << std::endl;
     return 0;
