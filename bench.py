@@ -234,6 +234,53 @@ def run_dense(args):
     return out
 
 
+def run_chi3(args):
+    """Three-point susceptibility (PH channel, c+_up c_up c+_dn c_dn) of config 2 on an n x n grid of
+    fermionic Matsubara pairs: merged terms x pairs per second of the whole device pipeline
+    (generation, merge, evaluation), one warm-up; and the same quantity on config 1 for the device
+    and the oracle (the CPU port, all host threads), on a 16 x 16 grid."""
+    import numpy as np
+
+    import pomerol_amd
+    from pomerol_amd import workloads as W
+
+    def grid(n, beta):
+        w = 1j * np.pi * (2 * np.arange(-n // 2, n // 2) + 1) / beta
+        return np.array([[a, b] for a in w for b in w])
+
+    out = {}
+    for name, c, n in (("config2", W.cfg2(), args.chi3_n), ("config1", W.cfg1(), 16)):
+        H = c.expr()
+        idx = H.indices()
+        u, d = idx.index(("C", 0, 1)), idx.index(("C", 0, 0))
+        e = pomerol_amd.ED(W.terms(H), c.beta)
+        if name == "config2":
+            e.truncate_blocks(1e-12)
+        tol = 1e-12 if name == "config2" else 1e-16
+        zs = grid(n, c.beta)
+        e.chi3(1, u, u, d, d, zs, 1e-8, tol)  # warm-up (operators, workspace)
+        r = e.chi3(1, u, u, d, d, zs, 1e-8, tol)
+        merged = r["n_ff"] + r["n_fb"] + r["n_res"]
+        rec = {"pairs": len(zs), "parts": r["n_parts"], "raw_terms": r["n_raw"], "merged_terms": merged,
+               "terms_ff_fb_res": [r["n_ff"], r["n_fb"], r["n_res"]], "seconds": r["seconds"],
+               "terms_pairs_per_s": merged * len(zs) / r["seconds"], "coeff_tol": tol}
+        if name == "config1" and not args.no_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import pomerol_oracle
+
+            o = pomerol_oracle.ED(len(idx), H.mode_terms(), c.beta)
+            t0 = time.perf_counter()
+            ro = o.chi3(1, u, u, d, d, zs.ravel(), 1e-8, tol)
+            secs = time.perf_counter() - t0
+            rec["oracle_seconds"] = secs
+            rec["oracle_terms_pairs_per_s"] = (ro["n_ff"] + ro["n_fb"] + ro["n_res"]) * len(zs) / secs
+            rec["oracle_cores"] = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count()))
+            rec["max_rel_diff_vs_oracle"] = float(np.abs(r["values"] - ro["values"]).max() / np.abs(ro["values"]).max())
+        log(f"[bench] chi3 {name}: {rec}")
+        out[name] = rec
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -251,6 +298,8 @@ def main():
     ap.add_argument("--dense-only", action="store_true", help="only the dense eigensolve and GEMM legs")
     ap.add_argument("--batch-only", action="store_true", help="only the batched-solve leg")
     ap.add_argument("--batch-eager", action="store_true", help="batched solves launched eagerly (no graph replay)")
+    ap.add_argument("--chi3-n", type=int, default=64, help="3-point susceptibility leg: n x n pairs (0: skip)")
+    ap.add_argument("--chi3-only", action="store_true", help="only the 3-point susceptibility leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path's multi-GPU "
@@ -262,6 +311,9 @@ def main():
         return
     if args.batch_only:
         print(json.dumps({"batch": run_batch(args)}), flush=True)
+        return
+    if args.chi3_only:
+        print(json.dumps({"chi3": run_chi3(args)}), flush=True)
         return
 
     import numpy as np
@@ -329,6 +381,7 @@ def main():
 
     batched = run_batch(args) if args.batch_inst > 0 else None
     dense = run_dense(args) if (args.eigh_n > 0 or args.gemm_n > 0) else None
+    chi3 = run_chi3(args) if args.chi3_n > 0 else None
 
     out = {
         "metric": "2PGF terms*freqs/s",
@@ -359,6 +412,7 @@ def main():
         "cpu_baseline": cpu,
         "batch": batched,
         "dense": dense,
+        "chi3": chi3,
     }
     print(json.dumps(out), flush=True)
 
