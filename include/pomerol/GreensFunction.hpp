@@ -92,6 +92,9 @@ struct IndexCombination2 {
     bool operator<(IndexCombination2 const& o) const { return std::tie(Index1, Index2) < std::tie(o.Index1, o.Index2); }
     bool operator==(IndexCombination2 const& o) const { return Index1 == o.Index1 && Index2 == o.Index2; }
     bool operator!=(IndexCombination2 const& o) const { return !(*this == o); }
+    friend std::ostream& operator<<(std::ostream& os, IndexCombination2 const& ic) {
+        return os << "(" << ic.Index1 << ic.Index2 << ")";
+    }
 };
 
 /// G_ij for a set of index pairs (GFContainer.cpp:20-48; every pair if the set is empty).

@@ -10,6 +10,9 @@
 // state and list their states in ascending order.
 #pragma once
 
+#include <ostream>
+#include <tuple>
+
 #include "Misc.hpp"
 #include "Operators.hpp"
 
@@ -87,6 +90,20 @@ public:
         if(!checkIndex(in))
             throw std::runtime_error("IndexClassification: wrong index " + std::to_string(in));
         return info_[in];
+    }
+    /// "Index i = (label, orbital, spin)" per line (IndexClassification.hpp:122-129).
+    friend std::ostream& operator<<(std::ostream& os, IndexClassification const& ic) {
+        for(ParticleIndex i = 0; i < ic.info_.size(); ++i) {
+            os << "Index " << i << " = (";
+            std::apply(
+                [&os](auto const&... x) {
+                    int k = 0;
+                    ((os << (k++ ? ", " : "") << x), ...);
+                },
+                ic.info_[i]);
+            os << ")\n";
+        }
+        return os;
     }
 
     template <typename ScalarType> FermionOperator toOperator(Operators::expression<ScalarType, IndexTypes...> const& e) const {

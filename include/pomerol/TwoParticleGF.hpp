@@ -152,6 +152,9 @@ struct IndexCombination4 {
         return Index1 == o.Index1 && Index2 == o.Index2 && Index3 == o.Index3 && Index4 == o.Index4;
     }
     bool operator!=(IndexCombination4 const& o) const { return !(*this == o); }
+    friend std::ostream& operator<<(std::ostream& os, IndexCombination4 const& ic) {
+        return os << "(" << ic.Index1 << ic.Index2 << ic.Index3 << ic.Index4 << ")";
+    }
 };
 
 /// An element with a frequency permutation: a crossing-symmetry image of a computed element
