@@ -123,7 +123,7 @@ __global__ void chi3_widen_kernel(int64_t n, const int* __restrict__ in, int64_t
 
 constexpr uint64_t kRankMask = (uint64_t(1) << 31) - 1;
 
-__global__ void __launch_bounds__(256) chi3_emit_kernel(Chi3Dev D, int64_t nrows, const int64_t* __restrict__ off,
+__global__ void __launch_bounds__(256) chi3_emit_kernel(Chi3Dev D, int64_t nrows, int64_t ntot, const int64_t* __restrict__ off,
                                                         const uint32_t* __restrict__ r12, const uint32_t* __restrict__ r23,
                                                         const uint32_t* __restrict__ r13, uint64_t* __restrict__ keys,
                                                         uint64_t* __restrict__ vals) {
@@ -134,6 +134,7 @@ __global__ void __launch_bounds__(256) chi3_emit_kernel(Chi3Dev D, int64_t nrows
         if(fa == 0.0)
             continue; // wave-uniform
         int64_t base = off[row];
+        int64_t const end = (row + 1 < nrows) ? off[row + 1] : ntot; // never write past this row's slots
         uint64_t const a12 = r12[row];
         for(int c0 = 0; c0 < D.n3; c0 += 64) {
             int const i3 = c0 + lane;
@@ -149,7 +150,7 @@ __global__ void __launch_bounds__(256) chi3_emit_kernel(Chi3Dev D, int64_t nrows
                     incl += t;
             }
             int const total = __shfl(incl, 63);
-            if(f) {
+            if(f && base + incl <= end) {
                 int64_t pos = base + incl - c;
                 uint64_t const a23 = r23[int64_t(i2) * D.n3 + i3], a13 = r13[int64_t(i1) * D.n3 + i3];
                 uint64_t const v = (uint64_t(i1) << 40) | (uint64_t(i2) << 24) | (uint64_t(i3) << 8);
@@ -705,7 +706,7 @@ int chi3_part_run(pomx_ctx* ctx, pomx_chi3_part const& P, double beta, double re
         CHI3_WS(ctx, k1);
         CHI3_WS(ctx, v0);
         CHI3_WS(ctx, v1);
-        chi3_emit_kernel<<<rblocks, 256, 0, ctx->stream>>>(D, nrows, off, r12, r23, r13, k0, v0);
+        chi3_emit_kernel<<<rblocks, 256, 0, ctx->stream>>>(D, nrows, n, off, r12, r23, r13, k0, v0);
         POMX_CHECK_LAUNCH(ctx);
         if((rc = pomx_sort_pairs(ctx, n, k0, k1, v0, v1, 0, 64)))
             return rc;
