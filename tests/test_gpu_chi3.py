@@ -126,3 +126,27 @@ def test_chi3_container(pa):
     for k, v in out.items():
         assert len(v) == (len(zs) if k == first else 0), k
     assert np.abs(out[first] - e.chi3(1, u, d, u, d, zs)["values"]).max() < 1e-14
+
+
+def test_chi3_config2_matches_oracle(pa, oracle):
+    """The workload of the bench's 3-point susceptibility leg (Anderson 1+5 bath, 4096 states,
+    truncateBlocks(1e-12), CoefficientTolerance 1e-12, PH up-up-dn-dn): values on 8 x 8 pairs at
+    1e-10 relative and every part's merged-term counts equal to the oracle's (whose merge runs the
+    reference's sequential TermList algorithm), erasures included."""
+    c = W.cfg2()
+    H = c.expr()
+    idx = H.indices()
+    u, d = idx.index(("C", 0, 1)), idx.index(("C", 0, 0))
+    zs = chi3_grid(np.arange(-4, 4), c.beta)
+    print("[gpu-chi3] config 2 on the device", flush=True)
+    e = pa.ED(W.terms(H), c.beta)
+    e.truncate_blocks(1e-12)
+    r = e.chi3(1, u, u, d, d, zs, 1e-8, 1e-12)
+    print("[gpu-chi3] config 2 on the oracle (about 25 s)", flush=True)
+    o = oracle.ED(len(idx), H.mode_terms(), c.beta)
+    o.truncate_blocks(1e-12)
+    ref = o.chi3(1, u, u, d, d, zs.ravel(), 1e-8, 1e-12)
+    assert r["n_parts"] == ref["n_parts"] and list(r["part_blocks"]) == list(ref["part_blocks"])
+    assert rel_err(r["values"], ref["values"]) < 1e-10
+    for key in ("part_ff", "part_fb", "part_res"):
+        assert list(r[key]) == list(ref[key]), key
