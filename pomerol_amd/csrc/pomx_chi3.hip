@@ -201,18 +201,6 @@ __global__ void chi3_first_kernel(int64_t ngroups, const int64_t* __restrict__ s
     }
 }
 
-// number of entries of the ascending array a[lo, hi) that are <= q, plus lo
-__device__ inline int64_t upper_index(const uint64_t* __restrict__ a, int64_t lo, int64_t hi, uint64_t q) {
-    while(lo < hi) {
-        int64_t const mid = lo + (hi - lo) / 2;
-        if(a[mid] <= q)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
 // One thread per key group.  Coefficient and poles of each member as addMultiterm forms them:
 //   FF:  c = +-M (Wi + Wj), poles (Eij, -+Ejk) or, swapped, (Ejk, -+Eij);
 //   FF2: c = +-M (Wk - Wi), poles (Eij, -+Ejk);
@@ -224,9 +212,7 @@ __device__ inline int64_t upper_index(const uint64_t* __restrict__ a, int64_t lo
 // at the offsets evn[gi].
 template <bool EMIT>
 __global__ void chi3_reduce_kernel(Chi3Dev D, int64_t ngroups, int64_t n, const int64_t* __restrict__ start,
-                                   const uint64_t* __restrict__ keys, const uint64_t* __restrict__ vals,
-                                   const uint64_t* __restrict__ fk_sorted, int64_t nev,
-                                   const uint64_t* __restrict__ ev_key, const int64_t* __restrict__ ev_cum,
+                                   const uint64_t* __restrict__ keys, const uint64_t* __restrict__ vals, EraseRule R,
                                    double* __restrict__ g, int64_t* __restrict__ live, int64_t* __restrict__ evn,
                                    uint64_t* __restrict__ ev_key_out, uint64_t* __restrict__ ev_w_out) {
     int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
@@ -234,10 +220,8 @@ __global__ void chi3_reduce_kernel(Chi3Dev D, int64_t ngroups, int64_t n, const 
         return;
     int64_t const s = start[gi], e = (gi + 1 < ngroups) ? start[gi + 1] : n;
     int const kind = int(keys[s] >> 62);
-    uint64_t const kbits = keys[s] & kKindMask;
-    int64_t const kind_lo = kbits ? upper_index(fk_sorted, 0, ngroups, kbits - 1) : 0; // first group of the kind
-    int64_t const ev_lo = (nev > 0 && kbits) ? upper_index(ev_key, 0, nev, kbits - 1) : 0;
-    int64_t const ev_base = ev_lo > 0 ? ev_cum[ev_lo - 1] : 0;
+    uint64_t const kbits = keys[s] & kKindMask; // each kind is its own TermList
+    EraseCursor const cur = erase_cursor(R, kbits);
     double const pm = D.pp ? 1.0 : -1.0;
     bool alive = false;
     double c = 0, p0 = 0, p1 = 0;
@@ -295,13 +279,7 @@ __global__ void chi3_reduce_kernel(Chi3Dev D, int64_t ngroups, int64_t n, const 
             p1 += P1;
             ++cnt;
             uint64_t const q = kbits | v;
-            int64_t const started = upper_index(fk_sorted, kind_lo, ngroups, q) - kind_lo;
-            int64_t dead = 0;
-            if(nev > 0) { // events of this kind strictly before q
-                int64_t const hi = upper_index(ev_key, ev_lo, nev, q - 1);
-                dead = (hi > 0 ? ev_cum[hi - 1] : 0) - ev_base;
-            }
-            if(fabs(c) <= D.tol / double(max<int64_t>(started - dead, 1))) {
+            if(fabs(c) <= erase_threshold(R, cur, q)) {
                 alive = false;
                 if(EMIT) {
                     ev_key_out[at] = q;
@@ -319,22 +297,6 @@ __global__ void chi3_reduce_kernel(Chi3Dev D, int64_t ngroups, int64_t n, const 
         live[gi] = alive ? 1 : 0;
         evn[gi] = nevents;
     }
-}
-
-// inclusive prefix sums of the event weights: cum = exclusive + own
-__global__ void chi3_event_cum_kernel(int64_t n, const uint64_t* __restrict__ w, int64_t* __restrict__ cum) {
-    int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if(k < n)
-        cum[k] += int64_t(w[k]);
-}
-
-// flag[0] = 1 if the sorted event lists differ
-__global__ void chi3_event_diff_kernel(int64_t n, const uint64_t* __restrict__ a, const uint64_t* __restrict__ aw,
-                                       const uint64_t* __restrict__ b, const uint64_t* __restrict__ bw,
-                                       int* __restrict__ flag) {
-    int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if(k < n && (a[k] != b[k] || aw[k] != bw[k]))
-        flag[0] = 1;
 }
 
 // kinfo[0..2]: live index of the first group of each kind; kinfo[3..5]: live groups per kind.
@@ -566,72 +528,6 @@ int chi3_eval(pomx_ctx* ctx, const double* data, int64_t const count[3], int xi,
     return POMX_OK;
 }
 
-// Reduce passes until the erase/restart events a pass produces are the ones it assumed.
-int chi3_reduce_fixed_point(pomx_ctx* ctx, Chi3Dev const& D, int64_t ng, int64_t n, const int64_t* start,
-                            const uint64_t* keys, const uint64_t* vals, const uint64_t* fk_sorted, double* g,
-                            int64_t* live) {
-    constexpr int kMaxPasses = 64;
-    int rc;
-    WsBuf<int64_t> evn(ctx, ng), evoff(ctx, ng);
-    WsBuf<int> flag(ctx, 1);
-    CHI3_WS(ctx, evn);
-    CHI3_WS(ctx, evoff);
-    CHI3_WS(ctx, flag);
-    WsBuf<uint64_t> ev_key, ev_w; // assumed events (sorted)
-    WsBuf<int64_t> ev_cum;
-    int64_t nev = 0;
-    for(int pass = 0; pass < kMaxPasses; ++pass) {
-        chi3_reduce_kernel<false><<<grid_for(ng), kBlock, 0, ctx->stream>>>(
-            D, ng, n, start, keys, vals, fk_sorted, nev, ev_key.p, ev_cum.p, g, live, evn, nullptr, nullptr);
-        POMX_CHECK_LAUNCH(ctx);
-        if((rc = pomx_exclusive_scan(ctx, ng, evn, evoff)))
-            return rc;
-        int64_t la = 0, lb = 0;
-        if((rc = read_i64(ctx, evn.p + ng - 1, &la)) || (rc = read_i64(ctx, evoff.p + ng - 1, &lb)))
-            return rc;
-        int64_t const ne = la + lb;
-        if(ne == 0 && nev == 0)
-            return POMX_OK; // no erasure anywhere: the first pass is exact
-        WsBuf<uint64_t> k0(ctx, std::max<int64_t>(ne, 1)), w0(ctx, std::max<int64_t>(ne, 1));
-        WsBuf<uint64_t> k1(ctx, std::max<int64_t>(ne, 1)), w1(ctx, std::max<int64_t>(ne, 1));
-        CHI3_WS(ctx, k0);
-        CHI3_WS(ctx, w0);
-        CHI3_WS(ctx, k1);
-        CHI3_WS(ctx, w1);
-        if(ne > 0) {
-            chi3_reduce_kernel<true><<<grid_for(ng), kBlock, 0, ctx->stream>>>(
-                D, ng, n, start, keys, vals, fk_sorted, nev, ev_key.p, ev_cum.p, g, live, evoff, k0, w0);
-            POMX_CHECK_LAUNCH(ctx);
-            if((rc = pomx_sort_pairs(ctx, ne, k0, k1, w0, w1, 0, 64)))
-                return rc;
-        }
-        if(ne == nev) {
-            if((rc = pomx_fill_zero(ctx, flag.p, sizeof(int))))
-                return rc;
-            chi3_event_diff_kernel<<<grid_for(ne), kBlock, 0, ctx->stream>>>(ne, k1, w1, ev_key, ev_w, flag);
-            POMX_CHECK_LAUNCH(ctx);
-            int hf = 1;
-            POMX_HIP(ctx, hipMemcpyAsync(&hf, flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-            POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            if(!hf)
-                return POMX_OK; // this pass reproduced the events it assumed
-        }
-        WsBuf<int64_t> cum(ctx, std::max<int64_t>(ne, 1));
-        CHI3_WS(ctx, cum);
-        if(ne > 0) {
-            if((rc = pomx_exclusive_scan(ctx, ne, reinterpret_cast<const int64_t*>(w1.p), cum)))
-                return rc;
-            chi3_event_cum_kernel<<<grid_for(ne), kBlock, 0, ctx->stream>>>(ne, w1, cum);
-            POMX_CHECK_LAUNCH(ctx);
-        }
-        ev_key = std::move(k1);
-        ev_w = std::move(w1);
-        ev_cum = std::move(cum);
-        nev = ne;
-    }
-    return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_chi3_part_compute: term erasures did not settle");
-}
-
 int chi3_part_run(pomx_ctx* ctx, pomx_chi3_part const& P, double beta, double res, double tol, int nfreq,
                   const double* d_freq, double* d_acc, pomx_chi3_terms* keep, int64_t* h_counts, int64_t* h_raw) {
     POMX_REQUIRE(ctx, P.n1 >= 1 && P.n2 >= 1 && P.n3 >= 1 && P.n4 >= 1, "pomx_chi3_part_compute: empty block");
@@ -741,7 +637,19 @@ int chi3_part_run(pomx_ctx* ctx, pomx_chi3_part const& P, double beta, double re
             POMX_CHECK_LAUNCH(ctx);
             if((rc = pomx_sort_pairs(ctx, ng, f0, f1, i0, i1, 0, 64)))
                 return rc;
-            if((rc = chi3_reduce_fixed_point(ctx, D, ng, n, start, k1, v1, f1, g, live)))
+            EraseRule const rule{tol, f1.p, ng, nullptr, nullptr, 0};
+            EraseEvents ev;
+            auto pass = [&](EraseRule const& R, bool emit, int64_t* evn, uint64_t* ko, uint64_t* wo) -> int {
+                if(emit)
+                    chi3_reduce_kernel<true><<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, ng, n, start, k1, v1, R, g,
+                                                                                         live, evn, ko, wo);
+                else
+                    chi3_reduce_kernel<false><<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, ng, n, start, k1, v1, R, g,
+                                                                                          live, evn, ko, wo);
+                POMX_CHECK_LAUNCH(ctx);
+                return POMX_OK;
+            };
+            if((rc = erase_fixed_point(ctx, ng, rule, pass, ev, "pomx_chi3_part_compute")))
                 return rc;
         }
         if((rc = pomx_exclusive_scan(ctx, ng, live, lpos)))

@@ -241,15 +241,59 @@ __global__ void group_start_kernel(int64_t n, const int64_t* __restrict__ head, 
         start[gid[k]] = k;
 }
 
+// Visiting position of a raw term in the reference's loop (TwoParticleGFPart.cpp:209-250:
+// i1, i3, i2, i4, then addMultiterm's order NR z2, NR z4 | R z1z2, R z2z3), with the TermList
+// (non-resonant 0, resonant 1) in the top bit.
+__device__ inline uint64_t g2_position(G2Dev const& D, int kind, int i1, int i2, int i3, int i4) {
+    uint64_t const q = ((uint64_t(i1) * D.n3 + i3) * D.n2 + i2) * D.n4 + i4;
+    uint64_t const slot = (kind == 1 || kind == 2) ? 1u : 0u;
+    return (uint64_t(kind >= 2) << 63) | (q << 1) | slot;
+}
+__device__ inline void g2_unpack(uint64_t v, int& i1, int& i2, int& i3, int& i4) {
+    i1 = int(v & 0xffff);
+    i2 = int((v >> 16) & 0xffff);
+    i3 = int((v >> 32) & 0xffff);
+    i4 = int(v >> 48);
+}
+
+// first (smallest) visiting position of each group's members
+__global__ void g2_first_kernel(G2Dev D, int64_t n, int64_t ngroups, const int64_t* __restrict__ start,
+                                const uint64_t* __restrict__ keys, const uint64_t* __restrict__ vals,
+                                uint64_t* __restrict__ fk, uint64_t* __restrict__ gi_out) {
+    int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(gi >= ngroups)
+        return;
+    int64_t const s = start[gi], e = (gi + 1 < ngroups) ? start[gi + 1] : n;
+    int const kind = int(keys[s] >> 62);
+    uint64_t m = ~uint64_t(0);
+    for(int64_t k = s; k < e; ++k) {
+        int i1, i2, i3, i4;
+        g2_unpack(vals[k], i1, i2, i3, i4);
+        uint64_t const q = g2_position(D, kind, i1, i2, i3, i4);
+        m = q < m ? q : m;
+    }
+    fk[gi] = m;
+    gi_out[gi] = uint64_t(gi);
+}
+
 // One thread per key group: sum the members' coefficients (pole = count-weighted mean).
+// The erase threshold is thr, or, with R.fk set, the reference's tol/(size+1) (EraseRule); EMIT
+// as for erase_fixed_point (events written at evn[gi], group results untouched).  With the fixed
+// thr, ambig (if given) is set when a merge's max(|c0|, |c1|) lies in (thr, R.tol]: only such a
+// merge can be decided differently by the reference's threshold, which lies in that interval.
+template <bool EMIT>
 __global__ void g2_reduce_kernel(G2Dev D, int64_t n, int64_t ngroups, const int64_t* __restrict__ start,
                                  const uint64_t* __restrict__ keys, const uint64_t* __restrict__ vals, double thr,
-                                 double* __restrict__ g, int64_t* __restrict__ live) {
+                                 EraseRule R, double* __restrict__ g, int64_t* __restrict__ live,
+                                 int64_t* __restrict__ evn, uint64_t* __restrict__ ev_key_out,
+                                 uint64_t* __restrict__ ev_w_out, int* __restrict__ ambig) {
     int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if(gi >= ngroups)
         return;
     int64_t s = start[gi], e = (gi + 1 < ngroups) ? start[gi + 1] : n;
     int kind = int(keys[s] >> 62);
+    EraseCursor const cur = R.fk ? erase_cursor(R, uint64_t(kind >= 2) << 63) : EraseCursor{0, 0, 0};
+    int64_t nevents = 0, at = EMIT ? evn[gi] : 0;
     bool alive = false;
     double c0 = 0, c1 = 0, p1 = 0, p2 = 0, p3 = 0;
     long cnt = 0;
@@ -272,6 +316,13 @@ __global__ void g2_reduce_kernel(G2Dev D, int64_t n, int64_t ngroups, const int6
         }
         double P1 = D.E2[i2] - D.E1[i1], P2 = D.E3[i3] - D.E2[i2], P3 = D.E4[i4] - D.E3[i3];
         if(!alive) {
+            if(k > s && R.fk) { // restart of an erased group
+                if(EMIT) {
+                    ev_key_out[at] = g2_position(D, kind, i1, i2, i3, i4);
+                    ev_w_out[at++] = uint64_t(int64_t(-1));
+                }
+                ++nevents;
+            }
             alive = true;
             c0 = a;
             c1 = b;
@@ -286,17 +337,35 @@ __global__ void g2_reduce_kernel(G2Dev D, int64_t n, int64_t ngroups, const int6
             p2 += P2;
             p3 += P3;
             ++cnt;
-            if(fabs(c0) <= thr && fabs(c1) <= thr)
+            double const t = R.fk ? erase_threshold(R, cur, g2_position(D, kind, i1, i2, i3, i4)) : thr;
+            if(ambig) {
+                double const x = fmax(fabs(c0), fabs(c1));
+                if(x > thr && x <= R.tol)
+                    ambig[0] = 1;
+            }
+            if(fabs(c0) <= t && fabs(c1) <= t) {
                 alive = false;
+                if(R.fk) {
+                    if(EMIT) {
+                        ev_key_out[at] = g2_position(D, kind, i1, i2, i3, i4);
+                        ev_w_out[at++] = 1;
+                    }
+                    ++nevents;
+                }
+            }
         }
     }
-    double inv = 1.0 / double(cnt);
-    g[gi * 5 + 0] = c0;
-    g[gi * 5 + 1] = c1;
-    g[gi * 5 + 2] = p1 * inv;
-    g[gi * 5 + 3] = p2 * inv;
-    g[gi * 5 + 4] = p3 * inv;
-    live[gi] = alive ? 1 : 0;
+    if(!EMIT) {
+        double inv = 1.0 / double(cnt);
+        g[gi * 5 + 0] = c0;
+        g[gi * 5 + 1] = c1;
+        g[gi * 5 + 2] = p1 * inv;
+        g[gi * 5 + 3] = p2 * inv;
+        g[gi * 5 + 4] = p3 * inv;
+        live[gi] = alive ? 1 : 0;
+        if(evn)
+            evn[gi] = nevents;
+    }
 }
 
 // scatter live groups into kind-segmented SoA output; seg_base[kind] = first index of the kind
@@ -331,12 +400,13 @@ __global__ void g2_compact_kernel(int64_t ngroups, const int64_t* __restrict__ l
 // c0, c1, P1, P2, P3, dc0, dc1, dP1, dP2, dP3.
 __global__ void g2_reduce_tangent_kernel(G2Dev D, G2Tan T, int64_t n, int64_t ngroups, const int64_t* __restrict__ start,
                                          const uint64_t* __restrict__ keys, const uint64_t* __restrict__ vals, double thr,
-                                         double* __restrict__ g, int64_t* __restrict__ live) {
+                                         EraseRule R, double* __restrict__ g, int64_t* __restrict__ live) {
     int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if(gi >= ngroups)
         return;
     int64_t s = start[gi], e = (gi + 1 < ngroups) ? start[gi + 1] : n;
     int kind = int(keys[s] >> 62);
+    EraseCursor const cur = R.fk ? erase_cursor(R, uint64_t(kind >= 2) << 63) : EraseCursor{0, 0, 0};
     bool alive = false;
     double c0 = 0, c1 = 0, p1 = 0, p2 = 0, p3 = 0;
     double dc0 = 0, dc1 = 0, dp1 = 0, dp2 = 0, dp3 = 0;
@@ -396,7 +466,8 @@ __global__ void g2_reduce_tangent_kernel(G2Dev D, G2Tan T, int64_t n, int64_t ng
             dp2 += dP2;
             dp3 += dP3;
             ++cnt;
-            if(fabs(c0) <= thr && fabs(c1) <= thr)
+            double const t = R.fk ? erase_threshold(R, cur, g2_position(D, kind, i1, i2, i3, i4)) : thr;
+            if(fabs(c0) <= t && fabs(c1) <= t)
                 alive = false;
         }
     }
@@ -1217,14 +1288,15 @@ namespace {
 
 // Tangent reduction, compaction and evaluation of one chunk's sorted groups.
 int g2_chunk_tangent(pomx_ctx* ctx, G2Dev const& D, pomx_g2_tangent const& t, int64_t n, int64_t ng,
-                     const int64_t* start, const uint64_t* keys, const uint64_t* vals, double thr, int perm, double res,
+                     const int64_t* start, const uint64_t* keys, const uint64_t* vals, double thr, EraseRule const& R,
+                     int perm, double res,
                      int nfreq, const double* d_freq, double* d_acc, double* d_dacc, int64_t counts[4]) {
     G2Tan T{t.dO1, t.dO2, t.dO3, t.dCX4, t.dE1, t.dE2, t.dE3, t.dE4, t.dw1, t.dw2, t.dw3, t.dw4};
     WsBuf<int64_t> live(ctx, ng), lpos(ctx, ng), kc(ctx, 4), d_kf(ctx, 4);
     WsBuf<double> g(ctx, size_t(ng) * 10);
     if(!live.p || !lpos.p || !g.p)
         return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_tangent: out of device memory");
-    g2_reduce_tangent_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, T, n, ng, start, keys, vals, thr, g, live);
+    g2_reduce_tangent_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, T, n, ng, start, keys, vals, thr, R, g, live);
     int rc = pomx_exclusive_scan(ctx, ng, live, lpos);
     if(rc)
         return rc;
@@ -1346,6 +1418,8 @@ int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* 
         if(npairs > b)
             chunks.emplace_back(b, npairs);
     }
+    // one chunk holds every group of the part; positions need n1 n2 n3 n4 * 2 < 2^63
+    bool const exact_merge = chunks.size() == 1 && double(P.n1) * P.n2 * P.n3 * P.n4 * 2.0 < 9.0e18;
     struct KeptChunk {
         double* nr;
         double* r;
@@ -1380,17 +1454,71 @@ int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* 
         int64_t ng = lh + lg;
         WsBuf<int64_t> start(ctx, ng), live(ctx, ng), lpos(ctx, ng);
         WsBuf<double> g(ctx, size_t(ng) * 5);
+        if(!start.p || !live.p || !lpos.p || !g.p)
+            return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
         group_start_kernel<<<grid_for(n), kBlock, 0, ctx->stream>>>(n, head, gid, start);
-        // TermList erases a merged term when |coeff| <= tol/(size+1); size is taken as the
-        // number of groups in this chunk
-        double thr = tol / double(ng + 1);
+        // TermList erases a merged term when |coeff| <= tol/(size+1), size+1 being the number of
+        // terms in the list at that moment.  A part generated in one chunk holds every group, so
+        // the reference's rule is applied exactly: a pass with the fixed threshold tol/(ng+1)
+        // decides every merge as the reference does unless some merge is ambiguous, and only then
+        // the first positions are sorted and the events settled (EraseRule, erase_fixed_point).
+        // Across chunks, size is taken as the number of groups in this chunk.
+        double const thr = tol / double(ng + 1);
+        EraseRule rule{tol, nullptr, 0, nullptr, nullptr, 0};
+        EraseEvents ev;
+        WsBuf<uint64_t> fk0, fk1;
+        bool exact_pass = false;
+        if(exact_merge) { // a pass with the fixed threshold tells whether any decision could differ
+            WsBuf<int> ambig(ctx, 1);
+            if(!ambig.p)
+                return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+            if((rc = pomx_fill_zero(ctx, ambig.p, sizeof(int))))
+                return rc;
+            g2_reduce_kernel<false><<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, n, ng, start, k1, v1, thr, rule, g, live,
+                                                                                nullptr, nullptr, nullptr, ambig);
+            POMX_CHECK_LAUNCH(ctx);
+            int ha = 0;
+            POMX_HIP(ctx, hipMemcpyAsync(&ha, ambig.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            exact_pass = ha != 0;
+        }
+        if(exact_pass) {
+            fk0 = WsBuf<uint64_t>(ctx, ng);
+            fk1 = WsBuf<uint64_t>(ctx, ng);
+            WsBuf<uint64_t> i0(ctx, ng), i1(ctx, ng);
+            if(!fk0.p || !fk1.p || !i0.p || !i1.p)
+                return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+            g2_first_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, n, ng, start, k1, v1, fk0, i0);
+            POMX_CHECK_LAUNCH(ctx);
+            if((rc = pomx_sort_pairs(ctx, ng, fk0, fk1, i0, i1, 0, 64)))
+                return rc;
+            rule.fk = fk1.p;
+            rule.nfk = ng;
+            auto pass = [&](EraseRule const& R, bool emit, int64_t* evn, uint64_t* ko, uint64_t* wo) -> int {
+                if(emit)
+                    g2_reduce_kernel<true><<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, n, ng, start, k1, v1, thr, R, g,
+                                                                                       live, evn, ko, wo, nullptr);
+                else
+                    g2_reduce_kernel<false><<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, n, ng, start, k1, v1, thr, R, g,
+                                                                                        live, evn, ko, wo, nullptr);
+                POMX_CHECK_LAUNCH(ctx);
+                return POMX_OK;
+            };
+            if((rc = erase_fixed_point(ctx, ng, rule, pass, ev, "pomx_g2_part_compute")))
+                return rc;
+            rule.ev_key = ev.key.p;
+            rule.ev_cum = ev.cum.p;
+            rule.nev = ev.n;
+        }
         if(tan) {
-            if((rc = g2_chunk_tangent(ctx, D, *tan, n, ng, start, k1, v1, thr, P.perm, res, nfreq, d_freq, d_acc, d_dacc,
-                                      counts)))
+            if((rc = g2_chunk_tangent(ctx, D, *tan, n, ng, start, k1, v1, thr, rule, P.perm, res, nfreq, d_freq, d_acc,
+                                      d_dacc, counts)))
                 return rc;
             continue;
         }
-        g2_reduce_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, n, ng, start, k1, v1, thr, g, live);
+        if(!exact_merge) // (with exact_merge, g and live already hold the exact result)
+            g2_reduce_kernel<false><<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, n, ng, start, k1, v1, thr, rule, g, live,
+                                                                                nullptr, nullptr, nullptr, nullptr);
         WsBuf<int64_t> kc(ctx, 4);
         if((rc = pomx_exclusive_scan(ctx, ng, live, lpos)))
             return rc;

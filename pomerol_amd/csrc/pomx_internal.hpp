@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -91,6 +92,77 @@ int read_i64(pomx_ctx* ctx, const int64_t* d, int64_t* h);
 // |bin| >= 1e18.
 int rank_table(pomx_ctx* ctx, int nx, int ny, const double* Ex, const double* Ey, double res, WsBuf<uint32_t>& rank,
                WsBuf<uint64_t>* sorted, int64_t* nranks, int* d_err);
+
+// ---- TermList erase rule (the 3-point susceptibility and 2PGF merges) ---------------------
+// A TermList erases a merged term when |C| <= tol/(size+1), size+1 being the number of terms in
+// that list at that moment (TermList.hpp, add_term).  For a one-thread-per-group merge that
+// number is the count of groups of the same list whose first member precedes the merge in the
+// reference's visiting order, less those erased and not restarted before it.  Positions carry
+// the list in their top bits, so one ascending array holds every list.
+
+// lo + number of entries of the ascending a[lo, hi) that are <= q
+__device__ inline int64_t upper_index(const uint64_t* __restrict__ a, int64_t lo, int64_t hi, uint64_t q) {
+    while(lo < hi) {
+        int64_t const mid = lo + (hi - lo) / 2;
+        if(a[mid] <= q)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+struct EraseRule {
+    double tol;
+    const uint64_t* fk;     // (list, first position) of every group, ascending
+    int64_t nfk;
+    const uint64_t* ev_key; // assumed erase (+1) and restart (-1) events, (list, position) ascending
+    const int64_t* ev_cum;  // inclusive prefix sums of their weights
+    int64_t nev;
+};
+
+struct EraseCursor {
+    int64_t fk_lo, ev_lo, ev_base;
+};
+
+// where the list with top bits 'lbits' starts in fk and in the events
+__device__ inline EraseCursor erase_cursor(EraseRule const& R, uint64_t lbits) {
+    EraseCursor c;
+    c.fk_lo = lbits ? upper_index(R.fk, 0, R.nfk, lbits - 1) : 0;
+    c.ev_lo = (R.nev > 0 && lbits) ? upper_index(R.ev_key, 0, R.nev, lbits - 1) : 0;
+    c.ev_base = c.ev_lo > 0 ? R.ev_cum[c.ev_lo - 1] : 0;
+    return c;
+}
+
+// threshold of a merge at position q (q > 0, list bits included)
+__device__ inline double erase_threshold(EraseRule const& R, EraseCursor const& c, uint64_t q) {
+    int64_t const started = upper_index(R.fk, c.fk_lo, R.nfk, q) - c.fk_lo;
+    int64_t dead = 0;
+    if(R.nev > 0) { // events of this list strictly before q
+        int64_t const hi = upper_index(R.ev_key, c.ev_lo, R.nev, q - 1);
+        dead = (hi > 0 ? R.ev_cum[hi - 1] : 0) - c.ev_base;
+    }
+    int64_t const size1 = started - dead;
+    return R.tol / double(size1 > 1 ? size1 : 1);
+}
+
+// Erase and restart events settled by fixed-point passes of a group reduce: a pass takes the
+// previous pass's events (none in the first) into the thresholds, and the iteration stops when a
+// pass reproduces them.  Every decision depends only on events at earlier positions, so by
+// induction over the visiting order the settled pass is the sequential result.
+// pass(rule, emit, evn, key_out, w_out) launches one reduce over all groups: emit == false writes
+// each group's event count to evn (and the group results), emit == true writes the events at the
+// offsets evn (key = list | position, weight +1 erase, -1 restart as uint64).
+struct EraseEvents {
+    WsBuf<uint64_t> key, w;
+    WsBuf<int64_t> cum;
+    int64_t n = 0;
+};
+using ErasePass = std::function<int(EraseRule const&, bool emit, int64_t* evn, uint64_t* key_out, uint64_t* w_out)>;
+// 'rule' brings tol and fk; on return ev holds the settled events and the last pass's group
+// results are in place.
+int erase_fixed_point(pomx_ctx* ctx, int64_t ng, EraseRule rule, ErasePass const& pass, EraseEvents& ev,
+                      const char* who);
 
 // Evaluation-kernel timing hooks (HIP events on the context stream).
 void eval_timer_begin(pomx_ctx* ctx);

@@ -173,10 +173,9 @@ def test_two_particle_gf(pa, oracle, name, comp):
     oi = inject(e, oracle.ED(len(idx), H.mode_terms(), beta))
     same = oi.g2(*ijkl, zs.ravel(), 1e-8, 1e-16)
     assert rel_err(r["values"], same["values"]) < 1e-12
-    assert list(r["part_nonres"]) == list(same["part_nonres"])  # non-resonant terms: exact
-    # resonant terms: the TermList erasure threshold tol/(size+1) depends on the insertion
-    # history; counts agree to a few terms in 1e6 (the differing terms have |coeff| < 1e-15)
-    assert abs(r["n_res"] - same["n_res"]) <= max(2, 2e-5 * same["n_res"])
+    # merged-term counts of every part, erasures by TermList's tol/(size+1) included: exact
+    assert list(r["part_nonres"]) == list(same["part_nonres"])
+    assert list(r["part_res"]) == list(same["part_res"])
 
 
 def test_2pgf_container_crossing_images(pa):
