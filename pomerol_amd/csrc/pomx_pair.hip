@@ -8,7 +8,8 @@
 // TermList (same hash bin, TermList.hpp:65-76) is contiguous and in emission order.  A
 // group's residues are summed in that order and its first pole is kept (GF/chi terms do not
 // average poles, GreensFunctionPart.cpp:34-37).  A running sum that becomes negligible
-// (|R| <= tol/(terms+1)) is dropped and the group restarts, as TermList::add_term does.
+// (|R| <= tol/(size+1), size+1 the number of terms of the part's list at that moment) is
+// dropped and the group restarts, as TermList::add_term does (EraseRule).
 #include "pomx_internal.hpp"
 
 #include <algorithm>
@@ -126,12 +127,19 @@ __global__ void group_start_kernel(int64_t n, const int64_t* __restrict__ head, 
         start[gid[k]] = k;
 }
 
-// One thread per group: walk members in emission order.
+// One thread per group: walk members in emission order, which is the reference's visiting order
+// (part, outer, inner), the candidate index being the visiting position.  The erase threshold is
+// tol/(groups of the part + 1), or with R.fk set the reference's tol/(size+1) (EraseRule, each
+// part its own TermList starting at its first candidate index); ambig and EMIT as in the 2PGF
+// merge (pomx_g2.hip).
+template <bool EMIT>
 __global__ void pair_reduce_kernel(int kind, const PairPartDev* __restrict__ parts, int nparts, int64_t n,
                                    int64_t ngroups, const int64_t* __restrict__ start, const uint64_t* __restrict__ keys,
                                    const uint64_t* __restrict__ vals, const int64_t* __restrict__ part_groups,
-                                   double res, double tol, double* __restrict__ gR, double* __restrict__ gP,
-                                   int64_t* __restrict__ live, double* __restrict__ zero_pole) {
+                                   double res, double tol, EraseRule R, double* __restrict__ gR, double* __restrict__ gP,
+                                   int64_t* __restrict__ live, double* __restrict__ zero_pole, int64_t* __restrict__ evn,
+                                   uint64_t* __restrict__ ev_key_out, uint64_t* __restrict__ ev_w_out,
+                                   int* __restrict__ ambig) {
     int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if(gi >= ngroups)
         return;
@@ -139,6 +147,8 @@ __global__ void pair_reduce_kernel(int kind, const PairPartDev* __restrict__ par
     uint64_t key = keys[s];
     int part = int(key >> kPartShift);
     if((key & kBinMask) == kZeroPoleBin) {
+        if(EMIT)
+            return;
         double z = 0.0;
         for(int64_t k = s; k < e; ++k) {
             Cand c = load_cand(parts, nparts, int64_t(vals[k]));
@@ -148,28 +158,68 @@ __global__ void pair_reduce_kernel(int kind, const PairPartDev* __restrict__ par
         }
         zero_pole[part] = z;
         live[gi] = 0;
+        if(evn)
+            evn[gi] = 0;
         return;
     }
-    double thr = tol / double(part_groups[part] + 1);
+    double const thr = tol / double(part_groups[part] + 1);
+    EraseCursor const cur = R.fk ? erase_cursor(R, uint64_t(parts[part].cand_off)) : EraseCursor{0, 0, 0};
+    int64_t nevents = 0, at = EMIT ? evn[gi] : 0;
     bool alive = false;
     double sum = 0.0, pole = 0.0;
     for(int64_t k = s; k < e; ++k) {
-        Cand c = load_cand(parts, nparts, int64_t(vals[k]));
-        double R;
-        classify(kind, parts, c, res, tol, R);
+        uint64_t const q = vals[k];
+        Cand c = load_cand(parts, nparts, int64_t(q));
+        double Rv;
+        classify(kind, parts, c, res, tol, Rv);
         if(!alive) {
+            if(k > s && R.fk) { // restart of an erased group
+                if(EMIT) {
+                    ev_key_out[at] = q;
+                    ev_w_out[at++] = uint64_t(int64_t(-1));
+                }
+                ++nevents;
+            }
             alive = true;
-            sum = R;
+            sum = Rv;
             pole = c.P;
         } else {
-            sum += R;
-            if(fabs(sum) <= thr)
+            sum += Rv;
+            double const t = R.fk ? erase_threshold(R, cur, q) : thr;
+            if(ambig && fabs(sum) > thr && fabs(sum) <= tol)
+                ambig[0] = 1;
+            if(fabs(sum) <= t) {
                 alive = false;
+                if(R.fk) {
+                    if(EMIT) {
+                        ev_key_out[at] = q;
+                        ev_w_out[at++] = 1;
+                    }
+                    ++nevents;
+                }
+            }
         }
     }
-    gR[gi] = sum;
-    gP[gi] = pole;
-    live[gi] = alive ? 1 : 0;
+    if(!EMIT) {
+        gR[gi] = sum;
+        gP[gi] = pole;
+        live[gi] = alive ? 1 : 0;
+        if(evn)
+            evn[gi] = nevents;
+    }
+}
+
+// visiting position of each group's first member (zero-pole groups are no TermList terms and
+// sort last)
+__global__ void pair_first_kernel(int64_t ngroups, const int64_t* __restrict__ start, const uint64_t* __restrict__ keys,
+                                  const uint64_t* __restrict__ vals, uint64_t* __restrict__ fk,
+                                  uint64_t* __restrict__ gi_out) {
+    int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(gi < ngroups) {
+        uint64_t const key = keys[start[gi]];
+        fk[gi] = (key & kBinMask) == kZeroPoleBin ? ~uint64_t(0) : vals[start[gi]];
+        gi_out[gi] = uint64_t(gi);
+    }
 }
 
 // Group ranges per part: groups are sorted by key and the part is the top key field, so
@@ -439,8 +489,47 @@ extern "C" int pomx_pair_terms(pomx_ctx* ctx, int kind, int nparts, const pomx_p
         if((rc = pomx_fill_zero(ctx, d_zp.p, sizeof(double) * nparts)))
             return rc;
         part_group_count_kernel<<<grid_for(nparts + 1), kBlock, 0, ctx->stream>>>(nparts, ngroups, start, k1, first, pgroups);
-        pair_reduce_kernel<<<grid_for(ngroups), kBlock, 0, ctx->stream>>>(kind, d_parts, nparts, n, ngroups, start, k1, v1,
-                                                                           pgroups, res, tol, gR, gP, live, d_zp);
+        // TermList's erase threshold tol/(size+1): a pass with tol/(groups of the part + 1) is exact
+        // unless some merge is ambiguous; then the reference's rule is settled (EraseRule)
+        WsBuf<int> ambig(ctx, 1);
+        if(!ambig.p)
+            return pomx_fail(ctx, POMX_ERR_HIP, "pomx_pair_terms: out of device memory");
+        if((rc = pomx_fill_zero(ctx, ambig.p, sizeof(int))))
+            return rc;
+        EraseRule rule{tol, nullptr, 0, nullptr, nullptr, 0};
+        pair_reduce_kernel<false><<<grid_for(ngroups), kBlock, 0, ctx->stream>>>(
+            kind, d_parts, nparts, n, ngroups, start, k1, v1, pgroups, res, tol, rule, gR, gP, live, d_zp, nullptr, nullptr,
+            nullptr, ambig);
+        POMX_CHECK_LAUNCH(ctx);
+        int ha = 0;
+        POMX_HIP(ctx, hipMemcpyAsync(&ha, ambig.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if(ha) {
+            WsBuf<uint64_t> f0(ctx, ngroups), f1(ctx, ngroups), i0(ctx, ngroups), i1(ctx, ngroups);
+            if(!f0.p || !f1.p || !i0.p || !i1.p)
+                return pomx_fail(ctx, POMX_ERR_HIP, "pomx_pair_terms: out of device memory");
+            pair_first_kernel<<<grid_for(ngroups), kBlock, 0, ctx->stream>>>(ngroups, start, k1, v1, f0, i0);
+            POMX_CHECK_LAUNCH(ctx);
+            if((rc = pomx_sort_pairs(ctx, ngroups, f0, f1, i0, i1, 0, 64)))
+                return rc;
+            rule.fk = f1.p;
+            rule.nfk = ngroups;
+            EraseEvents ev;
+            auto pass = [&](EraseRule const& R, bool emit, int64_t* evn, uint64_t* ko, uint64_t* wo) -> int {
+                if(emit)
+                    pair_reduce_kernel<true><<<grid_for(ngroups), kBlock, 0, ctx->stream>>>(
+                        kind, d_parts, nparts, n, ngroups, start, k1, v1, pgroups, res, tol, R, gR, gP, live, d_zp, evn,
+                        ko, wo, nullptr);
+                else
+                    pair_reduce_kernel<false><<<grid_for(ngroups), kBlock, 0, ctx->stream>>>(
+                        kind, d_parts, nparts, n, ngroups, start, k1, v1, pgroups, res, tol, R, gR, gP, live, d_zp, evn,
+                        ko, wo, nullptr);
+                POMX_CHECK_LAUNCH(ctx);
+                return POMX_OK;
+            };
+            if((rc = erase_fixed_point(ctx, ngroups, rule, pass, ev, "pomx_pair_terms")))
+                return rc;
+        }
         if((rc = pomx_exclusive_scan(ctx, ngroups, live, lpos)))
             return rc;
         int64_t ll = 0, lp = 0;

@@ -151,10 +151,13 @@ def test_susceptibility_on_gpu(pa, oracle):
     e, o = build(pa, oracle, H, beta)
     idx = H.indices()
     u, d = idx.index(("C", 0, 1)), idx.index(("C", 0, 0))
+    oi = inject(e, oracle.ED(len(idx), H.mode_terms(), beta))
     for A, B in [([(True, u), (False, d)], [(True, d), (False, u)]), ([(True, u), (False, u)], [(True, d), (False, d)])]:
         v = e.chi(A, B, z, 1e-8, 1e-8, True)
         r = o.chi(A, B, z, 1e-8, 1e-8, True)
         assert rel_err(v["values"], r["values"]) < 1e-10
+        for tol in (1e-8, 1e-16):  # merged-term counts, TermList erasures included
+            assert e.chi(A, B, z, 1e-8, tol, True)["n_terms"] == oi.chi(A, B, z, 1e-8, tol, True)["n_terms"]
 
 
 @pytest.mark.parametrize("name", ["anderson2pgf", "cfg1"])
