@@ -10,9 +10,9 @@
 // on the device, one part per call):
 //  1. contiguous copies: Bt[i1][i3] = B(i3, i1) (LDS-tiled product of the two operator views) and
 //     Fbc[i2][i3] = Fb(i2, i3), so the lanes of a wave running over i3 read coalesced rows;
-//  2. rank tables: dense ranks of the pole bins round(P/res) of E1-E2 (r12), E2-E3 (r23) and
-//     E1-E3 (r13), so a term's TermList identity becomes one 64-bit key
-//     kind (2 bits) | rank (31 bits) | rank (31 bits);
+//  2. key fields: the pole bins round(P/res) of E1-E2, E2-E3 and E1-E3 themselves when they fit
+//     31 bits (every |P|/res < 2^30), else their dense ranks (rank tables r12, r23, r13), so a
+//     term's TermList identity becomes one 64-bit key kind (2 bits) | field (31) | field (31);
 //  3. one wave per (i1, i2) row with the lanes over i3: count the terms of every row, prefix-sum,
 //     emit (key, packed indices) in the reference's (i1, i2, i3) visiting order, a triple's own
 //     terms in the order addMultiterm adds them; the packed value (i1, i2, i3, slot) is
@@ -32,6 +32,7 @@
 #include "pomx_internal.hpp"
 
 #include <algorithm>
+#include <cmath>
 
 namespace {
 
@@ -122,11 +123,16 @@ __global__ void chi3_widen_kernel(int64_t n, const int* __restrict__ in, int64_t
 }
 
 constexpr uint64_t kRankMask = (uint64_t(1) << 31) - 1;
+constexpr int64_t kDirectOff = int64_t(1) << 30;
 
+// the pole bin round(P/res) itself as a 31-bit key field (when every |P|/res < 2^30)
+__device__ inline uint64_t chi3_bin(double P, double res) { return uint64_t(int64_t(round(P / res)) + kDirectOff); }
+
+// direct: key fields are the bins themselves (r12, r23, r13 unused); otherwise their dense ranks
 __global__ void __launch_bounds__(256) chi3_emit_kernel(Chi3Dev D, int64_t nrows, int64_t ntot, const int64_t* __restrict__ off,
-                                                        const uint32_t* __restrict__ r12, const uint32_t* __restrict__ r23,
-                                                        const uint32_t* __restrict__ r13, uint64_t* __restrict__ keys,
-                                                        uint64_t* __restrict__ vals) {
+                                                        int direct, const uint32_t* __restrict__ r12,
+                                                        const uint32_t* __restrict__ r23, const uint32_t* __restrict__ r13,
+                                                        uint64_t* __restrict__ keys, uint64_t* __restrict__ vals) {
     int const lane = threadIdx.x & 63;
     for(int64_t row = blockIdx.x * 4 + (threadIdx.x >> 6); row < nrows; row += int64_t(gridDim.x) * 4) {
         int const i1 = int(row / D.n2), i2 = int(row % D.n2);
@@ -135,7 +141,7 @@ __global__ void __launch_bounds__(256) chi3_emit_kernel(Chi3Dev D, int64_t nrows
             continue; // wave-uniform
         int64_t base = off[row];
         int64_t const end = (row + 1 < nrows) ? off[row + 1] : ntot; // never write past this row's slots
-        uint64_t const a12 = r12[row];
+        uint64_t const a12 = direct ? chi3_bin(D.E1[i1] - D.E2[i2], D.res) : uint64_t(r12[row]);
         for(int c0 = 0; c0 < D.n3; c0 += 64) {
             int const i3 = c0 + lane;
             unsigned f = 0;
@@ -152,7 +158,8 @@ __global__ void __launch_bounds__(256) chi3_emit_kernel(Chi3Dev D, int64_t nrows
             int const total = __shfl(incl, 63);
             if(f && base + incl <= end) {
                 int64_t pos = base + incl - c;
-                uint64_t const a23 = r23[int64_t(i2) * D.n3 + i3], a13 = r13[int64_t(i1) * D.n3 + i3];
+                uint64_t const a23 = direct ? chi3_bin(D.E2[i2] - D.E3[i3], D.res) : uint64_t(r23[int64_t(i2) * D.n3 + i3]);
+                uint64_t const a13 = direct ? chi3_bin(D.E1[i1] - D.E3[i3], D.res) : uint64_t(r13[int64_t(i1) * D.n3 + i3]);
                 uint64_t const v = (uint64_t(i1) << 40) | (uint64_t(i2) << 24) | (uint64_t(i3) << 8);
                 uint64_t const kff = ((a12 & kRankMask) << 31) | (a23 & kRankMask);
                 if(f & 1u) {
@@ -174,6 +181,38 @@ __global__ void __launch_bounds__(256) chi3_emit_kernel(Chi3Dev D, int64_t nrows
             }
             base += total;
         }
+    }
+}
+
+// out = (min E1, max E1, min E2, max E2, min E3, max E3), one block of 256 threads
+__global__ void __launch_bounds__(256) chi3_range_kernel(int n1, int n2, int n3, const double* __restrict__ E1,
+                                                         const double* __restrict__ E2, const double* __restrict__ E3,
+                                                         double* __restrict__ out) {
+    __shared__ double lo[3][256], hi[3][256];
+    int const t = threadIdx.x;
+    const double* E[3] = {E1, E2, E3};
+    int const n[3] = {n1, n2, n3};
+    for(int v = 0; v < 3; ++v) {
+        double a = E[v][0], b = E[v][0];
+        for(int k = t; k < n[v]; k += 256) {
+            a = fmin(a, E[v][k]);
+            b = fmax(b, E[v][k]);
+        }
+        lo[v][t] = a;
+        hi[v][t] = b;
+    }
+    __syncthreads();
+    for(int w = 128; w > 0; w >>= 1) {
+        if(t < w)
+            for(int v = 0; v < 3; ++v) {
+                lo[v][t] = fmin(lo[v][t], lo[v][t + w]);
+                hi[v][t] = fmax(hi[v][t], hi[v][t + w]);
+            }
+        __syncthreads();
+    }
+    if(t < 3) {
+        out[2 * t] = lo[t][0];
+        out[2 * t + 1] = hi[t][0];
     }
 }
 
@@ -424,6 +463,85 @@ __global__ void __launch_bounds__(256) chi3_eval_kernel(int64_t nterms, int64_t 
     }
 }
 
+// Every z purely imaginary (z1 = i y1, z2 = i y2; the Matsubara grids): with u = y2 (KIND 0) or
+// u = y1 - xi y2 (KIND 1), and A, B the term's two poles (P0, P1 or P1, P12), the denominator is
+// D = (i y1 - A)(i u - B) = (A B - y1 u) - i (A u + B y1), so a pair costs D, |D|^2, the two
+// accumulating FMAs and a quarter of a reciprocal shared by four pairs
+// (1/q_h = (product of the other three) / (q0 q1 q2 q3)); |D|^2 stays within ~1e-20..1e60 for
+// Matsubara frequencies, so the product of four is far inside the double range.
+constexpr int kPI = 8; // pairs per thread
+template <int KIND>
+__global__ void __launch_bounds__(256) chi3_eval_imag_kernel(int64_t nterms, int64_t slice_len,
+                                                             const double* __restrict__ T, int nfreq,
+                                                             const double* __restrict__ fz, double xi,
+                                                             double* __restrict__ partial) {
+    __shared__ double sC[kStage], sA[kStage], sB[kStage], sAB[kStage];
+    int64_t const t0 = int64_t(blockIdx.y) * slice_len;
+    int64_t const t1 = min(nterms, t0 + slice_len);
+    int const f0 = blockIdx.x * (256 * kPI) + threadIdx.x;
+    double y1[kPI], u[kPI], yu[kPI], ar[kPI], ai[kPI];
+#pragma unroll
+    for(int h = 0; h < kPI; ++h) {
+        int const f = min(f0 + h * 256, nfreq - 1);
+        y1[h] = fz[4 * f + 1];
+        u[h] = KIND == 0 ? fz[4 * f + 3] : y1[h] - xi * fz[4 * f + 3];
+        yu[h] = y1[h] * u[h];
+        ar[h] = 0.0;
+        ai[h] = 0.0;
+    }
+    for(int64_t c0 = t0; c0 < t1; c0 += kStage) {
+        int const cnt = int(min<int64_t>(kStage, t1 - c0));
+        __syncthreads();
+        if(threadIdx.x < cnt) {
+            double const A = T[nterms + c0 + threadIdx.x], B = T[2 * nterms + c0 + threadIdx.x];
+            sC[threadIdx.x] = T[c0 + threadIdx.x];
+            sA[threadIdx.x] = A;
+            sB[threadIdx.x] = B;
+            sAB[threadIdx.x] = A * B;
+        }
+        __syncthreads();
+#pragma unroll 2
+        for(int k = 0; k < cnt; ++k) {
+            double const C = sC[k], A = sA[k], B = sB[k], AB = sAB[k];
+#pragma unroll
+            for(int h0 = 0; h0 < kPI; h0 += 4) {
+                double Dr[4], Di[4], q[4];
+#pragma unroll
+                for(int h = 0; h < 4; ++h) {
+                    Dr[h] = AB - yu[h0 + h];
+                    Di[h] = fma(A, u[h0 + h], B * y1[h0 + h]); // -Im D
+                    q[h] = fma(Dr[h], Dr[h], Di[h] * Di[h]);
+                }
+                double const q01 = q[0] * q[1], q23 = q[2] * q[3];
+                double const r = C * rcp_refined(q01 * q23);
+                double const r01 = q23 * r, r23 = q01 * r;
+                double const sc[4] = {q[1] * r01, q[0] * r01, q[3] * r23, q[2] * r23};
+#pragma unroll
+                for(int h = 0; h < 4; ++h) { // C conj(D) / |D|^2 with conj(D) = Dr + i Di
+                    ar[h0 + h] = fma(Dr[h], sc[h], ar[h0 + h]);
+                    ai[h0 + h] = fma(Di[h], sc[h], ai[h0 + h]);
+                }
+            }
+        }
+    }
+    double* p = partial + size_t(blockIdx.y) * nfreq * 2;
+#pragma unroll
+    for(int h = 0; h < kPI; ++h) {
+        int const f = f0 + h * 256;
+        if(f < nfreq) {
+            p[2 * f] += ar[h];
+            p[2 * f + 1] += ai[h];
+        }
+    }
+}
+
+// flag[0] = 1 if some pair has a non-zero real part
+__global__ void chi3_any_real_kernel(int nfreq, const double* __restrict__ fz, int* __restrict__ flag) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if(f < nfreq && (fz[4 * f] != 0.0 || fz[4 * f + 2] != 0.0))
+        flag[0] = 1;
+}
+
 __global__ void chi3_eval_reduce_kernel(int nfreq, int nslices, const double* __restrict__ partial,
                                         double* __restrict__ acc) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -477,7 +595,17 @@ int chi3_eval(pomx_ctx* ctx, const double* data, int64_t const count[3], int xi,
     if(nfreq <= 0)
         return POMX_OK;
     if(c0 + c1 > 0) {
-        int const ftiles = (nfreq + 256 * kPF - 1) / (256 * kPF);
+        WsBuf<int> anyre(ctx, 1);
+        CHI3_WS(ctx, anyre);
+        if((rc = pomx_fill_zero(ctx, anyre.p, sizeof(int))))
+            return rc;
+        chi3_any_real_kernel<<<grid_for(nfreq), kBlock, 0, ctx->stream>>>(nfreq, d_freq, anyre);
+        POMX_CHECK_LAUNCH(ctx);
+        int has_real = 1;
+        POMX_HIP(ctx, hipMemcpyAsync(&has_real, anyre.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        int const per = has_real ? kPF : kPI; // pairs per thread
+        int const ftiles = (nfreq + 256 * per - 1) / (256 * per);
         int64_t const want = std::max<int64_t>(1, 2048 / ftiles);
         int64_t const big = std::max(c0, c1);
         int64_t slice = std::max<int64_t>(kStage, (big + want - 1) / want);
@@ -487,12 +615,21 @@ int chi3_eval(pomx_ctx* ctx, const double* data, int64_t const count[3], int xi,
         CHI3_WS(ctx, partial);
         if((rc = pomx_fill_zero(ctx, partial.p, sizeof(double) * size_t(nslices) * nfreq * 2)))
             return rc;
-        if(c0 > 0)
-            chi3_eval_kernel<0><<<dim3(ftiles, unsigned((c0 + slice - 1) / slice)), 256, 0, ctx->stream>>>(
-                c0, slice, data, nfreq, d_freq, double(xi), partial);
-        if(c1 > 0)
-            chi3_eval_kernel<1><<<dim3(ftiles, unsigned((c1 + slice - 1) / slice)), 256, 0, ctx->stream>>>(
-                c1, slice, data + 3 * c0, nfreq, d_freq, double(xi), partial);
+        dim3 const g0(ftiles, unsigned((c0 + slice - 1) / slice)), g1(ftiles, unsigned((c1 + slice - 1) / slice));
+        if(c0 > 0) {
+            if(has_real)
+                chi3_eval_kernel<0><<<g0, 256, 0, ctx->stream>>>(c0, slice, data, nfreq, d_freq, double(xi), partial);
+            else
+                chi3_eval_imag_kernel<0><<<g0, 256, 0, ctx->stream>>>(c0, slice, data, nfreq, d_freq, double(xi), partial);
+        }
+        if(c1 > 0) {
+            if(has_real)
+                chi3_eval_kernel<1><<<g1, 256, 0, ctx->stream>>>(c1, slice, data + 3 * c0, nfreq, d_freq, double(xi),
+                                                                 partial);
+            else
+                chi3_eval_imag_kernel<1><<<g1, 256, 0, ctx->stream>>>(c1, slice, data + 3 * c0, nfreq, d_freq,
+                                                                      double(xi), partial);
+        }
         chi3_eval_reduce_kernel<<<grid_for(nfreq), kBlock, 0, ctx->stream>>>(nfreq, nslices, partial, d_acc);
         POMX_CHECK_LAUNCH(ctx);
     }
@@ -550,27 +687,41 @@ int chi3_part_run(pomx_ctx* ctx, pomx_chi3_part const& P, double beta, double re
         P.n1, P.n3, P.n4, P.B1, P.B2, Bt);
     chi3_copy_kernel<<<grid_for(int64_t(P.n2) * P.n3), kBlock, 0, ctx->stream>>>(P.n2, P.n3, P.Fb, Fbc);
     POMX_CHECK_LAUNCH(ctx);
-    // 2. rank tables (bins of E_y - E_x: ranks of the negated poles, which group alike)
-    WsBuf<int> err(ctx, 1);
-    CHI3_WS(ctx, err);
-    if((rc = pomx_fill_zero(ctx, err.p, sizeof(int))))
-        return rc;
-    WsBuf<uint32_t> r12, r23, r13;
-    int64_t nr12 = 0, nr23 = 0, nr13 = 0;
-    if((rc = rank_table(ctx, P.n1, P.n2, P.E1, P.E2, res, r12, nullptr, &nr12, err)) ||
-       (rc = rank_table(ctx, P.n2, P.n3, P.E2, P.E3, res, r23, nullptr, &nr23, err)) ||
-       (rc = rank_table(ctx, P.n1, P.n3, P.E1, P.E3, res, r13, nullptr, &nr13, err)))
-        return rc;
-    CHI3_WS(ctx, r12);
-    CHI3_WS(ctx, r23);
-    CHI3_WS(ctx, r13);
-    int herr = 0;
-    POMX_HIP(ctx, hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    // 2. key fields: the pole bins themselves when every |E_a - E_b|/res < 2^30 (one range
+    //    reduction), else dense ranks of the bins (rank tables of E_y - E_x: ranks of the negated
+    //    poles, which group alike)
+    WsBuf<double> rng(ctx, 6);
+    CHI3_WS(ctx, rng);
+    chi3_range_kernel<<<1, 256, 0, ctx->stream>>>(P.n1, P.n2, P.n3, P.E1, P.E2, P.E3, rng);
+    POMX_CHECK_LAUNCH(ctx);
+    double hr[6];
+    POMX_HIP(ctx, hipMemcpyAsync(hr, rng.p, sizeof(hr), hipMemcpyDeviceToHost, ctx->stream));
     POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if(herr)
-        return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_chi3_part_compute: |pole/pole_res| out of range");
-    if(nr12 > int64_t(kRankMask) || nr23 > int64_t(kRankMask) || nr13 > int64_t(kRankMask))
-        return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_chi3_part_compute: too many distinct pole bins for the 64-bit key");
+    auto span = [&](int a, int b) { return std::max(std::fabs(hr[2 * a + 1] - hr[2 * b]), std::fabs(hr[2 * b + 1] - hr[2 * a])); };
+    double const widest = std::max({span(0, 1), span(1, 2), span(0, 2)});
+    bool const direct = widest / res < 1.0e9; // |bin| <= 1e9 + 1 < 2^30
+    WsBuf<uint32_t> r12, r23, r13;
+    if(!direct) {
+        WsBuf<int> err(ctx, 1);
+        CHI3_WS(ctx, err);
+        if((rc = pomx_fill_zero(ctx, err.p, sizeof(int))))
+            return rc;
+        int64_t nr12 = 0, nr23 = 0, nr13 = 0;
+        if((rc = rank_table(ctx, P.n1, P.n2, P.E1, P.E2, res, r12, nullptr, &nr12, err)) ||
+           (rc = rank_table(ctx, P.n2, P.n3, P.E2, P.E3, res, r23, nullptr, &nr23, err)) ||
+           (rc = rank_table(ctx, P.n1, P.n3, P.E1, P.E3, res, r13, nullptr, &nr13, err)))
+            return rc;
+        CHI3_WS(ctx, r12);
+        CHI3_WS(ctx, r23);
+        CHI3_WS(ctx, r13);
+        int herr = 0;
+        POMX_HIP(ctx, hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if(herr)
+            return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_chi3_part_compute: |pole/pole_res| out of range");
+        if(nr12 > int64_t(kRankMask) || nr23 > int64_t(kRankMask) || nr13 > int64_t(kRankMask))
+            return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_chi3_part_compute: too many distinct pole bins for the 64-bit key");
+    }
     Chi3Dev D{P.n1, P.n2, P.n3, P.Fa, Fbc, Bt, P.E1, P.E2, P.E3, P.w1, P.w2, P.w3,
               ((P.channel == 1) == (P.swapped != 0)) ? 1.0 : -1.0, beta, tol, res, P.channel == 0, P.swapped != 0};
     // 3. count, scan, emit
@@ -602,7 +753,7 @@ int chi3_part_run(pomx_ctx* ctx, pomx_chi3_part const& P, double beta, double re
         CHI3_WS(ctx, k1);
         CHI3_WS(ctx, v0);
         CHI3_WS(ctx, v1);
-        chi3_emit_kernel<<<rblocks, 256, 0, ctx->stream>>>(D, nrows, n, off, r12, r23, r13, k0, v0);
+        chi3_emit_kernel<<<rblocks, 256, 0, ctx->stream>>>(D, nrows, n, off, direct ? 1 : 0, r12, r23, r13, k0, v0);
         POMX_CHECK_LAUNCH(ctx);
         if((rc = pomx_sort_pairs(ctx, n, k0, k1, v0, v1, 0, 64)))
             return rc;
