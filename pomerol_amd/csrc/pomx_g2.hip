@@ -20,6 +20,7 @@
 #include "pomx_internal.hpp"
 
 #include <algorithm>
+#include <cmath>
 
 namespace {
 
@@ -934,6 +935,211 @@ __global__ void __launch_bounds__(256) g2_eval_s_kernel(int64_t nterms, int64_t 
     }
 }
 
+// ---- evaluation on separable grids (matrix cores) --------------------------------------------
+// Grids of the reference's programs: triplet f = (b NR + r) NC + c with z1 depending on (b, r),
+// z2 on c and z3 on r only (quantum_model.cpp:220-231: (W + w3, w2, w3), W outermost), all purely
+// imaginary.  Then F1 = z1 is a (b, r) quantity, F2 = z2 a c quantity and F3 = -z3 an r quantity,
+// and every non-resonant term factorizes per bosonic index b into a row factor (its r-dependent
+// denominators) times a column factor (its c-dependent ones, times C):
+//   kind 0 (D1 D2 D3): the slot holding F2 is the column factor, the other two the row factor;
+//   kind 1 (D1 D2' D3, D2' = z1 + z2 + z3 - P1 - P2 - P3, Im(z1 + z2 + z3) = Y_b + Im F2(c) with
+//          Y_b = Im(F1 + F3) the same for every r): D2' and the slot holding F2 (if any) form the
+//          column factor.
+// So V_b[r][c] = sum_t Row_t(b, r) Col_t(b, c): a complex GEMM over the terms, on the FP64 matrix
+// cores (3 real v_mfma_f64_16x16x4f64 per 16 x 16 tile and 4 terms, the complex product formed
+// from three real ones), 6 FLOP per term x triplet against ~16 on the vector pipe.  Each wave owns one K-slice of the terms and the whole
+// 32 x 32 (r, c) tile of one b; its lane (i, k) = (lane & 15, lane >> 4) forms the row and
+// column elements of term k of the step for rows/columns i and i + 16 (one shared reciprocal per
+// element).  Partial tiles per wave are summed in a fixed order afterwards.
+constexpr int kMT = 32; // tile rows / columns (NR, NC <= 32)
+
+// Row / column factors of a non-resonant kind under a permutation (slot s holds F[perm[s]]):
+// pole 0, 1, 2 = P1, P2, P3, 3 = P1 + P2 + P3; y source 0 = F1(b, r), 1 = F2(c), 2 = F3(r),
+// 3 = Y_b + F2(c).
+struct SepPlan {
+    int nrow, ncol;
+    int rp[2], rs[2], cp[2], cs[2];
+};
+__host__ __device__ constexpr SepPlan sep_plan_of(int kind, int perm) {
+    int const p3[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    SepPlan pl{0, 0, {0, 0}, {2, 2}, {0, 0}, {1, 1}};
+    if(kind == 0) {
+        for(int sl = 0; sl < 3; ++sl) {
+            if(p3[perm][sl] == 1) {
+                pl.cp[0] = sl;
+                pl.cs[0] = 1;
+            } else {
+                pl.rp[pl.nrow] = sl;
+                pl.rs[pl.nrow] = p3[perm][sl];
+                ++pl.nrow;
+            }
+        }
+        pl.ncol = 1;
+    } else {
+        pl.cp[0] = 3; // z1 + z2 + z3 - P1 - P2 - P3
+        pl.cs[0] = 3;
+        if(p3[perm][1] == 1) { // F2 inside the sum only
+            pl.rp[0] = 0;
+            pl.rs[0] = p3[perm][0];
+            pl.rp[1] = 2;
+            pl.rs[1] = p3[perm][2];
+            pl.nrow = 2;
+            pl.ncol = 1;
+        } else {
+            int const sc = p3[perm][0] == 1 ? 0 : 2, sr = 2 - sc;
+            pl.cp[1] = sc;
+            pl.cs[1] = 1;
+            pl.rp[0] = sr;
+            pl.rs[0] = p3[perm][sr];
+            pl.nrow = 1;
+            pl.ncol = 2;
+        }
+    }
+    return pl;
+}
+
+template <int W>
+__device__ inline double sep_pole(double P1, double P2, double P3) {
+    if constexpr(W == 0)
+        return P1;
+    else if constexpr(W == 1)
+        return P2;
+    else if constexpr(W == 2)
+        return P3;
+    else
+        return (P1 + P2) + P3;
+}
+
+// element = scale / prod_f (i y_f - P_f) = scale prod_f (-P_f - i y_f) / prod_f (P_f^2 + y_f^2);
+// y2 = y^2 and, for two factors, yy = y_a y_b are the lane's precomputed constants
+template <int NF>
+__device__ inline void sep_element(double Pa, double Pb, double ya, double yb, double ya2, double yb2, double yy,
+                                   double PaPb, double scale, double& er, double& ei) {
+    if constexpr(NF == 1) {
+        double const r = scale * rcp_nr(fma(Pa, Pa, ya2));
+        er = -Pa * r;
+        ei = -ya * r;
+    } else {
+        double const r = scale * rcp_nr(fma(Pa, Pa, ya2) * fma(Pb, Pb, yb2));
+        er = (PaPb - yy) * r;
+        ei = fma(Pa, yb, Pb * ya) * r;
+    }
+}
+
+template <int KIND, int PERM>
+__global__ void __launch_bounds__(256) g2_eval_mfma_kernel(int64_t nterms, int64_t slice_len, const double* __restrict__ T,
+                                                           int64_t cap, int NR, int NC, const double* __restrict__ yF1,
+                                                           const double* __restrict__ yF2, const double* __restrict__ yF3,
+                                                           const double* __restrict__ yB, double* __restrict__ partial) {
+    constexpr SepPlan PL = sep_plan_of(KIND, PERM);
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int const b = blockIdx.y;
+    int64_t const slice = int64_t(blockIdx.x) * 4 + wave;
+    int64_t const t0 = slice * slice_len, t1 = min(nterms, t0 + slice_len);
+    int const i = lane & 15, k = lane >> 4;
+    // this lane's factor arguments for rows / columns i and i + 16
+    double ya[2], yb[2], ya2[2], yb2[2], yy[2], xa[2], xb[2], xa2[2], xb2[2], xx[2];
+    double rs[2], cs[2]; // 1 or 0 (rows / columns beyond NR / NC)
+#pragma unroll
+    for(int m = 0; m < 2; ++m) {
+        int const r = min(i + 16 * m, NR - 1), c = min(i + 16 * m, NC - 1);
+        rs[m] = i + 16 * m < NR ? 1.0 : 0.0;
+        cs[m] = i + 16 * m < NC ? 1.0 : 0.0;
+        ya[m] = PL.rs[0] == 0 ? yF1[b * NR + r] : yF3[r];
+        yb[m] = PL.rs[1] == 0 ? yF1[b * NR + r] : yF3[r];
+        xa[m] = PL.cs[0] == 1 ? yF2[c] : yB[b] + yF2[c];
+        xb[m] = PL.cs[1] == 1 ? yF2[c] : yB[b] + yF2[c];
+        ya2[m] = ya[m] * ya[m];
+        yb2[m] = yb[m] * yb[m];
+        yy[m] = ya[m] * yb[m];
+        xa2[m] = xa[m] * xa[m];
+        xb2[m] = xb[m] * xb[m];
+        xx[m] = xa[m] * xb[m];
+    }
+    typedef double dbl4 __attribute__((ext_vector_type(4)));
+    // complex products from three real ones: with A1 = ar + ai, B2 = bi - br, B3 = br + bi,
+    // Re = sum A1 br - sum ai B3, Im = sum A1 br + sum ar B2
+    dbl4 acc[2][2][3];
+#pragma unroll
+    for(int m = 0; m < 2; ++m)
+#pragma unroll
+        for(int n = 0; n < 2; ++n)
+#pragma unroll
+            for(int x = 0; x < 3; ++x)
+                acc[m][n][x] = dbl4{0.0, 0.0, 0.0, 0.0};
+    // term k of each step of 4; the next step's term is loaded while this one is used
+    auto load = [&](int64_t s, double& C, double& P1, double& P2, double& P3) {
+        int64_t const t = s + k;
+        bool const tv = t < t1;
+        int64_t const tt = tv ? t : t0;
+        C = tv ? T[tt] : 0.0;
+        P1 = T[cap + tt];
+        P2 = T[2 * cap + tt];
+        P3 = T[3 * cap + tt];
+    };
+    double C = 0, P1 = 1, P2 = 1, P3 = 1;
+    if(t0 < t1)
+        load(t0, C, P1, P2, P3);
+    for(int64_t s = t0; s < t1; s += 4) {
+        double const Cc = C, Q1 = P1, Q2 = P2, Q3 = P3;
+        if(s + 4 < t1)
+            load(s + 4, C, P1, P2, P3);
+        double const Ra = sep_pole<PL.rp[0]>(Q1, Q2, Q3), Rb = sep_pole<PL.rp[1]>(Q1, Q2, Q3);
+        double const Ka = sep_pole<PL.cp[0]>(Q1, Q2, Q3), Kb = sep_pole<PL.cp[1]>(Q1, Q2, Q3);
+        double const RaRb = Ra * Rb, KaKb = Ka * Kb;
+        double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+        for(int m = 0; m < 2; ++m) {
+            sep_element<PL.nrow>(Ra, Rb, ya[m], yb[m], ya2[m], yb2[m], yy[m], RaRb, rs[m], ar[m], ai[m]);
+            sep_element<PL.ncol>(Ka, Kb, xa[m], xb[m], xa2[m], xb2[m], xx[m], KaKb, cs[m] * Cc, br[m], bi[m]);
+        }
+        double a1[2], b2[2], b3[2];
+#pragma unroll
+        for(int m = 0; m < 2; ++m) {
+            a1[m] = ar[m] + ai[m];
+            b2[m] = bi[m] - br[m];
+            b3[m] = br[m] + bi[m];
+        }
+#pragma unroll
+        for(int m = 0; m < 2; ++m)
+#pragma unroll
+            for(int n = 0; n < 2; ++n) {
+                acc[m][n][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[m], br[n], acc[m][n][0], 0, 0, 0);
+                acc[m][n][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[m], b3[n], acc[m][n][1], 0, 0, 0);
+                acc[m][n][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[m], b2[n], acc[m][n][2], 0, 0, 0);
+            }
+    }
+    // D[i][j] of a 16 x 16 tile sits in lane j + 16 (i % 4), register i / 4
+    double* out = partial + (slice * gridDim.y + b) * (kMT * kMT * 2);
+#pragma unroll
+    for(int m = 0; m < 2; ++m)
+#pragma unroll
+        for(int n = 0; n < 2; ++n)
+#pragma unroll
+            for(int x = 0; x < 4; ++x) {
+                int const row = 16 * m + (lane >> 4) + 4 * x, col = 16 * n + (lane & 15);
+                out[(row * kMT + col) * 2] = acc[m][n][0][x] - acc[m][n][1][x];
+                out[(row * kMT + col) * 2 + 1] = acc[m][n][0][x] + acc[m][n][2][x];
+            }
+}
+
+// acc[(b NR + r) NC + c] += sum over slices of the (b, r, c) entries, in slice order
+__global__ void g2_mfma_reduce_kernel(int nslices, int NB, int NR, int NC, const double* __restrict__ partial,
+                                      double* __restrict__ acc) {
+    int const f = blockIdx.x * blockDim.x + threadIdx.x;
+    if(f >= NB * NR * NC)
+        return;
+    int const c = f % NC, r = (f / NC) % NR, b = f / (NC * NR);
+    double sr = 0.0, si = 0.0;
+    for(int sl = 0; sl < nslices; ++sl) {
+        const double* p = partial + ((int64_t(sl) * NB + b) * kMT * kMT + r * kMT + c) * 2;
+        sr += p[0];
+        si += p[1];
+    }
+    acc[2 * f] += sr;
+    acc[2 * f + 1] += si;
+}
+
 // 1 in flag[0] if some triplet f does not share z2 and z3 with triplet f mod stride
 __global__ void stride_mismatch_kernel(int nfreq, int stride, const double* __restrict__ fz, int* __restrict__ flag) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1150,6 +1356,111 @@ void launch_kind_s(pomx_ctx* ctx, int k, int slot, dim3 grid, TermSpan const& s,
 #undef POMX_G2S
 }
 
+// Separable grid structure of a triplet list (see g2_eval_mfma_kernel), found on the host.
+struct SepGrid {
+    bool ok = false, sum_ok = false;
+    int NB = 0, NR = 0, NC = 0;
+    std::vector<double> yF1, yF2, yF3, yB; // Im F1(b, r), Im F2(c), Im F3(r), Y_b = Im(F1 + F3)
+};
+
+SepGrid detect_sep_grid(std::vector<double> const& h, int nfreq) {
+    SepGrid g;
+    auto same = [&](int f, int e, int slot) {
+        return h[6 * f + 2 * slot] == h[6 * e + 2 * slot] && h[6 * f + 2 * slot + 1] == h[6 * e + 2 * slot + 1];
+    };
+    int NC = 0;
+    for(int nc = 1; nc <= std::min(nfreq, kMT) && !NC; ++nc) {
+        if(nfreq % nc)
+            continue;
+        bool okc = true;
+        for(int f = 0; f < nfreq && okc; ++f)
+            okc = same(f, f - f % nc, 0) && same(f, f - f % nc, 2) && same(f, f % nc, 1);
+        if(okc)
+            NC = nc;
+    }
+    if(!NC)
+        return g;
+    int const R = nfreq / NC;
+    int NR = 0;
+    for(int nr = 1; nr <= std::min(R, kMT) && !NR; ++nr) {
+        if(R % nr)
+            continue;
+        bool okr = true;
+        for(int run = 0; run < R && okr; ++run)
+            okr = same(run * NC, (run % nr) * NC, 2);
+        if(okr)
+            NR = nr;
+    }
+    if(!NR)
+        return g;
+    g.NB = R / NR;
+    g.NR = NR;
+    g.NC = NC;
+    g.yF1.resize(std::size_t(g.NB) * NR);
+    g.yF2.resize(NC);
+    g.yF3.resize(NR);
+    g.yB.resize(g.NB);
+    for(int c = 0; c < NC; ++c)
+        g.yF2[c] = h[6 * c + 3];
+    for(int r = 0; r < NR; ++r)
+        g.yF3[r] = -h[6 * (r * NC) + 5];
+    for(int b = 0; b < g.NB; ++b)
+        for(int r = 0; r < NR; ++r)
+            g.yF1[std::size_t(b) * NR + r] = h[6 * ((b * NR + r) * NC) + 1];
+    g.ok = true;
+    // Im(F1 + F3) must not depend on r for the z1 + z2 + z3 factor to be a column factor
+    g.sum_ok = true;
+    for(int b = 0; b < g.NB; ++b) {
+        g.yB[b] = g.yF1[std::size_t(b) * NR] + g.yF3[0];
+        for(int r = 0; r < NR && g.sum_ok; ++r) {
+            double const v = g.yF1[std::size_t(b) * NR + r] + g.yF3[r];
+            double const scale = std::fabs(g.yF1[std::size_t(b) * NR + r]) + std::fabs(g.yF3[r]);
+            g.sum_ok = std::fabs(v - g.yB[b]) <= 8.0 * 2.220446049250313e-16 * scale;
+        }
+    }
+    return g;
+}
+
+// One non-resonant kind on a separable grid: matrix-core partial tiles, then their sum into acc.
+template <int KIND, int PERM>
+void launch_sep_k(pomx_ctx* ctx, dim3 grid, TermSpan const& s, int64_t slice, const double* T, SepGrid const& g,
+                  const double* yF1, const double* yF2, const double* yF3, const double* yB, double* partial) {
+    g2_eval_mfma_kernel<KIND, PERM><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, g.NR, g.NC, yF1, yF2, yF3,
+                                                                   yB, partial);
+}
+
+int launch_sep(pomx_ctx* ctx, TermSpan const& s, int kind, int perm, SepGrid const& g, const double* d_y,
+               double* d_acc) {
+    int64_t const want = std::max<int64_t>(4, 4096 / g.NB);
+    int64_t slice = std::max<int64_t>(4, (s.count + want - 1) / want);
+    slice = (slice + 3) / 4 * 4;
+    int64_t nsl = (s.count + slice - 1) / slice;
+    nsl = (nsl + 3) / 4 * 4;
+    WsBuf<double> partial(ctx, size_t(nsl) * g.NB * kMT * kMT * 2);
+    if(!partial.p)
+        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+    const double* yF1 = d_y;
+    const double* yF2 = yF1 + std::size_t(g.NB) * g.NR;
+    const double* yF3 = yF2 + g.NC;
+    const double* yB = yF3 + g.NR;
+    const double* T = s.base + s.begin;
+    dim3 const grid(unsigned(nsl / 4), unsigned(g.NB));
+#define POMX_SEP_CASE(K, P)                                                                        \
+    case K * 6 + P: launch_sep_k<K, P>(ctx, grid, s, slice, T, g, yF1, yF2, yF3, yB, partial); break
+    switch(kind * 6 + perm) {
+        POMX_SEP_CASE(0, 0); POMX_SEP_CASE(0, 1); POMX_SEP_CASE(0, 2); POMX_SEP_CASE(0, 3); POMX_SEP_CASE(0, 4);
+        POMX_SEP_CASE(0, 5); POMX_SEP_CASE(1, 0); POMX_SEP_CASE(1, 1); POMX_SEP_CASE(1, 2); POMX_SEP_CASE(1, 3);
+        POMX_SEP_CASE(1, 4); POMX_SEP_CASE(1, 5);
+    default: return pomx_fail(ctx, POMX_ERR_ARG, "pomx_g2_part_compute: no separable plan for this kind");
+    }
+#undef POMX_SEP_CASE
+    POMX_CHECK_LAUNCH(ctx);
+    int const nout = g.NB * g.NR * g.NC;
+    g2_mfma_reduce_kernel<<<grid_for(nout), kBlock, 0, ctx->stream>>>(int(nsl), g.NB, g.NR, g.NC, partial, d_acc);
+    POMX_CHECK_LAUNCH(ctx);
+    return POMX_OK;
+}
+
 int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res, int nfreq, const double* d_freq,
                    double* d_acc) {
     int64_t total = 0;
@@ -1172,6 +1483,25 @@ int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res,
     POMX_HIP(ctx, hipMemcpyAsync(hflag, flag.p, 3 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     int const has_real = hflag[0];
+    // separable grids take the matrix cores for the non-resonant kinds
+    SepGrid sep;
+    WsBuf<double> d_y;
+    if(!has_real) {
+        std::vector<double> h(std::size_t(6) * nfreq);
+        POMX_HIP(ctx, hipMemcpyAsync(h.data(), d_freq, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+        POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        sep = detect_sep_grid(h, nfreq);
+        if(sep.ok) {
+            std::vector<double> y(sep.yF1);
+            y.insert(y.end(), sep.yF2.begin(), sep.yF2.end());
+            y.insert(y.end(), sep.yF3.begin(), sep.yF3.end());
+            y.insert(y.end(), sep.yB.begin(), sep.yB.end());
+            d_y = WsBuf<double>(ctx, y.size());
+            if(!d_y.p)
+                return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+            POMX_HIP(ctx, hipMemcpyAsync(d_y.p, y.data(), sizeof(double) * y.size(), hipMemcpyHostToDevice, ctx->stream));
+        }
+    }
     // purely imaginary triplets whose aligned groups share z1 and z3 (F2 = z2 varies inside a
     // group), or whose stride-apart triplets share z2 and z3 (F1 = z1 varies): the slot of the
     // varying F under this part's permutation decides which kinds take the grouped kernels
@@ -1196,7 +1526,10 @@ int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res,
         if(!s.count)
             continue;
         dim3 grid(ftiles, unsigned((s.count + slice - 1) / slice));
-        if(grouped && g2_structured(k, slot2))
+        if(sep.ok && (k == 0 || (k == 1 && sep.sum_ok))) {
+            if((rc = launch_sep(ctx, s, k, perm, sep, d_y, d_acc)))
+                return rc;
+        } else if(grouped && g2_structured(k, slot2))
             launch_kind_s<false>(ctx, k, slot2, grid, s, slice, nfreq, stride, d_freq, perm, res, partial);
         else if(strided && g2_structured(k, slot1))
             launch_kind_s<true>(ctx, k, slot1, grid, s, slice, nfreq, stride, d_freq, perm, res, partial);
