@@ -975,6 +975,16 @@ __host__ __device__ constexpr SepPlan sep_plan_of(int kind, int perm) {
             }
         }
         pl.ncol = 1;
+    } else if(kind >= 2) {
+        // E = z2 + z3 (kind 2) or z1 + z2 (kind 3) is F1 + F3 here, a function of b alone; D1 and
+        // D3 are one row and one column factor
+        int const sl_c = p3[perm][0] == 1 ? 0 : 2, sl_r = 2 - sl_c;
+        pl.cp[0] = sl_c;
+        pl.cs[0] = 1;
+        pl.rp[0] = sl_r;
+        pl.rs[0] = p3[perm][sl_r];
+        pl.nrow = 1;
+        pl.ncol = 1;
     } else {
         pl.cp[0] = 3; // z1 + z2 + z3 - P1 - P2 - P3
         pl.cs[0] = 3;
@@ -1026,12 +1036,16 @@ __device__ inline void sep_element(double Pa, double Pb, double ya, double yb, d
     }
 }
 
+// Resonant kinds (2: E = z2 + z3 - P2 - P3, 3: E = z1 + z2 - P1 - P2) when E = i Y_b - (P sum):
+// the column element carries the coefficient K = Rc where |E| <= res, else Nc / E.
 template <int KIND, int PERM>
 __global__ void __launch_bounds__(256) g2_eval_mfma_kernel(int64_t nterms, int64_t slice_len, const double* __restrict__ T,
                                                            int64_t cap, int NR, int NC, const double* __restrict__ yF1,
                                                            const double* __restrict__ yF2, const double* __restrict__ yF3,
-                                                           const double* __restrict__ yB, double* __restrict__ partial) {
+                                                           const double* __restrict__ yB, double res2,
+                                                           double* __restrict__ partial) {
     constexpr SepPlan PL = sep_plan_of(KIND, PERM);
+    constexpr int F0 = KIND >= 2 ? 1 : 0; // SoA field of P1 (resonant terms: Rc, Nc, P1, P2, P3)
     int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int const b = blockIdx.y;
     int64_t const slice = int64_t(blockIdx.x) * 4 + wave;
@@ -1068,22 +1082,35 @@ __global__ void __launch_bounds__(256) g2_eval_mfma_kernel(int64_t nterms, int64
             for(int x = 0; x < 3; ++x)
                 acc[m][n][x] = dbl4{0.0, 0.0, 0.0, 0.0};
     // term k of each step of 4; the next step's term is loaded while this one is used
-    auto load = [&](int64_t s, double& C, double& P1, double& P2, double& P3) {
+    double const Yb = yB[b], Yb2 = Yb * Yb;
+    auto load = [&](int64_t s, double& C, double& N, double& P1, double& P2, double& P3) {
         int64_t const t = s + k;
         bool const tv = t < t1;
         int64_t const tt = tv ? t : t0;
         C = tv ? T[tt] : 0.0;
-        P1 = T[cap + tt];
-        P2 = T[2 * cap + tt];
-        P3 = T[3 * cap + tt];
+        N = (KIND >= 2 && tv) ? T[cap + tt] : 0.0;
+        P1 = T[(F0 + 1) * cap + tt];
+        P2 = T[(F0 + 2) * cap + tt];
+        P3 = T[(F0 + 3) * cap + tt];
     };
-    double C = 0, P1 = 1, P2 = 1, P3 = 1;
+    double C = 0, N = 0, P1 = 1, P2 = 1, P3 = 1;
     if(t0 < t1)
-        load(t0, C, P1, P2, P3);
+        load(t0, C, N, P1, P2, P3);
     for(int64_t s = t0; s < t1; s += 4) {
-        double const Cc = C, Q1 = P1, Q2 = P2, Q3 = P3;
+        double const Cc = C, Nc = N, Q1 = P1, Q2 = P2, Q3 = P3;
         if(s + 4 < t1)
-            load(s + 4, C, P1, P2, P3);
+            load(s + 4, C, N, P1, P2, P3);
+        // coefficient of the column element: C (non-resonant) or K = Rc / Nc / E (resonant)
+        double Kr = Cc, Ki = 0.0;
+        if constexpr(KIND >= 2) {
+            double const Er = -(KIND == 2 ? Q2 + Q3 : Q1 + Q2); // E = Er + i Yb
+            double const e2 = fma(Er, Er, Yb2);
+            if(!(e2 <= res2)) {
+                double const ie = Nc * rcp_nr(e2);
+                Kr = Er * ie;
+                Ki = -Yb * ie;
+            }
+        }
         double const Ra = sep_pole<PL.rp[0]>(Q1, Q2, Q3), Rb = sep_pole<PL.rp[1]>(Q1, Q2, Q3);
         double const Ka = sep_pole<PL.cp[0]>(Q1, Q2, Q3), Kb = sep_pole<PL.cp[1]>(Q1, Q2, Q3);
         double const RaRb = Ra * Rb, KaKb = Ka * Kb;
@@ -1091,7 +1118,14 @@ __global__ void __launch_bounds__(256) g2_eval_mfma_kernel(int64_t nterms, int64
 #pragma unroll
         for(int m = 0; m < 2; ++m) {
             sep_element<PL.nrow>(Ra, Rb, ya[m], yb[m], ya2[m], yb2[m], yy[m], RaRb, rs[m], ar[m], ai[m]);
-            sep_element<PL.ncol>(Ka, Kb, xa[m], xb[m], xa2[m], xb2[m], xx[m], KaKb, cs[m] * Cc, br[m], bi[m]);
+            if constexpr(KIND >= 2) {
+                double er, ei;
+                sep_element<1>(Ka, Kb, xa[m], xb[m], xa2[m], xb2[m], xx[m], KaKb, cs[m], er, ei);
+                br[m] = fma(Kr, er, -Ki * ei);
+                bi[m] = fma(Kr, ei, Ki * er);
+            } else {
+                sep_element<PL.ncol>(Ka, Kb, xa[m], xb[m], xa2[m], xb2[m], xx[m], KaKb, cs[m] * Kr, br[m], bi[m]);
+            }
         }
         double a1[2], b2[2], b3[2];
 #pragma unroll
@@ -1424,12 +1458,26 @@ SepGrid detect_sep_grid(std::vector<double> const& h, int nfreq) {
 // One non-resonant kind on a separable grid: matrix-core partial tiles, then their sum into acc.
 template <int KIND, int PERM>
 void launch_sep_k(pomx_ctx* ctx, dim3 grid, TermSpan const& s, int64_t slice, const double* T, SepGrid const& g,
-                  const double* yF1, const double* yF2, const double* yF3, const double* yB, double* partial) {
+                  const double* yF1, const double* yF2, const double* yF3, const double* yB, double res,
+                  double* partial) {
     g2_eval_mfma_kernel<KIND, PERM><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, g.NR, g.NC, yF1, yF2, yF3,
-                                                                   yB, partial);
+                                                                   yB, res * res, partial);
 }
 
-int launch_sep(pomx_ctx* ctx, TermSpan const& s, int kind, int perm, SepGrid const& g, const double* d_y,
+// kinds and permutations the matrix-core path evaluates on a separable grid: every non-resonant
+// one (kind 1 needs Im(F1 + F3) independent of r), and the resonant ones whose E is F1 + F3
+bool sep_applies(int kind, int perm, bool sum_ok) {
+    static constexpr int p3[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    if(kind == 0)
+        return true;
+    if(!sum_ok)
+        return false;
+    if(kind == 1)
+        return true;
+    return kind == 2 ? p3[perm][0] == 1 : p3[perm][2] == 1;
+}
+
+int launch_sep(pomx_ctx* ctx, TermSpan const& s, int kind, int perm, SepGrid const& g, const double* d_y, double res,
                double* d_acc) {
     int64_t const want = std::max<int64_t>(4, 4096 / g.NB);
     int64_t slice = std::max<int64_t>(4, (s.count + want - 1) / want);
@@ -1446,11 +1494,12 @@ int launch_sep(pomx_ctx* ctx, TermSpan const& s, int kind, int perm, SepGrid con
     const double* T = s.base + s.begin;
     dim3 const grid(unsigned(nsl / 4), unsigned(g.NB));
 #define POMX_SEP_CASE(K, P)                                                                        \
-    case K * 6 + P: launch_sep_k<K, P>(ctx, grid, s, slice, T, g, yF1, yF2, yF3, yB, partial); break
+    case K * 6 + P: launch_sep_k<K, P>(ctx, grid, s, slice, T, g, yF1, yF2, yF3, yB, res, partial); break
     switch(kind * 6 + perm) {
         POMX_SEP_CASE(0, 0); POMX_SEP_CASE(0, 1); POMX_SEP_CASE(0, 2); POMX_SEP_CASE(0, 3); POMX_SEP_CASE(0, 4);
         POMX_SEP_CASE(0, 5); POMX_SEP_CASE(1, 0); POMX_SEP_CASE(1, 1); POMX_SEP_CASE(1, 2); POMX_SEP_CASE(1, 3);
-        POMX_SEP_CASE(1, 4); POMX_SEP_CASE(1, 5);
+        POMX_SEP_CASE(1, 4); POMX_SEP_CASE(1, 5); POMX_SEP_CASE(2, 2); POMX_SEP_CASE(2, 3); POMX_SEP_CASE(3, 1);
+        POMX_SEP_CASE(3, 4);
     default: return pomx_fail(ctx, POMX_ERR_ARG, "pomx_g2_part_compute: no separable plan for this kind");
     }
 #undef POMX_SEP_CASE
@@ -1526,8 +1575,8 @@ int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res,
         if(!s.count)
             continue;
         dim3 grid(ftiles, unsigned((s.count + slice - 1) / slice));
-        if(sep.ok && (k == 0 || (k == 1 && sep.sum_ok))) {
-            if((rc = launch_sep(ctx, s, k, perm, sep, d_y, d_acc)))
+        if(sep.ok && sep_applies(k, perm, sep.sum_ok)) {
+            if((rc = launch_sep(ctx, s, k, perm, sep, d_y, res, d_acc)))
                 return rc;
         } else if(grouped && g2_structured(k, slot2))
             launch_kind_s<false>(ctx, k, slot2, grid, s, slice, nfreq, stride, d_freq, perm, res, partial);
