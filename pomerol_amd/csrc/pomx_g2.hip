@@ -1178,6 +1178,129 @@ __global__ void g2_mfma_reduce_kernel(int nslices, int NB, int NR, int NC, const
     acc[2 * f + 1] += si;
 }
 
+// ---- resonant terms with E along the diagonals of the (r, c) plane (matrix cores) ----------
+// On the same grids, the resonant kinds under the permutations (F1, F2, F3) and (F3, F2, F1)
+// have one factor of D1 D3 on F1(b, r) and the other on F3(r), and E on F2 + F3 (kind 2 under
+// the first, kind 3 under the second: Im E = Im F2(c) + Im F3(r), a function of d = c - r on a
+// Matsubara grid) or on F1 + F2 (the other two: Im E a function of s = b + r + c).  Then
+//   term(b, r, c) = Row_t(b, r) H_t(e),  Row_t = 1 / ((F1 - Pa)(F3 - Pb)),
+//   H_t(e) = Rc where |E| <= res, else Nc / E,
+// with e = d or s, and for the 16 rows r0 .. r0 + 15 of one b the needed e form 16 + NC - 1 <= 47
+// consecutive values.  A wave forms the 16 x 48 product of the rows and the 48 diagonal values
+// H_t (3 column tiles; about 1.5 times the needed entries) over its terms; entry (i, q) holds
+// output column c = q + i - 15 (d grids) or c = q - i (s grids) of row r0 + i.  Im E for index e
+// comes from the table yE (checked on the host against every (b, r, c) of the grid).
+constexpr int kDQ = 48; // diagonal tile columns
+
+template <int KIND, int PERM>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g2_eval_diag_kernel(
+    int64_t nterms, int64_t slice_len, const double* __restrict__ T, int64_t cap, int NR, int nrt,
+    const double* __restrict__ yF1, const double* __restrict__ yF3, const double* __restrict__ yE, int nE, double res2,
+    double* __restrict__ partial) {
+    static_assert((KIND == 2 || KIND == 3) && (PERM == 0 || PERM == 5), "no diagonal plan");
+    constexpr bool TOEP = (KIND == 2) == (PERM == 0); // E on F2 + F3 (index d), else on F1 + F2 (index s)
+    // pole of the F1 factor and of the F3 factor: D1 = F[perm 0] - P1, D3 = F[perm 2] - P3
+    constexpr bool F1_IN_D1 = PERM == 0;
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int const b = int(blockIdx.y) / nrt, r0 = 16 * (int(blockIdx.y) % nrt);
+    int64_t const slice = int64_t(blockIdx.x) * 4 + wave;
+    int64_t const t0 = slice * slice_len, t1 = min(nterms, t0 + slice_len);
+    int const i = lane & 15, k = lane >> 4;
+    // row element of row r0 + i (zero beyond NR) and diagonal values of columns q = i + 16 n
+    int const rr = min(r0 + i, NR - 1);
+    double const rsc = r0 + i < NR ? 1.0 : 0.0;
+    double const ya = yF1[b * NR + rr], yb = yF3[rr];
+    double const ya2 = ya * ya, yb2 = yb * yb, yy = ya * yb;
+    double ye[3], ye2[3];
+#pragma unroll
+    for(int n = 0; n < 3; ++n) {
+        int const q = i + 16 * n;
+        int const e = TOEP ? q - 15 - r0 + (NR - 1) : b + r0 + q;
+        ye[n] = yE[min(max(e, 0), nE - 1)];
+        ye2[n] = ye[n] * ye[n];
+    }
+    typedef double dbl4 __attribute__((ext_vector_type(4)));
+    dbl4 acc[3][3];
+#pragma unroll
+    for(int n = 0; n < 3; ++n)
+#pragma unroll
+        for(int x = 0; x < 3; ++x)
+            acc[n][x] = dbl4{0.0, 0.0, 0.0, 0.0};
+    // resonant SoA: Rc, Nc, P1, P2, P3; term k of each step of 4, loaded one step ahead
+    auto load = [&](int64_t s, double& R, double& N, double& P1, double& P2, double& P3) {
+        int64_t const t = s + k;
+        bool const tv = t < t1;
+        int64_t const tt = tv ? t : t0;
+        R = tv ? T[tt] : 0.0;
+        N = tv ? T[cap + tt] : 0.0;
+        P1 = T[2 * cap + tt];
+        P2 = T[3 * cap + tt];
+        P3 = T[4 * cap + tt];
+    };
+    double R = 0, N = 0, P1 = 1, P2 = 1, P3 = 1;
+    if(t0 < t1)
+        load(t0, R, N, P1, P2, P3);
+    for(int64_t s = t0; s < t1; s += 4) {
+        double const Rc = R, Nc = N, Q1 = P1, Q2 = P2, Q3 = P3;
+        if(s + 4 < t1)
+            load(s + 4, R, N, P1, P2, P3);
+        double const Pa = F1_IN_D1 ? Q1 : Q3, Pb = F1_IN_D1 ? Q3 : Q1;
+        double ar, ai;
+        sep_element<2>(Pa, Pb, ya, yb, ya2, yb2, yy, Pa * Pb, rsc, ar, ai);
+        double const Er = -(KIND == 2 ? Q2 + Q3 : Q1 + Q2);
+        double br[3], bi[3];
+#pragma unroll
+        for(int n = 0; n < 3; ++n) {
+            double const e2 = fma(Er, Er, ye2[n]);
+            if(e2 <= res2) {
+                br[n] = Rc;
+                bi[n] = 0.0;
+            } else {
+                double const ie = Nc * rcp_nr(e2);
+                br[n] = Er * ie;
+                bi[n] = -ye[n] * ie;
+            }
+        }
+        double const a1 = ar + ai;
+#pragma unroll
+        for(int n = 0; n < 3; ++n) {
+            double const b2 = bi[n] - br[n], b3 = br[n] + bi[n];
+            acc[n][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, br[n], acc[n][0], 0, 0, 0);
+            acc[n][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, b3, acc[n][1], 0, 0, 0);
+            acc[n][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b2, acc[n][2], 0, 0, 0);
+        }
+    }
+    // D[i][j] of a 16 x 16 tile sits in lane j + 16 (i % 4), register i / 4
+    double* out = partial + (slice * gridDim.y + blockIdx.y) * (16 * kDQ * 2);
+#pragma unroll
+    for(int n = 0; n < 3; ++n)
+#pragma unroll
+        for(int x = 0; x < 4; ++x) {
+            int const row = (lane >> 4) + 4 * x, col = 16 * n + (lane & 15);
+            out[(row * kDQ + col) * 2] = acc[n][0][x] - acc[n][1][x];
+            out[(row * kDQ + col) * 2 + 1] = acc[n][0][x] + acc[n][2][x];
+        }
+}
+
+// acc[(b NR + r) NC + c] += sum over slices of entry (r - r0, q) of tile (b, r / 16), in slice order
+__global__ void g2_diag_reduce_kernel(int nslices, int NB, int NR, int NC, int nrt, int toep,
+                                      const double* __restrict__ partial, double* __restrict__ acc) {
+    int const f = blockIdx.x * blockDim.x + threadIdx.x;
+    if(f >= NB * NR * NC)
+        return;
+    int const c = f % NC, r = (f / NC) % NR, b = f / (NC * NR);
+    int const i = r % 16, q = toep ? c + 15 - i : c + i;
+    int64_t const tile = int64_t(b) * nrt + r / 16;
+    double sr = 0.0, si = 0.0;
+    for(int sl = 0; sl < nslices; ++sl) {
+        const double* p = partial + (((int64_t(sl) * NB * nrt + tile) * 16 + i) * kDQ + q) * 2;
+        sr += p[0];
+        si += p[1];
+    }
+    acc[2 * f] += sr;
+    acc[2 * f + 1] += si;
+}
+
 // 1 in flag[0] if some triplet f does not share z2 and z3 with triplet f mod stride
 __global__ void stride_mismatch_kernel(int nfreq, int stride, const double* __restrict__ fz, int* __restrict__ flag) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1396,9 +1519,11 @@ void launch_kind_s(pomx_ctx* ctx, int k, int slot, dim3 grid, TermSpan const& s,
 
 // Separable grid structure of a triplet list (see g2_eval_mfma_kernel), found on the host.
 struct SepGrid {
-    bool ok = false, sum_ok = false;
+    bool ok = false, sum_ok = false, toep_ok = false, hank_ok = false;
     int NB = 0, NR = 0, NC = 0;
     std::vector<double> yF1, yF2, yF3, yB; // Im F1(b, r), Im F2(c), Im F3(r), Y_b = Im(F1 + F3)
+    std::vector<double> yT;                // Im(F2 + F3) at d = c - r, index d + NR - 1
+    std::vector<double> yH;                // Im(F1 + F2) at s = b + r + c
 };
 
 SepGrid detect_sep_grid(std::vector<double> const& h, int nfreq) {
@@ -1456,6 +1581,30 @@ SepGrid detect_sep_grid(std::vector<double> const& h, int nfreq) {
             g.sum_ok = std::fabs(v - g.yB[b]) <= 8.0 * 2.220446049250313e-16 * scale;
         }
     }
+    // diagonal tables (g2_eval_diag_kernel), each entry taken from one (b, r, c) of its diagonal
+    // and checked against all others
+    auto near = [](double v, double ref, double scale) { return std::fabs(v - ref) <= 8.0 * 2.220446049250313e-16 * scale; };
+    g.yT.resize(std::size_t(NR + NC - 1));
+    for(int e = 0; e < NR + NC - 1; ++e) {
+        int const d = e - (NR - 1), c = d >= 0 ? d : 0, r = d >= 0 ? 0 : -d;
+        g.yT[e] = g.yF2[c] + g.yF3[r];
+    }
+    g.toep_ok = true;
+    for(int r = 0; r < NR && g.toep_ok; ++r)
+        for(int c = 0; c < NC && g.toep_ok; ++c)
+            g.toep_ok = near(g.yF2[c] + g.yF3[r], g.yT[c - r + NR - 1], std::fabs(g.yF2[c]) + std::fabs(g.yF3[r]));
+    g.yH.resize(std::size_t(g.NB + NR + NC - 2));
+    for(int e = 0; e < g.NB + NR + NC - 2; ++e) {
+        int const b = std::min(e, g.NB - 1), r = std::min(e - b, NR - 1), c = e - b - r;
+        g.yH[e] = g.yF1[std::size_t(b) * NR + r] + g.yF2[c];
+    }
+    g.hank_ok = true;
+    for(int b = 0; b < g.NB && g.hank_ok; ++b)
+        for(int r = 0; r < NR && g.hank_ok; ++r)
+            for(int c = 0; c < NC && g.hank_ok; ++c) {
+                double const y1 = g.yF1[std::size_t(b) * NR + r];
+                g.hank_ok = near(y1 + g.yF2[c], g.yH[b + r + c], std::fabs(y1) + std::fabs(g.yF2[c]));
+            }
     return g;
 }
 
@@ -1514,6 +1663,52 @@ int launch_sep(pomx_ctx* ctx, TermSpan const& s, int kind, int perm, SepGrid con
     return POMX_OK;
 }
 
+// resonant kinds and permutations the diagonal matrix-core path takes (g2_eval_diag_kernel)
+bool diag_applies(int kind, int perm, SepGrid const& g) {
+    if(!g.ok || (kind != 2 && kind != 3) || (perm != 0 && perm != 5))
+        return false;
+    bool const toep = (kind == 2) == (perm == 0);
+    return toep ? g.toep_ok : g.hank_ok;
+}
+
+int launch_diag(pomx_ctx* ctx, TermSpan const& s, int kind, int perm, SepGrid const& g, const double* d_y,
+                double res, double* d_acc) {
+    bool const toep = (kind == 2) == (perm == 0);
+    int const nrt = (g.NR + 15) / 16;
+    int const ny = g.NB * nrt;
+    int64_t const want = std::max<int64_t>(4, 4096 / ny);
+    int64_t slice = std::max<int64_t>(4, (s.count + want - 1) / want);
+    slice = (slice + 3) / 4 * 4;
+    int64_t nsl = (s.count + slice - 1) / slice;
+    nsl = (nsl + 3) / 4 * 4;
+    WsBuf<double> partial(ctx, size_t(nsl) * ny * 16 * kDQ * 2);
+    if(!partial.p)
+        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+    // d_y holds yF1, yF2, yF3, yB, yT, yH
+    const double* yF1 = d_y;
+    const double* yF3 = yF1 + std::size_t(g.NB) * g.NR + g.NC;
+    const double* yT = yF3 + g.NR + g.NB;
+    const double* yH = yT + g.yT.size();
+    const double* yE = toep ? yT : yH;
+    int const nE = int(toep ? g.yT.size() : g.yH.size());
+    const double* T = s.base + s.begin;
+    dim3 const grid(unsigned(nsl / 4), unsigned(ny));
+    double const res2 = res * res;
+    switch(kind * 6 + perm) {
+    case 2 * 6 + 0: g2_eval_diag_kernel<2, 0><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, g.NR, nrt, yF1, yF3, yE, nE, res2, partial); break;
+    case 2 * 6 + 5: g2_eval_diag_kernel<2, 5><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, g.NR, nrt, yF1, yF3, yE, nE, res2, partial); break;
+    case 3 * 6 + 0: g2_eval_diag_kernel<3, 0><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, g.NR, nrt, yF1, yF3, yE, nE, res2, partial); break;
+    case 3 * 6 + 5: g2_eval_diag_kernel<3, 5><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, g.NR, nrt, yF1, yF3, yE, nE, res2, partial); break;
+    default: return pomx_fail(ctx, POMX_ERR_ARG, "pomx_g2_part_compute: no diagonal plan for this kind");
+    }
+    POMX_CHECK_LAUNCH(ctx);
+    int const nout = g.NB * g.NR * g.NC;
+    g2_diag_reduce_kernel<<<grid_for(nout), kBlock, 0, ctx->stream>>>(int(nsl), g.NB, g.NR, g.NC, nrt, toep ? 1 : 0,
+                                                                      partial, d_acc);
+    POMX_CHECK_LAUNCH(ctx);
+    return POMX_OK;
+}
+
 int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res, int nfreq, const double* d_freq,
                    double* d_acc) {
     int64_t total = 0;
@@ -1549,6 +1744,8 @@ int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res,
             y.insert(y.end(), sep.yF2.begin(), sep.yF2.end());
             y.insert(y.end(), sep.yF3.begin(), sep.yF3.end());
             y.insert(y.end(), sep.yB.begin(), sep.yB.end());
+            y.insert(y.end(), sep.yT.begin(), sep.yT.end());
+            y.insert(y.end(), sep.yH.begin(), sep.yH.end());
             d_y = WsBuf<double>(ctx, y.size());
             if(!d_y.p)
                 return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
@@ -1581,6 +1778,9 @@ int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res,
         dim3 grid(ftiles, unsigned((s.count + slice - 1) / slice));
         if(sep.ok && sep_applies(k, perm, sep.sum_ok)) {
             if((rc = launch_sep(ctx, s, k, perm, sep, d_y, res, d_acc)))
+                return rc;
+        } else if(diag_applies(k, perm, sep)) {
+            if((rc = launch_diag(ctx, s, k, perm, sep, d_y, res, d_acc)))
                 return rc;
         } else if(grouped && g2_structured(k, slot2))
             launch_kind_s<false>(ctx, k, slot2, grid, s, slice, nfreq, stride, d_freq, perm, res, partial);
