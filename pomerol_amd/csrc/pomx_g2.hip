@@ -1211,13 +1211,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g
     double const rsc = r0 + i < NR ? 1.0 : 0.0;
     double const ya = yF1[b * NR + rr], yb = yF3[rr];
     double const ya2 = ya * ya, yb2 = yb * yb, yy = ya * yb;
-    double ye[3], ye2[3];
+    double ye[3];
 #pragma unroll
     for(int n = 0; n < 3; ++n) {
         int const q = i + 16 * n;
         int const e = TOEP ? q - 15 - r0 + (NR - 1) : b + r0 + q;
         ye[n] = yE[min(max(e, 0), nE - 1)];
-        ye2[n] = ye[n] * ye[n];
     }
     typedef double dbl4 __attribute__((ext_vector_type(4)));
     dbl4 acc[3][3];
@@ -1251,7 +1250,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g
         double br[3], bi[3];
 #pragma unroll
         for(int n = 0; n < 3; ++n) {
-            double const e2 = fma(Er, Er, ye2[n]);
+            double const e2 = fma(Er, Er, ye[n] * ye[n]);
             if(e2 <= res2) {
                 br[n] = Rc;
                 bi[n] = 0.0;
@@ -1582,8 +1581,17 @@ SepGrid detect_sep_grid(std::vector<double> const& h, int nfreq) {
         }
     }
     // diagonal tables (g2_eval_diag_kernel), each entry taken from one (b, r, c) of its diagonal
-    // and checked against all others
-    auto near = [](double v, double ref, double scale) { return std::fabs(v - ref) <= 8.0 * 2.220446049250313e-16 * scale; };
+    // and checked against all others to within a few roundings of the grid's largest frequency
+    // (z1 = W + w3 is itself rounded at the magnitude of W and w3, so the sums along a diagonal
+    // agree to that scale, not to their own)
+    auto amax = [](std::vector<double> const& v) {
+        double m = 0.0;
+        for(double x : v)
+            m = std::max(m, std::fabs(x));
+        return m;
+    };
+    double const tolT = 4.0 * 2.220446049250313e-16 * (amax(g.yF2) + amax(g.yF3));
+    double const tolH = 4.0 * 2.220446049250313e-16 * (amax(g.yF1) + amax(g.yF2));
     g.yT.resize(std::size_t(NR + NC - 1));
     for(int e = 0; e < NR + NC - 1; ++e) {
         int const d = e - (NR - 1), c = d >= 0 ? d : 0, r = d >= 0 ? 0 : -d;
@@ -1592,7 +1600,7 @@ SepGrid detect_sep_grid(std::vector<double> const& h, int nfreq) {
     g.toep_ok = true;
     for(int r = 0; r < NR && g.toep_ok; ++r)
         for(int c = 0; c < NC && g.toep_ok; ++c)
-            g.toep_ok = near(g.yF2[c] + g.yF3[r], g.yT[c - r + NR - 1], std::fabs(g.yF2[c]) + std::fabs(g.yF3[r]));
+            g.toep_ok = std::fabs(g.yF2[c] + g.yF3[r] - g.yT[c - r + NR - 1]) <= tolT;
     g.yH.resize(std::size_t(g.NB + NR + NC - 2));
     for(int e = 0; e < g.NB + NR + NC - 2; ++e) {
         int const b = std::min(e, g.NB - 1), r = std::min(e - b, NR - 1), c = e - b - r;
@@ -1601,10 +1609,8 @@ SepGrid detect_sep_grid(std::vector<double> const& h, int nfreq) {
     g.hank_ok = true;
     for(int b = 0; b < g.NB && g.hank_ok; ++b)
         for(int r = 0; r < NR && g.hank_ok; ++r)
-            for(int c = 0; c < NC && g.hank_ok; ++c) {
-                double const y1 = g.yF1[std::size_t(b) * NR + r];
-                g.hank_ok = near(y1 + g.yF2[c], g.yH[b + r + c], std::fabs(y1) + std::fabs(g.yF2[c]));
-            }
+            for(int c = 0; c < NC && g.hank_ok; ++c)
+                g.hank_ok = std::fabs(g.yF1[std::size_t(b) * NR + r] + g.yF2[c] - g.yH[b + r + c]) <= tolH;
     return g;
 }
 
