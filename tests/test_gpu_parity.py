@@ -222,12 +222,14 @@ def test_imaginary_time(pa, oracle, name):
                 assert abs(v["tau0_parts"] - v["tau0"]) < 1e-13
 
 
-@pytest.mark.parametrize("grid", [(16, 4), (8, 8)])
+@pytest.mark.parametrize("grid", [(16, 4), (8, 8), (20, 2), (32, 1)])
 @pytest.mark.parametrize("comp", ["uuuu", "udud", "duud", "uddu"])
 def test_2pgf_grouped_grid_matches_general_path(pa, comp, grid):
-    """The Matsubara grid takes the grouped evaluation kernels (aligned groups sharing (z1, z3);
-    with 8 bosonic frequencies also triplets a stride apart sharing (z2, z3)); the same triplets
-    in a shuffled order take the general kernel.  Both must agree."""
+    """The Matsubara grid takes the matrix-core kernels (separable and diagonal, the latter with a
+    partial second block of 16 rows at 20 fermionic frequencies and two full blocks at 32) and the
+    grouped evaluation kernels (aligned groups sharing (z1, z3); with 8 bosonic frequencies also
+    triplets a stride apart sharing (z2, z3)); the same triplets in a shuffled order take the
+    general kernel.  Both must agree."""
     c = W.cfg1()
     H = c.expr()
     idx = H.indices()
