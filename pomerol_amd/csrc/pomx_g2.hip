@@ -1008,6 +1008,40 @@ __host__ __device__ constexpr SepPlan sep_plan_of(int kind, int perm) {
     return pl;
 }
 
+// Term staging of the matrix-core kernels.  A wave works through its slice of terms in chunks of
+// kStage; each lane loads one term of the next chunk while the current one is evaluated, and at the
+// chunk boundary the wave stores its loads into its own LDS slots, from which every step reads its
+// 4 terms (16 lanes per address).  The global loads thus have a whole chunk to arrive.  Fields:
+// 0 = C (Rc), 1 = Nc (resonant lists only), 2..4 = P1, P2, P3; terms past the slice get zero
+// coefficients.
+constexpr int kStage = 64;
+
+template <bool RES>
+__device__ inline void stage_load(const double* __restrict__ T, int64_t cap, int64_t t, int64_t t0, int64_t t1,
+                                  double f[5]) {
+    constexpr int F0 = RES ? 1 : 0;
+    bool const tv = t < t1;
+    int64_t const tt = tv ? t : t0;
+    f[0] = tv ? T[tt] : 0.0;
+    f[1] = (RES && tv) ? T[cap + tt] : 0.0;
+    f[2] = T[(F0 + 1) * cap + tt];
+    f[3] = T[(F0 + 2) * cap + tt];
+    f[4] = T[(F0 + 3) * cap + tt];
+}
+
+// The LDS operations of one wave are carried out in issue order, so the previous chunk's reads are
+// served before these writes and the writes before the next chunk's reads; the fences keep the
+// compiler from moving accesses across the boundary.
+__device__ inline void stage_store(double (*W)[kStage], int lane, double const f[5]) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for(int x = 0; x < 5; ++x)
+        W[x][lane] = f[x];
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 template <int W>
 __device__ inline double sep_pole(double P1, double P2, double P3) {
     if constexpr(W == 0)
@@ -1049,7 +1083,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g
                                                            const double* __restrict__ yB, double res2,
                                                            double* __restrict__ partial) {
     constexpr SepPlan PL = sep_plan_of(KIND, PERM);
-    constexpr int F0 = KIND >= 2 ? 1 : 0; // SoA field of P1 (resonant terms: Rc, Nc, P1, P2, P3)
     int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int const b = blockIdx.y;
     int64_t const slice = int64_t(blockIdx.x) * 4 + wave;
@@ -1085,67 +1118,64 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g
 #pragma unroll
             for(int x = 0; x < 3; ++x)
                 acc[m][n][x] = dbl4{0.0, 0.0, 0.0, 0.0};
-    // term k of each step of 4; the next step's term is loaded while this one is used
+    // term k of each step of 4, read from the wave's staged chunk
     double const Yb = yB[b], Yb2 = Yb * Yb;
-    auto load = [&](int64_t s, double& C, double& N, double& P1, double& P2, double& P3) {
-        int64_t const t = s + k;
-        bool const tv = t < t1;
-        int64_t const tt = tv ? t : t0;
-        C = tv ? T[tt] : 0.0;
-        N = (KIND >= 2 && tv) ? T[cap + tt] : 0.0;
-        P1 = T[(F0 + 1) * cap + tt];
-        P2 = T[(F0 + 2) * cap + tt];
-        P3 = T[(F0 + 3) * cap + tt];
-    };
-    double C = 0, N = 0, P1 = 1, P2 = 1, P3 = 1;
+    __shared__ double sT[4][5][kStage];
+    double(*W)[kStage] = sT[wave];
+    double nf[5];
     if(t0 < t1)
-        load(t0, C, N, P1, P2, P3);
-    for(int64_t s = t0; s < t1; s += 4) {
-        double const Cc = C, Nc = N, Q1 = P1, Q2 = P2, Q3 = P3;
-        if(s + 4 < t1)
-            load(s + 4, C, N, P1, P2, P3);
-        // coefficient of the column element: C (non-resonant) or K = Rc / Nc / E (resonant)
-        double Kr = Cc, Ki = 0.0;
-        if constexpr(KIND >= 2) {
-            double const Er = -(KIND == 2 ? Q2 + Q3 : Q1 + Q2); // E = Er + i Yb
-            double const e2 = fma(Er, Er, Yb2);
-            if(!(e2 <= res2)) {
-                double const ie = Nc * rcp_nr(e2);
-                Kr = Er * ie;
-                Ki = -Yb * ie;
-            }
-        }
-        double const Ra = sep_pole<PL.rp[0]>(Q1, Q2, Q3), Rb = sep_pole<PL.rp[1]>(Q1, Q2, Q3);
-        double const Ka = sep_pole<PL.cp[0]>(Q1, Q2, Q3), Kb = sep_pole<PL.cp[1]>(Q1, Q2, Q3);
-        double const RaRb = Ra * Rb, KaKb = Ka * Kb;
-        double ar[2], ai[2], br[2], bi[2];
-#pragma unroll
-        for(int m = 0; m < 2; ++m) {
-            sep_element<PL.nrow>(Ra, Rb, ya[m], yb[m], ya2[m], yb2[m], yy[m], RaRb, rs[m], ar[m], ai[m]);
+        stage_load<(KIND >= 2)>(T, cap, t0 + lane, t0, t1, nf);
+    for(int64_t c0 = t0; c0 < t1; c0 += kStage) {
+        stage_store(W, lane, nf);
+        if(c0 + kStage < t1)
+            stage_load<(KIND >= 2)>(T, cap, c0 + kStage + lane, t0, t1, nf);
+        int const nsteps = (int(min<int64_t>(kStage, t1 - c0)) + 3) / 4;
+        for(int j = 0; j < nsteps; ++j) {
+            int const q = 4 * j + k;
+            double const Cc = W[0][q], Nc = W[1][q], Q1 = W[2][q], Q2 = W[3][q], Q3 = W[4][q];
+            // coefficient of the column element: C (non-resonant) or K = Rc / Nc / E (resonant)
+            double Kr = Cc, Ki = 0.0;
             if constexpr(KIND >= 2) {
-                double er, ei;
-                sep_element<1>(Ka, Kb, xa[m], xb[m], xa2[m], xb2[m], xx[m], KaKb, cs[m], er, ei);
-                br[m] = fma(Kr, er, -Ki * ei);
-                bi[m] = fma(Kr, ei, Ki * er);
-            } else {
-                sep_element<PL.ncol>(Ka, Kb, xa[m], xb[m], xa2[m], xb2[m], xx[m], KaKb, cs[m] * Kr, br[m], bi[m]);
+                double const Er = -(KIND == 2 ? Q2 + Q3 : Q1 + Q2); // E = Er + i Yb
+                double const e2 = fma(Er, Er, Yb2);
+                if(!(e2 <= res2)) {
+                    double const ie = Nc * rcp_nr(e2);
+                    Kr = Er * ie;
+                    Ki = -Yb * ie;
+                }
             }
-        }
-        double a1[2], b2[2], b3[2];
-#pragma unroll
-        for(int m = 0; m < 2; ++m) {
-            a1[m] = ar[m] + ai[m];
-            b2[m] = bi[m] - br[m];
-            b3[m] = br[m] + bi[m];
-        }
-#pragma unroll
-        for(int m = 0; m < 2; ++m)
-#pragma unroll
-            for(int n = 0; n < 2; ++n) {
-                acc[m][n][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[m], br[n], acc[m][n][0], 0, 0, 0);
-                acc[m][n][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[m], b3[n], acc[m][n][1], 0, 0, 0);
-                acc[m][n][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[m], b2[n], acc[m][n][2], 0, 0, 0);
+            double const Ra = sep_pole<PL.rp[0]>(Q1, Q2, Q3), Rb = sep_pole<PL.rp[1]>(Q1, Q2, Q3);
+            double const Ka = sep_pole<PL.cp[0]>(Q1, Q2, Q3), Kb = sep_pole<PL.cp[1]>(Q1, Q2, Q3);
+            double const RaRb = Ra * Rb, KaKb = Ka * Kb;
+            double ar[2], ai[2], br[2], bi[2];
+    #pragma unroll
+            for(int m = 0; m < 2; ++m) {
+                sep_element<PL.nrow>(Ra, Rb, ya[m], yb[m], ya2[m], yb2[m], yy[m], RaRb, rs[m], ar[m], ai[m]);
+                if constexpr(KIND >= 2) {
+                    double er, ei;
+                    sep_element<1>(Ka, Kb, xa[m], xb[m], xa2[m], xb2[m], xx[m], KaKb, cs[m], er, ei);
+                    br[m] = fma(Kr, er, -Ki * ei);
+                    bi[m] = fma(Kr, ei, Ki * er);
+                } else {
+                    sep_element<PL.ncol>(Ka, Kb, xa[m], xb[m], xa2[m], xb2[m], xx[m], KaKb, cs[m] * Kr, br[m], bi[m]);
+                }
             }
+            double a1[2], b2[2], b3[2];
+    #pragma unroll
+            for(int m = 0; m < 2; ++m) {
+                a1[m] = ar[m] + ai[m];
+                b2[m] = bi[m] - br[m];
+                b3[m] = br[m] + bi[m];
+            }
+    #pragma unroll
+            for(int m = 0; m < 2; ++m)
+    #pragma unroll
+                for(int n = 0; n < 2; ++n) {
+                    acc[m][n][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[m], br[n], acc[m][n][0], 0, 0, 0);
+                    acc[m][n][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[m], b3[n], acc[m][n][1], 0, 0, 0);
+                    acc[m][n][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[m], b2[n], acc[m][n][2], 0, 0, 0);
+                }
+        }
     }
     // D[i][j] of a 16 x 16 tile sits in lane j + 16 (i % 4), register i / 4
     double* out = partial + (slice * gridDim.y + b) * (kMT * kMT * 2);
@@ -1225,48 +1255,45 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g
 #pragma unroll
         for(int x = 0; x < 3; ++x)
             acc[n][x] = dbl4{0.0, 0.0, 0.0, 0.0};
-    // resonant SoA: Rc, Nc, P1, P2, P3; term k of each step of 4, loaded one step ahead
-    auto load = [&](int64_t s, double& R, double& N, double& P1, double& P2, double& P3) {
-        int64_t const t = s + k;
-        bool const tv = t < t1;
-        int64_t const tt = tv ? t : t0;
-        R = tv ? T[tt] : 0.0;
-        N = tv ? T[cap + tt] : 0.0;
-        P1 = T[2 * cap + tt];
-        P2 = T[3 * cap + tt];
-        P3 = T[4 * cap + tt];
-    };
-    double R = 0, N = 0, P1 = 1, P2 = 1, P3 = 1;
+    // resonant SoA (Rc, Nc, P1, P2, P3); term k of each step of 4, read from the staged chunk
+    __shared__ double sT[4][5][kStage];
+    double(*W)[kStage] = sT[wave];
+    double nf[5];
     if(t0 < t1)
-        load(t0, R, N, P1, P2, P3);
-    for(int64_t s = t0; s < t1; s += 4) {
-        double const Rc = R, Nc = N, Q1 = P1, Q2 = P2, Q3 = P3;
-        if(s + 4 < t1)
-            load(s + 4, R, N, P1, P2, P3);
-        double const Pa = F1_IN_D1 ? Q1 : Q3, Pb = F1_IN_D1 ? Q3 : Q1;
-        double ar, ai;
-        sep_element<2>(Pa, Pb, ya, yb, ya2, yb2, yy, Pa * Pb, rsc, ar, ai);
-        double const Er = -(KIND == 2 ? Q2 + Q3 : Q1 + Q2);
-        double br[3], bi[3];
-#pragma unroll
-        for(int n = 0; n < 3; ++n) {
-            double const e2 = fma(Er, Er, ye[n] * ye[n]);
-            if(e2 <= res2) {
-                br[n] = Rc;
-                bi[n] = 0.0;
-            } else {
-                double const ie = Nc * rcp_nr(e2);
-                br[n] = Er * ie;
-                bi[n] = -ye[n] * ie;
+        stage_load<true>(T, cap, t0 + lane, t0, t1, nf);
+    for(int64_t c0 = t0; c0 < t1; c0 += kStage) {
+        stage_store(W, lane, nf);
+        if(c0 + kStage < t1)
+            stage_load<true>(T, cap, c0 + kStage + lane, t0, t1, nf);
+        int const nsteps = (int(min<int64_t>(kStage, t1 - c0)) + 3) / 4;
+        for(int j = 0; j < nsteps; ++j) {
+            int const q = 4 * j + k;
+            double const Rc = W[0][q], Nc = W[1][q], Q1 = W[2][q], Q2 = W[3][q], Q3 = W[4][q];
+            double const Pa = F1_IN_D1 ? Q1 : Q3, Pb = F1_IN_D1 ? Q3 : Q1;
+            double ar, ai;
+            sep_element<2>(Pa, Pb, ya, yb, ya2, yb2, yy, Pa * Pb, rsc, ar, ai);
+            double const Er = -(KIND == 2 ? Q2 + Q3 : Q1 + Q2);
+            double br[3], bi[3];
+    #pragma unroll
+            for(int n = 0; n < 3; ++n) {
+                double const e2 = fma(Er, Er, ye[n] * ye[n]);
+                if(e2 <= res2) {
+                    br[n] = Rc;
+                    bi[n] = 0.0;
+                } else {
+                    double const ie = Nc * rcp_nr(e2);
+                    br[n] = Er * ie;
+                    bi[n] = -ye[n] * ie;
+                }
             }
-        }
-        double const a1 = ar + ai;
-#pragma unroll
-        for(int n = 0; n < 3; ++n) {
-            double const b2 = bi[n] - br[n], b3 = br[n] + bi[n];
-            acc[n][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, br[n], acc[n][0], 0, 0, 0);
-            acc[n][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, b3, acc[n][1], 0, 0, 0);
-            acc[n][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b2, acc[n][2], 0, 0, 0);
+            double const a1 = ar + ai;
+    #pragma unroll
+            for(int n = 0; n < 3; ++n) {
+                double const b2 = bi[n] - br[n], b3 = br[n] + bi[n];
+                acc[n][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, br[n], acc[n][0], 0, 0, 0);
+                acc[n][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, b3, acc[n][1], 0, 0, 0);
+                acc[n][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b2, acc[n][2], 0, 0, 0);
+            }
         }
     }
     // D[i][j] of a 16 x 16 tile sits in lane j + 16 (i % 4), register i / 4
