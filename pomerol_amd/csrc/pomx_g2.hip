@@ -1074,8 +1074,9 @@ __device__ inline void sep_element(double Pa, double Pb, double ya, double yb, d
 // the column element carries the coefficient K = Rc where |E| <= res, else Nc / E.
 // Occupancy: left to itself the compiler puts the accumulators in 96 AGPRs on top of ~180 VGPRs,
 // one wave per SIMD, so the element arithmetic and the term loads of a wave stall its matrix
-// instructions.  Asking for two waves per SIMD keeps everything in 166 VGPRs without spills,
-// which allows three.
+// instructions.  Asking for two waves per SIMD keeps everything in 184-196 VGPRs without spills;
+// with the terms staged a chunk ahead, two waves keep the matrix cores as busy as three did
+// (three fit in 168 VGPRs with the lane constants' squares recomputed, and measured no faster).
 template <int KIND, int PERM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g2_eval_mfma_kernel(int64_t nterms, int64_t slice_len, const double* __restrict__ T,
                                                            int64_t cap, int NR, int NC, const double* __restrict__ yF1,
