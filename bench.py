@@ -33,12 +33,12 @@ PEAK_FP64_TFLOPS = 78.6
 # HBM bytes per evaluation-kernel dispatch of the default step (--stride 8), measured with
 # rocprofv3 --pmc: FETCH_SIZE x 2 + WRITE_SIZE x 1 (the factors are what the same counters read
 # for a copy kernel of known size); the counters cannot be read from inside this process.
-EVAL_TRAFFIC_BYTES = 1.87e9
+EVAL_TRAFFIC_BYTES = 1.84e9
 # Hardware FP64 FLOP per term x frequency of the default step (FMA = 2, one MFMA MOPS unit =
 # 512 FLOP), measured with rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 and
 # SQ_INSTS_VALU_MFMA_MOPS_F64 over its evaluation dispatches: the matrix-core and grouped kernels
 # evaluate fewer operations than the 28/36-FLOP direct formula the metric counts.
-EVAL_HW_FLOP_PER_TERM_FREQ = 9.32
+EVAL_HW_FLOP_PER_TERM_FREQ = 9.28
 DEFAULT_STRIDE = 8
 
 
