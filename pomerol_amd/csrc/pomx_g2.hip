@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 namespace {
 
@@ -1946,6 +1947,17 @@ int g2_chunk_tangent(pomx_ctx* ctx, G2Dev const& D, pomx_g2_tangent const& t, in
 // Chunk budget in terms (keys + values + sort buffers: 32 bytes per term).
 constexpr int64_t kChunkTerms = int64_t(1) << 27;
 
+// The chunk budget in force: a positive integer in the environment variable POMX_G2_CHUNK_TERMS
+// replaces kChunkTerms, so that small parts can be generated in several chunks (a test hook).
+int64_t chunk_terms() {
+    if(const char* e = std::getenv("POMX_G2_CHUNK_TERMS")) {
+        long long const v = std::atoll(e);
+        if(v > 0)
+            return int64_t(v);
+    }
+    return kChunkTerms;
+}
+
 } // namespace
 
 extern "C" int pomx_g2_terms_eval(pomx_ctx* ctx, const pomx_g2_terms* t, int perm, double res, int nfreq,
@@ -2024,10 +2036,11 @@ int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* 
     int64_t quads_total = 0;
     std::vector<std::pair<int64_t, int64_t>> chunks; // [pair_begin, pair_end)
     {
+        int64_t const chunk_cap = chunk_terms();
         int64_t b = 0, acc = 0;
         for(int64_t p = 0; p < npairs; ++p) {
             bool boundary = (p == 0) || hr12[hpairs[p]] != hr12[hpairs[p - 1]];
-            if(boundary && acc > 0 && acc + hpair[p] > kChunkTerms) {
+            if(boundary && acc > 0 && acc + hpair[p] > chunk_cap) {
                 chunks.emplace_back(b, p);
                 b = p;
                 acc = 0;

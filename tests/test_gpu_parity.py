@@ -181,6 +181,32 @@ def test_two_particle_gf(pa, oracle, name, comp):
     assert list(r["part_res"]) == list(same["part_res"])
 
 
+def test_2pgf_multichunk_matches_oracle(pa, oracle, monkeypatch):
+    """Parts generated in several chunks (a chunk budget of 20000 raw terms instead of 2^27, so the
+    config-1 parts take the multi-chunk generation and merge of the large parts) give the oracle's
+    values; the merged-term counts are printed, since across chunks the TermList size of the erase
+    rule is the chunk's group count."""
+    H, beta = MODELS["cfg1"]()
+    e, o = build(pa, oracle, H, beta)
+    idx = H.indices()
+    u, d = idx.index(("C", 0, 1)), idx.index(("C", 0, 0))
+    zs = W.g2_grid(8, 2, beta)
+    ref = o.g2(u, d, u, d, zs.ravel(), 1e-8, 1e-16)
+    pa.eval_timing(1)  # counts evaluation launches: one per generated chunk with terms
+    one = e.g2(u, d, u, d, zs, 1e-8, 1e-16)
+    _, l1 = pa.eval_timing(1)
+    monkeypatch.setenv("POMX_G2_CHUNK_TERMS", "20000")
+    many = e.g2(u, d, u, d, zs, 1e-8, 1e-16)
+    monkeypatch.delenv("POMX_G2_CHUNK_TERMS")
+    _, lm = pa.eval_timing(0)
+    n1 = sum(one["part_nonres"]) + sum(one["part_res"])
+    nm = sum(many["part_nonres"]) + sum(many["part_res"])
+    print(f"[multichunk] chunks evaluated {l1} -> {lm}; merged terms {n1} -> {nm}", flush=True)
+    assert lm > l1
+    assert rel_err(many["values"], ref["values"]) < 1e-10
+    assert rel_err(many["values"], one["values"]) < 1e-12
+
+
 def test_2pgf_container_crossing_images(pa):
     """IndexContainer4 images (IndexContainer4.hpp:190-237) equal the directly computed elements:
     fermionic antisymmetry chi_{jikl}(w1,w2;w3) = -chi_{ijkl}(w2,w1;w3) etc."""
