@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <map>
 #include <numeric>
@@ -383,6 +384,7 @@ static inline std::size_t mix(std::size_t h, std::size_t v) {
 template <class T> class TermSet {
     std::unordered_set<T, typename T::Hash, typename T::Eq> data_;
     double tol_;
+    long erased_ = 0;
 
 public:
     TermSet(double res, double tol) : data_(1, typename T::Hash{res}, typename T::Eq{res}), tol_(tol) {}
@@ -397,8 +399,11 @@ public:
         data_.erase(it);
         if(!sum.negligible(tol_, data_.size() + 1))
             data_.insert(sum);
+        else
+            ++erased_;
     }
     std::size_t size() const { return data_.size(); }
+    long erased() const { return erased_; }
     auto begin() const { return data_.begin(); }
     auto end() const { return data_.end(); }
 };
@@ -678,6 +683,7 @@ G2Result ED::g2(int i, int j, int k, int l, std::vector<cplx> const& zs, double 
             int n3 = int(S.blocks[S3].size()), n4 = int(S.blocks[S4].size());
             auto const &E1 = evals[S1], &E2 = evals[S2], &E3 = evals[S3], &E4 = evals[S4];
             auto const &W1 = weights[S1], &W2 = weights[S2], &W3 = weights[S3], &W4 = weights[S4];
+            auto const t_gen = std::chrono::steady_clock::now();
             TermSet<NRTerm> nr(pole_res, coeff_tol);
             TermSet<RTerm> rs(pole_res, coeff_tol);
             double sign = perm3_sign[p];
@@ -721,12 +727,15 @@ G2Result ED::g2(int i, int j, int k, int l, std::vector<cplx> const& zs, double 
             out.n_parts++;
             out.n_nonres += long(nr.size());
             out.n_res += long(rs.size());
+            out.n_erased += nr.erased() + rs.erased();
             out.part_nonres.push_back(long(nr.size()));
             out.part_res.push_back(long(rs.size()));
             for(int x : {S1, S2, S3, S4, p})
                 out.part_blocks.push_back(x);
             std::vector<NRTerm> vnr(nr.begin(), nr.end());
             std::vector<RTerm> vr(rs.begin(), rs.end());
+            auto const t_eval = std::chrono::steady_clock::now();
+            out.gen_seconds += std::chrono::duration<double>(t_eval - t_gen).count();
 #pragma omp parallel for schedule(static)
             for(long w = 0; w < long(nw); ++w) {
                 cplx F[3] = {zs[3 * w], zs[3 * w + 1], -zs[3 * w + 2]};
@@ -739,6 +748,7 @@ G2Result ED::g2(int i, int j, int k, int l, std::vector<cplx> const& zs, double 
                     s2 += t(z1, z2, z3, pole_res);
                 out.values[w] += s + s2;
             }
+            out.eval_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_eval).count();
         }
     }
     return out;

@@ -78,6 +78,8 @@ struct G2Result {
     std::vector<cplx> values;
     long n_parts = 0, n_nonres = 0, n_res = 0;
     long n_quadruples = 0; // weighted index quadruples visited (sum of weights above tolerance)
+    long n_erased = 0;     // merged terms dropped by TermList's tol/(size+1) rule
+    double gen_seconds = 0, eval_seconds = 0; // term generation + merge; frequency fill
     std::vector<long> part_nonres, part_res;
     std::vector<int> part_blocks; // 5 ints per part: S1 S2 S3 S4 perm
 };

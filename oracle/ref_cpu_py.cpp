@@ -157,6 +157,9 @@ PYBIND11_MODULE(pomerol_oracle, m) {
                  d["n_nonres"] = r.n_nonres;
                  d["n_res"] = r.n_res;
                  d["n_quadruples"] = r.n_quadruples;
+                 d["n_erased"] = r.n_erased;
+                 d["gen_seconds"] = r.gen_seconds;
+                 d["eval_seconds"] = r.eval_seconds;
                  d["part_nonres"] = arr(r.part_nonres);
                  d["part_res"] = arr(r.part_res);
                  d["part_blocks"] = arr(r.part_blocks);

@@ -107,9 +107,9 @@ struct RowCtx {
     bool active;
 };
 
-__device__ inline RowCtx row_ctx(G2Dev const& D, const uint64_t* __restrict__ pairs, int64_t pair_pos, int i3) {
+// pr = i1 * n2 + i2
+__device__ inline RowCtx row_ctx(G2Dev const& D, uint64_t pr, int i3) {
     RowCtx r;
-    uint64_t pr = pairs[pair_pos];
     r.i1 = int(pr / uint64_t(D.n2));
     r.i2 = int(pr % uint64_t(D.n2));
     r.i3 = i3;
@@ -122,6 +122,18 @@ __device__ inline RowCtx row_ctx(G2Dev const& D, const uint64_t* __restrict__ pa
     // w4 is non-increasing (ascending energies), so w4[0] bounds every i4
     r.active = (a != 0.0 && b != 0.0) && (r.s3 + D.w4[0] > D.tol);
     return r;
+}
+
+// Rows of one chunk in the reference's visiting order (i1, i3, i2): the chunk's (i1, i2) pairs
+// sorted ascending, each entry x = i1 * n2 + i2, y = the pair's index in the r12-sorted pair list
+// (the order of the row counts), z = first entry of its i1 segment, w = entries in that segment.
+// Chunk row R lies in the segment of entry R / n3; with local = R - n3 z, i3 = local / w and the
+// pair is entry z + local % w.
+__device__ inline void chunk_row(const uint4* __restrict__ rmap, int n3, int64_t R, uint4& pr, int& i3) {
+    uint4 const s = rmap[R / n3];
+    int64_t const local = R - int64_t(n3) * s.z;
+    i3 = int(local / s.w);
+    pr = rmap[s.z + local % s.w];
 }
 
 __device__ inline int wave_sum_int(int v) {
@@ -159,7 +171,7 @@ __global__ void __launch_bounds__(256) g2_count_kernel(G2Dev D, const double* __
     int const lane = threadIdx.x & 63;
     int64_t const nwaves = int64_t(gridDim.x) * (blockDim.x >> 6);
     for(int64_t row = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6; row < nrows; row += nwaves) {
-        RowCtx r = row_ctx(D, pairs, row / D.n3, int(row % D.n3));
+        RowCtx r = row_ctx(D, pairs[row / D.n3], int(row % D.n3));
         int c = 0;
         if(r.active) {
             const double* o3 = O3c + int64_t(r.i3) * D.n4;
@@ -183,23 +195,34 @@ __global__ void pair_sum_kernel(int64_t npairs, int n3, const int* __restrict__ 
     out[p] = s;
 }
 
-__global__ void widen_kernel(int64_t n, const int* __restrict__ in, int64_t* __restrict__ out) {
-    int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if(k < n)
-        out[k] = in[k];
+// row counts of a chunk in visiting order (cnt is in r12-sorted pair order)
+__global__ void chunk_rowcnt_kernel(int64_t nrows, int n3, const uint4* __restrict__ rmap, const int* __restrict__ cnt,
+                                    int64_t* __restrict__ out) {
+    int64_t R = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(R >= nrows)
+        return;
+    uint4 pr;
+    int i3;
+    chunk_row(rmap, n3, R, pr, i3);
+    out[R] = cnt[int64_t(pr.y) * n3 + i3];
 }
 
+// Emission in visiting order: rows of the chunk as chunk_row orders them, lanes over i4, kinds in
+// addMultiterm's order, so after a stable sort by key the members of every group are in the
+// reference's visiting order.
 __global__ void __launch_bounds__(256) g2_emit_kernel(G2Dev D, const double* __restrict__ O3c,
-                                                      const double* __restrict__ C4t, const uint64_t* __restrict__ pairs,
-                                                      int64_t row0, int64_t nrows, int64_t ntot,
+                                                      const double* __restrict__ C4t, const uint4* __restrict__ rmap,
+                                                      int64_t nrows, int64_t ntot,
                                                       const int64_t* __restrict__ off, const uint32_t* __restrict__ r12,
                                                       const uint32_t* __restrict__ r23, const uint32_t* __restrict__ r34,
                                                       uint64_t* __restrict__ keys, uint64_t* __restrict__ vals) {
     int const lane = threadIdx.x & 63;
     int64_t const nwaves = int64_t(gridDim.x) * (blockDim.x >> 6);
     for(int64_t lr = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6; lr < nrows; lr += nwaves) {
-        int64_t row = row0 + lr;
-        RowCtx r = row_ctx(D, pairs, row / D.n3, int(row % D.n3));
+        uint4 pr;
+        int i3;
+        chunk_row(rmap, D.n3, lr, pr, i3);
+        RowCtx r = row_ctx(D, pr.x, i3);
         if(!r.active)
             continue;
         int64_t at = off[lr];
@@ -258,24 +281,77 @@ __device__ inline void g2_unpack(uint64_t v, int& i1, int& i2, int& i3, int& i4)
     i4 = int(v >> 48);
 }
 
-// first (smallest) visiting position of each group's members
-__global__ void g2_first_kernel(G2Dev D, int64_t n, int64_t ngroups, const int64_t* __restrict__ start,
-                                const uint64_t* __restrict__ keys, const uint64_t* __restrict__ vals,
-                                uint64_t* __restrict__ fk, uint64_t* __restrict__ gi_out) {
+// Group members are in visiting order (g2_emit_kernel + a stable sort), so a group's first
+// member starts it.  Visiting position of every group's first member:
+__global__ void g2_first_pos_kernel(G2Dev D, int64_t ngroups, const int64_t* __restrict__ start,
+                                    const uint64_t* __restrict__ keys, const uint64_t* __restrict__ vals,
+                                    uint64_t* __restrict__ fk) {
+    int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(gi >= ngroups)
+        return;
+    int64_t const s = start[gi];
+    int i1, i2, i3, i4;
+    g2_unpack(vals[s], i1, i2, i3, i4);
+    fk[gi] = g2_position(D, int(keys[s] >> 62), i1, i2, i3, i4);
+}
+
+// Positions of every merge (every member but the first of a group): group gi's merges go to
+// q[start[gi] - gi ...] (the number of merges of the groups before it).
+__global__ void g2_merge_pos_kernel(G2Dev D, int64_t n, int64_t ngroups, const int64_t* __restrict__ start,
+                                    const uint64_t* __restrict__ keys, const uint64_t* __restrict__ vals,
+                                    uint64_t* __restrict__ q) {
     int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if(gi >= ngroups)
         return;
     int64_t const s = start[gi], e = (gi + 1 < ngroups) ? start[gi + 1] : n;
     int const kind = int(keys[s] >> 62);
-    uint64_t m = ~uint64_t(0);
-    for(int64_t k = s; k < e; ++k) {
+    uint64_t* out = q + (s - gi);
+    for(int64_t k = s + 1; k < e; ++k) {
         int i1, i2, i3, i4;
         g2_unpack(vals[k], i1, i2, i3, i4);
-        uint64_t const q = g2_position(D, kind, i1, i2, i3, i4);
-        m = q < m ? q : m;
+        out[k - s - 1] = g2_position(D, kind, i1, i2, i3, i4);
     }
-    fk[gi] = m;
-    gi_out[gi] = uint64_t(gi);
+}
+
+// hist[j] += number of groups whose first position fk lies between q[j-1] and q[j] (q ascending,
+// m entries; j = number of q below fk), so that the inclusive prefix sum at j counts the groups
+// started before the merge q[j].  Workgroup histograms in LDS when the bins fit.
+constexpr int kHistLds = 8192;
+__global__ void __launch_bounds__(256) g2_first_hist_kernel(int64_t nfk, const uint64_t* __restrict__ fk,
+                                                            const uint64_t* __restrict__ q, int64_t m,
+                                                            unsigned long long* __restrict__ hist) {
+    __shared__ unsigned int lh[kHistLds];
+    bool const local = m + 1 <= kHistLds;
+    if(local)
+        for(int k = threadIdx.x; k <= m; k += blockDim.x)
+            lh[k] = 0u;
+    __syncthreads();
+    for(int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < nfk; i += int64_t(gridDim.x) * blockDim.x) {
+        int64_t const j = upper_index(q, 0, m, fk[i]);
+        if(local)
+            atomicAdd(&lh[j], 1u);
+        else
+            atomicAdd(&hist[j], 1ull);
+    }
+    __syncthreads();
+    if(local)
+        for(int k = threadIdx.x; k <= m; k += blockDim.x)
+            if(lh[k])
+                atomicAdd(&hist[k], (unsigned long long)lh[k]);
+}
+
+// inclusive = exclusive + own
+__global__ void add_own_kernel(int64_t n, const int64_t* __restrict__ own, int64_t* __restrict__ incl) {
+    int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(k < n)
+        incl[k] += own[k];
+}
+
+// out = started[j] at the entry q[j] == key
+__global__ void lookup_started_kernel(const uint64_t* __restrict__ q, int64_t m, const int64_t* __restrict__ started,
+                                      uint64_t key, int64_t* __restrict__ out) {
+    if(blockIdx.x == 0 && threadIdx.x == 0)
+        out[0] = started[upper_index(q, 0, m, key) - 1];
 }
 
 // One thread per key group: sum the members' coefficients (pole = count-weighted mean).
@@ -370,11 +446,11 @@ __global__ void g2_reduce_kernel(G2Dev D, int64_t n, int64_t ngroups, const int6
     }
 }
 
-// scatter live groups into kind-segmented SoA output; seg_base[kind] = first index of the kind
+// scatter live groups into kind-segmented SoA output (non-resonant kinds first, then resonant)
 __global__ void g2_compact_kernel(int64_t ngroups, const int64_t* __restrict__ live, const int64_t* __restrict__ lpos,
                                   const double* __restrict__ g, const int64_t* __restrict__ start,
                                   const uint64_t* __restrict__ keys, int64_t nr_cap, int64_t r_cap,
-                                  const int64_t* __restrict__ kind_first, double* __restrict__ nr,
+                                  int64_t nr_live, double* __restrict__ nr,
                                   double* __restrict__ rs) {
     int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if(gi >= ngroups || !live[gi])
@@ -387,7 +463,7 @@ __global__ void g2_compact_kernel(int64_t ngroups, const int64_t* __restrict__ l
         nr[2 * nr_cap + at] = g[gi * 5 + 3];
         nr[3 * nr_cap + at] = g[gi * 5 + 4];
     } else {
-        at -= kind_first[2];
+        at -= nr_live; // resonant terms follow the nr_live non-resonant ones in the live order
         rs[0 * r_cap + at] = g[gi * 5 + 0];
         rs[1 * r_cap + at] = g[gi * 5 + 1];
         rs[2 * r_cap + at] = g[gi * 5 + 2];
@@ -493,7 +569,7 @@ __global__ void g2_reduce_tangent_kernel(G2Dev D, G2Tan T, int64_t n, int64_t ng
 __global__ void g2_compact_tangent_kernel(int64_t ngroups, const int64_t* __restrict__ live,
                                           const int64_t* __restrict__ lpos, const double* __restrict__ g,
                                           const int64_t* __restrict__ start, const uint64_t* __restrict__ keys,
-                                          int64_t nr_cap, int64_t r_cap, const int64_t* __restrict__ kind_first,
+                                          int64_t nr_cap, int64_t r_cap, int64_t nr_live,
                                           double* __restrict__ nr, double* __restrict__ rs) {
     int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if(gi >= ngroups || !live[gi])
@@ -507,7 +583,7 @@ __global__ void g2_compact_tangent_kernel(int64_t ngroups, const int64_t* __rest
         for(int q = 0; q < 8; ++q)
             nr[q * nr_cap + at] = x[f[q]];
     } else {
-        at -= kind_first[2];
+        at -= nr_live; // resonant terms follow the nr_live non-resonant ones in the live order
 #pragma unroll
         for(int q = 0; q < 10; ++q)
             rs[q * r_cap + at] = x[q];
@@ -1913,9 +1989,9 @@ int g2_chunk_tangent(pomx_ctx* ctx, G2Dev const& D, pomx_g2_tangent const& t, in
                      int perm, double res,
                      int nfreq, const double* d_freq, double* d_acc, double* d_dacc, int64_t counts[4]) {
     G2Tan T{t.dO1, t.dO2, t.dO3, t.dCX4, t.dE1, t.dE2, t.dE3, t.dE4, t.dw1, t.dw2, t.dw3, t.dw4};
-    WsBuf<int64_t> live(ctx, ng), lpos(ctx, ng), kc(ctx, 4), d_kf(ctx, 4);
+    WsBuf<int64_t> live(ctx, ng), lpos(ctx, ng), kc(ctx, 4);
     WsBuf<double> g(ctx, size_t(ng) * 10);
-    if(!live.p || !lpos.p || !g.p)
+    if(!live.p || !lpos.p || !g.p || !kc.p)
         return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_tangent: out of device memory");
     g2_reduce_tangent_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, T, n, ng, start, keys, vals, thr, R, g, live);
     int rc = pomx_exclusive_scan(ctx, ng, live, lpos);
@@ -1926,26 +2002,24 @@ int g2_chunk_tangent(pomx_ctx* ctx, G2Dev const& D, pomx_g2_tangent const& t, in
     POMX_HIP(ctx, hipMemcpyAsync(c, kc.p, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
     POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     int64_t const nnr = std::max<int64_t>(c[0] + c[1], 1), nr = std::max<int64_t>(c[2] + c[3], 1);
-    int64_t kf[4] = {0, c[0], c[0] + c[1], c[0] + c[1] + c[2]};
-    POMX_HIP(ctx, hipMemcpyAsync(d_kf.p, kf, sizeof(kf), hipMemcpyHostToDevice, ctx->stream));
     WsBuf<double> nrbuf(ctx, size_t(8 * nnr)), rbuf(ctx, size_t(10 * nr));
     if(!nrbuf.p || !rbuf.p)
         return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_tangent: out of device memory");
-    g2_compact_tangent_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(ng, live, lpos, g, start, keys, nnr, nr, d_kf,
-                                                                         nrbuf, rbuf);
+    g2_compact_tangent_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(ng, live, lpos, g, start, keys, nnr, nr,
+                                                                         c[0] + c[1], nrbuf, rbuf);
     POMX_CHECK_LAUNCH(ctx);
     for(int k = 0; k < 4; ++k)
         counts[k] += c[k];
     TermSpan spans[4] = {{nrbuf, nnr, 0, c[0]}, {nrbuf, nnr, c[0], c[1]}, {rbuf, nr, 0, c[2]}, {rbuf, nr, c[2], c[3]}};
     if((rc = launch_g2_tangent_eval(ctx, spans, perm, res, nfreq, d_freq, d_acc, d_dacc)))
         return rc;
-    // kf lives on this stack frame: the copy that reads it must finish before the return
-    POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return POMX_OK;
 }
 
-// Chunk budget in terms (keys + values + sort buffers: 32 bytes per term).
-constexpr int64_t kChunkTerms = int64_t(1) << 27;
+// Chunk budget in raw terms.  A chunk's generation holds keys, values, sort buffers and group
+// arrays (about 110 bytes per term while every group is a singleton): 2^29 terms keep every part
+// of the 4096-state Anderson workload (largest 4.5e8 terms) in one chunk at about 60 GB.
+constexpr int64_t kChunkTerms = int64_t(1) << 29;
 
 // The chunk budget in force: a positive integer in the environment variable POMX_G2_CHUNK_TERMS
 // replaces kChunkTerms, so that small parts can be generated in several chunks (a test hook).
@@ -1976,6 +2050,248 @@ extern "C" int pomx_g2_terms_eval(pomx_ctx* ctx, const pomx_g2_terms* t, int per
 }
 
 namespace {
+
+// One chunk's terms, sorted by key (the members of every group in visiting order), and, once
+// reduced, its group results.
+struct G2Chunk {
+    WsBuf<uint64_t> keys, vals;
+    WsBuf<int64_t> start; // first member of every group
+    int64_t n = 0, ng = 0;
+    WsBuf<double> g;      // 5 doubles per group (g2_reduce_kernel)
+    WsBuf<int64_t> live;
+    bool reduced = false; // g and live hold the final decisions
+};
+
+// Tables shared by a part's chunks.
+struct G2Gen {
+    const double *O3c, *C4t;
+    const int* rowcnt;     // per (r12-sorted pair, i3)
+    const uint32_t *r12, *r23, *r34;
+};
+
+// Generate, sort and group the terms of the pairs [pb, pe) of the r12-sorted pair list.
+int g2_generate_chunk(pomx_ctx* ctx, G2Dev const& D, G2Gen const& T, std::vector<uint64_t> const& hpairs,
+                      int64_t pb, int64_t pe, int64_t n, G2Chunk& out) {
+    // the chunk's pairs in ascending (i1, i2) order, in i1 segments
+    std::vector<std::pair<uint64_t, uint32_t>> ps;
+    ps.reserve(size_t(pe - pb));
+    for(int64_t p = pb; p < pe; ++p)
+        ps.emplace_back(hpairs[size_t(p)], uint32_t(p));
+    std::sort(ps.begin(), ps.end());
+    std::vector<uint4> rmap(ps.size());
+    for(size_t a = 0; a < ps.size();) {
+        size_t b = a;
+        uint64_t const i1 = ps[a].first / uint64_t(D.n2);
+        while(b < ps.size() && ps[b].first / uint64_t(D.n2) == i1)
+            ++b;
+        for(size_t k = a; k < b; ++k)
+            rmap[k] = make_uint4(uint32_t(ps[k].first), ps[k].second, uint32_t(a), uint32_t(b - a));
+        a = b;
+    }
+    int64_t const nrow = int64_t(rmap.size()) * D.n3;
+    WsBuf<uint4> d_rmap(ctx, rmap.size());
+    WsBuf<int64_t> wide(ctx, nrow), off(ctx, nrow);
+    if(!d_rmap.p || !wide.p || !off.p)
+        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+    POMX_HIP(ctx, hipMemcpyAsync(d_rmap.p, rmap.data(), sizeof(uint4) * rmap.size(), hipMemcpyHostToDevice, ctx->stream));
+    chunk_rowcnt_kernel<<<grid_for(nrow), kBlock, 0, ctx->stream>>>(nrow, D.n3, d_rmap, T.rowcnt, wide);
+    POMX_CHECK_LAUNCH(ctx);
+    int rc = pomx_exclusive_scan(ctx, nrow, wide, off);
+    if(rc)
+        return rc;
+    WsBuf<uint64_t> k0(ctx, n), v0(ctx, n);
+    out.keys = WsBuf<uint64_t>(ctx, n);
+    out.vals = WsBuf<uint64_t>(ctx, n);
+    if(!k0.p || !v0.p || !out.keys.p || !out.vals.p)
+        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+    unsigned const eblocks = unsigned(std::min<int64_t>((nrow + 3) / 4, 1 << 16));
+    g2_emit_kernel<<<eblocks, kBlock, 0, ctx->stream>>>(D, T.O3c, T.C4t, d_rmap, nrow, n, off, T.r12, T.r23, T.r34, k0,
+                                                         v0);
+    POMX_CHECK_LAUNCH(ctx);
+    if((rc = pomx_sort_pairs(ctx, n, k0, out.keys, v0, out.vals, 0, 64)))
+        return rc;
+    k0 = WsBuf<uint64_t>();
+    v0 = WsBuf<uint64_t>();
+    WsBuf<int64_t> head(ctx, n), gid(ctx, n);
+    if(!head.p || !gid.p)
+        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+    heads_kernel<<<grid_for(n), kBlock, 0, ctx->stream>>>(n, out.keys, head);
+    if((rc = pomx_exclusive_scan(ctx, n, head, gid)))
+        return rc;
+    int64_t lh = 0, lg = 0; // (read_i64 synchronises: rmap's copy has finished before it is freed)
+    if((rc = read_i64(ctx, head.p + n - 1, &lh)) || (rc = read_i64(ctx, gid.p + n - 1, &lg)))
+        return rc;
+    out.n = n;
+    out.ng = lh + lg;
+    out.start = WsBuf<int64_t>(ctx, out.ng);
+    if(!out.start.p)
+        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+    group_start_kernel<<<grid_for(n), kBlock, 0, ctx->stream>>>(n, head, gid, out.start);
+    POMX_CHECK_LAUNCH(ctx);
+    return POMX_OK;
+}
+
+int g2_alloc_groups(pomx_ctx* ctx, G2Chunk& c) {
+    if(!c.g.p)
+        c.g = WsBuf<double>(ctx, size_t(c.ng) * 5);
+    if(!c.live.p)
+        c.live = WsBuf<int64_t>(ctx, c.ng);
+    if(!c.g.p || !c.live.p)
+        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+    return POMX_OK;
+}
+
+struct KeptChunk {
+    double* nr;
+    double* r;
+    int64_t nr_cap, r_cap, c[4];
+};
+
+// Reduce (unless already reduced), compact and evaluate one chunk; then free its terms.
+int g2_finish_chunk(pomx_ctx* ctx, G2Dev const& D, G2Chunk& c, double thr, EraseRule const& rule,
+                    const pomx_g2_tangent* tan, int perm, double res, int nfreq, const double* d_freq, double* d_acc,
+                    double* d_dacc, bool keep, std::vector<KeptChunk>& kept, int64_t counts[4]) {
+    int rc;
+    if(tan) {
+        rc = g2_chunk_tangent(ctx, D, *tan, c.n, c.ng, c.start, c.keys, c.vals, thr, rule, perm, res, nfreq, d_freq,
+                              d_acc, d_dacc, counts);
+        c = G2Chunk();
+        return rc;
+    }
+    int64_t const ng = c.ng;
+    if(!c.reduced) {
+        if((rc = g2_alloc_groups(ctx, c)))
+            return rc;
+        g2_reduce_kernel<false><<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, c.n, ng, c.start, c.keys, c.vals, thr, rule,
+                                                                            c.g, c.live, nullptr, nullptr, nullptr,
+                                                                            nullptr);
+        POMX_CHECK_LAUNCH(ctx);
+    }
+    WsBuf<int64_t> lpos(ctx, ng), kc(ctx, 4);
+    if(!lpos.p || !kc.p)
+        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+    if((rc = pomx_exclusive_scan(ctx, ng, c.live, lpos)))
+        return rc;
+    kind_count_kernel<<<1, 64, 0, ctx->stream>>>(ng, c.live, lpos, c.start, c.keys, kc);
+    int64_t k[4];
+    POMX_HIP(ctx, hipMemcpyAsync(k, kc.p, sizeof(k), hipMemcpyDeviceToHost, ctx->stream));
+    POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int64_t const nnr = k[0] + k[1], nr = k[2] + k[3];
+    double* nrbuf = static_cast<double*>(ws_get(ctx, sizeof(double) * 4 * std::max<int64_t>(nnr, 1)));
+    double* rbuf = static_cast<double*>(ws_get(ctx, sizeof(double) * 5 * std::max<int64_t>(nr, 1)));
+    if(!nrbuf || !rbuf)
+        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+    g2_compact_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(ng, c.live, lpos, c.g, c.start, c.keys,
+                                                                 std::max<int64_t>(nnr, 1), std::max<int64_t>(nr, 1), nnr,
+                                                                 nrbuf, rbuf);
+    POMX_CHECK_LAUNCH(ctx);
+    c = G2Chunk(); // the terms are no longer needed
+    for(int q = 0; q < 4; ++q)
+        counts[q] += k[q];
+    if(nfreq > 0) {
+        TermSpan spans[4] = {{nrbuf, std::max<int64_t>(nnr, 1), 0, k[0]},
+                             {nrbuf, std::max<int64_t>(nnr, 1), k[0], k[1]},
+                             {rbuf, std::max<int64_t>(nr, 1), 0, k[2]},
+                             {rbuf, std::max<int64_t>(nr, 1), k[2], k[3]}};
+        if((rc = launch_g2_eval(ctx, spans, perm, res, nfreq, d_freq, d_acc)))
+            return rc;
+    }
+    if(keep)
+        kept.push_back(KeptChunk{nrbuf, rbuf, std::max<int64_t>(nnr, 1), std::max<int64_t>(nr, 1), {k[0], k[1], k[2], k[3]}});
+    else {
+        ws_put(ctx, nrbuf);
+        ws_put(ctx, rbuf);
+    }
+    return POMX_OK;
+}
+
+// The reference's erase decisions (TermList.hpp add_term: |C| <= tol/(size+1)) for the held
+// chunks, whose groups include every merge of the part.  The threshold of a merge needs the
+// number of groups of its list started before it, over the whole part: the merge positions of the
+// held chunks are sorted (with a sentinel between the two lists) and every group's first position
+// (fks, one array per chunk of the part) is counted into the gap it falls in; the erase and restart
+// events are then settled by the fixed point of erase_fixed_point over all held chunks.  On
+// return every held chunk is reduced and 'rule' is the settled rule.
+int g2_settle(pomx_ctx* ctx, G2Dev const& D, double tol, std::vector<G2Chunk*> const& held,
+              std::vector<std::pair<const uint64_t*, int64_t>> const& fks, EraseRule& rule, EraseEvents& ev,
+              WsBuf<uint64_t>& qs, WsBuf<int64_t>& started) {
+    constexpr uint64_t kListGap = (uint64_t(1) << 63) - 1; // above every first-list and below every second-list position
+    int64_t m = 1, ng = 0;
+    for(G2Chunk const* c : held) {
+        m += c->n - c->ng;
+        ng += c->ng;
+    }
+    WsBuf<uint64_t> q0(ctx, m), v0(ctx, m), v1(ctx, m);
+    qs = WsBuf<uint64_t>(ctx, m);
+    WsBuf<int64_t> hist(ctx, m + 1);
+    started = WsBuf<int64_t>(ctx, m + 1);
+    WsBuf<int64_t> base(ctx, 1);
+    if(!q0.p || !v0.p || !v1.p || !qs.p || !hist.p || !started.p || !base.p)
+        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+    int64_t at = 0;
+    for(G2Chunk const* c : held) {
+        g2_merge_pos_kernel<<<grid_for(c->ng), kBlock, 0, ctx->stream>>>(D, c->n, c->ng, c->start, c->keys, c->vals,
+                                                                          q0.p + at);
+        POMX_CHECK_LAUNCH(ctx);
+        at += c->n - c->ng;
+    }
+    POMX_HIP(ctx, hipMemcpyAsync(q0.p + at, &kListGap, sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
+    int rc = pomx_sort_pairs(ctx, m, q0, qs, v0, v1, 0, 64);
+    if(rc)
+        return rc;
+    if((rc = pomx_fill_zero(ctx, hist.p, sizeof(int64_t) * size_t(m + 1))))
+        return rc;
+    for(auto const& f : fks) {
+        if(f.second == 0)
+            continue;
+        unsigned const blocks = unsigned(std::min<int64_t>((f.second + kBlock - 1) / kBlock, 4096));
+        g2_first_hist_kernel<<<blocks, kBlock, 0, ctx->stream>>>(f.second, f.first, qs, m,
+                                                                  reinterpret_cast<unsigned long long*>(hist.p));
+        POMX_CHECK_LAUNCH(ctx);
+    }
+    if((rc = pomx_exclusive_scan(ctx, m + 1, hist, started)))
+        return rc;
+    add_own_kernel<<<grid_for(m + 1), kBlock, 0, ctx->stream>>>(m + 1, hist, started);
+    lookup_started_kernel<<<1, 64, 0, ctx->stream>>>(qs, m, started, kListGap, base);
+    POMX_CHECK_LAUNCH(ctx);
+    int64_t base_hi = 0; // (read_i64 synchronises: kListGap's copy has finished)
+    if((rc = read_i64(ctx, base.p, &base_hi)))
+        return rc;
+    rule.fk = qs.p;
+    rule.nfk = m;
+    rule.qstarted = started.p;
+    rule.base_hi = base_hi;
+    rule.ev_key = nullptr;
+    rule.ev_cum = nullptr;
+    rule.nev = 0;
+    for(G2Chunk* c : held)
+        if((rc = g2_alloc_groups(ctx, *c)))
+            return rc;
+    auto pass = [&](EraseRule const& R, bool emit, int64_t* evn, uint64_t* ko, uint64_t* wo) -> int {
+        int64_t goff = 0;
+        for(G2Chunk* c : held) {
+            if(emit)
+                g2_reduce_kernel<true><<<grid_for(c->ng), kBlock, 0, ctx->stream>>>(D, c->n, c->ng, c->start, c->keys,
+                                                                                    c->vals, 0.0, R, c->g, c->live,
+                                                                                    evn + goff, ko, wo, nullptr);
+            else
+                g2_reduce_kernel<false><<<grid_for(c->ng), kBlock, 0, ctx->stream>>>(D, c->n, c->ng, c->start, c->keys,
+                                                                                     c->vals, 0.0, R, c->g, c->live,
+                                                                                     evn + goff, ko, wo, nullptr);
+            POMX_CHECK_LAUNCH(ctx);
+            goff += c->ng;
+        }
+        return POMX_OK;
+    };
+    if((rc = erase_fixed_point(ctx, ng, rule, pass, ev, "pomx_g2_part_compute")))
+        return rc;
+    rule.ev_key = ev.key.p;
+    rule.ev_cum = ev.cum.p;
+    rule.nev = ev.n;
+    for(G2Chunk* c : held)
+        c->reduced = true;
+    return POMX_OK;
+}
 
 // Shared by the value path (tan == NULL) and the tangent path (keep must be NULL).
 int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* tan, double beta, double res, double tol,
@@ -2034,160 +2350,108 @@ int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* 
     if(herr)
         return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_g2_part_compute: |pole/pole_res| out of range");
     int64_t quads_total = 0;
-    std::vector<std::pair<int64_t, int64_t>> chunks; // [pair_begin, pair_end)
+    struct Span {
+        int64_t pb, pe, n;
+    };
+    std::vector<Span> chunks; // [pair_begin, pair_end) of the r12-sorted pairs, n terms
     {
         int64_t const chunk_cap = chunk_terms();
         int64_t b = 0, acc = 0;
         for(int64_t p = 0; p < npairs; ++p) {
             bool boundary = (p == 0) || hr12[hpairs[p]] != hr12[hpairs[p - 1]];
             if(boundary && acc > 0 && acc + hpair[p] > chunk_cap) {
-                chunks.emplace_back(b, p);
+                chunks.push_back(Span{b, p, acc});
                 b = p;
                 acc = 0;
             }
             acc += hpair[p];
             quads_total += hpair[p];
         }
-        if(npairs > b)
-            chunks.emplace_back(b, npairs);
+        if(npairs > b && acc > 0)
+            chunks.push_back(Span{b, npairs, acc});
     }
-    // one chunk holds every group of the part; positions need n1 n2 n3 n4 * 2 < 2^63
-    bool const exact_merge = chunks.size() == 1 && double(P.n1) * P.n2 * P.n3 * P.n4 * 2.0 < 9.0e18;
-    struct KeptChunk {
-        double* nr;
-        double* r;
-        int64_t nr_cap, r_cap, c[4];
-    };
+    // visiting positions (list bit, ((i1 n3 + i3) n2 + i2) n4 + i4, slot) must fit below the list gap
+    bool const positions_ok = double(P.n1) * P.n2 * P.n3 * P.n4 * 2.0 < 9.2e18;
+    bool const multi = chunks.size() > 1;
+    G2Gen const T{O3c, C4t, rowcnt, r12, r23, r34};
     std::vector<KeptChunk> kept;
+    EraseRule rule{tol, nullptr, 0, nullptr, nullptr, 0};
+    // 3. per chunk: generate; a chunk without merges (or, alone in its part, without a merge the
+    // reference's threshold could decide otherwise) is final at once; the others are held until
+    // the part's erase decisions are settled
+    std::vector<G2Chunk> held_store;
+    held_store.reserve(chunks.size());
+    std::vector<WsBuf<uint64_t>> fk_store;
+    std::vector<std::pair<const uint64_t*, int64_t>> fks;
     for(auto const& ch : chunks) {
-        int64_t row0 = ch.first * P.n3, nrow = (ch.second - ch.first) * P.n3;
-        int64_t n = 0;
-        for(int64_t p = ch.first; p < ch.second; ++p)
-            n += hpair[p];
-        if(n == 0)
-            continue;
-        WsBuf<int64_t> off(ctx, nrow);
-        WsBuf<int64_t> wide(ctx, nrow);
-        widen_kernel<<<grid_for(nrow), kBlock, 0, ctx->stream>>>(nrow, rowcnt.p + row0, wide);
-        if((rc = pomx_exclusive_scan(ctx, nrow, wide, off)))
+        G2Chunk c;
+        if((rc = g2_generate_chunk(ctx, D, T, hpairs, ch.pb, ch.pe, ch.n, c)))
             return rc;
-        WsBuf<uint64_t> k0(ctx, n), k1(ctx, n), v0(ctx, n), v1(ctx, n);
-        unsigned const eblocks = unsigned(std::min<int64_t>((nrow + 3) / 4, 1 << 16));
-        g2_emit_kernel<<<eblocks, kBlock, 0, ctx->stream>>>(D, O3c, C4t, pairs, row0, nrow, n, off, r12, r23, r34, k0, v0);
-        POMX_CHECK_LAUNCH(ctx);
-        if((rc = pomx_sort_pairs(ctx, n, k0, k1, v0, v1, 0, 64)))
-            return rc;
-        WsBuf<int64_t> head(ctx, n), gid(ctx, n);
-        heads_kernel<<<grid_for(n), kBlock, 0, ctx->stream>>>(n, k1, head);
-        if((rc = pomx_exclusive_scan(ctx, n, head, gid)))
-            return rc;
-        int64_t lh = 0, lg = 0;
-        if((rc = read_i64(ctx, head.p + n - 1, &lh)) || (rc = read_i64(ctx, gid.p + n - 1, &lg)))
-            return rc;
-        int64_t ng = lh + lg;
-        WsBuf<int64_t> start(ctx, ng), live(ctx, ng), lpos(ctx, ng);
-        WsBuf<double> g(ctx, size_t(ng) * 5);
-        if(!start.p || !live.p || !lpos.p || !g.p)
-            return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
-        group_start_kernel<<<grid_for(n), kBlock, 0, ctx->stream>>>(n, head, gid, start);
-        // TermList erases a merged term when |coeff| <= tol/(size+1), size+1 being the number of
-        // terms in the list at that moment.  A part generated in one chunk holds every group, so
-        // the reference's rule is applied exactly: a pass with the fixed threshold tol/(ng+1)
-        // decides every merge as the reference does unless some merge is ambiguous, and only then
-        // the first positions are sorted and the events settled (EraseRule, erase_fixed_point).
-        // Across chunks, size is taken as the number of groups in this chunk.
-        double const thr = tol / double(ng + 1);
-        EraseRule rule{tol, nullptr, 0, nullptr, nullptr, 0};
-        EraseEvents ev;
-        WsBuf<uint64_t> fk0, fk1;
-        bool exact_pass = false;
-        if(exact_merge) { // a pass with the fixed threshold tells whether any decision could differ
+        int64_t const merges = c.n - c.ng;
+        if(multi && positions_ok) { // every group's first position, for the thresholds of other chunks' merges
+            fk_store.emplace_back(ctx, c.ng);
+            if(!fk_store.back().p)
+                return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+            g2_first_pos_kernel<<<grid_for(c.ng), kBlock, 0, ctx->stream>>>(D, c.ng, c.start, c.keys, c.vals,
+                                                                             fk_store.back());
+            POMX_CHECK_LAUNCH(ctx);
+            fks.emplace_back(fk_store.back().p, c.ng);
+        }
+        // a pass with a threshold below every one the reference can use here, tol/(groups + 1),
+        // tells whether any decision could differ: only a merge whose max(|c0|, |c1|) lies in
+        // (that, tol] can
+        bool final_now = merges == 0;
+        if(!final_now && !multi) {
             WsBuf<int> ambig(ctx, 1);
             if(!ambig.p)
                 return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
-            if((rc = pomx_fill_zero(ctx, ambig.p, sizeof(int))))
+            if((rc = pomx_fill_zero(ctx, ambig.p, sizeof(int))) || (rc = g2_alloc_groups(ctx, c)))
                 return rc;
-            g2_reduce_kernel<false><<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, n, ng, start, k1, v1, thr, rule, g, live,
-                                                                                nullptr, nullptr, nullptr, ambig);
+            double const thr = tol / double(c.ng + 1);
+            g2_reduce_kernel<false><<<grid_for(c.ng), kBlock, 0, ctx->stream>>>(D, c.n, c.ng, c.start, c.keys, c.vals,
+                                                                                  thr, rule, c.g, c.live, nullptr,
+                                                                                  nullptr, nullptr, ambig);
             POMX_CHECK_LAUNCH(ctx);
             int ha = 0;
             POMX_HIP(ctx, hipMemcpyAsync(&ha, ambig.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
             POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            exact_pass = ha != 0;
+            c.reduced = ha == 0;
+            final_now = c.reduced;
         }
-        if(exact_pass) {
-            fk0 = WsBuf<uint64_t>(ctx, ng);
-            fk1 = WsBuf<uint64_t>(ctx, ng);
-            WsBuf<uint64_t> i0(ctx, ng), i1(ctx, ng);
-            if(!fk0.p || !fk1.p || !i0.p || !i1.p)
-                return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
-            g2_first_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, n, ng, start, k1, v1, fk0, i0);
-            POMX_CHECK_LAUNCH(ctx);
-            if((rc = pomx_sort_pairs(ctx, ng, fk0, fk1, i0, i1, 0, 64)))
-                return rc;
-            rule.fk = fk1.p;
-            rule.nfk = ng;
-            auto pass = [&](EraseRule const& R, bool emit, int64_t* evn, uint64_t* ko, uint64_t* wo) -> int {
-                if(emit)
-                    g2_reduce_kernel<true><<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, n, ng, start, k1, v1, thr, R, g,
-                                                                                       live, evn, ko, wo, nullptr);
-                else
-                    g2_reduce_kernel<false><<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, n, ng, start, k1, v1, thr, R, g,
-                                                                                        live, evn, ko, wo, nullptr);
-                POMX_CHECK_LAUNCH(ctx);
-                return POMX_OK;
-            };
-            if((rc = erase_fixed_point(ctx, ng, rule, pass, ev, "pomx_g2_part_compute")))
-                return rc;
-            rule.ev_key = ev.key.p;
-            rule.ev_cum = ev.cum.p;
-            rule.nev = ev.n;
-        }
-        if(tan) {
-            if((rc = g2_chunk_tangent(ctx, D, *tan, n, ng, start, k1, v1, thr, rule, P.perm, res, nfreq, d_freq, d_acc,
-                                      d_dacc, counts)))
+        if(final_now) { // (the tangent path re-reduces with the same threshold as that pass)
+            if((rc = g2_finish_chunk(ctx, D, c, tol / double(c.ng + 1), rule, tan, P.perm, res, nfreq, d_freq,
+                                     d_acc, d_dacc, keep != nullptr, kept, counts)))
                 return rc;
             continue;
         }
-        if(!exact_merge) // (with exact_merge, g and live already hold the exact result)
-            g2_reduce_kernel<false><<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, n, ng, start, k1, v1, thr, rule, g, live,
-                                                                                nullptr, nullptr, nullptr, nullptr);
-        WsBuf<int64_t> kc(ctx, 4);
-        if((rc = pomx_exclusive_scan(ctx, ng, live, lpos)))
+        if(!positions_ok)
+            return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_g2_part_compute: blocks too large for the exact term merge");
+        if(!multi) { // the only chunk: its own first positions
+            fk_store.emplace_back(ctx, c.ng);
+            if(!fk_store.back().p)
+                return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+            g2_first_pos_kernel<<<grid_for(c.ng), kBlock, 0, ctx->stream>>>(D, c.ng, c.start, c.keys, c.vals,
+                                                                             fk_store.back());
+            POMX_CHECK_LAUNCH(ctx);
+            fks.emplace_back(fk_store.back().p, c.ng);
+        }
+        held_store.push_back(std::move(c));
+    }
+    // 4. settle the held chunks' erase decisions over the whole part, then finish them
+    if(!held_store.empty()) {
+        std::vector<G2Chunk*> held;
+        for(auto& c : held_store)
+            held.push_back(&c);
+        EraseEvents ev;
+        WsBuf<uint64_t> qs;
+        WsBuf<int64_t> started;
+        if((rc = g2_settle(ctx, D, tol, held, fks, rule, ev, qs, started)))
             return rc;
-        kind_count_kernel<<<1, 64, 0, ctx->stream>>>(ng, live, lpos, start, k1, kc);
-        int64_t hk[4];
-        POMX_HIP(ctx, hipMemcpyAsync(hk, kc.p, sizeof(hk), hipMemcpyDeviceToHost, ctx->stream));
-        POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        int64_t c[4] = {int64_t(hk[0]), int64_t(hk[1]), int64_t(hk[2]), int64_t(hk[3])};
-        int64_t nnr = c[0] + c[1], nr = c[2] + c[3];
-        int64_t kf[4] = {0, c[0], c[0] + c[1], c[0] + c[1] + c[2]};
-        WsBuf<int64_t> d_kf(ctx, 4);
-        POMX_HIP(ctx, hipMemcpyAsync(d_kf.p, kf, sizeof(kf), hipMemcpyHostToDevice, ctx->stream));
-        double* nrbuf = static_cast<double*>(ws_get(ctx, sizeof(double) * 4 * std::max<int64_t>(nnr, 1)));
-        double* rbuf = static_cast<double*>(ws_get(ctx, sizeof(double) * 5 * std::max<int64_t>(nr, 1)));
-        if(!nrbuf || !rbuf)
-            return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
-        g2_compact_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(ng, live, lpos, g, start, k1, std::max<int64_t>(nnr, 1),
-                                                                     std::max<int64_t>(nr, 1), d_kf, nrbuf, rbuf);
-        POMX_CHECK_LAUNCH(ctx);
-        for(int k = 0; k < 4; ++k)
-            counts[k] += c[k];
-        if(nfreq > 0) {
-            TermSpan spans[4] = {{nrbuf, std::max<int64_t>(nnr, 1), 0, c[0]},
-                                 {nrbuf, std::max<int64_t>(nnr, 1), c[0], c[1]},
-                                 {rbuf, std::max<int64_t>(nr, 1), 0, c[2]},
-                                 {rbuf, std::max<int64_t>(nr, 1), c[2], c[3]}};
-            if((rc = launch_g2_eval(ctx, spans, P.perm, res, nfreq, d_freq, d_acc)))
+        fk_store.clear();
+        for(auto& c : held_store)
+            if((rc = g2_finish_chunk(ctx, D, c, 0.0, rule, tan, P.perm, res, nfreq, d_freq, d_acc, d_dacc,
+                                     keep != nullptr, kept, counts)))
                 return rc;
-        }
-        if(keep)
-            kept.push_back(KeptChunk{nrbuf, rbuf, std::max<int64_t>(nnr, 1), std::max<int64_t>(nr, 1),
-                                     {c[0], c[1], c[2], c[3]}});
-        else {
-            ws_put(ctx, nrbuf);
-            ws_put(ctx, rbuf);
-        }
     }
     if(keep) {
         int64_t nnr = counts[0] + counts[1], nr = counts[2] + counts[3];

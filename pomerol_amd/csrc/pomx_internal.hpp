@@ -114,11 +114,17 @@ __device__ inline int64_t upper_index(const uint64_t* __restrict__ a, int64_t lo
 
 struct EraseRule {
     double tol;
-    const uint64_t* fk;     // (list, first position) of every group, ascending
+    const uint64_t* fk;     // (list, first position) of every group, ascending; with qstarted set:
+                            // the (list, position) of every merge, ascending
     int64_t nfk;
     const uint64_t* ev_key; // assumed erase (+1) and restart (-1) events, (list, position) ascending
     const int64_t* ev_cum;  // inclusive prefix sums of their weights
     int64_t nev;
+    // merge-position mode (two lists, the second with bit 63 set): qstarted[j] is the number of
+    // groups of either list whose first member precedes the merge at fk[j]; base_hi the number of
+    // groups of the first list
+    const int64_t* qstarted = nullptr;
+    int64_t base_hi = 0;
 };
 
 struct EraseCursor {
@@ -128,7 +134,7 @@ struct EraseCursor {
 // where the list with top bits 'lbits' starts in fk and in the events
 __device__ inline EraseCursor erase_cursor(EraseRule const& R, uint64_t lbits) {
     EraseCursor c;
-    c.fk_lo = lbits ? upper_index(R.fk, 0, R.nfk, lbits - 1) : 0;
+    c.fk_lo = (lbits && !R.qstarted) ? upper_index(R.fk, 0, R.nfk, lbits - 1) : 0;
     c.ev_lo = (R.nev > 0 && lbits) ? upper_index(R.ev_key, 0, R.nev, lbits - 1) : 0;
     c.ev_base = c.ev_lo > 0 ? R.ev_cum[c.ev_lo - 1] : 0;
     return c;
@@ -136,7 +142,11 @@ __device__ inline EraseCursor erase_cursor(EraseRule const& R, uint64_t lbits) {
 
 // threshold of a merge at position q (q > 0, list bits included)
 __device__ inline double erase_threshold(EraseRule const& R, EraseCursor const& c, uint64_t q) {
-    int64_t const started = upper_index(R.fk, c.fk_lo, R.nfk, q) - c.fk_lo;
+    int64_t started;
+    if(R.qstarted) // q is one of the merges listed in fk
+        started = R.qstarted[upper_index(R.fk, 0, R.nfk, q) - 1] - ((q >> 63) ? R.base_hi : 0);
+    else
+        started = upper_index(R.fk, c.fk_lo, R.nfk, q) - c.fk_lo;
     int64_t dead = 0;
     if(R.nev > 0) { // events of this list strictly before q
         int64_t const hi = upper_index(R.ev_key, c.ev_lo, R.nev, q - 1);

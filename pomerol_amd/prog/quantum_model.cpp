@@ -160,6 +160,8 @@ void quantum_model::compute() {
                 freqs.emplace_back(bosonic(m) + fermionic(n3), fermionic(n2), fermionic(n3));
     std::cout << "2PGF : " << freqs.size() << " freqs to evaluate\n";
     std::vector<ComplexType> chi = G4.compute(true, freqs);
+    if(chi.empty()) // a vanishing 2PGF (no parts) comes back empty: it is zero
+        chi.assign(freqs.size(), 0.0);
 
     std::cout << "Saving 2PGF " << index_comb << '\n';
     auto full = open_out("chi" + ind_str + ".dat");

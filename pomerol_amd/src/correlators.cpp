@@ -656,9 +656,11 @@ std::vector<ComplexType> TwoParticleGF::compute(bool clear, FreqVec3 const& freq
     std::vector<ComplexType> m_data;
     if(getStatus() >= Computed)
         return m_data;
-    std::vector<std::size_t> all(parts.size());
-    std::iota(all.begin(), all.end(), std::size_t(0));
-    m_data = computeParts(all, clear, freqs);
+    if(!Vanishing) { // a vanishing function returns an empty vector (TwoParticleGF.cpp:169-201)
+        std::vector<std::size_t> all(parts.size());
+        std::iota(all.begin(), all.end(), std::size_t(0));
+        m_data = computeParts(all, clear, freqs);
+    }
     setStatus(Computed);
     return m_data;
 }
@@ -766,11 +768,19 @@ void TwoParticleGFContainer::prepareAll(std::set<IndexCombination4> const& Indic
     }
 }
 
+// TwoParticleGFContainer.cpp:33-136.  split (the default): one entry per computed element (on one
+// rank the reference's communicator split gives every element to this rank).  Without split,
+// every element and image in index order: the first of an element's entries computes it, the
+// others find it computed and get an empty vector, as TwoParticleGF::compute returns then.
 std::map<IndexCombination4, std::vector<ComplexType>>
-TwoParticleGFContainer::computeAll(bool clearTerms, FreqVec3 const& freqs, MPI_Comm const& comm, bool) {
+TwoParticleGFContainer::computeAll(bool clearTerms, FreqVec3 const& freqs, MPI_Comm const& comm, bool split) {
     std::map<IndexCombination4, std::vector<ComplexType>> out;
-    for(auto& kv : NonTrivialElements)
-        out.emplace(kv.first, kv.second->compute(clearTerms, freqs, comm));
+    if(split)
+        for(auto& kv : NonTrivialElements)
+            out.emplace(kv.first, kv.second->compute(clearTerms, freqs, comm));
+    else
+        for(auto& kv : ElementsMap)
+            out.emplace(kv.first, static_cast<TwoParticleGF&>(kv.second).compute(clearTerms, freqs, comm));
     return out;
 }
 

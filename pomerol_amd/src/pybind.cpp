@@ -122,6 +122,28 @@ struct ED {
     }
 };
 
+// A TwoParticleGFContainer over an ED session (prepareAll on construction).
+struct G2Box {
+    ED& e;
+    std::unique_ptr<TwoParticleGFContainer> C;
+    std::vector<IndexCombination4> elems; // computed elements in the container's order
+
+    G2Box(ED& e, std::vector<std::tuple<int, int, int, int>> const& idx, double pr, double ct) : e(e) {
+        auto& ops = e.field_ops();
+        C.reset(new TwoParticleGFContainer(*e.info, *e.S, *e.H, *e.rho, ops));
+        C->PoleResolution = pr;
+        C->CoefficientTolerance = ct;
+        std::set<IndexCombination4> s;
+        for(auto const& t : idx)
+            s.emplace(ParticleIndex(std::get<0>(t)), ParticleIndex(std::get<1>(t)), ParticleIndex(std::get<2>(t)),
+                      ParticleIndex(std::get<3>(t)));
+        C->prepareAll(s);
+        for(auto const& kv : C->getNonTrivialElements())
+            elems.push_back(kv.first);
+    }
+    TwoParticleGF& element(std::size_t k) { return *C->getNonTrivialElements().at(elems.at(k)); }
+};
+
 FreqVec3 to_freqs(py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> zs) {
     if(zs.ndim() != 2 || zs.shape(1) != 3)
         throw std::runtime_error("frequency triplets must have shape (N, 3)");
@@ -223,6 +245,81 @@ PYBIND11_MODULE(_pomerol, m) {
             return a;
         });
 
+    py::class_<G2Box>(m, "G2Container")
+        .def(py::init([](ED& e, std::vector<std::tuple<int, int, int, int>> idx, double pr, double ct) {
+                 return new G2Box(e, idx, pr, ct);
+             }),
+             py::arg("ed"), py::arg("indices"), py::arg("pole_res") = 1e-8, py::arg("coeff_tol") = 1e-16,
+             py::keep_alive<1, 2>())
+        .def("elements", [](G2Box& b) {
+            std::vector<std::tuple<int, int, int, int>> out;
+            for(auto const& I : b.elems)
+                out.emplace_back(int(I.Index1), int(I.Index2), int(I.Index3), int(I.Index4));
+            return out;
+        })
+        .def("parts", [](G2Box& b, std::size_t k) {
+            // (S1, S2, S3, S4, permutation, n1, n2, n3, n4) per part of computed element k
+            std::vector<std::vector<int64_t>> out;
+            for(auto const& p : b.element(k).getParts()) {
+                auto bl = p.getBlocks();
+                int perm = 0;
+                for(int pp = 0; pp < 6; ++pp)
+                    if(permutations3[pp] == p.getPermutation())
+                        perm = pp;
+                out.push_back({bl[0], bl[1], bl[2], bl[3], perm, int64_t(b.e.S->getBlockSize(bl[0])),
+                               int64_t(b.e.S->getBlockSize(bl[1])), int64_t(b.e.S->getBlockSize(bl[2])),
+                               int64_t(b.e.S->getBlockSize(bl[3]))});
+            }
+            return out;
+        })
+        .def("compute_parts", [](G2Box& b, std::size_t k, std::vector<std::size_t> which,
+                                 py::array_t<std::complex<double>> zs) {
+            // TwoParticleGF::computeParts of computed element k (terms cleared after the fill)
+            TwoParticleGF& G = b.element(k);
+            FreqVec3 f = to_freqs(zs);
+            auto t0 = std::chrono::steady_clock::now();
+            std::vector<ComplexType> vals = G.computeParts(which, true, f);
+            double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            py::dict d;
+            d["values"] = to_np(vals);
+            d["seconds"] = secs;
+            std::vector<int64_t> pnr, pr_, praw;
+            for(std::size_t q : which) {
+                auto const& p = G.getParts().at(q);
+                pnr.push_back(int64_t(p.getNumNonResonantTerms()));
+                pr_.push_back(int64_t(p.getNumResonantTerms()));
+                praw.push_back(p.getNumRawTerms());
+            }
+            d["part_nonres"] = pnr;
+            d["part_res"] = pr_;
+            d["part_raw"] = praw;
+            return d;
+        })
+        .def("compute_all", [](G2Box& b, py::array_t<std::complex<double>> zs, bool clear, bool split) {
+            auto all = b.C->computeAll(clear, to_freqs(zs), MPI_COMM_WORLD, split);
+            py::dict d;
+            for(auto const& kv : all) {
+                auto const& I = kv.first;
+                d[py::make_tuple(int(I.Index1), int(I.Index2), int(I.Index3), int(I.Index4))] = to_np(kv.second);
+            }
+            return d;
+        }, py::arg("zs"), py::arg("clear") = true, py::arg("split") = true)
+        .def("value", [](G2Box& b, std::tuple<int, int, int, int> idx, std::vector<std::tuple<long, long, long>> n) {
+            // container element (computed element or permutation image) at Matsubara index triplets
+            auto& el = (*b.C)(IndexCombination4(ParticleIndex(std::get<0>(idx)), ParticleIndex(std::get<1>(idx)),
+                                                ParticleIndex(std::get<2>(idx)), ParticleIndex(std::get<3>(idx))));
+            std::vector<ComplexType> v;
+            for(auto const& t : n)
+                v.push_back(el(std::get<0>(t), std::get<1>(t), std::get<2>(t)));
+            return to_np(v);
+        })
+        .def("permutation", [](G2Box& b, std::tuple<int, int, int, int> idx) {
+            // the frequency permutation (perm, sign) that maps idx to its computed element
+            auto& el = (*b.C)(IndexCombination4(ParticleIndex(std::get<0>(idx)), ParticleIndex(std::get<1>(idx)),
+                                                ParticleIndex(std::get<2>(idx)), ParticleIndex(std::get<3>(idx))));
+            auto const& P = el.FrequenciesPermutation;
+            return py::make_tuple(std::vector<int>(P.perm.begin(), P.perm.end()), P.sign);
+        });
     py::class_<ED>(m, "ED")
         .def(py::init<py::list, double, bool>(), py::arg("terms"), py::arg("beta"), py::arg("keep_hmat") = false)
         .def_readonly("t_eigh", &ED::t_eigh)

@@ -181,17 +181,62 @@ def test_two_particle_gf(pa, oracle, name, comp):
     assert list(r["part_res"]) == list(same["part_res"])
 
 
+def hub2x2():
+    """Periodic 2x2 Hubbard cluster (t = 1, t' = -0.1, U = 4, half filling, beta = 5): a degenerate
+    spectrum whose 2PGF merges many terms and erases some by TermList's tol/(size+1) rule."""
+    from pomerol_amd.presets import hubbard_cluster
+
+    return hubbard_cluster(2, 2, 1.0, -0.1, 4.0, 2.0), 5.0
+
+
+@pytest.mark.parametrize("name,tol,budget", [("hub2x2", 1e-12, None), ("hub2x2", 1e-12, "3000"),
+                                             ("hub2x2", 1e-6, None), ("hub2x2", 1e-6, "20000"),
+                                             ("anderson2pgf", 1e-12, "500")])
+def test_2pgf_erase_rule_exact(pa, oracle, monkeypatch, name, tol, budget):
+    """Merged-term counts of every part equal the oracle's (the reference's unordered_set TermList:
+    merge on equal pole bins in visiting order, erase when |C| <= tol/(size+1)), with the GPU's
+    spectrum injected, for parts generated in one chunk and in many (POMX_G2_CHUNK_TERMS, a test
+    hook, sets the chunk budget in raw terms); values agree to 1e-12."""
+    H, beta = hub2x2() if name == "hub2x2" else MODELS[name]()
+    e = pa.ED(W.terms(H), beta)
+    idx = H.indices()
+    site = "s00" if name == "hub2x2" else "C"
+    u, d = idx.index((site, 0, 1)), idx.index((site, 0, 0))
+    zs = W.g2_grid(6, 2, beta)
+    same = inject(e, oracle.ED(len(idx), H.mode_terms(), beta)).g2(u, d, u, d, zs.ravel(), 1e-8, tol)
+    if name == "hub2x2":
+        assert same["n_erased"] > 0  # the rule is exercised
+    pa.eval_timing(1)  # counts evaluation launches: one per generated chunk with terms
+    one = e.g2(u, d, u, d, zs, 1e-8, tol)
+    _, l1 = pa.eval_timing(1)
+    if budget:
+        monkeypatch.setenv("POMX_G2_CHUNK_TERMS", budget)
+    r = e.g2(u, d, u, d, zs, 1e-8, tol)
+    if budget:
+        monkeypatch.delenv("POMX_G2_CHUNK_TERMS")
+    _, lm = pa.eval_timing(0)
+    print(f"[erase-rule] {name} tol {tol} budget {budget}: chunks evaluated {l1} -> {lm}, merged "
+          f"{sum(r['part_nonres']) + sum(r['part_res'])}, oracle erased {same['n_erased']}", flush=True)
+    if budget:
+        assert lm > l1  # several chunks
+    assert list(r["part_blocks"]) == list(same["part_blocks"])
+    assert list(r["part_nonres"]) == list(same["part_nonres"])
+    assert list(r["part_res"]) == list(same["part_res"])
+    assert rel_err(r["values"], same["values"]) < 1e-12
+    assert rel_err(r["values"], one["values"]) < 1e-12
+
+
 def test_2pgf_multichunk_matches_oracle(pa, oracle, monkeypatch):
-    """Parts generated in several chunks (a chunk budget of 20000 raw terms instead of 2^27, so the
-    config-1 parts take the multi-chunk generation and merge of the large parts) give the oracle's
-    values; the merged-term counts are printed, since across chunks the TermList size of the erase
-    rule is the chunk's group count."""
+    """Config 1's parts generated in several chunks (a chunk budget of 20000 raw terms, so its
+    large parts take the multi-chunk generation and merge) give the oracle's values and, with the
+    GPU's spectrum injected, its merged-term counts part by part."""
     H, beta = MODELS["cfg1"]()
     e, o = build(pa, oracle, H, beta)
     idx = H.indices()
     u, d = idx.index(("C", 0, 1)), idx.index(("C", 0, 0))
     zs = W.g2_grid(8, 2, beta)
     ref = o.g2(u, d, u, d, zs.ravel(), 1e-8, 1e-16)
+    same = inject(e, oracle.ED(len(idx), H.mode_terms(), beta)).g2(u, d, u, d, zs.ravel(), 1e-8, 1e-16)
     pa.eval_timing(1)  # counts evaluation launches: one per generated chunk with terms
     one = e.g2(u, d, u, d, zs, 1e-8, 1e-16)
     _, l1 = pa.eval_timing(1)
@@ -199,12 +244,12 @@ def test_2pgf_multichunk_matches_oracle(pa, oracle, monkeypatch):
     many = e.g2(u, d, u, d, zs, 1e-8, 1e-16)
     monkeypatch.delenv("POMX_G2_CHUNK_TERMS")
     _, lm = pa.eval_timing(0)
-    n1 = sum(one["part_nonres"]) + sum(one["part_res"])
-    nm = sum(many["part_nonres"]) + sum(many["part_res"])
-    print(f"[multichunk] chunks evaluated {l1} -> {lm}; merged terms {n1} -> {nm}", flush=True)
+    print(f"[multichunk] chunks evaluated {l1} -> {lm}", flush=True)
     assert lm > l1
     assert rel_err(many["values"], ref["values"]) < 1e-10
     assert rel_err(many["values"], one["values"]) < 1e-12
+    assert list(many["part_nonres"]) == list(same["part_nonres"])
+    assert list(many["part_res"]) == list(same["part_res"])
 
 
 def test_2pgf_container_crossing_images(pa):
