@@ -3,18 +3,26 @@
 Workload (config 2 of the benchmark set): single-orbital Anderson impurity with 5 seeded bath
 sites (12 spin-orbitals, 4096 states, largest N/Sz block 400), U = 3, beta = 20, Boltzmann
 cutoff 1e-12 (DensityMatrix::truncateBlocks(1e-12) and CoefficientTolerance = 1e-12,
-PoleResolution 1e-8), 2PGF <c_up c_dn c^+_up c^+_dn> on the 32 x 32 x 8 Matsubara grid
-(8192 frequency triplets), all FP64.
+PoleResolution 1e-8); the full 2PGF for all spin combinations of the impurity: a
+TwoParticleGFContainer over every (i <= j, k <= l) of {dn, up} (three computed elements
+dn-dn-dn-dn, dn-up-dn-up, up-up-up-up with 582 parts and 1.97e10 merged terms; their images and
+the six combinations that vanish by spin conservation), on the 32 x 32 x 8 Matsubara grid
+(8192 frequency triplets (W + w3, w2, w3)), all FP64.
 
-One step = one pass of the TwoParticleGF hot path (term generation, pole merge, Lehmann sum on
-all 8192 triplets, TwoParticleGFPart::compute + frequency fill) over a fixed batch of parts of
-that component.  The full component holds 206 parts and 6.5e9 merged terms (5e13 terms x
-frequencies), so a step is a deterministic sample of its parts (every --stride-th part, which
-includes the 400-state blocks); the metric counts exactly the merged terms the timed region
-processes, times 8192.
+A step is one slice of that container: the 582 (element, part) pairs are dealt to --slices (20)
+slices of equal size by the largest-first rule, and step i runs slice i mod 20 through the
+container's elements (TwoParticleGF::computeParts: term generation, pole merge, Lehmann sum on all
+8192 triplets, terms cleared after the fill).  20 consecutive steps are the whole container once
+(the default and the driver's --steps 20); the metric counts exactly the merged terms of the
+timed steps, times 8192.  --slices 1 makes every step the whole container.
 
 Setup (eigensolve, rotation, part enumeration) is outside the timed region; inputs are resident
-in HBM.  Prints one JSON line.
+in HBM.  Prints one JSON line.  Further objects: the roofline of the dominant kernel from the
+library's per-class work model and HIP-event timing, the CPU baseline (the oracle, pomerol's
+algorithm restated on the CPU, on the largest affordable parts, extrapolated to the container),
+parity of that sample against the oracle, the batched ED+G solve (config 4) against the plain
+path and the oracle, the FP64 eigensolve and GEMM rates, the bath-fit gradient (config 5)
+against the oracle's finite differences, and the 3-point susceptibility.
 """
 from __future__ import annotations
 
@@ -27,47 +35,137 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FLOP_NONRES = 28  # complex division 11, products 12, pole subtractions 3, accumulate 2
-FLOP_RES = 36
-PEAK_FP64_TFLOPS = 78.6
-# HBM bytes per evaluation-kernel dispatch of the default step (--stride 8), measured with
-# rocprofv3 --pmc: FETCH_SIZE x 2 + WRITE_SIZE x 1 (the factors are what the same counters read
-# for a copy kernel of known size); the counters cannot be read from inside this process.
-EVAL_TRAFFIC_BYTES = 1.84e9
-# Hardware FP64 FLOP per term x frequency of the default step (FMA = 2, one MFMA MOPS unit =
-# 512 FLOP), measured with rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 and
-# SQ_INSTS_VALU_MFMA_MOPS_F64 over its evaluation dispatches: the matrix-core and grouped kernels
-# evaluate fewer operations than the 28/36-FLOP direct formula the metric counts.
-EVAL_HW_FLOP_PER_TERM_FREQ = 9.28
-DEFAULT_STRIDE = 8
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector and matrix (dense)
+PEAK_HBM_GBS = 8000.0
+DEFAULT_SLICES = 20
+# HBM traffic per dispatch of the dominant kernel on this workload, measured with rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE (copy-calibrated; the command and the factors are in the file)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "g2_traffic_cfg2.json")
+EVAL_CLASSES = ("g2_sep", "g2_diag", "g2_grouped", "g2_general", "g2_reduce")
+KERNEL_OF_CLASS = {"g2_sep": "g2_eval_mfma_kernel", "g2_diag": "g2_eval_diag_kernel",
+                   "g2_grouped": "g2_eval_s_kernel", "g2_general": "g2_eval_kernel",
+                   "g2_reduce": "g2_eval_reduce_kernel"}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def select_parts(parts, stride: int):
-    """Deterministic batch: every stride-th part in TwoParticleGF::prepare order (a mix of the
-    component's small and 400-state blocks)."""
-    return list(range(0, len(parts), stride))
+def cores():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
 
-def cpu_sample(batch, counts, budget_terms: float):
-    """Parts of the batch for the CPU baseline: smallest non-empty parts first until the merged
-    term count reaches budget_terms (about 10-30 s of oracle work at 8192 frequencies)."""
-    order = sorted((c, k) for k, c in zip(batch, counts) if c > 0)
-    out, acc = [], 0
-    for c, k in order:
-        if acc + c > budget_terms and out:
-            break
-        out.append(k)
-        acc += c
-    return sorted(out)
+def cfg2_container(pa, W):
+    c = W.cfg2()
+    H = c.expr()
+    idx = H.indices()
+    u, d = idx.index(("C", 0, 1)), idx.index(("C", 0, 0))
+    e = pa.ED(W.terms(H), c.beta)
+    e.truncate_blocks(1e-12)
+    combos = [(i, j, k, l) for i in (d, u) for j in (d, u) for k in (d, u) for l in (d, u) if i <= j and k <= l]
+    box = pa.G2Container(e, combos, 1e-8, 1e-12)
+    return c, H, e, box
 
 
-def cpu_baseline(e, H, beta, zs, ijkl, parts, sample_parts):
-    """The oracle (pomerol's algorithm restated on the CPU, OpenMP over frequencies) on a
-    bounded sample of parts, fed the same spectrum as the GPU."""
+def slice_plan(box, nslices):
+    """(element, part) pairs dealt to nslices slices, largest n1 n2 n3 n4 first, each to the
+    currently lightest slice (ties: lowest slice).  Returns [{element: [parts]}]."""
+    work = []
+    for k in range(len(box.elements())):
+        for p, desc in enumerate(box.parts(k)):
+            n1, n2, n3, n4 = desc[5:9]
+            work.append((n1 * n2 * n3 * n4, k, p))
+    work.sort(key=lambda x: (-x[0], x[1], x[2]))
+    load = [0] * nslices
+    slices = [dict() for _ in range(nslices)]
+    for w, k, p in work:
+        s = min(range(nslices), key=lambda i: (load[i], i))
+        load[s] += w
+        slices[s].setdefault(k, []).append(p)
+    for sl in slices:
+        for k in sl:
+            sl[k].sort()
+    return slices, len(work)
+
+
+def run_slice(box, sl, zs, acc, counts):
+    merged = 0
+    for k, parts in sorted(sl.items()):
+        r = box.compute_parts(k, parts, zs)
+        if len(zs):
+            acc[k] += r["values"]
+        for p, a, b, raw in zip(parts, r["part_nonres"], r["part_res"], r["part_raw"]):
+            counts[(k, p)] = (int(a) + int(b), int(raw))
+            merged += int(a) + int(b)
+    return merged
+
+
+def load_traffic(kernel):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    rec = t.get("kernels", {}).get(kernel)
+    if not rec:
+        return None
+    return {"bytes_per_dispatch": rec["bytes_per_dispatch"], "fetch_bytes": rec.get("fetch_bytes"),
+            "write_bytes": rec.get("write_bytes"), "source": os.path.relpath(TRAFFIC_FILE, ROOT),
+            "measured": t.get("measured")}
+
+
+def roofline(stats, wall_s):
+    """The dominant evaluation kernel (most time) with its work-model FLOP rate, and every class."""
+    classes = {}
+    for name, s in stats.items():
+        if s["launches"] == 0:
+            continue
+        secs = s["ms"] * 1e-3
+        rec = {"launches": s["launches"], "seconds": secs, "share_of_timed": secs / wall_s, "flop": s["flop"],
+               "term_freqs": s["term_freqs"], "bytes": s["bytes"]}
+        if s["flop"] > 0 and secs > 0:
+            rec["tflops"] = s["flop"] / secs / 1e12
+            rec["frac_fp64_peak"] = rec["tflops"] / PEAK_FP64_TFLOPS
+        if name != "g2_gen" and s["term_freqs"] > 0 and secs > 0:
+            rec["term_freqs_per_s"] = s["term_freqs"] / secs
+        if s["bytes"] > 0 and secs > 0:
+            rec["algorithmic_GBs"] = s["bytes"] / secs / 1e9
+        classes[name] = rec
+    evals = [n for n in EVAL_CLASSES if n in classes and n != "g2_reduce"]
+    if not evals:
+        return None, classes
+    dom = max(evals, key=lambda n: classes[n]["seconds"])
+    s = stats[dom]
+    secs = s["ms"] * 1e-3
+    achieved = s["flop"] / secs / 1e12
+    traffic = load_traffic(KERNEL_OF_CLASS[dom])
+    out = {"bound": "mfma" if dom in ("g2_sep", "g2_diag") else "valu", "achieved": achieved,
+           "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+           "traffic": traffic["bytes_per_dispatch"] if traffic else None,
+           "kernel": KERNEL_OF_CLASS[dom], "launches": s["launches"], "avg_launch_ms": s["ms"] / s["launches"],
+           "flop_per_launch": s["flop"] / s["launches"],
+           "algorithmic_bytes_per_launch": s["bytes"] / s["launches"],
+           "flop_model": "work model of the algorithm the kernel runs (FMA 2, add/mul 1, v_rcp_f64 1): per term and "
+                         "bosonic frequency 6 NR NC (complex GEMM from three real MFMA products) + row and column "
+                         "elements (about 18 NR + 13 NC) + 2",
+           "traffic_unit": "HBM bytes per dispatch (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; copy-calibrated)",
+           "traffic_source": traffic}
+    all_eval = sum(stats[n]["ms"] for n in EVAL_CLASSES) * 1e-3
+    all_flop = sum(stats[n]["flop"] for n in EVAL_CLASSES)
+    out["all_evaluation"] = {"seconds": all_eval, "share_of_timed": all_eval / wall_s,
+                             "tflops": all_flop / all_eval / 1e12 if all_eval > 0 else None,
+                             "frac_fp64_peak": all_flop / all_eval / 1e12 / PEAK_FP64_TFLOPS if all_eval > 0 else None}
+    return out, classes
+
+
+def cpu_baseline(pa, e, H, beta, box, zs, counts, args):
+    """The oracle (pomerol's algorithm restated on the CPU: TermList merge, addMultiterm, OpenMP
+    frequency fill), fed the device spectrum, on the largest parts of each computed element with at
+    most --cpu-part-terms merged terms, on all 8192 triplets; the same parts on the device for
+    parity; and the container's CPU time extrapolated from the sample: generation by raw-term
+    count, the frequency fill by merged-term count."""
+    import numpy as np
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pomerol_oracle
 
@@ -75,11 +173,47 @@ def cpu_baseline(e, H, beta, zs, ijkl, parts, sample_parts):
     for b in range(e.n_blocks()):
         o.set_spectrum(b, e.evals(b), e.evecs(b).ravel(), e.weights(b))
     o.truncate_blocks(1e-12)
-    t0 = time.perf_counter()
-    r = o.g2(*ijkl, zs.ravel(), 1e-8, 1e-12, -1, sample_parts)
-    secs = time.perf_counter() - t0
-    terms = r["n_nonres"] + r["n_res"]
-    return terms * len(zs) / secs, secs, terms
+    sample = {}
+    for k, el in enumerate(box.elements()):
+        cand = [(m, p) for (kk, p), (m, raw) in counts.items() if kk == k and m <= args.cpu_part_terms]
+        if cand:
+            sample[k] = [max(cand)[1]]
+    gen_s = eval_s = 0.0
+    s_merged = s_raw = 0
+    max_rel = 0.0
+    counts_equal = True
+    for k, parts in sorted(sample.items()):
+        el = box.elements()[k]
+        r = o.g2(*el, zs.ravel(), 1e-8, 1e-12, -1, parts)
+        g = box.compute_parts(k, parts, zs)
+        gen_s += r["gen_seconds"]
+        eval_s += r["eval_seconds"]
+        s_merged += r["n_nonres"] + r["n_res"]
+        s_raw += sum(counts[(k, p)][1] for p in parts)
+        counts_equal = counts_equal and list(g["part_nonres"]) == list(r["part_nonres"]) and \
+            list(g["part_res"]) == list(r["part_res"])
+        max_rel = max(max_rel, float(np.abs(g["values"] - r["values"]).max() / np.abs(r["values"]).max()))
+        log(f"[bench] cpu sample {el} parts {parts}: {r['n_nonres'] + r['n_res']} terms, gen {r['gen_seconds']:.2f}s, "
+            f"fill {r['eval_seconds']:.2f}s")
+    tot_merged = sum(m for m, raw in counts.values())
+    tot_raw = sum(raw for m, raw in counts.values())
+    nf = len(zs)
+    t_serial = gen_s * tot_raw / s_raw + eval_s * tot_merged / s_merged
+    nc = cores()
+    # the reference farms parts over MPI ranks: with one rank per core, generation runs nc parts at
+    # once (ideal scaling assumed) while each fill is single-threaded; the fill timed here used nc
+    # OpenMP threads, so it already is the per-core rate times nc
+    t_farm = gen_s * tot_raw / s_raw / nc + eval_s * tot_merged / s_merged
+    return {"value": tot_merged * nf / t_farm, "unit": "terms*freqs/s", "cores": nc, "kind": "port",
+            "sample": f"oracle on {len(sample)} parts ({s_merged} merged terms, {s_raw} raw) x {nf} triplets: "
+                      f"generation {gen_s:.1f}s (one thread), fill {eval_s:.1f}s ({nc} OpenMP threads); "
+                      f"extrapolated to the container ({tot_merged} merged, {tot_raw} raw terms) by raw terms "
+                      f"(generation) and merged terms (fill), generation farmed over {nc} cores",
+            "sample_rate": s_merged * nf / (gen_s + eval_s),
+            "extrapolated_container_seconds": t_farm,
+            "extrapolated_container_seconds_serial_generation": t_serial,
+            "parity": {"parts": {str(box.elements()[k]): v for k, v in sample.items()},
+                       "max_rel_diff_vs_oracle": max_rel, "merged_counts_equal": counts_equal}}
 
 
 def run_batch(args):
@@ -126,9 +260,38 @@ def run_batch(args):
     plain = 1.0 / plain_secs
     ns = sum(len(v) for v in per_size.values())
     log(f"[bench] plain path s/solve by bath size: { {nb: round(t, 4) for nb, t in t_size.items()} }, mix {mix}")
-    return {"instances": len(cfgs), "freqs": 128, "solves_per_s": batched, "ms_per_batch": secs * 1e3,
-            "plain_sample": ns, "plain_solves_per_s": plain, "speedup_vs_plain": batched / plain,
-            "plain_s_per_solve_by_bath": t_size, "bath_mix": mix, "autotune": tune, "graph": graph}
+    out = {"instances": len(cfgs), "freqs": 128, "solves_per_s": batched, "ms_per_batch": secs * 1e3,
+           "plain_sample": ns, "plain_solves_per_s": plain, "speedup_vs_plain": batched / plain,
+           "plain_s_per_solve_by_bath": t_size, "bath_mix": mix, "autotune": tune, "graph": graph}
+    if not args.no_cpu:
+        # the oracle (Householder + implicit QR per block, the algorithm of the reference's Eigen
+        # solver; blocks over the host cores; rotation and G terms as GreensFunctionPart) on one
+        # instance of every bath size, weighted by the batch's mix; its G against the batch's
+        import numpy as np
+
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pomerol_oracle
+
+        G = b.G()
+        c_size, rel = {}, 0.0
+        for nb, ks in sorted(per_size.items()):
+            k = ks[0]
+            H = cfgs[k].expr()
+            idx = H.indices()
+            i = idx.index(("C", 0, 1))
+            t0 = time.perf_counter()
+            o = pomerol_oracle.ED(len(idx), H.mode_terms(), betas[k], "qr")
+            g = o.gf(i, i, W.g1_grid(128, betas[k]), 1e-8, 1e-8)["values"]
+            c_size[nb] = time.perf_counter() - t0
+            rel = max(rel, float(np.abs(G[k] - g).max() / np.abs(g).max()))
+        cpu_secs = sum(mix[nb] * c_size[nb] for nb in c_size)
+        out["cpu_baseline"] = {"solves_per_s": 1.0 / cpu_secs, "cores": cores(), "kind": "port",
+                               "s_per_solve_by_bath": c_size,
+                               "sample": "oracle ED (Householder + implicit QR per block, blocks over the host cores) "
+                                         "+ G on 128 frequencies, one instance per bath size, weighted by the mix",
+                               "max_rel_diff_batch_vs_oracle": rel}
+        log(f"[bench] batch cpu baseline: {out['cpu_baseline']}")
+    return out
 
 
 def run_dense(args):
@@ -282,13 +445,81 @@ def run_chi3(args):
     return out
 
 
+def run_grad(args):
+    """Bath-fit gradient (config 5): dL/d(eps_k, V_k, U) of L = sum_n |G(i w_n) - G_target(i w_n)|^2
+    over 128 frequencies for 64 Anderson instances with 5 bath sites, by the device adjoint, against
+    the forward pass on the device and against central finite differences (h = 1e-6) of the oracle on
+    the host (one instance, extrapolated to the batch).  CoefficientTolerance 0, as in the tests:
+    with a per-term cutoff the loss jumps where a term crosses it, and a difference quotient across a
+    jump is not a derivative."""
+    import numpy as np
+
+    from pomerol_amd import workloads as W
+    from pomerol_amd.adjoint import GSolver, anderson_operators, anderson_theta
+
+    cfgs, targets = W.adjoint_configs(args.grad_inst)
+    nfreq, h = 128, 1e-6
+    ops = [anderson_operators(c.n_bath) for c in cfgs]
+    theta = np.array([anderson_theta(c) for c in cfgs])
+    solver = GSolver(ops, theta, [c.beta for c in cfgs], ("C", 0, 1), nfreq, 0.0)
+    Gt = solver.forward(np.array([anderson_theta(t) for t in targets]))
+    G = solver.forward(theta)  # warm: graph captured
+    t0 = time.perf_counter()
+    for _ in range(3):
+        G = solver.forward(theta)
+    t_fwd = (time.perf_counter() - t0) / 3
+    gbar = 2.0 * (G - Gt)
+    g = solver.backward(gbar)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        g = solver.backward(gbar)
+    t_bwd = (time.perf_counter() - t0) / 3
+    out = {"instances": len(cfgs), "parameters_per_instance": theta.shape[1], "freqs": nfreq,
+           "forward_s": t_fwd, "backward_s": t_bwd, "backward_over_forward": t_bwd / t_fwd}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pomerol_oracle
+
+        from pomerol_amd.presets import anderson
+
+        c0 = cfgs[0]
+        z = W.g1_grid(nfreq, c0.beta)
+
+        def loss(th):
+            nb = c0.n_bath
+            Hx = anderson(th[:nb], th[nb:2 * nb], th[-1], th[-1] / 2)
+            idx = Hx.indices()
+            i = idx.index(("C", 0, 1))
+            o = pomerol_oracle.ED(len(idx), Hx.mode_terms(), c0.beta, "qr")
+            gx = o.gf(i, i, z, 1e-8, 0.0)["values"]
+            return float(np.sum(np.abs(gx - Gt[0]) ** 2))
+
+        t0 = time.perf_counter()
+        fd = np.empty(theta.shape[1])
+        for p in range(theta.shape[1]):
+            tp, tm = theta[0].copy(), theta[0].copy()
+            tp[p] += h
+            tm[p] -= h
+            fd[p] = (loss(tp) - loss(tm)) / (2 * h)
+        t_fd = time.perf_counter() - t0
+        err = float(np.max(np.abs(g[0] - fd) / np.maximum(1.0, np.abs(fd))))
+        out["cpu_baseline"] = {"fd_seconds_per_instance": t_fd, "fd_seconds_batch_extrapolated": t_fd * len(cfgs),
+                               "cores": cores(), "kind": "port",
+                               "sample": f"oracle central differences (h = {h}) of instance 0: "
+                                         f"{2 * theta.shape[1]} ED + G solves (Householder + implicit QR)",
+                               "max_rel_diff_adjoint_vs_fd": err}
+    log(f"[bench] grad: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--stride", type=int, default=DEFAULT_STRIDE, help="every stride-th part of the component forms a step")
-    ap.add_argument("--cpu-terms", type=float, default=1.5e6, help="merged-term budget of the CPU baseline sample")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--slices", type=int, default=DEFAULT_SLICES, help="slices of the container (a step is one)")
+    ap.add_argument("--cpu-part-terms", type=float, default=1.6e6,
+                    help="CPU baseline: per computed element, the largest part with at most this many merged terms")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--survey", action="store_true", help="print per-part term counts (no frequencies)")
     ap.add_argument("--batch-inst", type=int, default=512, help="instances of the batched ED+G solve (0: skip)")
@@ -296,11 +527,14 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=4, help="instances per bath size timed on the plain per-solve path")
     ap.add_argument("--eigh-n", type=int, default=15876, help="dense eigensolve size (0: skip)")
     ap.add_argument("--gemm-n", type=int, default=8192, help="dense GEMM size (0: skip)")
+    ap.add_argument("--grad-inst", type=int, default=64, help="bath-fit gradient instances (0: skip)")
     ap.add_argument("--dense-only", action="store_true", help="only the dense eigensolve and GEMM legs")
     ap.add_argument("--batch-only", action="store_true", help="only the batched-solve leg")
     ap.add_argument("--batch-eager", action="store_true", help="batched solves launched eagerly (no graph replay)")
     ap.add_argument("--chi3-n", type=int, default=64, help="3-point susceptibility leg: n x n pairs (0: skip)")
     ap.add_argument("--chi3-only", action="store_true", help="only the 3-point susceptibility leg")
+    ap.add_argument("--grad-only", action="store_true", help="only the bath-fit gradient leg")
+    ap.add_argument("--g2-only", action="store_true", help="only the 2PGF leg (and its CPU baseline)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path's multi-GPU "
@@ -316,73 +550,80 @@ def main():
     if args.chi3_only:
         print(json.dumps({"chi3": run_chi3(args)}), flush=True)
         return
+    if args.grad_only:
+        print(json.dumps({"grad": run_grad(args)}), flush=True)
+        return
 
     import numpy as np
 
-    import pomerol_amd
+    import pomerol_amd as pa
     from pomerol_amd import workloads as W
 
-    c = W.cfg2()
-    H = c.expr()
-    idx = H.indices()
-    u, d = idx.index(("C", 0, 1)), idx.index(("C", 0, 0))
-    ijkl = (u, d, u, d)
     t0 = time.perf_counter()
-    e = pomerol_amd.ED(W.terms(H), c.beta)
-    e.truncate_blocks(1e-12)
+    c, H, e, box = cfg2_container(pa, W)
     zs = W.g2_grid(32, 8, c.beta)
-    parts = e.g2_parts(*ijkl)
+    nslices = max(1, args.slices)
+    slices, nparts = slice_plan(box, nslices)
     log(f"[bench] setup {time.perf_counter() - t0:.2f}s: {e.n_blocks()} blocks, eigh {e.t_eigh:.3f}s, "
-        f"rot {e.t_rot:.3f}s, {len(parts)} parts, {len(zs)} frequency triplets")
+        f"rot {e.t_rot:.3f}s, elements {box.elements()}, {nparts} parts in {nslices} slices, {len(zs)} triplets")
 
     if args.survey:
-        for k, p in enumerate(parts):
-            t = time.perf_counter()
-            r = e.g2(*ijkl, zs[:0], 1e-8, 1e-12, [k], False)
-            log(f"part {k} blocks {p[:4]} perm {p[4]} dims {p[5:]} raw {r['n_raw']} merged "
-                f"{r['n_nonres'] + r['n_res']} ({time.perf_counter() - t:.2f}s)")
+        for k, el in enumerate(box.elements()):
+            for p, desc in enumerate(box.parts(k)):
+                t = time.perf_counter()
+                r = box.compute_parts(k, [p], zs[:0])
+                log(f"element {el} part {p} blocks {desc[:4]} perm {desc[4]} dims {desc[5:]} raw {r['part_raw'][0]} "
+                    f"merged {r['part_nonres'][0] + r['part_res'][0]} ({time.perf_counter() - t:.2f}s)")
         return
 
-    batch = select_parts(parts, args.stride)
-    log(f"[bench] batch: {len(batch)} parts {batch}")
-
-    def step():
-        return e.g2(*ijkl, zs, 1e-8, 1e-12, batch, False)
-
-    for _ in range(args.warmup):
-        step()
-    pomerol_amd.eval_timing(1)
+    nel = len(box.elements())
+    counts = {}
+    warm = {k: np.zeros(len(zs), complex) for k in range(nel)}
+    for i in range(args.warmup):
+        run_slice(box, slices[i % nslices], zs, warm, counts)
+    acc = {k: np.zeros(len(zs), complex) for k in range(nel)}
+    pa.eval_timing(1)
     t0 = time.perf_counter()
-    terms_nr = terms_r = 0
-    for _ in range(args.steps):
-        r = step()
-        terms_nr += r["n_nonres"]
-        terms_r += r["n_res"]
-        log(f"[bench] step: {r['n_nonres'] + r['n_res']} merged terms, {r['seconds']:.3f}s")
+    merged = 0
+    step_s = []
+    for i in range(args.steps):
+        ts = time.perf_counter()
+        m = run_slice(box, slices[i % nslices], zs, acc, counts)
+        step_s.append(time.perf_counter() - ts)
+        merged += m
+        log(f"[bench] step {i} (slice {i % nslices}): {m} merged terms, {step_s[-1]:.3f}s")
     wall = time.perf_counter() - t0
-    eval_ms, launches = pomerol_amd.eval_timing(0)
+    stats = pa.kernel_stats()
+    pa.eval_timing(0)
     nfreq = len(zs)
-    term_freqs = (terms_nr + terms_r) * nfreq
-    value = term_freqs / wall
-    flops = (FLOP_NONRES * terms_nr + FLOP_RES * terms_r) * nfreq
-    achieved = flops / (eval_ms * 1e-3) / 1e12 if eval_ms > 0 else 0.0
-    hw = (EVAL_HW_FLOP_PER_TERM_FREQ * term_freqs / (eval_ms * 1e-3) / 1e12
-          if eval_ms > 0 and args.stride == DEFAULT_STRIDE else None)
+    value = merged * nfreq / wall
+    roof, classes = roofline(stats, wall)
+    # counts of the parts no step reached (generation only), for the container totals
+    for k in range(nel):
+        rest = [p for p in range(len(box.parts(k))) if (k, p) not in counts]
+        if rest:
+            r = box.compute_parts(k, rest, zs[:0])
+            for p, a, b_, raw in zip(rest, r["part_nonres"], r["part_res"], r["part_raw"]):
+                counts[(k, p)] = (int(a) + int(b_), int(raw))
+    tot_merged = sum(m for m, raw in counts.values())
+    container = {"elements": [list(x) for x in box.elements()],
+                 "computed": [list(box.elements()[k]) for k in range(nel) if len(box.parts(k))],
+                 "parts": nparts, "merged_terms": tot_merged, "raw_terms": sum(raw for m, raw in counts.values()),
+                 "slices": nslices}
+    if args.steps >= nslices:
+        container["pass_seconds"] = sum(step_s[:nslices])
+        container["pass_term_freqs_per_s"] = tot_merged * nfreq / container["pass_seconds"]
+    log(f"[bench] container {container}")
+    log(f"[bench] kernel classes {classes}")
 
-    cpu = None
-    if not args.no_cpu:
-        counts = [a + b for a, b in zip(r["part_nonres"], r["part_res"])]
-        small = cpu_sample(batch, counts, args.cpu_terms)
-        log(f"[bench] cpu sample parts {small}")
-        v, secs, terms = cpu_baseline(e, H, c.beta, zs, ijkl, parts, small)
-        cpu = {"value": v, "unit": "terms*freqs/s", "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
-               "kind": "port",
-               "sample": f"oracle TwoParticleGFPart compute + fill of {len(small)} config-2 parts ({terms} merged "
-                         f"terms x {nfreq} triplets, {secs:.1f}s); generation serial, fill OpenMP"}
-
-    batched = run_batch(args) if args.batch_inst > 0 else None
-    dense = run_dense(args) if (args.eigh_n > 0 or args.gemm_n > 0) else None
-    chi3 = run_chi3(args) if args.chi3_n > 0 else None
+    cpu = None if args.no_cpu else cpu_baseline(pa, e, H, c.beta, box, zs, counts, args)
+    if cpu:
+        log(f"[bench] cpu baseline {cpu}")
+    g2_only = args.g2_only
+    batched = run_batch(args) if args.batch_inst > 0 and not g2_only else None
+    dense = run_dense(args) if (args.eigh_n > 0 or args.gemm_n > 0) and not g2_only else None
+    chi3 = run_chi3(args) if args.chi3_n > 0 and not g2_only else None
+    grad = run_grad(args) if args.grad_inst > 0 and not g2_only else None
 
     out = {
         "metric": "2PGF terms*freqs/s",
@@ -397,23 +638,19 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded Anderson bath parameters)",
-        "config": {"workload": "config 2: Anderson 1+5 bath, 4096 states, U=3, beta=20, 2PGF up-dn-up-dn on "
-                               "32x32x8 Matsubara triplets, cutoff 1e-12",
-                   "parts_per_step": len(batch), "merged_terms_per_step": (terms_nr + terms_r) // args.steps,
-                   "freq_triplets": nfreq},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": EVAL_TRAFFIC_BYTES if args.stride == DEFAULT_STRIDE else None,
-                     "traffic_unit": "bytes per launch (rocprofv3 PMC FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-                     "kernel": "g2_eval_mfma_kernel / g2_eval_diag_kernel / g2_eval_s_kernel / g2_eval_kernel (FP64 Lehmann sum)", "eval_ms": eval_ms,
-                     "eval_launches": launches,
-                     "flop_model": "28 per non-resonant, 36 per resonant term x frequency (direct formula)",
-                     "hw_achieved": hw, "hw_frac": hw / PEAK_FP64_TFLOPS if hw else None,
-                     "hw_flop_model": f"{EVAL_HW_FLOP_PER_TERM_FREQ} FP64 FLOP per term x frequency (PMC, vector + matrix)"},
+        "config": {"workload": "config 2: Anderson 1+5 bath, 4096 states, U=3, beta=20, cutoff 1e-12; full 2PGF "
+                               "container over all spin combinations (3 computed elements, 582 parts) on 32x32x8 "
+                               f"Matsubara triplets; a step is one of {nslices} equal slices of the container's parts, "
+                               f"{nslices} consecutive steps are the whole container",
+                   "merged_terms_per_step": merged // max(1, args.steps), "freq_triplets": nfreq,
+                   "container": container},
+        "roofline": roof,
+        "kernel_classes": classes,
         "cpu_baseline": cpu,
         "batch": batched,
         "dense": dense,
         "chi3": chi3,
+        "grad": grad,
     }
     print(json.dumps(out), flush=True)
 

@@ -293,7 +293,31 @@ int pomx_g2_part_tangent(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_
                          double* d_dacc, int64_t* h_counts, int64_t* h_raw_terms);
 
 /* ---- timing of the dominant kernel (HIP events on the context stream) ---------------- */
+/* Total time and count of the Lehmann-sum evaluations (every class below except
+ * POMX_STAT_G2_GEN) since the last call with enable >= 0; enable >= 0 also resets the
+ * statistics and switches them on (1) or off (0). */
 int pomx_eval_timing(pomx_ctx* ctx, int enable, double* h_total_ms, int64_t* h_launches);
+
+/* Per-class statistics recorded while timing is on (HIP events around each launch group on the
+ * context stream, resolved when read; no synchronisation per launch).  flop: the work model of
+ * the algorithm the class runs (FMA = 2, add/mul = 1, reciprocal estimate = 1, one f64 MFMA
+ * 16x16x4 = 2048); term_freqs: merged terms x frequencies evaluated; bytes: algorithmic HBM bytes
+ * (terms read once, outputs written once).  No counterpart in the reference. */
+enum {
+    POMX_STAT_G2_SEP = 0,     /* 2PGF, separable grids, matrix cores (g2_eval_mfma_kernel + reduce) */
+    POMX_STAT_G2_DIAG = 1,    /* 2PGF, diagonal resonant terms, matrix cores (g2_eval_diag_kernel + reduce) */
+    POMX_STAT_G2_GROUPED = 2, /* 2PGF, grouped Matsubara grids, vector (g2_eval_s_kernel) */
+    POMX_STAT_G2_GENERAL = 3, /* 2PGF, any triplets, vector (g2_eval_kernel) */
+    POMX_STAT_G2_REDUCE = 4,  /* 2PGF, per-slice partial sums into the grid (g2_eval_reduce_kernel) */
+    POMX_STAT_PAIR = 5,       /* G / chi Lehmann sums (pair_eval_kernel + reduce) */
+    POMX_STAT_G2_GEN = 6,     /* 2PGF term generation and merge of a chunk (count excluded) */
+    POMX_STAT_NCLASS = 7
+};
+typedef struct {
+    int64_t launches;
+    double ms, flop, term_freqs, bytes;
+} pomx_kernel_stat;
+int pomx_kernel_stats(pomx_ctx* ctx, pomx_kernel_stat* h_stats /* POMX_STAT_NCLASS entries */);
 
 #ifdef __cplusplus
 }

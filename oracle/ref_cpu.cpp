@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <limits>
 #include <cmath>
 #include <map>
 #include <numeric>
@@ -194,7 +195,131 @@ void jacobi_eigh(int n, std::vector<double>& A, std::vector<double>& V, std::vec
     V.swap(V2);
 }
 
-ED::ED(int n_modes_, std::vector<HTerm> H_, double beta_) : n_modes(n_modes_), beta(beta_), H(std::move(H_)) {
+// The algorithm of Eigen's SelfAdjointEigenSolver, which HamiltonianPart.cpp:98 calls (Eigen is
+// not in the reference tree; restated from Golub & Van Loan, Matrix Computations, 4th ed.,
+// algorithms 8.3.1 and 8.3.3): Householder tridiagonalisation Q^T A Q = T with Q accumulated,
+// then implicit symmetric QR steps with the Wilkinson shift on T, the Givens rotations applied to
+// Q's columns, deflating where |e_k| <= eps (|d_k| + |d_k+1|); eigenvalues ascending with their
+// columns.  O(n^3), used where Jacobi sweeps would be too slow (blocks of hundreds and more).
+void tridiag_qr_eigh(int n, std::vector<double>& A, std::vector<double>& V, std::vector<double>& w) {
+    auto a = [&](int r, int c) -> double& { return A[std::size_t(r) * n + c]; };
+    V.assign(std::size_t(n) * n, 0.0);
+    for(int i = 0; i < n; ++i)
+        V[std::size_t(i) * n + i] = 1.0;
+    std::vector<double> v(n), p(n), q(n);
+    // 1. A = Q T Q^T, Householder reflector P_k = I - beta v v^T zeroing column k below row k+1
+    for(int k = 0; k + 2 < n; ++k) {
+        int const m = n - k - 1; // trailing size
+        double sigma = 0;
+        for(int i = 1; i < m; ++i)
+            sigma += a(k + 1 + i, k) * a(k + 1 + i, k);
+        if(sigma == 0.0)
+            continue;
+        double const x0 = a(k + 1, k), mu = std::sqrt(x0 * x0 + sigma);
+        double const v0 = x0 <= 0 ? x0 - mu : -sigma / (x0 + mu);
+        double const beta = 2.0 * v0 * v0 / (sigma + v0 * v0);
+        v[0] = 1.0;
+        for(int i = 1; i < m; ++i)
+            v[i] = a(k + 1 + i, k) / v0;
+        // trailing update A22 -= v w^T + w v^T, p = beta A22 v, w = p - (beta/2)(p^T v) v
+        for(int i = 0; i < m; ++i) {
+            double t = 0;
+            for(int j = 0; j < m; ++j)
+                t += a(k + 1 + i, k + 1 + j) * v[j];
+            p[i] = beta * t;
+        }
+        double pv = 0;
+        for(int i = 0; i < m; ++i)
+            pv += p[i] * v[i];
+        for(int i = 0; i < m; ++i)
+            q[i] = p[i] - 0.5 * beta * pv * v[i];
+        for(int i = 0; i < m; ++i)
+            for(int j = 0; j < m; ++j)
+                a(k + 1 + i, k + 1 + j) -= v[i] * q[j] + q[i] * v[j];
+        a(k + 1, k) = a(k, k + 1) = mu; // P x = |x| e_1
+        for(int i = 2; i <= m; ++i)
+            a(k + i, k) = a(k, k + i) = 0.0;
+        // Q <- Q P_k (columns k+1 .. n-1)
+        for(int r = 0; r < n; ++r) {
+            double t = 0;
+            for(int j = 0; j < m; ++j)
+                t += V[std::size_t(r) * n + k + 1 + j] * v[j];
+            t *= beta;
+            for(int j = 0; j < m; ++j)
+                V[std::size_t(r) * n + k + 1 + j] -= t * v[j];
+        }
+    }
+    std::vector<double> d(n), e(n > 0 ? n : 1, 0.0);
+    for(int i = 0; i < n; ++i)
+        d[i] = a(i, i);
+    for(int i = 0; i + 1 < n; ++i)
+        e[i] = a(i + 1, i);
+    // 2. implicit symmetric QR steps with the Wilkinson shift on the unreduced block [lo, hi]
+    double const eps = std::numeric_limits<double>::epsilon();
+    int hi = n - 1, iter = 0;
+    while(hi > 0) {
+        if(std::fabs(e[hi - 1]) <= eps * (std::fabs(d[hi - 1]) + std::fabs(d[hi]))) {
+            e[hi - 1] = 0.0;
+            --hi;
+            continue;
+        }
+        int lo = hi - 1;
+        while(lo > 0 && std::fabs(e[lo - 1]) > eps * (std::fabs(d[lo - 1]) + std::fabs(d[lo])))
+            --lo;
+        if(++iter > 60 * n)
+            throw std::runtime_error("tridiag_qr_eigh: no convergence");
+        double const td = 0.5 * (d[hi - 1] - d[hi]), e2 = e[hi - 1] * e[hi - 1];
+        double const shift = d[hi] - e2 / (td + (td >= 0 ? 1.0 : -1.0) * std::hypot(td, e[hi - 1]));
+        double x = d[lo] - shift, z = e[lo];
+        for(int k = lo; k < hi; ++k) {
+            // Givens (c, s) with [c -s; s c]^T [x; z] = [r; 0]
+            double c, s;
+            if(z == 0.0) {
+                c = 1.0;
+                s = 0.0;
+            } else if(std::fabs(z) > std::fabs(x)) {
+                double const t = -x / z;
+                s = 1.0 / std::sqrt(1.0 + t * t);
+                c = s * t;
+            } else {
+                double const t = -z / x;
+                c = 1.0 / std::sqrt(1.0 + t * t);
+                s = c * t;
+            }
+            // T <- G^T T G on rows / columns k, k+1
+            double const dk = d[k], dk1 = d[k + 1], ek = e[k];
+            if(k > lo)
+                e[k - 1] = c * x - s * z;
+            d[k] = c * c * dk - 2.0 * c * s * ek + s * s * dk1;
+            d[k + 1] = s * s * dk + 2.0 * c * s * ek + c * c * dk1;
+            e[k] = c * s * (dk - dk1) + (c * c - s * s) * ek;
+            if(k + 1 < hi) {
+                x = e[k];
+                z = -s * e[k + 1];
+                e[k + 1] *= c;
+            }
+            for(int r = 0; r < n; ++r) {
+                double const vk = V[std::size_t(r) * n + k], vk1 = V[std::size_t(r) * n + k + 1];
+                V[std::size_t(r) * n + k] = c * vk - s * vk1;
+                V[std::size_t(r) * n + k + 1] = s * vk + c * vk1;
+            }
+        }
+    }
+    std::vector<int> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int x_, int y_) { return d[x_] < d[y_]; });
+    w.resize(n);
+    std::vector<double> V2(V.size());
+    for(int j = 0; j < n; ++j) {
+        w[j] = d[idx[j]];
+        for(int k = 0; k < n; ++k)
+            V2[std::size_t(k) * n + j] = V[std::size_t(k) * n + idx[j]];
+    }
+    V.swap(V2);
+}
+
+ED::ED(int n_modes_, std::vector<HTerm> H_, double beta_, int eigensolver)
+    : n_modes(n_modes_), beta(beta_), H(std::move(H_)) {
     S = partition(n_modes, H);
     int nb = int(S.blocks.size());
     hmat.resize(nb);
@@ -220,7 +345,10 @@ ED::ED(int n_modes_, std::vector<HTerm> H_, double beta_) : n_modes(n_modes_), b
             evecs[b] = {1.0};
         } else {
             std::vector<double> A = hmat[b];
-            jacobi_eigh(n, A, evecs[b], evals[b]);
+            if(eigensolver == 1)
+                tridiag_qr_eigh(n, A, evecs[b], evals[b]);
+            else
+                jacobi_eigh(n, A, evecs[b], evals[b]);
         }
     }
     // Hamiltonian.cpp:125-130

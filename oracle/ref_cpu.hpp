@@ -50,6 +50,7 @@ Sectors partition(int n_modes, std::vector<HTerm> const& H);
 
 // cyclic Jacobi; A row-major n x n (destroyed); V receives eigenvectors as columns; w ascending
 void jacobi_eigh(int n, std::vector<double>& A, std::vector<double>& V, std::vector<double>& w);
+void tridiag_qr_eigh(int n, std::vector<double>& A, std::vector<double>& V, std::vector<double>& w);
 
 // Rotated monomial operator, one entry per connected (from -> to) block pair.
 struct OpPart {
@@ -93,7 +94,8 @@ struct Chi3Result {
 
 class ED {
 public:
-    ED(int n_modes, std::vector<HTerm> H, double beta);
+    // eigensolver 0: cyclic Jacobi sweeps; 1: Householder tridiagonalisation + implicit QR
+    ED(int n_modes, std::vector<HTerm> H, double beta, int eigensolver = 0);
 
     int n_modes;
     double beta;

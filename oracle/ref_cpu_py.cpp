@@ -46,8 +46,27 @@ static std::vector<cplx> zvec(py::array_t<std::complex<double>, py::array::c_sty
 PYBIND11_MODULE(pomerol_oracle, m) {
     m.doc() = "CPU restatement of pomerol's full-ED path (test oracle, not product)";
     py::class_<ED>(m, "ED")
-        .def(py::init([](int n_modes, py::list terms, double beta) { return new ED(n_modes, to_terms(terms), beta); }),
-             py::arg("n_modes"), py::arg("terms"), py::arg("beta"))
+        .def(py::init([](int n_modes, py::list terms, double beta, std::string const& eigensolver) {
+                 if(eigensolver != "jacobi" && eigensolver != "qr")
+                     throw std::runtime_error("eigensolver must be 'jacobi' or 'qr'");
+                 return new ED(n_modes, to_terms(terms), beta, eigensolver == "qr" ? 1 : 0);
+             }),
+             py::arg("n_modes"), py::arg("terms"), py::arg("beta"), py::arg("eigensolver") = "jacobi")
+        .def_static("eigh", [](py::array_t<double, py::array::c_style | py::array::forcecast> A, std::string const& how) {
+            // one dense symmetric eigensolve (tests of the oracle's eigensolvers; CPU timings)
+            if(A.ndim() != 2 || A.shape(0) != A.shape(1))
+                throw std::runtime_error("eigh: square matrix expected");
+            int n = int(A.shape(0));
+            std::vector<double> a(A.data(), A.data() + A.size()), V, w;
+            {
+                py::gil_scoped_release rel;
+                if(how == "qr")
+                    tridiag_qr_eigh(n, a, V, w);
+                else
+                    jacobi_eigh(n, a, V, w);
+            }
+            return py::make_tuple(arr(w), mat(V, n, n));
+        }, py::arg("A"), py::arg("how") = "qr")
         .def_readonly("E0", &ED::E0)
         .def_readonly("Z", &ED::Z)
         .def("n_blocks", [](ED const& e) { return e.S.blocks.size(); })

@@ -22,6 +22,6 @@ except ImportError as exc:  # pragma: no cover - exercised only on broken instal
         f"{exc}"
     ) from exc
 
-from ._pomerol import ED, Batch, device_ok, eval_timing  # noqa: E402
+from ._pomerol import ED, Batch, G2Container, device_ok, eval_timing, kernel_stats  # noqa: E402
 
-__all__ = ["ED", "Batch", "device_ok", "eval_timing", "POMX_LIB", "API_LIB", "LIB_DIR"]
+__all__ = ["ED", "Batch", "G2Container", "device_ok", "eval_timing", "kernel_stats", "POMX_LIB", "API_LIB", "LIB_DIR"]

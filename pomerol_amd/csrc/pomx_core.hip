@@ -95,6 +95,12 @@ void pomx_ctx_destroy(pomx_ctx* ctx) {
         hipFree(kv.first);
     hipEventDestroy(ctx->ev0);
     hipEventDestroy(ctx->ev1);
+    for(auto const& r : ctx->stat_recs) {
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
+    for(hipEvent_t e : ctx->stat_pool)
+        hipEventDestroy(e);
     hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -322,31 +328,92 @@ int pomx_exclusive_scan(pomx_ctx* ctx, int64_t n, const int64_t* in, int64_t* ou
     return POMX_OK;
 }
 
-void eval_timer_begin(pomx_ctx* ctx) {
-    if(ctx->timing)
-        hipEventRecord(ctx->ev0, ctx->stream);
-}
-void eval_timer_end(pomx_ctx* ctx) {
-    if(!ctx->timing)
+namespace {
+
+// Fold the pending event pairs into the per-class totals (waits for the last one).
+void stat_resolve(pomx_ctx* ctx) {
+    if(ctx->stat_recs.empty())
         return;
-    hipEventRecord(ctx->ev1, ctx->stream);
-    hipEventSynchronize(ctx->ev1);
-    float ms = 0;
-    hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-    ctx->eval_ms += ms;
-    ctx->eval_launches += 1;
+    hipEventSynchronize(ctx->stat_recs.back().b);
+    for(auto const& r : ctx->stat_recs) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, r.a, r.b);
+        pomx_kernel_stat& s = ctx->stats[r.cls];
+        s.launches += 1;
+        s.ms += ms;
+        s.flop += r.flop;
+        s.term_freqs += r.term_freqs;
+        s.bytes += r.bytes;
+        ctx->stat_pool.push_back(r.a);
+        ctx->stat_pool.push_back(r.b);
+    }
+    ctx->stat_recs.clear();
+}
+
+hipEvent_t stat_event(pomx_ctx* ctx) {
+    if(ctx->stat_pool.empty()) {
+        hipEvent_t e = nullptr;
+        if(hipEventCreate(&e) != hipSuccess)
+            return nullptr;
+        return e;
+    }
+    hipEvent_t e = ctx->stat_pool.back();
+    ctx->stat_pool.pop_back();
+    return e;
+}
+
+} // namespace
+
+int stat_begin(pomx_ctx* ctx, int cls) {
+    if(!ctx->timing || cls < 0 || cls >= POMX_STAT_NCLASS)
+        return -1;
+    if(ctx->stat_recs.size() >= 4096)
+        stat_resolve(ctx);
+    hipEvent_t a = stat_event(ctx), b = stat_event(ctx);
+    if(!a || !b)
+        return -1;
+    hipEventRecord(a, ctx->stream);
+    ctx->stat_recs.push_back(pomx_ctx::StatRec{cls, a, b, 0.0, 0.0, 0.0});
+    return int(ctx->stat_recs.size()) - 1;
+}
+
+void stat_end(pomx_ctx* ctx, int handle, double flop, double term_freqs, double bytes) {
+    if(handle < 0 || handle >= int(ctx->stat_recs.size()))
+        return;
+    auto& r = ctx->stat_recs[std::size_t(handle)];
+    hipEventRecord(r.b, ctx->stream);
+    r.flop = flop;
+    r.term_freqs = term_freqs;
+    r.bytes = bytes;
 }
 
 extern "C" int pomx_eval_timing(pomx_ctx* ctx, int enable, double* h_total_ms, int64_t* h_launches) {
     POMX_REQUIRE(ctx, ctx, "null context");
+    stat_resolve(ctx);
+    double ms = 0;
+    int64_t n = 0;
+    for(int c = 0; c < POMX_STAT_NCLASS; ++c)
+        if(c != POMX_STAT_G2_GEN && c != POMX_STAT_G2_REDUCE) {
+            ms += ctx->stats[c].ms;
+            n += ctx->stats[c].launches;
+        }
+    ms += ctx->stats[POMX_STAT_G2_REDUCE].ms;
     if(h_total_ms)
-        *h_total_ms = ctx->eval_ms;
+        *h_total_ms = ms;
     if(h_launches)
-        *h_launches = ctx->eval_launches;
+        *h_launches = n;
     if(enable >= 0) {
         ctx->timing = enable != 0;
-        ctx->eval_ms = 0;
-        ctx->eval_launches = 0;
+        for(auto& st : ctx->stats)
+            st = pomx_kernel_stat{};
     }
+    return POMX_OK;
+}
+
+extern "C" int pomx_kernel_stats(pomx_ctx* ctx, pomx_kernel_stat* h_stats) {
+    POMX_REQUIRE(ctx, ctx && h_stats, "pomx_kernel_stats: bad arguments");
+    stat_resolve(ctx);
+    for(int c = 0; c < POMX_STAT_NCLASS; ++c)
+        h_stats[c] = ctx->stats[c];
     return POMX_OK;
 }

@@ -1820,6 +1820,43 @@ int launch_diag(pomx_ctx* ctx, TermSpan const& s, int kind, int perm, SepGrid co
     return POMX_OK;
 }
 
+// ---- work model of the evaluation kernels (FLOP: FMA 2, add / mul 1, v_rcp_f64 1; rcp_nr 5) ----
+// Separable grids (g2_eval_mfma_kernel), per term and bosonic frequency: the complex GEMM over the
+// NR x NC tile from three real products (6 NR NC, matrix cores), one row element per row (two
+// factors 17, one 10, plus the a1 sum), one column element per column (one factor 11, two 18,
+// resonant 16 with K = Rc or Nc / E folded in, plus the b2, b3 sums), and per term the pole sums
+// and products (2) and, resonant, K's reciprocal (11).
+double sep_flop_per_term_b(int kind, int perm, int NR, int NC) {
+    SepPlan const pl = sep_plan_of(kind, perm);
+    double const r_el = pl.nrow == 2 ? 17.0 : 10.0;
+    double const c_el = kind >= 2 ? 16.0 : (pl.ncol == 2 ? 18.0 : 11.0);
+    double const t_el = 2.0 + (kind >= 2 ? 11.0 : 0.0);
+    return 6.0 * NR * NC + NR * (r_el + 1.0) + NC * (c_el + 2.0) + t_el;
+}
+// Diagonal resonant terms (g2_eval_diag_kernel), per term, bosonic frequency and 16-row tile: the
+// 16 x (16 + NC - 1) band product (6 per entry), 16 row elements (17 + 1), 16 + NC - 1 diagonal
+// values (|E|^2, Nc / E: 11, plus 2 sums) and 3 per term.
+double diag_flop_per_term_tile(int NC) {
+    double const ne = 16.0 + NC - 1.0;
+    return 6.0 * 16.0 * ne + 16.0 * 18.0 + ne * 13.0 + 3.0;
+}
+// Grouped grids (g2_eval_s_kernel), per term and frequency: 54 per four frequencies (|V|^2, a
+// quarter of one reciprocal, K conj(V) / |V|^2 accumulated), plus K = Kc / G once per term and
+// thread of kFPT frequencies (20 kind 0, 22 kinds 1 and 2/3 with E varying, 27 with E fixed).
+double grouped_flop_per_term_freq(int kind, int slot) {
+    bool const evar = (kind == 2 || kind == 3) && slot == 1;
+    double const per_thread = kind == 0 ? 20.0 : (kind == 1 || evar) ? 22.0 : 27.0;
+    return 54.0 / 4.0 + per_thread / kFPT;
+}
+// Any triplets (g2_eval_kernel): purely imaginary triplets 83 per four frequencies (one shared
+// reciprocal); otherwise the direct formula as written, 28 / 31 (non-resonant z2 / z4) and 32
+// (resonant) per term and frequency.
+double general_flop_per_term_freq(int kind, bool imag) {
+    if(imag)
+        return 83.0 / 4.0;
+    return kind == 0 ? 28.0 : kind == 1 ? 31.0 : 32.0;
+}
+
 int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res, int nfreq, const double* d_freq,
                    double* d_acc) {
     int64_t total = 0;
@@ -1878,33 +1915,60 @@ int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res,
     int nslices = 1;
     for(int k = 0; k < 4; ++k)
         nslices = std::max<int64_t>(nslices, (spans[k].count + slice - 1) / slice);
-    WsBuf<double> partial(ctx, size_t(nslices) * nfreq * 2);
-    if((rc = pomx_fill_zero(ctx, partial.p, sizeof(double) * size_t(nslices) * nfreq * 2)))
-        return rc;
-    eval_timer_begin(ctx);
+    bool any_vector = false; // some kind takes the vector kernels, which need zeroed partial sums
+    for(int k = 0; k < 4; ++k)
+        if(spans[k].count && !(sep.ok && sep_applies(k, perm, sep.sum_ok)) && !diag_applies(k, perm, sep))
+            any_vector = true;
+    WsBuf<double> partial;
+    if(any_vector) {
+        partial = WsBuf<double>(ctx, size_t(nslices) * nfreq * 2);
+        if(!partial.p)
+            return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+        if((rc = pomx_fill_zero(ctx, partial.p, sizeof(double) * size_t(nslices) * nfreq * 2)))
+            return rc;
+    }
     for(int k = 0; k < 4; ++k) {
         TermSpan const& s = spans[k];
         if(!s.count)
             continue;
+        double const cnt = double(s.count), tbytes = 8.0 * (k < 2 ? 4 : 5); // a term's fields, read once
         dim3 grid(ftiles, unsigned((s.count + slice - 1) / slice));
         if(sep.ok && sep_applies(k, perm, sep.sum_ok)) {
+            double const nout = double(sep.NB) * sep.NR * sep.NC;
+            int const h = stat_begin(ctx, POMX_STAT_G2_SEP);
             if((rc = launch_sep(ctx, s, k, perm, sep, d_y, res, d_acc)))
                 return rc;
+            stat_end(ctx, h, cnt * sep.NB * sep_flop_per_term_b(k, perm, sep.NR, sep.NC), cnt * nout,
+                     cnt * tbytes + 32.0 * nout);
         } else if(diag_applies(k, perm, sep)) {
+            double const nout = double(sep.NB) * sep.NR * sep.NC;
+            int const h = stat_begin(ctx, POMX_STAT_G2_DIAG);
             if((rc = launch_diag(ctx, s, k, perm, sep, d_y, res, d_acc)))
                 return rc;
-        } else if(grouped && g2_structured(k, slot2))
-            launch_kind_s<false>(ctx, k, slot2, grid, s, slice, nfreq, stride, d_freq, perm, res, partial);
-        else if(strided && g2_structured(k, slot1))
-            launch_kind_s<true>(ctx, k, slot1, grid, s, slice, nfreq, stride, d_freq, perm, res, partial);
-        else if(has_real)
-            launch_kind<false>(ctx, k, grid, s, slice, nfreq, d_freq, perm, res, partial);
-        else
-            launch_kind<true>(ctx, k, grid, s, slice, nfreq, d_freq, perm, res, partial);
+            stat_end(ctx, h, cnt * sep.NB * ((sep.NR + 15) / 16) * diag_flop_per_term_tile(sep.NC), cnt * nout,
+                     cnt * tbytes + 32.0 * nout);
+        } else {
+            bool const g = grouped && g2_structured(k, slot2), st = !g && strided && g2_structured(k, slot1);
+            double const fpt = (g || st) ? grouped_flop_per_term_freq(k, g ? slot2 : slot1)
+                                         : general_flop_per_term_freq(k, !has_real);
+            int const h = stat_begin(ctx, (g || st) ? POMX_STAT_G2_GROUPED : POMX_STAT_G2_GENERAL);
+            if(g)
+                launch_kind_s<false>(ctx, k, slot2, grid, s, slice, nfreq, stride, d_freq, perm, res, partial);
+            else if(st)
+                launch_kind_s<true>(ctx, k, slot1, grid, s, slice, nfreq, stride, d_freq, perm, res, partial);
+            else if(has_real)
+                launch_kind<false>(ctx, k, grid, s, slice, nfreq, d_freq, perm, res, partial);
+            else
+                launch_kind<true>(ctx, k, grid, s, slice, nfreq, d_freq, perm, res, partial);
+            stat_end(ctx, h, cnt * nfreq * fpt, cnt * nfreq, cnt * tbytes);
+        }
     }
-    g2_eval_reduce_kernel<<<grid_for(nfreq), kBlock, 0, ctx->stream>>>(nfreq, nslices, partial, d_acc);
-    POMX_CHECK_LAUNCH(ctx);
-    eval_timer_end(ctx);
+    if(any_vector) { // the vector kernels' per-slice partial sums into the grid
+        int const h = stat_begin(ctx, POMX_STAT_G2_REDUCE);
+        g2_eval_reduce_kernel<<<grid_for(nfreq), kBlock, 0, ctx->stream>>>(nfreq, nslices, partial, d_acc);
+        POMX_CHECK_LAUNCH(ctx);
+        stat_end(ctx, h, 2.0 * nslices * nfreq, 0.0, 16.0 * nslices * nfreq + 32.0 * nfreq);
+    }
     return POMX_OK;
 }
 
@@ -2385,8 +2449,12 @@ int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* 
     std::vector<std::pair<const uint64_t*, int64_t>> fks;
     for(auto const& ch : chunks) {
         G2Chunk c;
+        // statistics: raw terms in the term_freqs field; bytes = the terms' keys and values written
+        // once, sorted (read and written once) and grouped (keys read once), 48 per term
+        int const hg = stat_begin(ctx, POMX_STAT_G2_GEN);
         if((rc = g2_generate_chunk(ctx, D, T, hpairs, ch.pb, ch.pe, ch.n, c)))
             return rc;
+        stat_end(ctx, hg, 0.0, double(ch.n), 48.0 * double(ch.n));
         int64_t const merges = c.n - c.ng;
         if(multi && positions_ok) { // every group's first position, for the thresholds of other chunks' merges
             fk_store.emplace_back(ctx, c.ng);

@@ -18,10 +18,17 @@ struct pomx_ctx {
     // caching pool for temporaries: size -> free blocks
     std::multimap<size_t, void*> free_blocks;
     std::map<void*, size_t> live_blocks;
-    // timing of the Lehmann evaluation kernels (the dominant kernel)
+    // timing of the Lehmann evaluation kernels (the dominant kernel) and the term generation:
+    // event pairs recorded per launch group, resolved when read (or when many are pending)
     bool timing = false;
-    double eval_ms = 0.0;
-    int64_t eval_launches = 0;
+    struct StatRec {
+        int cls;
+        hipEvent_t a, b;
+        double flop, term_freqs, bytes;
+    };
+    std::vector<StatRec> stat_recs;
+    std::vector<hipEvent_t> stat_pool;
+    pomx_kernel_stat stats[POMX_STAT_NCLASS] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -174,9 +181,10 @@ using ErasePass = std::function<int(EraseRule const&, bool emit, int64_t* evn, u
 int erase_fixed_point(pomx_ctx* ctx, int64_t ng, EraseRule rule, ErasePass const& pass, EraseEvents& ev,
                       const char* who);
 
-// Evaluation-kernel timing hooks (HIP events on the context stream).
-void eval_timer_begin(pomx_ctx* ctx);
-void eval_timer_end(pomx_ctx* ctx);
+// Kernel statistics hooks (HIP events on the context stream, when timing is on): stat_begin
+// returns a handle (-1 when off) that stat_end closes with the launch group's work model.
+int stat_begin(pomx_ctx* ctx, int cls);
+void stat_end(pomx_ctx* ctx, int handle, double flop, double term_freqs, double bytes);
 
 constexpr int kBlock = 256;
 inline unsigned grid_for(int64_t n, int per_block = kBlock) {

@@ -578,12 +578,14 @@ extern "C" int pomx_pair_eval(pomx_ctx* ctx, int kind, int64_t nterms, const dou
     slice = (slice + kEvalChunk - 1) / kEvalChunk * kEvalChunk;
     int nslices = int((nterms + slice - 1) / slice);
     WsBuf<double> partial(ctx, size_t(nslices) * nz * 2);
-    eval_timer_begin(ctx);
+    int const h = stat_begin(ctx, POMX_STAT_PAIR);
     pair_eval_kernel<<<dim3(ftiles, nslices), 256, 0, ctx->stream>>>(nterms, slice, d_residue, d_pole, nz, d_z, partial);
     pair_eval_reduce_kernel<<<grid_for(nz), kBlock, 0, ctx->stream>>>(nz, nslices, kind == POMX_KIND_GF ? 1.0 : -1.0,
                                                                        partial, d_acc);
     POMX_CHECK_LAUNCH(ctx);
-    eval_timer_end(ctx);
+    // 14 FLOP per term and frequency (R / (z - P)); residue and pole read once
+    stat_end(ctx, h, 14.0 * double(nterms) * nz, double(nterms) * nz, 16.0 * double(nterms) + 32.0 * nz);
     POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return POMX_OK;
 }
+

@@ -178,6 +178,24 @@ PYBIND11_MODULE(_pomerol, m) {
         detail::check(pomx_eval_timing(detail::gpu(), enable, &ms, &n), "timing");
         return py::make_tuple(ms, n);
     }, py::arg("enable") = -1);
+    m.def("kernel_stats", []() {
+        // per-class statistics since eval_timing(1): launches, ms, flop, term_freqs, bytes
+        pomx_kernel_stat st[POMX_STAT_NCLASS];
+        detail::check(pomx_kernel_stats(detail::gpu(), st), "kernel statistics");
+        static char const* names[POMX_STAT_NCLASS] = {"g2_sep", "g2_diag", "g2_grouped", "g2_general", "g2_reduce",
+                                                      "pair", "g2_gen"};
+        py::dict d;
+        for(int c = 0; c < POMX_STAT_NCLASS; ++c) {
+            py::dict x;
+            x["launches"] = st[c].launches;
+            x["ms"] = st[c].ms;
+            x["flop"] = st[c].flop;
+            x["term_freqs"] = st[c].term_freqs;
+            x["bytes"] = st[c].bytes;
+            d[names[c]] = x;
+        }
+        return d;
+    });
     m.def("device_ok", []() {
         try {
             detail::gpu();

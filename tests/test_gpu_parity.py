@@ -252,20 +252,39 @@ def test_2pgf_multichunk_matches_oracle(pa, oracle, monkeypatch):
     assert list(many["part_res"]) == list(same["part_res"])
 
 
-def test_2pgf_container_crossing_images(pa):
-    """IndexContainer4 images (IndexContainer4.hpp:190-237) equal the directly computed elements:
-    fermionic antisymmetry chi_{jikl}(w1,w2;w3) = -chi_{ijkl}(w2,w1;w3) etc."""
-    H, beta = anderson2pgf_model()
+@pytest.mark.parametrize("name", ["anderson2pgf", "anderson_test"])
+def test_2pgf_container_images_match_oracle(pa, oracle, name):
+    """TwoParticleGFContainer over all spin combinations of the impurity: every element the
+    container returns, computed ones and their IndexContainer4 images (permutations4[1], [6], [7]
+    with their frequency permutations and signs, IndexContainer4.hpp:190-237), equals the oracle's
+    direct computation of that index combination at the same Matsubara triplets; computeAll
+    returns the computed elements on the grid (TwoParticleGFContainer.cpp:33-136)."""
+    H, beta = MODELS[name]()
     e = pa.ED(W.terms(H), beta)
+    o = oracle.ED(len(H.indices()), H.mode_terms(), beta)
     idx = H.indices()
     u, d = idx.index(("C", 0, 1)), idx.index(("C", 0, 0))
-    ns = [(a, b, c) for a in range(-2, 2) for b in range(-2, 2) for c in range(-2, 2)]
-    images = [(d, u, d, u), (u, d, d, u), (d, u, u, d), (u, d, u, d)]
-    out = e.g2_container([(d, u, d, u)], ns, 1e-8, 1e-16, images)
+    combos = [(i, j, k, l) for i in (d, u) for j in (d, u) for k in (d, u) for l in (d, u) if i <= j and k <= l]
+    box = pa.G2Container(e, combos, 1e-8, 1e-16)
+    ns = [(a, b, c) for a in range(-3, 2) for b in range(-2, 2) for c in range(-2, 3)]
     zs = np.array([[fermionic(a, beta), fermionic(b, beta), fermionic(c, beta)] for a, b, c in ns])
-    for ijkl in images:
-        direct = e.g2(*ijkl, zs, 1e-8, 1e-16)["values"]
-        assert np.abs(out[ijkl] - direct).max() < 1e-11 * max(1.0, np.abs(direct).max()), ijkl
+    grid = W.g2_grid(4, 2, beta)
+    full = box.compute_all(grid, False)
+    assert sorted(full) == sorted(box.elements())
+    every = [(i, j, k, l) for i in (d, u) for j in (d, u) for k in (d, u) for l in (d, u)]
+    for ijkl in every:
+        both = o.g2(*ijkl, np.concatenate([zs, grid]).ravel(), 1e-8, 1e-16)["values"]
+        direct, ref = both[:len(zs)], both[len(zs):]
+        got = box.value(ijkl, ns)
+        assert np.abs(got - direct).max() <= 1e-10 * max(1.0, np.abs(direct).max()), ijkl
+        if ijkl in full:
+            if len(full[ijkl]) == 0:  # a vanishing element (no parts) comes back empty, as in the reference
+                assert np.abs(ref).max() == 0, ijkl
+            else:
+                assert rel_err(full[ijkl], ref) < 1e-10, ijkl
+    # an image is its computed element with permuted frequencies and a sign
+    perm, sign = box.permutation((u, d, d, u))
+    assert sorted(perm[:3] + [perm[3]]) == [0, 1, 2, 3] and sign in (1, -1)
 
 
 @pytest.mark.parametrize("name", ["anderson_test", "cfg1"])
