@@ -23,6 +23,7 @@ struct BlockStore {
 };
 
 class HamiltonianPart : public ComputableObject {
+    std::shared_ptr<FermionOperator const> owned; // the operator, when the part was given a temporary
     FermionOperator const* HOp;
     StatesClassification const& S;
     BlockNumber Block;
@@ -35,6 +36,9 @@ class HamiltonianPart : public ComputableObject {
 
 public:
     HamiltonianPart(FermionOperator const& HOp, StatesClassification const& S, BlockNumber Block);
+    /// From a temporary operator (e.g. LOperatorType<RealType>(expr, HS.getFullHilbertSpace())): the
+    /// part keeps its own copy.
+    HamiltonianPart(FermionOperator&& HOp, StatesClassification const& S, BlockNumber Block);
 
     /// Assemble the dense block on the device (HamiltonianPart.cpp:56-76).
     void prepare();
@@ -49,7 +53,7 @@ public:
     RealType getMinimumEigenvalue() const;
     /// The block before compute(), its eigenvectors (columns) after.
     template <bool C> MatrixType<C> const& getMatrix() const;
-    template <bool C> std::vector<MelemType<C>> getEigenState(InnerQuantumState state) const;
+    template <bool C> VectorType<C> getEigenState(InnerQuantumState state) const;
     /// Not available on the GPU path (eigenvector truncation); throws std::logic_error.
     bool reduce(RealType Cutoff);
 
@@ -57,7 +61,18 @@ public:
     double const* deviceMatrix() const { return store->mat.data() + mat_off; }
     double const* deviceEigenvalues() const { return store->evals.data() + val_off; }
     void invalidateHostCache() const { host_matrix.reset(); }
+
+    /// The block (before compute()) or its eigenvectors (after), as HamiltonianPart.hpp:132-139.
+    friend std::ostream& operator<<(std::ostream& os, HamiltonianPart const& part);
 };
+
+template <> MatrixType<false> const& HamiltonianPart::getMatrix<false>() const;
+template <> MatrixType<true> const& HamiltonianPart::getMatrix<true>() const;
+template <> VectorType<false> HamiltonianPart::getEigenState<false>(InnerQuantumState state) const;
+template <> VectorType<true> HamiltonianPart::getEigenState<true>(InnerQuantumState state) const;
+inline std::ostream& operator<<(std::ostream& os, HamiltonianPart const& part) {
+    return os << part.getMatrix<false>() << '\n';
+}
 
 class Hamiltonian : public ComputableObject {
     StatesClassification const& S;

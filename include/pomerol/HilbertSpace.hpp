@@ -127,6 +127,26 @@ IndexClassification<IndexTypes...> MakeIndexClassification(Operators::expression
     return IndexClassification<IndexTypes...>(e);
 }
 
+/// The full Fock space of a set of indices (the role of libcommute's hilbert_space): the index
+/// classification that maps an expression's operators to modes.
+template <typename... IndexTypes> class FullHilbertSpace {
+    IndexClassification<IndexTypes...> const* info_;
+
+public:
+    explicit FullHilbertSpace(IndexClassification<IndexTypes...> const& info) : info_(&info) {}
+    IndexClassification<IndexTypes...> const& getIndexInfo() const { return *info_; }
+    QuantumState dim() const { return QuantumState(1) << info_->getIndexSize(); }
+};
+
+/// An expression as a linear operator on a full Fock space (the role of libcommute's loperator,
+/// Misc.hpp's LOperatorType<ScalarType>); real coefficients on the GPU path.
+template <typename ScalarType> class LOperatorType : public FermionOperator {
+public:
+    template <typename S2, typename... IndexTypes>
+    LOperatorType(Operators::expression<S2, IndexTypes...> const& e, FullHilbertSpace<IndexTypes...> const& hs)
+        : FermionOperator(hs.getIndexInfo().toOperator(e)) {}
+};
+
 template <typename... IndexTypes> class HilbertSpace : public ComputableObject {
     IndexClassification<IndexTypes...> const& IndexInfo;
     FermionOperator HOp;
@@ -148,6 +168,7 @@ public:
     }
 
     IndexClassification<IndexTypes...> const& getIndexInfo() const { return IndexInfo; }
+    FullHilbertSpace<IndexTypes...> getFullHilbertSpace() const { return FullHilbertSpace<IndexTypes...>(IndexInfo); }
     int getNumModes() const { return int(IndexInfo.getIndexSize()); }
     QuantumState getDim() const { return QuantumState(1) << IndexInfo.getIndexSize(); }
     FermionOperator const& getHamiltonianOperator() const { return HOp; }

@@ -76,6 +76,23 @@ inline RealExpr Magnetization(std::string const& Label, RealType H, unsigned sho
     return res;
 }
 
+/// Delta c^+_{1 o1 s1} c^+_{2 o2 s2} + h.c. (LatticePresets.cpp:159-211, real amplitudes)
+inline RealExpr Pairing(std::string const& Label1, std::string const& Label2, RealType Delta, unsigned short Orbital1,
+                        unsigned short Orbital2, spin Spin1, spin Spin2) {
+    return (Delta * c_dag(Label1, Orbital1, Spin1) * c_dag(Label2, Orbital2, Spin2)) + Operators::hc;
+}
+inline RealExpr Pairing(std::string const& Label1, std::string const& Label2, RealType Delta, unsigned short Orbital1,
+                        unsigned short Orbital2) {
+    return Pairing(Label1, Label2, Delta, Orbital1, Orbital2, up, down);
+}
+/// Delta c^+_{o up} c^+_{o dn} + h.c. on every orbital of one site
+inline RealExpr Pairing(std::string const& Label, RealType Delta, unsigned short NOrbitals = 1) {
+    RealExpr res;
+    for(unsigned short o = 0; o < NOrbitals; ++o)
+        res += Pairing(Label, Label, Delta, o, o);
+    return res;
+}
+
 /// U n_{o1 s1} n_{o2 s2}
 inline RealExpr NupNdown(std::string const& Label, RealType U, unsigned short Orbital1, unsigned short Orbital2,
                          spin Spin1, spin Spin2) {

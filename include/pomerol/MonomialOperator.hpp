@@ -34,6 +34,7 @@ struct BlocksBimap {
 };
 
 class MonomialOperatorPart : public ComputableObject {
+    std::shared_ptr<FermionOperator const> owned; // the operator, when the part was given a temporary
     FermionOperator const* MOp;
     StatesClassification const& S;
     HamiltonianPart const& HFrom;
@@ -46,6 +47,9 @@ class MonomialOperatorPart : public ComputableObject {
 
 public:
     MonomialOperatorPart(FermionOperator const& MOp, StatesClassification const& S, HamiltonianPart const& HFrom,
+                         HamiltonianPart const& HTo);
+    /// From a temporary operator: the part keeps its own copy.
+    MonomialOperatorPart(FermionOperator&& MOp, StatesClassification const& S, HamiltonianPart const& HFrom,
                          HamiltonianPart const& HTo);
 
     /// Rotate this part alone (MonomialOperatorPart.cpp:32-92).
@@ -62,7 +66,16 @@ public:
     template <bool C> MatrixType<C> const& getColMajorValue() const { return getRowMajorValue<C>(); }
     /// Device view of the rotated block: element (r, c) at base[r * rs + c * cs].
     pomx_view deviceView() const { return pomx_view{store->data() + off, rs, cs}; }
+
+    /// The rotated block, as MonomialOperatorPart.hpp:130-136 prints it.
+    friend std::ostream& operator<<(std::ostream& os, MonomialOperatorPart const& part);
 };
+
+template <> MatrixType<false> const& MonomialOperatorPart::getRowMajorValue<false>() const;
+template <> MatrixType<true> const& MonomialOperatorPart::getRowMajorValue<true>() const;
+inline std::ostream& operator<<(std::ostream& os, MonomialOperatorPart const& part) {
+    return os << part.getRowMajorValue<false>() << '\n';
+}
 
 class MonomialOperator : public ComputableObject {
 protected:

@@ -2,9 +2,11 @@
 // which re-exports libcommute's expression, c, c_dag, n and hc).
 //
 // An expression is a sum of monomials (ordered products of c / c^dag, the rightmost acting first)
-// with real or complex coefficients.  Identical monomials are combined; no normal ordering is
-// performed, so two spellings of the same operator are kept as separate monomials (they act
-// identically on Fock states, which is all the ED path uses).
+// with real or complex coefficients, kept in the canonical normal order of libcommute's
+// expressions: creation operators first in ascending index order, then annihilation operators in
+// descending index order, reached with the anticommutation relations {c_i, c^dag_j} = delta_ij,
+// c_i c_i = c^dag_i c^dag_i = 0.  Equal operators are therefore equal expressions (Sz * N ==
+// N * Sz), and monomials that cancel disappear.
 #pragma once
 
 #include "Misc.hpp"
@@ -40,6 +42,36 @@ public:
 private:
     std::map<monomial_type, ScalarType> terms_;
 
+    // canonical position: creation operators before annihilation operators; ascending indices
+    // among the former, descending among the latter
+    static bool before(op_type const& a, op_type const& b) {
+        if(a.dag != b.dag)
+            return a.dag;
+        return a.dag ? a.index < b.index : b.index < a.index;
+    }
+
+    // add c * m after bringing m to canonical order (an adjacent out-of-order pair x y becomes
+    // -y x, plus delta for x = c_i, y = c^dag_i; a repeated operator gives zero)
+    void add_canonical(monomial_type m, ScalarType c) {
+        for(std::size_t k = 0; k + 1 < m.size(); ++k) {
+            op_type const x = m[k], y = m[k + 1];
+            if(x == y)
+                return; // c_i c_i = c^dag_i c^dag_i = 0
+            if(before(y, x)) {
+                if(x.index == y.index) { // x = c_i, y = c^dag_i: c_i c^dag_i = 1 - c^dag_i c_i
+                    monomial_type contracted;
+                    contracted.insert(contracted.end(), m.begin(), m.begin() + std::ptrdiff_t(k));
+                    contracted.insert(contracted.end(), m.begin() + std::ptrdiff_t(k + 2), m.end());
+                    add_canonical(contracted, c);
+                }
+                std::swap(m[k], m[k + 1]);
+                add_canonical(m, -c);
+                return;
+            }
+        }
+        add(m, c);
+    }
+
     void add(monomial_type const& m, ScalarType c) {
         auto it = terms_.find(m);
         if(it == terms_.end()) {
@@ -55,7 +87,7 @@ private:
 public:
     expression() = default;
     expression(ScalarType c) { add({}, c); } // NOLINT
-    expression(ScalarType c, monomial_type const& m) { add(m, c); }
+    expression(ScalarType c, monomial_type const& m) { add_canonical(m, c); }
     template <typename S2, typename = typename std::enable_if<!std::is_same<S2, ScalarType>::value>::type>
     expression(expression<S2, IndexTypes...> const& o) { // real -> complex promotion
         for(auto const& t : o.get_monomials())
@@ -96,7 +128,7 @@ public:
             for(auto const& y : b.terms_) {
                 monomial_type m = x.first;
                 m.insert(m.end(), y.first.begin(), y.first.end());
-                r.add(m, x.second * y.second);
+                r.add_canonical(m, x.second * y.second);
             }
         return r;
     }
@@ -113,7 +145,7 @@ public:
             monomial_type m;
             for(auto it = t.first.rbegin(); it != t.first.rend(); ++it)
                 m.push_back(op_type{!it->dag, it->index});
-            r.add(m, conj_(t.second));
+            r.add_canonical(m, conj_(t.second));
         }
         return r;
     }

@@ -78,8 +78,9 @@ class FullED:
             out.append(v)
         return np.array(out)
 
-    def g2(self, i: int, j: int, k: int, l: int, triplets: np.ndarray, res_tol: float = 1e-8) -> np.ndarray:
-        """chi_{ijkl}(w1,w2;w3) of gamma4.tex:38-70 by direct quadruple sums."""
+    def g2(self, i: int, j: int, k: int, l: int, triplets: np.ndarray, res_tol: float = 1e-8, mask=None) -> np.ndarray:
+        """chi_{ijkl}(w1,w2;w3) of gamma4.tex:38-70 by direct quadruple sums (over the eigenstate
+        quadruples where mask, if given, is true)."""
         ops = [self.rot([(False, i)]), self.rot([(False, j)]), self.rot([(True, k)])]
         CX4 = self.rot([(True, l)])
         E, w, beta = self.E, self.w, self.beta
@@ -90,6 +91,8 @@ class FullED:
             sign = np.linalg.det(np.eye(3)[list(perm)])
             O1, O2, O3 = (ops[p] for p in perm)
             M = np.einsum("ij,jk,kl,li->ijkl", O1, O2, O3, CX4) * sign
+            if mask is not None:
+                M = M * mask
             nz = M != 0
             for t, (z1, z2, z3) in enumerate(triplets):
                 Z = [z1, z2, -z3]

@@ -50,6 +50,9 @@ void assemble(FermionOperator const& H, StatesClassification const& S, std::vect
 HamiltonianPart::HamiltonianPart(FermionOperator const& HOp, StatesClassification const& S, BlockNumber Block)
     : HOp(&HOp), S(S), Block(Block) {}
 
+HamiltonianPart::HamiltonianPart(FermionOperator&& HOp, StatesClassification const& S, BlockNumber Block)
+    : owned(std::make_shared<FermionOperator const>(std::move(HOp))), HOp(owned.get()), S(S), Block(Block) {}
+
 InnerQuantumState HamiltonianPart::getSize() const { return S.getBlockSize(Block); }
 
 void HamiltonianPart::prepare() {
@@ -109,12 +112,12 @@ template <> MatrixType<true> const& HamiltonianPart::getMatrix<true>() const {
     throw std::runtime_error("Stored matrix type mismatch (real/complex)");
 }
 
-template <> std::vector<RealType> HamiltonianPart::getEigenState<false>(InnerQuantumState state) const {
+template <> VectorType<false> HamiltonianPart::getEigenState<false>(InnerQuantumState state) const {
     if(getStatus() < Computed)
         throw StatusMismatch("HamiltonianPart is not computed yet.");
     return getMatrix<false>().col(long(state));
 }
-template <> std::vector<ComplexType> HamiltonianPart::getEigenState<true>(InnerQuantumState) const {
+template <> VectorType<true> HamiltonianPart::getEigenState<true>(InnerQuantumState) const {
     throw std::runtime_error("Stored matrix type mismatch (real/complex)");
 }
 

@@ -10,6 +10,10 @@ MonomialOperatorPart::MonomialOperatorPart(FermionOperator const& MOp, StatesCla
                                            HamiltonianPart const& HFrom, HamiltonianPart const& HTo)
     : MOp(&MOp), S(S), HFrom(HFrom), HTo(HTo) {}
 
+MonomialOperatorPart::MonomialOperatorPart(FermionOperator&& MOp, StatesClassification const& S,
+                                           HamiltonianPart const& HFrom, HamiltonianPart const& HTo)
+    : owned(std::make_shared<FermionOperator const>(std::move(MOp))), MOp(owned.get()), S(S), HFrom(HFrom), HTo(HTo) {}
+
 namespace {
 
 struct PendingJob {

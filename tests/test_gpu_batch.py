@@ -53,3 +53,28 @@ def test_batch_graph_replays_and_autotune(pa):
     b.solve(True)
     assert rel_err(b.G(), first) < 1e-12  # only the summation grouping changed
     assert b.n_structures() == len({c.n_bath for c in cfgs})
+
+
+def test_batch_config4_instances_match_oracle(pa, oracle):
+    """Instances of the 512-instance workload itself (bath sizes 3-6, CoefficientTolerance 1e-8,
+    128 frequencies) from one batched solve against the oracle (Householder + implicit QR per
+    block, the algorithm of the reference's Eigen solver; GreensFunction terms and TermList merge)."""
+    cfgs = W.batch_configs(512)
+    pick = []
+    for nb in (3, 4, 5, 6):
+        pick += [k for k, c in enumerate(cfgs) if c.n_bath == nb][:2 if nb < 6 else 1]
+    sub = [cfgs[k] for k in pick]
+    print("[batch] config-4 instances", pick, "bath sizes", [c.n_bath for c in sub], flush=True)
+    b = pa.Batch([W.terms(c.expr()) for c in sub], [c.beta for c in sub], ("C", 0, 1), 128, 1e-8)
+    b.solve(True)
+    G, E0 = b.G(), b.E0()
+    for k, c in enumerate(sub):
+        H = c.expr()
+        idx = H.indices()
+        i = idx.index(("C", 0, 1))
+        o = oracle.ED(len(idx), H.mode_terms(), c.beta, "qr")
+        g = o.gf(i, i, W.g1_grid(128, c.beta), 1e-8, 1e-8)["values"]
+        err = rel_err(G[k], g)
+        print(f"[batch] instance {pick[k]} (bath {c.n_bath}, beta {c.beta:.1f}, U {c.U:.2f}): rel err {err:.1e}", flush=True)
+        assert abs(E0[k] - o.E0) < 1e-12 * max(1.0, abs(o.E0))
+        assert err < 1e-10

@@ -362,3 +362,30 @@ def test_vertex4_hubbard_atom_closed_form(pa):
     print(f"[vertex4] max |Gamma - closed form| {np.abs(got - ref).max():.2e} over {len(q)} triplets", flush=True)
     assert np.abs(got - ref).max() < 1e-10
     assert np.abs(r["gamma"] - r["gamma_value"]).max() < 1e-12  # container == point-by-point value()
+
+
+@pytest.mark.parametrize("name", ["anderson_test", "cfg1"])
+def test_quartic_and_quadratic_rotation(pa, oracle, name):
+    """QuarticOperator c+_i c+_j c_k c_l and QuadraticOperator c+_i c_j rotated on the device
+    (MonomialOperator.hpp:166-200, MonomialOperatorPart.cpp:48-92) against the oracle's rotation with
+    the same eigenvectors: identical block connectivity, elements to 1e-12; and the gauge-invariant
+    ensemble average <c+_u c+_d c_d c_u> = <n_u n_d> against the oracle's own eigensolver."""
+    H, beta = MODELS[name]()
+    e = pa.ED(W.terms(H), beta)
+    idx = H.indices()
+    u, d = idx.index(("C", 0, 1)), idx.index(("C", 0, 0))
+    o = inject(e, oracle.ED(len(idx), H.mode_terms(), beta))
+    quartic = [(True, u), (True, d), (False, d), (False, u)]
+    quadratic = [(True, u), (False, d)]
+    for op in (quartic, quadratic):
+        got = e.rotate(op, 1e-8)
+        ref = o.rotate(op, 1e-8)
+        assert sorted((p["from"], p["to"]) for p in got) == sorted((p["from"], p["to"]) for p in ref)
+        rm = {(p["from"], p["to"]): p["m"] for p in ref}
+        for p in got:
+            m = rm[(p["from"], p["to"])]
+            assert np.abs(p["m"] - m).max() <= 1e-12 * max(1.0, np.abs(m).max())
+    own = oracle.ED(len(idx), H.mode_terms(), beta)
+    nn = e.ensemble_average(quartic)
+    assert abs(nn - own.ensemble_average(quartic)) < 1e-12
+    assert abs(nn - own.ensemble_average([(True, u), (False, u), (True, d), (False, d)])) < 1e-12

@@ -181,3 +181,81 @@ def test_chi_tau_fourier_and_symmetry(oracle):
     t = np.linspace(0.0, beta, 9)
     r = ed.chi(nu, nd, z[:1], 1e-8, 0.0, False, list(t))
     assert np.abs(r["tau_values"] - r["tau_values"][::-1]).max() < 1e-12
+
+
+def _block_diag_ed(ed, H, n, beta):
+    """Full-space eigensystem assembled block by block on the oracle's partition (numpy eigh per
+    block), with the block of every eigenstate and the Boltzmann weights of the whole space."""
+    Hf = brute.hamiltonian(n, H.mode_terms())
+    dim = 1 << n
+    U = np.zeros((dim, dim))
+    E = np.zeros(dim)
+    blk = np.zeros(dim, int)
+    at = 0
+    for b, states in enumerate(ed.blocks()):
+        Eb, Vb = np.linalg.eigh(Hf[np.ix_(states, states)])
+        cols = np.arange(at, at + len(states))
+        U[np.ix_(states, cols)] = Vb
+        E[cols], blk[cols] = Eb, b
+        at += len(states)
+    w = np.exp(-beta * (E - E.min()))
+    return U, E, blk, w / w.sum()
+
+
+@pytest.mark.parametrize("tol,changes_g", [(1e-1, True), (5e-3, False), (2e-6, False)])
+def test_truncate_blocks_vs_brute(oracle, tol, changes_g):
+    """DensityMatrix::truncateBlocks (DensityMatrixPart.cpp:41-50): a block is retained iff one of
+    its Boltzmann weights exceeds tol, the weights themselves unchanged.  Then G keeps the block
+    pairs with either block retained (GreensFunction.cpp:62) and the 2PGF the block quadruples with
+    any block retained (TwoParticleGF.cpp:87-95).  Pinned against weights, part counts and sums
+    restricted to those block pairs and quadruples, all from numpy eigensystems of the blocks."""
+    H, beta = small_anderson(2, seed=11, U=2.5, beta=4.0)
+    n = len(H.indices())
+    ed = oracle.ED(n, H.mode_terms(), beta)
+    U, E, blk, w = _block_diag_ed(ed, H, n, beta)
+    nb = ed.n_blocks()
+    want = [bool(w[blk == b].max() > tol) for b in range(nb)]
+    assert any(want) and not all(want)  # the cut falls inside the spectrum
+    ed.truncate_blocks(tol)
+    assert list(ed.retained()) == want
+    for b in range(nb):
+        assert np.abs(ed.weights(b) - w[blk == b]).max() < 1e-12  # both in ascending energy
+    ret = np.array(want)[blk]  # retained flag of every eigenstate's block
+    z = fermionic(np.arange(-4, 12), beta)
+    i = H.indices().index(("C", 0, 1))
+    r = ed.gf(i, i, z, 1e-8, 0.0)
+    # parts: connected (outer, inner) block pairs of c_i / c_i^+ with either block retained
+    pairs = {(p["to"], p["from"]) for p in ed.rotate([(False, i)])}
+    assert r["n_parts"] == sum(1 for a, b in pairs if want[a] or want[b])
+    A = U.T @ brute.op_matrix(n, [(False, i)]) @ U
+    B = U.T @ brute.op_matrix(n, [(True, i)]) @ U
+    R = A * B.T * (w[:, None] + w[None, :])  # R[n, m] = <n|c|m><m|c+|n>(w_n + w_m)
+    P = E[None, :] - E[:, None]
+    keep = (ret[:, None] | ret[None, :]) & (np.abs(R) > 0)
+    gb = np.array([np.sum(R[keep] / (zz - P[keep])) for zz in z])
+    assert rel_err(r["values"], gb) < 1e-12
+    full = np.array([np.sum(R[np.abs(R) > 0] / (zz - P[np.abs(R) > 0])) for zz in z])
+    if changes_g:  # the cut drops pairs that carry weight: a real test of the part selection
+        assert rel_err(gb, full) > 1e-6
+
+
+def test_truncated_2pgf_vs_brute(oracle):
+    """The 2PGF part list and values under truncateBlocks: quadruples with any block retained."""
+    H, beta = dimer_model(), 2.0
+    ed = oracle.ED(4, H.mode_terms(), beta)
+    U, E, blk, w = _block_diag_ed(ed, H, 4, beta)
+    tol = 1e-2
+    want = [bool(w[blk == b].max() > tol) for b in range(ed.n_blocks())]
+    assert any(want) and not all(want)
+    ed.truncate_blocks(tol)
+    assert list(ed.retained()) == want
+    ret = np.array(want)[blk]
+    from pomerol_amd.workloads import g2_grid
+
+    zs = g2_grid(2, 2, beta)
+    for ijkl in [(1, 0, 1, 0), (1, 1, 1, 1)]:
+        v = ed.g2(*ijkl, zs.ravel(), 1e-8, 0.0)["values"]
+        fe = brute.FullED(4, H.mode_terms(), beta)
+        fe.E, fe.U, fe.w = E, U, w
+        keep4 = ret[:, None, None, None] | ret[None, :, None, None] | ret[None, None, :, None] | ret[None, None, None, :]
+        assert rel_err(v, fe.g2(*ijkl, zs, mask=keep4)) < 1e-11
