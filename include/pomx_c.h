@@ -113,6 +113,14 @@ int pomx_pair_terms(pomx_ctx* ctx, int kind, int nparts, const pomx_pair_part* h
 int pomx_pair_eval(pomx_ctx* ctx, int kind, int64_t nterms, const double* d_residue, const double* d_pole, int nz,
                    const double* d_z, double* d_acc);
 
+/* As pomx_pair_eval, allocating nothing: the partial sums live in a 16 MiB device array of the
+ * library (one per device, set up by pomx_ctx_create, held until the process ends).  No
+ * counterpart in the reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  nz <= 1048576
+ * (POMX_ERR_RANGE beyond).  Calls that overlap on one device race on the array. */
+int pomx_pair_eval_noalloc(pomx_ctx* ctx, int kind, int64_t nterms, const double* d_residue, const double* d_pole,
+                           int nz, const double* d_z, double* d_acc);
+
 /* Imaginary-time Lehmann sums on tau in [0, beta] (d_tau: ntau doubles; d_acc: ntau interleaved
  * complex, accumulated, imaginary parts untouched):
  * GF  GreensFunctionPart::Term::operator()(tau, beta) (GreensFunctionPart.cpp:28-31),

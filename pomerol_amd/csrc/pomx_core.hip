@@ -80,6 +80,10 @@ int pomx_ctx_create(int device, pomx_ctx** out) {
         delete ctx;
         return POMX_ERR_HIP;
     }
+    if(pair_scratch_setup(ctx) != POMX_OK) { // the no-allocation entry points' scratch on this device
+        pomx_ctx_destroy(ctx);
+        return POMX_ERR_HIP;
+    }
     *out = ctx;
     return POMX_OK;
 }

@@ -181,6 +181,10 @@ using ErasePass = std::function<int(EraseRule const&, bool emit, int64_t* evn, u
 int erase_fixed_point(pomx_ctx* ctx, int64_t ng, EraseRule rule, ErasePass const& pass, EraseEvents& ev,
                       const char* who);
 
+// Device address of the no-allocation entry points' scratch array on ctx->device (pomx_pair.hip),
+// looked up once per device; pomx_ctx_create calls it with the device current.
+int pair_scratch_setup(pomx_ctx* ctx);
+
 // Kernel statistics hooks (HIP events on the context stream, when timing is on): stat_begin
 // returns a handle (-1 when off) that stat_end closes with the launch group's work model.
 int stat_begin(pomx_ctx* ctx, int cls);

@@ -13,6 +13,7 @@
 #include "pomx_internal.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 namespace {
 
@@ -589,3 +590,66 @@ extern "C" int pomx_pair_eval(pomx_ctx* ctx, int kind, int64_t nterms, const dou
     return POMX_OK;
 }
 
+// ---- the Lehmann sum without allocation ----------------------------------------------------
+// pomx_pair_eval_noalloc keeps its per-slice partial sums in this file-scope device array (16 MiB
+// on every device the library runs on, held until the process ends) instead of the context's
+// pool.  Host code never passes the array's name to a kernel: it stands for a host-side shadow;
+// the device address comes from hipGetSymbolAddress, once per device, at context set-up.
+constexpr int64_t kPairScratchDoubles = int64_t(1) << 21;
+__device__ __align__(256) double scratch[kPairScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+} // namespace
+
+// Called by pomx_ctx_create with ctx->device current.
+int pair_scratch_setup(pomx_ctx* ctx) {
+    if(ctx->device < 0 || ctx->device >= kMaxDevices)
+        return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_ctx_create: device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if(g_scratch[ctx->device])
+        return POMX_OK;
+    void* p = nullptr;
+    hipError_t const e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if(e != hipSuccess || !p)
+        return pomx_fail(ctx, POMX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[ctx->device] = static_cast<double*>(p);
+    return POMX_OK;
+}
+
+extern "C" int pomx_pair_eval_noalloc(pomx_ctx* ctx, int kind, int64_t nterms, const double* d_residue,
+                                      const double* d_pole, int nz, const double* d_z, double* d_acc) {
+    POMX_REQUIRE(ctx, (kind == POMX_KIND_GF || kind == POMX_KIND_CHI) && nterms >= 0 && nz >= 0,
+                 "pomx_pair_eval_noalloc: bad arguments");
+    if(nterms == 0 || nz == 0)
+        return POMX_OK;
+    double* part = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        part = (ctx->device >= 0 && ctx->device < kMaxDevices) ? g_scratch[ctx->device] : nullptr;
+    }
+    if(!part)
+        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_pair_eval_noalloc: no scratch address for this device");
+    int64_t const max_slices = kPairScratchDoubles / (2 * int64_t(nz));
+    if(max_slices < 1)
+        return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_pair_eval_noalloc: more than 1048576 frequencies");
+    // the slicing of pomx_pair_eval, with fewer and longer slices when its partial sums would not fit
+    int ftiles = (nz + 255) / 256;
+    int64_t want = std::max<int64_t>(1, 2048 / ftiles);
+    int64_t slice = std::max<int64_t>(kEvalChunk, (nterms + want - 1) / want);
+    slice = (slice + kEvalChunk - 1) / kEvalChunk * kEvalChunk;
+    if((nterms + slice - 1) / slice > max_slices) {
+        slice = (nterms + max_slices - 1) / max_slices;
+        slice = (slice + kEvalChunk - 1) / kEvalChunk * kEvalChunk;
+    }
+    int nslices = int((nterms + slice - 1) / slice);
+    // every (slice, frequency) entry of the partial sums is written before it is read: no clearing
+    pair_eval_kernel<<<dim3(ftiles, nslices), 256, 0, ctx->stream>>>(nterms, slice, d_residue, d_pole, nz, d_z, part);
+    pair_eval_reduce_kernel<<<grid_for(nz), kBlock, 0, ctx->stream>>>(nz, nslices, kind == POMX_KIND_GF ? 1.0 : -1.0,
+                                                                       part, d_acc);
+    POMX_CHECK_LAUNCH(ctx);
+    POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return POMX_OK;
+}
