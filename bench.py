@@ -126,7 +126,7 @@ def roofline(stats, wall_s):
         if s["flop"] > 0 and secs > 0:
             rec["tflops"] = s["flop"] / secs / 1e12
             rec["frac_fp64_peak"] = rec["tflops"] / PEAK_FP64_TFLOPS
-        if name != "g2_gen" and s["term_freqs"] > 0 and secs > 0:
+        if name not in ("g2_gen", "g2_sort") and s["term_freqs"] > 0 and secs > 0:
             rec["term_freqs_per_s"] = s["term_freqs"] / secs
         if s["bytes"] > 0 and secs > 0:
             rec["algorithmic_GBs"] = s["bytes"] / secs / 1e9

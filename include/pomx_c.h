@@ -318,8 +318,9 @@ enum {
     POMX_STAT_G2_GENERAL = 3, /* 2PGF, any triplets, vector (g2_eval_kernel) */
     POMX_STAT_G2_REDUCE = 4,  /* 2PGF, per-slice partial sums into the grid (g2_eval_reduce_kernel) */
     POMX_STAT_PAIR = 5,       /* G / chi Lehmann sums (pair_eval_kernel + reduce) */
-    POMX_STAT_G2_GEN = 6,     /* 2PGF term generation and merge of a chunk (count excluded) */
-    POMX_STAT_NCLASS = 7
+    POMX_STAT_G2_GEN = 6,     /* 2PGF term generation and merge of a part, up to its evaluation */
+    POMX_STAT_G2_SORT = 7,    /* 2PGF: one sorted chunk (terms with shared pole bins) emitted, sorted, grouped */
+    POMX_STAT_NCLASS = 8
 };
 typedef struct {
     int64_t launches;

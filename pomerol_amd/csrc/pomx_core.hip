@@ -397,7 +397,7 @@ extern "C" int pomx_eval_timing(pomx_ctx* ctx, int enable, double* h_total_ms, i
     double ms = 0;
     int64_t n = 0;
     for(int c = 0; c < POMX_STAT_NCLASS; ++c)
-        if(c != POMX_STAT_G2_GEN && c != POMX_STAT_G2_REDUCE) {
+        if(c != POMX_STAT_G2_GEN && c != POMX_STAT_G2_SORT && c != POMX_STAT_G2_REDUCE) {
             ms += ctx->stats[c].ms;
             n += ctx->stats[c].launches;
         }

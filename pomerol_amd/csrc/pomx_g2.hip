@@ -66,6 +66,14 @@ __global__ void rank_heads_kernel(int64_t n, const uint64_t* __restrict__ keys, 
         head[k] = (k == 0 || keys[k] != keys[k - 1]) ? 1 : 0;
 }
 
+// uniq[idx[k]] = 1 when the sorted key k differs from both neighbours (its bin is held once)
+__global__ void unique_bin_kernel(int64_t n, const uint64_t* __restrict__ keys, const uint64_t* __restrict__ idx,
+                                  uint8_t* __restrict__ uniq) {
+    int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(k < n)
+        uniq[idx[k]] = ((k == 0 || keys[k] != keys[k - 1]) && (k + 1 == n || keys[k] != keys[k + 1])) ? 1 : 0;
+}
+
 __global__ void rank_scatter_kernel(int64_t n, const int64_t* __restrict__ head, const int64_t* __restrict__ excl,
                                     const uint64_t* __restrict__ idx, uint32_t* __restrict__ rank) {
     int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
@@ -165,35 +173,68 @@ __global__ void g2_stage_kernel(G2Dev D, double* __restrict__ O3c, double* __res
     }
 }
 
+// Direct terms.  A term's (kind, bin(P1), bin(P2), bin(P3)) can equal another term's only if its
+// (i1, i2) shares the P1 bin with another pair or its (i3, i4) shares the P3 bin with another pair
+// (equal bins of both pin (i1, i2) and (i3, i4), hence the quadruple).  Terms whose (i1, i2) and
+// (i3, i4) bins are both unique in their rank tables (u12, u34) are therefore groups of one: they
+// skip the sort and the reduce and are written straight into the merged term lists (direct);
+// the others (shared bins; 0.3 % of the config-2 terms) go through the sort (sorted).
+
+// One wave per (i1, i2) pair (position pos of the r12-sorted pair list), rows i3 in turn, lanes
+// over i4: the sorted terms per row (rowcnt[pos n3 + i3]) and per pair (paircnt[pos]), and the
+// direct terms per pair and kind (dpc[k npairs + pos]).
 __global__ void __launch_bounds__(256) g2_count_kernel(G2Dev D, const double* __restrict__ O3c,
                                                        const double* __restrict__ C4t, const uint64_t* __restrict__ pairs,
-                                                       int64_t nrows, int* __restrict__ cnt) {
+                                                       int64_t npairs, const uint8_t* __restrict__ u12,
+                                                       const uint8_t* __restrict__ u34, int* __restrict__ rowcnt,
+                                                       int64_t* __restrict__ paircnt, int64_t* __restrict__ dpc) {
     int const lane = threadIdx.x & 63;
     int64_t const nwaves = int64_t(gridDim.x) * (blockDim.x >> 6);
-    for(int64_t row = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6; row < nrows; row += nwaves) {
-        RowCtx r = row_ctx(D, pairs[row / D.n3], int(row % D.n3));
-        int c = 0;
-        if(r.active) {
-            const double* o3 = O3c + int64_t(r.i3) * D.n4;
-            const double* c4 = C4t + int64_t(r.i1) * D.n4;
-            for(int i4 = lane; i4 < D.n4; i4 += 64)
-                c += __popc(kinds_of(D, r.ab, r.s3, r.wi, r.wj, r.wk, o3[i4], c4[i4], D.w4[i4]));
+    for(int64_t pos = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6; pos < npairs; pos += nwaves) {
+        uint64_t const pr = pairs[pos];
+        bool const d12 = u12 && u12[pr];
+        int64_t sorted_pair = 0;
+        int dk[4] = {0, 0, 0, 0};
+        for(int i3 = 0; i3 < D.n3; ++i3) {
+            RowCtx r = row_ctx(D, pr, i3);
+            int c = 0;
+            if(r.active) {
+                const double* o3 = O3c + int64_t(i3) * D.n4;
+                const double* c4 = C4t + int64_t(r.i1) * D.n4;
+                for(int i4 = lane; i4 < D.n4; i4 += 64) {
+                    unsigned const m = kinds_of(D, r.ab, r.s3, r.wi, r.wj, r.wk, o3[i4], c4[i4], D.w4[i4]);
+                    if(d12 && u34[int64_t(i3) * D.n4 + i4]) {
+                        dk[0] += int(m & 1u);
+                        dk[1] += int((m >> 1) & 1u);
+                        dk[2] += int((m >> 2) & 1u);
+                        dk[3] += int((m >> 3) & 1u);
+                    } else
+                        c += __popc(m);
+                }
+            }
+            c = wave_sum_int(c);
+            sorted_pair += c;
+            if(lane == 0)
+                rowcnt[pos * D.n3 + i3] = c;
         }
-        c = wave_sum_int(c);
-        if(lane == 0)
-            cnt[row] = c;
+#pragma unroll
+        for(int k = 0; k < 4; ++k)
+            dk[k] = wave_sum_int(dk[k]);
+        if(lane == 0) {
+            paircnt[pos] = sorted_pair;
+#pragma unroll
+            for(int k = 0; k < 4; ++k)
+                dpc[k * npairs + pos] = dk[k];
+        }
     }
 }
 
-__global__ void pair_sum_kernel(int64_t npairs, int n3, const int* __restrict__ cnt, int64_t* __restrict__ out) {
-    int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if(p >= npairs)
-        return;
-    int64_t s = 0;
-    for(int i3 = 0; i3 < n3; ++i3)
-        s += cnt[p * n3 + i3];
-    out[p] = s;
-}
+// The merged term lists of one part, one buffer per kind: non-resonant kinds (C, P1, P2, P3), resonant
+// kinds (Rc, Nc, P1, P2, P3), field f of term t at base[k][f cap[k] + t].
+struct KindLists {
+    double* base[4];
+    int64_t cap[4];
+};
 
 // row counts of a chunk in visiting order (cnt is in r12-sorted pair order)
 __global__ void chunk_rowcnt_kernel(int64_t nrows, int n3, const uint4* __restrict__ rmap, const int* __restrict__ cnt,
@@ -215,6 +256,7 @@ __global__ void __launch_bounds__(256) g2_emit_kernel(G2Dev D, const double* __r
                                                       int64_t nrows, int64_t ntot,
                                                       const int64_t* __restrict__ off, const uint32_t* __restrict__ r12,
                                                       const uint32_t* __restrict__ r23, const uint32_t* __restrict__ r34,
+                                                      const uint8_t* __restrict__ u12, const uint8_t* __restrict__ u34,
                                                       uint64_t* __restrict__ keys, uint64_t* __restrict__ vals) {
     int const lane = threadIdx.x & 63;
     int64_t const nwaves = int64_t(gridDim.x) * (blockDim.x >> 6);
@@ -223,6 +265,7 @@ __global__ void __launch_bounds__(256) g2_emit_kernel(G2Dev D, const double* __r
         int i3;
         chunk_row(rmap, D.n3, lr, pr, i3);
         RowCtx r = row_ctx(D, pr.x, i3);
+        bool const d12 = u12 && u12[pr.x]; // direct terms of this row are written elsewhere
         if(!r.active)
             continue;
         int64_t at = off[lr];
@@ -236,7 +279,7 @@ __global__ void __launch_bounds__(256) g2_emit_kernel(G2Dev D, const double* __r
         for(int b4 = 0; b4 < D.n4; b4 += 64) {
             int const i4 = b4 + lane;
             unsigned m = 0u;
-            if(i4 < D.n4)
+            if(i4 < D.n4 && !(d12 && u34[int64_t(r.i3) * D.n4 + i4]))
                 m = kinds_of(D, r.ab, r.s3, r.wi, r.wj, r.wk, o3[i4], c4[i4], D.w4[i4]);
             int total;
             int64_t p = at + wave_excl_scan_int(__popc(m), lane, total);
@@ -354,6 +397,93 @@ __global__ void lookup_started_kernel(const uint64_t* __restrict__ q, int64_t m,
         out[0] = started[upper_index(q, 0, m, key) - 1];
 }
 
+// Direct terms: one wave per pair with unique bins, rows i3 in turn, lanes over i4; the terms of
+// each kind are placed by ballot prefix counts from the pair's offset doff[k npairs + pos].  With
+// q (the part's merge positions, m entries) the visiting position of every direct term, a group of
+// one, is counted into the gap between merges it falls in (as g2_first_hist_kernel does).
+__global__ void __launch_bounds__(256) g2_direct_emit_kernel(G2Dev D, const double* __restrict__ O3c,
+                                                             const double* __restrict__ C4t,
+                                                             const uint64_t* __restrict__ pairs, int64_t npairs,
+                                                             const uint8_t* __restrict__ u12,
+                                                             const uint8_t* __restrict__ u34,
+                                                             const int64_t* __restrict__ doff, KindLists out,
+                                                             const uint64_t* __restrict__ q, int64_t m,
+                                                             unsigned long long* __restrict__ hist) {
+    __shared__ unsigned int lh[kHistLds];
+    bool const local = q && m + 1 <= kHistLds;
+    if(local)
+        for(int k = threadIdx.x; k <= m; k += blockDim.x)
+            lh[k] = 0u;
+    __syncthreads();
+    int const lane = threadIdx.x & 63;
+    uint64_t const below = (uint64_t(1) << lane) - 1u;
+    int64_t const nwaves = int64_t(gridDim.x) * (blockDim.x >> 6);
+    for(int64_t pos = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6; pos < npairs; pos += nwaves) {
+        uint64_t const pr = pairs[pos];
+        if(!u12[pr])
+            continue;
+        int64_t at[4];
+#pragma unroll
+        for(int k = 0; k < 4; ++k)
+            at[k] = doff[k * npairs + pos];
+        for(int i3 = 0; i3 < D.n3; ++i3) {
+            RowCtx r = row_ctx(D, pr, i3);
+            if(!r.active)
+                continue;
+            const double* o3 = O3c + int64_t(i3) * D.n4;
+            const double* c4 = C4t + int64_t(r.i1) * D.n4;
+            double const P1 = D.E2[r.i2] - D.E1[r.i1], P2 = D.E3[i3] - D.E2[r.i2];
+            for(int b4 = 0; b4 < D.n4; b4 += 64) {
+                int const i4 = b4 + lane;
+                unsigned mk = 0u;
+                double C = 0.0, wl = 0.0, P3 = 0.0;
+                if(i4 < D.n4 && u34[int64_t(i3) * D.n4 + i4]) {
+                    double const c = o3[i4], d = c4[i4];
+                    wl = D.w4[i4];
+                    mk = kinds_of(D, r.ab, r.s3, r.wi, r.wj, r.wk, c, d, wl);
+                    C = r.ab * c * d * D.sign; // ((O1 O2) O3) CX4 sign, as the reduce forms it
+                    P3 = D.E4[i4] - D.E3[i3];
+                }
+#pragma unroll
+                for(int k = 0; k < 4; ++k) {
+                    bool const bit = (mk >> k) & 1u;
+                    uint64_t const ball = __ballot(bit);
+                    if(bit) {
+                        int64_t const t = at[k] + __popcll(ball & below);
+                        double* B = out.base[k];
+                        int64_t const cap = out.cap[k];
+                        if(k < 2) {
+                            B[t] = (k == 0) ? -C * (r.wj + r.wk) : C * (r.wi + wl);
+                            B[cap + t] = P1;
+                            B[2 * cap + t] = P2;
+                            B[3 * cap + t] = P3;
+                        } else {
+                            B[t] = (k == 2) ? -C * D.beta * r.wj : C * D.beta * r.wi;
+                            B[cap + t] = (k == 2) ? C * (r.wj - wl) : C * (r.wk - r.wi);
+                            B[2 * cap + t] = P1;
+                            B[3 * cap + t] = P2;
+                            B[4 * cap + t] = P3;
+                        }
+                        if(q) {
+                            int64_t const j = upper_index(q, 0, m, g2_position(D, k, r.i1, r.i2, i3, i4));
+                            if(local)
+                                atomicAdd(&lh[j], 1u);
+                            else
+                                atomicAdd(&hist[j], 1ull);
+                        }
+                    }
+                    at[k] += __popcll(ball);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if(local)
+        for(int k = threadIdx.x; k <= m; k += blockDim.x)
+            if(lh[k])
+                atomicAdd(&hist[k], (unsigned long long)lh[k]);
+}
+
 // One thread per key group: sum the members' coefficients (pole = count-weighted mean).
 // The erase threshold is thr, or, with R.fk set, the reference's tol/(size+1) (EraseRule); EMIT
 // as for erase_fixed_point (events written at evn[gi], group results untouched).  With the fixed
@@ -446,29 +576,35 @@ __global__ void g2_reduce_kernel(G2Dev D, int64_t n, int64_t ngroups, const int6
     }
 }
 
-// scatter live groups into kind-segmented SoA output (non-resonant kinds first, then resonant)
+// Where a chunk's live groups go in the part's kind lists: the live groups of kind k (contiguous in
+// the live order, from live index first[k]) to positions at[k] on.
+struct KindSlots {
+    int64_t first[4], at[4];
+};
+
+// scatter live groups into the kind lists (NR kinds: C, P1, P2, P3; R kinds: Rc, Nc, P1, P2, P3)
 __global__ void g2_compact_kernel(int64_t ngroups, const int64_t* __restrict__ live, const int64_t* __restrict__ lpos,
                                   const double* __restrict__ g, const int64_t* __restrict__ start,
-                                  const uint64_t* __restrict__ keys, int64_t nr_cap, int64_t r_cap,
-                                  int64_t nr_live, double* __restrict__ nr,
-                                  double* __restrict__ rs) {
+                                  const uint64_t* __restrict__ keys, KindLists out, KindSlots slot) {
     int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if(gi >= ngroups || !live[gi])
         return;
-    int kind = int(keys[start[gi]] >> 62);
-    int64_t at = lpos[gi];
+    int const kind = int(keys[start[gi]] >> 62);
+    int64_t const t = slot.at[kind] + (lpos[gi] - slot.first[kind]);
+    double* B = out.base[kind];
+    int64_t const cap = out.cap[kind];
+    const double* x = g + gi * 5;
     if(kind < 2) {
-        nr[0 * nr_cap + at] = g[gi * 5 + 0];
-        nr[1 * nr_cap + at] = g[gi * 5 + 2];
-        nr[2 * nr_cap + at] = g[gi * 5 + 3];
-        nr[3 * nr_cap + at] = g[gi * 5 + 4];
+        B[t] = x[0];
+        B[cap + t] = x[2];
+        B[2 * cap + t] = x[3];
+        B[3 * cap + t] = x[4];
     } else {
-        at -= nr_live; // resonant terms follow the nr_live non-resonant ones in the live order
-        rs[0 * r_cap + at] = g[gi * 5 + 0];
-        rs[1 * r_cap + at] = g[gi * 5 + 1];
-        rs[2 * r_cap + at] = g[gi * 5 + 2];
-        rs[3 * r_cap + at] = g[gi * 5 + 3];
-        rs[4 * r_cap + at] = g[gi * 5 + 4];
+        B[t] = x[0];
+        B[cap + t] = x[1];
+        B[2 * cap + t] = x[2];
+        B[3 * cap + t] = x[3];
+        B[4 * cap + t] = x[4];
     }
 }
 
@@ -2019,7 +2155,7 @@ int read_i64(pomx_ctx* ctx, const int64_t* d, int64_t* h) {
 
 // dense ranks of round((Ey - Ex)/res) over all (x, y); 'sorted' receives the (x*ny+y) order
 int rank_table(pomx_ctx* ctx, int nx, int ny, const double* Ex, const double* Ey, double res, WsBuf<uint32_t>& rank,
-               WsBuf<uint64_t>* sorted, int64_t* nranks, int* d_err) {
+               WsBuf<uint64_t>* sorted, int64_t* nranks, int* d_err, WsBuf<uint8_t>* uniq) {
     int64_t n = int64_t(nx) * ny;
     WsBuf<uint64_t> k0(ctx, n), k1(ctx, n), v0(ctx, n), v1(ctx, n);
     WsBuf<int64_t> head(ctx, n), excl(ctx, n);
@@ -2041,6 +2177,13 @@ int rank_table(pomx_ctx* ctx, int nx, int ny, const double* Ex, const double* Ey
     if(sorted) {
         *sorted = WsBuf<uint64_t>(ctx, n);
         POMX_HIP(ctx, hipMemcpyAsync(sorted->p, v1.p, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    if(uniq) {
+        *uniq = WsBuf<uint8_t>(ctx, n);
+        if(!uniq->p)
+            return pomx_fail(ctx, POMX_ERR_HIP, "rank_table: out of device memory");
+        unique_bin_kernel<<<grid_for(n), kBlock, 0, ctx->stream>>>(n, k1, v1, uniq->p);
+        POMX_CHECK_LAUNCH(ctx);
     }
     return POMX_OK;
 }
@@ -2096,6 +2239,14 @@ int64_t chunk_terms() {
     return kChunkTerms;
 }
 
+// Direct terms in force: the environment variable POMX_G2_DIRECT=0 sends every term through the sort
+// (a test hook, so that the sorted path and its chunking are tested on every part).
+bool direct_terms_enabled() {
+    if(const char* e = std::getenv("POMX_G2_DIRECT"))
+        return std::atoi(e) != 0;
+    return true;
+}
+
 } // namespace
 
 extern "C" int pomx_g2_terms_eval(pomx_ctx* ctx, const pomx_g2_terms* t, int perm, double res, int nfreq,
@@ -2129,8 +2280,9 @@ struct G2Chunk {
 // Tables shared by a part's chunks.
 struct G2Gen {
     const double *O3c, *C4t;
-    const int* rowcnt;     // per (r12-sorted pair, i3)
+    const int* rowcnt;     // sorted terms per (r12-sorted pair, i3)
     const uint32_t *r12, *r23, *r34;
+    const uint8_t *u12, *u34; // unique-bin flags (direct terms), or null (every term sorted)
 };
 
 // Generate, sort and group the terms of the pairs [pb, pe) of the r12-sorted pair list.
@@ -2169,8 +2321,8 @@ int g2_generate_chunk(pomx_ctx* ctx, G2Dev const& D, G2Gen const& T, std::vector
     if(!k0.p || !v0.p || !out.keys.p || !out.vals.p)
         return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
     unsigned const eblocks = unsigned(std::min<int64_t>((nrow + 3) / 4, 1 << 16));
-    g2_emit_kernel<<<eblocks, kBlock, 0, ctx->stream>>>(D, T.O3c, T.C4t, d_rmap, nrow, n, off, T.r12, T.r23, T.r34, k0,
-                                                         v0);
+    g2_emit_kernel<<<eblocks, kBlock, 0, ctx->stream>>>(D, T.O3c, T.C4t, d_rmap, nrow, n, off, T.r12, T.r23, T.r34, T.u12,
+                                                         T.u34, k0, v0);
     POMX_CHECK_LAUNCH(ctx);
     if((rc = pomx_sort_pairs(ctx, n, k0, out.keys, v0, out.vals, 0, 64)))
         return rc;
@@ -2205,16 +2357,12 @@ int g2_alloc_groups(pomx_ctx* ctx, G2Chunk& c) {
     return POMX_OK;
 }
 
-struct KeptChunk {
-    double* nr;
-    double* r;
-    int64_t nr_cap, r_cap, c[4];
-};
-
-// Reduce (unless already reduced), compact and evaluate one chunk; then free its terms.
+// Reduce (unless already reduced) and finish one chunk, then free its terms.  Value path: the live
+// groups are appended to the part's kind lists at fill[k]; tangent path: reduced, compacted and
+// evaluated at once (g2_chunk_tangent).
 int g2_finish_chunk(pomx_ctx* ctx, G2Dev const& D, G2Chunk& c, double thr, EraseRule const& rule,
                     const pomx_g2_tangent* tan, int perm, double res, int nfreq, const double* d_freq, double* d_acc,
-                    double* d_dacc, bool keep, std::vector<KeptChunk>& kept, int64_t counts[4]) {
+                    double* d_dacc, KindLists const& out, int64_t fill[4], int64_t counts[4]) {
     int rc;
     if(tan) {
         rc = g2_chunk_tangent(ctx, D, *tan, c.n, c.ng, c.start, c.keys, c.vals, thr, rule, perm, res, nfreq, d_freq,
@@ -2240,31 +2388,22 @@ int g2_finish_chunk(pomx_ctx* ctx, G2Dev const& D, G2Chunk& c, double thr, Erase
     int64_t k[4];
     POMX_HIP(ctx, hipMemcpyAsync(k, kc.p, sizeof(k), hipMemcpyDeviceToHost, ctx->stream));
     POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    int64_t const nnr = k[0] + k[1], nr = k[2] + k[3];
-    double* nrbuf = static_cast<double*>(ws_get(ctx, sizeof(double) * 4 * std::max<int64_t>(nnr, 1)));
-    double* rbuf = static_cast<double*>(ws_get(ctx, sizeof(double) * 5 * std::max<int64_t>(nr, 1)));
-    if(!nrbuf || !rbuf)
-        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
-    g2_compact_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(ng, c.live, lpos, c.g, c.start, c.keys,
-                                                                 std::max<int64_t>(nnr, 1), std::max<int64_t>(nr, 1), nnr,
-                                                                 nrbuf, rbuf);
-    POMX_CHECK_LAUNCH(ctx);
-    c = G2Chunk(); // the terms are no longer needed
-    for(int q = 0; q < 4; ++q)
-        counts[q] += k[q];
-    if(nfreq > 0) {
-        TermSpan spans[4] = {{nrbuf, std::max<int64_t>(nnr, 1), 0, k[0]},
-                             {nrbuf, std::max<int64_t>(nnr, 1), k[0], k[1]},
-                             {rbuf, std::max<int64_t>(nr, 1), 0, k[2]},
-                             {rbuf, std::max<int64_t>(nr, 1), k[2], k[3]}};
-        if((rc = launch_g2_eval(ctx, spans, perm, res, nfreq, d_freq, d_acc)))
-            return rc;
+    KindSlots slot;
+    int64_t first = 0;
+    for(int q = 0; q < 4; ++q) {
+        slot.first[q] = first; // live groups of kind q start at this live index
+        slot.at[q] = fill[q];
+        first += k[q];
+        if(fill[q] + k[q] > out.cap[q] || (k[q] > 0 && !out.base[q]))
+            return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_g2_part_compute: term list capacity exceeded");
     }
-    if(keep)
-        kept.push_back(KeptChunk{nrbuf, rbuf, std::max<int64_t>(nnr, 1), std::max<int64_t>(nr, 1), {k[0], k[1], k[2], k[3]}});
-    else {
-        ws_put(ctx, nrbuf);
-        ws_put(ctx, rbuf);
+    g2_compact_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(ng, c.live, lpos, c.g, c.start, c.keys, out, slot);
+    POMX_CHECK_LAUNCH(ctx);
+    POMX_HIP(ctx, hipStreamSynchronize(ctx->stream)); // slot and the chunk's buffers outlive the kernel
+    c = G2Chunk();
+    for(int q = 0; q < 4; ++q) {
+        fill[q] += k[q];
+        counts[q] += k[q];
     }
     return POMX_OK;
 }
@@ -2272,25 +2411,22 @@ int g2_finish_chunk(pomx_ctx* ctx, G2Dev const& D, G2Chunk& c, double thr, Erase
 // The reference's erase decisions (TermList.hpp add_term: |C| <= tol/(size+1)) for the held
 // chunks, whose groups include every merge of the part.  The threshold of a merge needs the
 // number of groups of its list started before it, over the whole part: the merge positions of the
-// held chunks are sorted (with a sentinel between the two lists) and every group's first position
-// (fks, one array per chunk of the part) is counted into the gap it falls in; the erase and restart
-// events are then settled by the fixed point of erase_fixed_point over all held chunks.  On
-// return every held chunk is reduced and 'rule' is the settled rule.
-int g2_settle(pomx_ctx* ctx, G2Dev const& D, double tol, std::vector<G2Chunk*> const& held,
-              std::vector<std::pair<const uint64_t*, int64_t>> const& fks, EraseRule& rule, EraseEvents& ev,
-              WsBuf<uint64_t>& qs, WsBuf<int64_t>& started) {
-    constexpr uint64_t kListGap = (uint64_t(1) << 63) - 1; // above every first-list and below every second-list position
-    int64_t m = 1, ng = 0;
-    for(G2Chunk const* c : held) {
+// held chunks are sorted (with a sentinel between the two lists, g2_merge_positions) and every
+// group's first position is counted into the gap it falls in (g2_first_hist_kernel over the
+// sorted chunks' groups, g2_direct_emit_kernel over the direct terms, into hist); g2_settle then
+// settles the erase and restart events by the fixed point of erase_fixed_point over all held
+// chunks.  On return every held chunk is reduced and 'rule' is the settled rule.
+constexpr uint64_t kListGap = (uint64_t(1) << 63) - 1; // above every first-list, below every second-list position
+
+int g2_merge_positions(pomx_ctx* ctx, G2Dev const& D, std::vector<G2Chunk*> const& held, WsBuf<uint64_t>& qs,
+                       int64_t& m, WsBuf<int64_t>& hist) {
+    m = 1;
+    for(G2Chunk const* c : held)
         m += c->n - c->ng;
-        ng += c->ng;
-    }
     WsBuf<uint64_t> q0(ctx, m), v0(ctx, m), v1(ctx, m);
     qs = WsBuf<uint64_t>(ctx, m);
-    WsBuf<int64_t> hist(ctx, m + 1);
-    started = WsBuf<int64_t>(ctx, m + 1);
-    WsBuf<int64_t> base(ctx, 1);
-    if(!q0.p || !v0.p || !v1.p || !qs.p || !hist.p || !started.p || !base.p)
+    hist = WsBuf<int64_t>(ctx, m + 1);
+    if(!q0.p || !v0.p || !v1.p || !qs.p || !hist.p)
         return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
     int64_t at = 0;
     for(G2Chunk const* c : held) {
@@ -2305,6 +2441,21 @@ int g2_settle(pomx_ctx* ctx, G2Dev const& D, double tol, std::vector<G2Chunk*> c
         return rc;
     if((rc = pomx_fill_zero(ctx, hist.p, sizeof(int64_t) * size_t(m + 1))))
         return rc;
+    POMX_HIP(ctx, hipStreamSynchronize(ctx->stream)); // kListGap's copy has finished
+    return POMX_OK;
+}
+
+int g2_settle(pomx_ctx* ctx, G2Dev const& D, double tol, std::vector<G2Chunk*> const& held,
+              std::vector<std::pair<const uint64_t*, int64_t>> const& fks, WsBuf<uint64_t> const& qs, int64_t m,
+              WsBuf<int64_t> const& hist, EraseRule& rule, EraseEvents& ev, WsBuf<int64_t>& started) {
+    int64_t ng = 0;
+    for(G2Chunk const* c : held)
+        ng += c->ng;
+    started = WsBuf<int64_t>(ctx, m + 1);
+    WsBuf<int64_t> base(ctx, 1);
+    if(!started.p || !base.p)
+        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+    int rc;
     for(auto const& f : fks) {
         if(f.second == 0)
             continue;
@@ -2318,7 +2469,7 @@ int g2_settle(pomx_ctx* ctx, G2Dev const& D, double tol, std::vector<G2Chunk*> c
     add_own_kernel<<<grid_for(m + 1), kBlock, 0, ctx->stream>>>(m + 1, hist, started);
     lookup_started_kernel<<<1, 64, 0, ctx->stream>>>(qs, m, started, kListGap, base);
     POMX_CHECK_LAUNCH(ctx);
-    int64_t base_hi = 0; // (read_i64 synchronises: kListGap's copy has finished)
+    int64_t base_hi = 0;
     if((rc = read_i64(ctx, base.p, &base_hi)))
         return rc;
     rule.fk = qs.p;
@@ -2357,7 +2508,9 @@ int g2_settle(pomx_ctx* ctx, G2Dev const& D, double tol, std::vector<G2Chunk*> c
     return POMX_OK;
 }
 
-// Shared by the value path (tan == NULL) and the tangent path (keep must be NULL).
+// Shared by the value path (tan == NULL) and the tangent path (keep must be NULL).  The value path
+// writes the direct terms (unique bins) straight into the part's kind lists and sends the others
+// through the sort; the tangent path sorts every term.
 int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* tan, double beta, double res, double tol,
                 int nfreq, const double* d_freq, double* d_acc, double* d_dacc, pomx_g2_terms* keep, int64_t* h_counts,
                 int64_t* h_raw_terms) {
@@ -2371,53 +2524,76 @@ int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* 
         return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_g2_part_compute: block larger than 65535");
     G2Dev D{P.n1, P.n2, P.n3, P.n4, P.O1, P.O2, P.O3, P.CX4, P.E1, P.E2, P.E3, P.E4,
             P.w1, P.w2, P.w3, P.w4, P.sign, beta, tol};
+    bool const direct = tan == nullptr && direct_terms_enabled();
     int64_t counts[4] = {0, 0, 0, 0};
     if(keep) {
         keep->data = nullptr;
         for(int k = 0; k < 4; ++k)
             keep->count[k] = 0;
     }
+    // statistics: term generation and merge up to the evaluation; raw terms in the term_freqs field;
+    // bytes = the terms written once (the direct ones in their final form, 32-40 B; the sorted ones
+    // as keys and values, sorted and grouped once: 48 B)
+    int const hg = stat_begin(ctx, POMX_STAT_G2_GEN);
     WsBuf<int> err(ctx, 1);
     int rc = pomx_fill_zero(ctx, err.p, sizeof(int));
     if(rc)
         return rc;
-    // 1. rank tables
+    // 1. rank tables (and, for direct terms, which bins of (i1, i2) and (i3, i4) are unique)
     WsBuf<uint32_t> r12, r23, r34;
     WsBuf<uint64_t> pairs;
+    WsBuf<uint8_t> u12, u34;
     int64_t nr12 = 0, nr23 = 0, nr34 = 0;
-    if((rc = rank_table(ctx, P.n1, P.n2, P.E1, P.E2, res, r12, &pairs, &nr12, err)) ||
+    if((rc = rank_table(ctx, P.n1, P.n2, P.E1, P.E2, res, r12, &pairs, &nr12, err, direct ? &u12 : nullptr)) ||
        (rc = rank_table(ctx, P.n2, P.n3, P.E2, P.E3, res, r23, nullptr, &nr23, err)) ||
-       (rc = rank_table(ctx, P.n3, P.n4, P.E3, P.E4, res, r34, nullptr, &nr34, err)))
+       (rc = rank_table(ctx, P.n3, P.n4, P.E3, P.E4, res, r34, nullptr, &nr34, err, direct ? &u34 : nullptr)))
         return rc;
     if(nr12 >= (int64_t(1) << 20) || nr23 >= (int64_t(1) << 21) || nr34 >= (int64_t(1) << 21))
         return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_g2_part_compute: too many distinct pole bins for the 64-bit key");
-    // 2. count terms per row, per pair, and cut chunks at r12 group boundaries
+    // 2. count terms: sorted ones per row and pair, direct ones per pair and kind
     int64_t const npairs = int64_t(P.n1) * P.n2;
     int64_t const nrows = npairs * P.n3;
     WsBuf<double> O3c(ctx, int64_t(P.n3) * P.n4), C4t(ctx, int64_t(P.n1) * P.n4);
     g2_stage_kernel<<<grid_for(int64_t(P.n3) * P.n4 + int64_t(P.n1) * P.n4), kBlock, 0, ctx->stream>>>(D, O3c, C4t);
     WsBuf<int> rowcnt(ctx, nrows);
-    unsigned const row_blocks = unsigned(std::min<int64_t>((nrows + 3) / 4, 1 << 16)); // 4 waves per block
-    g2_count_kernel<<<row_blocks, kBlock, 0, ctx->stream>>>(D, O3c, C4t, pairs, nrows, rowcnt);
-    WsBuf<int64_t> paircnt(ctx, npairs);
-    pair_sum_kernel<<<grid_for(npairs), kBlock, 0, ctx->stream>>>(npairs, P.n3, rowcnt, paircnt);
+    WsBuf<int64_t> paircnt(ctx, npairs), dpc(ctx, 4 * npairs), doff(ctx, 4 * npairs);
+    if(!rowcnt.p || !paircnt.p || !dpc.p || !doff.p)
+        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+    unsigned const pair_blocks = unsigned(std::min<int64_t>((npairs + 3) / 4, 1 << 16)); // 4 waves per block
+    g2_count_kernel<<<pair_blocks, kBlock, 0, ctx->stream>>>(D, O3c, C4t, pairs, npairs, u12.p, u34.p, rowcnt, paircnt,
+                                                             dpc);
     POMX_CHECK_LAUNCH(ctx);
+    for(int k = 0; k < 4; ++k)
+        if((rc = pomx_exclusive_scan(ctx, npairs, dpc.p + k * npairs, doff.p + k * npairs)))
+            return rc;
     std::vector<int64_t> hpair(npairs);
     std::vector<uint32_t> hr12(npairs);
     std::vector<uint64_t> hpairs(npairs);
     POMX_HIP(ctx, hipMemcpyAsync(hpair.data(), paircnt.p, sizeof(int64_t) * npairs, hipMemcpyDeviceToHost, ctx->stream));
     POMX_HIP(ctx, hipMemcpyAsync(hr12.data(), r12.p, sizeof(uint32_t) * npairs, hipMemcpyDeviceToHost, ctx->stream));
     POMX_HIP(ctx, hipMemcpyAsync(hpairs.data(), pairs.p, sizeof(uint64_t) * npairs, hipMemcpyDeviceToHost, ctx->stream));
+    int64_t dlast[8];
+    for(int k = 0; k < 4; ++k) {
+        POMX_HIP(ctx, hipMemcpyAsync(dlast + 2 * k, doff.p + k * npairs + npairs - 1, sizeof(int64_t),
+                                     hipMemcpyDeviceToHost, ctx->stream));
+        POMX_HIP(ctx, hipMemcpyAsync(dlast + 2 * k + 1, dpc.p + k * npairs + npairs - 1, sizeof(int64_t),
+                                     hipMemcpyDeviceToHost, ctx->stream));
+    }
     int herr = 0;
     POMX_HIP(ctx, hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if(herr)
         return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_g2_part_compute: |pole/pole_res| out of range");
-    int64_t quads_total = 0;
+    int64_t ndirect[4], direct_total = 0;
+    for(int k = 0; k < 4; ++k) {
+        ndirect[k] = dlast[2 * k] + dlast[2 * k + 1];
+        direct_total += ndirect[k];
+    }
+    int64_t sorted_total = 0;
     struct Span {
         int64_t pb, pe, n;
     };
-    std::vector<Span> chunks; // [pair_begin, pair_end) of the r12-sorted pairs, n terms
+    std::vector<Span> chunks; // [pair_begin, pair_end) of the r12-sorted pairs, n sorted terms
     {
         int64_t const chunk_cap = chunk_terms();
         int64_t b = 0, acc = 0;
@@ -2429,32 +2605,47 @@ int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* 
                 acc = 0;
             }
             acc += hpair[p];
-            quads_total += hpair[p];
+            sorted_total += hpair[p];
         }
         if(npairs > b && acc > 0)
             chunks.push_back(Span{b, npairs, acc});
     }
+    int64_t const quads_total = direct_total + sorted_total;
+    // 3. the part's kind lists (value path): direct terms first, then the sorted groups
+    KindLists lists{};
+    std::vector<WsBuf<double>> list_store;
+    int64_t fill[4];
+    for(int k = 0; k < 4; ++k) {
+        lists.cap[k] = std::max<int64_t>(ndirect[k] + sorted_total, 1);
+        fill[k] = ndirect[k];
+        if(!tan) { // (the tangent path evaluates chunk by chunk)
+            list_store.emplace_back(ctx, size_t(lists.cap[k]) * (k < 2 ? 4 : 5));
+            if(!list_store.back().p)
+                return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+            lists.base[k] = list_store.back().p;
+            counts[k] = ndirect[k];
+        }
+    }
     // visiting positions (list bit, ((i1 n3 + i3) n2 + i2) n4 + i4, slot) must fit below the list gap
     bool const positions_ok = double(P.n1) * P.n2 * P.n3 * P.n4 * 2.0 < 9.2e18;
     bool const multi = chunks.size() > 1;
-    G2Gen const T{O3c, C4t, rowcnt, r12, r23, r34};
-    std::vector<KeptChunk> kept;
+    G2Gen const T{O3c, C4t, rowcnt, r12, r23, r34, u12.p, u34.p};
     EraseRule rule{tol, nullptr, 0, nullptr, nullptr, 0};
-    // 3. per chunk: generate; a chunk without merges (or, alone in its part, without a merge the
-    // reference's threshold could decide otherwise) is final at once; the others are held until
-    // the part's erase decisions are settled
+    // 4. sorted terms, per chunk: generate; a chunk without merges (or, alone in its part, without a
+    // merge the reference's threshold could decide otherwise) is final at once; the others are held
+    // until the part's erase decisions are settled
     std::vector<G2Chunk> held_store;
     held_store.reserve(chunks.size());
     std::vector<WsBuf<uint64_t>> fk_store;
     std::vector<std::pair<const uint64_t*, int64_t>> fks;
+    int64_t ng_sorted = 0;
     for(auto const& ch : chunks) {
         G2Chunk c;
-        // statistics: raw terms in the term_freqs field; bytes = the terms' keys and values written
-        // once, sorted (read and written once) and grouped (keys read once), 48 per term
-        int const hg = stat_begin(ctx, POMX_STAT_G2_GEN);
+        int const hs = stat_begin(ctx, POMX_STAT_G2_SORT);
         if((rc = g2_generate_chunk(ctx, D, T, hpairs, ch.pb, ch.pe, ch.n, c)))
             return rc;
-        stat_end(ctx, hg, 0.0, double(ch.n), 48.0 * double(ch.n));
+        stat_end(ctx, hs, 0.0, double(ch.n), 48.0 * double(ch.n));
+        ng_sorted += c.ng;
         int64_t const merges = c.n - c.ng;
         if(multi && positions_ok) { // every group's first position, for the thresholds of other chunks' merges
             fk_store.emplace_back(ctx, c.ng);
@@ -2469,13 +2660,13 @@ int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* 
         // tells whether any decision could differ: only a merge whose max(|c0|, |c1|) lies in
         // (that, tol] can
         bool final_now = merges == 0;
+        double const thr = tol / double(c.ng + direct_total + 1);
         if(!final_now && !multi) {
             WsBuf<int> ambig(ctx, 1);
             if(!ambig.p)
                 return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
             if((rc = pomx_fill_zero(ctx, ambig.p, sizeof(int))) || (rc = g2_alloc_groups(ctx, c)))
                 return rc;
-            double const thr = tol / double(c.ng + 1);
             g2_reduce_kernel<false><<<grid_for(c.ng), kBlock, 0, ctx->stream>>>(D, c.n, c.ng, c.start, c.keys, c.vals,
                                                                                   thr, rule, c.g, c.live, nullptr,
                                                                                   nullptr, nullptr, ambig);
@@ -2487,8 +2678,8 @@ int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* 
             final_now = c.reduced;
         }
         if(final_now) { // (the tangent path re-reduces with the same threshold as that pass)
-            if((rc = g2_finish_chunk(ctx, D, c, tol / double(c.ng + 1), rule, tan, P.perm, res, nfreq, d_freq,
-                                     d_acc, d_dacc, keep != nullptr, kept, counts)))
+            if((rc = g2_finish_chunk(ctx, D, c, thr, rule, tan, P.perm, res, nfreq, d_freq, d_acc, d_dacc, lists, fill,
+                                     counts)))
                 return rc;
             continue;
         }
@@ -2505,21 +2696,40 @@ int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* 
         }
         held_store.push_back(std::move(c));
     }
-    // 4. settle the held chunks' erase decisions over the whole part, then finish them
-    if(!held_store.empty()) {
-        std::vector<G2Chunk*> held;
-        for(auto& c : held_store)
-            held.push_back(&c);
+    // 5. direct terms (their first positions counted against the merges when some chunk is held),
+    // then the held chunks' erase decisions over the whole part
+    std::vector<G2Chunk*> held;
+    for(auto& c : held_store)
+        held.push_back(&c);
+    WsBuf<uint64_t> qs;
+    WsBuf<int64_t> hist, started;
+    int64_t m = 0;
+    if(!held.empty() && (rc = g2_merge_positions(ctx, D, held, qs, m, hist)))
+        return rc;
+    if(direct && direct_total > 0) {
+        g2_direct_emit_kernel<<<pair_blocks, kBlock, 0, ctx->stream>>>(
+            D, O3c, C4t, pairs, npairs, u12.p, u34.p, doff, lists, held.empty() ? nullptr : qs.p, m,
+            held.empty() ? nullptr : reinterpret_cast<unsigned long long*>(hist.p));
+        POMX_CHECK_LAUNCH(ctx);
+    }
+    if(!held.empty()) {
         EraseEvents ev;
-        WsBuf<uint64_t> qs;
-        WsBuf<int64_t> started;
-        if((rc = g2_settle(ctx, D, tol, held, fks, rule, ev, qs, started)))
+        if((rc = g2_settle(ctx, D, tol, held, fks, qs, m, hist, rule, ev, started)))
             return rc;
         fk_store.clear();
-        for(auto& c : held_store)
-            if((rc = g2_finish_chunk(ctx, D, c, 0.0, rule, tan, P.perm, res, nfreq, d_freq, d_acc, d_dacc,
-                                     keep != nullptr, kept, counts)))
+        for(G2Chunk* c : held)
+            if((rc = g2_finish_chunk(ctx, D, *c, 0.0, rule, tan, P.perm, res, nfreq, d_freq, d_acc, d_dacc, lists, fill,
+                                     counts)))
                 return rc;
+    }
+    stat_end(ctx, hg, 0.0, double(quads_total), 36.0 * double(direct_total) + 48.0 * double(sorted_total));
+    // 6. the Lehmann sum over the kind lists (value path)
+    if(!tan && nfreq > 0) {
+        TermSpan spans[4];
+        for(int k = 0; k < 4; ++k)
+            spans[k] = TermSpan{lists.base[k], lists.cap[k], 0, fill[k]};
+        if((rc = launch_g2_eval(ctx, spans, P.perm, res, nfreq, d_freq, d_acc)))
+            return rc;
     }
     if(keep) {
         int64_t nnr = counts[0] + counts[1], nr = counts[2] + counts[3];
@@ -2527,30 +2737,17 @@ int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* 
         POMX_HIP(ctx, device_malloc(ctx, reinterpret_cast<void**>(&keep->data), bytes));
         double* NR = keep->data;
         double* R = keep->data + 4 * nnr;
-        // kind-major concatenation of the chunks
-        int64_t at[4] = {0, counts[0], 0, counts[2]};
-        for(auto const& kc : kept) {
-            for(int k = 0; k < 2; ++k) {
-                int64_t src0 = (k == 0) ? 0 : kc.c[0];
-                for(int f = 0; f < 4; ++f)
-                    if(kc.c[k])
-                        POMX_HIP(ctx, hipMemcpyAsync(NR + f * nnr + at[k], kc.nr + f * kc.nr_cap + src0,
-                                                     sizeof(double) * kc.c[k], hipMemcpyDeviceToDevice, ctx->stream));
-                at[k] += kc.c[k];
-            }
-            for(int k = 2; k < 4; ++k) {
-                int64_t src0 = (k == 2) ? 0 : kc.c[2];
-                for(int f = 0; f < 5; ++f)
-                    if(kc.c[k])
-                        POMX_HIP(ctx, hipMemcpyAsync(R + f * nr + at[k], kc.r + f * kc.r_cap + src0,
-                                                     sizeof(double) * kc.c[k], hipMemcpyDeviceToDevice, ctx->stream));
-                at[k] += kc.c[k];
-            }
-        }
-        POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        for(auto const& kc : kept) {
-            ws_put(ctx, kc.nr);
-            ws_put(ctx, kc.r);
+        // kind-major packing: NR z2 | NR z4 in the non-resonant fields, R z2z3 | R z1z2 in the resonant ones
+        int64_t const at[4] = {0, counts[0], 0, counts[2]};
+        for(int k = 0; k < 4; ++k) {
+            if(!fill[k])
+                continue;
+            int const nf = k < 2 ? 4 : 5;
+            double* dst = k < 2 ? NR : R;
+            int64_t const len = k < 2 ? nnr : nr;
+            for(int f = 0; f < nf; ++f)
+                POMX_HIP(ctx, hipMemcpyAsync(dst + f * len + at[k], lists.base[k] + f * lists.cap[k],
+                                             sizeof(double) * fill[k], hipMemcpyDeviceToDevice, ctx->stream));
         }
         for(int k = 0; k < 4; ++k)
             keep->count[k] = counts[k];

@@ -96,9 +96,9 @@ int read_i64(pomx_ctx* ctx, const int64_t* d, int64_t* h);
 // Dense ranks of round((Ey[y] - Ex[x]) / res) over all (x, y), stored at rank[x * ny + y]; equal
 // bins get equal ranks, ascending with the bin.  'sorted' (optional) receives the x * ny + y
 // indices in rank order; *nranks the number of distinct bins; d_err[0] is set to 1 where
-// |bin| >= 1e18.
+// |bin| >= 1e18; 'uniq' (optional) receives 1 at x * ny + y where no other entry shares its bin.
 int rank_table(pomx_ctx* ctx, int nx, int ny, const double* Ex, const double* Ey, double res, WsBuf<uint32_t>& rank,
-               WsBuf<uint64_t>* sorted, int64_t* nranks, int* d_err);
+               WsBuf<uint64_t>* sorted, int64_t* nranks, int* d_err, WsBuf<uint8_t>* uniq = nullptr);
 
 // ---- TermList erase rule (the 3-point susceptibility and 2PGF merges) ---------------------
 // A TermList erases a merged term when |C| <= tol/(size+1), size+1 being the number of terms in

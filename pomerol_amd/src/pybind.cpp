@@ -183,7 +183,7 @@ PYBIND11_MODULE(_pomerol, m) {
         pomx_kernel_stat st[POMX_STAT_NCLASS];
         detail::check(pomx_kernel_stats(detail::gpu(), st), "kernel statistics");
         static char const* names[POMX_STAT_NCLASS] = {"g2_sep", "g2_diag", "g2_grouped", "g2_general", "g2_reduce",
-                                                      "pair", "g2_gen"};
+                                                      "pair", "g2_gen", "g2_sort"};
         py::dict d;
         for(int c = 0; c < POMX_STAT_NCLASS; ++c) {
             py::dict x;
