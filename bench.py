@@ -42,6 +42,18 @@ DEFAULT_SLICES = 20
 # FETCH_SIZE and WRITE_SIZE (copy-calibrated; the command and the factors are in the file)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "g2_traffic_cfg2.json")
 EVAL_CLASSES = ("g2_sep", "g2_diag", "g2_grouped", "g2_general", "g2_reduce")
+_FLOP_RULES = "work model of the algorithm the kernel runs (FMA 2, add/mul 1, v_rcp_f64 1, Newton reciprocal 5)"
+FLOP_MODEL = {
+    "g2_sep": _FLOP_RULES + ": per term and bosonic frequency 6 NR NC (complex GEMM from three real MFMA products) "
+                            "+ row and column elements (about 18 NR + 13 NC) + 2",
+    "g2_diag": _FLOP_RULES + ": per term, bosonic frequency and 16-row tile 6 x 16 x (NC + 15) (complex product "
+                             "with the diagonal values, three real MFMA products) + 18 per row element + 13 per "
+                             "diagonal value + 3",
+    "g2_grouped": _FLOP_RULES + ": per term and frequency 13.5 (|V|^2, a quarter of a shared Newton reciprocal, "
+                                "K conj(V) / |V|^2 accumulated) + 20-27 per term and thread of 8 frequencies",
+    "g2_general": _FLOP_RULES + ": per term and frequency 20.75 on purely imaginary triplets (one shared "
+                                "reciprocal per four), else 28 / 31 / 32 (the direct formula)",
+}
 KERNEL_OF_CLASS = {"g2_sep": "g2_eval_mfma_kernel", "g2_diag": "g2_eval_diag_kernel",
                    "g2_grouped": "g2_eval_s_kernel", "g2_general": "g2_eval_kernel",
                    "g2_reduce": "g2_eval_reduce_kernel"}
@@ -145,9 +157,7 @@ def roofline(stats, wall_s):
            "kernel": KERNEL_OF_CLASS[dom], "launches": s["launches"], "avg_launch_ms": s["ms"] / s["launches"],
            "flop_per_launch": s["flop"] / s["launches"],
            "algorithmic_bytes_per_launch": s["bytes"] / s["launches"],
-           "flop_model": "work model of the algorithm the kernel runs (FMA 2, add/mul 1, v_rcp_f64 1): per term and "
-                         "bosonic frequency 6 NR NC (complex GEMM from three real MFMA products) + row and column "
-                         "elements (about 18 NR + 13 NC) + 2",
+           "flop_model": FLOP_MODEL[dom],
            "traffic_unit": "HBM bytes per dispatch (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; copy-calibrated)",
            "traffic_source": traffic}
     all_eval = sum(stats[n]["ms"] for n in EVAL_CLASSES) * 1e-3
@@ -291,6 +301,111 @@ def run_batch(args):
                                          "+ G on 128 frequencies, one instance per bath size, weighted by the mix",
                                "max_rel_diff_batch_vs_oracle": rel}
         log(f"[bench] batch cpu baseline: {out['cpu_baseline']}")
+    return out
+
+
+def run_cfg3(args):
+    """Config 3 end to end on the GPU: the 3x3 periodic Hubbard cluster (18 spin-orbitals,
+    262144 states, 100 N/Sz blocks up to 15876), t = 1, seeded t', U = 4, half filling, beta = 5.
+    Hamiltonian blocks and eigensolve (timed), truncateBlocks(1e-8), the rotated c / c^+ of the
+    --cfg3-sites up-spin sites (the down-spin ones are their spin images), G_ij(i w_n) for every
+    pair of them on 128 fermionic frequencies, and the transverse spin susceptibility
+    <S+_0 ; S-_0>(i nu_m) on 32 bosonic frequencies (CoefficientTolerance 1e-8, PoleResolution
+    1e-8 as the reference's defaults).
+
+    CPU baseline (the oracle, kind "port"): its Householder + implicit-QR eigensolver
+    (HamiltonianPart.cpp:98) timed on the leading n_s x n_s block of the largest Hamiltonian block
+    and extrapolated to every block by n^3; its operator rotation (MonomialOperatorPart.cpp:48-92)
+    timed on config 2's c^+ operators and extrapolated by the rotation FLOP count
+    (2 n_L^2 n_R per connected block pair)."""
+    import numpy as np
+
+    import pomerol_amd as pa
+    from pomerol_amd import workloads as W
+
+    H, beta, tp = W.cfg3_expr()
+    out = {"model": f"3x3 periodic Hubbard, t=1, t'={tp:.4f}, U=4, mu=2, beta={beta}", "states": 1 << 18}
+    t0 = time.perf_counter()
+    e = pa.ED(W.terms(H), beta, not args.no_cpu)  # (Hamiltonian blocks kept for the CPU sample)
+    out["ed_seconds"] = time.perf_counter() - t0
+    out["eigensolve_seconds"] = e.t_eigh
+    sizes = [len(b) for b in e.blocks()]
+    out["blocks"] = len(sizes)
+    out["largest_block"] = max(sizes)
+    e.truncate_blocks(1e-8)
+    out["retained_blocks"] = int(sum(e.retained()))
+    log(f"[bench] cfg3: ED {out['ed_seconds']:.2f}s (eigensolve {e.t_eigh:.2f}s), {len(sizes)} blocks, "
+        f"retained {out['retained_blocks']}")
+    idx = [tuple(x) for x in e.indices()]
+    ns = max(1, min(9, args.cfg3_sites))
+    up = [idx.index((f"s{k:02d}", 0, 1)) for k in range(ns)]
+    e.use_indices(up)
+    zf = W.g1_grid(128, beta)
+    t0 = time.perf_counter()
+    G = np.empty((ns, ns, len(zf)), complex)
+    nterms = 0
+    for a, i in enumerate(up):
+        for b, j in enumerate(up):
+            r = e.gf(i, j, zf, 1e-8, 1e-8)
+            G[a, b] = r["values"]
+            nterms += int(r["n_terms"])
+    out["g_seconds"] = time.perf_counter() - t0
+    out["rotation_seconds"] = e.t_rot
+    out["g_pairs"] = ns * ns
+    out["g_terms"] = nterms
+    out["g_symmetric_max_diff"] = float(np.abs(G - G.transpose(1, 0, 2)).max())
+    d0 = idx.index(("s00", 0, 0))
+    zb = W.bosonic(np.arange(32), beta)
+    t0 = time.perf_counter()
+    x = e.chi([(True, up[0]), (False, d0)], [(True, d0), (False, up[0])], zb, 1e-8, 1e-8)
+    out["chi_seconds"] = time.perf_counter() - t0
+    out["chi_terms"] = int(x["n_terms"])
+    out["chi_static"] = float(np.real(x["values"][0]))
+    log(f"[bench] cfg3: {ns} sites, G_ij ({ns * ns} pairs, {nterms} terms) {out['g_seconds']:.2f}s incl. rotation "
+        f"{e.t_rot:.2f}s; chi S+S- {out['chi_seconds']:.2f}s ({out['chi_terms']} terms)")
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pomerol_oracle as o
+
+        b = int(np.argmax(sizes))
+        n_s = min(max(sizes), args.cfg3_cpu_n)
+        A = np.asarray(e.hmat(b))[:n_s, :n_s].copy()
+        t0 = time.perf_counter()
+        o.ED.eigh(A, "qr")
+        t_s = time.perf_counter() - t0
+        t_eigh = t_s * sum((n / n_s) ** 3 for n in sizes)
+        # rotation rate on config 2's c^+ operators (whole ED built first, not timed)
+        c2 = W.cfg2()
+        H2 = c2.expr()
+        ed2 = o.ED(len(H2.indices()), H2.mode_terms(), c2.beta, "qr")
+        flop = 0.0
+        t0 = time.perf_counter()
+        for i in range(len(H2.indices())):
+            for part in ed2.rotate([(True, i)], 1e-8):
+                nl, nr = part["m"].shape
+                flop += 2.0 * nl * nl * nr
+        t_rot2 = time.perf_counter() - t0
+        rate = flop / t_rot2
+        # the config-3 rotation of the ns c^+ operators: every (n_up, n_dn) block pair they connect
+        spin = {k: t[2] for k, t in enumerate(idx)}
+        sector = {}
+        for bl in e.blocks():
+            st = bl[0]
+            nu = sum(1 for k in range(len(idx)) if (st >> k) & 1 and spin[k] == 1)
+            sector[(nu, sum(1 for k in range(len(idx)) if (st >> k) & 1 and spin[k] == 0))] = len(bl)
+        flop3 = 0.0
+        for (nu, nd), n_r in sector.items():
+            n_l = sector.get((nu + 1, nd), 0)
+            flop3 += ns * 2.0 * n_l * n_l * n_r
+        out["cpu_baseline"] = {
+            "eigensolve_seconds_extrapolated": t_eigh, "rotation_seconds_extrapolated": flop3 / rate,
+            "cores": 1, "kind": "port",
+            "sample": f"oracle Householder + implicit QR on the leading {n_s} x {n_s} block of the largest "
+                      f"({max(sizes)}) block: {t_s:.2f}s, scaled by n^3 over the {len(sizes)} blocks; oracle rotation "
+                      f"of config 2's {len(H2.indices())} c^+ operators ({flop:.3g} FLOP, {t_rot2:.2f}s, "
+                      f"{rate / 1e9:.2f} GFLOP/s) scaled to the {ns} config-3 operators' {flop3:.3g} FLOP",
+            "gpu_vs_cpu_eigensolve": t_eigh / max(e.t_eigh, 1e-9)}
+        log(f"[bench] cfg3 cpu: {out['cpu_baseline']}")
     return out
 
 
@@ -534,6 +649,9 @@ def main():
     ap.add_argument("--chi3-n", type=int, default=64, help="3-point susceptibility leg: n x n pairs (0: skip)")
     ap.add_argument("--chi3-only", action="store_true", help="only the 3-point susceptibility leg")
     ap.add_argument("--grad-only", action="store_true", help="only the bath-fit gradient leg")
+    ap.add_argument("--cfg3-sites", type=int, default=9, help="config-3 leg: up-spin sites of G_ij (0: skip the leg)")
+    ap.add_argument("--cfg3-cpu-n", type=int, default=2000, help="config-3 CPU eigensolve sample size")
+    ap.add_argument("--cfg3-only", action="store_true", help="only the config-3 leg")
     ap.add_argument("--g2-only", action="store_true", help="only the 2PGF leg (and its CPU baseline)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -552,6 +670,9 @@ def main():
         return
     if args.grad_only:
         print(json.dumps({"grad": run_grad(args)}), flush=True)
+        return
+    if args.cfg3_only:
+        print(json.dumps({"cfg3": run_cfg3(args)}), flush=True)
         return
 
     import numpy as np
@@ -624,6 +745,7 @@ def main():
     dense = run_dense(args) if (args.eigh_n > 0 or args.gemm_n > 0) and not g2_only else None
     chi3 = run_chi3(args) if args.chi3_n > 0 and not g2_only else None
     grad = run_grad(args) if args.grad_inst > 0 and not g2_only else None
+    cfg3 = run_cfg3(args) if args.cfg3_sites > 0 and not g2_only else None
 
     out = {
         "metric": "2PGF terms*freqs/s",
@@ -651,6 +773,7 @@ def main():
         "dense": dense,
         "chi3": chi3,
         "grad": grad,
+        "cfg3": cfg3,
     }
     print(json.dumps(out), flush=True)
 

@@ -31,6 +31,11 @@ struct G2Dev {
     const double *E1, *E2, *E3, *E4;
     const double *w1, *w2, *w3, *w4;
     double sign, beta, tol;
+    // bounds of the quadruple sweep (row_bound): wsm[i] = max_{j >= i} w4[j], cdm[i1 n3 + i3] =
+    // max_i4 |O3(i3, i4) CX4(i4, i1)|, bf = max(beta, 1) (1 + 1e-9)
+    const double* wsm = nullptr;
+    const double* cdm = nullptr;
+    double bf = 0.0;
 };
 
 // derivatives of a part's inputs along a Hamiltonian direction (pomx_g2_tangent)
@@ -127,9 +132,22 @@ __device__ inline RowCtx row_ctx(G2Dev const& D, uint64_t pr, int i3) {
     r.wj = D.w2[r.i2];
     r.wk = D.w3[i3];
     r.s3 = r.wi + r.wj + r.wk;
-    // w4 is non-increasing (ascending energies), so w4[0] bounds every i4
-    r.active = (a != 0.0 && b != 0.0) && (r.s3 + D.w4[0] > D.tol);
+    r.active = (a != 0.0 && b != 0.0) && (r.s3 + D.wsm[0] > D.tol); // wsm[0] = max w4
     return r;
+}
+
+// Pruning of the sweep over i4.  Every coefficient kinds_of tests (C (w2 + w3), C (w1 + w4),
+// C beta w2, C (w2 - w4), C beta w1, C (w3 - w1)) is at most |C| max(beta, 1) (s3 + w4) for
+// non-negative weights, and |C| <= |O1 O2| cdm[i1][i3].  With Bs = |O1 O2| cdm[i1][i3] bf (the
+// factor 1 + 1e-9 in bf covers every rounding of those tests and of this bound), i4 can give a
+// term only if Bs (s3 + w4[i4]) > tol, hence only if Bs (s3 + wsm[i4]) > tol; wsm is
+// non-increasing, so once that fails for some i4 it fails for every later one.  With tol = 0
+// nothing is pruned.
+__device__ inline double row_bound(G2Dev const& D, double ab, int i1, int i3) {
+    return fabs(ab) * D.cdm[int64_t(i1) * D.n3 + i3] * D.bf;
+}
+__device__ inline bool i4_may_pass(G2Dev const& D, double Bs, double s3, int i4) {
+    return !(D.tol > 0.0) || Bs * (s3 + D.wsm[i4]) > D.tol;
 }
 
 // Rows of one chunk in the reference's visiting order (i1, i3, i2): the chunk's (i1, i2) pairs
@@ -173,6 +191,47 @@ __global__ void g2_stage_kernel(G2Dev D, double* __restrict__ O3c, double* __res
     }
 }
 
+// wsm[i] = max_{j >= i} w4[j] (one block of kBlock threads, each over a contiguous segment)
+__global__ void __launch_bounds__(256) suffix_max_kernel(int n, const double* __restrict__ w, double* __restrict__ wsm) {
+    __shared__ double seg_max[256];
+    int const t = threadIdx.x, seg = (n + blockDim.x - 1) / blockDim.x;
+    int const lo = min(n, t * seg), hi = min(n, lo + seg);
+    double m = 0.0; // weights are non-negative
+    for(int i = hi - 1; i >= lo; --i)
+        m = fmax(m, w[i]);
+    seg_max[t] = m;
+    __syncthreads();
+    if(t == 0) // maxima of the segments to the right, from the last segment down
+        for(int u = int(blockDim.x) - 2; u >= 0; --u)
+            seg_max[u] = fmax(seg_max[u], seg_max[u + 1]);
+    __syncthreads();
+    double r = (t + 1 < int(blockDim.x)) ? seg_max[t + 1] : 0.0;
+    for(int i = hi - 1; i >= lo; --i) {
+        r = fmax(r, w[i]);
+        wsm[i] = r;
+    }
+}
+
+// cdm[i1 n3 + i3] = max_i4 |O3c[i3][i4] C4t[i1][i4]| (one wave per entry, lanes over i4)
+__global__ void __launch_bounds__(256) cd_max_kernel(G2Dev D, const double* __restrict__ O3c,
+                                                     const double* __restrict__ C4t, double* __restrict__ cdm) {
+    int const lane = threadIdx.x & 63;
+    int64_t const n = int64_t(D.n1) * D.n3;
+    int64_t const nwaves = int64_t(gridDim.x) * (blockDim.x >> 6);
+    for(int64_t e = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6; e < n; e += nwaves) {
+        int const i1 = int(e / D.n3), i3 = int(e % D.n3);
+        const double* o3 = O3c + int64_t(i3) * D.n4;
+        const double* c4 = C4t + int64_t(i1) * D.n4;
+        double m = 0.0;
+        for(int i4 = lane; i4 < D.n4; i4 += 64)
+            m = fmax(m, fabs(o3[i4] * c4[i4]));
+        for(int o = 32; o > 0; o >>= 1)
+            m = fmax(m, __shfl_xor(m, o, 64));
+        if(lane == 0)
+            cdm[e] = m;
+    }
+}
+
 // Direct terms.  A term's (kind, bin(P1), bin(P2), bin(P3)) can equal another term's only if its
 // (i1, i2) shares the P1 bin with another pair or its (i3, i4) shares the P3 bin with another pair
 // (equal bins of both pin (i1, i2) and (i3, i4), hence the quadruple).  Terms whose (i1, i2) and
@@ -180,9 +239,39 @@ __global__ void g2_stage_kernel(G2Dev D, double* __restrict__ O3c, double* __res
 // skip the sort and the reduce and are written straight into the merged term lists (direct);
 // the others (shared bins; 0.3 % of the config-2 terms) go through the sort (sorted).
 
-// One wave per (i1, i2) pair (position pos of the r12-sorted pair list), rows i3 in turn, lanes
-// over i4: the sorted terms per row (rowcnt[pos n3 + i3]) and per pair (paircnt[pos]), and the
-// direct terms per pair and kind (dpc[k npairs + pos]).
+// The live rows of one (i1, i2) pair, 64 at a time: lane l tests row i3 = b3 + l (non-zero
+// O1 O2, and some i4 that may pass, i4_may_pass at i4 = 0), and the wave then visits the live rows
+// in ascending i3 with its lanes over i4, up to the first 64 i4 of which none may pass.
+struct PairRows {
+    int i1, i2;
+    double a, wi, wj;
+};
+__device__ inline PairRows pair_rows(G2Dev const& D, uint64_t pr) {
+    PairRows p;
+    p.i1 = int(pr / uint64_t(D.n2));
+    p.i2 = int(pr % uint64_t(D.n2));
+    p.a = mat(D.O1, p.i1, p.i2);
+    p.wi = D.w1[p.i1];
+    p.wj = D.w2[p.i2];
+    return p;
+}
+// lane's row i3: ab, s3, wk, Bs; returns whether it is live
+__device__ inline bool lane_row(G2Dev const& D, PairRows const& p, int i3, double& ab, double& s3, double& wk,
+                                double& Bs) {
+    ab = s3 = wk = Bs = 0.0;
+    if(p.a == 0.0 || i3 >= D.n3)
+        return false;
+    double const b = mat(D.O2, p.i2, i3);
+    ab = p.a * b;
+    wk = D.w3[i3];
+    s3 = p.wi + p.wj + wk;
+    Bs = row_bound(D, ab, p.i1, i3);
+    return b != 0.0 && (s3 + D.wsm[0] > D.tol) && i4_may_pass(D, Bs, s3, 0);
+}
+
+// One wave per (i1, i2) pair (position pos of the r12-sorted pair list), live rows i3 in turn,
+// lanes over i4: the sorted terms per row (rowcnt[pos n3 + i3]) and per pair (paircnt[pos]), and
+// the direct terms per pair and kind (dpc[k npairs + pos]).
 __global__ void __launch_bounds__(256) g2_count_kernel(G2Dev D, const double* __restrict__ O3c,
                                                        const double* __restrict__ C4t, const uint64_t* __restrict__ pairs,
                                                        int64_t npairs, const uint8_t* __restrict__ u12,
@@ -192,30 +281,47 @@ __global__ void __launch_bounds__(256) g2_count_kernel(G2Dev D, const double* __
     int64_t const nwaves = int64_t(gridDim.x) * (blockDim.x >> 6);
     for(int64_t pos = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6; pos < npairs; pos += nwaves) {
         uint64_t const pr = pairs[pos];
+        PairRows const P = pair_rows(D, pr);
         bool const d12 = u12 && u12[pr];
         int64_t sorted_pair = 0;
         int dk[4] = {0, 0, 0, 0};
-        for(int i3 = 0; i3 < D.n3; ++i3) {
-            RowCtx r = row_ctx(D, pr, i3);
-            int c = 0;
-            if(r.active) {
+        const double* c4 = C4t + int64_t(P.i1) * D.n4;
+        for(int b3 = 0; b3 < D.n3; b3 += 64) {
+            double lab, ls3, lwk, lBs;
+            uint64_t live = __ballot(lane_row(D, P, b3 + lane, lab, ls3, lwk, lBs));
+            int mine = 0; // sorted terms of this lane's row
+            while(live) {
+                int const l = __ffsll((unsigned long long)live) - 1;
+                live &= live - 1;
+                int const i3 = b3 + l;
+                double const ab = __shfl(lab, l, 64), s3 = __shfl(ls3, l, 64), wk = __shfl(lwk, l, 64);
+                double const Bs = __shfl(lBs, l, 64);
                 const double* o3 = O3c + int64_t(i3) * D.n4;
-                const double* c4 = C4t + int64_t(r.i1) * D.n4;
-                for(int i4 = lane; i4 < D.n4; i4 += 64) {
-                    unsigned const m = kinds_of(D, r.ab, r.s3, r.wi, r.wj, r.wk, o3[i4], c4[i4], D.w4[i4]);
-                    if(d12 && u34[int64_t(i3) * D.n4 + i4]) {
-                        dk[0] += int(m & 1u);
-                        dk[1] += int((m >> 1) & 1u);
-                        dk[2] += int((m >> 2) & 1u);
-                        dk[3] += int((m >> 3) & 1u);
-                    } else
-                        c += __popc(m);
+                const uint8_t* u34row = u34 ? u34 + int64_t(i3) * D.n4 : nullptr;
+                int c = 0;
+                for(int b4 = 0; b4 < D.n4; b4 += 64) {
+                    int const i4 = b4 + lane;
+                    bool const may = i4 < D.n4 && i4_may_pass(D, Bs, s3, i4);
+                    if(!__ballot(may))
+                        break;
+                    if(may) {
+                        unsigned const m = kinds_of(D, ab, s3, P.wi, P.wj, wk, o3[i4], c4[i4], D.w4[i4]);
+                        if(d12 && u34row[i4]) {
+                            dk[0] += int(m & 1u);
+                            dk[1] += int((m >> 1) & 1u);
+                            dk[2] += int((m >> 2) & 1u);
+                            dk[3] += int((m >> 3) & 1u);
+                        } else
+                            c += __popc(m);
+                    }
                 }
+                c = wave_sum_int(c);
+                sorted_pair += c;
+                if(lane == l)
+                    mine = c;
             }
-            c = wave_sum_int(c);
-            sorted_pair += c;
-            if(lane == 0)
-                rowcnt[pos * D.n3 + i3] = c;
+            if(b3 + lane < D.n3)
+                rowcnt[pos * D.n3 + b3 + lane] = mine;
         }
 #pragma unroll
         for(int k = 0; k < 4; ++k)
@@ -264,12 +370,15 @@ __global__ void __launch_bounds__(256) g2_emit_kernel(G2Dev D, const double* __r
         uint4 pr;
         int i3;
         chunk_row(rmap, D.n3, lr, pr, i3);
+        int64_t at = off[lr];
+        int64_t const end = (lr + 1 < nrows) ? off[lr + 1] : ntot; // never write past this row's slots
+        if(at >= end) // no sorted term in this row
+            continue;
         RowCtx r = row_ctx(D, pr.x, i3);
         bool const d12 = u12 && u12[pr.x]; // direct terms of this row are written elsewhere
         if(!r.active)
             continue;
-        int64_t at = off[lr];
-        int64_t const end = (lr + 1 < nrows) ? off[lr + 1] : ntot; // never write past this row's slots
+        double const Bs = row_bound(D, r.ab, r.i1, r.i3);
         uint64_t const k12 = uint64_t(r12[int64_t(r.i1) * D.n2 + r.i2]) << 42;
         uint64_t const k23 = uint64_t(r23[int64_t(r.i2) * D.n3 + r.i3]) << 21;
         uint64_t const base = uint64_t(r.i1) | (uint64_t(r.i2) << 16) | (uint64_t(r.i3) << 32);
@@ -278,8 +387,11 @@ __global__ void __launch_bounds__(256) g2_emit_kernel(G2Dev D, const double* __r
         const uint32_t* r34row = r34 + int64_t(r.i3) * D.n4;
         for(int b4 = 0; b4 < D.n4; b4 += 64) {
             int const i4 = b4 + lane;
+            bool const may = i4 < D.n4 && i4_may_pass(D, Bs, r.s3, i4);
+            if(!__ballot(may))
+                break;
             unsigned m = 0u;
-            if(i4 < D.n4 && !(d12 && u34[int64_t(r.i3) * D.n4 + i4]))
+            if(may && !(d12 && u34[int64_t(r.i3) * D.n4 + i4]))
                 m = kinds_of(D, r.ab, r.s3, r.wi, r.wj, r.wk, o3[i4], c4[i4], D.w4[i4]);
             int total;
             int64_t p = at + wave_excl_scan_int(__popc(m), lane, total);
@@ -400,8 +512,11 @@ __global__ void lookup_started_kernel(const uint64_t* __restrict__ q, int64_t m,
 // Direct terms: one wave per pair with unique bins, rows i3 in turn, lanes over i4; the terms of
 // each kind are placed by ballot prefix counts from the pair's offset doff[k npairs + pos].  With
 // q (the part's merge positions, m entries) the visiting position of every direct term, a group of
-// one, is counted into the gap between merges it falls in (as g2_first_hist_kernel does).
-__global__ void __launch_bounds__(256) g2_direct_emit_kernel(G2Dev D, const double* __restrict__ O3c,
+// one, is counted into the gap between merges it falls in (as g2_first_hist_kernel does).  TAN:
+// the tangent kind lists, each term also carrying the derivatives of its coefficients and poles
+// along TD (as g2_reduce_tangent_kernel forms them for a group of one).
+template <bool TAN>
+__global__ void __launch_bounds__(256) g2_direct_emit_kernel(G2Dev D, G2Tan TD, const double* __restrict__ O3c,
                                                              const double* __restrict__ C4t,
                                                              const uint64_t* __restrict__ pairs, int64_t npairs,
                                                              const uint8_t* __restrict__ u12,
@@ -422,27 +537,55 @@ __global__ void __launch_bounds__(256) g2_direct_emit_kernel(G2Dev D, const doub
         uint64_t const pr = pairs[pos];
         if(!u12[pr])
             continue;
+        PairRows const PR = pair_rows(D, pr);
         int64_t at[4];
 #pragma unroll
         for(int k = 0; k < 4; ++k)
             at[k] = doff[k * npairs + pos];
-        for(int i3 = 0; i3 < D.n3; ++i3) {
-            RowCtx r = row_ctx(D, pr, i3);
-            if(!r.active)
-                continue;
+        const double* c4 = C4t + int64_t(PR.i1) * D.n4;
+        double const P1 = D.E2[PR.i2] - D.E1[PR.i1];
+        for(int b3 = 0; b3 < D.n3; b3 += 64) {
+            double lab, ls3, lwk, lBs;
+            uint64_t live = __ballot(lane_row(D, PR, b3 + lane, lab, ls3, lwk, lBs));
+            while(live) {
+            int const l = __ffsll((unsigned long long)live) - 1;
+            live &= live - 1;
+            int const i3 = b3 + l;
+            double const ab = __shfl(lab, l, 64), s3 = __shfl(ls3, l, 64), wk = __shfl(lwk, l, 64);
+            double const Bs = __shfl(lBs, l, 64);
             const double* o3 = O3c + int64_t(i3) * D.n4;
-            const double* c4 = C4t + int64_t(r.i1) * D.n4;
-            double const P1 = D.E2[r.i2] - D.E1[r.i1], P2 = D.E3[i3] - D.E2[r.i2];
+            double const P2 = D.E3[i3] - D.E2[PR.i2];
+            // tangent: row constants (o2, dw1..dw3, dP1, dP2)
+            double o2 = 0.0, dwi = 0.0, dwj = 0.0, dwk = 0.0, dP1 = 0.0, dP2 = 0.0;
+            if constexpr(TAN) {
+                o2 = mat(D.O2, PR.i2, i3);
+                dwi = TD.dw1[PR.i1];
+                dwj = TD.dw2[PR.i2];
+                dwk = TD.dw3[i3];
+                dP1 = TD.dE2[PR.i2] - TD.dE1[PR.i1];
+                dP2 = TD.dE3[i3] - TD.dE2[PR.i2];
+            }
             for(int b4 = 0; b4 < D.n4; b4 += 64) {
                 int const i4 = b4 + lane;
+                bool const may = i4 < D.n4 && i4_may_pass(D, Bs, s3, i4);
+                if(!__ballot(may))
+                    break;
                 unsigned mk = 0u;
-                double C = 0.0, wl = 0.0, P3 = 0.0;
-                if(i4 < D.n4 && u34[int64_t(i3) * D.n4 + i4]) {
+                double C = 0.0, wl = 0.0, P3 = 0.0, dC = 0.0, dwl = 0.0, dP3 = 0.0;
+                if(may && u34[int64_t(i3) * D.n4 + i4]) {
                     double const c = o3[i4], d = c4[i4];
                     wl = D.w4[i4];
-                    mk = kinds_of(D, r.ab, r.s3, r.wi, r.wj, r.wk, c, d, wl);
-                    C = r.ab * c * d * D.sign; // ((O1 O2) O3) CX4 sign, as the reduce forms it
+                    mk = kinds_of(D, ab, s3, PR.wi, PR.wj, wk, c, d, wl);
+                    C = ab * c * d * D.sign; // ((O1 O2) O3) CX4 sign, as the reduce forms it
                     P3 = D.E4[i4] - D.E3[i3];
+                    if constexpr(TAN) { // as g2_reduce_tangent_kernel: d(o1 o2 o3 o4) sign
+                        double const o1 = PR.a;
+                        dC = (mat(TD.dO1, PR.i1, PR.i2) * o2 * c * d + o1 * mat(TD.dO2, PR.i2, i3) * c * d +
+                              o1 * o2 * mat(TD.dO3, i3, i4) * d + o1 * o2 * c * mat(TD.dCX4, i4, PR.i1)) *
+                             D.sign;
+                        dwl = TD.dw4[i4];
+                        dP3 = TD.dE4[i4] - TD.dE3[i3];
+                    }
                 }
 #pragma unroll
                 for(int k = 0; k < 4; ++k) {
@@ -453,19 +596,35 @@ __global__ void __launch_bounds__(256) g2_direct_emit_kernel(G2Dev D, const doub
                         double* B = out.base[k];
                         int64_t const cap = out.cap[k];
                         if(k < 2) {
-                            B[t] = (k == 0) ? -C * (r.wj + r.wk) : C * (r.wi + wl);
+                            B[t] = (k == 0) ? -C * (PR.wj + wk) : C * (PR.wi + wl);
                             B[cap + t] = P1;
                             B[2 * cap + t] = P2;
                             B[3 * cap + t] = P3;
+                            if constexpr(TAN) {
+                                B[4 * cap + t] = (k == 0) ? -(dC * (PR.wj + wk) + C * (dwj + dwk))
+                                                          : dC * (PR.wi + wl) + C * (dwi + dwl);
+                                B[5 * cap + t] = dP1;
+                                B[6 * cap + t] = dP2;
+                                B[7 * cap + t] = dP3;
+                            }
                         } else {
-                            B[t] = (k == 2) ? -C * D.beta * r.wj : C * D.beta * r.wi;
-                            B[cap + t] = (k == 2) ? C * (r.wj - wl) : C * (r.wk - r.wi);
+                            B[t] = (k == 2) ? -C * D.beta * PR.wj : C * D.beta * PR.wi;
+                            B[cap + t] = (k == 2) ? C * (PR.wj - wl) : C * (wk - PR.wi);
                             B[2 * cap + t] = P1;
                             B[3 * cap + t] = P2;
                             B[4 * cap + t] = P3;
+                            if constexpr(TAN) {
+                                B[5 * cap + t] = (k == 2) ? -D.beta * (dC * PR.wj + C * dwj)
+                                                          : D.beta * (dC * PR.wi + C * dwi);
+                                B[6 * cap + t] = (k == 2) ? dC * (PR.wj - wl) + C * (dwj - dwl)
+                                                          : dC * (wk - PR.wi) + C * (dwk - dwi);
+                                B[7 * cap + t] = dP1;
+                                B[8 * cap + t] = dP2;
+                                B[9 * cap + t] = dP3;
+                            }
                         }
                         if(q) {
-                            int64_t const j = upper_index(q, 0, m, g2_position(D, k, r.i1, r.i2, i3, i4));
+                            int64_t const j = upper_index(q, 0, m, g2_position(D, k, PR.i1, PR.i2, i3, i4));
                             if(local)
                                 atomicAdd(&lh[j], 1u);
                             else
@@ -474,6 +633,7 @@ __global__ void __launch_bounds__(256) g2_direct_emit_kernel(G2Dev D, const doub
                     }
                     at[k] += __popcll(ball);
                 }
+            }
             }
         }
     }
@@ -700,29 +860,29 @@ __global__ void g2_reduce_tangent_kernel(G2Dev D, G2Tan T, int64_t n, int64_t ng
     live[gi] = alive ? 1 : 0;
 }
 
-// Live tangent groups into kind-segmented SoA lists: NR (c, P1, P2, P3, dc, dP1, dP2, dP3),
-// R (Rc, Nc, P1, P2, P3, dRc, dNc, dP1, dP2, dP3).
+// Live tangent groups (g2_reduce_tangent_kernel) into the part's tangent kind lists: NR (c, P1, P2,
+// P3, dc, dP1, dP2, dP3), R (Rc, Nc, P1, P2, P3, dRc, dNc, dP1, dP2, dP3).
 __global__ void g2_compact_tangent_kernel(int64_t ngroups, const int64_t* __restrict__ live,
                                           const int64_t* __restrict__ lpos, const double* __restrict__ g,
                                           const int64_t* __restrict__ start, const uint64_t* __restrict__ keys,
-                                          int64_t nr_cap, int64_t r_cap, int64_t nr_live,
-                                          double* __restrict__ nr, double* __restrict__ rs) {
+                                          KindLists out, KindSlots slot) {
     int64_t gi = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if(gi >= ngroups || !live[gi])
         return;
-    int kind = int(keys[start[gi]] >> 62);
-    int64_t at = lpos[gi];
+    int const kind = int(keys[start[gi]] >> 62);
+    int64_t const t = slot.at[kind] + (lpos[gi] - slot.first[kind]);
+    double* B = out.base[kind];
+    int64_t const cap = out.cap[kind];
     const double* x = g + gi * 10;
     if(kind < 2) {
         int const f[8] = {0, 2, 3, 4, 5, 7, 8, 9};
 #pragma unroll
         for(int q = 0; q < 8; ++q)
-            nr[q * nr_cap + at] = x[f[q]];
+            B[q * cap + t] = x[f[q]];
     } else {
-        at -= nr_live; // resonant terms follow the nr_live non-resonant ones in the live order
 #pragma unroll
         for(int q = 0; q < 10; ++q)
-            rs[q * r_cap + at] = x[q];
+            B[q * cap + t] = x[q];
     }
 }
 
@@ -1221,18 +1381,58 @@ __host__ __device__ constexpr SepPlan sep_plan_of(int kind, int perm) {
     return pl;
 }
 
+// ---- three-factor resonant terms split into a separable and a diagonal part -----------------
+// The resonant kinds under the permutations where one of D1, D3 and E depend on F2 both, and E on
+// F2 plus F1 or F3 (kind 2 under (F1, F3, F2) and (F3, F1, F2), kind 3 under (F2, F1, F3) and
+// (F2, F3, F1)), have three frequency-dependent factors.  With x = F2 and the two F2 factors
+// (x - a)(x - b), 1 / ((x - a)(x - b)) = (1 / (x - a) - 1 / (x - b)) / (a - b), where a - b
+// = F1 - P or F3 - P (never zero: Matsubara F1, F3 are odd multiples of i pi / beta).  Then
+//   Nc / (D1 D3 E) = Nc / ((s0 - P1)(s1 - P2)(s2 - P3))    [a kind-0 term, same permutation]
+//                  - Nc / ((F1 - Pa)(F3 - Pb) E)           [a diagonal term, E unchanged],
+//   kind 2, perm 1:  a - b = F3 - P2, Pa = P1, Pb = P2, E = F2 + F3 - P2 - P3 (d grid)
+//   kind 2, perm 4:  a - b = F1 - P2, Pa = P2, Pb = P1, E = F1 + F2 - P2 - P3 (s grid)
+//   kind 3, perm 2:  a - b = F1 - P2, Pa = P2, Pb = P3, E = F1 + F2 - P1 - P2 (s grid)
+//   kind 3, perm 3:  a - b = F3 - P2, Pa = P3, Pb = P2, E = F2 + F3 - P1 - P2 (d grid)
+// (checked against the direct formula on the CPU).  The diagonal kernel (dkind, dperm) forms
+// Pa = Q1, Pb = Q3 and E's real part -(Q2 + Q3) (dkind 2) or -(Q1 + Q2) (dkind 3), so the split
+// feeds it Q = (P[q0], P[q1], P[q2]); E's real part is then the same rounded sum of two poles.
+// Only terms that can never resonate on a purely imaginary grid ((P sum)^2 > res^2, so that
+// |E|^2 > res^2 at every frequency) are split; the others keep the grouped vector kernel.
+struct SplitPlan {
+    int dkind, dperm, q[3];
+};
+__host__ __device__ constexpr SplitPlan split_plan_of(int kind, int perm) {
+    return (kind == 2 && perm == 1)   ? SplitPlan{2, 0, {0, 2, 1}}
+           : (kind == 2 && perm == 4) ? SplitPlan{3, 0, {1, 2, 0}}
+           : (kind == 3 && perm == 2) ? SplitPlan{3, 0, {1, 0, 2}}
+           : (kind == 3 && perm == 3) ? SplitPlan{2, 0, {2, 0, 1}}
+                                      : SplitPlan{0, 0, {0, 1, 2}};
+}
+__host__ __device__ constexpr bool split_kind_perm(int kind, int perm) {
+    return (kind == 2 && (perm == 1 || perm == 4)) || (kind == 3 && (perm == 2 || perm == 3));
+}
+// E's real part is -(P2 + P3) (kind 2) or -(P1 + P2) (kind 3); its square above res^2 keeps |E|
+// above res at every purely imaginary frequency
+template <int KIND>
+__host__ __device__ inline bool never_resonant(double P1, double P2, double P3, double res2) {
+    double const e = (KIND == 2) ? P2 + P3 : P1 + P2;
+    return e * e > res2;
+}
+
 // Term staging of the matrix-core kernels.  A wave works through its slice of terms in chunks of
 // kStage; each lane loads one term of the next chunk while the current one is evaluated, and at the
 // chunk boundary the wave stores its loads into its own LDS slots, from which every step reads its
-// 4 terms (16 lanes per address).  The global loads thus have a whole chunk to arrive.  Fields:
-// 0 = C (Rc), 1 = Nc (resonant lists only), 2..4 = P1, P2, P3; terms past the slice get zero
-// coefficients.
+// 4 terms (16 lanes per address).  The global loads thus have a whole chunk to arrive.  Staged
+// slots: 0 = C (Rc), 1 = Nc (resonant lists only), 2..4 = P1, P2, P3, and for tangent lists
+// (NR: C, P1, P2, P3, dC, dP1, dP2, dP3; R: Rc, Nc, P1, P2, P3, dRc, dNc, dP1, dP2, dP3) 5 = dC
+// (dRc), 6 = dNc, 7..9 = dP1, dP2, dP3; terms past the slice get zero coefficients.
 constexpr int kStage = 64;
 
-template <bool RES>
+template <bool RES, bool TAN = false>
 __device__ inline void stage_load(const double* __restrict__ T, int64_t cap, int64_t t, int64_t t0, int64_t t1,
-                                  double f[5]) {
+                                  double f[]) {
     constexpr int F0 = RES ? 1 : 0;
+    constexpr int NV = RES ? 5 : 4; // value fields
     bool const tv = t < t1;
     int64_t const tt = tv ? t : t0;
     f[0] = tv ? T[tt] : 0.0;
@@ -1240,16 +1440,24 @@ __device__ inline void stage_load(const double* __restrict__ T, int64_t cap, int
     f[2] = T[(F0 + 1) * cap + tt];
     f[3] = T[(F0 + 2) * cap + tt];
     f[4] = T[(F0 + 3) * cap + tt];
+    if constexpr(TAN) {
+        f[5] = tv ? T[NV * cap + tt] : 0.0;
+        f[6] = (RES && tv) ? T[(NV + 1) * cap + tt] : 0.0;
+        f[7] = T[(NV + F0 + 1) * cap + tt];
+        f[8] = T[(NV + F0 + 2) * cap + tt];
+        f[9] = T[(NV + F0 + 3) * cap + tt];
+    }
 }
 
 // The LDS operations of one wave are carried out in issue order, so the previous chunk's reads are
 // served before these writes and the writes before the next chunk's reads; the fences keep the
 // compiler from moving accesses across the boundary.
-__device__ inline void stage_store(double (*W)[kStage], int lane, double const f[5]) {
+template <int NS = 5>
+__device__ inline void stage_store(double (*W)[kStage], int lane, double const f[]) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
-    for(int x = 0; x < 5; ++x)
+    for(int x = 0; x < NS; ++x)
         W[x][lane] = f[x];
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -1283,6 +1491,34 @@ __device__ inline void sep_element(double Pa, double Pb, double ya, double yb, d
     }
 }
 
+// The element and its derivative along the poles' derivatives dPa, dPb:
+//   E = scale / prod_f (i y_f - P_f),  E' = E sum_f dP_f / (i y_f - P_f),
+// with 1 / (i y_f - P_f) = (-P_f - i y_f) / |.|^2 from the same shared reciprocal.
+template <int NF>
+__device__ inline void sep_element_tan(double Pa, double Pb, double dPa, double dPb, double ya, double yb, double ya2,
+                                       double yb2, double yy, double PaPb, double scale, double& er, double& ei,
+                                       double& dr, double& di) {
+    if constexpr(NF == 1) {
+        double const rr = rcp_nr(fma(Pa, Pa, ya2));
+        double const ur = -Pa * rr, ui = -ya * rr; // 1 / (i y - P)
+        er = scale * ur;
+        ei = scale * ui;
+        double const sr = dPa * ur, si = dPa * ui; // sigma
+        dr = fma(er, sr, -ei * si);
+        di = fma(er, si, ei * sr);
+    } else {
+        double const ma = fma(Pa, Pa, ya2), mb = fma(Pb, Pb, yb2);
+        double const rr = rcp_nr(ma * mb);
+        double const r = scale * rr;
+        er = (PaPb - yy) * r;
+        ei = fma(Pa, yb, Pb * ya) * r;
+        double const wa = dPa * mb, wb = dPb * ma;
+        double const sr = -rr * fma(wa, Pa, wb * Pb), si = -rr * fma(wa, ya, wb * yb);
+        dr = fma(er, sr, -ei * si);
+        di = fma(er, si, ei * sr);
+    }
+}
+
 // Resonant kinds (2: E = z2 + z3 - P2 - P3, 3: E = z1 + z2 - P1 - P2) when E = i Y_b - (P sum):
 // the column element carries the coefficient K = Rc where |E| <= res, else Nc / E.
 // Occupancy: left to itself the compiler puts the accumulators in 96 AGPRs on top of ~180 VGPRs,
@@ -1290,7 +1526,15 @@ __device__ inline void sep_element(double Pa, double Pb, double ya, double yb, d
 // instructions.  Asking for two waves per SIMD keeps everything in 184-196 VGPRs without spills;
 // with the terms staged a chunk ahead, two waves keep the matrix cores as busy as three did
 // (three fit in 168 VGPRs with the lane constants' squares recomputed, and measured no faster).
-template <int KIND, int PERM>
+// SRC = KIND: the list holds terms of kind KIND.  SRC = 2 or 3 with KIND = 0: the list holds
+// resonant terms of that kind, read as the kind-0 term Nc / ((s0 - P1)(s1 - P2)(s2 - P3)), the
+// separable half of their partial-fraction split (split_applies); terms that may resonate
+// (never_resonant false) get coefficient 0 here and are evaluated elsewhere.
+// TAN: the list is a tangent list and the kernel forms the derivative instead of the value: with
+// Row' and Col' the elements' derivatives (sep_element_tan; the column's coefficient C, or K = Rc
+// or Nc / E with dK = (dNc + K dPe) / E, carries its own), the tile is sum_t Row'_t Col_t +
+// Row_t Col'_t, two complex products per term.
+template <int KIND, int PERM, int SRC = KIND, bool TAN = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g2_eval_mfma_kernel(int64_t nterms, int64_t slice_len, const double* __restrict__ T,
                                                            int64_t cap, int NR, int NC, const double* __restrict__ yF1,
                                                            const double* __restrict__ yF2, const double* __restrict__ yF3,
@@ -1334,21 +1578,35 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g
                 acc[m][n][x] = dbl4{0.0, 0.0, 0.0, 0.0};
     // term k of each step of 4, read from the wave's staged chunk
     double const Yb = yB[b], Yb2 = Yb * Yb;
-    __shared__ double sT[4][5][kStage];
+    constexpr int NS = TAN ? 10 : 5; // staged slots per term
+    __shared__ double sT[4][NS][kStage];
     double(*W)[kStage] = sT[wave];
-    double nf[5];
+    double nf[NS];
     if(t0 < t1)
-        stage_load<(KIND >= 2)>(T, cap, t0 + lane, t0, t1, nf);
+        stage_load<(SRC >= 2), TAN>(T, cap, t0 + lane, t0, t1, nf);
     for(int64_t c0 = t0; c0 < t1; c0 += kStage) {
-        stage_store(W, lane, nf);
+        stage_store<NS>(W, lane, nf);
         if(c0 + kStage < t1)
-            stage_load<(KIND >= 2)>(T, cap, c0 + kStage + lane, t0, t1, nf);
+            stage_load<(SRC >= 2), TAN>(T, cap, c0 + kStage + lane, t0, t1, nf);
         int const nsteps = (int(min<int64_t>(kStage, t1 - c0)) + 3) / 4;
         for(int j = 0; j < nsteps; ++j) {
             int const q = 4 * j + k;
             double const Cc = W[0][q], Nc = W[1][q], Q1 = W[2][q], Q2 = W[3][q], Q3 = W[4][q];
+            double dKr = 0.0, dKi = 0.0, dQ1 = 0.0, dQ2 = 0.0, dQ3 = 0.0;
+            if constexpr(TAN) {
+                dKr = W[5][q];
+                dQ1 = W[7][q];
+                dQ2 = W[8][q];
+                dQ3 = W[9][q];
+            }
             // coefficient of the column element: C (non-resonant) or K = Rc / Nc / E (resonant)
             double Kr = Cc, Ki = 0.0;
+            if constexpr(KIND == 0 && SRC >= 2) {
+                bool const nrs = never_resonant<SRC>(Q1, Q2, Q3, res2);
+                Kr = nrs ? Nc : 0.0;
+                if constexpr(TAN)
+                    dKr = nrs ? W[6][q] : 0.0;
+            }
             if constexpr(KIND >= 2) {
                 double const Er = -(KIND == 2 ? Q2 + Q3 : Q1 + Q2); // E = Er + i Yb
                 double const e2 = fma(Er, Er, Yb2);
@@ -1356,22 +1614,44 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g
                     double const ie = Nc * rcp_nr(e2);
                     Kr = Er * ie;
                     Ki = -Yb * ie;
+                    if constexpr(TAN) { // dK = (dNc + K dPe) / E
+                        double const rr = rcp_nr(e2), dPe = KIND == 2 ? dQ2 + dQ3 : dQ1 + dQ2;
+                        double const nr = fma(Kr, dPe, W[6][q]), ni = Ki * dPe;
+                        dKr = fma(nr, Er, ni * Yb) * rr;
+                        dKi = fma(ni, Er, -nr * Yb) * rr;
+                    }
                 }
             }
             double const Ra = sep_pole<PL.rp[0]>(Q1, Q2, Q3), Rb = sep_pole<PL.rp[1]>(Q1, Q2, Q3);
             double const Ka = sep_pole<PL.cp[0]>(Q1, Q2, Q3), Kb = sep_pole<PL.cp[1]>(Q1, Q2, Q3);
             double const RaRb = Ra * Rb, KaKb = Ka * Kb;
             double ar[2], ai[2], br[2], bi[2];
+            double dar[2], dai[2], dbr[2], dbi[2]; // derivatives (TAN)
     #pragma unroll
             for(int m = 0; m < 2; ++m) {
-                sep_element<PL.nrow>(Ra, Rb, ya[m], yb[m], ya2[m], yb2[m], yy[m], RaRb, rs[m], ar[m], ai[m]);
-                if constexpr(KIND >= 2) {
-                    double er, ei;
-                    sep_element<1>(Ka, Kb, xa[m], xb[m], xa2[m], xb2[m], xx[m], KaKb, cs[m], er, ei);
-                    br[m] = fma(Kr, er, -Ki * ei);
-                    bi[m] = fma(Kr, ei, Ki * er);
+                if constexpr(TAN) {
+                    double const dRa = sep_pole<PL.rp[0]>(dQ1, dQ2, dQ3), dRb = sep_pole<PL.rp[1]>(dQ1, dQ2, dQ3);
+                    double const dKa = sep_pole<PL.cp[0]>(dQ1, dQ2, dQ3), dKb = sep_pole<PL.cp[1]>(dQ1, dQ2, dQ3);
+                    sep_element_tan<PL.nrow>(Ra, Rb, dRa, dRb, ya[m], yb[m], ya2[m], yb2[m], yy[m], RaRb, rs[m], ar[m],
+                                             ai[m], dar[m], dai[m]);
+                    double pr, pi, dpr, dpi; // column element without its coefficient, and its derivative
+                    sep_element_tan<(KIND >= 2 ? 1 : PL.ncol)>(Ka, Kb, dKa, dKb, xa[m], xb[m], xa2[m], xb2[m], xx[m],
+                                                               KaKb, cs[m], pr, pi, dpr, dpi);
+                    br[m] = fma(Kr, pr, -Ki * pi);
+                    bi[m] = fma(Kr, pi, Ki * pr);
+                    dbr[m] = fma(dKr, pr, -dKi * pi) + fma(Kr, dpr, -Ki * dpi);
+                    dbi[m] = fma(dKr, pi, dKi * pr) + fma(Kr, dpi, Ki * dpr);
                 } else {
-                    sep_element<PL.ncol>(Ka, Kb, xa[m], xb[m], xa2[m], xb2[m], xx[m], KaKb, cs[m] * Kr, br[m], bi[m]);
+                    sep_element<PL.nrow>(Ra, Rb, ya[m], yb[m], ya2[m], yb2[m], yy[m], RaRb, rs[m], ar[m], ai[m]);
+                    if constexpr(KIND >= 2) {
+                        double er, ei;
+                        sep_element<1>(Ka, Kb, xa[m], xb[m], xa2[m], xb2[m], xx[m], KaKb, cs[m], er, ei);
+                        br[m] = fma(Kr, er, -Ki * ei);
+                        bi[m] = fma(Kr, ei, Ki * er);
+                    } else {
+                        sep_element<PL.ncol>(Ka, Kb, xa[m], xb[m], xa2[m], xb2[m], xx[m], KaKb, cs[m] * Kr, br[m],
+                                             bi[m]);
+                    }
                 }
             }
             double a1[2], b2[2], b3[2];
@@ -1381,14 +1661,35 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g
                 b2[m] = bi[m] - br[m];
                 b3[m] = br[m] + bi[m];
             }
+            if constexpr(TAN) { // Row' Col + Row Col'
+                double da1[2], db2[2], db3[2];
     #pragma unroll
-            for(int m = 0; m < 2; ++m)
-    #pragma unroll
-                for(int n = 0; n < 2; ++n) {
-                    acc[m][n][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[m], br[n], acc[m][n][0], 0, 0, 0);
-                    acc[m][n][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[m], b3[n], acc[m][n][1], 0, 0, 0);
-                    acc[m][n][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[m], b2[n], acc[m][n][2], 0, 0, 0);
+                for(int m = 0; m < 2; ++m) {
+                    da1[m] = dar[m] + dai[m];
+                    db2[m] = dbi[m] - dbr[m];
+                    db3[m] = dbr[m] + dbi[m];
                 }
+    #pragma unroll
+                for(int m = 0; m < 2; ++m)
+    #pragma unroll
+                    for(int n = 0; n < 2; ++n) {
+                        acc[m][n][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(da1[m], br[n], acc[m][n][0], 0, 0, 0);
+                        acc[m][n][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(dai[m], b3[n], acc[m][n][1], 0, 0, 0);
+                        acc[m][n][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(dar[m], b2[n], acc[m][n][2], 0, 0, 0);
+                        acc[m][n][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[m], dbr[n], acc[m][n][0], 0, 0, 0);
+                        acc[m][n][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[m], db3[n], acc[m][n][1], 0, 0, 0);
+                        acc[m][n][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[m], db2[n], acc[m][n][2], 0, 0, 0);
+                    }
+            } else {
+    #pragma unroll
+                for(int m = 0; m < 2; ++m)
+    #pragma unroll
+                    for(int n = 0; n < 2; ++n) {
+                        acc[m][n][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[m], br[n], acc[m][n][0], 0, 0, 0);
+                        acc[m][n][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[m], b3[n], acc[m][n][1], 0, 0, 0);
+                        acc[m][n][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[m], b2[n], acc[m][n][2], 0, 0, 0);
+                    }
+            }
         }
     }
     // D[i][j] of a 16 x 16 tile sits in lane j + 16 (i % 4), register i / 4
@@ -1436,7 +1737,13 @@ __global__ void g2_mfma_reduce_kernel(int nslices, int NB, int NR, int NC, const
 // comes from the table yE (checked on the host against every (b, r, c) of the grid).
 constexpr int kDQ = 48; // diagonal tile columns
 
-template <int KIND, int PERM>
+// SK = 0: the list holds resonant terms of kind KIND under permutation PERM.  SK = 2 or 3: it holds
+// terms of kind SK under permutation SP, read as the diagonal half of their partial-fraction split
+// (split_applies): Row = -Nc / ((F1 - Pa)(F3 - Pb)), E never resonant, fields (Q1, Q2, Q3)
+// arranged so that this kernel's Pa, Pb and E are the split's; terms that may resonate get
+// coefficient 0.  TAN: a tangent list, and the tile is sum_t Row'_t H_t + Row_t H'_t (the
+// derivative), with H' = dRc (|E| <= res) or (dNc + H dPe) / E.
+template <int KIND, int PERM, int SK = 0, int SP = 0, bool TAN = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g2_eval_diag_kernel(
     int64_t nterms, int64_t slice_len, const double* __restrict__ T, int64_t cap, int NR, int nrt,
     const double* __restrict__ yF1, const double* __restrict__ yF3, const double* __restrict__ yE, int nE, double res2,
@@ -1469,44 +1776,101 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g
 #pragma unroll
         for(int x = 0; x < 3; ++x)
             acc[n][x] = dbl4{0.0, 0.0, 0.0, 0.0};
-    // resonant SoA (Rc, Nc, P1, P2, P3); term k of each step of 4, read from the staged chunk
-    __shared__ double sT[4][5][kStage];
+    // resonant SoA (Rc, Nc, P1, P2, P3; tangent lists also dRc, dNc, dP1, dP2, dP3); term k of each
+    // step of 4, read from the staged chunk
+    constexpr int NS = TAN ? 10 : 5;
+    __shared__ double sT[4][NS][kStage];
     double(*W)[kStage] = sT[wave];
-    double nf[5];
+    double nf[NS];
     if(t0 < t1)
-        stage_load<true>(T, cap, t0 + lane, t0, t1, nf);
+        stage_load<true, TAN>(T, cap, t0 + lane, t0, t1, nf);
     for(int64_t c0 = t0; c0 < t1; c0 += kStage) {
-        stage_store(W, lane, nf);
+        stage_store<NS>(W, lane, nf);
         if(c0 + kStage < t1)
-            stage_load<true>(T, cap, c0 + kStage + lane, t0, t1, nf);
+            stage_load<true, TAN>(T, cap, c0 + kStage + lane, t0, t1, nf);
         int const nsteps = (int(min<int64_t>(kStage, t1 - c0)) + 3) / 4;
         for(int j = 0; j < nsteps; ++j) {
             int const q = 4 * j + k;
-            double const Rc = W[0][q], Nc = W[1][q], Q1 = W[2][q], Q2 = W[3][q], Q3 = W[4][q];
+            double Rc = W[0][q], Nc = W[1][q], Q1 = W[2][q], Q2 = W[3][q], Q3 = W[4][q];
+            double dRc = 0.0, dNc = 0.0, dQ1 = 0.0, dQ2 = 0.0, dQ3 = 0.0;
+            if constexpr(TAN) {
+                dRc = W[5][q];
+                dNc = W[6][q];
+                dQ1 = W[7][q];
+                dQ2 = W[8][q];
+                dQ3 = W[9][q];
+            }
+            if constexpr(SK != 0) {
+                constexpr SplitPlan SPL = split_plan_of(SK, SP);
+                static_assert(SPL.dkind == KIND && SPL.dperm == PERM, "split routed to the wrong diagonal kernel");
+                double const P[3] = {Q1, Q2, Q3};
+                bool const nrs = never_resonant<SK>(Q1, Q2, Q3, res2);
+                Nc = nrs ? -Nc : 0.0;
+                Rc = 0.0;
+                Q1 = P[SPL.q[0]];
+                Q2 = P[SPL.q[1]];
+                Q3 = P[SPL.q[2]];
+                if constexpr(TAN) {
+                    double const dP[3] = {dQ1, dQ2, dQ3};
+                    dNc = nrs ? -dNc : 0.0;
+                    dRc = 0.0;
+                    dQ1 = dP[SPL.q[0]];
+                    dQ2 = dP[SPL.q[1]];
+                    dQ3 = dP[SPL.q[2]];
+                }
+            }
             double const Pa = F1_IN_D1 ? Q1 : Q3, Pb = F1_IN_D1 ? Q3 : Q1;
-            double ar, ai;
-            sep_element<2>(Pa, Pb, ya, yb, ya2, yb2, yy, Pa * Pb, rsc, ar, ai);
+            double ar, ai, dar = 0.0, dai = 0.0;
+            if constexpr(TAN)
+                sep_element_tan<2>(Pa, Pb, F1_IN_D1 ? dQ1 : dQ3, F1_IN_D1 ? dQ3 : dQ1, ya, yb, ya2, yb2, yy, Pa * Pb,
+                                   rsc, ar, ai, dar, dai);
+            else
+                sep_element<2>(Pa, Pb, ya, yb, ya2, yb2, yy, Pa * Pb, rsc, ar, ai);
             double const Er = -(KIND == 2 ? Q2 + Q3 : Q1 + Q2);
-            double br[3], bi[3];
+            double const dPe = KIND == 2 ? dQ2 + dQ3 : dQ1 + dQ2;
+            double br[3], bi[3], dbr[3], dbi[3];
     #pragma unroll
             for(int n = 0; n < 3; ++n) {
                 double const e2 = fma(Er, Er, ye[n] * ye[n]);
                 if(e2 <= res2) {
                     br[n] = Rc;
                     bi[n] = 0.0;
+                    dbr[n] = dRc;
+                    dbi[n] = 0.0;
                 } else {
-                    double const ie = Nc * rcp_nr(e2);
+                    double const rr = rcp_nr(e2);
+                    double const ie = Nc * rr;
                     br[n] = Er * ie;
                     bi[n] = -ye[n] * ie;
+                    if constexpr(TAN) { // H' = (dNc + H dPe) / E
+                        double const nr = fma(br[n], dPe, dNc), ni = bi[n] * dPe;
+                        dbr[n] = fma(nr, Er, ni * ye[n]) * rr;
+                        dbi[n] = fma(ni, Er, -nr * ye[n]) * rr;
+                    }
                 }
             }
             double const a1 = ar + ai;
+            if constexpr(TAN) { // Row' H + Row H'
+                double const da1 = dar + dai;
     #pragma unroll
-            for(int n = 0; n < 3; ++n) {
-                double const b2 = bi[n] - br[n], b3 = br[n] + bi[n];
-                acc[n][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, br[n], acc[n][0], 0, 0, 0);
-                acc[n][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, b3, acc[n][1], 0, 0, 0);
-                acc[n][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b2, acc[n][2], 0, 0, 0);
+                for(int n = 0; n < 3; ++n) {
+                    double const b2 = bi[n] - br[n], b3 = br[n] + bi[n];
+                    double const db2 = dbi[n] - dbr[n], db3 = dbr[n] + dbi[n];
+                    acc[n][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(da1, br[n], acc[n][0], 0, 0, 0);
+                    acc[n][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(dai, b3, acc[n][1], 0, 0, 0);
+                    acc[n][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(dar, b2, acc[n][2], 0, 0, 0);
+                    acc[n][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, dbr[n], acc[n][0], 0, 0, 0);
+                    acc[n][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, db3, acc[n][1], 0, 0, 0);
+                    acc[n][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, db2, acc[n][2], 0, 0, 0);
+                }
+            } else {
+    #pragma unroll
+                for(int n = 0; n < 3; ++n) {
+                    double const b2 = bi[n] - br[n], b3 = br[n] + bi[n];
+                    acc[n][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, br[n], acc[n][0], 0, 0, 0);
+                    acc[n][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, b3, acc[n][1], 0, 0, 0);
+                    acc[n][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b2, acc[n][2], 0, 0, 0);
+                }
             }
         }
     }
@@ -1724,6 +2088,26 @@ struct TermSpan {
     int64_t cap, begin, count;
 };
 
+// The terms of a split resonant list (kind KIND) that may resonate: flag[t] = 1 (never_resonant
+// false), gathered in list order into out (capacity ocap) by gather_terms_kernel.
+template <int KIND>
+__global__ void maybe_resonant_kernel(int64_t n, const double* __restrict__ T, int64_t cap, double res2,
+                                      int64_t* __restrict__ flag) {
+    int64_t const t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(t < n)
+        flag[t] = never_resonant<KIND>(T[2 * cap + t], T[3 * cap + t], T[4 * cap + t], res2) ? 0 : 1;
+}
+__global__ void gather_terms_kernel(int64_t n, int nf, const double* __restrict__ T, int64_t cap,
+                                    const int64_t* __restrict__ flag, const int64_t* __restrict__ pos,
+                                    double* __restrict__ out, int64_t ocap) {
+    int64_t const t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if(t >= n || !flag[t])
+        return;
+    int64_t const o = pos[t];
+    for(int f = 0; f < nf; ++f)
+        out[f * ocap + o] = T[f * cap + t];
+}
+
 template <bool IMAG>
 void launch_kind(pomx_ctx* ctx, int k, dim3 grid, TermSpan const& s, int64_t slice, int nfreq, const double* d_freq,
                  int perm, double res, double* partial) {
@@ -1856,11 +2240,11 @@ SepGrid detect_sep_grid(std::vector<double> const& h, int nfreq) {
 }
 
 // One non-resonant kind on a separable grid: matrix-core partial tiles, then their sum into acc.
-template <int KIND, int PERM>
+template <int KIND, int PERM, int SRC, bool TAN>
 void launch_sep_k(pomx_ctx* ctx, dim3 grid, TermSpan const& s, int64_t slice, const double* T, SepGrid const& g,
                   const double* yF1, const double* yF2, const double* yF3, const double* yB, double res,
                   double* partial) {
-    g2_eval_mfma_kernel<KIND, PERM><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, g.NR, g.NC, yF1, yF2, yF3,
+    g2_eval_mfma_kernel<KIND, PERM, SRC, TAN><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, g.NR, g.NC, yF1, yF2, yF3,
                                                                    yB, res * res, partial);
 }
 
@@ -1877,8 +2261,12 @@ bool sep_applies(int kind, int perm, bool sum_ok) {
     return kind == 2 ? p3[perm][0] == 1 : p3[perm][2] == 1;
 }
 
+// The separable matrix-core evaluation of span s (kind, perm) into d_acc; src >= 0: a resonant list
+// of kind src read as kind-0 terms (the separable half of the split).  TAN: a tangent list, whose
+// derivative goes to d_acc.
+template <bool TAN>
 int launch_sep(pomx_ctx* ctx, TermSpan const& s, int kind, int perm, SepGrid const& g, const double* d_y, double res,
-               double* d_acc) {
+               double* d_acc, int src = -1) {
     int64_t const want = std::max<int64_t>(4, 4096 / g.NB);
     int64_t slice = std::max<int64_t>(4, (s.count + want - 1) / want);
     slice = (slice + 3) / 4 * 4;
@@ -1894,8 +2282,11 @@ int launch_sep(pomx_ctx* ctx, TermSpan const& s, int kind, int perm, SepGrid con
     const double* T = s.base + s.begin;
     dim3 const grid(unsigned(nsl / 4), unsigned(g.NB));
 #define POMX_SEP_CASE(K, P)                                                                        \
-    case K * 6 + P: launch_sep_k<K, P>(ctx, grid, s, slice, T, g, yF1, yF2, yF3, yB, res, partial); break
-    switch(kind * 6 + perm) {
+    case K * 6 + P: launch_sep_k<K, P, K, TAN>(ctx, grid, s, slice, T, g, yF1, yF2, yF3, yB, res, partial); break
+#define POMX_SEP_SPLIT(S, P)                                                                       \
+    case 100 + S * 6 + P: launch_sep_k<0, P, S, TAN>(ctx, grid, s, slice, T, g, yF1, yF2, yF3, yB, res, partial); break
+    switch(src >= 0 ? 100 + src * 6 + perm : kind * 6 + perm) {
+        POMX_SEP_SPLIT(2, 1); POMX_SEP_SPLIT(2, 4); POMX_SEP_SPLIT(3, 2); POMX_SEP_SPLIT(3, 3);
         POMX_SEP_CASE(0, 0); POMX_SEP_CASE(0, 1); POMX_SEP_CASE(0, 2); POMX_SEP_CASE(0, 3); POMX_SEP_CASE(0, 4);
         POMX_SEP_CASE(0, 5); POMX_SEP_CASE(1, 0); POMX_SEP_CASE(1, 1); POMX_SEP_CASE(1, 2); POMX_SEP_CASE(1, 3);
         POMX_SEP_CASE(1, 4); POMX_SEP_CASE(1, 5); POMX_SEP_CASE(2, 2); POMX_SEP_CASE(2, 3); POMX_SEP_CASE(3, 1);
@@ -1903,6 +2294,7 @@ int launch_sep(pomx_ctx* ctx, TermSpan const& s, int kind, int perm, SepGrid con
     default: return pomx_fail(ctx, POMX_ERR_ARG, "pomx_g2_part_compute: no separable plan for this kind");
     }
 #undef POMX_SEP_CASE
+#undef POMX_SEP_SPLIT
     POMX_CHECK_LAUNCH(ctx);
     int const nout = g.NB * g.NR * g.NC;
     g2_mfma_reduce_kernel<<<grid_for(nout), kBlock, 0, ctx->stream>>>(int(nsl), g.NB, g.NR, g.NC, partial, d_acc);
@@ -1918,8 +2310,56 @@ bool diag_applies(int kind, int perm, SepGrid const& g) {
     return toep ? g.toep_ok : g.hank_ok;
 }
 
+// resonant kinds and permutations whose three-factor terms split into a separable and a diagonal
+// part (split_plan_of) on this grid
+bool split_applies(int kind, int perm, SepGrid const& g) {
+    if(!g.ok || !split_kind_perm(kind, perm))
+        return false;
+    SplitPlan const pl = split_plan_of(kind, perm);
+    bool const toep = (pl.dkind == 2) == (pl.dperm == 0);
+    return toep ? g.toep_ok : g.hank_ok;
+}
+
+// The terms of span s (resonant, kind 2 or 3; nf fields per term) that may resonate, copied into
+// store: rem (count 0 when there are none).
+int split_remainder(pomx_ctx* ctx, TermSpan const& s, int kind, double res, int nf, WsBuf<double>& store,
+                    TermSpan& rem) {
+    int64_t const n = s.count;
+    rem = TermSpan{nullptr, 0, 0, 0};
+    WsBuf<int64_t> flag(ctx, n), pos(ctx, n);
+    if(!flag.p || !pos.p)
+        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+    const double* T = s.base + s.begin;
+    if(kind == 2)
+        maybe_resonant_kernel<2><<<grid_for(n), kBlock, 0, ctx->stream>>>(n, T, s.cap, res * res, flag);
+    else
+        maybe_resonant_kernel<3><<<grid_for(n), kBlock, 0, ctx->stream>>>(n, T, s.cap, res * res, flag);
+    POMX_CHECK_LAUNCH(ctx);
+    int rc = pomx_exclusive_scan(ctx, n, flag, pos);
+    if(rc)
+        return rc;
+    int64_t last_pos = 0, last_flag = 0;
+    if((rc = read_i64(ctx, pos.p + n - 1, &last_pos)) || (rc = read_i64(ctx, flag.p + n - 1, &last_flag)))
+        return rc;
+    int64_t const m = last_pos + last_flag;
+    if(m == 0)
+        return POMX_OK;
+    store = WsBuf<double>(ctx, size_t(nf) * m);
+    if(!store.p)
+        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+    gather_terms_kernel<<<grid_for(n), kBlock, 0, ctx->stream>>>(n, nf, T, s.cap, flag, pos, store, m);
+    POMX_CHECK_LAUNCH(ctx);
+    POMX_HIP(ctx, hipStreamSynchronize(ctx->stream)); // flag and pos outlive the gather
+    rem = TermSpan{store.p, m, 0, m};
+    return POMX_OK;
+}
+
+// The diagonal matrix-core evaluation of span s (kind, perm) into d_acc; src != 0: terms of kind src
+// under src_perm, the diagonal half of their split ((kind, perm) being split_plan_of(src,
+// src_perm)'s diagonal kernel).  TAN: a tangent list, whose derivative goes to d_acc.
+template <bool TAN>
 int launch_diag(pomx_ctx* ctx, TermSpan const& s, int kind, int perm, SepGrid const& g, const double* d_y,
-                double res, double* d_acc) {
+                double res, double* d_acc, int src = 0, int src_perm = 0) {
     bool const toep = (kind == 2) == (perm == 0);
     int const nrt = (g.NR + 15) / 16;
     int const ny = g.NB * nrt;
@@ -1941,13 +2381,21 @@ int launch_diag(pomx_ctx* ctx, TermSpan const& s, int kind, int perm, SepGrid co
     const double* T = s.base + s.begin;
     dim3 const grid(unsigned(nsl / 4), unsigned(ny));
     double const res2 = res * res;
-    switch(kind * 6 + perm) {
-    case 2 * 6 + 0: g2_eval_diag_kernel<2, 0><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, g.NR, nrt, yF1, yF3, yE, nE, res2, partial); break;
-    case 2 * 6 + 5: g2_eval_diag_kernel<2, 5><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, g.NR, nrt, yF1, yF3, yE, nE, res2, partial); break;
-    case 3 * 6 + 0: g2_eval_diag_kernel<3, 0><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, g.NR, nrt, yF1, yF3, yE, nE, res2, partial); break;
-    case 3 * 6 + 5: g2_eval_diag_kernel<3, 5><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, g.NR, nrt, yF1, yF3, yE, nE, res2, partial); break;
+#define POMX_DIAG(K, P, S, SP)                                                                                         \
+    g2_eval_diag_kernel<K, P, S, SP, TAN><<<grid, 256, 0, ctx->stream>>>(s.count, slice, T, s.cap, g.NR, nrt, yF1, yF3, \
+                                                                         yE, nE, res2, partial)
+    switch(src ? 1000 + src * 6 + src_perm : kind * 6 + perm) {
+    case 1000 + 2 * 6 + 1: POMX_DIAG(2, 0, 2, 1); break;
+    case 1000 + 2 * 6 + 4: POMX_DIAG(3, 0, 2, 4); break;
+    case 1000 + 3 * 6 + 2: POMX_DIAG(3, 0, 3, 2); break;
+    case 1000 + 3 * 6 + 3: POMX_DIAG(2, 0, 3, 3); break;
+    case 2 * 6 + 0: POMX_DIAG(2, 0, 0, 0); break;
+    case 2 * 6 + 5: POMX_DIAG(2, 5, 0, 0); break;
+    case 3 * 6 + 0: POMX_DIAG(3, 0, 0, 0); break;
+    case 3 * 6 + 5: POMX_DIAG(3, 5, 0, 0); break;
     default: return pomx_fail(ctx, POMX_ERR_ARG, "pomx_g2_part_compute: no diagonal plan for this kind");
     }
+#undef POMX_DIAG
     POMX_CHECK_LAUNCH(ctx);
     int const nout = g.NB * g.NR * g.NC;
     g2_diag_reduce_kernel<<<grid_for(nout), kBlock, 0, ctx->stream>>>(int(nsl), g.NB, g.NR, g.NC, nrt, toep ? 1 : 0,
@@ -1993,8 +2441,16 @@ double general_flop_per_term_freq(int kind, bool imag) {
     return kind == 0 ? 28.0 : kind == 1 ? 31.0 : 32.0;
 }
 
+int launch_g2_tangent_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res, int nfreq,
+                           const double* d_freq, double* d_acc, double* d_dacc);
+
+// The Lehmann sum of a part's kind lists at the triplets d_freq into d_acc.  With d_dacc the lists
+// are tangent lists (value fields, then the derivatives) and the derivative goes to d_dacc: the
+// matrix-core classes run once for the value and once in their tangent form, and the kinds left
+// to the vector kernels take the tangent vector kernel (value and derivative together).
 int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res, int nfreq, const double* d_freq,
-                   double* d_acc) {
+                   double* d_acc, double* d_dacc) {
+    bool const tan = d_dacc != nullptr;
     int64_t total = 0;
     for(int k = 0; k < 4; ++k)
         total += spans[k].count;
@@ -2054,7 +2510,7 @@ int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res,
     bool any_vector = false; // some kind takes the vector kernels, which need zeroed partial sums
     for(int k = 0; k < 4; ++k)
         if(spans[k].count && !(sep.ok && sep_applies(k, perm, sep.sum_ok)) && !diag_applies(k, perm, sep))
-            any_vector = true;
+            any_vector = !tan; // (split kinds: the terms that may resonate)
     WsBuf<double> partial;
     if(any_vector) {
         partial = WsBuf<double>(ctx, size_t(nslices) * nfreq * 2);
@@ -2063,41 +2519,90 @@ int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res,
         if((rc = pomx_fill_zero(ctx, partial.p, sizeof(double) * size_t(nslices) * nfreq * 2)))
             return rc;
     }
+    // tangent: the spans left to the tangent vector kernel (a kind's whole span, or the terms of a
+    // split kind that may resonate, held in vstore)
+    TermSpan vspans[4] = {};
+    WsBuf<double> vstore[4];
+    // the vector kernels (grouped, strided or general) on span v of kind k
+    auto vector_eval = [&](TermSpan const& v, int k) {
+        if(tan) {
+            vspans[k] = v;
+            return;
+        }
+        double const cnt = double(v.count), tbytes = 8.0 * (k < 2 ? 4 : 5);
+        // a span smaller than the part (the terms of a split kind that may resonate) is spread over
+        // up to nslices slices of at least kChunk terms
+        int64_t const vsl = std::min<int64_t>(nslices, (v.count + kChunk - 1) / kChunk);
+        int64_t vslice = (v.count + vsl - 1) / vsl;
+        vslice = (vslice + kChunk - 1) / kChunk * kChunk;
+        dim3 grid(ftiles, unsigned((v.count + vslice - 1) / vslice));
+        bool const g = grouped && g2_structured(k, slot2), st = !g && strided && g2_structured(k, slot1);
+        double const fpt = (g || st) ? grouped_flop_per_term_freq(k, g ? slot2 : slot1)
+                                     : general_flop_per_term_freq(k, !has_real);
+        int const h = stat_begin(ctx, (g || st) ? POMX_STAT_G2_GROUPED : POMX_STAT_G2_GENERAL);
+        if(g)
+            launch_kind_s<false>(ctx, k, slot2, grid, v, vslice, nfreq, stride, d_freq, perm, res, partial);
+        else if(st)
+            launch_kind_s<true>(ctx, k, slot1, grid, v, vslice, nfreq, stride, d_freq, perm, res, partial);
+        else if(has_real)
+            launch_kind<false>(ctx, k, grid, v, vslice, nfreq, d_freq, perm, res, partial);
+        else
+            launch_kind<true>(ctx, k, grid, v, vslice, nfreq, d_freq, perm, res, partial);
+        stat_end(ctx, h, cnt * nfreq * fpt, cnt * nfreq, cnt * tbytes);
+    };
     for(int k = 0; k < 4; ++k) {
         TermSpan const& s = spans[k];
         if(!s.count)
             continue;
         double const cnt = double(s.count), tbytes = 8.0 * (k < 2 ? 4 : 5); // a term's fields, read once
-        dim3 grid(ftiles, unsigned((s.count + slice - 1) / slice));
         if(sep.ok && sep_applies(k, perm, sep.sum_ok)) {
             double const nout = double(sep.NB) * sep.NR * sep.NC;
             int const h = stat_begin(ctx, POMX_STAT_G2_SEP);
-            if((rc = launch_sep(ctx, s, k, perm, sep, d_y, res, d_acc)))
+            if((rc = launch_sep<false>(ctx, s, k, perm, sep, d_y, res, d_acc)) ||
+               (tan && (rc = launch_sep<true>(ctx, s, k, perm, sep, d_y, res, d_dacc))))
                 return rc;
             stat_end(ctx, h, cnt * sep.NB * sep_flop_per_term_b(k, perm, sep.NR, sep.NC), cnt * nout,
                      cnt * tbytes + 32.0 * nout);
         } else if(diag_applies(k, perm, sep)) {
             double const nout = double(sep.NB) * sep.NR * sep.NC;
             int const h = stat_begin(ctx, POMX_STAT_G2_DIAG);
-            if((rc = launch_diag(ctx, s, k, perm, sep, d_y, res, d_acc)))
+            if((rc = launch_diag<false>(ctx, s, k, perm, sep, d_y, res, d_acc)) ||
+               (tan && (rc = launch_diag<true>(ctx, s, k, perm, sep, d_y, res, d_dacc))))
                 return rc;
             stat_end(ctx, h, cnt * sep.NB * ((sep.NR + 15) / 16) * diag_flop_per_term_tile(sep.NC), cnt * nout,
                      cnt * tbytes + 32.0 * nout);
-        } else {
-            bool const g = grouped && g2_structured(k, slot2), st = !g && strided && g2_structured(k, slot1);
-            double const fpt = (g || st) ? grouped_flop_per_term_freq(k, g ? slot2 : slot1)
-                                         : general_flop_per_term_freq(k, !has_real);
-            int const h = stat_begin(ctx, (g || st) ? POMX_STAT_G2_GROUPED : POMX_STAT_G2_GENERAL);
-            if(g)
-                launch_kind_s<false>(ctx, k, slot2, grid, s, slice, nfreq, stride, d_freq, perm, res, partial);
-            else if(st)
-                launch_kind_s<true>(ctx, k, slot1, grid, s, slice, nfreq, stride, d_freq, perm, res, partial);
-            else if(has_real)
-                launch_kind<false>(ctx, k, grid, s, slice, nfreq, d_freq, perm, res, partial);
-            else
-                launch_kind<true>(ctx, k, grid, s, slice, nfreq, d_freq, perm, res, partial);
-            stat_end(ctx, h, cnt * nfreq * fpt, cnt * nfreq, cnt * tbytes);
-        }
+        } else if(split_applies(k, perm, sep)) {
+            // the separable and the diagonal half of the never-resonant terms, then the vector
+            // kernels on the terms that may resonate
+            double const nout = double(sep.NB) * sep.NR * sep.NC;
+            int h = stat_begin(ctx, POMX_STAT_G2_SEP);
+            if((rc = launch_sep<false>(ctx, s, 0, perm, sep, d_y, res, d_acc, k)) ||
+               (tan && (rc = launch_sep<true>(ctx, s, 0, perm, sep, d_y, res, d_dacc, k))))
+                return rc;
+            stat_end(ctx, h, cnt * sep.NB * sep_flop_per_term_b(0, perm, sep.NR, sep.NC), cnt * nout,
+                     cnt * tbytes + 32.0 * nout);
+            SplitPlan const pl = split_plan_of(k, perm);
+            h = stat_begin(ctx, POMX_STAT_G2_DIAG);
+            if((rc = launch_diag<false>(ctx, s, pl.dkind, pl.dperm, sep, d_y, res, d_acc, k, perm)) ||
+               (tan && (rc = launch_diag<true>(ctx, s, pl.dkind, pl.dperm, sep, d_y, res, d_dacc, k, perm))))
+                return rc;
+            stat_end(ctx, h, cnt * sep.NB * ((sep.NR + 15) / 16) * diag_flop_per_term_tile(sep.NC), cnt * nout,
+                     cnt * tbytes + 32.0 * nout);
+            TermSpan rem;
+            if((rc = split_remainder(ctx, s, k, res, tan ? 10 : 5, vstore[k], rem)))
+                return rc;
+            if(rem.count) {
+                vector_eval(rem, k);
+                if(!tan)
+                    POMX_HIP(ctx, hipStreamSynchronize(ctx->stream)); // the store outlives the kernels
+            }
+        } else
+            vector_eval(s, k);
+    }
+    if(tan) { // value and derivative of the rest by the tangent vector kernel
+        if((rc = launch_g2_tangent_eval(ctx, vspans, perm, res, nfreq, d_freq, d_acc, d_dacc)))
+            return rc;
+        POMX_HIP(ctx, hipStreamSynchronize(ctx->stream)); // vstore outlives the kernels
     }
     if(any_vector) { // the vector kernels' per-slice partial sums into the grid
         int const h = stat_begin(ctx, POMX_STAT_G2_REDUCE);
@@ -2191,38 +2696,6 @@ int rank_table(pomx_ctx* ctx, int nx, int ny, const double* Ex, const double* Ey
 namespace {
 
 // Tangent reduction, compaction and evaluation of one chunk's sorted groups.
-int g2_chunk_tangent(pomx_ctx* ctx, G2Dev const& D, pomx_g2_tangent const& t, int64_t n, int64_t ng,
-                     const int64_t* start, const uint64_t* keys, const uint64_t* vals, double thr, EraseRule const& R,
-                     int perm, double res,
-                     int nfreq, const double* d_freq, double* d_acc, double* d_dacc, int64_t counts[4]) {
-    G2Tan T{t.dO1, t.dO2, t.dO3, t.dCX4, t.dE1, t.dE2, t.dE3, t.dE4, t.dw1, t.dw2, t.dw3, t.dw4};
-    WsBuf<int64_t> live(ctx, ng), lpos(ctx, ng), kc(ctx, 4);
-    WsBuf<double> g(ctx, size_t(ng) * 10);
-    if(!live.p || !lpos.p || !g.p || !kc.p)
-        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_tangent: out of device memory");
-    g2_reduce_tangent_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, T, n, ng, start, keys, vals, thr, R, g, live);
-    int rc = pomx_exclusive_scan(ctx, ng, live, lpos);
-    if(rc)
-        return rc;
-    kind_count_kernel<<<1, 64, 0, ctx->stream>>>(ng, live, lpos, start, keys, kc);
-    int64_t c[4];
-    POMX_HIP(ctx, hipMemcpyAsync(c, kc.p, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
-    POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    int64_t const nnr = std::max<int64_t>(c[0] + c[1], 1), nr = std::max<int64_t>(c[2] + c[3], 1);
-    WsBuf<double> nrbuf(ctx, size_t(8 * nnr)), rbuf(ctx, size_t(10 * nr));
-    if(!nrbuf.p || !rbuf.p)
-        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_tangent: out of device memory");
-    g2_compact_tangent_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(ng, live, lpos, g, start, keys, nnr, nr,
-                                                                         c[0] + c[1], nrbuf, rbuf);
-    POMX_CHECK_LAUNCH(ctx);
-    for(int k = 0; k < 4; ++k)
-        counts[k] += c[k];
-    TermSpan spans[4] = {{nrbuf, nnr, 0, c[0]}, {nrbuf, nnr, c[0], c[1]}, {rbuf, nr, 0, c[2]}, {rbuf, nr, c[2], c[3]}};
-    if((rc = launch_g2_tangent_eval(ctx, spans, perm, res, nfreq, d_freq, d_acc, d_dacc)))
-        return rc;
-    return POMX_OK;
-}
-
 // Chunk budget in raw terms.  A chunk's generation holds keys, values, sort buffers and group
 // arrays (about 110 bytes per term while every group is a singleton): 2^29 terms keep every part
 // of the 4096-state Anderson workload (largest 4.5e8 terms) in one chunk at about 60 GB.
@@ -2257,7 +2730,7 @@ extern "C" int pomx_g2_terms_eval(pomx_ctx* ctx, const pomx_g2_terms* t, int per
     const double* R = t->data + 4 * nnr;
     TermSpan spans[4] = {{NR, nnr, 0, t->count[0]}, {NR, nnr, t->count[0], t->count[1]},
                          {R, nr, 0, t->count[2]},   {R, nr, t->count[2], t->count[3]}};
-    int rc = launch_g2_eval(ctx, spans, perm, res, nfreq, d_freq, d_acc);
+    int rc = launch_g2_eval(ctx, spans, perm, res, nfreq, d_freq, d_acc, nullptr);
     if(rc)
         return rc;
     POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -2287,12 +2760,14 @@ struct G2Gen {
 
 // Generate, sort and group the terms of the pairs [pb, pe) of the r12-sorted pair list.
 int g2_generate_chunk(pomx_ctx* ctx, G2Dev const& D, G2Gen const& T, std::vector<uint64_t> const& hpairs,
-                      int64_t pb, int64_t pe, int64_t n, G2Chunk& out) {
-    // the chunk's pairs in ascending (i1, i2) order, in i1 segments
+                      std::vector<int64_t> const& hpair, int64_t pb, int64_t pe, int64_t n, G2Chunk& out) {
+    // the chunk's pairs with sorted terms in ascending (i1, i2) order, in i1 segments (dropping the
+    // pairs without any keeps the visiting order of the others)
     std::vector<std::pair<uint64_t, uint32_t>> ps;
     ps.reserve(size_t(pe - pb));
     for(int64_t p = pb; p < pe; ++p)
-        ps.emplace_back(hpairs[size_t(p)], uint32_t(p));
+        if(hpair[size_t(p)] > 0)
+            ps.emplace_back(hpairs[size_t(p)], uint32_t(p));
     std::sort(ps.begin(), ps.end());
     std::vector<uint4> rmap(ps.size());
     for(size_t a = 0; a < ps.size();) {
@@ -2357,34 +2832,45 @@ int g2_alloc_groups(pomx_ctx* ctx, G2Chunk& c) {
     return POMX_OK;
 }
 
-// Reduce (unless already reduced) and finish one chunk, then free its terms.  Value path: the live
-// groups are appended to the part's kind lists at fill[k]; tangent path: reduced, compacted and
-// evaluated at once (g2_chunk_tangent).
-int g2_finish_chunk(pomx_ctx* ctx, G2Dev const& D, G2Chunk& c, double thr, EraseRule const& rule,
-                    const pomx_g2_tangent* tan, int perm, double res, int nfreq, const double* d_freq, double* d_acc,
-                    double* d_dacc, KindLists const& out, int64_t fill[4], int64_t counts[4]) {
+// A chunk's groups, reduced (with the threshold thr, or the settled rule), appended to the part's
+// kind lists at fill[k]; with T the tangent reduce, whose groups carry their derivatives, into
+// the tangent kind lists.
+int g2_finish_chunk(pomx_ctx* ctx, G2Dev const& D, G2Chunk& c, double thr, EraseRule const& rule, const G2Tan* T,
+                    KindLists const& out, int64_t fill[4], int64_t counts[4]) {
     int rc;
-    if(tan) {
-        rc = g2_chunk_tangent(ctx, D, *tan, c.n, c.ng, c.start, c.keys, c.vals, thr, rule, perm, res, nfreq, d_freq,
-                              d_acc, d_dacc, counts);
-        c = G2Chunk();
-        return rc;
-    }
     int64_t const ng = c.ng;
-    if(!c.reduced) {
-        if((rc = g2_alloc_groups(ctx, c)))
-            return rc;
-        g2_reduce_kernel<false><<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, c.n, ng, c.start, c.keys, c.vals, thr, rule,
-                                                                            c.g, c.live, nullptr, nullptr, nullptr,
-                                                                            nullptr);
+    WsBuf<double> gt;
+    WsBuf<int64_t> lt;
+    const double* g;
+    const int64_t* live;
+    if(T) {
+        gt = WsBuf<double>(ctx, size_t(ng) * 10);
+        lt = WsBuf<int64_t>(ctx, ng);
+        if(!gt.p || !lt.p)
+            return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_tangent: out of device memory");
+        g2_reduce_tangent_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, *T, c.n, ng, c.start, c.keys, c.vals, thr,
+                                                                           rule, gt, lt);
         POMX_CHECK_LAUNCH(ctx);
+        g = gt.p;
+        live = lt.p;
+    } else {
+        if(!c.reduced) {
+            if((rc = g2_alloc_groups(ctx, c)))
+                return rc;
+            g2_reduce_kernel<false><<<grid_for(ng), kBlock, 0, ctx->stream>>>(D, c.n, ng, c.start, c.keys, c.vals, thr,
+                                                                                rule, c.g, c.live, nullptr, nullptr,
+                                                                                nullptr, nullptr);
+            POMX_CHECK_LAUNCH(ctx);
+        }
+        g = c.g.p;
+        live = c.live.p;
     }
     WsBuf<int64_t> lpos(ctx, ng), kc(ctx, 4);
     if(!lpos.p || !kc.p)
         return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
-    if((rc = pomx_exclusive_scan(ctx, ng, c.live, lpos)))
+    if((rc = pomx_exclusive_scan(ctx, ng, live, lpos)))
         return rc;
-    kind_count_kernel<<<1, 64, 0, ctx->stream>>>(ng, c.live, lpos, c.start, c.keys, kc);
+    kind_count_kernel<<<1, 64, 0, ctx->stream>>>(ng, live, lpos, c.start, c.keys, kc);
     int64_t k[4];
     POMX_HIP(ctx, hipMemcpyAsync(k, kc.p, sizeof(k), hipMemcpyDeviceToHost, ctx->stream));
     POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -2397,7 +2883,11 @@ int g2_finish_chunk(pomx_ctx* ctx, G2Dev const& D, G2Chunk& c, double thr, Erase
         if(fill[q] + k[q] > out.cap[q] || (k[q] > 0 && !out.base[q]))
             return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_g2_part_compute: term list capacity exceeded");
     }
-    g2_compact_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(ng, c.live, lpos, c.g, c.start, c.keys, out, slot);
+    if(T)
+        g2_compact_tangent_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(ng, live, lpos, g, c.start, c.keys, out,
+                                                                            slot);
+    else
+        g2_compact_kernel<<<grid_for(ng), kBlock, 0, ctx->stream>>>(ng, live, lpos, g, c.start, c.keys, out, slot);
     POMX_CHECK_LAUNCH(ctx);
     POMX_HIP(ctx, hipStreamSynchronize(ctx->stream)); // slot and the chunk's buffers outlive the kernel
     c = G2Chunk();
@@ -2524,7 +3014,12 @@ int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* 
         return pomx_fail(ctx, POMX_ERR_RANGE, "pomx_g2_part_compute: block larger than 65535");
     G2Dev D{P.n1, P.n2, P.n3, P.n4, P.O1, P.O2, P.O3, P.CX4, P.E1, P.E2, P.E3, P.E4,
             P.w1, P.w2, P.w3, P.w4, P.sign, beta, tol};
-    bool const direct = tan == nullptr && direct_terms_enabled();
+    bool const direct = direct_terms_enabled();
+    G2Tan TT{};
+    if(tan)
+        TT = G2Tan{tan->dO1, tan->dO2, tan->dO3, tan->dCX4, tan->dE1, tan->dE2, tan->dE3, tan->dE4,
+                   tan->dw1, tan->dw2, tan->dw3, tan->dw4};
+    const G2Tan* Tp = tan ? &TT : nullptr;
     int64_t counts[4] = {0, 0, 0, 0};
     if(keep) {
         keep->data = nullptr;
@@ -2555,6 +3050,17 @@ int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* 
     int64_t const nrows = npairs * P.n3;
     WsBuf<double> O3c(ctx, int64_t(P.n3) * P.n4), C4t(ctx, int64_t(P.n1) * P.n4);
     g2_stage_kernel<<<grid_for(int64_t(P.n3) * P.n4 + int64_t(P.n1) * P.n4), kBlock, 0, ctx->stream>>>(D, O3c, C4t);
+    // bounds that prune the sweep over i4 (row_bound, i4_may_pass)
+    WsBuf<double> wsm(ctx, P.n4), cdm(ctx, int64_t(P.n1) * P.n3);
+    if(!O3c.p || !C4t.p || !wsm.p || !cdm.p)
+        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+    suffix_max_kernel<<<1, 256, 0, ctx->stream>>>(P.n4, P.w4, wsm);
+    cd_max_kernel<<<unsigned(std::min<int64_t>((int64_t(P.n1) * P.n3 + 3) / 4, 1 << 16)), kBlock, 0, ctx->stream>>>(
+        D, O3c, C4t, cdm);
+    POMX_CHECK_LAUNCH(ctx);
+    D.wsm = wsm.p;
+    D.cdm = cdm.p;
+    D.bf = std::max(beta, 1.0) * (1.0 + 1e-9);
     WsBuf<int> rowcnt(ctx, nrows);
     WsBuf<int64_t> paircnt(ctx, npairs), dpc(ctx, 4 * npairs), doff(ctx, 4 * npairs);
     if(!rowcnt.p || !paircnt.p || !dpc.p || !doff.p)
@@ -2611,20 +3117,20 @@ int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* 
             chunks.push_back(Span{b, npairs, acc});
     }
     int64_t const quads_total = direct_total + sorted_total;
-    // 3. the part's kind lists (value path): direct terms first, then the sorted groups
+    // 3. the part's kind lists (tangent path: with the derivative fields): direct terms first, then
+    // the sorted groups
     KindLists lists{};
     std::vector<WsBuf<double>> list_store;
     int64_t fill[4];
     for(int k = 0; k < 4; ++k) {
         lists.cap[k] = std::max<int64_t>(ndirect[k] + sorted_total, 1);
         fill[k] = ndirect[k];
-        if(!tan) { // (the tangent path evaluates chunk by chunk)
-            list_store.emplace_back(ctx, size_t(lists.cap[k]) * (k < 2 ? 4 : 5));
-            if(!list_store.back().p)
-                return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
-            lists.base[k] = list_store.back().p;
-            counts[k] = ndirect[k];
-        }
+        int const nf = tan ? (k < 2 ? 8 : 10) : (k < 2 ? 4 : 5);
+        list_store.emplace_back(ctx, size_t(lists.cap[k]) * nf);
+        if(!list_store.back().p)
+            return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
+        lists.base[k] = list_store.back().p;
+        counts[k] = ndirect[k];
     }
     // visiting positions (list bit, ((i1 n3 + i3) n2 + i2) n4 + i4, slot) must fit below the list gap
     bool const positions_ok = double(P.n1) * P.n2 * P.n3 * P.n4 * 2.0 < 9.2e18;
@@ -2642,7 +3148,7 @@ int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* 
     for(auto const& ch : chunks) {
         G2Chunk c;
         int const hs = stat_begin(ctx, POMX_STAT_G2_SORT);
-        if((rc = g2_generate_chunk(ctx, D, T, hpairs, ch.pb, ch.pe, ch.n, c)))
+        if((rc = g2_generate_chunk(ctx, D, T, hpairs, hpair, ch.pb, ch.pe, ch.n, c)))
             return rc;
         stat_end(ctx, hs, 0.0, double(ch.n), 48.0 * double(ch.n));
         ng_sorted += c.ng;
@@ -2678,8 +3184,7 @@ int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* 
             final_now = c.reduced;
         }
         if(final_now) { // (the tangent path re-reduces with the same threshold as that pass)
-            if((rc = g2_finish_chunk(ctx, D, c, thr, rule, tan, P.perm, res, nfreq, d_freq, d_acc, d_dacc, lists, fill,
-                                     counts)))
+            if((rc = g2_finish_chunk(ctx, D, c, thr, rule, Tp, lists, fill, counts)))
                 return rc;
             continue;
         }
@@ -2707,9 +3212,14 @@ int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* 
     if(!held.empty() && (rc = g2_merge_positions(ctx, D, held, qs, m, hist)))
         return rc;
     if(direct && direct_total > 0) {
-        g2_direct_emit_kernel<<<pair_blocks, kBlock, 0, ctx->stream>>>(
-            D, O3c, C4t, pairs, npairs, u12.p, u34.p, doff, lists, held.empty() ? nullptr : qs.p, m,
-            held.empty() ? nullptr : reinterpret_cast<unsigned long long*>(hist.p));
+        const uint64_t* dq = held.empty() ? nullptr : qs.p;
+        unsigned long long* dh = held.empty() ? nullptr : reinterpret_cast<unsigned long long*>(hist.p);
+        if(tan)
+            g2_direct_emit_kernel<true><<<pair_blocks, kBlock, 0, ctx->stream>>>(D, TT, O3c, C4t, pairs, npairs, u12.p,
+                                                                                u34.p, doff, lists, dq, m, dh);
+        else
+            g2_direct_emit_kernel<false><<<pair_blocks, kBlock, 0, ctx->stream>>>(D, TT, O3c, C4t, pairs, npairs, u12.p,
+                                                                                 u34.p, doff, lists, dq, m, dh);
         POMX_CHECK_LAUNCH(ctx);
     }
     if(!held.empty()) {
@@ -2718,17 +3228,16 @@ int g2_part_run(pomx_ctx* ctx, const pomx_g2_part* part, const pomx_g2_tangent* 
             return rc;
         fk_store.clear();
         for(G2Chunk* c : held)
-            if((rc = g2_finish_chunk(ctx, D, *c, 0.0, rule, tan, P.perm, res, nfreq, d_freq, d_acc, d_dacc, lists, fill,
-                                     counts)))
+            if((rc = g2_finish_chunk(ctx, D, *c, 0.0, rule, Tp, lists, fill, counts)))
                 return rc;
     }
     stat_end(ctx, hg, 0.0, double(quads_total), 36.0 * double(direct_total) + 48.0 * double(sorted_total));
     // 6. the Lehmann sum over the kind lists (value path)
-    if(!tan && nfreq > 0) {
+    if(nfreq > 0) {
         TermSpan spans[4];
         for(int k = 0; k < 4; ++k)
             spans[k] = TermSpan{lists.base[k], lists.cap[k], 0, fill[k]};
-        if((rc = launch_g2_eval(ctx, spans, P.perm, res, nfreq, d_freq, d_acc)))
+        if((rc = launch_g2_eval(ctx, spans, P.perm, res, nfreq, d_freq, d_acc, tan ? d_dacc : nullptr)))
             return rc;
     }
     if(keep) {
