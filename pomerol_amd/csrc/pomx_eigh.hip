@@ -333,8 +333,7 @@ extern "C" int pomx_eigh_batched(pomx_ctx* ctx, int nblocks, const int* h_n, con
         std::vector<int> gn;
         std::vector<int64_t> gm, gv;
         while(g1 < large.size()) {
-            size_t const nb = size_t(h_n[large[g1]]);
-            size_t const need = sizeof(double) * (4 * nb * nb + 128 * nb + 80000);
+            size_t const need = big_block_bytes(h_n[large[g1]]);
             if(!gn.empty() && bytes + need > budget)
                 break;
             bytes += need;

@@ -42,7 +42,41 @@ struct BigBlk {
     int n, gsym;              // block size; symv grid width (partial sums per block)
     int64_t mat_off, val_off; // caller's matrix arena / eigenvalue array
     int64_t dbase, ibase, rbase;
+    int64_t xbase; // two-stage plans: band and bulge-chasing reflector scratch (doubles)
 };
+
+// ---- two-stage reduction parameters ------------------------------------------------------------
+constexpr int kBand = kNB;         // semi-bandwidth of the intermediate band matrix
+constexpr int kLQT = 256;          // panel columns per workgroup of the panel LQ kernel
+constexpr int kGramW = 128;        // columns per workgroup of the 32 x 32 Gram partial sums
+static_assert(kLQT >= 7 * 33 && kLQT % 64 == 0, "the LQ kernel finishes 33 partial sums with 7 threads each");
+constexpr int kQ2G = 32;           // bulge-chasing sweeps per group of the Q2 back-transformation
+constexpr int kQ2Waves = 12;       // waves (groups in flight) per workgroup of the Q2 kernel
+constexpr int kTwoStageMin = 2048; // blocks at least this large take the two-stage reduction
+
+// Two-stage per-block scratch (doubles from xbase).  AB: the band in lower band storage,
+// AB[j * 2 kBand + d] = A[j + d][j] (d < 2 kBand: the band plus room for the bulge).  V2: the
+// bulge-chasing reflectors, row (s * kmax + k) = [tau, v_1 .. v_{kBand-1}] of sweep s, step k
+// (v_0 = 1).  lq: panel LQ partial sums (two parities).  gp: Gram partial sums.
+struct XOff {
+    int64_t AB, V2, lq, gp, end;
+    int kmax, nchunk, ngram;
+};
+__host__ __device__ inline XOff xoff(int n) {
+    XOff o;
+    int64_t at = 0;
+    o.kmax = (n - 2) / kBand + 1;
+    o.nchunk = (n + kLQT - 1) / kLQT;
+    o.ngram = (n + kGramW - 1) / kGramW;
+    o.AB = at, at += int64_t(n) * 2 * kBand;
+    o.V2 = at, at += int64_t(n > 2 ? n - 2 : 1) * o.kmax * kBand;
+    o.lq = at, at += 2 * (int64_t(o.nchunk) * 40 + 40);
+    o.gp = at, at += int64_t(o.ngram) * kBand * kBand;
+    o.end = at;
+    return o;
+}
+// bulge-chasing steps of sweep s of an n-block (sweeps 0 .. n-3)
+__host__ __device__ inline int sweep_steps(int n, int s) { return (n - 2 - s) / kBand + 1; }
 
 // Per-block scratch layout (doubles from dbase).
 struct DOff {
@@ -932,12 +966,12 @@ __global__ void copy_z_kernel(const BigBlk* __restrict__ blks, const double* __r
 // Thread r owns row r of T and builds it in place in the block's scratch (it only reads back
 // its own writes); the column factor is shared through LDS.
 __global__ void __launch_bounds__(kBT) wy_t_kernel(const BigBlk* __restrict__ blks, const int* __restrict__ ids, int j0,
-                                                   double* __restrict__ D) {
+                                                   int shift, double* __restrict__ D) {
     __shared__ double col[kBT];
     BigBlk const B = blks[ids[blockIdx.x]];
     DOff const od = doff(B.n, B.gsym);
     double* base = D + B.dbase;
-    int const nb = min(kBT, B.n - 1 - j0);
+    int const nb = min(kBT, B.n - shift - j0); // reflector j acts on rows j+shift..
     int const r = threadIdx.x;
     double* __restrict__ Tr = base + od.T + int64_t(r) * nb; // row r (valid for r < nb)
     if(r < nb)
@@ -965,6 +999,532 @@ __global__ void __launch_bounds__(kBT) wy_t_kernel(const BigBlk* __restrict__ bl
         }
         if(r == i)
             Tr[i] = ti;
+    }
+}
+
+// ---- two-stage reduction: dense -> band (kBand) -> tridiagonal ---------------------------------
+//
+// Stage 1 brings A to a band of semi-bandwidth kBand panel by panel.  The rows k..k+kBand-1 of the
+// upper triangle right of the band, P = A[k:k+kBand, k+kBand:n], are factored P = [L 0] Q^T by
+// Householder reflectors (an LQ factorisation, one launch per reflector), Q = I - V T V^T, and the
+// trailing matrix becomes Q^T A22 Q = A22 - V Z^T - Z V^T with W = A22 V T and
+// Z = W - V (T^T V^T W) / 2: products of kBand-wide panels on the matrix cores instead of one
+// matrix-vector product per reflector.  A22 is kept in full storage (both triangles).
+// Stage 2 chases the band down to tridiagonal form: sweep s annihilates column s below the
+// subdiagonal with a reflector of length kBand and chases the bulge this creates down the band,
+// one step per kBand rows.  A persistent kernel runs many sweeps at once.
+// Eigenvectors: X = Q1 Q2 Z; Q2 (the bulge-chasing reflectors) by groups of sweeps on the vector
+// units with the eigenvector columns across the lanes, Q1 by the blocked WY back-transformation
+// of the one-stage path with the reflectors shifted by kBand.
+
+// Lower triangle of each block overwritten with its upper triangle (tiles of 64 through LDS).
+__global__ void __launch_bounds__(256) sym_lower_kernel(const BigBlk* __restrict__ blks, double* __restrict__ Aar) {
+    __shared__ double t[64][65];
+    BigBlk const B = blks[blockIdx.y];
+    int const n = B.n, nt = (n + 63) / 64;
+    double* __restrict__ a = Aar + B.mat_off;
+    for(int q = blockIdx.x; q < nt * nt; q += gridDim.x) {
+        int const ti = q / nt, tj = q % nt; // destination tile, on or below the diagonal
+        if(ti < tj)
+            continue; // uniform over the workgroup
+        __syncthreads();
+        for(int e = threadIdx.x; e < 64 * 64; e += 256) {
+            int const r = tj * 64 + e / 64, c = ti * 64 + e % 64; // source: upper tile (tj, ti)
+            t[e / 64][e % 64] = (r < n && c < n) ? a[int64_t(r) * n + c] : 0.0;
+        }
+        __syncthreads();
+        for(int e = threadIdx.x; e < 64 * 64; e += 256) {
+            int const rr = e / 64, cc = e % 64;
+            int const r = ti * 64 + rr, c = tj * 64 + cc;
+            if(r < n && c < n && r > c)
+                a[int64_t(r) * n + c] = t[cc][rr];
+        }
+    }
+}
+
+// Stage 1, panel k (P[r][c] = A[k + r][k + kBand + c], c < m = n - k - kBand), launch i
+// (0 <= i <= nref, nref = min(kBand, m - 1): the reflectors with at least two entries).  Launch i
+// finishes reflector q = i-1 from the partial sums of launch i-1 (sigma = squared tail of row q,
+// d_r = tail of row r . tail of row q, and the column-q values P[r][q] from the workgroup holding
+// column q), applies it to the rows q..kBand-1 of this workgroup's columns (one column per
+// thread), stores v_q (reflector-major in Vt, column q of V) and tau; then, from the updated rows,
+// it writes the partial sums of reflector i.  The panel ends as [L 0] (L lower triangular).
+__global__ void __launch_bounds__(kLQT) lq_kernel(const BigBlk* __restrict__ blks, int k, int i, double* __restrict__ Aar,
+                                                  double* __restrict__ D) {
+    __shared__ double red[4][33];
+    __shared__ double fin[33], fin7[7][33];
+    __shared__ double colv[kBand];
+    BigBlk const B = blks[blockIdx.y];
+    int const n = B.n, m = n - k - kBand;
+    if(m < 2)
+        return;
+    int const nref = min(kBand, m - 1);
+    int const nch = (m + kLQT - 1) / kLQT;
+    if(i > nref || int(blockIdx.x) >= nch)
+        return; // uniform over the workgroup
+    XOff const x = xoff(n);
+    DOff const o = doff(n, B.gsym);
+    double* __restrict__ base = D + B.dbase;
+    double* __restrict__ lq = D + B.xbase + x.lq;
+    int64_t const par = int64_t(x.nchunk) * 40 + 40; // one parity's partial sums
+    int const tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int const c = blockIdx.x * kLQT + tid;
+    bool const live = c < m;
+    int const gc = k + kBand + c; // column in the block
+    double* __restrict__ P = Aar + B.mat_off + int64_t(k) * n + (k + kBand);
+    double p[kBand];
+#pragma unroll
+    for(int r = 0; r < kBand; ++r)
+        p[r] = (live && r >= i - 1) ? P[int64_t(r) * n + c] : 0.0;
+    if(i == 0 && live)
+        for(int q = nref; q < kBand; ++q) // rows of Vt beyond the panel's reflectors: zero
+            base[o.Vt + int64_t(q) * n + gc] = 0.0;
+    if(i == 0 && blockIdx.x == 0 && tid >= nref && tid < kBand)
+        base[o.tau + k + tid] = 0.0;
+    if(i >= 1) {
+        int const q = i - 1;
+        const double* __restrict__ src = lq + (q & 1) * par;
+        // the partial sums of every chunk, finished in a fixed order: 7 threads per value take
+        // every 7th chunk, then one thread per value adds their sums
+        if(tid < 7 * 33) {
+            int const e = tid % 33, g = tid / 33;
+            double s0 = 0.0, s1 = 0.0;
+            int b = g;
+            for(; b + 7 < nch; b += 14) {
+                s0 += src[int64_t(b) * 40 + e];
+                s1 += src[int64_t(b + 7) * 40 + e];
+            }
+            if(b < nch)
+                s0 += src[int64_t(b) * 40 + e];
+            fin7[g][e] = s0 + s1;
+        }
+        if(tid < kBand)
+            colv[tid] = src[int64_t(x.nchunk) * 40 + tid];
+        __syncthreads();
+        if(tid < 33)
+            fin[tid] = ((fin7[0][tid] + fin7[1][tid]) + (fin7[2][tid] + fin7[3][tid])) + ((fin7[4][tid] + fin7[5][tid]) + fin7[6][tid]);
+        __syncthreads();
+        double const alpha = colv[q], sigma = fin[32];
+        double tau = 0.0, beta = alpha, scale = 0.0;
+        if(sigma > 0.0) {
+            beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
+            tau = (beta - alpha) / beta;
+            scale = 1.0 / (alpha - beta);
+        }
+        double v = 0.0; // v_q[c]: 1 at c = q, the scaled tail beyond, 0 before
+        if(live)
+            v = (c == q) ? 1.0 : (c > q ? p[q] * scale : 0.0);
+#pragma unroll
+        for(int r = 0; r < kBand; ++r)
+            if(r > q)
+                p[r] = fma(-tau * fma(scale, fin[r], colv[r]), v, p[r]);
+        if(live) {
+            if(c == q)
+                p[q] = beta;
+            else if(c > q)
+                p[q] = 0.0;
+            base[o.Vt + int64_t(q) * n + gc] = v;
+            if(c >= q)
+                base[o.V + int64_t(gc) * n + (k + q)] = v;
+        }
+        if(blockIdx.x == 0 && tid == 0)
+            base[o.tau + k + q] = tau;
+    }
+    if(i < nref) {
+        double* __restrict__ dst = lq + (i & 1) * par;
+        bool const tail = live && c > i;
+        double const pi = tail ? p[i] : 0.0;
+#pragma unroll
+        for(int r = 0; r < kBand; ++r) {
+            double const s = wave_sum(r > i ? p[r] * pi : 0.0);
+            if(lane == 0)
+                red[wave][r] = s;
+        }
+        double const sg = wave_sum(pi * pi);
+        if(lane == 0)
+            red[wave][32] = sg;
+        __syncthreads();
+        if(tid < 33)
+            dst[int64_t(blockIdx.x) * 40 + tid] = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+        if(live && c == i) {
+#pragma unroll
+            for(int r = 0; r < kBand; ++r)
+                if(r >= i)
+                    dst[int64_t(x.nchunk) * 40 + r] = p[r];
+        }
+    }
+    if(i >= 1 && live) {
+#pragma unroll
+        for(int r = 0; r < kBand; ++r)
+            if(r >= i - 1)
+                P[int64_t(r) * n + c] = p[r];
+    }
+}
+
+// Partial sums over chunks of kGramW columns of C = X Y^T (kBand x kBand): X = Vt, Y = Vt
+// (which 0) or W (which 1), rows of length m from column k + kBand.
+__global__ void __launch_bounds__(256) gram_kernel(const BigBlk* __restrict__ blks, int k, int which, double* __restrict__ D) {
+    __shared__ double xs[kBand][kGramW + 1], ys[kBand][kGramW + 1];
+    BigBlk const B = blks[blockIdx.y];
+    int const n = B.n, m = n - k - kBand;
+    int const c0 = blockIdx.x * kGramW;
+    if(m < 2 || c0 >= m)
+        return;
+    XOff const x = xoff(n);
+    DOff const o = doff(n, B.gsym);
+    const double* __restrict__ X = D + B.dbase + o.Vt + (k + kBand);
+    const double* __restrict__ Y = D + B.dbase + (which ? o.W : o.Vt) + (k + kBand);
+    for(int e = threadIdx.x; e < kBand * kGramW; e += 256) {
+        int const r = e / kGramW, cc = e % kGramW, c = c0 + cc;
+        xs[r][cc] = c < m ? X[int64_t(r) * n + c] : 0.0;
+        ys[r][cc] = c < m ? Y[int64_t(r) * n + c] : 0.0;
+    }
+    __syncthreads();
+    double* __restrict__ out = D + B.xbase + x.gp + int64_t(blockIdx.x) * kBand * kBand;
+    for(int e = threadIdx.x; e < kBand * kBand; e += 256) {
+        int const p = e / kBand, q = e % kBand;
+        double s0 = 0.0, s1 = 0.0;
+        for(int cc = 0; cc < kGramW; cc += 2) {
+            s0 = fma(xs[p][cc], ys[q][cc], s0);
+            s1 = fma(xs[p][cc + 1], ys[q][cc + 1], s1);
+        }
+        out[e] = s0 + s1;
+    }
+}
+
+// One workgroup per block: the Gram partial sums finished (fixed order), then which 0: T of
+// Q = I - V T V^T = H_0 ... H_{kBand-1} (forward, upper; T[:i, i] = -tau_i T[:i, :i] S[:i, i],
+// T_ii = tau_i) to o.T; which 1: N = T^T M (M = V^T W) to o.S.  Both kBand x kBand, leading
+// dimension kBand.
+__global__ void __launch_bounds__(256) panel_t_kernel(const BigBlk* __restrict__ blks, int k, int which, double* __restrict__ D) {
+    __shared__ double S[kBand][kBand + 1], T[kBand][kBand + 1];
+    BigBlk const B = blks[blockIdx.y];
+    int const n = B.n, m = n - k - kBand;
+    if(m < 2)
+        return;
+    XOff const x = xoff(n);
+    DOff const o = doff(n, B.gsym);
+    double* __restrict__ base = D + B.dbase;
+    int const nchg = (m + kGramW - 1) / kGramW;
+    const double* __restrict__ gp = D + B.xbase + x.gp;
+    int const tid = threadIdx.x;
+    for(int e = tid; e < kBand * kBand; e += 256) {
+        double s = 0.0;
+        for(int b = 0; b < nchg; ++b)
+            s += gp[int64_t(b) * kBand * kBand + e];
+        S[e / kBand][e % kBand] = s;
+        T[e / kBand][e % kBand] = which ? base[o.T + e] : 0.0;
+    }
+    __syncthreads();
+    if(which == 0) {
+        if(tid < kBand) { // thread r builds row r of T (reads only its own row and S)
+            int const r = tid;
+            for(int i = 0; i < kBand; ++i) {
+                double const ti = base[o.tau + k + i];
+                if(r < i) {
+                    double a = 0.0;
+                    for(int q = r; q < i; ++q)
+                        a = fma(T[r][q], S[q][i], a);
+                    T[r][i] = -ti * a;
+                } else if(r == i) {
+                    T[r][i] = ti;
+                }
+            }
+        }
+        __syncthreads();
+        for(int e = tid; e < kBand * kBand; e += 256)
+            base[o.T + e] = T[e / kBand][e % kBand];
+    } else {
+        for(int e = tid; e < kBand * kBand; e += 256) {
+            int const a = e / kBand, c = e % kBand;
+            double s = 0.0;
+            for(int r = 0; r <= a; ++r) // T upper triangular: T[r][a] = 0 for r > a
+                s = fma(T[r][a], S[r][c], s);
+            base[o.S + e] = s;
+        }
+    }
+}
+
+// W3 = W over the panel's trailing columns (the trailing product reads [W; Vt] and [Vt; W3]).
+__global__ void copy_w3_kernel(const BigBlk* __restrict__ blks, int k, double* __restrict__ D) {
+    BigBlk const B = blks[blockIdx.y];
+    int const n = B.n, m = n - k - kBand;
+    if(m < 2)
+        return;
+    DOff const o = doff(n, B.gsym);
+    double* __restrict__ base = D + B.dbase;
+    int64_t const cnt = int64_t(kBand) * m;
+    for(int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < cnt; q += int64_t(gridDim.x) * blockDim.x) {
+        int64_t const at = (q / m) * n + (k + kBand) + (q % m);
+        base[o.W3 + at] = base[o.W + at];
+    }
+}
+
+// The band of the reduced matrix (upper rows, A[j][j+d] for d <= kBand) in lower band storage;
+// the bulge room (d > kBand) zeroed.
+__global__ void band_kernel(const BigBlk* __restrict__ blks, const double* __restrict__ Aar, double* __restrict__ D) {
+    BigBlk const B = blks[blockIdx.y];
+    int const n = B.n;
+    double* __restrict__ AB = D + B.xbase + xoff(n).AB;
+    int64_t const cnt = int64_t(n) * 2 * kBand;
+    for(int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < cnt; q += int64_t(gridDim.x) * blockDim.x) {
+        int64_t const j = q / (2 * kBand), d = q % (2 * kBand);
+        AB[q] = (d <= kBand && j + d < n) ? Aar[B.mat_off + j * n + j + d] : 0.0;
+    }
+}
+
+// Stage 2, the bulge chase, one launch per wavefront step t: the launch runs every task (s, k) of
+// every block with 3 s + k = t (one workgroup each).  Task (s, k) is step k of sweep s: with W_k =
+// rows [s+1+k kBand, s+(k+1) kBand] (clipped to the block), the reflector of the column being
+// annihilated (column s at k = 0, else the first column of W_{k-1}, the bulge left by step k-1)
+// over W_k is formed and applied from the left to the rest of that bulge block A[W_k, W_{k-1}],
+// from both sides to the diagonal block A[W_k, W_k], and from the right to A[W_{k+1}, W_k], which
+// creates the next bulge.  It needs step k-1 of its sweep and step k+2 of sweep s-1 (both at
+// t-1), and it reads and writes only the band columns [annihilated column, end of W_k), a
+// contiguous range of AB staged in LDS; the tasks of one step lie at least three steps apart
+// along their sweeps, so their column ranges are disjoint.  The kernel boundaries order the
+// steps.
+__host__ __device__ inline void sweep_range(int n, int t, int& lo, int& hi) {
+    // sweeps s of an n-block with a step k = t - 3 s, 0 <= k < sweep_steps(n, s):
+    // k <= (n-2-s)/kBand  <=>  kBand t - (n-2) <= (3 kBand - 1) s
+    hi = min(n - 3, t / 3);
+    int64_t const num = int64_t(kBand) * t - (n - 2);
+    lo = num <= 0 ? 0 : int((num + 3 * kBand - 2) / (3 * kBand - 1));
+}
+
+__global__ void __launch_bounds__(256) bulge_step_kernel(const BigBlk* __restrict__ blks, int nblk, int t, double* __restrict__ D) {
+    constexpr int b = kBand, LD = 2 * kBand;
+    __shared__ double raw[2 * kBand][2 * kBand]; // footprint: band column (from c) x offset d
+    __shared__ double vs[kBand], ws[kBand];
+    __shared__ double sc;
+    int const tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // this workgroup's task: blocks in order, sweeps ascending
+    int idx = blockIdx.x, blk = -1, s = 0;
+    for(int q = 0; q < nblk; ++q) {
+        int lo, hi;
+        sweep_range(blks[q].n, t, lo, hi);
+        int const cnt = max(0, hi - lo + 1);
+        if(idx < cnt) {
+            blk = q;
+            s = lo + idx;
+            break;
+        }
+        idx -= cnt;
+    }
+    if(blk < 0)
+        return;
+    BigBlk const B = blks[blk];
+    int const n = B.n, k = t - 3 * s;
+    XOff const x = xoff(n);
+    double* __restrict__ AB = D + B.xbase + x.AB;
+    double* __restrict__ V2 = D + B.xbase + x.V2;
+    int const r0 = s + 1 + k * b;       // first row of W_k
+    int const L = min(b, n - r0);       // rows of W_k (>= 1)
+    int const L2 = min(b, n - r0 - b);  // rows of W_{k+1}
+    int const c = (k == 0) ? s : r0 - b; // annihilated column = first footprint column
+    int const ncol = r0 + L - c;
+    for(int e = tid; e < ncol * LD; e += 256)
+        raw[e / LD][e % LD] = AB[int64_t(c) * LD + e];
+    __syncthreads();
+    // A[i][j] (i >= j) = raw[j - c][i - j]
+    if(wave == 0) {
+        double const xa = (lane < L) ? raw[0][r0 + lane - c] : 0.0;
+        double const alpha = __shfl(xa, 0, 64);
+        double const sig = wave_sum(lane >= 1 ? xa * xa : 0.0);
+        double tau = 0.0, beta = alpha, scale = 0.0;
+        if(sig > 0.0) {
+            beta = -copysign(sqrt(alpha * alpha + sig), alpha);
+            tau = (beta - alpha) / beta;
+            scale = 1.0 / (alpha - beta);
+        }
+        if(lane < b) {
+            double const v = (lane == 0) ? 1.0 : (lane < L ? xa * scale : 0.0);
+            vs[lane] = v;
+            V2[(int64_t(s) * x.kmax + k) * b + lane] = (lane == 0) ? tau : v;
+            if(lane < L)
+                raw[0][r0 + lane - c] = (lane == 0) ? beta : 0.0;
+        }
+        if(lane == 0)
+            sc = tau;
+    }
+    __syncthreads();
+    double const tau = sc;
+    if(tau != 0.0) {
+        if(k >= 1) { // left: columns c+1 .. c+b-1 of the bulge block, rows W_k (half-wave each)
+            int const hw = tid >> 5, a = tid & 31;
+            for(int qq = 1 + hw; qq < b; qq += 8) {
+                double const e = (a < L) ? raw[qq][b + a - qq] : 0.0;
+                double dot = (a < L) ? e * vs[a] : 0.0;
+                for(int o = 16; o > 0; o >>= 1)
+                    dot += __shfl_xor(dot, o, 32);
+                if(a < L)
+                    raw[qq][b + a - qq] = fma(-tau * dot, vs[a], e);
+            }
+            __syncthreads();
+        }
+        int const c0 = r0 - c; // raw column of row/column r0
+        { // both sides on D = A[W_k, W_k]: w = tau D v (8 threads per row)
+            int const a = tid >> 3, part = tid & 7;
+            double acc = 0.0;
+            if(a < L)
+                for(int u = 0; u < 4; ++u) {
+                    int const qv = part * 4 + u;
+                    if(qv < L) {
+                        int const lo = min(a, qv), hi = max(a, qv);
+                        acc = fma(raw[c0 + lo][hi - lo], vs[qv], acc);
+                    }
+                }
+            acc += __shfl_xor(acc, 1, 64);
+            acc += __shfl_xor(acc, 2, 64);
+            acc += __shfl_xor(acc, 4, 64);
+            if(part == 0)
+                ws[a] = (a < L) ? tau * acc : 0.0;
+        }
+        __syncthreads();
+        if(wave == 0) {
+            double const wv = (lane < L) ? ws[lane] * vs[lane] : 0.0;
+            double const al = -0.5 * tau * wave_sum(wv);
+            if(lane < L)
+                ws[lane] = fma(al, vs[lane], ws[lane]);
+        }
+        __syncthreads();
+        for(int e = tid; e < b * b; e += 256) {
+            int const a = e / b, qv = e % b;
+            if(a < L && qv <= a)
+                raw[c0 + qv][a - qv] -= vs[a] * ws[qv] + ws[a] * vs[qv];
+        }
+        // right on A[W_{k+1}, W_k] (rows below W_k; disjoint from the entries above)
+        {
+            int const a2 = tid >> 3, part = tid & 7;
+            double acc = 0.0;
+            if(a2 < L2)
+                for(int u = 0; u < 4; ++u) {
+                    int const qv = part * 4 + u;
+                    if(qv < L)
+                        acc = fma(raw[c0 + qv][b + a2 - qv], vs[qv], acc);
+                }
+            acc += __shfl_xor(acc, 1, 64);
+            acc += __shfl_xor(acc, 2, 64);
+            acc += __shfl_xor(acc, 4, 64);
+            if(a2 < L2)
+                for(int u = 0; u < 4; ++u) {
+                    int const qv = part * 4 + u;
+                    if(qv < L)
+                        raw[c0 + qv][b + a2 - qv] = fma(-tau * acc, vs[qv], raw[c0 + qv][b + a2 - qv]);
+                }
+        }
+    }
+    __syncthreads();
+    for(int e = tid; e < ncol * LD; e += 256)
+        AB[int64_t(c) * LD + e] = raw[e / LD][e % LD];
+}
+
+// d and e of the tridiagonal matrix left by the bulge chase.
+__global__ void band_de_kernel(const BigBlk* __restrict__ blks, double* __restrict__ D) {
+    BigBlk const B = blks[blockIdx.y];
+    int const n = B.n;
+    DOff const o = doff(n, B.gsym);
+    const double* __restrict__ AB = D + B.xbase + xoff(n).AB;
+    double* __restrict__ base = D + B.dbase;
+    for(int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        base[o.d + j] = AB[int64_t(j) * 2 * kBand];
+        base[o.e + j] = (j + 1 < n) ? AB[int64_t(j) * 2 * kBand + 1] : 0.0;
+    }
+}
+
+// X = Q2 Z in place on the eigenvector matrix (row-major; Q2 acts on its rows).  Q2 = product of
+// the bulge-chasing reflectors in generation order, so they are applied last first.  Sweeps go in
+// groups of kQ2G, groups last to first; inside a group, steps k ascending, and a step's
+// reflectors (rows [S+1+k kBand, S+k kBand+kBand+kQ2G-2] together) later sweep first.  Reflectors
+// of later steps of earlier sweeps lie below those of earlier steps of later sweeps, so they
+// commute and this order equals the reverse generation order.  One lane per column of Z: a step's
+// rows stay in registers and slide down by kBand rows per step, and the reflector entries are
+// wave-uniform.  The waves of a workgroup take consecutive groups, each a step behind the wave
+// before it (step k of group G-1 needs group G's steps up to k; group G's step k+1 and group
+// G-1's step k touch disjoint rows), one barrier per step.
+template <int NW>
+__global__ void __launch_bounds__(64 * NW) q2_kernel(const BigBlk* __restrict__ blks, const double* __restrict__ D,
+                                                     double* __restrict__ Aar) {
+    constexpr int b = kBand, G = kQ2G, WIN = kBand + kQ2G - 1;
+    BigBlk const B = blks[blockIdx.y];
+    int const n = B.n, nsw = n - 2;
+    if(int(blockIdx.x) * 64 >= n || nsw <= 0)
+        return; // uniform over the workgroup
+    int const lane = threadIdx.x & 63;
+    int const wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // wave-uniform: scalar branches and loads
+    int const col = blockIdx.x * 64 + lane;
+    bool const live = col < n;
+    XOff const x = xoff(n);
+    const double* __restrict__ V2 = D + B.xbase + x.V2;
+    double* __restrict__ Z = Aar + B.mat_off;
+    int const ngroups = (nsw + G - 1) / G;
+    double win[WIN];
+    for(int rd = 0; rd * NW < ngroups; ++rd) {
+        int tsteps = 0; // the round's steps: max over the waves of (steps of its group + its lag)
+        for(int w = 0; w < NW; ++w) {
+            int const g = ngroups - 1 - (rd * NW + w);
+            if(g >= 0)
+                tsteps = max(tsteps, sweep_steps(n, g * G) + w);
+        }
+        int const gid = ngroups - 1 - (rd * NW + wave);
+        int const S = gid * G;
+        int const kc = gid >= 0 ? sweep_steps(n, S) : 0;
+        for(int t = 0; t < tsteps; ++t) {
+            int const k = t - wave;
+            if(k >= 0 && k < kc) {
+                int const rb = S + 1 + k * b; // first row of the window
+                if(k == 0) {
+#pragma unroll
+                    for(int i = 0; i < WIN; ++i) {
+                        int const row = rb + i;
+                        win[i] = (live && row < n) ? Z[int64_t(row) * n + col] : 0.0;
+                    }
+                } else {
+#pragma unroll
+                    for(int i = 0; i < G - 1; ++i)
+                        win[i] = win[i + b];
+#pragma unroll
+                    for(int i = G - 1; i < WIN; ++i) {
+                        int const row = rb + i;
+                        win[i] = (live && row < n) ? Z[int64_t(row) * n + col] : 0.0;
+                    }
+                }
+#pragma unroll
+                for(int j = G - 1; j >= 0; --j) {
+                    int const sj = S + j;
+                    if(sj < nsw && k < sweep_steps(n, sj)) { // (tau = 0 rows hold v = 0: no branch on tau)
+                        const double* __restrict__ vr = V2 + (int64_t(sj) * x.kmax + k) * b;
+                        double const tau = vr[0];
+                        double d0 = win[j], d1 = 0.0, d2 = 0.0, d3 = 0.0;
+#pragma unroll
+                        for(int i = 1; i < b; i += 4) {
+                            d1 = fma(vr[i], win[j + i], d1);
+                            if(i + 1 < b)
+                                d2 = fma(vr[i + 1], win[j + i + 1], d2);
+                            if(i + 2 < b)
+                                d3 = fma(vr[i + 2], win[j + i + 2], d3);
+                            if(i + 3 < b)
+                                d0 = fma(vr[i + 3], win[j + i + 3], d0);
+                        }
+                        double const f = tau * ((d0 + d1) + (d2 + d3));
+                        win[j] -= f;
+#pragma unroll
+                        for(int i = 1; i < b; ++i)
+                            win[j + i] = fma(-f, vr[i], win[j + i]);
+                    }
+                    // one reflector's entries live in scalar registers at a time
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                int const nst = (k == kc - 1) ? WIN : b; // rows leaving the window (all at the last step)
+#pragma unroll
+                for(int i = 0; i < WIN; ++i) {
+                    int const row = rb + i;
+                    if(i < nst && live && row < n)
+                        Z[int64_t(row) * n + col] = win[i];
+                }
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -1049,13 +1609,23 @@ struct BigPlan {
     };
     std::vector<BtPanel> bt;
     size_t bytes = 0;
+    // two-stage reduction (dense -> band -> tridiagonal) instead of the one-stage one
+    bool two_stage = false;
+    struct Panel2 { // stage-1 panel k: blocks [0, nact) active; GEMM jobs X, W, Z, trailing (nact each)
+        int k, nact, nref_max, nchunk, ngram, mmax;
+        int64_t jobs_off;
+    };
+    std::vector<Panel2> p2;
+    std::vector<int> step_tasks; // bulge chase: tasks of wavefront step t (all blocks)
+    BigPlan* next = nullptr; // the two-stage plan of the larger blocks, run after this one
 };
 
-size_t big_plan_bytes(BigPlan const* p) { return p ? p->bytes : 0; }
+size_t big_plan_bytes(BigPlan const* p) { return p ? p->bytes + big_plan_bytes(p->next) : 0; }
 
 void big_plan_destroy(BigPlan* p) {
     if(!p)
         return;
+    big_plan_destroy(p->next);
     void* ptrs[] = {p->blk,       p->D,          p->I,       p->R,      p->ids,   p->gjobs,  p->leaves,
                     p->leaf_jobs, p->leaf_arena, p->leaf_ws, p->status, p->nodes, p->states, p->dc_jobs};
     for(void* q : ptrs)
@@ -1067,10 +1637,11 @@ void big_plan_destroy(BigPlan* p) {
 namespace {
 
 int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off, const int64_t* val_off, bool tri_only,
-               BigPlan** out) {
+               bool two_stage, BigPlan** out) {
     *out = nullptr;
     auto* P = new BigPlan();
     P->tri_only = tri_only;
+    P->two_stage = two_stage && !tri_only;
     hipStream_t s = ctx->stream;
     // blocks sorted by size, largest first (the active blocks of a reduction step are a prefix)
     std::vector<int> ord(static_cast<size_t>(nblk));
@@ -1090,8 +1661,12 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
             delete P;
             return pomx_fail(ctx, POMX_ERR_ARG, "large-block eigensolver: blocks of size >= 2 expected");
         }
-        P->h_blk.push_back(BigBlk{n, P->gsym, mat_off[x], val_off[x], D, I, R});
+        P->h_blk.push_back(BigBlk{n, P->gsym, mat_off[x], val_off[x], D, I, R, -1});
         D += doff(n, P->gsym).end;
+        if(P->two_stage) {
+            P->h_blk.back().xbase = D;
+            D += xoff(n).end;
+        }
         I += ioff(n).end;
         R += n;
     }
@@ -1100,7 +1675,50 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
     P->R_len = R;
     std::vector<int> ids;
     std::vector<pomx_gemm_job> gj;
-    if(!tri_only) {
+    if(P->two_stage) {
+        // stage-1 panels: per active block (m = n - k - kBand >= 2; a prefix, blocks largest first)
+        // the jobs X = Vt A22, W = T^T X, W -= N^T Vt / 2, A22 -= [W Vt]^T [Vt W3]
+        for(int k = 0; P->nmax - k - kBand >= 2; k += kBand) {
+            BigPlan::Panel2 Q{k, 0, 0, 0, 0, P->nmax - k - kBand, int64_t(gj.size())};
+            while(Q.nact < nblk && P->h_blk[size_t(Q.nact)].n - k - kBand >= 2)
+                ++Q.nact;
+            Q.nref_max = std::min(kBand, Q.mmax - 1);
+            Q.nchunk = (Q.mmax + kLQT - 1) / kLQT;
+            Q.ngram = (Q.mmax + kGramW - 1) / kGramW;
+            for(int kind = 0; kind < 4; ++kind)
+                for(int b = 0; b < Q.nact; ++b) {
+                    BigBlk const& B = P->h_blk[size_t(b)];
+                    int const n = B.n, m = n - k - kBand, k1 = k + kBand;
+                    DOff const o = doff(n, B.gsym);
+                    int64_t const a22 = B.mat_off + int64_t(k1) * n + k1;
+                    if(kind == 0)
+                        gj.push_back(pomx_gemm_job{kBand, m, m, 0, 0, n, n, n, B.dbase + o.Vt + k1, a22, B.dbase + o.W1 + k1,
+                                                   1.0, 0.0});
+                    else if(kind == 1)
+                        gj.push_back(pomx_gemm_job{kBand, m, kBand, 1, 0, kBand, n, n, B.dbase + o.T, B.dbase + o.W1 + k1,
+                                                   B.dbase + o.W + k1, 1.0, 0.0});
+                    else if(kind == 2)
+                        gj.push_back(pomx_gemm_job{kBand, m, kBand, 1, 0, kBand, n, n, B.dbase + o.S, B.dbase + o.Vt + k1,
+                                                   B.dbase + o.W + k1, -0.5, 1.0});
+                    else
+                        gj.push_back(pomx_gemm_job{m, m, 2 * kBand, 1, 0, n, n, n, B.dbase + o.W + k1, B.dbase + o.Vt + k1,
+                                                   a22, -1.0, 1.0});
+                }
+            P->p2.push_back(Q);
+        }
+        // bulge chase: tasks per wavefront step (step t runs the tasks (s, t - 3 s) of every block)
+        if(P->nmax >= 3) {
+            int const tend = 3 * (P->nmax - 3) + sweep_steps(P->nmax, 0);
+            P->step_tasks.assign(size_t(tend) + 1, 0);
+            for(int t = 0; t <= tend; ++t)
+                for(auto const& B : P->h_blk) {
+                    int lo, hi;
+                    sweep_range(B.n, t, lo, hi);
+                    P->step_tasks[size_t(t)] += std::max(0, hi - lo + 1);
+                }
+        }
+    }
+    if(!tri_only && !P->two_stage) {
         int const jmax = P->nmax - 2;
         std::vector<std::vector<int>> at(size_t(std::max(jmax + 1, 0)));
         for(int b = 0; b < nblk; ++b) {
@@ -1173,13 +1791,15 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
         nodes.insert(nodes.end(), L.begin(), L.end());
     }
     P->nleaves = int(leaves.size());
-    // back-transformation panels, last to first
-    if(!tri_only) {
-        int const last = ((P->nmax - 2) / kBT) * kBT;
+    // back-transformation panels, last to first; reflector j acts on rows j+sh.. (sh = 1, or kBand
+    // after the two-stage reduction, whose reflectors are j = 0 .. n-kBand-2)
+    int const sh = P->two_stage ? kBand : 1;
+    if(!tri_only && P->nmax - sh >= 1) {
+        int const last = ((P->nmax - sh - 1) / kBT) * kBT;
         for(int j0 = last; j0 >= 0; j0 -= kBT) {
             std::vector<int> L;
             for(int b = 0; b < nblk; ++b)
-                if(j0 < P->h_blk[size_t(b)].n - 1)
+                if(j0 < P->h_blk[size_t(b)].n - sh)
                     L.push_back(b);
             if(L.empty())
                 continue;
@@ -1187,23 +1807,23 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
             for(int kind = 0; kind < 4; ++kind) // jobs grouped by kind: S, W1, W2, X
                 for(int b : L) {
                     BigBlk const& B = P->h_blk[size_t(b)];
-                    int const n = B.n, nb = std::min(kBT, n - 1 - j0), rows = n - j0 - 1;
+                    int const n = B.n, nb = std::min(kBT, n - sh - j0), rows = n - j0 - sh;
                     DOff const o = doff(n, B.gsym);
-                    int64_t const voff = B.dbase + o.V + int64_t(j0 + 1) * n + j0;
+                    int64_t const voff = B.dbase + o.V + int64_t(j0 + sh) * n + j0;
                     if(kind == 0) {
                         gj.push_back(pomx_gemm_job{nb, nb, rows, 1, 0, n, n, nb, voff, voff, B.dbase + o.S, 1.0, 0.0});
                         ids.push_back(b);
                         T.nb_max = std::max(T.nb_max, nb);
                         T.rows_max = std::max(T.rows_max, rows);
                     } else if(kind == 1)
-                        gj.push_back(pomx_gemm_job{nb, n, rows, 1, 0, n, n, n, voff, B.mat_off + int64_t(j0 + 1) * n,
+                        gj.push_back(pomx_gemm_job{nb, n, rows, 1, 0, n, n, n, voff, B.mat_off + int64_t(j0 + sh) * n,
                                                    B.dbase + o.W1, 1.0, 0.0});
                     else if(kind == 2)
                         gj.push_back(pomx_gemm_job{nb, n, nb, 0, 0, nb, n, n, B.dbase + o.T, B.dbase + o.W1, B.dbase + o.W2,
                                                    1.0, 0.0});
                     else
                         gj.push_back(pomx_gemm_job{rows, n, nb, 0, 0, n, n, n, voff, B.dbase + o.W2,
-                                                   B.mat_off + int64_t(j0 + 1) * n, -1.0, 1.0});
+                                                   B.mat_off + int64_t(j0 + sh) * n, -1.0, 1.0});
                 }
             P->bt.push_back(T);
         }
@@ -1238,6 +1858,48 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
     return POMX_OK;
 }
 
+// Stage 1 (dense -> band) and stage 2 (band -> tridiagonal, d and e to the block scratch).
+int run_two_stage(pomx_ctx* ctx, BigPlan* P, double* Aar) {
+    hipStream_t s = ctx->stream;
+    int rc;
+    int const nb = P->nblk;
+    fill_blocks_kernel<<<dim3(gx_rows(int64_t(P->nmax) * P->nmax, 256, nb), unsigned(nb)), 256, 0, s>>>(P->blk, P->D, 1, 0.0);
+    {
+        int64_t const nt = (P->nmax + 63) / 64;
+        sym_lower_kernel<<<dim3(gx_rows(nt * nt, 1, nb), unsigned(nb)), 256, 0, s>>>(P->blk, Aar);
+    }
+    POMX_CHECK_LAUNCH(ctx);
+    for(auto const& Q : P->p2) {
+        unsigned const na = unsigned(Q.nact);
+        for(int i = 0; i <= Q.nref_max; ++i)
+            lq_kernel<<<dim3(unsigned(Q.nchunk), na), kLQT, 0, s>>>(P->blk, Q.k, i, Aar, P->D);
+        gram_kernel<<<dim3(unsigned(Q.ngram), na), 256, 0, s>>>(P->blk, Q.k, 0, P->D);
+        panel_t_kernel<<<dim3(1, na), 256, 0, s>>>(P->blk, Q.k, 0, P->D);
+        POMX_CHECK_LAUNCH(ctx);
+        int64_t const tc = (Q.mmax + 63) / 64;
+        pomx_gemm_job const* J = P->gjobs + Q.jobs_off;
+        if((rc = launch_gemm_jobs(ctx, 0, 0, J, Q.nact, tc, P->D, Aar, P->D)) ||
+           (rc = launch_gemm_jobs(ctx, 1, 0, J + Q.nact, Q.nact, tc, P->D, P->D, P->D)))
+            return rc;
+        gram_kernel<<<dim3(unsigned(Q.ngram), na), 256, 0, s>>>(P->blk, Q.k, 1, P->D);
+        panel_t_kernel<<<dim3(1, na), 256, 0, s>>>(P->blk, Q.k, 1, P->D);
+        POMX_CHECK_LAUNCH(ctx);
+        if((rc = launch_gemm_jobs(ctx, 1, 0, J + 2 * Q.nact, Q.nact, tc, P->D, P->D, P->D)))
+            return rc;
+        copy_w3_kernel<<<dim3(gx_rows(int64_t(kBand) * Q.mmax, 256, Q.nact), na), 256, 0, s>>>(P->blk, Q.k, P->D);
+        POMX_CHECK_LAUNCH(ctx);
+        if((rc = launch_gemm_jobs(ctx, 1, 0, J + 3 * Q.nact, Q.nact, tc * tc, P->D, P->D, Aar)))
+            return rc;
+    }
+    band_kernel<<<dim3(gx_rows(int64_t(P->nmax) * 2 * kBand, 256, nb), unsigned(nb)), 256, 0, s>>>(P->blk, Aar, P->D);
+    for(size_t t = 0; t < P->step_tasks.size(); ++t)
+        if(P->step_tasks[t] > 0)
+            bulge_step_kernel<<<unsigned(P->step_tasks[t]), 256, 0, s>>>(P->blk, nb, int(t), P->D);
+    band_de_kernel<<<dim3(gx_rows(P->nmax, 256, nb), unsigned(nb)), 256, 0, s>>>(P->blk, P->D);
+    POMX_CHECK_LAUNCH(ctx);
+    return POMX_OK;
+}
+
 int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
     hipStream_t s = ctx->stream;
     int rc;
@@ -1247,7 +1909,10 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
     zero_ints_kernel<<<gx_for(P->I_len), 256, 0, s>>>(P->I, P->I_len);
     zero_ints_kernel<<<gx_for(std::max(P->nleaves, 1)), 256, 0, s>>>(P->status, std::max(P->nleaves, 1));
     // 1. reduction to tridiagonal form
-    if(!P->tri_only) {
+    if(P->two_stage) {
+        if((rc = run_two_stage(ctx, P, Aar)))
+            return rc;
+    } else if(!P->tri_only) {
         fill_blocks_kernel<<<dim3(gx_rows(int64_t(P->nmax) * P->nmax, 256, nb), unsigned(nb)), 256, 0, s>>>(P->blk, P->D, 1,
                                                                                                         0.0);
         size_t e = 0;
@@ -1318,6 +1983,11 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
     // 3. back-transformation, eigenvalues
     if(!P->tri_only) {
         copy_z_kernel<<<dim3(gx_rows(int64_t(P->nmax) * P->nmax, 256, nb), unsigned(nb)), 256, 0, s>>>(P->blk, P->D, Aar);
+        if(P->two_stage) { // Q2 first: X = Q1 (Q2 Z)
+            q2_kernel<kQ2Waves><<<dim3(unsigned((P->nmax + 63) / 64), unsigned(nb)), 64 * kQ2Waves, 0, s>>>(P->blk, P->D, Aar);
+            POMX_CHECK_LAUNCH(ctx);
+        }
+        int const sh = P->two_stage ? kBand : 1;
         int64_t const tn = (P->nmax + 63) / 64;
         for(auto const& T : P->bt) {
             pomx_gemm_job const* J = P->gjobs + T.jobs_off;
@@ -1325,7 +1995,7 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
             int64_t const tr = (T.rows_max + 63) / 64;
             if((rc = launch_gemm_jobs(ctx, 1, 0, J, T.nblk, tb * tb, P->D, P->D, P->D)))
                 return rc;
-            wy_t_kernel<<<unsigned(T.nblk), kBT, 0, s>>>(P->blk, P->ids + T.ids_off, T.j0, P->D);
+            wy_t_kernel<<<unsigned(T.nblk), kBT, 0, s>>>(P->blk, P->ids + T.ids_off, T.j0, sh, P->D);
             if((rc = launch_gemm_jobs(ctx, 1, 0, J + T.nblk, T.nblk, tb * tn, P->D, Aar, P->D)) ||
                (rc = launch_gemm_jobs(ctx, 0, 0, J + 2 * T.nblk, T.nblk, tb * tn, P->D, P->D, P->D)) ||
                (rc = launch_gemm_jobs(ctx, 0, 0, J + 3 * T.nblk, T.nblk, tr * tn, P->D, P->D, Aar)))
@@ -1339,23 +2009,67 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
 
 } // namespace
 
+// Blocks of at least this size take the two-stage reduction: kTwoStageMin, or a positive integer
+// in the environment variable POMX_EIGH_TWO_STAGE_MIN (a test hook, so that the two-stage path
+// is tested on small blocks).
+int two_stage_min() {
+    if(const char* e = std::getenv("POMX_EIGH_TWO_STAGE_MIN")) {
+        long const v = std::atol(e);
+        if(v > 0)
+            return int(std::min<long>(v, 1L << 30));
+    }
+    return kTwoStageMin;
+}
+
+size_t big_block_bytes(int n) {
+    int const gsym = std::max(1, std::min(kMaxSymvGrid, (n - 1 + 15) / 16));
+    int64_t d = doff(n, gsym).end + (n >= two_stage_min() ? xoff(n).end : 0);
+    return sizeof(double) * size_t(d) + sizeof(int) * size_t(ioff(n).end) + sizeof(DcRot) * size_t(n);
+}
+
 int big_plan_create(pomx_ctx* ctx, int nblk, const int* n, const int64_t* mat_off, const int64_t* val_off, BigPlan** plan) {
-    return plan_build(ctx, nblk, n, mat_off, val_off, false, plan);
+    *plan = nullptr;
+    int const tmin = two_stage_min();
+    std::vector<int> n1, n2;
+    std::vector<int64_t> m1, m2, v1, v2;
+    for(int b = 0; b < nblk; ++b) {
+        bool const big = n[b] >= tmin;
+        (big ? n2 : n1).push_back(n[b]);
+        (big ? m2 : m1).push_back(mat_off[b]);
+        (big ? v2 : v1).push_back(val_off[b]);
+    }
+    BigPlan* P = nullptr;
+    int rc = plan_build(ctx, int(n1.size()), n1.data(), m1.data(), v1.data(), false, false, &P);
+    if(rc)
+        return rc;
+    if(!n2.empty() && (rc = plan_build(ctx, int(n2.size()), n2.data(), m2.data(), v2.data(), false, true, &P->next))) {
+        big_plan_destroy(P);
+        return rc;
+    }
+    *plan = P;
+    return POMX_OK;
 }
 
 int big_plan_enqueue(pomx_ctx* ctx, BigPlan* plan, double* d_mat, double* d_eval) {
-    return plan ? plan_run(ctx, plan, d_mat, d_eval) : POMX_OK;
+    for(BigPlan* p = plan; p; p = p->next) {
+        int rc = plan_run(ctx, p, d_mat, d_eval);
+        if(rc)
+            return rc;
+    }
+    return POMX_OK;
 }
 
 int big_plan_status(pomx_ctx* ctx, BigPlan* P) {
-    if(!P || P->nleaves == 0)
+    if(!P)
         return POMX_OK;
-    std::vector<int> st(size_t(P->nleaves));
-    POMX_HIP(ctx, hipMemcpy(st.data(), P->status, sizeof(int) * st.size(), hipMemcpyDeviceToHost));
-    for(int v : st)
-        if(v)
-            return pomx_fail(ctx, POMX_ERR_RANGE, "large-block eigensolver: a leaf eigensolve did not converge");
-    return POMX_OK;
+    if(P->nleaves > 0) {
+        std::vector<int> st(size_t(P->nleaves));
+        POMX_HIP(ctx, hipMemcpy(st.data(), P->status, sizeof(int) * st.size(), hipMemcpyDeviceToHost));
+        for(int v : st)
+            if(v)
+                return pomx_fail(ctx, POMX_ERR_RANGE, "large-block eigensolver: a leaf eigensolve did not converge");
+    }
+    return big_plan_status(ctx, P->next);
 }
 
 extern "C" int pomx_eigh_tridiag(pomx_ctx* ctx, int n, const double* h_d, const double* h_e, double* d_Z, double* h_w) {
@@ -1368,7 +2082,7 @@ extern "C" int pomx_eigh_tridiag(pomx_ctx* ctx, int n, const double* h_d, const 
     }
     int64_t zero = 0;
     BigPlan* P = nullptr;
-    int rc = plan_build(ctx, 1, &n, &zero, &zero, true, &P);
+    int rc = plan_build(ctx, 1, &n, &zero, &zero, true, false, &P);
     if(rc)
         return rc;
     BigBlk const& B = P->h_blk[0];

@@ -241,10 +241,14 @@ int launch_gemm_jobs(pomx_ctx* ctx, int trans_a, int trans_b, const pomx_gemm_jo
                      const double* d_A, const double* d_B, double* d_C, bool upper = false);
 
 // Device-driven symmetric eigensolver for a fixed set of blocks, all solved together: blocked
-// Householder tridiagonalisation with MFMA trailing updates, divide and conquer on the
-// tridiagonal matrices (deflation on the device), blocked (WY) MFMA back-transformation.  The
-// launch sequence depends only on the block sizes, so it can be captured in a hipGraph.
+// Householder tridiagonalisation with MFMA trailing updates (blocks of at least 2048 rows: a
+// two-stage reduction, dense to band by MFMA panel products, then a bulge chase to tridiagonal
+// form), divide and conquer on the tridiagonal matrices (deflation on the device), blocked (WY)
+// MFMA back-transformation.  The launch sequence depends only on the block sizes, so it can be
+// captured in a hipGraph.
 struct BigPlan;
+// Device scratch of one block of size n in a plan (bytes, about).
+size_t big_block_bytes(int n);
 // Blocks: size n[b], row-major matrix at d_mat + mat_off[b] (overwritten by the eigenvectors as
 // columns), eigenvalues ascending to d_eval + val_off[b].
 int big_plan_create(pomx_ctx* ctx, int nblk, const int* n, const int64_t* mat_off, const int64_t* val_off, BigPlan** plan);

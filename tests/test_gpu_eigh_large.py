@@ -84,14 +84,68 @@ def test_dense_large_blocks(cabi, degenerate):
         _check(A, w, V, 1e-11)
 
 
-def test_dense_hamiltonian_block(cabi, pa):
-    """The largest block of a 6-bath Anderson model (1225 states) through the ED path."""
+@pytest.mark.parametrize("two_stage", [False, True])
+def test_dense_hamiltonian_block(cabi, pa, monkeypatch, two_stage):
+    """The largest block of a 6-bath Anderson model (1225 states) through the ED path, with the
+    one-stage and (forced) two-stage reductions."""
     from pomerol_amd import workloads as W
+
+    if two_stage:
+        monkeypatch.setenv("POMX_EIGH_TWO_STAGE_MIN", "1000")
 
     cfg = W.AndersonConfig(6, 2.5, 10.0, seed=W.SEED + 77)
     e = pa.ED(W.terms(cfg.expr()), cfg.beta, True)
     sizes = [len(b) for b in e.blocks()]
     b = int(np.argmax(sizes))
-    print(f"[eigh-large] Anderson 1+6 bath, block {b} of size {sizes[b]}", flush=True)
+    print(f"[eigh-large] Anderson 1+6 bath, block {b} of size {sizes[b]}, two-stage {two_stage}", flush=True)
     H = e.hmat(b)
     _check(H, e.evals(b), e.evecs(b), 1e-11)
+
+
+def _sparse_integer(rng, n):
+    """Sparse symmetric matrix with small integer entries: rank-deficient panels (reflectors with
+    nothing to annihilate) and exactly degenerate eigenvalues, as in Hamiltonian blocks."""
+    A = np.zeros((n, n))
+    nz = rng.random((n, n)) < 4.0 / n
+    A[nz] = rng.integers(-2, 3, int(nz.sum()))
+    A = np.triu(A)
+    A = A + np.triu(A, 1).T
+    A[np.diag_indices(n)] = rng.integers(-3, 4, n)
+    return A
+
+
+@pytest.mark.parametrize("kind", ["random", "degenerate", "sparse"])
+def test_two_stage_reduction(cabi, monkeypatch, kind):
+    """The two-stage path (dense -> band by panel LQ and MFMA products, bulge chase to tridiagonal,
+    Q2 and Q1 back-transformations) forced on blocks of at least 100 rows: sizes just above the
+    band width, partial last panels, and blocks of different sizes in one plan."""
+    monkeypatch.setenv("POMX_EIGH_TWO_STAGE_MIN", "100")
+    rng = np.random.default_rng({"random": 31, "degenerate": 32, "sparse": 33}[kind])
+    sizes = [513, 700, 31, 1500, 200, 130, 161]
+    if kind == "sparse":
+        mats = [_sparse_integer(rng, n) for n in sizes]
+    else:
+        mats = [sym_with_spectrum(rng, n, kind == "degenerate") for n in sizes]
+    print(f"[eigh-large] two-stage {kind} sizes {sizes}", flush=True)
+    for A, (w, V) in zip(mats, _dense(cabi, mats)):
+        _check(A, w, V, 1e-11)
+
+
+def test_two_stage_and_one_stage_in_one_solve(cabi, monkeypatch):
+    """Blocks above the two-stage threshold and blocks below it in one call (two chained plans)."""
+    monkeypatch.setenv("POMX_EIGH_TWO_STAGE_MIN", "600")
+    rng = np.random.default_rng(41)
+    sizes = [513, 700, 1500, 200, 64]
+    mats = [sym_with_spectrum(rng, n, False) for n in sizes]
+    print(f"[eigh-large] mixed one-/two-stage sizes {sizes}", flush=True)
+    for A, (w, V) in zip(mats, _dense(cabi, mats)):
+        _check(A, w, V, 1e-11)
+
+
+def test_two_stage_default_threshold(cabi):
+    """A 2600-row block takes the two-stage path with the default threshold (2048)."""
+    rng = np.random.default_rng(43)
+    A = sym_with_spectrum(rng, 2600, False)
+    print("[eigh-large] two-stage by default, n=2600", flush=True)
+    (w, V), = _dense(cabi, [A])
+    _check(A, w, V, 1e-11)
