@@ -30,6 +30,8 @@
 
 namespace {
 
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
 constexpr int kNB = 32;   // reflectors per panel
 constexpr int kLeaf = 64; // largest divide-and-conquer leaf
 constexpr int kMaxSymvGrid = 1024;
@@ -50,8 +52,9 @@ constexpr int kBand = kNB;         // semi-bandwidth of the intermediate band ma
 constexpr int kLQT = 256;          // panel columns per workgroup of the panel LQ kernel
 constexpr int kGramW = 128;        // columns per workgroup of the 32 x 32 Gram partial sums
 static_assert(kLQT >= 7 * 33 && kLQT % 64 == 0, "the LQ kernel finishes 33 partial sums with 7 threads each");
-constexpr int kQ2G = 32;           // bulge-chasing sweeps per group of the Q2 back-transformation
-constexpr int kQ2Waves = 12;       // waves (groups in flight) per workgroup of the Q2 kernel
+constexpr int kXvSplit = 4;        // row ranges of the stage-1 product X^T = V^T A22 (partial sums)
+constexpr int kQ2G = 48;           // bulge-chasing sweeps per group of the Q2 back-transformation
+constexpr int kQ2Waves = 8;        // waves (groups in flight) per workgroup of the Q2 kernel
 constexpr int kTwoStageMin = 2048; // blocks at least this large take the two-stage reduction
 
 // Two-stage per-block scratch (doubles from xbase).  AB: the band in lower band storage,
@@ -1009,38 +1012,13 @@ __global__ void __launch_bounds__(kBT) wy_t_kernel(const BigBlk* __restrict__ bl
 // Householder reflectors (an LQ factorisation, one launch per reflector), Q = I - V T V^T, and the
 // trailing matrix becomes Q^T A22 Q = A22 - V Z^T - Z V^T with W = A22 V T and
 // Z = W - V (T^T V^T W) / 2: products of kBand-wide panels on the matrix cores instead of one
-// matrix-vector product per reflector.  A22 is kept in full storage (both triangles).
+// matrix-vector product per reflector.  Only the upper triangle of A22 is read and updated.
 // Stage 2 chases the band down to tridiagonal form: sweep s annihilates column s below the
 // subdiagonal with a reflector of length kBand and chases the bulge this creates down the band,
 // one step per kBand rows.  A persistent kernel runs many sweeps at once.
 // Eigenvectors: X = Q1 Q2 Z; Q2 (the bulge-chasing reflectors) by groups of sweeps on the vector
 // units with the eigenvector columns across the lanes, Q1 by the blocked WY back-transformation
 // of the one-stage path with the reflectors shifted by kBand.
-
-// Lower triangle of each block overwritten with its upper triangle (tiles of 64 through LDS).
-__global__ void __launch_bounds__(256) sym_lower_kernel(const BigBlk* __restrict__ blks, double* __restrict__ Aar) {
-    __shared__ double t[64][65];
-    BigBlk const B = blks[blockIdx.y];
-    int const n = B.n, nt = (n + 63) / 64;
-    double* __restrict__ a = Aar + B.mat_off;
-    for(int q = blockIdx.x; q < nt * nt; q += gridDim.x) {
-        int const ti = q / nt, tj = q % nt; // destination tile, on or below the diagonal
-        if(ti < tj)
-            continue; // uniform over the workgroup
-        __syncthreads();
-        for(int e = threadIdx.x; e < 64 * 64; e += 256) {
-            int const r = tj * 64 + e / 64, c = ti * 64 + e % 64; // source: upper tile (tj, ti)
-            t[e / 64][e % 64] = (r < n && c < n) ? a[int64_t(r) * n + c] : 0.0;
-        }
-        __syncthreads();
-        for(int e = threadIdx.x; e < 64 * 64; e += 256) {
-            int const rr = e / 64, cc = e % 64;
-            int const r = ti * 64 + rr, c = tj * 64 + cc;
-            if(r < n && c < n && r > c)
-                a[int64_t(r) * n + c] = t[cc][rr];
-        }
-    }
-}
 
 // Stage 1, panel k (P[r][c] = A[k + r][k + kBand + c], c < m = n - k - kBand), launch i
 // (0 <= i <= nref, nref = min(kBand, m - 1): the reflectors with at least two entries).  Launch i
@@ -1072,10 +1050,15 @@ __global__ void __launch_bounds__(kLQT) lq_kernel(const BigBlk* __restrict__ blk
     bool const live = c < m;
     int const gc = k + kBand + c; // column in the block
     double* __restrict__ P = Aar + B.mat_off + int64_t(k) * n + (k + kBand);
-    double p[kBand];
+    double p[kBand]; // all rows of the column (finished rows are stored back unchanged)
 #pragma unroll
     for(int r = 0; r < kBand; ++r)
-        p[r] = (live && r >= i - 1) ? P[int64_t(r) * n + c] : 0.0;
+        p[r] = 0.0;
+    if(live) {
+#pragma unroll
+        for(int r = 0; r < kBand; ++r)
+            p[r] = P[int64_t(r) * n + c];
+    }
     if(i == 0 && live)
         for(int q = nref; q < kBand; ++q) // rows of Vt beyond the panel's reflectors: zero
             base[o.Vt + int64_t(q) * n + gc] = 0.0;
@@ -1111,18 +1094,23 @@ __global__ void __launch_bounds__(kLQT) lq_kernel(const BigBlk* __restrict__ blk
             tau = (beta - alpha) / beta;
             scale = 1.0 / (alpha - beta);
         }
-        double v = 0.0; // v_q[c]: 1 at c = q, the scaled tail beyond, 0 before
-        if(live)
-            v = (c == q) ? 1.0 : (c > q ? p[q] * scale : 0.0);
+        // (rows are selected with compile-time indices: p stays in registers)
+        double pq = 0.0;
 #pragma unroll
         for(int r = 0; r < kBand; ++r)
+            pq = (r == q) ? p[r] : pq;
+        double v = 0.0; // v_q[c]: 1 at c = q, the scaled tail beyond, 0 before
+        if(live)
+            v = (c == q) ? 1.0 : (c > q ? pq * scale : 0.0);
+        double const newq = (c == q) ? beta : (c > q ? 0.0 : pq);
+#pragma unroll
+        for(int r = 0; r < kBand; ++r) {
             if(r > q)
                 p[r] = fma(-tau * fma(scale, fin[r], colv[r]), v, p[r]);
+            else if(r == q)
+                p[r] = newq;
+        }
         if(live) {
-            if(c == q)
-                p[q] = beta;
-            else if(c > q)
-                p[q] = 0.0;
             base[o.Vt + int64_t(q) * n + gc] = v;
             if(c >= q)
                 base[o.V + int64_t(gc) * n + (k + q)] = v;
@@ -1133,7 +1121,10 @@ __global__ void __launch_bounds__(kLQT) lq_kernel(const BigBlk* __restrict__ blk
     if(i < nref) {
         double* __restrict__ dst = lq + (i & 1) * par;
         bool const tail = live && c > i;
-        double const pi = tail ? p[i] : 0.0;
+        double pi = 0.0;
+#pragma unroll
+        for(int r = 0; r < kBand; ++r)
+            pi = (r == i && tail) ? p[r] : pi;
 #pragma unroll
         for(int r = 0; r < kBand; ++r) {
             double const s = wave_sum(r > i ? p[r] * pi : 0.0);
@@ -1156,8 +1147,7 @@ __global__ void __launch_bounds__(kLQT) lq_kernel(const BigBlk* __restrict__ blk
     if(i >= 1 && live) {
 #pragma unroll
         for(int r = 0; r < kBand; ++r)
-            if(r >= i - 1)
-                P[int64_t(r) * n + c] = p[r];
+            P[int64_t(r) * n + c] = p[r];
     }
 }
 
@@ -1192,6 +1182,99 @@ __global__ void __launch_bounds__(256) gram_kernel(const BigBlk* __restrict__ bl
     }
 }
 
+// Stage 1: X^T = Vt A22 (kBand x m) from the upper triangle of A22 only, A(r, c) =
+// A22[min(r, c)][max(r, c)].  Workgroup (column tile c0 of 64, block, split z) sums the rows r of
+// one of kXvSplit contiguous ranges, KC at a time: rows above the tile read A22[r][c0..c0+63]
+// (row segments), rows below it A22[c0+j][r..r+KC) (the transposed element, also contiguous), and
+// the chunks crossing the diagonal element by element.  The product runs on the matrix cores
+// (v_mfma_f64_16x16x4f64, each wave 16 x 32 of the 32 x 64 tile), the next chunk prefetched
+// into registers.  Split z writes its partial sum to rows [32 z, 32 z + 32) of W1; the next
+// product sums the splits (its K runs over all 4 x 32 rows).
+__global__ void __launch_bounds__(256) xv_kernel(const BigBlk* __restrict__ blks, int k, const double* __restrict__ Aar,
+                                                 double* __restrict__ D) {
+    constexpr int KC = 32;
+    __shared__ double Ts[KC][64 + 1];
+    __shared__ double Vs[kBand][KC + 1];
+    BigBlk const B = blks[blockIdx.y];
+    int const n = B.n, m = n - k - kBand, k1 = k + kBand;
+    int const c0 = blockIdx.x * 64;
+    if(m < 2 || c0 >= m)
+        return; // uniform over the workgroup
+    DOff const o = doff(n, B.gsym);
+    const double* __restrict__ A = Aar + B.mat_off + int64_t(k1) * n + k1;
+    const double* __restrict__ Vt = D + B.dbase + o.Vt + k1;
+    double* __restrict__ out = D + B.dbase + o.W1 + int64_t(blockIdx.z) * kBand * n + k1;
+    int const nchunk = (m + KC - 1) / KC;
+    int const per = (nchunk + kXvSplit - 1) / kXvSplit;
+    int const ch0 = blockIdx.z * per, ch1 = min(nchunk, ch0 + per);
+    int const tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int const wm = (wave >> 1) * 16, wn = (wave & 1) * 32;
+    dbl4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+    double ra[8], rv[4];
+    // region of chunk ch: 0 rows above the tile, 1 rows below it, 2 across the diagonal
+    auto region = [&](int ch) {
+        int const r0 = ch * KC;
+        return (r0 + KC <= c0) ? 0 : (r0 >= c0 + 64 ? 1 : 2);
+    };
+    auto load = [&](int ch) {
+        int const r0 = ch * KC, reg = region(ch);
+#pragma unroll
+        for(int u = 0; u < 8; ++u) {
+            if(reg == 1) { // (kk fastest: row c0+j of A22, columns r0..)
+                int const kk = tid & 31, j = (tid >> 5) + 8 * u, r = r0 + kk, c = c0 + j;
+                ra[u] = (r < m && c < m) ? A[int64_t(c) * n + r] : 0.0;
+            } else { // (j fastest)
+                int const kk = (tid >> 6) + 4 * u, j = tid & 63, r = r0 + kk, c = c0 + j;
+                ra[u] = (r < m && c < m) ? A[int64_t(min(r, c)) * n + max(r, c)] : 0.0;
+            }
+        }
+#pragma unroll
+        for(int u = 0; u < 4; ++u) {
+            int const kk = tid & 31, p = (tid >> 5) + 8 * u, r = r0 + kk;
+            rv[u] = (r < m) ? Vt[int64_t(p) * n + r] : 0.0;
+        }
+    };
+    auto store = [&](int ch) {
+        int const reg = region(ch);
+#pragma unroll
+        for(int u = 0; u < 8; ++u) {
+            if(reg == 1)
+                Ts[tid & 31][(tid >> 5) + 8 * u] = ra[u];
+            else
+                Ts[(tid >> 6) + 4 * u][tid & 63] = ra[u];
+        }
+#pragma unroll
+        for(int u = 0; u < 4; ++u)
+            Vs[(tid >> 5) + 8 * u][tid & 31] = rv[u];
+    };
+    if(ch0 < ch1)
+        load(ch0);
+    for(int ch = ch0; ch < ch1; ++ch) {
+        __syncthreads(); // the previous chunk's readers are done
+        store(ch);
+        __syncthreads();
+        if(ch + 1 < ch1)
+            load(ch + 1);
+#pragma unroll
+        for(int kq = 0; kq < KC / 4; ++kq) {
+            int const kr = kq * 4 + (lane >> 4);
+            double const a = Vs[wm + (lane & 15)][kr];
+            double const b0 = Ts[kr][wn + (lane & 15)], b1 = Ts[kr][wn + 16 + (lane & 15)];
+            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc1, 0, 0, 0);
+        }
+    }
+    // D[i][j] of v_mfma_f64_16x16x4f64 in lane j + 16 (i % 4), register i / 4
+#pragma unroll
+    for(int r = 0; r < 4; ++r) {
+        int const p = wm + (lane >> 4) + 4 * r, c = c0 + wn + (lane & 15);
+        if(c < m)
+            out[int64_t(p) * n + c] = acc0[r];
+        if(c + 16 < m)
+            out[int64_t(p) * n + c + 16] = acc1[r];
+    }
+}
+
 // One workgroup per block: the Gram partial sums finished (fixed order), then which 0: T of
 // Q = I - V T V^T = H_0 ... H_{kBand-1} (forward, upper; T[:i, i] = -tau_i T[:i, :i] S[:i, i],
 // T_ii = tau_i) to o.T; which 1: N = T^T M (M = V^T W) to o.S.  Both kBand x kBand, leading
@@ -1209,10 +1292,17 @@ __global__ void __launch_bounds__(256) panel_t_kernel(const BigBlk* __restrict__
     const double* __restrict__ gp = D + B.xbase + x.gp;
     int const tid = threadIdx.x;
     for(int e = tid; e < kBand * kBand; e += 256) {
-        double s = 0.0;
-        for(int b = 0; b < nchg; ++b)
-            s += gp[int64_t(b) * kBand * kBand + e];
-        S[e / kBand][e % kBand] = s;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0; // four chains: loads in flight together
+        int b = 0;
+        for(; b + 3 < nchg; b += 4) {
+            s0 += gp[int64_t(b) * kBand * kBand + e];
+            s1 += gp[int64_t(b + 1) * kBand * kBand + e];
+            s2 += gp[int64_t(b + 2) * kBand * kBand + e];
+            s3 += gp[int64_t(b + 3) * kBand * kBand + e];
+        }
+        for(; b < nchg; ++b)
+            s0 += gp[int64_t(b) * kBand * kBand + e];
+        S[e / kBand][e % kBand] = (s0 + s1) + (s2 + s3);
         T[e / kBand][e % kBand] = which ? base[o.T + e] : 0.0;
     }
     __syncthreads();
@@ -1234,6 +1324,12 @@ __global__ void __launch_bounds__(256) panel_t_kernel(const BigBlk* __restrict__
         __syncthreads();
         for(int e = tid; e < kBand * kBand; e += 256)
             base[o.T + e] = T[e / kBand][e % kBand];
+        // T4 = [T^T T^T T^T T^T] (kBand x kXvSplit kBand) to o.S: W^T = T4 [X_0; ..; X_3] sums the
+        // split partial products of X^T
+        for(int e = tid; e < kBand * kXvSplit * kBand; e += 256) {
+            int const p = e / (kXvSplit * kBand), q = e % kBand;
+            base[o.S + e] = T[q][p];
+        }
     } else {
         for(int e = tid; e < kBand * kBand; e += 256) {
             int const a = e / kBand, c = e % kBand;
@@ -1323,8 +1419,14 @@ __global__ void __launch_bounds__(256) bulge_step_kernel(const BigBlk* __restric
     int const L2 = min(b, n - r0 - b);  // rows of W_{k+1}
     int const c = (k == 0) ? s : r0 - b; // annihilated column = first footprint column
     int const ncol = r0 + L - c;
-    for(int e = tid; e < ncol * LD; e += 256)
-        raw[e / LD][e % LD] = AB[int64_t(c) * LD + e];
+    int const nel = ncol * LD;
+#pragma unroll
+    for(int u = 0; u < (2 * kBand * 2 * kBand) / 256; ++u) { // fixed trip count: loads in flight together
+        int const e = tid + 256 * u;
+        double const a = AB[int64_t(c) * LD + min(e, nel - 1)];
+        if(e < nel)
+            raw[e / LD][e % LD] = a;
+    }
     __syncthreads();
     // A[i][j] (i >= j) = raw[j - c][i - j]
     if(wave == 0) {
@@ -1415,8 +1517,12 @@ __global__ void __launch_bounds__(256) bulge_step_kernel(const BigBlk* __restric
         }
     }
     __syncthreads();
-    for(int e = tid; e < ncol * LD; e += 256)
-        AB[int64_t(c) * LD + e] = raw[e / LD][e % LD];
+#pragma unroll
+    for(int u = 0; u < (2 * kBand * 2 * kBand) / 256; ++u) {
+        int const e = tid + 256 * u;
+        if(e < nel)
+            AB[int64_t(c) * LD + e] = raw[e / LD][e % LD];
+    }
 }
 
 // d and e of the tridiagonal matrix left by the bulge chase.
@@ -1473,11 +1579,15 @@ __global__ void __launch_bounds__(64 * NW) q2_kernel(const BigBlk* __restrict__ 
             int const k = t - wave;
             if(k >= 0 && k < kc) {
                 int const rb = S + 1 + k * b; // first row of the window
+                // rows beyond the block read as zero (a clamped row, then a select: the loads
+                // issue back to back under one exec mask)
+                int const ccol = live ? col : n - 1;
                 if(k == 0) {
 #pragma unroll
                     for(int i = 0; i < WIN; ++i) {
                         int const row = rb + i;
-                        win[i] = (live && row < n) ? Z[int64_t(row) * n + col] : 0.0;
+                        double const z = Z[int64_t(min(row, n - 1)) * n + ccol];
+                        win[i] = (live && row < n) ? z : 0.0;
                     }
                 } else {
 #pragma unroll
@@ -1486,7 +1596,8 @@ __global__ void __launch_bounds__(64 * NW) q2_kernel(const BigBlk* __restrict__ 
 #pragma unroll
                     for(int i = G - 1; i < WIN; ++i) {
                         int const row = rb + i;
-                        win[i] = (live && row < n) ? Z[int64_t(row) * n + col] : 0.0;
+                        double const z = Z[int64_t(min(row, n - 1)) * n + ccol];
+                        win[i] = (live && row < n) ? z : 0.0;
                     }
                 }
 #pragma unroll
@@ -1677,7 +1788,8 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
     std::vector<pomx_gemm_job> gj;
     if(P->two_stage) {
         // stage-1 panels: per active block (m = n - k - kBand >= 2; a prefix, blocks largest first)
-        // the jobs X = Vt A22, W = T^T X, W -= N^T Vt / 2, A22 -= [W Vt]^T [Vt W3]
+        // the jobs W^T = T4 [X_0; ..; X_3] (the split partial products), W -= N^T Vt / 2, and
+        // A22 -= [W Vt]^T [Vt W3] on the upper tiles
         for(int k = 0; P->nmax - k - kBand >= 2; k += kBand) {
             BigPlan::Panel2 Q{k, 0, 0, 0, 0, P->nmax - k - kBand, int64_t(gj.size())};
             while(Q.nact < nblk && P->h_blk[size_t(Q.nact)].n - k - kBand >= 2)
@@ -1685,18 +1797,15 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
             Q.nref_max = std::min(kBand, Q.mmax - 1);
             Q.nchunk = (Q.mmax + kLQT - 1) / kLQT;
             Q.ngram = (Q.mmax + kGramW - 1) / kGramW;
-            for(int kind = 0; kind < 4; ++kind)
+            for(int kind = 1; kind < 4; ++kind)
                 for(int b = 0; b < Q.nact; ++b) {
                     BigBlk const& B = P->h_blk[size_t(b)];
                     int const n = B.n, m = n - k - kBand, k1 = k + kBand;
                     DOff const o = doff(n, B.gsym);
                     int64_t const a22 = B.mat_off + int64_t(k1) * n + k1;
-                    if(kind == 0)
-                        gj.push_back(pomx_gemm_job{kBand, m, m, 0, 0, n, n, n, B.dbase + o.Vt + k1, a22, B.dbase + o.W1 + k1,
-                                                   1.0, 0.0});
-                    else if(kind == 1)
-                        gj.push_back(pomx_gemm_job{kBand, m, kBand, 1, 0, kBand, n, n, B.dbase + o.T, B.dbase + o.W1 + k1,
-                                                   B.dbase + o.W + k1, 1.0, 0.0});
+                    if(kind == 1)
+                        gj.push_back(pomx_gemm_job{kBand, m, kXvSplit * kBand, 0, 0, kXvSplit * kBand, n, n, B.dbase + o.S,
+                                                   B.dbase + o.W1 + k1, B.dbase + o.W + k1, 1.0, 0.0});
                     else if(kind == 2)
                         gj.push_back(pomx_gemm_job{kBand, m, kBand, 1, 0, kBand, n, n, B.dbase + o.S, B.dbase + o.Vt + k1,
                                                    B.dbase + o.W + k1, -0.5, 1.0});
@@ -1864,10 +1973,6 @@ int run_two_stage(pomx_ctx* ctx, BigPlan* P, double* Aar) {
     int rc;
     int const nb = P->nblk;
     fill_blocks_kernel<<<dim3(gx_rows(int64_t(P->nmax) * P->nmax, 256, nb), unsigned(nb)), 256, 0, s>>>(P->blk, P->D, 1, 0.0);
-    {
-        int64_t const nt = (P->nmax + 63) / 64;
-        sym_lower_kernel<<<dim3(gx_rows(nt * nt, 1, nb), unsigned(nb)), 256, 0, s>>>(P->blk, Aar);
-    }
     POMX_CHECK_LAUNCH(ctx);
     for(auto const& Q : P->p2) {
         unsigned const na = unsigned(Q.nact);
@@ -1878,17 +1983,18 @@ int run_two_stage(pomx_ctx* ctx, BigPlan* P, double* Aar) {
         POMX_CHECK_LAUNCH(ctx);
         int64_t const tc = (Q.mmax + 63) / 64;
         pomx_gemm_job const* J = P->gjobs + Q.jobs_off;
-        if((rc = launch_gemm_jobs(ctx, 0, 0, J, Q.nact, tc, P->D, Aar, P->D)) ||
-           (rc = launch_gemm_jobs(ctx, 1, 0, J + Q.nact, Q.nact, tc, P->D, P->D, P->D)))
+        xv_kernel<<<dim3(unsigned(tc), na, unsigned(kXvSplit)), 256, 0, s>>>(P->blk, Q.k, Aar, P->D);
+        POMX_CHECK_LAUNCH(ctx);
+        if((rc = launch_gemm_jobs(ctx, 0, 0, J, Q.nact, tc, P->D, P->D, P->D)))
             return rc;
         gram_kernel<<<dim3(unsigned(Q.ngram), na), 256, 0, s>>>(P->blk, Q.k, 1, P->D);
         panel_t_kernel<<<dim3(1, na), 256, 0, s>>>(P->blk, Q.k, 1, P->D);
         POMX_CHECK_LAUNCH(ctx);
-        if((rc = launch_gemm_jobs(ctx, 1, 0, J + 2 * Q.nact, Q.nact, tc, P->D, P->D, P->D)))
+        if((rc = launch_gemm_jobs(ctx, 1, 0, J + Q.nact, Q.nact, tc, P->D, P->D, P->D)))
             return rc;
         copy_w3_kernel<<<dim3(gx_rows(int64_t(kBand) * Q.mmax, 256, Q.nact), na), 256, 0, s>>>(P->blk, Q.k, P->D);
         POMX_CHECK_LAUNCH(ctx);
-        if((rc = launch_gemm_jobs(ctx, 1, 0, J + 3 * Q.nact, Q.nact, tc * tc, P->D, P->D, Aar)))
+        if((rc = launch_gemm_jobs(ctx, 1, 0, J + 2 * Q.nact, Q.nact, tc * tc, P->D, P->D, Aar, true)))
             return rc;
     }
     band_kernel<<<dim3(gx_rows(int64_t(P->nmax) * 2 * kBand, 256, nb), unsigned(nb)), 256, 0, s>>>(P->blk, Aar, P->D);
