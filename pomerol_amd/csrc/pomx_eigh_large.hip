@@ -53,8 +53,8 @@ constexpr int kLQT = 256;          // panel columns per workgroup of the panel L
 constexpr int kGramW = 128;        // columns per workgroup of the 32 x 32 Gram partial sums
 static_assert(kLQT >= 7 * 33 && kLQT % 64 == 0, "the LQ kernel finishes 33 partial sums with 7 threads each");
 constexpr int kXvSplit = 4;        // row ranges of the stage-1 product X^T = V^T A22 (partial sums)
-constexpr int kQ2G = 48;           // bulge-chasing sweeps per group of the Q2 back-transformation
-constexpr int kQ2Waves = 8;        // waves (groups in flight) per workgroup of the Q2 kernel
+constexpr int kQ2G = 32;           // bulge-chasing sweeps per group of the Q2 back-transformation
+constexpr int kQ2Waves = 12;       // waves (groups in flight) per workgroup of the Q2 kernel
 constexpr int kTwoStageMin = 2048; // blocks at least this large take the two-stage reduction
 
 // Two-stage per-block scratch (doubles from xbase).  AB: the band in lower band storage,
@@ -1391,8 +1391,7 @@ __host__ __device__ inline void sweep_range(int n, int t, int& lo, int& hi) {
 __global__ void __launch_bounds__(256) bulge_step_kernel(const BigBlk* __restrict__ blks, int nblk, int t, double* __restrict__ D) {
     constexpr int b = kBand, LD = 2 * kBand;
     __shared__ double raw[2 * kBand][2 * kBand]; // footprint: band column (from c) x offset d
-    __shared__ double vs[kBand], ws[kBand];
-    __shared__ double sc;
+    __shared__ double vsw[4][kBand], wsw[4][kBand]; // each wave's copies of v and w
     int const tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // this workgroup's task: blocks in order, sweeps ascending
     int idx = blockIdx.x, blk = -1, s = 0;
@@ -1428,82 +1427,77 @@ __global__ void __launch_bounds__(256) bulge_step_kernel(const BigBlk* __restric
             raw[e / LD][e % LD] = a;
     }
     __syncthreads();
-    // A[i][j] (i >= j) = raw[j - c][i - j]
-    if(wave == 0) {
-        double const xa = (lane < L) ? raw[0][r0 + lane - c] : 0.0;
-        double const alpha = __shfl(xa, 0, 64);
-        double const sig = wave_sum(lane >= 1 ? xa * xa : 0.0);
-        double tau = 0.0, beta = alpha, scale = 0.0;
-        if(sig > 0.0) {
-            beta = -copysign(sqrt(alpha * alpha + sig), alpha);
-            tau = (beta - alpha) / beta;
-            scale = 1.0 / (alpha - beta);
-        }
-        if(lane < b) {
-            double const v = (lane == 0) ? 1.0 : (lane < L ? xa * scale : 0.0);
-            vs[lane] = v;
-            V2[(int64_t(s) * x.kmax + k) * b + lane] = (lane == 0) ? tau : v;
-            if(lane < L)
-                raw[0][r0 + lane - c] = (lane == 0) ? beta : 0.0;
-        }
-        if(lane == 0)
-            sc = tau;
+    // A[i][j] (i >= j) = raw[j - c][i - j].  Phase 1, every wave on its own (the reflector and
+    // w = tau D v are formed redundantly in each wave, from its own LDS copies): the reflector,
+    // the left application to the bulge block, w and its correction.  Phase 2, after a barrier
+    // (every wave has read D and x): the updates of D and of the block below.
+    double* const vw = vsw[wave];
+    double* const ww = wsw[wave];
+    double const xa = (lane < L) ? raw[0][r0 + lane - c] : 0.0;
+    double const alpha = __shfl(xa, 0, 64);
+    double const sig = wave_sum(lane >= 1 ? xa * xa : 0.0);
+    double tau = 0.0, beta = alpha, scale = 0.0;
+    if(sig > 0.0) {
+        beta = -copysign(sqrt(alpha * alpha + sig), alpha);
+        tau = (beta - alpha) / beta;
+        scale = 1.0 / (alpha - beta);
     }
-    __syncthreads();
-    double const tau = sc;
+    if(lane < b) {
+        double const v = (lane == 0) ? 1.0 : (lane < L ? xa * scale : 0.0);
+        vw[lane] = v;
+        if(wave == 0)
+            V2[(int64_t(s) * x.kmax + k) * b + lane] = (lane == 0) ? tau : v;
+    }
+    // the wave's own LDS copies are read by other lanes of the same wave
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    int const c0 = r0 - c; // raw column of row/column r0
     if(tau != 0.0) {
         if(k >= 1) { // left: columns c+1 .. c+b-1 of the bulge block, rows W_k (half-wave each)
             int const hw = tid >> 5, a = tid & 31;
             for(int qq = 1 + hw; qq < b; qq += 8) {
                 double const e = (a < L) ? raw[qq][b + a - qq] : 0.0;
-                double dot = (a < L) ? e * vs[a] : 0.0;
+                double dot = (a < L) ? e * vw[a] : 0.0;
                 for(int o = 16; o > 0; o >>= 1)
                     dot += __shfl_xor(dot, o, 32);
                 if(a < L)
-                    raw[qq][b + a - qq] = fma(-tau * dot, vs[a], e);
+                    raw[qq][b + a - qq] = fma(-tau * dot, vw[a], e);
             }
-            __syncthreads();
         }
-        int const c0 = r0 - c; // raw column of row/column r0
-        { // both sides on D = A[W_k, W_k]: w = tau D v (8 threads per row)
-            int const a = tid >> 3, part = tid & 7;
-            double acc = 0.0;
-            if(a < L)
-                for(int u = 0; u < 4; ++u) {
-                    int const qv = part * 4 + u;
-                    if(qv < L) {
-                        int const lo = min(a, qv), hi = max(a, qv);
-                        acc = fma(raw[c0 + lo][hi - lo], vs[qv], acc);
-                    }
+        // w = tau D v: lane pair (2a, 2a+1) takes row a, half of the columns each
+        int const a = lane >> 1, h = lane & 1;
+        double acc = 0.0;
+        if(a < L)
+            for(int u = 0; u < 16; ++u) {
+                int const qv = h * 16 + u;
+                if(qv < L) {
+                    int const lo = min(a, qv), hi = max(a, qv);
+                    acc = fma(raw[c0 + lo][hi - lo], vw[qv], acc);
                 }
-            acc += __shfl_xor(acc, 1, 64);
-            acc += __shfl_xor(acc, 2, 64);
-            acc += __shfl_xor(acc, 4, 64);
-            if(part == 0)
-                ws[a] = (a < L) ? tau * acc : 0.0;
-        }
-        __syncthreads();
-        if(wave == 0) {
-            double const wv = (lane < L) ? ws[lane] * vs[lane] : 0.0;
-            double const al = -0.5 * tau * wave_sum(wv);
-            if(lane < L)
-                ws[lane] = fma(al, vs[lane], ws[lane]);
-        }
-        __syncthreads();
+            }
+        acc += __shfl_xor(acc, 1, 64);
+        double const wa = (a < L) ? tau * acc : 0.0;
+        double const al = -0.5 * tau * wave_sum((h == 0 && a < L) ? wa * vw[a] : 0.0);
+        if(h == 0)
+            ww[a] = fma(al, vw[a], wa);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads(); // every wave has read x and D
+    if(tau != 0.0) {
         for(int e = tid; e < b * b; e += 256) {
             int const a = e / b, qv = e % b;
             if(a < L && qv <= a)
-                raw[c0 + qv][a - qv] -= vs[a] * ws[qv] + ws[a] * vs[qv];
+                raw[c0 + qv][a - qv] -= vw[a] * ww[qv] + ww[a] * vw[qv];
         }
-        // right on A[W_{k+1}, W_k] (rows below W_k; disjoint from the entries above)
-        {
+        { // right on A[W_{k+1}, W_k] (rows below W_k; disjoint from the entries above)
             int const a2 = tid >> 3, part = tid & 7;
             double acc = 0.0;
             if(a2 < L2)
                 for(int u = 0; u < 4; ++u) {
                     int const qv = part * 4 + u;
                     if(qv < L)
-                        acc = fma(raw[c0 + qv][b + a2 - qv], vs[qv], acc);
+                        acc = fma(raw[c0 + qv][b + a2 - qv], vw[qv], acc);
                 }
             acc += __shfl_xor(acc, 1, 64);
             acc += __shfl_xor(acc, 2, 64);
@@ -1512,9 +1506,11 @@ __global__ void __launch_bounds__(256) bulge_step_kernel(const BigBlk* __restric
                 for(int u = 0; u < 4; ++u) {
                     int const qv = part * 4 + u;
                     if(qv < L)
-                        raw[c0 + qv][b + a2 - qv] = fma(-tau * acc, vs[qv], raw[c0 + qv][b + a2 - qv]);
+                        raw[c0 + qv][b + a2 - qv] = fma(-tau * acc, vw[qv], raw[c0 + qv][b + a2 - qv]);
                 }
         }
+        if(wave == 0 && lane < L) // the annihilated column
+            raw[0][r0 + lane - c] = (lane == 0) ? beta : 0.0;
     }
     __syncthreads();
 #pragma unroll
@@ -1545,9 +1541,9 @@ __global__ void band_de_kernel(const BigBlk* __restrict__ blks, double* __restri
 // of later steps of earlier sweeps lie below those of earlier steps of later sweeps, so they
 // commute and this order equals the reverse generation order.  One lane per column of Z: a step's
 // rows stay in registers and slide down by kBand rows per step, and the reflector entries are
-// wave-uniform.  The waves of a workgroup take consecutive groups, each a step behind the wave
-// before it (step k of group G-1 needs group G's steps up to k; group G's step k+1 and group
-// G-1's step k touch disjoint rows), one barrier per step.
+// wave-uniform (scalar loads).  The waves of a workgroup take consecutive groups, each a step
+// behind the wave before it (step k of group G-1 needs group G's steps up to k; group G's step
+// k+1 and group G-1's step k touch disjoint rows), one barrier per step.
 template <int NW>
 __global__ void __launch_bounds__(64 * NW) q2_kernel(const BigBlk* __restrict__ blks, const double* __restrict__ D,
                                                      double* __restrict__ Aar) {
@@ -1579,15 +1575,11 @@ __global__ void __launch_bounds__(64 * NW) q2_kernel(const BigBlk* __restrict__ 
             int const k = t - wave;
             if(k >= 0 && k < kc) {
                 int const rb = S + 1 + k * b; // first row of the window
-                // rows beyond the block read as zero (a clamped row, then a select: the loads
-                // issue back to back under one exec mask)
-                int const ccol = live ? col : n - 1;
                 if(k == 0) {
 #pragma unroll
                     for(int i = 0; i < WIN; ++i) {
                         int const row = rb + i;
-                        double const z = Z[int64_t(min(row, n - 1)) * n + ccol];
-                        win[i] = (live && row < n) ? z : 0.0;
+                        win[i] = (live && row < n) ? Z[int64_t(row) * n + col] : 0.0;
                     }
                 } else {
 #pragma unroll
@@ -1596,8 +1588,7 @@ __global__ void __launch_bounds__(64 * NW) q2_kernel(const BigBlk* __restrict__ 
 #pragma unroll
                     for(int i = G - 1; i < WIN; ++i) {
                         int const row = rb + i;
-                        double const z = Z[int64_t(min(row, n - 1)) * n + ccol];
-                        win[i] = (live && row < n) ? z : 0.0;
+                        win[i] = (live && row < n) ? Z[int64_t(row) * n + col] : 0.0;
                     }
                 }
 #pragma unroll
