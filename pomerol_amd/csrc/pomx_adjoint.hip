@@ -223,7 +223,7 @@ template <class T> int upload(pomx_ctx* ctx, T*& d, std::vector<T> const& h) {
     if(device_malloc(ctx, reinterpret_cast<void**>(&d), sizeof(T) * std::max<size_t>(h.size(), 1)) != hipSuccess)
         return pomx_fail(ctx, POMX_ERR_HIP, "pomx_batch_adjoint: out of device memory");
     if(!h.empty())
-        POMX_HIP(ctx, hipMemcpyAsync(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+        POMX_HIP(ctx, h2d_sync(ctx, d, h.data(), sizeof(T) * h.size()));
     return POMX_OK;
 }
 

@@ -364,7 +364,7 @@ extern "C" int pomx_eigh_batched(pomx_ctx* ctx, int nblocks, const int* h_n, con
     WsBuf<int> d_status(ctx, nj);
     if(!d_jobs.p || !d_ws.p || !d_status.p)
         return pomx_fail(ctx, POMX_ERR_HIP, "pomx_eigh_batched: out of device memory");
-    POMX_HIP(ctx, hipMemcpyAsync(d_jobs.p, jobs.data(), sizeof(EighJob) * nj, hipMemcpyHostToDevice, ctx->stream));
+    POMX_HIP(ctx, h2d_sync(ctx, d_jobs.p, jobs.data(), sizeof(EighJob) * nj));
     int rc = pomx_fill_zero(ctx, d_status.p, sizeof(int) * nj);
     if(rc)
         return rc;

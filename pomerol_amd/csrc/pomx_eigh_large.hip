@@ -1950,7 +1950,7 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
     alloc(reinterpret_cast<void**>(&P->leaf_arena), sizeof(double) * size_t(larena));
     alloc(reinterpret_cast<void**>(&P->leaf_ws), sizeof(double) * size_t(std::max<int64_t>(lws, 1)));
     alloc(reinterpret_cast<void**>(&P->status), sizeof(int) * size_t(std::max(P->nleaves, 1)));
-    if(!ok || hipStreamSynchronize(s) != hipSuccess) {
+    if(hipStreamSynchronize(s) != hipSuccess || !ok) { // (the uploads from host vectors have finished)
         big_plan_destroy(P);
         return pomx_fail(ctx, POMX_ERR_HIP, "large-block eigensolver: out of device memory for the plan");
     }
@@ -2189,7 +2189,7 @@ extern "C" int pomx_eigh_tridiag(pomx_ctx* ctx, int n, const double* h_d, const 
         e[size_t(q)] = h_e[q];
     hipStream_t s = ctx->stream;
     if(hipMemcpyAsync(P->D + o.d, h_d, sizeof(double) * size_t(n), hipMemcpyHostToDevice, s) != hipSuccess ||
-       hipMemcpyAsync(P->D + o.e, e.data(), sizeof(double) * size_t(n), hipMemcpyHostToDevice, s) != hipSuccess) {
+       h2d_sync(ctx, P->D + o.e, e.data(), sizeof(double) * size_t(n)) != hipSuccess) {
         big_plan_destroy(P);
         return pomx_fail(ctx, POMX_ERR_HIP, "pomx_eigh_tridiag: copy failed");
     }

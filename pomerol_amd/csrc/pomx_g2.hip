@@ -2489,7 +2489,7 @@ int launch_g2_eval(pomx_ctx* ctx, TermSpan const spans[4], int perm, double res,
             d_y = WsBuf<double>(ctx, y.size());
             if(!d_y.p)
                 return pomx_fail(ctx, POMX_ERR_HIP, "pomx_g2_part_compute: out of device memory");
-            POMX_HIP(ctx, hipMemcpyAsync(d_y.p, y.data(), sizeof(double) * y.size(), hipMemcpyHostToDevice, ctx->stream));
+            POMX_HIP(ctx, h2d_sync(ctx, d_y.p, y.data(), sizeof(double) * y.size()));
         }
     }
     // purely imaginary triplets whose aligned groups share z1 and z3 (F2 = z2 varies inside a

@@ -90,6 +90,12 @@ int pomx_sort_pairs(pomx_ctx* ctx, int64_t n, uint64_t* keys_in, uint64_t* keys_
                     uint64_t* vals_out, int begin_bit, int end_bit);
 // Exclusive prefix sum of int64 (out may alias in).
 int pomx_exclusive_scan(pomx_ctx* ctx, int64_t n, const int64_t* in, int64_t* out);
+// Host-to-device copy from a host buffer the caller may release as soon as this returns: the
+// copy has finished (context stream synchronised) on return.
+inline hipError_t h2d_sync(pomx_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    hipError_t const e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    return e == hipSuccess ? hipStreamSynchronize(ctx->stream) : e;
+}
 
 // Copy one int64 from the device (synchronises the context stream).
 int read_i64(pomx_ctx* ctx, const int64_t* d, int64_t* h);

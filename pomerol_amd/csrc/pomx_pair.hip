@@ -449,7 +449,7 @@ extern "C" int pomx_pair_terms(pomx_ctx* ctx, int kind, int nparts, const pomx_p
     WsBuf<PairPartDev> d_parts(ctx, nparts);
     WsBuf<int64_t> flag(ctx, ncand), pos(ctx, ncand);
     WsBuf<int> err(ctx, 1);
-    POMX_HIP(ctx, hipMemcpyAsync(d_parts.p, parts.data(), sizeof(PairPartDev) * nparts, hipMemcpyHostToDevice, ctx->stream));
+    POMX_HIP(ctx, h2d_sync(ctx, d_parts.p, parts.data(), sizeof(PairPartDev) * nparts));
     int rc = pomx_fill_zero(ctx, err.p, sizeof(int));
     if(rc)
         return rc;

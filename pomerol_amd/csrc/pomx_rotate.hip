@@ -107,7 +107,7 @@ extern "C" int pomx_rotate_batched(pomx_ctx* ctx, int njobs, const pomx_rot_job*
     if(!d_tiles.p || !d_jobs.p)
         return pomx_fail(ctx, POMX_ERR_HIP, "pomx_rotate_batched: out of device memory");
     POMX_HIP(ctx, hipMemcpyAsync(d_tiles.p, tiles.data(), sizeof(RotTile) * tiles.size(), hipMemcpyHostToDevice, ctx->stream));
-    POMX_HIP(ctx, hipMemcpyAsync(d_jobs.p, h_jobs, sizeof(pomx_rot_job) * njobs, hipMemcpyHostToDevice, ctx->stream));
+    POMX_HIP(ctx, h2d_sync(ctx, d_jobs.p, h_jobs, sizeof(pomx_rot_job) * njobs)); // (both copies done)
     int rc = launch_rotate(ctx, d_tiles, int(tiles.size()), d_jobs, d_U, d_src, d_out, cutoff);
     if(rc)
         return rc;

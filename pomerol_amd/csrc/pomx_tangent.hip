@@ -116,8 +116,7 @@ extern "C" int pomx_eigh_tangent(pomx_ctx* ctx, int nb, const int* h_n, const in
             return pomx_fail(ctx, POMX_ERR_HIP, "pomx_eigh_tangent: out of device memory");
         POMX_HIP(ctx, hipMemcpyAsync(d_tiles.p, tiles.data(), sizeof(GemmTile) * tiles.size(), hipMemcpyHostToDevice,
                                      ctx->stream));
-        POMX_HIP(ctx, hipMemcpyAsync(d_jobs.p, js.data(), sizeof(pomx_gemm_job) * js.size(), hipMemcpyHostToDevice,
-                                     ctx->stream));
+        POMX_HIP(ctx, h2d_sync(ctx, d_jobs.p, js.data(), sizeof(pomx_gemm_job) * js.size())); // (both copies done)
         int rc = pass == 0 ? launch_gemm(ctx, 0, 0, d_tiles, int(tiles.size()), d_jobs, d_dH, d_U, T)
                            : launch_gemm(ctx, 1, 0, d_tiles, int(tiles.size()), d_jobs, d_U, T, d_K);
         if(rc)
@@ -125,8 +124,7 @@ extern "C" int pomx_eigh_tangent(pomx_ctx* ctx, int nb, const int* h_n, const in
         // the pooled descriptors are released at the end of this scope: wait for their reader
         POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
-    POMX_HIP(ctx, hipMemcpyAsync(d_blocks.p, blocks.data(), sizeof(EighJob) * blocks.size(), hipMemcpyHostToDevice,
-                                 ctx->stream));
+    POMX_HIP(ctx, h2d_sync(ctx, d_blocks.p, blocks.data(), sizeof(EighJob) * blocks.size()));
     tangent_k_kernel<<<nb, kBlock, 0, ctx->stream>>>(d_blocks, d_E, d_K, d_dE);
     POMX_CHECK_LAUNCH(ctx);
     POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
