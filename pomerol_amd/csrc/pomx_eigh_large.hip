@@ -55,6 +55,8 @@ static_assert(kLQT >= 7 * 33 && kLQT % 64 == 0, "the LQ kernel finishes 33 parti
 constexpr int kXvSplit = 4;        // row ranges of the stage-1 product X^T = V^T A22 (partial sums)
 constexpr int kQ2G = 32;           // bulge-chasing sweeps per group of the Q2 back-transformation
 constexpr int kQ2Waves = 12;       // waves (groups in flight) per workgroup of the Q2 kernel
+constexpr int kBtSplitS = 16;      // two-stage back-transformation: row ranges of S = V^T V
+constexpr int kBtSplitW = 4;       // and of W1 = V^T X (partial sums in the free D&C workspace U)
 constexpr int kTwoStageMin = 2048; // blocks at least this large take the two-stage reduction
 
 // Two-stage per-block scratch (doubles from xbase).  AB: the band in lower band storage,
@@ -964,6 +966,27 @@ __global__ void copy_z_kernel(const BigBlk* __restrict__ blks, const double* __r
         Aar[B.mat_off + q] = D[B.dbase + od.Z + q];
 }
 
+// Partial products of the split back-transformation summed in a fixed order, one thread per entry,
+// blocks listed in ids: which 0, S = sum of the ks products V_z^T V_z at U + z kBT^2 (row ranges z
+// of V); which 1, W1 = sum of the ks products V_z^T X_z at U + ks_S kBT^2 + z nb n.
+__global__ void bt_sum_kernel(const BigBlk* __restrict__ blks, const int* __restrict__ ids, int j0, int shift, int which,
+                              int ks, double* __restrict__ D) {
+    BigBlk const B = blks[ids[blockIdx.y]];
+    DOff const od = doff(B.n, B.gsym);
+    double* __restrict__ base = D + B.dbase;
+    int const n = B.n, nb = min(kBT, n - shift - j0);
+    int64_t const cnt = which == 0 ? int64_t(nb) * nb : int64_t(nb) * n;
+    int64_t const src = which == 0 ? od.U : od.U + int64_t(kBtSplitS) * kBT * kBT;
+    int64_t const stride = which == 0 ? int64_t(kBT) * kBT : int64_t(nb) * n;
+    double* __restrict__ dst = base + (which == 0 ? od.S : od.W1);
+    for(int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < cnt; e += int64_t(gridDim.x) * blockDim.x) {
+        double a = 0.0;
+        for(int z = 0; z < ks; ++z)
+            a += base[src + z * stride + e];
+        dst[e] = a;
+    }
+}
+
 // T (nb x nb, upper, nb <= kBT) of I - V T V^T = H_j0 ... H_{j0+nb-1} from S = V^T V and tau, one
 // workgroup of kBT threads per block: column i of T is -tau_i T[:i, :i] S[:i, i], T_ii = tau_i.
 // Thread r owns row r of T and builds it in place in the block's scratch (it only reads back
@@ -1705,8 +1728,8 @@ struct BigPlan {
     DcNode* nodes = nullptr;
     DcState* states = nullptr;
     pomx_gemm_job* dc_jobs = nullptr;
-    struct BtPanel { // back-transformation panel: 4 jobs per block (S, W1, W2, X)
-        int j0, nblk, nb_max, rows_max;
+    struct BtPanel { // back-transformation panel: jobs S (ks per block), W1 (kw per block), W2, X
+        int j0, nblk, nb_max, rows_max, ks, kw;
         int64_t ids_off, jobs_off;
     };
     std::vector<BtPanel> bt;
@@ -1903,28 +1926,47 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
                     L.push_back(b);
             if(L.empty())
                 continue;
-            BigPlan::BtPanel T{j0, int(L.size()), 0, 0, int64_t(ids.size()), int64_t(gj.size())};
+            // two-stage plans (blocks of 2048 rows and more, few of them) split the long-K products
+            // S = V^T V and W1 = V^T X over row ranges into partial sums in U (free after the
+            // divide and conquer), so that they run on many workgroups
+            // (when every block of the panel has room for the partial sums in U: n^2 doubles)
+            int nmin = P->nmax;
+            for(int b : L)
+                nmin = std::min(nmin, P->h_blk[size_t(b)].n);
+            bool const split = P->two_stage && int64_t(nmin) * nmin >= int64_t(kBtSplitS) * kBT * kBT +
+                                                                          int64_t(kBtSplitW) * kBT * nmin;
+            int const ks = split ? kBtSplitS : 1, kw = split ? kBtSplitW : 1;
+            BigPlan::BtPanel T{j0, int(L.size()), 0, 0, ks, kw, int64_t(ids.size()), int64_t(gj.size())};
             for(int kind = 0; kind < 4; ++kind) // jobs grouped by kind: S, W1, W2, X
-                for(int b : L) {
-                    BigBlk const& B = P->h_blk[size_t(b)];
-                    int const n = B.n, nb = std::min(kBT, n - sh - j0), rows = n - j0 - sh;
-                    DOff const o = doff(n, B.gsym);
-                    int64_t const voff = B.dbase + o.V + int64_t(j0 + sh) * n + j0;
-                    if(kind == 0) {
-                        gj.push_back(pomx_gemm_job{nb, nb, rows, 1, 0, n, n, nb, voff, voff, B.dbase + o.S, 1.0, 0.0});
-                        ids.push_back(b);
-                        T.nb_max = std::max(T.nb_max, nb);
-                        T.rows_max = std::max(T.rows_max, rows);
-                    } else if(kind == 1)
-                        gj.push_back(pomx_gemm_job{nb, n, rows, 1, 0, n, n, n, voff, B.mat_off + int64_t(j0 + sh) * n,
-                                                   B.dbase + o.W1, 1.0, 0.0});
-                    else if(kind == 2)
-                        gj.push_back(pomx_gemm_job{nb, n, nb, 0, 0, nb, n, n, B.dbase + o.T, B.dbase + o.W1, B.dbase + o.W2,
-                                                   1.0, 0.0});
-                    else
-                        gj.push_back(pomx_gemm_job{rows, n, nb, 0, 0, n, n, n, voff, B.dbase + o.W2,
-                                                   B.mat_off + int64_t(j0 + sh) * n, -1.0, 1.0});
-                }
+                for(int z = 0; z < (kind == 0 ? ks : kind == 1 ? kw : 1); ++z)
+                    for(int b : L) {
+                        BigBlk const& B = P->h_blk[size_t(b)];
+                        int const n = B.n, nb = std::min(kBT, n - sh - j0), rows = n - j0 - sh;
+                        DOff const o = doff(n, B.gsym);
+                        int64_t const voff = B.dbase + o.V + int64_t(j0 + sh) * n + j0;
+                        int64_t const w1 = kw > 1 ? B.dbase + o.U + int64_t(ks) * kBT * kBT : B.dbase + o.W1;
+                        if(kind == 0) {
+                            int const rc = (rows + ks - 1) / ks, r0 = std::min(rows, z * rc), r1 = std::min(rows, r0 + rc);
+                            int64_t const out = ks > 1 ? B.dbase + o.U + int64_t(z) * kBT * kBT : B.dbase + o.S;
+                            gj.push_back(pomx_gemm_job{nb, nb, r1 - r0, 1, 0, n, n, nb, voff + int64_t(r0) * n,
+                                                       voff + int64_t(r0) * n, out, 1.0, 0.0});
+                            if(z == 0) {
+                                ids.push_back(b);
+                                T.nb_max = std::max(T.nb_max, nb);
+                                T.rows_max = std::max(T.rows_max, rows);
+                            }
+                        } else if(kind == 1) {
+                            int const rc = (rows + kw - 1) / kw, r0 = std::min(rows, z * rc), r1 = std::min(rows, r0 + rc);
+                            gj.push_back(pomx_gemm_job{nb, n, r1 - r0, 1, 0, n, n, n, voff + int64_t(r0) * n,
+                                                       B.mat_off + int64_t(j0 + sh + r0) * n, w1 + int64_t(z) * nb * n,
+                                                       1.0, 0.0});
+                        } else if(kind == 2) { // (split: W1 summed from its partial products first)
+                            gj.push_back(pomx_gemm_job{nb, n, nb, 0, 0, nb, n, n, B.dbase + o.T, B.dbase + o.W1,
+                                                       B.dbase + o.W2, 1.0, 0.0});
+                        } else
+                            gj.push_back(pomx_gemm_job{rows, n, nb, 0, 0, n, n, n, voff, B.dbase + o.W2,
+                                                       B.mat_off + int64_t(j0 + sh) * n, -1.0, 1.0});
+                    }
             P->bt.push_back(T);
         }
     }
@@ -2088,14 +2130,22 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
         int64_t const tn = (P->nmax + 63) / 64;
         for(auto const& T : P->bt) {
             pomx_gemm_job const* J = P->gjobs + T.jobs_off;
+            int const ns = T.ks * T.nblk, nw = T.kw * T.nblk;
             int64_t const tb = (T.nb_max + 63) / 64;
             int64_t const tr = (T.rows_max + 63) / 64;
-            if((rc = launch_gemm_jobs(ctx, 1, 0, J, T.nblk, tb * tb, P->D, P->D, P->D)))
+            if((rc = launch_gemm_jobs(ctx, 1, 0, J, ns, tb * tb, P->D, P->D, P->D)))
                 return rc;
+            if(T.ks > 1)
+                bt_sum_kernel<<<dim3(unsigned((T.nb_max * T.nb_max + 255) / 256), unsigned(T.nblk)), 256, 0, s>>>(
+                    P->blk, P->ids + T.ids_off, T.j0, sh, 0, T.ks, P->D);
             wy_t_kernel<<<unsigned(T.nblk), kBT, 0, s>>>(P->blk, P->ids + T.ids_off, T.j0, sh, P->D);
-            if((rc = launch_gemm_jobs(ctx, 1, 0, J + T.nblk, T.nblk, tb * tn, P->D, Aar, P->D)) ||
-               (rc = launch_gemm_jobs(ctx, 0, 0, J + 2 * T.nblk, T.nblk, tb * tn, P->D, P->D, P->D)) ||
-               (rc = launch_gemm_jobs(ctx, 0, 0, J + 3 * T.nblk, T.nblk, tr * tn, P->D, P->D, Aar)))
+            if((rc = launch_gemm_jobs(ctx, 1, 0, J + ns, nw, tb * tn, P->D, Aar, P->D)))
+                return rc;
+            if(T.kw > 1)
+                bt_sum_kernel<<<dim3(gx_rows(int64_t(T.nb_max) * P->nmax, 256, T.nblk), unsigned(T.nblk)), 256, 0, s>>>(
+                    P->blk, P->ids + T.ids_off, T.j0, sh, 1, T.kw, P->D);
+            if((rc = launch_gemm_jobs(ctx, 0, 0, J + ns + nw, T.nblk, tb * tn, P->D, P->D, P->D)) ||
+               (rc = launch_gemm_jobs(ctx, 0, 0, J + ns + nw + T.nblk, T.nblk, tr * tn, P->D, P->D, Aar)))
                 return rc;
         }
         evals_out_kernel<<<dim3(gx_rows(P->nmax, 256, nb), unsigned(nb)), 256, 0, s>>>(P->blk, P->D, eval);
