@@ -164,7 +164,7 @@ namespace {
 void free_batch(pomx_batch* b) {
     void* ptrs[] = {b->mat,  b->eval,     b->w,        b->ws,     b->h_val,     b->beta,   b->ops,   b->partial,
                     b->G,    b->E0,       b->h_pos,    b->inst_off, b->inst_cnt, b->item_off, b->src, b->counters,
-                    b->eigh_jobs, b->jobs, b->tiles, b->jinfo, b->items}; // status lives inside counters
+                    b->eigh_jobs, b->jobs, b->tiles, b->jinfo, b->items, b->kmap, b->kcnt}; // status lives inside counters
     for(void* p : ptrs)
         if(p)
             (void)hipFree(p);
@@ -223,7 +223,7 @@ int enqueue_solve(pomx_ctx* ctx, pomx_batch* b) {
             return rc;
     }
     inst_weights_kernel<<<b->ninst, 256, 0, s>>>(b->inst_off, b->inst_cnt, b->beta, b->eval, b->w, b->E0);
-    int rc = launch_rotate(ctx, b->tiles, b->ntiles, b->jobs, b->mat, b->src, b->ops, 1e-20);
+    int rc = launch_rotate(ctx, b->tiles, b->ntiles, b->jobs, b->mat, b->kmap, b->kcnt, b->ops, 1e-20);
     if(rc)
         return rc;
     if(b->nitems)
@@ -250,6 +250,10 @@ extern "C" int pomx_batch_create(pomx_ctx* ctx, const pomx_batch_desc* d, pomx_b
     POMX_REQUIRE(ctx, d && out && d->ninst > 0 && d->nfreq > 0 && d->nfreq <= 128 && d->nblocks > 0,
                  "pomx_batch_create: bad descriptor (1 <= nfreq <= 128)");
     *out = nullptr;
+    for(int q = 0; q < d->njobs; ++q)
+        POMX_REQUIRE(ctx, d->jobs[q].n_to >= 1 && d->jobs[q].n_from >= 1 && d->jobs[q].src_off >= 0 &&
+                              d->jobs[q].src_off + d->jobs[q].n_to <= d->src_len,
+                     "pomx_batch_create: job " + std::to_string(q) + " is empty or its gather map exceeds src_len");
     auto* b = new pomx_batch();
     b->ninst = d->ninst;
     b->nfreq = d->nfreq;
@@ -310,10 +314,7 @@ extern "C" int pomx_batch_create(pomx_ctx* ctx, const pomx_batch_desc* d, pomx_b
     }
     // rotation tiles
     std::vector<RotTile> tiles;
-    for(int q = 0; q < d->njobs; ++q)
-        for(int i0 = 0; i0 < d->jobs[q].n_to; i0 += 64)
-            for(int j0 = 0; j0 < d->jobs[q].n_from; j0 += 64)
-                tiles.push_back(RotTile{q, i0, j0, 0});
+    rot_tiles(d->jobs, d->njobs, tiles);
     b->ntiles = int(tiles.size());
     // G jobs grouped by instance (the jobs array is in instance order)
     b->h_jinfo.resize(size_t(d->njobs));
@@ -349,6 +350,10 @@ extern "C" int pomx_batch_create(pomx_ctx* ctx, const pomx_batch_desc* d, pomx_b
     b->eigh_jobs = dev_copy(ctx, ej.data(), ej.size(), rc);
     b->jobs = dev_copy(ctx, d->jobs, size_t(d->njobs), rc);
     b->tiles = dev_copy(ctx, tiles.data(), tiles.size(), rc);
+    b->kmap = dev_copy<int2>(ctx, nullptr, size_t(std::max<int64_t>(d->src_len, 1)), rc);
+    b->kcnt = dev_copy<int>(ctx, nullptr, size_t(std::max(d->njobs, 1)), rc);
+    if(!rc)
+        rc = launch_rot_compact(ctx, b->jobs, d->njobs, b->src, b->kmap, b->kcnt);
     b->jinfo = dev_copy(ctx, b->h_jinfo.data(), b->h_jinfo.size(), rc);
     b->h_eigh = ej;
     b->h_jobs.assign(d->jobs, d->jobs + d->njobs);

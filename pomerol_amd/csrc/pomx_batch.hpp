@@ -38,6 +38,8 @@ struct pomx_batch {
     double *ops = nullptr, *partial = nullptr, *G = nullptr, *E0 = nullptr;
     int64_t *h_pos = nullptr, *inst_off = nullptr, *inst_cnt = nullptr, *item_off = nullptr;
     int *src = nullptr, *counters = nullptr, *status = nullptr;
+    int2* kmap = nullptr; // gather maps compacted to their mapped rows (built at create)
+    int* kcnt = nullptr;
     EighJob* eigh_jobs = nullptr;
     pomx_rot_job* jobs = nullptr;
     RotTile* tiles = nullptr;

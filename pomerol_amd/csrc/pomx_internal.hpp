@@ -223,9 +223,15 @@ int launch_eigh(pomx_ctx* ctx, const EighJob* d_jobs, int njobs, int nmax, int b
 struct RotTile {
     int job, i0, j0, pad;
 };
-// Launch the rotation GEMM over a device-resident tile list (no host synchronisation).
+// Rotation tile list of the jobs (L2-friendly walk dealt to the XCDs; padding tiles have job -1).
+void rot_tiles(const pomx_rot_job* jobs, int njobs, std::vector<RotTile>& tiles);
+// Compaction of the gather maps (no host synchronisation): for every job, the pairs
+// (t, src[t]) with src[t] != 0 in ascending t to d_kmap[src_off ..], their number to d_kcnt[job].
+int launch_rot_compact(pomx_ctx* ctx, const pomx_rot_job* d_jobs, int njobs, const int* d_src, int2* d_kmap, int* d_kcnt);
+// Launch the rotation GEMM over a device-resident tile list and compacted gather maps (no host
+// synchronisation).
 int launch_rotate(pomx_ctx* ctx, const RotTile* d_tiles, int ntiles, const pomx_rot_job* d_jobs, const double* d_U,
-                  const int* d_src, double* d_out, double cutoff);
+                  const int2* d_kmap, const int* d_kcnt, double* d_out, double cutoff);
 
 struct GemmTile {
     int job, i0, j0, pad;

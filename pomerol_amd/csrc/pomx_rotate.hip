@@ -3,89 +3,213 @@
 // Replaces MonomialOperatorPart::computeImpl (src/pomerol/MonomialOperatorPart.cpp:48-92):
 //   OURight = M * U_R   (M has at most one +-1 per column: a signed row gather)
 //   O       = U_L^T * OURight,  then sparseView(Tol) (entries |x| <= Tol*1e-12 dropped).
-// The gather is fused into the B-operand load of an FP64 MFMA GEMM
+// Rows t of OURight with no source state are zero and add nothing to O, so the product runs
+// over the mapped rows only: O = sum_q U_L[t_q, :]^T (+-U_R[s_q, :]), q over the (t, +-(s+1))
+// pairs of the gather map with a source, kept in ascending t (a compaction pass per job).
+// Both operands are then row gathers, fused into the tile loads of an FP64 MFMA GEMM
 // (v_mfma_f64_16x16x4f64): 64x64 output tile per workgroup, four waves of 32x32, K staged
-// through LDS 16 rows at a time.
+// through LDS 16 pairs at a time, the next slab's rows loaded while the current one is
+// multiplied.
 #include "pomx_internal.hpp"
+
+#include <algorithm>
 
 namespace {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
+// One workgroup per job: the pairs (t, src[t]) with src[t] != 0, in ascending t, to
+// kmap[src_off ..]; their number to kcnt[job].
+__global__ void __launch_bounds__(256) rot_compact_kernel(const pomx_rot_job* __restrict__ jobs, const int* __restrict__ src,
+                                                          int2* __restrict__ kmap, int* __restrict__ kcnt) {
+    __shared__ int wsum[4];
+    pomx_rot_job const J = jobs[blockIdx.x];
+    const int* __restrict__ sr = src + J.src_off;
+    int2* __restrict__ km = kmap + J.src_off;
+    int const tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    int base = 0;
+    for(int c0 = 0; c0 < J.n_to; c0 += 256) {
+        int const t = c0 + tid;
+        int const sv = t < J.n_to ? sr[t] : 0;
+        unsigned long long const m = __ballot(sv != 0);
+        int const before = __popcll(m & ((1ull << lane) - 1ull));
+        __syncthreads(); // wsum of the previous chunk consumed
+        if(lane == 0)
+            wsum[wave] = __popcll(m);
+        __syncthreads();
+        int off = base;
+        for(int w = 0; w < wave; ++w)
+            off += wsum[w];
+        if(sv != 0)
+            km[off + before] = int2{t, sv};
+        base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    }
+    if(tid == 0)
+        kcnt[blockIdx.x] = base;
+}
+
+#ifndef POMX_ROT_TILE
+#define POMX_ROT_TILE 64
+#endif
+constexpr int kRotTile = POMX_ROT_TILE; // output tile kRotTile x kRotTile per workgroup
+
+// Output tile TT x TT of one job per workgroup; four waves of TT/2 x TT/2, each as
+// (TT/32)^2 accumulators of v_mfma_f64_16x16x4f64.
+template <int TT>
 __global__ void __launch_bounds__(256) rotate_kernel(const RotTile* __restrict__ tiles, const pomx_rot_job* __restrict__ jobs,
-                                                     const double* __restrict__ U, const int* __restrict__ src,
-                                                     double* __restrict__ out, double cutoff) {
-    __shared__ double As[16][64 + 4];
-    __shared__ double Bs[16][64 + 4];
+                                                     const double* __restrict__ U, const int2* __restrict__ kmap,
+                                                     const int* __restrict__ kcnt, double* __restrict__ out, double cutoff) {
+    constexpr int WT = TT / 2, NB = WT / 16; // wave tile, 16 x 16 blocks per wave dimension
+    constexpr int RS = 256 / TT, NR = 16 / RS; // slab row stride of the loads, rows per thread
+    static_assert(TT == 64 || TT == 128, "tile of 64 or 128");
+    __shared__ double As[16][TT + 4];
+    __shared__ double Bs[16][TT + 4];
     RotTile const T = tiles[blockIdx.x];
+    if(T.job < 0) // padding of the tile list (uniform per workgroup)
+        return;
     pomx_rot_job const J = jobs[T.job];
-    int const M = J.n_to, N = J.n_from, K = J.n_to;
+    int const M = J.n_to, N = J.n_from, K = kcnt[T.job];
     const double* __restrict__ Ut = U + J.u_to_off;
     const double* __restrict__ Uf = U + J.u_from_off;
-    const int* __restrict__ sr = src + J.src_off;
-    int const tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    int const wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    dbl4 acc[2][2];
-    for(int a = 0; a < 2; ++a)
-        for(int b = 0; b < 2; ++b)
-            acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
-    for(int t0 = 0; t0 < K; t0 += 16) {
-        for(int e = tid; e < 16 * 64; e += 256) {
-            int kk = e >> 6, cc = e & 63;
-            int t = t0 + kk;
-            double a = 0.0, b = 0.0;
-            if(t < K) {
-                int i = T.i0 + cc;
-                if(i < M)
-                    a = Ut[int64_t(t) * M + i];
-                int sv = sr[t];
-                int j = T.j0 + cc;
-                if(sv != 0 && j < N) {
-                    int s = (sv > 0 ? sv : -sv) - 1;
-                    double x = Uf[int64_t(s) * N + j];
-                    b = sv > 0 ? x : -x;
-                }
-            }
-            As[kk][cc] = a;
-            Bs[kk][cc] = b;
+    const int2* __restrict__ km = kmap + J.src_off;
+    int const tid = threadIdx.x, lane = tid & 63;
+    int const wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int const wm = (wave >> 1) * WT, wn = (wave & 1) * WT;
+    int const col = tid % TT, row0 = __builtin_amdgcn_readfirstlane(tid / TT);
+    int const i = T.i0 + col, j = T.j0 + col;
+    bool const iok = i < M, jok = j < N;
+    // column read by this lane (clamped into the block: the loads need no branch)
+    int const ic = iok ? i : M - 1, jc = jok ? j : N - 1;
+    // slab rows of this thread: row0 + RS r (r < NR), uniform per wave; one lane per tile
+    // column.  kp: map entries of the slab after the one in flight; ra/rb: the slab in flight,
+    // unsigned; sg: its signs (0 for rows past the end of the map).  The sign and the masks are
+    // applied at the LDS store, so no arithmetic waits on the loads before the products of the
+    // current slab.
+    int2 kp[NR];
+    int sg[NR];
+    double ra[NR], rb[NR];
+    auto load_map = [&](int t0) {
+#pragma unroll
+        for(int r = 0; r < NR; ++r) {
+            int const q = t0 + row0 + RS * r;
+            kp[r] = q < K ? km[q] : int2{0, 0};
         }
+    };
+    auto load_vals = [&]() {
+#pragma unroll
+        for(int r = 0; r < NR; ++r) {
+            int const sv = kp[r].y;
+            int const s = sv > 0 ? sv - 1 : (sv < 0 ? -sv - 1 : 0);
+            sg[r] = sv;
+            ra[r] = Ut[int64_t(kp[r].x) * M + ic];
+            rb[r] = Uf[int64_t(s) * N + jc];
+        }
+    };
+    dbl4 acc[NB][NB];
+#pragma unroll
+    for(int a = 0; a < NB; ++a)
+#pragma unroll
+        for(int b = 0; b < NB; ++b)
+            acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+    load_map(0);
+    load_vals();
+    load_map(16);
+    for(int t0 = 0; t0 < K; t0 += 16) {
         __syncthreads();
 #pragma unroll
-        for(int kq = 0; kq < 4; ++kq) {
-            int kr = kq * 4 + (lane >> 4);
-            double a0 = As[kr][wm + (lane & 15)], a1 = As[kr][wm + 16 + (lane & 15)];
-            double b0 = Bs[kr][wn + (lane & 15)], b1 = Bs[kr][wn + 16 + (lane & 15)];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        for(int r = 0; r < NR; ++r) {
+            As[row0 + RS * r][col] = (sg[r] != 0 && iok) ? ra[r] : 0.0;
+            Bs[row0 + RS * r][col] = (sg[r] != 0 && jok) ? (sg[r] > 0 ? rb[r] : -rb[r]) : 0.0;
         }
         __syncthreads();
+        if(t0 + 16 < K) {
+            load_vals();
+            load_map(t0 + 32);
+        }
+#pragma unroll
+        for(int kq = 0; kq < 4; ++kq) {
+            int const kr = kq * 4 + (lane >> 4);
+            double av[NB], bv[NB];
+#pragma unroll
+            for(int m = 0; m < NB; ++m) {
+                av[m] = As[kr][wm + 16 * m + (lane & 15)];
+                bv[m] = Bs[kr][wn + 16 * m + (lane & 15)];
+            }
+#pragma unroll
+            for(int m = 0; m < NB; ++m)
+#pragma unroll
+                for(int n = 0; n < NB; ++n)
+                    acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv[n], acc[m][n], 0, 0, 0);
+        }
     }
     double* __restrict__ O = out + J.out_off;
 #pragma unroll
-    for(int mt = 0; mt < 2; ++mt)
+    for(int m = 0; m < NB; ++m)
 #pragma unroll
-        for(int nt2 = 0; nt2 < 2; ++nt2)
+        for(int n = 0; n < NB; ++n)
 #pragma unroll
             for(int r = 0; r < 4; ++r) {
                 // v_mfma_f64_16x16x4f64 result layout: D[i][j] in lane j + 16*(i%4), register i/4
-                int i = T.i0 + wm + mt * 16 + (lane >> 4) + 4 * r;
-                int j = T.j0 + wn + nt2 * 16 + (lane & 15);
-                if(i < M && j < N) {
-                    double x = acc[mt][nt2][r];
-                    O[int64_t(i) * N + j] = fabs(x) <= cutoff ? 0.0 : x;
+                int const oi = T.i0 + wm + m * 16 + (lane >> 4) + 4 * r;
+                int const oj = T.j0 + wn + n * 16 + (lane & 15);
+                if(oi < M && oj < N) {
+                    double const x = acc[m][n][r];
+                    O[int64_t(oi) * N + oj] = fabs(x) <= cutoff ? 0.0 : x;
                 }
             }
 }
 
 } // namespace
 
+#ifndef POMX_ROT_GROUP
+#define POMX_ROT_GROUP 8
+#endif
+#ifndef POMX_ROT_XCD
+#define POMX_ROT_XCD 1
+#endif
+
+void rot_tiles(const pomx_rot_job* jobs, int njobs, std::vector<RotTile>& tiles) {
+    // Walk of every job's output tiles in groups of kGroup tile rows, column by column inside a
+    // group: workgroups that run at the same time share the rows of U_L and of U_R they read.
+    constexpr int kGroup = POMX_ROT_GROUP;
+    std::vector<RotTile> walk;
+    constexpr int TT = kRotTile;
+    for(int q = 0; q < njobs; ++q) {
+        int const tm = (jobs[q].n_to + TT - 1) / TT, tn = (jobs[q].n_from + TT - 1) / TT;
+        if(kGroup <= 0) {
+            for(int a = 0; a < tm; ++a)
+                for(int c = 0; c < tn; ++c)
+                    walk.push_back(RotTile{q, TT * a, TT * c, 0});
+            continue;
+        }
+        for(int g0 = 0; g0 < tm; g0 += kGroup)
+            for(int c = 0; c < tn; ++c)
+                for(int a = g0; a < std::min(tm, g0 + kGroup); ++a)
+                    walk.push_back(RotTile{q, TT * a, TT * c, 0});
+    }
+    // Workgroup p of a launch runs on XCD p % 8, each XCD with its own L2: XCD x gets the
+    // contiguous slice x of the walk, in order (the list is padded to a multiple of 8 with
+    // tiles of job -1, which return at once).
+    constexpr int kXcd = POMX_ROT_XCD ? 8 : 1;
+    size_t const per = (walk.size() + kXcd - 1) / kXcd;
+    tiles.assign(per * kXcd, RotTile{-1, 0, 0, 0});
+    for(size_t r = 0; r < walk.size(); ++r)
+        tiles[(r % per) * kXcd + r / per] = walk[r];
+}
+
+int launch_rot_compact(pomx_ctx* ctx, const pomx_rot_job* d_jobs, int njobs, const int* d_src, int2* d_kmap, int* d_kcnt) {
+    if(njobs <= 0)
+        return POMX_OK;
+    rot_compact_kernel<<<unsigned(njobs), 256, 0, ctx->stream>>>(d_jobs, d_src, d_kmap, d_kcnt);
+    POMX_CHECK_LAUNCH(ctx);
+    return POMX_OK;
+}
+
 int launch_rotate(pomx_ctx* ctx, const RotTile* d_tiles, int ntiles, const pomx_rot_job* d_jobs, const double* d_U,
-                  const int* d_src, double* d_out, double cutoff) {
+                  const int2* d_kmap, const int* d_kcnt, double* d_out, double cutoff) {
     if(ntiles <= 0)
         return POMX_OK;
-    rotate_kernel<<<unsigned(ntiles), 256, 0, ctx->stream>>>(d_tiles, d_jobs, d_U, d_src, d_out, cutoff);
+    rotate_kernel<kRotTile><<<unsigned(ntiles), 256, 0, ctx->stream>>>(d_tiles, d_jobs, d_U, d_kmap, d_kcnt, d_out, cutoff);
     POMX_CHECK_LAUNCH(ctx);
     return POMX_OK;
 }
@@ -95,22 +219,29 @@ extern "C" int pomx_rotate_batched(pomx_ctx* ctx, int njobs, const pomx_rot_job*
     POMX_REQUIRE(ctx, njobs >= 0 && (njobs == 0 || (h_jobs && d_U && d_src && d_out)), "pomx_rotate_batched: bad arguments");
     if(njobs == 0)
         return POMX_OK;
-    std::vector<RotTile> tiles;
+    int64_t src_len = 0;
     for(int q = 0; q < njobs; ++q) {
-        POMX_REQUIRE(ctx, h_jobs[q].n_to >= 1 && h_jobs[q].n_from >= 1, "pomx_rotate_batched: empty block");
-        for(int i0 = 0; i0 < h_jobs[q].n_to; i0 += 64)
-            for(int j0 = 0; j0 < h_jobs[q].n_from; j0 += 64)
-                tiles.push_back(RotTile{q, i0, j0, 0});
+        POMX_REQUIRE(ctx, h_jobs[q].n_to >= 1 && h_jobs[q].n_from >= 1 && h_jobs[q].src_off >= 0,
+                     "pomx_rotate_batched: empty block or negative gather-map offset");
+        src_len = std::max<int64_t>(src_len, h_jobs[q].src_off + h_jobs[q].n_to);
     }
+    std::vector<RotTile> tiles;
+    rot_tiles(h_jobs, njobs, tiles);
     WsBuf<RotTile> d_tiles(ctx, tiles.size());
     WsBuf<pomx_rot_job> d_jobs(ctx, njobs);
-    if(!d_tiles.p || !d_jobs.p)
+    WsBuf<int2> d_kmap(ctx, size_t(src_len));
+    WsBuf<int> d_kcnt(ctx, size_t(njobs));
+    if(!d_tiles.p || !d_jobs.p || !d_kmap.p || !d_kcnt.p)
         return pomx_fail(ctx, POMX_ERR_HIP, "pomx_rotate_batched: out of device memory");
     POMX_HIP(ctx, hipMemcpyAsync(d_tiles.p, tiles.data(), sizeof(RotTile) * tiles.size(), hipMemcpyHostToDevice, ctx->stream));
     POMX_HIP(ctx, h2d_sync(ctx, d_jobs.p, h_jobs, sizeof(pomx_rot_job) * njobs)); // (both copies done)
-    int rc = launch_rotate(ctx, d_tiles, int(tiles.size()), d_jobs, d_U, d_src, d_out, cutoff);
-    if(rc)
+    int rc = launch_rot_compact(ctx, d_jobs, njobs, d_src, d_kmap, d_kcnt);
+    if(!rc)
+        rc = launch_rotate(ctx, d_tiles, int(tiles.size()), d_jobs, d_U, d_kmap, d_kcnt, d_out, cutoff);
+    if(rc) {
+        (void)hipStreamSynchronize(ctx->stream); // launches already queued read the pooled buffers
         return rc;
+    }
     POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return POMX_OK;
 }

@@ -134,6 +134,46 @@ def test_rotate_batched(cabi):
         assert np.abs(got - ref).max() < 1e-12 * max(1, np.abs(ref).max())
 
 
+def test_rotate_batched_empty_and_full_maps(cabi):
+    """Gather maps with no mapped row (the product is zero), with every row mapped, and with
+    mapped rows only past the first 16-row slab, in one batch (the kernel runs over the
+    compacted map of each job)."""
+    rng = np.random.default_rng(12)
+    cases = [(70, 40, "none"), (130, 130, "all"), (200, 90, "tail")]
+    U, src, jobs, refs = [], [], [], []
+    uoff = soff = ooff = 0
+    for nt, nf, kind in cases:
+        Ut, Uf = rng.standard_normal((nt, nt)), rng.standard_normal((nf, nf))
+        s = np.zeros(nt, dtype=np.int32)
+        if kind == "all":  # nt == nf: a signed permutation
+            s[:] = (rng.permutation(nf) + 1) * np.where(rng.random(nt) < 0.5, 1, -1)
+        elif kind == "tail":
+            tgt = np.arange(150, 200, 3)
+            s[tgt] = (rng.choice(nf, size=len(tgt), replace=False) + 1) * -1
+        X = np.zeros((nt, nf))
+        for t in range(nt):
+            if s[t]:
+                X[t] = np.sign(s[t]) * Uf[abs(s[t]) - 1]
+        refs.append((ooff, Ut.T @ X))
+        jobs.append(RotJob(nt, nf, uoff, uoff + nt * nt, soff, ooff))
+        U += [Ut.ravel(), Uf.ravel()]
+        src.append(s)
+        uoff += nt * nt + nf * nf
+        soff += nt
+        ooff += nt * nf
+    d_U, d_s = cabi.put(np.concatenate(U)), cabi.put(np.concatenate(src))
+    d_o = cabi.put(np.full(ooff, np.nan))
+    arr = (RotJob * len(jobs))(*jobs)
+    cabi.ok(cabi.lib.pomx_rotate_batched(cabi.ctx, len(jobs), arr, d_U, d_s, d_o, ctypes.c_double(1e-20)))
+    out = cabi.get(d_o, np.zeros(ooff))
+    for (off, ref), (nt, nf, kind) in zip(refs, cases):
+        got = out[off:off + nt * nf].reshape(nt, nf)
+        if kind == "none":
+            assert np.all(got == 0.0)
+        else:
+            assert np.abs(got - ref).max() < 1e-12 * max(1, np.abs(ref).max())
+
+
 @pytest.mark.parametrize("kind", [0, 1])
 def test_pair_eval(cabi, kind):
     rng = np.random.default_rng(5)
