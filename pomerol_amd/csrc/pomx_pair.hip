@@ -22,7 +22,12 @@ struct PairPartDev {
     pomx_view A, B;
     const double *E_out, *E_in, *w_out, *w_in;
     int64_t cand_off; // prefix of n_out * n_in
+    int64_t tile_off; // prefix of the parts' kPairTile x kPairTile candidate tiles
+    int tiles_in;     // tiles along the inner index
+    int outer_fast;   // 1: the views are contiguous along the outer index (load the tile along it)
 };
+
+constexpr int kPairTile = 64; // candidate tile (outer x inner) of one workgroup in mark / emit
 
 constexpr int kPartShift = 40;
 constexpr uint64_t kBinMask = (uint64_t(1) << kPartShift) - 1;
@@ -76,43 +81,112 @@ __device__ inline int classify(int kind, const PairPartDev* parts, Cand const& c
     return fabs(R) > tol ? 1 : 0;
 }
 
-__global__ void pair_mark_kernel(int kind, const PairPartDev* __restrict__ parts, int nparts, int64_t ncand,
-                                 double res, double tol, int64_t* __restrict__ flag) {
-    int64_t g = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if(g >= ncand)
-        return;
-    Cand c = load_cand(parts, nparts, g);
-    double R;
-    flag[g] = classify(kind, parts, c, res, tol, R) ? 1 : 0;
+__device__ inline int find_tile_part(const PairPartDev* parts, int nparts, int64_t t) {
+    int lo = 0, hi = nparts - 1;
+    while(lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if(parts[mid].tile_off <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
 }
 
-__global__ void pair_emit_kernel(int kind, const PairPartDev* __restrict__ parts, int nparts, int64_t ncand,
-                                 double res, double tol, const int64_t* __restrict__ pos, int64_t nout,
-                                 uint64_t* __restrict__ keys, uint64_t* __restrict__ vals, int* __restrict__ err) {
-    int64_t g = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if(g >= ncand)
-        return;
-    Cand c = load_cand(parts, nparts, g);
-    double R;
-    int cl = classify(kind, parts, c, res, tol, R);
-    if(!cl)
-        return;
-    uint64_t bin;
-    if(cl == 2)
-        bin = kZeroPoleBin;
-    else {
-        double q = round(c.P / res);
-        if(!(fabs(q) < double(kBinOffset - 1))) {
-            err[0] = 1;
-            q = 0;
+// Classes (classify) of the candidates of one kPairTile x kPairTile tile, cl[outer][inner] (0
+// outside the part).  The views are read along the index they are contiguous in (outer_fast),
+// so the strided operand of the emission order (inner fastest) is not read with a stride.
+__device__ inline void tile_classes(int kind, const PairPartDev* parts, int part, int o0, int i0, double res, double tol,
+                                    uint8_t (*cl)[kPairTile + 4]) {
+    PairPartDev const& p = parts[part];
+    int const lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+#pragma unroll 4
+    for(int r = 0; r < kPairTile / 4; ++r) {
+        int const u = lane, v = grp + 4 * r;
+        int const x1 = p.outer_fast ? u : v, x2 = p.outer_fast ? v : u;
+        int const i1 = o0 + x1, i2 = i0 + x2;
+        int c = 0;
+        if(i1 < p.n_out && i2 < p.n_in) {
+            Cand q;
+            q.part = part;
+            q.i1 = i1;
+            q.i2 = i2;
+            q.a = p.A.base[i1 * p.A.rs + i2 * p.A.cs];
+            q.b = p.B.base[i2 * p.B.rs + i1 * p.B.cs];
+            q.P = p.E_in[i2] - p.E_out[i1];
+            double R;
+            c = classify(kind, parts, q, res, tol, R);
         }
-        bin = uint64_t(int64_t(q) + kBinOffset);
+        cl[x1][x2] = uint8_t(c);
     }
-    int64_t at = pos[g];
-    if(at >= nout)
+}
+
+// One workgroup per candidate tile: flag[g] = 1 for an emitted candidate, g its position in the
+// emission order (part, outer, inner); written along the inner index.
+__global__ void __launch_bounds__(256) pair_mark_kernel(int kind, const PairPartDev* __restrict__ parts, int nparts,
+                                                        double res, double tol, int64_t* __restrict__ flag) {
+    __shared__ uint8_t cl[kPairTile][kPairTile + 4];
+    int64_t const t = blockIdx.x;
+    int const part = find_tile_part(parts, nparts, t);
+    PairPartDev const& p = parts[part];
+    int64_t const loc = t - p.tile_off;
+    int const o0 = int(loc / p.tiles_in) * kPairTile, i0 = int(loc % p.tiles_in) * kPairTile;
+    tile_classes(kind, parts, part, o0, i0, res, tol, cl);
+    __syncthreads();
+    int const lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    int const i2 = i0 + lane;
+    if(i2 >= p.n_in)
         return;
-    keys[at] = (uint64_t(c.part) << kPartShift) | bin;
-    vals[at] = uint64_t(g);
+    for(int r = 0; r < kPairTile / 4; ++r) {
+        int const x1 = grp + 4 * r, i1 = o0 + x1;
+        if(i1 < p.n_out)
+            flag[p.cand_off + int64_t(i1) * p.n_in + i2] = cl[x1][lane] ? 1 : 0;
+    }
+}
+
+// Same tiles: every emitted candidate g writes (part << kPartShift | pole bin, g) at pos[g].
+__global__ void __launch_bounds__(256) pair_emit_kernel(int kind, const PairPartDev* __restrict__ parts, int nparts,
+                                                        double res, double tol, const int64_t* __restrict__ pos,
+                                                        int64_t nout, uint64_t* __restrict__ keys,
+                                                        uint64_t* __restrict__ vals, int* __restrict__ err) {
+    __shared__ uint8_t cl[kPairTile][kPairTile + 4];
+    int64_t const t = blockIdx.x;
+    int const part = find_tile_part(parts, nparts, t);
+    PairPartDev const& p = parts[part];
+    int64_t const loc = t - p.tile_off;
+    int const o0 = int(loc / p.tiles_in) * kPairTile, i0 = int(loc % p.tiles_in) * kPairTile;
+    tile_classes(kind, parts, part, o0, i0, res, tol, cl);
+    __syncthreads();
+    int const lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    int const i2 = i0 + lane;
+    if(i2 >= p.n_in)
+        return;
+    double const e_in = p.E_in[i2];
+    for(int r = 0; r < kPairTile / 4; ++r) {
+        int const x1 = grp + 4 * r, i1 = o0 + x1;
+        if(i1 >= p.n_out)
+            break;
+        int const c = cl[x1][lane];
+        if(!c)
+            continue;
+        uint64_t bin;
+        if(c == 2)
+            bin = kZeroPoleBin;
+        else {
+            double q = round((e_in - p.E_out[i1]) / res);
+            if(!(fabs(q) < double(kBinOffset - 1))) {
+                err[0] = 1;
+                q = 0;
+            }
+            bin = uint64_t(int64_t(q) + kBinOffset);
+        }
+        int64_t const g = p.cand_off + int64_t(i1) * p.n_in + i2;
+        int64_t const at = pos[g];
+        if(at >= nout)
+            continue;
+        keys[at] = (uint64_t(part) << kPartShift) | bin;
+        vals[at] = uint64_t(g);
+    }
 }
 
 __global__ void heads_kernel(int64_t n, const uint64_t* __restrict__ keys, int64_t* __restrict__ head) {
@@ -427,12 +501,16 @@ extern "C" int pomx_pair_terms(pomx_ctx* ctx, int kind, int nparts, const pomx_p
     *d_residue = nullptr;
     *d_pole = nullptr;
     std::vector<PairPartDev> parts(nparts);
-    int64_t ncand = 0;
+    int64_t ncand = 0, ntiles = 0;
     for(int p = 0; p < nparts; ++p) {
         pomx_pair_part const& q = h_parts[p];
         POMX_REQUIRE(ctx, q.n_out >= 1 && q.n_in >= 1, "pomx_pair_terms: empty part");
-        parts[p] = PairPartDev{q.n_out, q.n_in, q.A, q.B, q.E_out, q.E_in, q.w_out, q.w_in, ncand};
+        int const to = (q.n_out + kPairTile - 1) / kPairTile, ti = (q.n_in + kPairTile - 1) / kPairTile;
+        // load along the outer index when more of the two operands are contiguous along it
+        int const ofast = int((q.A.rs == 1) + (q.B.cs == 1) > (q.A.cs == 1) + (q.B.rs == 1));
+        parts[p] = PairPartDev{q.n_out, q.n_in, q.A, q.B, q.E_out, q.E_in, q.w_out, q.w_in, ncand, ntiles, ti, ofast};
         ncand += int64_t(q.n_out) * q.n_in;
+        ntiles += int64_t(to) * ti;
         h_count[p] = 0;
         if(h_zero_pole)
             h_zero_pole[p] = 0.0;
@@ -453,7 +531,8 @@ extern "C" int pomx_pair_terms(pomx_ctx* ctx, int kind, int nparts, const pomx_p
     int rc = pomx_fill_zero(ctx, err.p, sizeof(int));
     if(rc)
         return rc;
-    pair_mark_kernel<<<grid_for(ncand), kBlock, 0, ctx->stream>>>(kind, d_parts, nparts, ncand, res, tol, flag);
+    POMX_REQUIRE(ctx, ntiles < (int64_t(1) << 31), "pomx_pair_terms: too many candidates");
+    pair_mark_kernel<<<unsigned(ntiles), 256, 0, ctx->stream>>>(kind, d_parts, nparts, res, tol, flag);
     POMX_CHECK_LAUNCH(ctx);
     if((rc = pomx_exclusive_scan(ctx, ncand, flag, pos)))
         return rc;
@@ -467,7 +546,7 @@ extern "C" int pomx_pair_terms(pomx_ctx* ctx, int kind, int nparts, const pomx_p
     double *outR = nullptr, *outP = nullptr;
     if(n > 0) {
         WsBuf<uint64_t> k0(ctx, n), k1(ctx, n), v0(ctx, n), v1(ctx, n);
-        pair_emit_kernel<<<grid_for(ncand), kBlock, 0, ctx->stream>>>(kind, d_parts, nparts, ncand, res, tol, pos, n, k0, v0, err);
+        pair_emit_kernel<<<unsigned(ntiles), 256, 0, ctx->stream>>>(kind, d_parts, nparts, res, tol, pos, n, k0, v0, err);
         POMX_CHECK_LAUNCH(ctx);
         int part_bits = 1;
         while((1 << part_bits) < nparts)

@@ -174,6 +174,74 @@ def test_rotate_batched_empty_and_full_maps(cabi):
             assert np.abs(got - ref).max() < 1e-12 * max(1, np.abs(ref).max())
 
 
+class View(ctypes.Structure):
+    _fields_ = [("base", ctypes.c_void_p), ("rs", ctypes.c_int64), ("cs", ctypes.c_int64)]
+
+
+class PairPart(ctypes.Structure):
+    _fields_ = [("n_out", ctypes.c_int), ("n_in", ctypes.c_int), ("A", View), ("B", View),
+                ("E_out", ctypes.c_void_p), ("E_in", ctypes.c_void_p), ("w_out", ctypes.c_void_p),
+                ("w_in", ctypes.c_void_p)]
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_pair_terms_view_layouts(cabi, kind):
+    """pomx_pair_terms reads the operator views along whichever index they are contiguous in:
+    the same parts given with row-major and with transposed storage give bit-identical term lists,
+    and each part's residues add up to the brute-force sum over its candidates (tol 0: nothing is
+    erased; GreensFunctionPart.cpp:84-94, SusceptibilityPart.cpp:93-116)."""
+    rng = np.random.default_rng(21 + kind)
+    shapes = [(150, 100), (64, 1), (3, 200)]
+    res = 1e-8
+    out = {}
+    for layout in ("outer_fast", "inner_fast"):
+        parts, sums, zps = [], [], []
+        for n_out, n_in in shapes:
+            A = rng.standard_normal((n_out, n_in)) * (rng.random((n_out, n_in)) < 0.3)
+            B = rng.standard_normal((n_in, n_out)) * (rng.random((n_in, n_out)) < 0.5)
+            Eo = np.round(rng.uniform(-2, 2, n_out), 2)  # repeated levels: shared pole bins
+            Ei = np.round(rng.uniform(-2, 2, n_in), 2)
+            wo, wi = rng.uniform(0, 1, n_out), rng.uniform(0, 1, n_in)
+            ab = A * B.T
+            P = Ei[None, :] - Eo[:, None]
+            if kind == 0:
+                sums.append((ab * (wo[:, None] + wi[None, :])).sum())
+                zps.append(0.0)
+            else:
+                zero = np.abs(P) < res
+                sums.append((ab * (wo[:, None] - wi[None, :]))[~zero].sum())
+                zps.append((ab * wo[:, None])[zero].sum())
+            if layout == "outer_fast":  # A stored transposed, B row-major
+                vA = View(cabi.put(A.T.copy()).value, 1, n_out)
+                vB = View(cabi.put(B).value, n_out, 1)
+            else:  # A row-major, B stored transposed
+                vA = View(cabi.put(A).value, n_in, 1)
+                vB = View(cabi.put(B.T.copy()).value, 1, n_in)
+            parts.append(PairPart(n_out, n_in, vA, vB, *(cabi.put(x).value for x in (Eo, Ei, wo, wi))))
+        # (the draws repeat for the second layout: same seed order per layout)
+        rng = np.random.default_rng(21 + kind)
+        arr = (PairPart * len(parts))(*parts)
+        d_r, d_p = ctypes.c_void_p(), ctypes.c_void_p()
+        cnt = (ctypes.c_int64 * len(parts))()
+        zp = (ctypes.c_double * len(parts))()
+        cabi.ok(cabi.lib.pomx_pair_terms(cabi.ctx, kind, len(parts), arr, ctypes.c_double(res), ctypes.c_double(0.0),
+                                         ctypes.byref(d_r), ctypes.byref(d_p), cnt, zp))
+        n = sum(cnt)
+        R = cabi.get(d_r, np.zeros(max(n, 1)))[:n]
+        Pl = cabi.get(d_p, np.zeros(max(n, 1)))[:n]
+        cabi.lib.pomx_free(cabi.ctx, d_r)
+        cabi.lib.pomx_free(cabi.ctx, d_p)
+        off = np.concatenate([[0], np.cumsum(list(cnt))])
+        for k in range(len(parts)):
+            assert abs(R[off[k]:off[k + 1]].sum() - sums[k]) < 1e-12 * max(1.0, abs(sums[k]))
+            if kind == 1:
+                assert abs(zp[k] - zps[k]) < 1e-12 * max(1.0, abs(zps[k]))
+        out[layout] = (list(cnt), R, Pl, list(zp))
+    a, b = out["outer_fast"], out["inner_fast"]
+    assert a[0] == b[0] and a[3] == b[3]
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+
+
 @pytest.mark.parametrize("kind", [0, 1])
 def test_pair_eval(cabi, kind):
     rng = np.random.default_rng(5)
