@@ -7,7 +7,7 @@
 // over the mapped rows only: O = sum_q U_L[t_q, :]^T (+-U_R[s_q, :]), q over the (t, +-(s+1))
 // pairs of the gather map with a source, kept in ascending t (a compaction pass per job).
 // Both operands are then row gathers, fused into the tile loads of an FP64 MFMA GEMM
-// (v_mfma_f64_16x16x4f64): 64x64 output tile per workgroup, four waves of 32x32, K staged
+// (v_mfma_f64_16x16x4f64): 128x128 output tile per workgroup, eight waves of 32x64, K staged
 // through LDS 16 pairs at a time, the next slab's rows loaded while the current one is
 // multiplied.
 #include "pomx_internal.hpp"
@@ -48,22 +48,25 @@ __global__ void __launch_bounds__(256) rot_compact_kernel(const pomx_rot_job* __
         kcnt[blockIdx.x] = base;
 }
 
-#ifndef POMX_ROT_TILE
-#define POMX_ROT_TILE 64
-#endif
-constexpr int kRotTile = POMX_ROT_TILE; // output tile kRotTile x kRotTile per workgroup
+// 128 x 128 output tile per workgroup of 8 waves (4 x 2, 32 x 64 each): against 64 x 64 tiles
+// of 4 waves it halves the operand rows each multiply-add reads from L2 (measured 60.6 against
+// 48.5 TFLOP/s on a 15876 x 15876 pair), at 112 registers, four waves per SIMD.
+constexpr int kRotTile = 128;
+constexpr int kRotWgm = 4; // wave rows of the workgroup (two wave columns)
 
-// Output tile TT x TT of one job per workgroup; four waves of TT/2 x TT/2, each as
-// (TT/32)^2 accumulators of v_mfma_f64_16x16x4f64.
-template <int TT>
-__global__ void __launch_bounds__(256) rotate_kernel(const RotTile* __restrict__ tiles, const pomx_rot_job* __restrict__ jobs,
-                                                     const double* __restrict__ U, const int2* __restrict__ kmap,
-                                                     const int* __restrict__ kcnt, double* __restrict__ out, double cutoff) {
-    constexpr int WT = TT / 2, NB = WT / 16; // wave tile, 16 x 16 blocks per wave dimension
-    constexpr int RS = 256 / TT, NR = 16 / RS; // slab row stride of the loads, rows per thread
-    static_assert(TT == 64 || TT == 128, "tile of 64 or 128");
-    __shared__ double As[16][TT + 4];
-    __shared__ double Bs[16][TT + 4];
+// Output tile TT x TT of one job per workgroup of WGM x 2 waves, each wave (TT/WGM) x (TT/2)
+// as 16 x 16 accumulators of v_mfma_f64_16x16x4f64.
+template <int TT, int WGM>
+__global__ void __launch_bounds__(128 * WGM) rotate_kernel(const RotTile* __restrict__ tiles, const pomx_rot_job* __restrict__ jobs,
+                                                           const double* __restrict__ U, const int2* __restrict__ kmap,
+                                                           const int* __restrict__ kcnt, double* __restrict__ out, double cutoff) {
+    constexpr int NT = 128 * WGM;                 // threads
+    constexpr int WM = TT / WGM, WN = TT / 2;     // wave tile
+    constexpr int NBM = WM / 16, NBN = WN / 16;   // 16 x 16 blocks per wave
+    constexpr int RS = NT / TT, NR = 16 / RS;     // slab row stride of the loads, rows per thread
+    static_assert(TT >= 64 && NT % TT == 0 && 16 % RS == 0 && WM % 16 == 0, "tile shape");
+    __shared__ double As[1][16][TT + 4];
+    __shared__ double Bs[1][16][TT + 4];
     RotTile const T = tiles[blockIdx.x];
     if(T.job < 0) // padding of the tile list (uniform per workgroup)
         return;
@@ -74,7 +77,7 @@ __global__ void __launch_bounds__(256) rotate_kernel(const RotTile* __restrict__
     const int2* __restrict__ km = kmap + J.src_off;
     int const tid = threadIdx.x, lane = tid & 63;
     int const wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    int const wm = (wave >> 1) * WT, wn = (wave & 1) * WT;
+    int const wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
     int const col = tid % TT, row0 = __builtin_amdgcn_readfirstlane(tid / TT);
     int const i = T.i0 + col, j = T.j0 + col;
     bool const iok = i < M, jok = j < N;
@@ -105,48 +108,55 @@ __global__ void __launch_bounds__(256) rotate_kernel(const RotTile* __restrict__
             rb[r] = Uf[int64_t(s) * N + jc];
         }
     };
-    dbl4 acc[NB][NB];
+    dbl4 acc[NBM][NBN];
 #pragma unroll
-    for(int a = 0; a < NB; ++a)
+    for(int a = 0; a < NBM; ++a)
 #pragma unroll
-        for(int b = 0; b < NB; ++b)
+        for(int b = 0; b < NBN; ++b)
             acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+    auto store = [&](int buf) {
+#pragma unroll
+        for(int r = 0; r < NR; ++r) {
+            As[buf][row0 + RS * r][col] = (sg[r] != 0 && iok) ? ra[r] : 0.0;
+            Bs[buf][row0 + RS * r][col] = (sg[r] != 0 && jok) ? (sg[r] > 0 ? rb[r] : -rb[r]) : 0.0;
+        }
+    };
+    auto mult = [&](int buf) {
+#pragma unroll
+        for(int kq = 0; kq < 4; ++kq) {
+            int const kr = kq * 4 + (lane >> 4);
+            double av[NBM], bv[NBN];
+#pragma unroll
+            for(int m = 0; m < NBM; ++m)
+                av[m] = As[buf][kr][wm + 16 * m + (lane & 15)];
+#pragma unroll
+            for(int n = 0; n < NBN; ++n)
+                bv[n] = Bs[buf][kr][wn + 16 * n + (lane & 15)];
+#pragma unroll
+            for(int m = 0; m < NBM; ++m)
+#pragma unroll
+                for(int n = 0; n < NBN; ++n)
+                    acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv[n], acc[m][n], 0, 0, 0);
+        }
+    };
     load_map(0);
     load_vals();
     load_map(16);
     for(int t0 = 0; t0 < K; t0 += 16) {
         __syncthreads();
-#pragma unroll
-        for(int r = 0; r < NR; ++r) {
-            As[row0 + RS * r][col] = (sg[r] != 0 && iok) ? ra[r] : 0.0;
-            Bs[row0 + RS * r][col] = (sg[r] != 0 && jok) ? (sg[r] > 0 ? rb[r] : -rb[r]) : 0.0;
-        }
+        store(0);
         __syncthreads();
         if(t0 + 16 < K) {
             load_vals();
             load_map(t0 + 32);
         }
-#pragma unroll
-        for(int kq = 0; kq < 4; ++kq) {
-            int const kr = kq * 4 + (lane >> 4);
-            double av[NB], bv[NB];
-#pragma unroll
-            for(int m = 0; m < NB; ++m) {
-                av[m] = As[kr][wm + 16 * m + (lane & 15)];
-                bv[m] = Bs[kr][wn + 16 * m + (lane & 15)];
-            }
-#pragma unroll
-            for(int m = 0; m < NB; ++m)
-#pragma unroll
-                for(int n = 0; n < NB; ++n)
-                    acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv[n], acc[m][n], 0, 0, 0);
-        }
+        mult(0);
     }
     double* __restrict__ O = out + J.out_off;
 #pragma unroll
-    for(int m = 0; m < NB; ++m)
+    for(int m = 0; m < NBM; ++m)
 #pragma unroll
-        for(int n = 0; n < NB; ++n)
+        for(int n = 0; n < NBN; ++n)
 #pragma unroll
             for(int r = 0; r < 4; ++r) {
                 // v_mfma_f64_16x16x4f64 result layout: D[i][j] in lane j + 16*(i%4), register i/4
@@ -161,27 +171,15 @@ __global__ void __launch_bounds__(256) rotate_kernel(const RotTile* __restrict__
 
 } // namespace
 
-#ifndef POMX_ROT_GROUP
-#define POMX_ROT_GROUP 8
-#endif
-#ifndef POMX_ROT_XCD
-#define POMX_ROT_XCD 1
-#endif
 
 void rot_tiles(const pomx_rot_job* jobs, int njobs, std::vector<RotTile>& tiles) {
     // Walk of every job's output tiles in groups of kGroup tile rows, column by column inside a
     // group: workgroups that run at the same time share the rows of U_L and of U_R they read.
-    constexpr int kGroup = POMX_ROT_GROUP;
+    constexpr int kGroup = 8;
     std::vector<RotTile> walk;
     constexpr int TT = kRotTile;
     for(int q = 0; q < njobs; ++q) {
         int const tm = (jobs[q].n_to + TT - 1) / TT, tn = (jobs[q].n_from + TT - 1) / TT;
-        if(kGroup <= 0) {
-            for(int a = 0; a < tm; ++a)
-                for(int c = 0; c < tn; ++c)
-                    walk.push_back(RotTile{q, TT * a, TT * c, 0});
-            continue;
-        }
         for(int g0 = 0; g0 < tm; g0 += kGroup)
             for(int c = 0; c < tn; ++c)
                 for(int a = g0; a < std::min(tm, g0 + kGroup); ++a)
@@ -190,7 +188,7 @@ void rot_tiles(const pomx_rot_job* jobs, int njobs, std::vector<RotTile>& tiles)
     // Workgroup p of a launch runs on XCD p % 8, each XCD with its own L2: XCD x gets the
     // contiguous slice x of the walk, in order (the list is padded to a multiple of 8 with
     // tiles of job -1, which return at once).
-    constexpr int kXcd = POMX_ROT_XCD ? 8 : 1;
+    constexpr int kXcd = 8;
     size_t const per = (walk.size() + kXcd - 1) / kXcd;
     tiles.assign(per * kXcd, RotTile{-1, 0, 0, 0});
     for(size_t r = 0; r < walk.size(); ++r)
@@ -209,7 +207,7 @@ int launch_rotate(pomx_ctx* ctx, const RotTile* d_tiles, int ntiles, const pomx_
                   const int2* d_kmap, const int* d_kcnt, double* d_out, double cutoff) {
     if(ntiles <= 0)
         return POMX_OK;
-    rotate_kernel<kRotTile><<<unsigned(ntiles), 256, 0, ctx->stream>>>(d_tiles, d_jobs, d_U, d_kmap, d_kcnt, d_out, cutoff);
+    rotate_kernel<kRotTile, kRotWgm><<<unsigned(ntiles), 128 * kRotWgm, 0, ctx->stream>>>(d_tiles, d_jobs, d_U, d_kmap, d_kcnt, d_out, cutoff);
     POMX_CHECK_LAUNCH(ctx);
     return POMX_OK;
 }
