@@ -58,6 +58,10 @@ constexpr int kQ2Waves = 12;       // waves (groups in flight) per workgroup of 
 constexpr int kBtSplitS = 16;      // two-stage back-transformation: row ranges of S = V^T V
 constexpr int kBtSplitW = 4;       // and of W1 = V^T X (partial sums in the free D&C workspace U)
 constexpr int kTwoStageMin = 2048; // blocks at least this large take the two-stage reduction
+// 128 x 128 GEMM tiles (eight waves) from these sizes on: divide-and-conquer merges of blocks
+// of at least kBigGemmDc rows, V^T X of the back-transformation in blocks of at least kBigGemmW1
+// rows, X -= V W2 when both the reflector rows and the block have at least kBigGemmX
+constexpr int kBigGemmDc = 512, kBigGemmW1 = 1024, kBigGemmX = 256;
 
 // Two-stage per-block scratch (doubles from xbase).  AB: the band in lower band storage,
 // AB[j * 2 kBand + d] = A[j + d][j] (d < 2 kBand: the band plus room for the bulge).  V2: the
@@ -2111,7 +2115,7 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
         dc_colnorm_kernel<<<grow, 256, 0, s>>>(P->blk, nd, st, P->D);
         POMX_CHECK_LAUNCH(ctx);
         int64_t const t = (L.nmax + 63) / 64;
-        if((rc = launch_gemm_jobs(ctx, 0, 0, jb, L.nnodes, t * t, P->D, P->D, P->D)))
+        if((rc = launch_gemm_jobs(ctx, 0, 0, jb, L.nnodes, t * t, P->D, P->D, P->D, false, L.nmax >= kBigGemmDc)))
             return rc;
         dc_copycols_kernel<<<gsq, 256, 0, s>>>(P->blk, nd, st, P->D);
         dc_vals_kernel<<<grow, 256, 0, s>>>(P->blk, nd, st, P->D, P->I);
@@ -2139,13 +2143,14 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
                 bt_sum_kernel<<<dim3(unsigned((T.nb_max * T.nb_max + 255) / 256), unsigned(T.nblk)), 256, 0, s>>>(
                     P->blk, P->ids + T.ids_off, T.j0, sh, 0, T.ks, P->D);
             wy_t_kernel<<<unsigned(T.nblk), kBT, 0, s>>>(P->blk, P->ids + T.ids_off, T.j0, sh, P->D);
-            if((rc = launch_gemm_jobs(ctx, 1, 0, J + ns, nw, tb * tn, P->D, Aar, P->D)))
+            if((rc = launch_gemm_jobs(ctx, 1, 0, J + ns, nw, tb * tn, P->D, Aar, P->D, false, P->nmax >= kBigGemmW1)))
                 return rc;
             if(T.kw > 1)
                 bt_sum_kernel<<<dim3(gx_rows(int64_t(T.nb_max) * P->nmax, 256, T.nblk), unsigned(T.nblk)), 256, 0, s>>>(
                     P->blk, P->ids + T.ids_off, T.j0, sh, 1, T.kw, P->D);
             if((rc = launch_gemm_jobs(ctx, 0, 0, J + ns + nw, T.nblk, tb * tn, P->D, P->D, P->D)) ||
-               (rc = launch_gemm_jobs(ctx, 0, 0, J + ns + nw + T.nblk, T.nblk, tr * tn, P->D, P->D, Aar)))
+               (rc = launch_gemm_jobs(ctx, 0, 0, J + ns + nw + T.nblk, T.nblk, tr * tn, P->D, P->D, Aar, false,
+                                      T.rows_max >= kBigGemmX && P->nmax >= kBigGemmX)))
                 return rc;
         }
         evals_out_kernel<<<dim3(gx_rows(P->nmax, 256, nb), unsigned(nb)), 256, 0, s>>>(P->blk, P->D, eval);

@@ -236,8 +236,8 @@ int launch_rotate(pomx_ctx* ctx, const RotTile* d_tiles, int ntiles, const pomx_
 struct GemmTile {
     int job, i0, j0, pad;
 };
-// 64 x 64 output tiles of every job, in job order.
-void gemm_tiles(const pomx_gemm_job* jobs, int njobs, std::vector<GemmTile>& tiles);
+// tt x tt output tiles of every job, in job order.
+void gemm_tiles(const pomx_gemm_job* jobs, int njobs, std::vector<GemmTile>& tiles, int tt = 64);
 bool gemm_job_ok(pomx_gemm_job const& j);
 // Launch the batched MFMA GEMM over device-resident tiles whose jobs share (trans_a, trans_b).
 int launch_gemm(pomx_ctx* ctx, int trans_a, int trans_b, const GemmTile* d_tiles, int ntiles,
@@ -248,9 +248,10 @@ int launch_gemm1(pomx_ctx* ctx, pomx_gemm_job const& J, const double* d_A, const
 // Batched GEMM, job blockIdx.y, tiles along x (max_tiles = largest job's 64 x 64 tile count);
 // job sizes may be written on the device before the launch.
 // upper: only output tiles touching the upper triangle (for symmetric updates of square outputs
-// that start on the diagonal; TN only).
+// that start on the diagonal; TN only).  big: 128 x 128 output tiles of eight waves (NN and TN,
+// not upper), for jobs large enough to fill the device with them.
 int launch_gemm_jobs(pomx_ctx* ctx, int trans_a, int trans_b, const pomx_gemm_job* d_jobs, int njobs, int64_t max_tiles,
-                     const double* d_A, const double* d_B, double* d_C, bool upper = false);
+                     const double* d_A, const double* d_B, double* d_C, bool upper = false, bool big = false);
 
 // Device-driven symmetric eigensolver for a fixed set of blocks, all solved together: blocked
 // Householder tridiagonalisation with MFMA trailing updates (blocks of at least 2048 rows: a

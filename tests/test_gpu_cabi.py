@@ -270,7 +270,9 @@ def test_gemm_batched(cabi):
     rng = np.random.default_rng(11)
     shapes = [(1, 1, 1), (5, 7, 3), (64, 64, 64), (65, 33, 129), (130, 70, 17), (17, 200, 64), (3, 5, 0), (257, 129, 300),
               # larger jobs, every transposition pattern
-              (300, 260, 70), (513, 300, 129), (256, 385, 16), (290, 257, 1)]
+              (300, 260, 70), (513, 300, 129), (256, 385, 16), (290, 257, 1),
+              # 128 x 128 tiles (m, n >= 256, k >= 64) with partial tiles, every transposition pattern
+              (300, 300, 64), (257, 260, 65), (260, 384, 100), (300, 257, 80)]
     A_parts, B_parts, C_parts, jobs, refs = [], [], [], [], []
     ao = bo = co = 0
     for q, (m, n, k) in enumerate(shapes):
