@@ -97,6 +97,13 @@ typedef struct {
  * residues and poles of all parts back to back; h_count[p] terms for part p; for CHI also
  * h_zero_pole[p] (the |pole| < pole_res weight). */
 enum { POMX_KIND_GF = 0, POMX_KIND_CHI = 1 };
+/* Flag OR'ed into the kind of pomx_pair_terms (GF only; ignored for CHI): every element of A and
+ * B has magnitude at most 1 (with a relative rounding margin of 1e-9), as for the rotated blocks
+ * of a single creation or annihilation operator, whose operator norm is 1.  A candidate can then
+ * pass |R| > coeff_tol only if |w_out| + |w_in| > coeff_tol, and the generation skips the pairs of
+ * "cold" states (|w| <= coeff_tol / 2 on both sides) without reading them.  The emitted terms
+ * are the same as without the flag.  No counterpart in the reference. */
+enum { POMX_PAIR_UNIT_ELEMENTS = 0x100 };
 typedef struct {
     int n_out, n_in;
     pomx_view A; /* <out|A|in>  n_out x n_in */

@@ -189,6 +189,112 @@ __global__ void __launch_bounds__(256) pair_emit_kernel(int kind, const PairPart
     }
 }
 
+// ---- G candidates with a weight bound (POMX_PAIR_UNIT_ELEMENTS) --------------------------
+// With |A|, |B| <= 1 a candidate can pass |R| > tol only if |w_out| + |w_in| > tol, so only the
+// rows of hot outer states (|w_out| > tol / 2, every inner index) and the hot inner columns of the
+// other rows need to be read.  Compressed candidate space: row i1 holds n_in positions (hot row)
+// or nhc positions (the hot columns, ascending); roff[i1] is its first position, rows in
+// ascending i1, so the emission order (part, outer, inner) is kept.
+struct PairHot {
+    const int* hr;       // hot rows, ascending
+    const int* cr;       // cold rows, ascending
+    const int* hc;       // hot columns, ascending
+    const int64_t* roff; // compressed position of (i1, first column of its row)
+    int nhr, ncr, nhc, pad;
+};
+
+struct HotTile {
+    int part, region, r0, c0; // region 0: hot rows x all columns, 1: cold rows x hot columns
+};
+
+__device__ inline void hot_classes(const PairPartDev* parts, const PairHot* hot, HotTile const& T, double res, double tol,
+                                   uint8_t (*cl)[kPairTile + 4]) {
+    PairPartDev const& p = parts[T.part];
+    PairHot const& h = hot[T.part];
+    const int* __restrict__ rows = T.region ? h.cr : h.hr;
+    int const nrows = T.region ? h.ncr : h.nhr, ncols = T.region ? h.nhc : p.n_in;
+    int const lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+#pragma unroll 4
+    for(int r = 0; r < kPairTile / 4; ++r) {
+        int const u = lane, v = grp + 4 * r;
+        int const x1 = p.outer_fast ? u : v, x2 = p.outer_fast ? v : u;
+        int const rr = T.r0 + x1, cc = T.c0 + x2;
+        int c = 0;
+        if(rr < nrows && cc < ncols) {
+            Cand q;
+            q.part = T.part;
+            q.i1 = rows[rr];
+            q.i2 = T.region ? h.hc[cc] : cc;
+            q.a = p.A.base[q.i1 * p.A.rs + q.i2 * p.A.cs];
+            q.b = p.B.base[q.i2 * p.B.rs + q.i1 * p.B.cs];
+            q.P = p.E_in[q.i2] - p.E_out[q.i1];
+            double R;
+            c = classify(POMX_KIND_GF, parts, q, res, tol, R);
+        }
+        cl[x1][x2] = uint8_t(c);
+    }
+}
+
+__global__ void __launch_bounds__(256) pair_hot_mark_kernel(const PairPartDev* __restrict__ parts, const PairHot* __restrict__ hot,
+                                                            const HotTile* __restrict__ tiles, double res, double tol,
+                                                            int64_t* __restrict__ flag) {
+    __shared__ uint8_t cl[kPairTile][kPairTile + 4];
+    HotTile const T = tiles[blockIdx.x];
+    hot_classes(parts, hot, T, res, tol, cl);
+    __syncthreads();
+    PairHot const& h = hot[T.part];
+    const int* __restrict__ rows = T.region ? h.cr : h.hr;
+    int const nrows = T.region ? h.ncr : h.nhr, ncols = T.region ? h.nhc : parts[T.part].n_in;
+    int const lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    int const cc = T.c0 + lane;
+    if(cc >= ncols)
+        return;
+    for(int r = 0; r < kPairTile / 4; ++r) {
+        int const x1 = grp + 4 * r, rr = T.r0 + x1;
+        if(rr < nrows)
+            flag[h.roff[rows[rr]] + cc] = cl[x1][lane] ? 1 : 0;
+    }
+}
+
+__global__ void __launch_bounds__(256) pair_hot_emit_kernel(const PairPartDev* __restrict__ parts, const PairHot* __restrict__ hot,
+                                                            const HotTile* __restrict__ tiles, double res, double tol,
+                                                            const int64_t* __restrict__ pos, int64_t nout,
+                                                            uint64_t* __restrict__ keys, uint64_t* __restrict__ vals,
+                                                            int* __restrict__ err) {
+    __shared__ uint8_t cl[kPairTile][kPairTile + 4];
+    HotTile const T = tiles[blockIdx.x];
+    hot_classes(parts, hot, T, res, tol, cl);
+    __syncthreads();
+    PairPartDev const& p = parts[T.part];
+    PairHot const& h = hot[T.part];
+    const int* __restrict__ rows = T.region ? h.cr : h.hr;
+    int const nrows = T.region ? h.ncr : h.nhr, ncols = T.region ? h.nhc : p.n_in;
+    int const lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    int const cc = T.c0 + lane;
+    if(cc >= ncols)
+        return;
+    int const i2 = T.region ? h.hc[cc] : cc;
+    double const e_in = p.E_in[i2];
+    for(int r = 0; r < kPairTile / 4; ++r) {
+        int const x1 = grp + 4 * r, rr = T.r0 + x1;
+        if(rr >= nrows)
+            break;
+        if(!cl[x1][lane])
+            continue;
+        int const i1 = rows[rr];
+        double q = round((e_in - p.E_out[i1]) / res);
+        if(!(fabs(q) < double(kBinOffset - 1))) {
+            err[0] = 1;
+            q = 0;
+        }
+        int64_t const at = pos[h.roff[i1] + cc];
+        if(at >= nout)
+            continue;
+        keys[at] = (uint64_t(T.part) << kPartShift) | uint64_t(int64_t(q) + kBinOffset);
+        vals[at] = uint64_t(p.cand_off + int64_t(i1) * p.n_in + i2);
+    }
+}
+
 __global__ void heads_kernel(int64_t n, const uint64_t* __restrict__ keys, int64_t* __restrict__ head) {
     int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if(k < n)
@@ -492,8 +598,9 @@ extern "C" int pomx_pair_eval_tau(pomx_ctx* ctx, int kind, int64_t nterms, const
     return POMX_OK;
 }
 
-extern "C" int pomx_pair_terms(pomx_ctx* ctx, int kind, int nparts, const pomx_pair_part* h_parts, double res,
+extern "C" int pomx_pair_terms(pomx_ctx* ctx, int kind_flags, int nparts, const pomx_pair_part* h_parts, double res,
                                double tol, double** d_residue, double** d_pole, int64_t* h_count, double* h_zero_pole) {
+    int const kind = kind_flags & ~int(POMX_PAIR_UNIT_ELEMENTS);
     POMX_REQUIRE(ctx, (kind == POMX_KIND_GF || kind == POMX_KIND_CHI) && nparts >= 0 && d_residue && d_pole && h_count,
                  "pomx_pair_terms: bad arguments");
     POMX_REQUIRE(ctx, res > 0.0, "pomx_pair_terms: pole resolution must be positive");
@@ -525,28 +632,131 @@ extern "C" int pomx_pair_terms(pomx_ctx* ctx, int kind, int nparts, const pomx_p
         return POMX_OK;
     }
     WsBuf<PairPartDev> d_parts(ctx, nparts);
-    WsBuf<int64_t> flag(ctx, ncand), pos(ctx, ncand);
     WsBuf<int> err(ctx, 1);
     POMX_HIP(ctx, h2d_sync(ctx, d_parts.p, parts.data(), sizeof(PairPartDev) * nparts));
     int rc = pomx_fill_zero(ctx, err.p, sizeof(int));
     if(rc)
         return rc;
-    POMX_REQUIRE(ctx, ntiles < (int64_t(1) << 31), "pomx_pair_terms: too many candidates");
-    pair_mark_kernel<<<unsigned(ntiles), 256, 0, ctx->stream>>>(kind, d_parts, nparts, res, tol, flag);
-    POMX_CHECK_LAUNCH(ctx);
-    if((rc = pomx_exclusive_scan(ctx, ncand, flag, pos)))
-        return rc;
-    int64_t last_flag = 0, last_pos = 0;
-    POMX_HIP(ctx, hipMemcpyAsync(&last_flag, flag.p + ncand - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
-    POMX_HIP(ctx, hipMemcpyAsync(&last_pos, pos.p + ncand - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
-    POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    int64_t n = last_pos + last_flag;
+    // G with unit-bounded elements: split every part into hot and cold states by the weights
+    // (host side; the weight vectors are short) and enumerate the compressed candidate space
+    bool hot = kind == POMX_KIND_GF && (kind_flags & POMX_PAIR_UNIT_ELEMENTS) && tol > 0.0;
+    WsBuf<int> hot_rows, hot_cols;
+    WsBuf<int64_t> hot_roff;
+    std::vector<PairHot> hh;
+    std::vector<int> hrows, hcols;
+    std::vector<int64_t> hroff;
+    std::vector<HotTile> htiles;
+    int64_t nq = 0;
+    if(hot) {
+        // |a b (w_out + w_in)| <= (1 + 1e-9)^2 (|w_out| + |w_in|) (1 + 4u): a pair of states with
+        // both |w| <= thr cannot pass |R| > tol
+        double const thr = 0.5 * tol / (1.0 + 1e-8);
+        struct Off {
+            size_t hr, cr, hc, roff;
+        };
+        std::vector<Off> off(nparts);
+        std::vector<double> wo, wi;
+        for(int p = 0; p < nparts && hot; ++p) {
+            pomx_pair_part const& q = h_parts[p];
+            wo.resize(size_t(q.n_out));
+            wi.resize(size_t(q.n_in));
+            POMX_HIP(ctx, hipMemcpyAsync(wo.data(), q.w_out, sizeof(double) * q.n_out, hipMemcpyDeviceToHost, ctx->stream));
+            POMX_HIP(ctx, hipMemcpyAsync(wi.data(), q.w_in, sizeof(double) * q.n_in, hipMemcpyDeviceToHost, ctx->stream));
+            POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            off[p].hr = hrows.size();
+            for(int i = 0; i < q.n_out; ++i)
+                if(!(fabs(wo[size_t(i)]) <= thr)) // (NaN counts as hot)
+                    hrows.push_back(i);
+            int const nhr = int(hrows.size() - off[p].hr);
+            off[p].cr = hrows.size();
+            for(int i = 0; i < q.n_out; ++i)
+                if(fabs(wo[size_t(i)]) <= thr)
+                    hrows.push_back(i);
+            off[p].hc = hcols.size();
+            for(int i = 0; i < q.n_in; ++i)
+                if(!(fabs(wi[size_t(i)]) <= thr))
+                    hcols.push_back(i);
+            int const nhc = int(hcols.size() - off[p].hc);
+            off[p].roff = hroff.size();
+            for(int i = 0, at = 0; i < q.n_out; ++i) {
+                bool const h = at < nhr && hrows[off[p].hr + size_t(at)] == i;
+                hroff.push_back(nq);
+                nq += h ? q.n_in : nhc;
+                at += h;
+            }
+            PairHot x{nullptr, nullptr, nullptr, nullptr, nhr, q.n_out - nhr, nhc, 0};
+            hh.push_back(x);
+            for(int r0 = 0; r0 < x.nhr; r0 += kPairTile)
+                for(int c0 = 0; c0 < q.n_in; c0 += kPairTile)
+                    htiles.push_back(HotTile{p, 0, r0, c0});
+            for(int r0 = 0; r0 < x.ncr; r0 += kPairTile)
+                for(int c0 = 0; c0 < x.nhc; c0 += kPairTile)
+                    htiles.push_back(HotTile{p, 1, r0, c0});
+        }
+        if(2 * nq > ncand) // little to skip: the plain enumeration reads fewer index lists
+            hot = false;
+        else {
+            WsBuf<int> dr(ctx, std::max<size_t>(hrows.size(), 1)), dc(ctx, std::max<size_t>(hcols.size(), 1));
+            WsBuf<int64_t> dro(ctx, std::max<size_t>(hroff.size(), 1));
+            if(!dr.p || !dc.p || !dro.p)
+                return pomx_fail(ctx, POMX_ERR_HIP, "pomx_pair_terms: out of device memory");
+            for(int p = 0; p < nparts; ++p) {
+                hh[size_t(p)].hr = dr.p + off[p].hr;
+                hh[size_t(p)].cr = dr.p + off[p].cr;
+                hh[size_t(p)].hc = dc.p + off[p].hc;
+                hh[size_t(p)].roff = dro.p + off[p].roff;
+            }
+            if(!hrows.empty())
+                POMX_HIP(ctx, h2d_sync(ctx, dr.p, hrows.data(), sizeof(int) * hrows.size()));
+            if(!hcols.empty())
+                POMX_HIP(ctx, h2d_sync(ctx, dc.p, hcols.data(), sizeof(int) * hcols.size()));
+            POMX_HIP(ctx, h2d_sync(ctx, dro.p, hroff.data(), sizeof(int64_t) * hroff.size()));
+            hot_rows = std::move(dr);
+            hot_cols = std::move(dc);
+            hot_roff = std::move(dro);
+        }
+    }
+    int64_t const nflag = hot ? nq : ncand;
+    WsBuf<PairHot> d_hot(ctx, hot ? hh.size() : 1);
+    WsBuf<HotTile> d_htiles(ctx, hot ? std::max<size_t>(htiles.size(), 1) : 1);
+    if(hot) {
+        if(!d_hot.p || !d_htiles.p)
+            return pomx_fail(ctx, POMX_ERR_HIP, "pomx_pair_terms: out of device memory");
+        POMX_HIP(ctx, h2d_sync(ctx, d_hot.p, hh.data(), sizeof(PairHot) * hh.size()));
+        if(!htiles.empty())
+            POMX_HIP(ctx, h2d_sync(ctx, d_htiles.p, htiles.data(), sizeof(HotTile) * htiles.size()));
+    }
+    WsBuf<int64_t> flag(ctx, size_t(std::max<int64_t>(nflag, 1))), pos(ctx, size_t(std::max<int64_t>(nflag, 1)));
+    if(!flag.p || !pos.p)
+        return pomx_fail(ctx, POMX_ERR_HIP, "pomx_pair_terms: out of device memory");
+    int64_t n = 0;
+    if(nflag > 0) {
+        if(hot) {
+            POMX_REQUIRE(ctx, htiles.size() < (size_t(1) << 31), "pomx_pair_terms: too many candidates");
+            pair_hot_mark_kernel<<<unsigned(htiles.size()), 256, 0, ctx->stream>>>(d_parts, d_hot, d_htiles, res, tol, flag);
+        } else {
+            POMX_REQUIRE(ctx, ntiles < (int64_t(1) << 31), "pomx_pair_terms: too many candidates");
+            pair_mark_kernel<<<unsigned(ntiles), 256, 0, ctx->stream>>>(kind, d_parts, nparts, res, tol, flag);
+        }
+        POMX_CHECK_LAUNCH(ctx);
+        if((rc = pomx_exclusive_scan(ctx, nflag, flag, pos)))
+            return rc;
+        int64_t last_flag = 0, last_pos = 0;
+        POMX_HIP(ctx, hipMemcpyAsync(&last_flag, flag.p + nflag - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
+        POMX_HIP(ctx, hipMemcpyAsync(&last_pos, pos.p + nflag - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
+        POMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        n = last_pos + last_flag;
+    }
     std::vector<int64_t> cnt(nparts, 0);
     std::vector<double> zp(nparts, 0.0);
     double *outR = nullptr, *outP = nullptr;
     if(n > 0) {
         WsBuf<uint64_t> k0(ctx, n), k1(ctx, n), v0(ctx, n), v1(ctx, n);
-        pair_emit_kernel<<<unsigned(ntiles), 256, 0, ctx->stream>>>(kind, d_parts, nparts, res, tol, pos, n, k0, v0, err);
+        if(hot)
+            pair_hot_emit_kernel<<<unsigned(htiles.size()), 256, 0, ctx->stream>>>(d_parts, d_hot, d_htiles, res, tol, pos, n,
+                                                                                   k0, v0, err);
+        else
+            pair_emit_kernel<<<unsigned(ntiles), 256, 0, ctx->stream>>>(kind, d_parts, nparts, res, tol, pos, n, k0, v0, err);
         POMX_CHECK_LAUNCH(ctx);
         int part_bits = 1;
         while((1 << part_bits) < nparts)

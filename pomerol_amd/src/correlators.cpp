@@ -148,7 +148,10 @@ void GreensFunction::compute() {
             dp.push_back(p.devicePart());
         std::vector<int64_t> cnt(parts.size());
         double *r = nullptr, *p = nullptr;
-        detail::check(pomx_pair_terms(detail::gpu(), POMX_KIND_GF, int(parts.size()), dp.data(), PoleResolution,
+        // F1 and F2 are single fermion operators: their rotated blocks have elements of magnitude
+        // at most 1, so pairs of states with negligible weights are skipped unread
+        detail::check(pomx_pair_terms(detail::gpu(), POMX_KIND_GF | POMX_PAIR_UNIT_ELEMENTS, int(parts.size()), dp.data(),
+                                      PoleResolution,
                                       CoefficientTolerance, &r, &p, cnt.data(), nullptr),
                       "Green's function terms");
         n_terms = std::accumulate(cnt.begin(), cnt.end(), int64_t(0));

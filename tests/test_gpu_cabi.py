@@ -242,6 +242,49 @@ def test_pair_terms_view_layouts(cabi, kind):
     assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
 
 
+@pytest.mark.parametrize("layout", ["outer_fast", "inner_fast"])
+def test_pair_terms_weight_bound(cabi, layout):
+    """POMX_PAIR_UNIT_ELEMENTS (elements of magnitude <= 1): G candidates whose two states both
+    have |w| <= tol / 2 are skipped unread; the term lists are bit-identical to the plain
+    enumeration.  Boltzmann weights at a large beta leave few hot states (hot rows and columns
+    scattered, not a prefix), plus parts with no hot state and with only hot states."""
+    rng = np.random.default_rng(31)
+    tol, res = 1e-8, 1e-8
+    parts, keep = [], []
+    for n_out, n_in, beta, shift in [(300, 200, 30.0, 0.0), (130, 257, 20.0, 0.0), (70, 90, 400.0, 1.5), (65, 40, 1e-3, 0.0),
+                                     (1, 150, 20.0, 0.0)]:
+        A = rng.uniform(-1, 1, (n_out, n_in)) * (rng.random((n_out, n_in)) < 0.4)
+        B = rng.uniform(-1, 1, (n_in, n_out)) * (rng.random((n_in, n_out)) < 0.6)
+        Eo, Ei = rng.permutation(np.linspace(0, 3, n_out)), rng.permutation(np.linspace(0.01, 3, n_in))
+        E0 = min(Eo.min(), Ei.min())
+        wo, wi = np.exp(-beta * (Eo - E0 + shift)), np.exp(-beta * (Ei - E0 + shift))
+        Z = max(wo.sum() + wi.sum(), 1.0)
+        wo, wi = wo / Z, wi / Z
+        if layout == "outer_fast":
+            vA, vB = View(cabi.put(A.T.copy()).value, 1, n_out), View(cabi.put(B).value, n_out, 1)
+        else:
+            vA, vB = View(cabi.put(A).value, n_in, 1), View(cabi.put(B.T.copy()).value, 1, n_in)
+        parts.append(PairPart(n_out, n_in, vA, vB, *(cabi.put(x).value for x in (Eo, Ei, wo, wi))))
+        keep.append(int(((np.abs(A * B.T * (wo[:, None] + wi[None, :]))) > tol).sum()))
+    print("[cabi] pair weight bound: candidates kept per part", keep, flush=True)
+    arr = (PairPart * len(parts))(*parts)
+    out = []
+    for flags in (0, 0x100):
+        d_r, d_p = ctypes.c_void_p(), ctypes.c_void_p()
+        cnt = (ctypes.c_int64 * len(parts))()
+        cabi.ok(cabi.lib.pomx_pair_terms(cabi.ctx, flags, len(parts), arr, ctypes.c_double(res), ctypes.c_double(tol),
+                                         ctypes.byref(d_r), ctypes.byref(d_p), cnt, None))
+        n = sum(cnt)
+        R = cabi.get(d_r, np.zeros(max(n, 1)))[:n]
+        P = cabi.get(d_p, np.zeros(max(n, 1)))[:n]
+        cabi.lib.pomx_free(cabi.ctx, d_r)
+        cabi.lib.pomx_free(cabi.ctx, d_p)
+        out.append((list(cnt), R, P))
+    assert sum(out[0][0]) > 0 and out[0][0][2] == 0
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
+
+
 @pytest.mark.parametrize("kind", [0, 1])
 def test_pair_eval(cabi, kind):
     rng = np.random.default_rng(5)
