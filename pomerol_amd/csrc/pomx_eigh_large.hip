@@ -291,7 +291,8 @@ __global__ void __launch_bounds__(256) trd_refl_kernel(const BigBlk* __restrict_
 __global__ void __launch_bounds__(256) trd_symv_kernel(const BigBlk* __restrict__ blks, int j, const double* __restrict__ Aar,
                                                        double* __restrict__ D) {
     __shared__ double vc[kSymChunk];
-    __shared__ double cred[4][kSymChunk];
+    // the waves' column partials; before them (same storage) every lane's row partials, [row][lane]
+    __shared__ double cred[kSymTile * 65 > 4 * kSymChunk ? kSymTile * 65 : 4 * kSymChunk];
     BigBlk const B = blks[blockIdx.y];
     int const n = B.n;
     DOff const o = doff(n, B.gsym);
@@ -342,23 +343,32 @@ __global__ void __launch_bounds__(256) trd_symv_kernel(const BigBlk* __restrict_
                 cacc[u] = fma(c > ra ? xa[u] : 0.0, va, cacc[u]);
                 cacc[u] = fma(c > rb ? xb[u] : 0.0, vb, cacc[u]);
             }
-            sa = wave_sum(sa);
-            sb = wave_sum(sb);
-            if(lane == 0) {
-                double* rp = base + o.rp + (int64_t(t) * ncb + k) * kSymTile;
-                rp[ra - R0] = sa;
-                if(hb)
-                    rp[rb - R0] = sb;
-            }
+            // lane partials of the row sums (reduced across lanes once per tile, below)
+            cred[(ra - R0) * 65 + lane] = sa;
+            cred[(rb - R0) * 65 + lane] = sb;
         }
+        __syncthreads();
+        { // row sums: 4 threads per row, 16 lanes each (rows past the matrix are not written)
+            int const row = threadIdx.x >> 2, part = threadIdx.x & 3;
+            double s4 = 0.0;
+#pragma unroll
+            for(int l = 0; l < 16; ++l)
+                s4 += cred[row * 65 + 16 * part + l];
+            s4 += __shfl_xor(s4, 1, 64);
+            s4 += __shfl_xor(s4, 2, 64);
+            if(part == 0 && R0 + row < m)
+                base[o.rp + (int64_t(t) * ncb + k) * kSymTile + row] = s4;
+        }
+        __syncthreads();
 #pragma unroll
         for(int u = 0; u < U; ++u)
-            cred[wave][lane + 64 * u] = cacc[u];
+            cred[wave * kSymChunk + lane + 64 * u] = cacc[u];
         __syncthreads();
         for(int c = threadIdx.x; c < kSymChunk; c += 256) {
             int const cc = C0 + c;
             if(cc < m)
-                base[o.cp + int64_t(t) * n + cc] = (cred[0][c] + cred[1][c]) + (cred[2][c] + cred[3][c]);
+                base[o.cp + int64_t(t) * n + cc] =
+                    (cred[c] + cred[kSymChunk + c]) + (cred[2 * kSymChunk + c] + cred[3 * kSymChunk + c]);
         }
     }
 }
