@@ -39,6 +39,9 @@ constexpr int kDotParts = 64; // row chunks of the panel dot products W^T v, V^T
 constexpr int kBT = 128;       // reflectors per block of the back-transformation (4 panels)
 constexpr int kSymTile = 64;   // rows per tile of the symmetric matrix-vector product
 constexpr int kSymChunk = 512; // columns per work item of the symmetric matrix-vector product
+// rows of a wave whose loads are in flight together (4: 138 registers, three waves per SIMD;
+// 1317 / 1274 / 1352 ms per config-4 batch with 2 / 4 / 8)
+constexpr int kSymRows = 4;
 
 struct BigBlk {
     int n, gsym;              // block size; symv grid width (partial sums per block)
@@ -316,36 +319,39 @@ __global__ void __launch_bounds__(256) trd_symv_kernel(const BigBlk* __restrict_
 #pragma unroll
         for(int u = 0; u < U; ++u)
             cacc[u] = 0.0;
-        // rows wave + 8 h and wave + 8 h + 4 of the tile, h = 0..7
-        for(int h = 0; h < kSymTile / 8; ++h) {
-            int const ra = R0 + wave + 8 * h, rb = ra + 4;
-            if(ra >= m)
+        // rows wave + 4 i of the tile, i = 0..15, kSymRows of them per step (all their loads in
+        // flight together)
+        constexpr int NRW = kSymRows;
+        for(int g = 0; g < kSymTile / (4 * NRW); ++g) {
+            int const r0 = R0 + wave + 4 * NRW * g;
+            if(r0 >= m)
                 break;
-            bool const hb = rb < m;
-            const double* __restrict__ a = Aar + B.mat_off + int64_t(jb + ra) * n + jb;
-            const double* __restrict__ b = a + int64_t(4) * n;
-            double xa[U], xb[U];
+            double x[NRW][U];
 #pragma unroll
-            for(int u = 0; u < U; ++u) {
-                int const c = C0 + lane + 64 * u;
-                xa[u] = (c < m && c >= ra) ? a[c] : 0.0;
-                xb[u] = (hb && c < m && c >= rb) ? b[c] : 0.0;
-            }
-            double const va = v[ra], vb = hb ? v[rb] : 0.0;
-            double sa = 0.0, sb = 0.0;
+            for(int rr = 0; rr < NRW; ++rr) {
+                int const r = r0 + 4 * rr;
+                const double* __restrict__ a = Aar + B.mat_off + int64_t(jb + r) * n + jb;
 #pragma unroll
-            for(int u = 0; u < U; ++u) {
-                int const c = C0 + lane + 64 * u;
-                double const vv = vc[lane + 64 * u];
-                sa = fma(xa[u], vv, sa);
-                sb = fma(xb[u], vv, sb);
-                // column parts (c > r); the diagonal element counts once, in the row part
-                cacc[u] = fma(c > ra ? xa[u] : 0.0, va, cacc[u]);
-                cacc[u] = fma(c > rb ? xb[u] : 0.0, vb, cacc[u]);
+                for(int u = 0; u < U; ++u) {
+                    int const c = C0 + lane + 64 * u;
+                    x[rr][u] = (r < m && c < m && c >= r) ? a[c] : 0.0;
+                }
             }
-            // lane partials of the row sums (reduced across lanes once per tile, below)
-            cred[(ra - R0) * 65 + lane] = sa;
-            cred[(rb - R0) * 65 + lane] = sb;
+#pragma unroll
+            for(int rr = 0; rr < NRW; ++rr) {
+                int const r = r0 + 4 * rr;
+                double const vr = r < m ? v[r] : 0.0;
+                double sr = 0.0;
+#pragma unroll
+                for(int u = 0; u < U; ++u) {
+                    int const c = C0 + lane + 64 * u;
+                    sr = fma(x[rr][u], vc[lane + 64 * u], sr);
+                    // column parts (c > r); the diagonal element counts once, in the row part
+                    cacc[u] = fma(c > r ? x[rr][u] : 0.0, vr, cacc[u]);
+                }
+                // lane partials of the row sums (reduced across lanes once per tile, below)
+                cred[(r - R0) * 65 + lane] = sr;
+            }
         }
         __syncthreads();
         { // row sums: 4 threads per row, 16 lanes each (rows past the matrix are not written)
