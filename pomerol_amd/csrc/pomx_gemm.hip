@@ -69,6 +69,26 @@ __device__ inline void gemm_tile(pomx_gemm_job const& J, int const i0, int const
             acc[x][y] = dbl4{0.0, 0.0, 0.0, 0.0};
     double ra[NE], rb[NE];
     load(0, ra, rb);
+    double* __restrict__ c = C + J.c_off;
+    double const alpha = J.alpha, beta = J.beta;
+    // 64 x 64 tiles with beta != 0 (updates, often with a short K): the C tile is loaded before
+    // the products, so its latency overlaps them
+    constexpr bool PRE = TT == 64;
+    double cpre[PRE ? NBM : 1][PRE ? NBN : 1][4];
+    if constexpr(PRE) {
+        if(beta != 0.0) {
+#pragma unroll
+            for(int mt = 0; mt < NBM; ++mt)
+#pragma unroll
+                for(int nt = 0; nt < NBN; ++nt)
+#pragma unroll
+                    for(int r = 0; r < 4; ++r) {
+                        int const i = i0 + wm + mt * 16 + (lane >> 4) + 4 * r;
+                        int const j = j0 + wn + nt * 16 + (lane & 15);
+                        cpre[mt][nt][r] = (i < M && j < N) ? c[int64_t(i) * J.ldc + j] : 0.0;
+                    }
+        }
+    }
     for(int t0 = 0; t0 < K; t0 += 16) {
         __syncthreads();
 #pragma unroll
@@ -96,8 +116,6 @@ __device__ inline void gemm_tile(pomx_gemm_job const& J, int const i0, int const
                     acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv[n], acc[m][n], 0, 0, 0);
         }
     }
-    double* __restrict__ c = C + J.c_off;
-    double const alpha = J.alpha, beta = J.beta;
 #pragma unroll
     for(int mt = 0; mt < NBM; ++mt)
 #pragma unroll
@@ -110,7 +128,10 @@ __device__ inline void gemm_tile(pomx_gemm_job const& J, int const i0, int const
                 if(i < M && j < N) {
                     double* p = c + int64_t(i) * J.ldc + j;
                     double v = alpha * acc[mt][nt][r];
-                    *p = (beta == 0.0) ? v : fma(beta, *p, v);
+                    if constexpr(PRE)
+                        *p = (beta == 0.0) ? v : fma(beta, cpre[mt][nt][r], v);
+                    else
+                        *p = (beta == 0.0) ? v : fma(beta, *p, v);
                 }
             }
 }
