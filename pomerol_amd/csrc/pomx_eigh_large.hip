@@ -55,7 +55,11 @@ constexpr int kBand = kNB;         // semi-bandwidth of the intermediate band ma
 constexpr int kLQT = 256;          // panel columns per workgroup of the panel LQ kernel
 constexpr int kGramW = 128;        // columns per workgroup of the 32 x 32 Gram partial sums
 static_assert(kLQT >= 7 * 33 && kLQT % 64 == 0, "the LQ kernel finishes 33 partial sums with 7 threads each");
-constexpr int kXvSplit = 4;        // row ranges of the stage-1 product X^T = V^T A22 (partial sums)
+// row ranges of the stage-1 product X^T = V^T A22 (partial sums; the plan takes min(this,
+// smallest block / kBand) so that the partials fit U; 16 against 4: 1.887 against 1.917 s at
+// n = 15876)
+constexpr int kXvSplitMax = 16;
+static_assert(kXvSplitMax * kBand * kBand <= kBT * kBT, "T4 fits the S scratch");
 constexpr int kQ2G = 32;           // bulge-chasing sweeps per group of the Q2 back-transformation
 constexpr int kQ2Waves = 12;       // waves (groups in flight) per workgroup of the Q2 kernel
 constexpr int kBtSplitS = 16;      // two-stage back-transformation: row ranges of S = V^T V
@@ -1227,13 +1231,13 @@ __global__ void __launch_bounds__(256) gram_kernel(const BigBlk* __restrict__ bl
 
 // Stage 1: X^T = Vt A22 (kBand x m) from the upper triangle of A22 only, A(r, c) =
 // A22[min(r, c)][max(r, c)].  Workgroup (column tile c0 of 64, block, split z) sums the rows r of
-// one of kXvSplit contiguous ranges, KC at a time: rows above the tile read A22[r][c0..c0+63]
+// one of xs contiguous ranges, KC at a time: rows above the tile read A22[r][c0..c0+63]
 // (row segments), rows below it A22[c0+j][r..r+KC) (the transposed element, also contiguous), and
 // the chunks crossing the diagonal element by element.  The product runs on the matrix cores
 // (v_mfma_f64_16x16x4f64, each wave 16 x 32 of the 32 x 64 tile), the next chunk prefetched
-// into registers.  Split z writes its partial sum to rows [32 z, 32 z + 32) of W1; the next
-// product sums the splits (its K runs over all 4 x 32 rows).
-__global__ void __launch_bounds__(256) xv_kernel(const BigBlk* __restrict__ blks, int k, const double* __restrict__ Aar,
+// into registers.  Split z writes its partial sum to rows [32 z, 32 z + 32) of U (free until the
+// divide and conquer); the next product sums the splits (its K runs over all xs x 32 rows).
+__global__ void __launch_bounds__(256) xv_kernel(const BigBlk* __restrict__ blks, int k, int xs, const double* __restrict__ Aar,
                                                  double* __restrict__ D) {
     constexpr int KC = 32;
     __shared__ double Ts[KC][64 + 1];
@@ -1246,9 +1250,9 @@ __global__ void __launch_bounds__(256) xv_kernel(const BigBlk* __restrict__ blks
     DOff const o = doff(n, B.gsym);
     const double* __restrict__ A = Aar + B.mat_off + int64_t(k1) * n + k1;
     const double* __restrict__ Vt = D + B.dbase + o.Vt + k1;
-    double* __restrict__ out = D + B.dbase + o.W1 + int64_t(blockIdx.z) * kBand * n + k1;
+    double* __restrict__ out = D + B.dbase + o.U + int64_t(blockIdx.z) * kBand * n + k1;
     int const nchunk = (m + KC - 1) / KC;
-    int const per = (nchunk + kXvSplit - 1) / kXvSplit;
+    int const per = (nchunk + xs - 1) / xs;
     int const ch0 = blockIdx.z * per, ch1 = min(nchunk, ch0 + per);
     int const tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int const wm = (wave >> 1) * 16, wn = (wave & 1) * 32;
@@ -1322,7 +1326,8 @@ __global__ void __launch_bounds__(256) xv_kernel(const BigBlk* __restrict__ blks
 // Q = I - V T V^T = H_0 ... H_{kBand-1} (forward, upper; T[:i, i] = -tau_i T[:i, :i] S[:i, i],
 // T_ii = tau_i) to o.T; which 1: N = T^T M (M = V^T W) to o.S.  Both kBand x kBand, leading
 // dimension kBand.
-__global__ void __launch_bounds__(256) panel_t_kernel(const BigBlk* __restrict__ blks, int k, int which, double* __restrict__ D) {
+__global__ void __launch_bounds__(256) panel_t_kernel(const BigBlk* __restrict__ blks, int k, int which, int xs,
+                                                      double* __restrict__ D) {
     __shared__ double S[kBand][kBand + 1], T[kBand][kBand + 1];
     BigBlk const B = blks[blockIdx.y];
     int const n = B.n, m = n - k - kBand;
@@ -1367,10 +1372,10 @@ __global__ void __launch_bounds__(256) panel_t_kernel(const BigBlk* __restrict__
         __syncthreads();
         for(int e = tid; e < kBand * kBand; e += 256)
             base[o.T + e] = T[e / kBand][e % kBand];
-        // T4 = [T^T T^T T^T T^T] (kBand x kXvSplit kBand) to o.S: W^T = T4 [X_0; ..; X_3] sums the
-        // split partial products of X^T
-        for(int e = tid; e < kBand * kXvSplit * kBand; e += 256) {
-            int const p = e / (kXvSplit * kBand), q = e % kBand;
+        // T4 = [T^T ... T^T] (kBand x xs kBand) to o.S: W^T = T4 [X_0; X_1; ..] sums the split
+        // partial products of X^T
+        for(int e = tid; e < kBand * xs * kBand; e += 256) {
+            int const p = e / (xs * kBand), q = e % kBand;
             base[o.S + e] = T[q][p];
         }
     } else {
@@ -1756,6 +1761,7 @@ struct BigPlan {
     size_t bytes = 0;
     // two-stage reduction (dense -> band -> tridiagonal) instead of the one-stage one
     bool two_stage = false;
+    int xsplit = 1; // row ranges of the stage-1 product X^T = V^T A22
     struct Panel2 { // stage-1 panel k: blocks [0, nact) active; GEMM jobs X, W, Z, trailing (nact each)
         int k, nact, nref_max, nchunk, ngram, mmax;
         int64_t jobs_off;
@@ -1794,6 +1800,8 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return n_in[a] > n_in[b]; });
     P->nblk = nblk;
     P->nmax = nblk ? n_in[ord[0]] : 0;
+    if(nblk) // partial products of X^T: xsplit kBand rows of n in U (n^2 doubles) per block
+        P->xsplit = std::max(1, std::min(kXvSplitMax, n_in[ord[nblk - 1]] / kBand));
     P->gsym = std::max(1, std::min(kMaxSymvGrid, (P->nmax - 1 + 15) / 16));
     if(nblk > 65535) {
         delete P;
@@ -1822,7 +1830,7 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
     std::vector<pomx_gemm_job> gj;
     if(P->two_stage) {
         // stage-1 panels: per active block (m = n - k - kBand >= 2; a prefix, blocks largest first)
-        // the jobs W^T = T4 [X_0; ..; X_3] (the split partial products), W -= N^T Vt / 2, and
+        // the jobs W^T = T4 [X_0; X_1; ..] (the xsplit partial products), W -= N^T Vt / 2, and
         // A22 -= [W Vt]^T [Vt W3] on the upper tiles
         for(int k = 0; P->nmax - k - kBand >= 2; k += kBand) {
             BigPlan::Panel2 Q{k, 0, 0, 0, 0, P->nmax - k - kBand, int64_t(gj.size())};
@@ -1838,8 +1846,8 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
                     DOff const o = doff(n, B.gsym);
                     int64_t const a22 = B.mat_off + int64_t(k1) * n + k1;
                     if(kind == 1)
-                        gj.push_back(pomx_gemm_job{kBand, m, kXvSplit * kBand, 0, 0, kXvSplit * kBand, n, n, B.dbase + o.S,
-                                                   B.dbase + o.W1 + k1, B.dbase + o.W + k1, 1.0, 0.0});
+                        gj.push_back(pomx_gemm_job{kBand, m, P->xsplit * kBand, 0, 0, P->xsplit * kBand, n, n, B.dbase + o.S,
+                                                   B.dbase + o.U + k1, B.dbase + o.W + k1, 1.0, 0.0});
                     else if(kind == 2)
                         gj.push_back(pomx_gemm_job{kBand, m, kBand, 1, 0, kBand, n, n, B.dbase + o.S, B.dbase + o.Vt + k1,
                                                    B.dbase + o.W + k1, -0.5, 1.0});
@@ -2032,16 +2040,16 @@ int run_two_stage(pomx_ctx* ctx, BigPlan* P, double* Aar) {
         for(int i = 0; i <= Q.nref_max; ++i)
             lq_kernel<<<dim3(unsigned(Q.nchunk), na), kLQT, 0, s>>>(P->blk, Q.k, i, Aar, P->D);
         gram_kernel<<<dim3(unsigned(Q.ngram), na), 256, 0, s>>>(P->blk, Q.k, 0, P->D);
-        panel_t_kernel<<<dim3(1, na), 256, 0, s>>>(P->blk, Q.k, 0, P->D);
+        panel_t_kernel<<<dim3(1, na), 256, 0, s>>>(P->blk, Q.k, 0, P->xsplit, P->D);
         POMX_CHECK_LAUNCH(ctx);
         int64_t const tc = (Q.mmax + 63) / 64;
         pomx_gemm_job const* J = P->gjobs + Q.jobs_off;
-        xv_kernel<<<dim3(unsigned(tc), na, unsigned(kXvSplit)), 256, 0, s>>>(P->blk, Q.k, Aar, P->D);
+        xv_kernel<<<dim3(unsigned(tc), na, unsigned(P->xsplit)), 256, 0, s>>>(P->blk, Q.k, P->xsplit, Aar, P->D);
         POMX_CHECK_LAUNCH(ctx);
         if((rc = launch_gemm_jobs(ctx, 0, 0, J, Q.nact, tc, P->D, P->D, P->D)))
             return rc;
         gram_kernel<<<dim3(unsigned(Q.ngram), na), 256, 0, s>>>(P->blk, Q.k, 1, P->D);
-        panel_t_kernel<<<dim3(1, na), 256, 0, s>>>(P->blk, Q.k, 1, P->D);
+        panel_t_kernel<<<dim3(1, na), 256, 0, s>>>(P->blk, Q.k, 1, P->xsplit, P->D);
         POMX_CHECK_LAUNCH(ctx);
         if((rc = launch_gemm_jobs(ctx, 1, 0, J + Q.nact, Q.nact, tc, P->D, P->D, P->D)))
             return rc;
