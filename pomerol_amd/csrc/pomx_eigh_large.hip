@@ -624,6 +624,11 @@ __global__ void __launch_bounds__(256) dc_deflate_kernel(const BigBlk* __restric
     double* zl = base + od.zl + o;
     int* perm = I + B.ibase + oi.perm + o;
     int* order = I + B.ibase + oi.order + o;
+    // column types by sorted position (1: rows of the first half only, 3: second half only, 2:
+    // both, after a rotation that mixed the halves); the org array is free until the secular
+    // equations.  sig: type-sorted position of each kept column (fperm is free until the ranking).
+    int* typ = I + B.ibase + oi.org + o;
+    int* sig = I + B.ibase + oi.fperm + o;
     DcRot* rot = R + B.rbase + o;
     double const rho_signed = base[od.e + o + n1 - 1];
     double const sgn = rho_signed >= 0.0 ? 1.0 : -1.0;
@@ -658,6 +663,7 @@ __global__ void __launch_bounds__(256) dc_deflate_kernel(const BigBlk* __restric
             z = sgn * Z[int64_t(o + n1) * NB + o + c];
         }
         perm[pos] = c;
+        typ[pos] = c < n1 ? 1 : 3;
         ds[pos] = x;
         zs[pos] = z;
     }
@@ -704,6 +710,8 @@ __global__ void __launch_bounds__(256) dc_deflate_kernel(const BigBlk* __restric
                 zs[c] = t2;
                 zs[pj] = 0.0;
                 rot[nrot++] = DcRot{pj, c, cc, ss};
+                if(typ[c] != typ[pj])
+                    typ[c] = 2;
                 double const dp = ds[pj] * cc * cc + ds[c] * ss * ss;
                 double const dq = ds[pj] * ss * ss + ds[c] * cc * cc;
                 ds[pj] = dp;
@@ -724,10 +732,32 @@ __global__ void __launch_bounds__(256) dc_deflate_kernel(const BigBlk* __restric
             ++nk;
         }
     }
+    // The kept columns (Z after the gather by order) sorted by type, first half only, mixed,
+    // second half only: C = Z U splits into the first half's rows times the first t1 + tm
+    // columns and the second half's rows times the last tm + t3 (the other entries of Z are
+    // zero), half the product of the dense n x nk x nk.  U's rows follow (dc_u_kernel, sig).
+    int t1 = 0, tm = 0;
+    for(int i = 0; i < nk; ++i) {
+        int const ty = typ[order[i]];
+        t1 += ty == 1;
+        tm += ty == 2;
+    }
+    int const t3 = nk - t1 - tm;
+    for(int i = 0, a = 0, bm = t1, c3 = t1 + tm; i < nk; ++i) {
+        int const ty = typ[order[i]];
+        sig[i] = ty == 1 ? a++ : ty == 2 ? bm++ : c3++;
+    }
+    for(int i = 0; i < nk; ++i) // (typ is no longer needed: the order's copy)
+        typ[i] = order[i];
+    for(int i = 0; i < nk; ++i)
+        order[sig[i]] = typ[i];
     st[blockIdx.y] = DcState{nk, nrot, rho};
     int64_t const sq = int64_t(o) * NB + o;
-    jobs[blockIdx.y] = pomx_gemm_job{n, nk, nk, 0, 0, NB, NB, NB, B.dbase + od.Z + sq, B.dbase + od.U + sq, B.dbase + od.Q2 + sq,
-                                     1.0, 0.0};
+    jobs[2 * blockIdx.y] = pomx_gemm_job{n1, nk, t1 + tm, 0, 0, NB, NB, NB, B.dbase + od.Z + sq, B.dbase + od.U + sq,
+                                         B.dbase + od.Q2 + sq, 1.0, 0.0};
+    jobs[2 * blockIdx.y + 1] = pomx_gemm_job{n - n1, nk, tm + t3, 0, 0, NB, NB, NB, B.dbase + od.Z + sq + int64_t(n1) * NB + t1,
+                                             B.dbase + od.U + sq + int64_t(t1) * NB, B.dbase + od.Q2 + sq + int64_t(n1) * NB,
+                                             1.0, 0.0};
 }
 
 // dst[o+r][o+c] = src[o+r][o+idx[c]] over each merge's square (which: 0 perm Z->Q2, 1 order
@@ -895,10 +925,11 @@ __global__ void dc_u_kernel(const BigBlk* __restrict__ blks, const DcNode* __res
     double* base = D + B.dbase;
     const double* d = base + od.dl + N.o;
     const int* og = I + B.ibase + ioff(NB).org + N.o;
+    const int* sg = I + B.ibase + ioff(NB).fperm + N.o; // row of U for kept column i (type-sorted)
     int64_t const cnt = int64_t(k) * k;
     for(int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < cnt; q += int64_t(gridDim.x) * blockDim.x) {
         int const i = int(q / k), jj = int(q % k);
-        base[od.U + int64_t(N.o + i) * NB + N.o + jj] =
+        base[od.U + int64_t(N.o + sg[i]) * NB + N.o + jj] =
             base[od.zh + N.o + i] / ((d[i] - d[og[jj]]) - base[od.rt + N.o + jj]);
     }
 }
@@ -2006,7 +2037,7 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
     P->leaf_jobs = dupload(ctx, ljobs, s, ok);
     P->nodes = dupload(ctx, nodes, s, ok);
     std::vector<DcState> st0(nodes.size(), DcState{0, 0, 0.0}); // host sources live until the sync below
-    std::vector<pomx_gemm_job> jb0(nodes.size());
+    std::vector<pomx_gemm_job> jb0(2 * nodes.size()); // two products per merge (the halves' rows)
     P->states = dupload(ctx, st0, s, ok);
     P->dc_jobs = dupload(ctx, jb0, s, ok);
     auto alloc = [&](void** q, size_t bytes) {
@@ -2125,7 +2156,7 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
     for(auto const& L : P->levels) {
         DcNode const* nd = P->nodes + L.node_off;
         DcState* st = P->states + L.node_off;
-        pomx_gemm_job* jb = P->dc_jobs + L.node_off;
+        pomx_gemm_job* jb = P->dc_jobs + 2 * L.node_off; // two products per merge
         dim3 const gsq(gx_rows(int64_t(L.nmax) * L.nmax, 256, L.nnodes), unsigned(L.nnodes));
         dim3 const grow(gx_rows(L.nmax, 256, L.nnodes), unsigned(L.nnodes));
         dim3 const gwave(gx_rows(L.nmax, 4, L.nnodes), unsigned(L.nnodes));
@@ -2139,8 +2170,10 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
         dc_colnorm_kernel<<<grow, 256, 0, s>>>(P->blk, nd, st, P->D);
         POMX_CHECK_LAUNCH(ctx);
         int64_t const t = (L.nmax + 63) / 64;
-        if((rc = launch_gemm_jobs(ctx, 0, 0, jb, L.nnodes, t * t, P->D, P->D, P->D, false, L.nmax >= kBigGemmDc)))
-            return rc;
+        for(int j0 = 0; j0 < 2 * L.nnodes; j0 += 65534) // (at most 65535 jobs per launch)
+            if((rc = launch_gemm_jobs(ctx, 0, 0, jb + j0, std::min(65534, 2 * L.nnodes - j0), t * t, P->D, P->D, P->D, false,
+                                      L.nmax >= kBigGemmDc)))
+                return rc;
         dc_copycols_kernel<<<gsq, 256, 0, s>>>(P->blk, nd, st, P->D);
         dc_vals_kernel<<<grow, 256, 0, s>>>(P->blk, nd, st, P->D, P->I);
         dc_rank_kernel<<<grow, 256, 0, s>>>(P->blk, nd, P->D, P->I);
