@@ -64,6 +64,7 @@ constexpr int kQ2G = 32;           // bulge-chasing sweeps per group of the Q2 b
 constexpr int kQ2Waves = 12;       // waves (groups in flight) per workgroup of the Q2 kernel
 constexpr int kBtSplitS = 16;      // two-stage back-transformation: row ranges of S = V^T V
 constexpr int kBtSplitW = 4;       // and of W1 = V^T X (partial sums in the free D&C workspace U)
+constexpr int kBtBatchMin = 2048;  // two-stage plans of blocks this large: S and T of all panels first
 constexpr int kTwoStageMin = 2048; // blocks at least this large take the two-stage reduction
 // 128 x 128 GEMM tiles (eight waves) from these sizes on: divide-and-conquer merges of blocks
 // of at least kBigGemmDc rows, V^T X of the back-transformation in blocks of at least kBigGemmW1
@@ -1024,16 +1025,20 @@ __global__ void copy_z_kernel(const BigBlk* __restrict__ blks, const double* __r
 // Partial products of the split back-transformation summed in a fixed order, one thread per entry,
 // blocks listed in ids: which 0, S = sum of the ks products V_z^T V_z at U + z kBT^2 (row ranges z
 // of V); which 1, W1 = sum of the ks products V_z^T X_z at U + ks_S kBT^2 + z nb n.
+// batched (plans with bt_batch): the slots of panel j0 (S partials at U + (j0/kBT) ks kBT^2, S to
+// Z + (j0/kBT) kBT^2; W1 partials at U).
 __global__ void bt_sum_kernel(const BigBlk* __restrict__ blks, const int* __restrict__ ids, int j0, int shift, int which,
-                              int ks, double* __restrict__ D) {
+                              int ks, int batched, double* __restrict__ D) {
     BigBlk const B = blks[ids[blockIdx.y]];
     DOff const od = doff(B.n, B.gsym);
     double* __restrict__ base = D + B.dbase;
     int const n = B.n, nb = min(kBT, n - shift - j0);
+    int64_t const slot = j0 / kBT;
     int64_t const cnt = which == 0 ? int64_t(nb) * nb : int64_t(nb) * n;
-    int64_t const src = which == 0 ? od.U : od.U + int64_t(kBtSplitS) * kBT * kBT;
+    int64_t const src = which == 0 ? od.U + (batched ? slot * ks * kBT * kBT : 0)
+                                   : od.U + (batched ? 0 : int64_t(kBtSplitS) * kBT * kBT);
     int64_t const stride = which == 0 ? int64_t(kBT) * kBT : int64_t(nb) * n;
-    double* __restrict__ dst = base + (which == 0 ? od.S : od.W1);
+    double* __restrict__ dst = base + (which == 0 ? (batched ? od.Z + slot * kBT * kBT : od.S) : od.W1);
     for(int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < cnt; e += int64_t(gridDim.x) * blockDim.x) {
         double a = 0.0;
         for(int z = 0; z < ks; ++z)
@@ -1046,15 +1051,21 @@ __global__ void bt_sum_kernel(const BigBlk* __restrict__ blks, const int* __rest
 // workgroup of kBT threads per block: column i of T is -tau_i T[:i, :i] S[:i, i], T_ii = tau_i.
 // Thread r owns row r of T and builds it in place in the block's scratch (it only reads back
 // its own writes); the column factor is shared through LDS.
-__global__ void __launch_bounds__(kBT) wy_t_kernel(const BigBlk* __restrict__ blks, const int* __restrict__ ids, int j0,
-                                                   int shift, double* __restrict__ D) {
+// BATCH: one workgroup per (block, j0) of pairs, S and T in the panel's slots of Z (S at
+// Z + (j0/kBT) kBT^2, T at Z + (np + j0/kBT) kBT^2, np = panels of the block).
+template <bool BATCH>
+__global__ void __launch_bounds__(kBT) wy_t_kernel(const BigBlk* __restrict__ blks, const int* __restrict__ ids, int j0_,
+                                                   const int2* __restrict__ pairs, int shift, double* __restrict__ D) {
     __shared__ double col[kBT];
-    BigBlk const B = blks[ids[blockIdx.x]];
+    int const j0 = BATCH ? pairs[blockIdx.x].y : j0_;
+    BigBlk const B = blks[BATCH ? pairs[blockIdx.x].x : ids[blockIdx.x]];
     DOff const od = doff(B.n, B.gsym);
     double* base = D + B.dbase;
     int const nb = min(kBT, B.n - shift - j0); // reflector j acts on rows j+shift..
     int const r = threadIdx.x;
-    double* __restrict__ Tr = base + od.T + int64_t(r) * nb; // row r (valid for r < nb)
+    int64_t const slot = j0 / kBT, np = (B.n - shift + kBT - 1) / kBT;
+    int64_t const oS = BATCH ? od.Z + slot * kBT * kBT : od.S, oT = BATCH ? od.Z + (np + slot) * kBT * kBT : od.T;
+    double* __restrict__ Tr = base + oT + int64_t(r) * nb; // row r (valid for r < nb)
     if(r < nb)
         for(int c = 0; c < nb; ++c)
             Tr[c] = 0.0;
@@ -1062,7 +1073,7 @@ __global__ void __launch_bounds__(kBT) wy_t_kernel(const BigBlk* __restrict__ bl
         double const ti = base[od.tau + j0 + i];
         __syncthreads(); // readers of the previous column factor are done
         if(r < i)
-            col[r] = -ti * base[od.S + r * nb + i];
+            col[r] = -ti * base[oS + r * nb + i];
         __syncthreads();
         if(r < i) {
             // four independent chains (loads in flight), added in a fixed order
@@ -1789,6 +1800,16 @@ struct BigPlan {
         int64_t ids_off, jobs_off;
     };
     std::vector<BtPanel> bt;
+    // two-stage plans whose blocks all have kBtBatchMin rows or more: S = V^T V and T of every
+    // panel first (the products in one launch, T of all panels and blocks in one launch), then the
+    // updates of X panel by panel.  Slots: the partial products of S of panel j0 at
+    // U + (j0 / kBT) kBtSplitS kBT^2, S at Z + (j0 / kBT) kBT^2, T at Z + (np + j0 / kBT) kBT^2
+    // (np = panels of the block), the partial products of W1 at U.
+    bool bt_batch = false;
+    int64_t bta_off = 0; // S jobs of all panels
+    int bta_n = 0;
+    int2* pairs = nullptr; // (block, j0) of every panel's T
+    int npairs = 0;
     size_t bytes = 0;
     // two-stage reduction (dense -> band -> tridiagonal) instead of the one-stage one
     bool two_stage = false;
@@ -1809,7 +1830,7 @@ void big_plan_destroy(BigPlan* p) {
         return;
     big_plan_destroy(p->next);
     void* ptrs[] = {p->blk,       p->D,          p->I,       p->R,      p->ids,   p->gjobs,  p->leaves,
-                    p->leaf_jobs, p->leaf_arena, p->leaf_ws, p->status, p->nodes, p->states, p->dc_jobs};
+                    p->leaf_jobs, p->leaf_arena, p->leaf_ws, p->status, p->nodes, p->states, p->dc_jobs, p->pairs};
     for(void* q : ptrs)
         if(q)
             (void)hipFree(q);
@@ -1976,6 +1997,9 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
     // back-transformation panels, last to first; reflector j acts on rows j+sh.. (sh = 1, or kBand
     // after the two-stage reduction, whose reflectors are j = 0 .. n-kBand-2)
     int const sh = P->two_stage ? kBand : 1;
+    std::vector<pomx_gemm_job> gja; // batched S jobs, appended after the panels' jobs
+    std::vector<int2> pairs;
+    P->bt_batch = P->two_stage && nblk > 0 && P->h_blk[size_t(nblk - 1)].n >= kBtBatchMin;
     if(!tri_only && P->nmax - sh >= 1) {
         int const last = ((P->nmax - sh - 1) / kBT) * kBT;
         for(int j0 = last; j0 >= 0; j0 -= kBT) {
@@ -1995,6 +2019,10 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
             bool const split = P->two_stage && int64_t(nmin) * nmin >= int64_t(kBtSplitS) * kBT * kBT +
                                                                           int64_t(kBtSplitW) * kBT * nmin;
             int const ks = split ? kBtSplitS : 1, kw = split ? kBtSplitW : 1;
+            if(P->bt_batch && !split) {
+                delete P;
+                return pomx_fail(ctx, POMX_ERR_ARG, "large-block eigensolver: batched back-transformation without split");
+            }
             BigPlan::BtPanel T{j0, int(L.size()), 0, 0, ks, kw, int64_t(ids.size()), int64_t(gj.size())};
             for(int kind = 0; kind < 4; ++kind) // jobs grouped by kind: S, W1, W2, X
                 for(int z = 0; z < (kind == 0 ? ks : kind == 1 ? kw : 1); ++z)
@@ -2003,12 +2031,19 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
                         int const n = B.n, nb = std::min(kBT, n - sh - j0), rows = n - j0 - sh;
                         DOff const o = doff(n, B.gsym);
                         int64_t const voff = B.dbase + o.V + int64_t(j0 + sh) * n + j0;
-                        int64_t const w1 = kw > 1 ? B.dbase + o.U + int64_t(ks) * kBT * kBT : B.dbase + o.W1;
+                        int64_t const slot = j0 / kBT, np = (n - sh + kBT - 1) / kBT;
+                        int64_t const w1 = P->bt_batch ? B.dbase + o.U
+                                                       : (kw > 1 ? B.dbase + o.U + int64_t(ks) * kBT * kBT : B.dbase + o.W1);
                         if(kind == 0) {
                             int const rc = (rows + ks - 1) / ks, r0 = std::min(rows, z * rc), r1 = std::min(rows, r0 + rc);
-                            int64_t const out = ks > 1 ? B.dbase + o.U + int64_t(z) * kBT * kBT : B.dbase + o.S;
-                            gj.push_back(pomx_gemm_job{nb, nb, r1 - r0, 1, 0, n, n, nb, voff + int64_t(r0) * n,
-                                                       voff + int64_t(r0) * n, out, 1.0, 0.0});
+                            int64_t const out = P->bt_batch ? B.dbase + o.U + (slot * ks + z) * kBT * kBT
+                                                : ks > 1     ? B.dbase + o.U + int64_t(z) * kBT * kBT
+                                                             : B.dbase + o.S;
+                            (P->bt_batch ? gja : gj)
+                                .push_back(pomx_gemm_job{nb, nb, r1 - r0, 1, 0, n, n, nb, voff + int64_t(r0) * n,
+                                                         voff + int64_t(r0) * n, out, 1.0, 0.0});
+                            if(z == 0 && P->bt_batch)
+                                pairs.push_back(int2{b, j0});
                             if(z == 0) {
                                 ids.push_back(b);
                                 T.nb_max = std::max(T.nb_max, nb);
@@ -2020,8 +2055,8 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
                                                        B.mat_off + int64_t(j0 + sh + r0) * n, w1 + int64_t(z) * nb * n,
                                                        1.0, 0.0});
                         } else if(kind == 2) { // (split: W1 summed from its partial products first)
-                            gj.push_back(pomx_gemm_job{nb, n, nb, 0, 0, nb, n, n, B.dbase + o.T, B.dbase + o.W1,
-                                                       B.dbase + o.W2, 1.0, 0.0});
+                            int64_t const tt = P->bt_batch ? B.dbase + o.Z + (np + slot) * kBT * kBT : B.dbase + o.T;
+                            gj.push_back(pomx_gemm_job{nb, n, nb, 0, 0, nb, n, n, tt, B.dbase + o.W1, B.dbase + o.W2, 1.0, 0.0});
                         } else
                             gj.push_back(pomx_gemm_job{rows, n, nb, 0, 0, n, n, n, voff, B.dbase + o.W2,
                                                        B.mat_off + int64_t(j0 + sh) * n, -1.0, 1.0});
@@ -2029,7 +2064,12 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
             P->bt.push_back(T);
         }
     }
+    P->bta_off = int64_t(gj.size());
+    P->bta_n = int(gja.size());
+    gj.insert(gj.end(), gja.begin(), gja.end());
+    P->npairs = int(pairs.size());
     bool ok = true;
+    P->pairs = pairs.empty() ? nullptr : dupload(ctx, pairs, s, ok);
     P->blk = dupload(ctx, P->h_blk, s, ok);
     P->ids = dupload(ctx, ids, s, ok);
     P->gjobs = dupload(ctx, gj, s, ok);
@@ -2189,22 +2229,36 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
         }
         int const sh = P->two_stage ? kBand : 1;
         int64_t const tn = (P->nmax + 63) / 64;
+        int const bb = P->bt_batch ? 1 : 0;
+        if(P->bt_batch) { // S and T of every panel first: they do not depend on X
+            for(int64_t j0 = 0; j0 < P->bta_n; j0 += 65535)
+                if((rc = launch_gemm_jobs(ctx, 1, 0, P->gjobs + P->bta_off + j0, int(std::min<int64_t>(65535, P->bta_n - j0)),
+                                          4, P->D, P->D, P->D)))
+                    return rc;
+            for(auto const& T : P->bt)
+                bt_sum_kernel<<<dim3(unsigned((T.nb_max * T.nb_max + 255) / 256), unsigned(T.nblk)), 256, 0, s>>>(
+                    P->blk, P->ids + T.ids_off, T.j0, sh, 0, T.ks, 1, P->D);
+            wy_t_kernel<true><<<unsigned(P->npairs), kBT, 0, s>>>(P->blk, nullptr, 0, P->pairs, sh, P->D);
+            POMX_CHECK_LAUNCH(ctx);
+        }
         for(auto const& T : P->bt) {
             pomx_gemm_job const* J = P->gjobs + T.jobs_off;
-            int const ns = T.ks * T.nblk, nw = T.kw * T.nblk;
+            int const ns = P->bt_batch ? 0 : T.ks * T.nblk, nw = T.kw * T.nblk;
             int64_t const tb = (T.nb_max + 63) / 64;
             int64_t const tr = (T.rows_max + 63) / 64;
-            if((rc = launch_gemm_jobs(ctx, 1, 0, J, ns, tb * tb, P->D, P->D, P->D)))
-                return rc;
-            if(T.ks > 1)
-                bt_sum_kernel<<<dim3(unsigned((T.nb_max * T.nb_max + 255) / 256), unsigned(T.nblk)), 256, 0, s>>>(
-                    P->blk, P->ids + T.ids_off, T.j0, sh, 0, T.ks, P->D);
-            wy_t_kernel<<<unsigned(T.nblk), kBT, 0, s>>>(P->blk, P->ids + T.ids_off, T.j0, sh, P->D);
+            if(!P->bt_batch) {
+                if((rc = launch_gemm_jobs(ctx, 1, 0, J, ns, tb * tb, P->D, P->D, P->D)))
+                    return rc;
+                if(T.ks > 1)
+                    bt_sum_kernel<<<dim3(unsigned((T.nb_max * T.nb_max + 255) / 256), unsigned(T.nblk)), 256, 0, s>>>(
+                        P->blk, P->ids + T.ids_off, T.j0, sh, 0, T.ks, 0, P->D);
+                wy_t_kernel<false><<<unsigned(T.nblk), kBT, 0, s>>>(P->blk, P->ids + T.ids_off, T.j0, nullptr, sh, P->D);
+            }
             if((rc = launch_gemm_jobs(ctx, 1, 0, J + ns, nw, tb * tn, P->D, Aar, P->D, false, P->nmax >= kBigGemmW1)))
                 return rc;
             if(T.kw > 1)
                 bt_sum_kernel<<<dim3(gx_rows(int64_t(T.nb_max) * P->nmax, 256, T.nblk), unsigned(T.nblk)), 256, 0, s>>>(
-                    P->blk, P->ids + T.ids_off, T.j0, sh, 1, T.kw, P->D);
+                    P->blk, P->ids + T.ids_off, T.j0, sh, 1, T.kw, bb, P->D);
             if((rc = launch_gemm_jobs(ctx, 0, 0, J + ns + nw, T.nblk, tb * tn, P->D, P->D, P->D)) ||
                (rc = launch_gemm_jobs(ctx, 0, 0, J + ns + nw + T.nblk, T.nblk, tr * tn, P->D, P->D, Aar, false,
                                       T.rows_max >= kBigGemmX && P->nmax >= kBigGemmX)))
