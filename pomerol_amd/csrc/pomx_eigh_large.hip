@@ -38,10 +38,11 @@ constexpr int kMaxSymvGrid = 1024;
 constexpr int kDotParts = 64; // row chunks of the panel dot products W^T v, V^T v
 constexpr int kBT = 128;       // reflectors per block of the back-transformation (4 panels)
 constexpr int kSymTile = 64;   // rows per tile of the symmetric matrix-vector product
-constexpr int kSymChunk = 512; // columns per work item of the symmetric matrix-vector product
-// rows of a wave whose loads are in flight together (4: 138 registers, three waves per SIMD;
-// 1317 / 1274 / 1352 ms per config-4 batch with 2 / 4 / 8)
-constexpr int kSymRows = 4;
+constexpr int kSymChunk = 256; // columns per work item of the symmetric matrix-vector product
+// rows of a wave whose loads are in flight together.  With 256-column items, 8 rows (128
+// registers, four waves per SIMD): config-4 batch trd_symv 446 ms against 528 ms with 512-column
+// items and 4 rows (same box); 512 columns with 2 / 4 / 8 rows: 1317 / 1274 / 1352 ms per batch
+constexpr int kSymRows = 8;
 
 struct BigBlk {
     int n, gsym;              // block size; symv grid width (partial sums per block)
