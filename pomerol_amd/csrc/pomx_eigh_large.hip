@@ -292,11 +292,11 @@ __global__ void __launch_bounds__(256) trd_refl_kernel(const BigBlk* __restrict_
 }
 
 // w_raw = A22 v from the upper triangle of A22 only (A22 symmetric; its lower triangle is never
-// updated): work item (row tile t, column chunk k) covers rows [64t, 64t+64) and columns
-// [64t + 512k, +512) on or above the diagonal; it adds A[r][c] v[c] to the row partial of r and
-// A[r][c] v[r] (c > r) to the column partial of c.  Only the item's 512 entries of v are staged
-// in LDS (about 20 KB per workgroup, so several workgroups share a CU and keep enough loads in
-// flight); each wave streams two 4 KB row segments at a time.
+// updated): work item (row tile t, column chunk k) covers rows [kSymTile t, +kSymTile) and
+// columns [kSymTile t + kSymChunk k, +kSymChunk) on or above the diagonal; it adds A[r][c] v[c]
+// to the row partial of r and A[r][c] v[r] (c > r) to the column partial of c.  Only the item's
+// kSymChunk entries of v are staged in LDS; each wave has the row segments of kSymRows rows in
+// flight at a time, and the lanes' partial row sums are reduced once per item through LDS.
 __global__ void __launch_bounds__(256) trd_symv_kernel(const BigBlk* __restrict__ blks, int j, const double* __restrict__ Aar,
                                                        double* __restrict__ D) {
     __shared__ double vc[kSymChunk];

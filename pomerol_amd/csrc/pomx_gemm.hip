@@ -2,9 +2,11 @@
 // independent jobs of arbitrary shapes (row-major operands at offsets into three arenas).
 //
 // Used by the adjoint (U^T Ubar, U K, X Abar^T, U_t Abar over every block / block pair) and by
-// the large-block eigensolver.  Tile: 64 x 64 outputs per workgroup of four waves, each wave
-// 32 x 32 as 2 x 2 v_mfma_f64_16x16x4f64 accumulators; K advances 16 at a time through LDS,
-// with the next K slab prefetched into registers while the current one is multiplied.
+// the large-block eigensolver.  Tiles: 64 x 64 outputs per workgroup of four waves, each wave
+// 32 x 32 as 2 x 2 v_mfma_f64_16x16x4f64 accumulators, or, for large jobs, 128 x 128 outputs per
+// workgroup of eight waves of 32 x 64 (half the operand reads per multiply-add); K advances 16
+// at a time through LDS, with the next K slab prefetched into registers while the current one
+// is multiplied.
 #include "pomx_internal.hpp"
 
 #include <algorithm>
