@@ -326,8 +326,10 @@ def run_cfg3(args):
     H, beta, tp = W.cfg3_expr()
     out = {"model": f"3x3 periodic Hubbard, t=1, t'={tp:.4f}, U=4, mu=2, beta={beta}", "states": 1 << 18}
     t0 = time.perf_counter()
-    e = pa.ED(W.terms(H), beta, not args.no_cpu)  # (Hamiltonian blocks kept for the CPU sample)
-    out["ed_seconds"] = time.perf_counter() - t0
+    # (the largest Hamiltonian block is kept on the host for the CPU sample; that copy is not ED)
+    e = pa.ED(W.terms(H), beta, "largest" if not args.no_cpu else False)
+    out["ed_seconds"] = time.perf_counter() - t0 - e.t_keep
+    out["ed_host_copy_seconds"] = e.t_keep
     out["eigensolve_seconds"] = e.t_eigh
     sizes = [len(b) for b in e.blocks()]
     out["blocks"] = len(sizes)

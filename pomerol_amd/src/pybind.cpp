@@ -62,10 +62,12 @@ struct ED {
     std::unique_ptr<Hamiltonian> H;
     std::unique_ptr<DensityMatrix> rho;
     std::unique_ptr<FieldOperatorContainer> ops;
-    std::vector<MatrixType<false>> hmats;
-    double t_eigh = 0, t_rot = 0;
+    std::map<BlockNumber, MatrixType<false>> hmats; // Hamiltonian blocks kept before the eigensolve
+    double t_eigh = 0, t_rot = 0, t_keep = 0;
 
-    ED(py::list terms, double beta, bool keep_hmat) : HExpr(build_expr(terms)), beta(beta) {
+    // keep: 0 no Hamiltonian block kept, 1 every block, 2 the largest block only (host copies taken
+    // before the eigensolve overwrites them; their time is t_keep)
+    ED(py::list terms, double beta, int keep) : HExpr(build_expr(terms)), beta(beta) {
         info.reset(new IndexInfoT(MakeIndexClassification(HExpr)));
         HS.reset(new HST(*info, HExpr));
         HS->compute();
@@ -73,9 +75,15 @@ struct ED {
         S->compute(*HS);
         H.reset(new Hamiltonian(*S));
         H->prepare(HExpr, *HS);
-        if(keep_hmat)
-            for(BlockNumber b = 0; b < S->getNumberOfBlocks(); ++b)
-                hmats.push_back(H->getPart(b).getMatrix<false>());
+        auto const tk = std::chrono::steady_clock::now();
+        BlockNumber big = 0;
+        for(BlockNumber b = 0; b < S->getNumberOfBlocks(); ++b)
+            if(S->getBlockSize(b) > S->getBlockSize(big))
+                big = b;
+        for(BlockNumber b = 0; b < S->getNumberOfBlocks(); ++b)
+            if(keep == 1 || (keep == 2 && b == big))
+                hmats.emplace(b, H->getPart(b).getMatrix<false>());
+        t_keep = std::chrono::duration<double>(std::chrono::steady_clock::now() - tk).count();
         auto t0 = std::chrono::steady_clock::now();
         H->compute();
         t_eigh = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -339,8 +347,17 @@ PYBIND11_MODULE(_pomerol, m) {
             return py::make_tuple(std::vector<int>(P.perm.begin(), P.perm.end()), P.sign);
         });
     py::class_<ED>(m, "ED")
-        .def(py::init<py::list, double, bool>(), py::arg("terms"), py::arg("beta"), py::arg("keep_hmat") = false)
+        .def(py::init([](py::list terms, double beta, py::object keep_hmat) {
+                 // False / True (every block), or "largest"
+                 int keep = py::isinstance<py::str>(keep_hmat) ? (keep_hmat.cast<std::string>() == "largest" ? 2 : -1)
+                                                               : (keep_hmat.cast<bool>() ? 1 : 0);
+                 if(keep < 0)
+                     throw std::invalid_argument("keep_hmat: False, True or 'largest'");
+                 return new ED(terms, beta, keep);
+             }),
+             py::arg("terms"), py::arg("beta"), py::arg("keep_hmat") = false)
         .def_readonly("t_eigh", &ED::t_eigh)
+        .def_readonly("t_keep", &ED::t_keep)
         .def_property_readonly("E0", [](ED& e) { return e.H->getGroundEnergy(); })
         .def_property_readonly("Z", [](ED& e) { return e.rho->getPartitionFunction(); })
         .def("indices", [](ED& e) {
@@ -358,7 +375,7 @@ PYBIND11_MODULE(_pomerol, m) {
                 b.push_back(e.S->getFockStates(k));
             return b;
         })
-        .def("hmat", [](ED& e, int b) { return to_np(e.hmats.at(std::size_t(b))); })
+        .def("hmat", [](ED& e, int b) { return to_np(e.hmats.at(BlockNumber(b))); })
         .def("evals", [](ED& e, int b) {
             auto const& v = e.H->getEigenValues(b);
             return to_np(std::vector<double>(v.begin(), v.end()));
