@@ -140,6 +140,11 @@ __host__ __device__ inline DOff doff(int n, int gsym) {
     o.end = at;
     return o;
 }
+// Stage-1 panel buffers [W; Vt; W3] (n x kNB each, contiguous) of the panel starting at row k:
+// even panels at W, odd panels in Z (free until the divide and conquer; n >= 3 kNB), so that
+// the next panel's LQ can fill its own set while this panel's trailing update reads the other.
+__host__ __device__ inline int64_t wv_set(DOff const& o, int k) { return ((k / kBand) & 1) ? o.Z : o.W; }
+
 // Per-block integer scratch (from ibase).
 struct IOff {
     int64_t perm, order, org, fperm, cnt, end;
@@ -1151,7 +1156,7 @@ __global__ void __launch_bounds__(kLQT) lq_kernel(const BigBlk* __restrict__ blk
     }
     if(i == 0 && live)
         for(int q = nref; q < kBand; ++q) // rows of Vt beyond the panel's reflectors: zero
-            base[o.Vt + int64_t(q) * n + gc] = 0.0;
+            base[wv_set(o, k) + int64_t(n) * kNB + int64_t(q) * n + gc] = 0.0;
     if(i == 0 && blockIdx.x == 0 && tid >= nref && tid < kBand)
         base[o.tau + k + tid] = 0.0;
     if(i >= 1) {
@@ -1201,7 +1206,7 @@ __global__ void __launch_bounds__(kLQT) lq_kernel(const BigBlk* __restrict__ blk
                 p[r] = newq;
         }
         if(live) {
-            base[o.Vt + int64_t(q) * n + gc] = v;
+            base[wv_set(o, k) + int64_t(n) * kNB + int64_t(q) * n + gc] = v;
             if(c >= q)
                 base[o.V + int64_t(gc) * n + (k + q)] = v;
         }
@@ -1252,8 +1257,9 @@ __global__ void __launch_bounds__(256) gram_kernel(const BigBlk* __restrict__ bl
         return;
     XOff const x = xoff(n);
     DOff const o = doff(n, B.gsym);
-    const double* __restrict__ X = D + B.dbase + o.Vt + (k + kBand);
-    const double* __restrict__ Y = D + B.dbase + (which ? o.W : o.Vt) + (k + kBand);
+    int64_t const wv = wv_set(o, k), vt = wv + int64_t(n) * kNB;
+    const double* __restrict__ X = D + B.dbase + vt + (k + kBand);
+    const double* __restrict__ Y = D + B.dbase + (which ? wv : vt) + (k + kBand);
     for(int e = threadIdx.x; e < kBand * kGramW; e += 256) {
         int const r = e / kGramW, cc = e % kGramW, c = c0 + cc;
         xs[r][cc] = c < m ? X[int64_t(r) * n + c] : 0.0;
@@ -1292,7 +1298,7 @@ __global__ void __launch_bounds__(256) xv_kernel(const BigBlk* __restrict__ blks
         return; // uniform over the workgroup
     DOff const o = doff(n, B.gsym);
     const double* __restrict__ A = Aar + B.mat_off + int64_t(k1) * n + k1;
-    const double* __restrict__ Vt = D + B.dbase + o.Vt + k1;
+    const double* __restrict__ Vt = D + B.dbase + wv_set(o, k) + int64_t(n) * kNB + k1;
     double* __restrict__ out = D + B.dbase + o.U + int64_t(blockIdx.z) * kBand * n + k1;
     int const nchunk = (m + KC - 1) / KC;
     int const per = (nchunk + xs - 1) / xs;
@@ -1443,7 +1449,7 @@ __global__ void copy_w3_kernel(const BigBlk* __restrict__ blks, int k, double* _
     int64_t const cnt = int64_t(kBand) * m;
     for(int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < cnt; q += int64_t(gridDim.x) * blockDim.x) {
         int64_t const at = (q / m) * n + (k + kBand) + (q % m);
-        base[o.W3 + at] = base[o.W + at];
+        base[wv_set(o, k) + 2 * int64_t(n) * kNB + at] = base[wv_set(o, k) + at];
     }
 }
 
@@ -1820,6 +1826,11 @@ struct BigPlan {
         int64_t jobs_off;
     };
     std::vector<Panel2> p2;
+    // look-ahead of stage 1: panel p+1's LQ (and its T) runs on the side stream while panel p's
+    // trailing update of the other rows runs on the context stream (events: A after the update
+    // of the next panel's rows, B after the next panel's LQ; two of each, by panel parity)
+    hipStream_t side = nullptr;
+    hipEvent_t evA[2] = {nullptr, nullptr}, evB[2] = {nullptr, nullptr};
     std::vector<int> step_tasks; // bulge chase: tasks of wavefront step t (all blocks)
     BigPlan* next = nullptr; // the two-stage plan of the larger blocks, run after this one
 };
@@ -1835,6 +1846,14 @@ void big_plan_destroy(BigPlan* p) {
     for(void* q : ptrs)
         if(q)
             (void)hipFree(q);
+    for(int x = 0; x < 2; ++x) {
+        if(p->evA[x])
+            (void)hipEventDestroy(p->evA[x]);
+        if(p->evB[x])
+            (void)hipEventDestroy(p->evB[x]);
+    }
+    if(p->side)
+        (void)hipStreamDestroy(p->side);
     delete p;
 }
 
@@ -1892,21 +1911,28 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
             Q.nref_max = std::min(kBand, Q.mmax - 1);
             Q.nchunk = (Q.mmax + kLQT - 1) / kLQT;
             Q.ngram = (Q.mmax + kGramW - 1) / kGramW;
-            for(int kind = 1; kind < 4; ++kind)
+            // kinds: 1 W^T, 2 Z^T, 3 the trailing update of the next panel's rows (first kBand rows
+            // of A22), 4 the rest of the trailing update (rows and columns from kBand on)
+            for(int kind = 1; kind < 5; ++kind)
                 for(int b = 0; b < Q.nact; ++b) {
                     BigBlk const& B = P->h_blk[size_t(b)];
                     int const n = B.n, m = n - k - kBand, k1 = k + kBand;
                     DOff const o = doff(n, B.gsym);
                     int64_t const a22 = B.mat_off + int64_t(k1) * n + k1;
+                    int64_t const wv = B.dbase + wv_set(o, k), vt = wv + int64_t(n) * kNB; // [W; Vt; W3]
                     if(kind == 1)
                         gj.push_back(pomx_gemm_job{kBand, m, P->xsplit * kBand, 0, 0, P->xsplit * kBand, n, n, B.dbase + o.S,
-                                                   B.dbase + o.U + k1, B.dbase + o.W + k1, 1.0, 0.0});
+                                                   B.dbase + o.U + k1, wv + k1, 1.0, 0.0});
                     else if(kind == 2)
-                        gj.push_back(pomx_gemm_job{kBand, m, kBand, 1, 0, kBand, n, n, B.dbase + o.S, B.dbase + o.Vt + k1,
-                                                   B.dbase + o.W + k1, -0.5, 1.0});
-                    else
-                        gj.push_back(pomx_gemm_job{m, m, 2 * kBand, 1, 0, n, n, n, B.dbase + o.W + k1, B.dbase + o.Vt + k1,
-                                                   a22, -1.0, 1.0});
+                        gj.push_back(pomx_gemm_job{kBand, m, kBand, 1, 0, kBand, n, n, B.dbase + o.S, vt + k1, wv + k1, -0.5, 1.0});
+                    else if(kind == 3)
+                        gj.push_back(pomx_gemm_job{std::min(m, kBand), m, 2 * kBand, 1, 0, n, n, n, wv + k1, vt + k1, a22, -1.0,
+                                                   1.0});
+                    else {
+                        int const mr = std::max(0, m - kBand);
+                        gj.push_back(pomx_gemm_job{mr, mr, 2 * kBand, 1, 0, n, n, n, wv + k1 + kBand, vt + k1 + kBand,
+                                                   a22 + int64_t(kBand) * n + kBand, -1.0, 1.0});
+                    }
                 }
             P->p2.push_back(Q);
         }
@@ -2092,6 +2118,12 @@ int plan_build(pomx_ctx* ctx, int nblk, const int* n_in, const int64_t* mat_off,
     alloc(reinterpret_cast<void**>(&P->leaf_arena), sizeof(double) * size_t(larena));
     alloc(reinterpret_cast<void**>(&P->leaf_ws), sizeof(double) * size_t(std::max<int64_t>(lws, 1)));
     alloc(reinterpret_cast<void**>(&P->status), sizeof(int) * size_t(std::max(P->nleaves, 1)));
+    if(ok && P->p2.size() > 1) { // stage-1 look-ahead (a non-blocking stream of default priority)
+        ok = hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking) == hipSuccess;
+        for(int x = 0; ok && x < 2; ++x)
+            ok = hipEventCreateWithFlags(&P->evA[x], hipEventDisableTiming) == hipSuccess &&
+                 hipEventCreateWithFlags(&P->evB[x], hipEventDisableTiming) == hipSuccess;
+    }
     if(hipStreamSynchronize(s) != hipSuccess || !ok) { // (the uploads from host vectors have finished)
         big_plan_destroy(P);
         return pomx_fail(ctx, POMX_ERR_HIP, "large-block eigensolver: out of device memory for the plan");
@@ -2107,12 +2139,26 @@ int run_two_stage(pomx_ctx* ctx, BigPlan* P, double* Aar) {
     int const nb = P->nblk;
     fill_blocks_kernel<<<dim3(gx_rows(int64_t(P->nmax) * P->nmax, 256, nb), unsigned(nb)), 256, 0, s>>>(P->blk, P->D, 1, 0.0);
     POMX_CHECK_LAUNCH(ctx);
-    for(auto const& Q : P->p2) {
+    // the panel LQ with its T (on stream q)
+    auto panel_lq = [&](BigPlan::Panel2 const& Q, hipStream_t q) {
         unsigned const na = unsigned(Q.nact);
         for(int i = 0; i <= Q.nref_max; ++i)
-            lq_kernel<<<dim3(unsigned(Q.nchunk), na), kLQT, 0, s>>>(P->blk, Q.k, i, Aar, P->D);
-        gram_kernel<<<dim3(unsigned(Q.ngram), na), 256, 0, s>>>(P->blk, Q.k, 0, P->D);
-        panel_t_kernel<<<dim3(1, na), 256, 0, s>>>(P->blk, Q.k, 0, P->xsplit, P->D);
+            lq_kernel<<<dim3(unsigned(Q.nchunk), na), kLQT, 0, q>>>(P->blk, Q.k, i, Aar, P->D);
+        gram_kernel<<<dim3(unsigned(Q.ngram), na), 256, 0, q>>>(P->blk, Q.k, 0, P->D);
+        panel_t_kernel<<<dim3(1, na), 256, 0, q>>>(P->blk, Q.k, 0, P->xsplit, P->D);
+    };
+    // look-ahead on the side stream, unless the context stream is being captured into a graph
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    POMX_HIP(ctx, hipStreamIsCapturing(s, &cap));
+    bool const ahead = P->side && cap == hipStreamCaptureStatusNone;
+    size_t const np = P->p2.size();
+    for(size_t pi = 0; pi < np; ++pi) {
+        auto const& Q = P->p2[pi];
+        unsigned const na = unsigned(Q.nact);
+        if(pi == 0 || !ahead)
+            panel_lq(Q, s);
+        else
+            POMX_HIP(ctx, hipStreamWaitEvent(s, P->evB[pi & 1], 0)); // this panel's LQ, run ahead
         POMX_CHECK_LAUNCH(ctx);
         int64_t const tc = (Q.mmax + 63) / 64;
         pomx_gemm_job const* J = P->gjobs + Q.jobs_off;
@@ -2127,7 +2173,17 @@ int run_two_stage(pomx_ctx* ctx, BigPlan* P, double* Aar) {
             return rc;
         copy_w3_kernel<<<dim3(gx_rows(int64_t(kBand) * Q.mmax, 256, Q.nact), na), 256, 0, s>>>(P->blk, Q.k, P->D);
         POMX_CHECK_LAUNCH(ctx);
-        if((rc = launch_gemm_jobs(ctx, 1, 0, J + 2 * Q.nact, Q.nact, tc * tc, P->D, P->D, Aar, true)))
+        // trailing update: the next panel's rows first, then (beside the next panel's LQ) the rest
+        if((rc = launch_gemm_jobs(ctx, 1, 0, J + 2 * Q.nact, Q.nact, tc, P->D, P->D, Aar, true)))
+            return rc;
+        if(ahead && pi + 1 < np) {
+            POMX_HIP(ctx, hipEventRecord(P->evA[pi & 1], s));
+            POMX_HIP(ctx, hipStreamWaitEvent(P->side, P->evA[pi & 1], 0));
+            panel_lq(P->p2[pi + 1], P->side);
+            POMX_CHECK_LAUNCH(ctx);
+            POMX_HIP(ctx, hipEventRecord(P->evB[(pi + 1) & 1], P->side));
+        }
+        if((rc = launch_gemm_jobs(ctx, 1, 0, J + 3 * Q.nact, Q.nact, tc * tc, P->D, P->D, Aar, true)))
             return rc;
     }
     band_kernel<<<dim3(gx_rows(int64_t(P->nmax) * 2 * kBand, 256, nb), unsigned(nb)), 256, 0, s>>>(P->blk, Aar, P->D);
@@ -2279,8 +2335,8 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
 int two_stage_min() {
     if(const char* e = std::getenv("POMX_EIGH_TWO_STAGE_MIN")) {
         long const v = std::atol(e);
-        if(v > 0)
-            return int(std::min<long>(v, 1L << 30));
+        if(v > 0) // at least 3 kNB: the odd panels' stage-1 buffers live in Z
+            return int(std::min<long>(std::max<long>(v, 3 * kNB), 1L << 30));
     }
     return kTwoStageMin;
 }
