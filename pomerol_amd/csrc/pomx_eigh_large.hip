@@ -4,7 +4,9 @@
 // (src/pomerol/HamiltonianPart.cpp:90-102) for the blocks that do not fit one workgroup.  Every
 // stage works on all blocks of a plan together (blockIdx.y = block, tree node or job), the launch
 // sequence depends only on the block sizes and every data-dependent quantity (deflation counts,
-// permutations, GEMM sizes) lives in device memory, so a plan can be replayed inside a hipGraph.
+// permutations, GEMM sizes) lives in device memory, so a plan can be replayed inside a hipGraph
+// (in a captured stream everything runs on that stream; outside capture, two-stage plans run
+// the next panel's LQ on a second stream of their own, joined by events).
 //   1. Blocked Householder tridiagonalisation T = Q^T A Q, panels of kNB reflectors.  Inside a
 //      panel the trailing matrix is not updated; w = tau (A22 v - V W^T v - W V^T v) comes from
 //      one device-wide matrix-vector product over the upper triangle of A22 (the HBM-bound
