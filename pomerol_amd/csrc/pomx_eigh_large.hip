@@ -1165,18 +1165,21 @@ __global__ void __launch_bounds__(kLQT) lq_kernel(const BigBlk* __restrict__ blk
         int const q = i - 1;
         const double* __restrict__ src = lq + (q & 1) * par;
         // the partial sums of every chunk, finished in a fixed order: 7 threads per value take
-        // every 7th chunk, then one thread per value adds their sums
+        // every 7th chunk, eight chunks' loads in flight at a time, then one thread per value
+        // adds their sums
         if(tid < 7 * 33) {
             int const e = tid % 33, g = tid / 33;
-            double s0 = 0.0, s1 = 0.0;
-            int b = g;
-            for(; b + 7 < nch; b += 14) {
-                s0 += src[int64_t(b) * 40 + e];
-                s1 += src[int64_t(b + 7) * 40 + e];
+            double s0 = 0.0;
+            for(int b0 = g; b0 < nch; b0 += 7 * 8) {
+                double t[8];
+#pragma unroll
+                for(int u = 0; u < 8; ++u) {
+                    int const b = b0 + 7 * u;
+                    t[u] = b < nch ? src[int64_t(b) * 40 + e] : 0.0;
+                }
+                s0 += ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
             }
-            if(b < nch)
-                s0 += src[int64_t(b) * 40 + e];
-            fin7[g][e] = s0 + s1;
+            fin7[g][e] = s0;
         }
         if(tid < kBand)
             colv[tid] = src[int64_t(x.nchunk) * 40 + tid];
@@ -1390,18 +1393,25 @@ __global__ void __launch_bounds__(256) panel_t_kernel(const BigBlk* __restrict__
     int const nchg = (m + kGramW - 1) / kGramW;
     const double* __restrict__ gp = D + B.xbase + x.gp;
     int const tid = threadIdx.x;
-    for(int e = tid; e < kBand * kBand; e += 256) {
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0; // four chains: loads in flight together
-        int b = 0;
-        for(; b + 3 < nchg; b += 4) {
-            s0 += gp[int64_t(b) * kBand * kBand + e];
-            s1 += gp[int64_t(b + 1) * kBand * kBand + e];
-            s2 += gp[int64_t(b + 2) * kBand * kBand + e];
-            s3 += gp[int64_t(b + 3) * kBand * kBand + e];
-        }
-        for(; b < nchg; ++b)
-            s0 += gp[int64_t(b) * kBand * kBand + e];
-        S[e / kBand][e % kBand] = (s0 + s1) + (s2 + s3);
+    // the Gram partial sums of every column chunk: each thread's four entries of S over eight
+    // chunks at a time (32 loads in flight), chunks in a fixed order
+    static_assert(kBand * kBand == 4 * 256, "four entries of S per thread");
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for(int b0 = 0; b0 < nchg; b0 += 8) {
+        double t[4][8];
+#pragma unroll
+        for(int u = 0; u < 4; ++u)
+#pragma unroll
+            for(int w = 0; w < 8; ++w)
+                t[u][w] = (b0 + w < nchg) ? gp[int64_t(b0 + w) * kBand * kBand + tid + 256 * u] : 0.0;
+#pragma unroll
+        for(int u = 0; u < 4; ++u)
+            acc[u] += ((t[u][0] + t[u][1]) + (t[u][2] + t[u][3])) + ((t[u][4] + t[u][5]) + (t[u][6] + t[u][7]));
+    }
+#pragma unroll
+    for(int u = 0; u < 4; ++u) {
+        int const e = tid + 256 * u;
+        S[e / kBand][e % kBand] = acc[u];
         T[e / kBand][e % kBand] = which ? base[o.T + e] : 0.0;
     }
     __syncthreads();
