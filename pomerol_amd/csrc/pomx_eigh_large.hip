@@ -923,7 +923,45 @@ __global__ void dc_zhat_kernel(const BigBlk* __restrict__ blks, const DcNode* __
     }
 }
 
-// U[i][j] = zhat_i / (d_i - lambda_j) in the merge's square of U (columns normalised next).
+// 1 / |column jj| of the merge's eigenvector matrix, |column jj|^2 = sum_i zhat_i^2 / ((d_i -
+// d_org) - tau_jj)^2, from zhat, d and the roots alone (before U is written), to wraw (free after
+// the reduction).  One thread per column; the (d_i, zhat_i) come through LDS 256 at a time.
+__global__ void __launch_bounds__(256) dc_unorm_kernel(const BigBlk* __restrict__ blks, const DcNode* __restrict__ nodes,
+                                                       const DcState* __restrict__ st, double* __restrict__ D,
+                                                       const int* __restrict__ I) {
+    __shared__ double dsh[256], zsh[256];
+    DcNode const N = nodes[blockIdx.y];
+    int const k = st[blockIdx.y].k;
+    BigBlk const B = blks[N.blk];
+    int const NB = B.n;
+    DOff const od = doff(NB, B.gsym);
+    double* base = D + B.dbase;
+    const double* d = base + od.dl + N.o;
+    const int* og = I + B.ibase + ioff(NB).org + N.o;
+    for(int j0 = blockIdx.x * 256; j0 < k; j0 += gridDim.x * 256) { // uniform over the workgroup
+        int const jj = j0 + threadIdx.x;
+        bool const live = jj < k;
+        double const dj = live ? d[og[jj]] : 0.0, rj = live ? base[od.rt + N.o + jj] : 0.0;
+        double s = 0.0;
+        for(int i0 = 0; i0 < k; i0 += 256) {
+            __syncthreads(); // the previous chunk's readers are done
+            int const i = i0 + threadIdx.x;
+            dsh[threadIdx.x] = i < k ? d[i] : 0.0;
+            zsh[threadIdx.x] = i < k ? base[od.zh + N.o + i] : 0.0;
+            __syncthreads();
+            int const cnt = min(256, k - i0);
+            if(live)
+                for(int q = 0; q < cnt; ++q) {
+                    double const u = zsh[q] / ((dsh[q] - dj) - rj);
+                    s = fma(u, u, s);
+                }
+        }
+        if(live)
+            base[od.wraw + N.o + jj] = 1.0 / sqrt(s);
+    }
+}
+
+// U[i][j] = zhat_i / (d_i - lambda_j) / |column j| in the merge's square of U.
 __global__ void dc_u_kernel(const BigBlk* __restrict__ blks, const DcNode* __restrict__ nodes, const DcState* __restrict__ st,
                             double* __restrict__ D, const int* __restrict__ I) {
     DcNode const N = nodes[blockIdx.y];
@@ -939,26 +977,7 @@ __global__ void dc_u_kernel(const BigBlk* __restrict__ blks, const DcNode* __res
     for(int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < cnt; q += int64_t(gridDim.x) * blockDim.x) {
         int const i = int(q / k), jj = int(q % k);
         base[od.U + int64_t(N.o + sg[i]) * NB + N.o + jj] =
-            base[od.zh + N.o + i] / ((d[i] - d[og[jj]]) - base[od.rt + N.o + jj]);
-    }
-}
-
-__global__ void __launch_bounds__(256) dc_colnorm_kernel(const BigBlk* __restrict__ blks, const DcNode* __restrict__ nodes,
-                                                         const DcState* __restrict__ st, double* __restrict__ D) {
-    DcNode const N = nodes[blockIdx.y];
-    int const k = st[blockIdx.y].k;
-    BigBlk const B = blks[N.blk];
-    int const NB = B.n;
-    DOff const od = doff(NB, B.gsym);
-    double* U = D + B.dbase + od.U + int64_t(N.o) * NB + N.o;
-    // one thread per column: consecutive threads read consecutive columns of a row
-    for(int jj = blockIdx.x * blockDim.x + threadIdx.x; jj < k; jj += gridDim.x * blockDim.x) {
-        double s = 0.0;
-        for(int i = 0; i < k; ++i)
-            s = fma(U[int64_t(i) * NB + jj], U[int64_t(i) * NB + jj], s);
-        double const inv = 1.0 / sqrt(s);
-        for(int i = 0; i < k; ++i)
-            U[int64_t(i) * NB + jj] *= inv;
+            base[od.zh + N.o + i] / ((d[i] - d[og[jj]]) - base[od.rt + N.o + jj]) * base[od.wraw + N.o + jj];
     }
 }
 
@@ -2275,8 +2294,8 @@ int plan_run(pomx_ctx* ctx, BigPlan* P, double* Aar, double* eval) {
         dc_gather_kernel<<<gsq, 256, 0, s>>>(P->blk, nd, 1, P->D, P->I);
         dc_secular_kernel<<<gwave, 256, 0, s>>>(P->blk, nd, st, P->D, P->I);
         dc_zhat_kernel<<<grow, 256, 0, s>>>(P->blk, nd, st, P->D, P->I);
+        dc_unorm_kernel<<<grow, 256, 0, s>>>(P->blk, nd, st, P->D, P->I);
         dc_u_kernel<<<gsq, 256, 0, s>>>(P->blk, nd, st, P->D, P->I);
-        dc_colnorm_kernel<<<grow, 256, 0, s>>>(P->blk, nd, st, P->D);
         POMX_CHECK_LAUNCH(ctx);
         int64_t const t = (L.nmax + 63) / 64;
         for(int j0 = 0; j0 < 2 * L.nnodes; j0 += 65534) // (at most 65535 jobs per launch)
