@@ -619,6 +619,7 @@ __global__ void __launch_bounds__(256) dc_deflate_kernel(const BigBlk* __restric
                                                          double* __restrict__ D, int* __restrict__ I,
                                                          DcRot* __restrict__ R) {
     __shared__ double red[4];
+    __shared__ int s_nk, s_nrot, s_c1[256], s_c2[256];
     DcNode const N = nodes[blockIdx.y];
     BigBlk const B = blks[N.blk];
     int const NB = B.n, o = N.o, n = N.n, n1 = N.n1, tid = threadIdx.x;
@@ -692,75 +693,96 @@ __global__ void __launch_bounds__(256) dc_deflate_kernel(const BigBlk* __restric
     }
     double const dmax = wg_max(dm, red), zmax = wg_max(zm, red);
     __syncthreads();
-    if(tid != 0)
-        return;
-    double const tol = 8.0 * DBL_EPSILON * fmax(dmax, zmax);
-    int nk = 0, ndf = 0, nrot = 0;
-    if(!(nz > 0.0) || rho * zmax <= tol) {
-        for(int c = 0; c < n; ++c)
-            order[n - 1 - ndf++] = c;
-    } else {
-        int pj = -1;
-        for(int c = 0; c < n; ++c) {
-            double const zc = zs[c];
-            if(rho * fabs(zc) <= tol) {
+    if(tid == 0) { // the deflation scan is sequential (a close pair rotates into the next candidate)
+        double const tol = 8.0 * DBL_EPSILON * fmax(dmax, zmax);
+        int nk = 0, ndf = 0, nrot = 0;
+        if(!(nz > 0.0) || rho * zmax <= tol) {
+            for(int c = 0; c < n; ++c)
                 order[n - 1 - ndf++] = c;
-                continue;
-            }
-            if(pj < 0) {
+        } else {
+            int pj = -1;
+            for(int c = 0; c < n; ++c) {
+                double const zc = zs[c];
+                if(rho * fabs(zc) <= tol) {
+                    order[n - 1 - ndf++] = c;
+                    continue;
+                }
+                if(pj < 0) {
+                    pj = c;
+                    continue;
+                }
+                double const s0 = zs[pj];
+                double const t2 = hypot(zc, s0);
+                double const t = ds[c] - ds[pj];
+                double const cc = zc / t2, ss = -s0 / t2;
+                if(fabs(t * cc * ss) <= tol) {
+                    zs[c] = t2;
+                    zs[pj] = 0.0;
+                    rot[nrot++] = DcRot{pj, c, cc, ss};
+                    if(typ[c] != typ[pj])
+                        typ[c] = 2;
+                    double const dp = ds[pj] * cc * cc + ds[c] * ss * ss;
+                    double const dq = ds[pj] * ss * ss + ds[c] * cc * cc;
+                    ds[pj] = dp;
+                    ds[c] = dq;
+                    order[n - 1 - ndf++] = pj;
+                } else {
+                    order[nk] = pj;
+                    dl[nk] = ds[pj];
+                    zl[nk] = zs[pj];
+                    ++nk;
+                }
                 pj = c;
-                continue;
             }
-            double const s0 = zs[pj];
-            double const t2 = hypot(zc, s0);
-            double const t = ds[c] - ds[pj];
-            double const cc = zc / t2, ss = -s0 / t2;
-            if(fabs(t * cc * ss) <= tol) {
-                zs[c] = t2;
-                zs[pj] = 0.0;
-                rot[nrot++] = DcRot{pj, c, cc, ss};
-                if(typ[c] != typ[pj])
-                    typ[c] = 2;
-                double const dp = ds[pj] * cc * cc + ds[c] * ss * ss;
-                double const dq = ds[pj] * ss * ss + ds[c] * cc * cc;
-                ds[pj] = dp;
-                ds[c] = dq;
-                order[n - 1 - ndf++] = pj;
-            } else {
+            if(pj >= 0) {
                 order[nk] = pj;
                 dl[nk] = ds[pj];
                 zl[nk] = zs[pj];
                 ++nk;
             }
-            pj = c;
         }
-        if(pj >= 0) {
-            order[nk] = pj;
-            dl[nk] = ds[pj];
-            zl[nk] = zs[pj];
-            ++nk;
-        }
+        s_nk = nk;
+        s_nrot = nrot;
     }
+    __syncthreads();
     // The kept columns (Z after the gather by order) sorted by type, first half only, mixed,
     // second half only: C = Z U splits into the first half's rows times the first t1 + tm
     // columns and the second half's rows times the last tm + t3 (the other entries of Z are
     // zero), half the product of the dense n x nk x nk.  U's rows follow (dc_u_kernel, sig).
-    int t1 = 0, tm = 0;
-    for(int i = 0; i < nk; ++i) {
+    // sig (a stable counting sort by type) with every thread: counts over contiguous ranges of
+    // the kept columns, then each thread's offsets from the counts of the ranges before it.
+    int const nk = s_nk;
+    int const per = (nk + 255) / 256, i0 = min(nk, tid * per), i1 = min(nk, i0 + per);
+    int c1 = 0, c2 = 0;
+    for(int i = i0; i < i1; ++i) {
         int const ty = typ[order[i]];
-        t1 += ty == 1;
-        tm += ty == 2;
+        c1 += ty == 1;
+        c2 += ty == 2;
+    }
+    s_c1[tid] = c1;
+    s_c2[tid] = c2;
+    __syncthreads();
+    int p1 = 0, p2 = 0, t1 = 0, tm = 0;
+    for(int q = 0; q < 256; ++q) {
+        p1 += q < tid ? s_c1[q] : 0;
+        p2 += q < tid ? s_c2[q] : 0;
+        t1 += s_c1[q];
+        tm += s_c2[q];
     }
     int const t3 = nk - t1 - tm;
-    for(int i = 0, a = 0, bm = t1, c3 = t1 + tm; i < nk; ++i) {
+    for(int i = i0, a = p1, bm = t1 + p2, c3 = t1 + tm + (i0 - p1 - p2); i < i1; ++i) {
         int const ty = typ[order[i]];
         sig[i] = ty == 1 ? a++ : ty == 2 ? bm++ : c3++;
     }
-    for(int i = 0; i < nk; ++i) // (typ is no longer needed: the order's copy)
+    __syncthreads(); // every thread has read typ
+    for(int i = tid; i < nk; i += 256) // (typ is no longer needed: the order's copy)
         typ[i] = order[i];
-    for(int i = 0; i < nk; ++i)
+    __syncthreads();
+    for(int i = tid; i < nk; i += 256)
         order[sig[i]] = typ[i];
-    st[blockIdx.y] = DcState{nk, nrot, rho};
+    if(tid != 0)
+        return;
+    st[blockIdx.y] = DcState{nk, s_nrot, rho};
     int64_t const sq = int64_t(o) * NB + o;
     jobs[2 * blockIdx.y] = pomx_gemm_job{n1, nk, t1 + tm, 0, 0, NB, NB, NB, B.dbase + od.Z + sq, B.dbase + od.U + sq,
                                          B.dbase + od.Q2 + sq, 1.0, 0.0};
